@@ -1,0 +1,769 @@
+"""Python host mirror of DiffMoM.jl's surface-EFIE API over libdiffmom (HIP, gfx950).
+
+Names, argument meaning and errors follow the Julia functions they mirror
+(citations are path:line in DiffMoM.jl):
+
+    make_rect_plate            src/geometry/Mesh.jl:309-396
+    make_icosphere             test/runtests.jl:373-427 (icosphere test helper)
+    build_rwg                  src/basis/RWG.jl:102-143
+    assemble_Z_efie            src/assembly/EFIE.jl:699-753
+    matrixfree_efie_operator   src/assembly/EFIE.jl:975-1000
+    assemble_v_plane_wave / assemble_excitation / assemble_multiple_excitations
+                               src/assembly/Excitation.jl:2684-2896
+    solve_forward              src/solver/Solve.jl:1342-1382
+    solve_gmres                src/solver/IterativeSolve.jl:341-392
+    build_nearfield_preconditioner  src/solver/NearFieldPreconditioner.jl:732-759
+    solve_scattering           src/Workflow.jl:67-317
+    make_sph_grid, radiation_vectors, compute_farfield, bistatic_rcs
+                               src/postprocessing/FarField.jl:216-605,
+                               src/postprocessing/Diagnostics.jl:460-473
+
+Every numerical kernel runs on the GPU through libdiffmom.so; importing this
+package never falls back to host arithmetic.  When the library is missing the
+first call raises RuntimeError.
+
+Array conventions: a mesh's ``xyz`` is (Nv, 3) float64 and ``tri`` (Nt, 3)
+int64 with 1-based vertex ids — the same memory as Julia's 3xNv / 3xNt
+column-major matrices, so buffers pass to the C ABI unchanged.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdiffmom.so")
+
+# error codes (include/diffmom.h)
+DM_OK, DM_ERR_ARGUMENT, DM_ERR_OVERFLOW, DM_ERR_DIMENSION = 0, 1, 2, 3
+DM_ERR_CONVERGENCE, DM_ERR_DEVICE, DM_ERR_MEMORY, DM_ERR_DOMAIN = 4, 5, 6, 7
+DM_Z64, DM_Z32 = 1, 2
+DM_METHOD_LU, DM_METHOD_GMRES, DM_METHOD_MIXED = 0, 1, 2
+DM_PRECOND_NONE, DM_PRECOND_DIAG, DM_PRECOND_NEARFIELD, DM_PRECOND_LEARNED = 0, 1, 2, 3
+
+ETA0 = 376.730313668
+C0_DEFAULT = 299792458.0
+_DEFAULT_MAX_DENSE_PAYLOAD_BYTES = 2_000_000_000
+
+
+class ConvergenceError(RuntimeError):
+    """GMRES did not converge (the reference raises ErrorException)."""
+
+
+class DeviceError(RuntimeError):
+    pass
+
+
+class DomainError(ValueError):
+    pass
+
+
+_EXC = {DM_ERR_ARGUMENT: ValueError, DM_ERR_OVERFLOW: OverflowError, DM_ERR_DIMENSION: ValueError,
+        DM_ERR_CONVERGENCE: ConvergenceError, DM_ERR_DEVICE: DeviceError, DM_ERR_MEMORY: MemoryError,
+        DM_ERR_DOMAIN: DomainError}
+
+
+class DimensionMismatch(ValueError):
+    pass
+
+
+_EXC[DM_ERR_DIMENSION] = DimensionMismatch
+
+
+class dm_solve_opts(C.Structure):
+    _fields_ = [("method", C.c_int), ("precond", C.c_int), ("precond_side", C.c_int), ("maxiter", C.c_int),
+                ("tol", C.c_double), ("nf_cutoff", C.c_double), ("restart", C.c_int),
+                ("check_convergence", C.c_int), ("inner_tol", C.c_double), ("max_outer", C.c_int),
+                ("block_size", C.c_int)]
+
+
+class dm_solve_stats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("outer_iterations", C.c_int), ("solved", C.c_int),
+                ("residual", C.c_double), ("precond_time_s", C.c_double), ("solve_time_s", C.c_double)]
+
+
+_L = None
+
+
+def lib():
+    """Load libdiffmom.so (built by diffmom.jl_amd/build.py); raises when absent."""
+    global _L
+    if _L is not None:
+        return _L
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libdiffmom.so is not built ({LIB_PATH}); run python diffmom.jl_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    vp, i64, d, i32 = C.c_void_p, C.c_int64, C.c_double, C.c_int
+    pp = C.POINTER(C.c_void_p)
+    sig = {
+        "dm_create": ([i32, pp], i32), "dm_destroy": ([vp], i32), "dm_last_error": ([], C.c_char_p),
+        "dm_version": ([], i32), "dm_sync": ([vp], i32),
+        "dm_mesh_create": ([vp, vp, i64, vp, i64, pp], i32), "dm_mesh_free": ([vp], i32),
+        "dm_mesh_check": ([vp, i32, i32, d], i32), "dm_mesh_areas": ([vp, vp], i32),
+        "dm_build_rwg": ([vp, i32, i32, i32, d, pp], i32),
+        "dm_rwg_create": ([vp, i64] + [vp] * 10 + [pp], i32),
+        "dm_rwg_size": ([vp, C.POINTER(i64)], i32), "dm_rwg_export": ([vp] + [vp] * 8, i32),
+        "dm_rwg_free": ([vp], i32), "dm_rwg_adjacent_pairs": ([vp, C.POINTER(i64)], i32),
+        "dm_assemble_z": ([vp, vp, d, d, d, i32, i32, i64, pp], i32),
+        "dm_zmat_size": ([vp, C.POINTER(i64), C.POINTER(i32)], i32),
+        "dm_zmat_get": ([vp, i32, vp], i32), "dm_zmat_entries": ([vp, i32, vp, vp, i64, vp], i32),
+        "dm_zmat_free": ([vp], i32),
+        "dm_apply": ([vp, vp, i32, i32, vp, i64, i64, vp, i64], i32),
+        "dm_mf_create": ([vp, vp, d, d, d, i32, pp], i32), "dm_mf_apply": ([vp, vp, i32, vp, vp], i32),
+        "dm_mf_free": ([vp], i32),
+        "dm_excitation_planewave": ([vp, vp, vp, vp, vp, i64, i32, vp], i32),
+        "dm_solve_opts_default": ([C.POINTER(dm_solve_opts)], None),
+        "dm_solve": ([vp, vp, vp, i64, i64, C.POINTER(dm_solve_opts), vp, i64, C.POINTER(dm_solve_stats)], i32),
+        "dm_radiation_vectors": ([vp, vp, vp, i64, d, d, i32, vp], i32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _L = L
+    return L
+
+
+def _check(rc):
+    if rc != DM_OK:
+        msg = lib().dm_last_error().decode(errors="replace")
+        raise _EXC.get(rc, RuntimeError)(msg)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Context:
+    """One GPU device and its HIP streams (dm_create)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().dm_create(int(device), C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def sync(self):
+        _check(lib().dm_sync(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_CTX = None
+
+
+def default_context():
+    global _CTX
+    if _CTX is None:
+        _CTX = Context(0)
+    return _CTX
+
+
+# ---------------------------------------------------------------------- mesh
+class TriMesh:
+    """Vertices (Nv, 3) float64 and 1-based triangles (Nt, 3) int64 (src/Types.jl:123-129)."""
+
+    def __init__(self, xyz, tri, ctx=None):
+        self.xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        self.tri = np.ascontiguousarray(tri, dtype=np.int64)
+        if self.xyz.ndim != 2 or self.xyz.shape[1] != 3 or self.tri.ndim != 2 or self.tri.shape[1] != 3:
+            raise ValueError("xyz must be (Nv, 3) and tri (Nt, 3)")
+        self.ctx = ctx or default_context()
+        h = C.c_void_p()
+        _check(lib().dm_mesh_create(self.ctx.h, _ptr(self.xyz), self.nv, _ptr(self.tri), self.nt, C.byref(h)))
+        self.h = h
+
+    @property
+    def nv(self):
+        return self.xyz.shape[0]
+
+    @property
+    def nt(self):
+        return self.tri.shape[0]
+
+    def areas(self):
+        out = np.zeros(self.nt)
+        _check(lib().dm_mesh_areas(self.h, _ptr(out)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().dm_mesh_free(self.h)
+            self.h = None
+
+
+def nvertices(mesh):
+    return mesh.nv
+
+
+def ntriangles(mesh):
+    return mesh.nt
+
+
+def make_rect_plate(Lx, Ly, Nx, Ny, ctx=None):
+    """Plate in the xy-plane centred at the origin, (Nx+1)(Ny+1) vertices and
+    2 Nx Ny triangles (src/geometry/Mesh.jl:309-396)."""
+    Lx, Ly = float(Lx), float(Ly)
+    if not (math.isfinite(Lx) and Lx > 0 and math.isfinite(Ly) and Ly > 0):
+        raise ValueError("Lx and Ly must be positive and finite")
+    if int(Nx) < 1 or int(Ny) < 1:
+        raise ValueError("Nx and Ny must be positive")
+    Nx, Ny = int(Nx), int(Ny)
+    hx, hy = Lx / 2, Ly / 2
+
+    def axis(half, cells):
+        i = np.arange(cells + 1, dtype=np.float64)
+        c = half * (2 * (i / cells) - 1)
+        c[0], c[-1] = -half, half
+        return c
+
+    X, Y = np.meshgrid(axis(hx, Nx), axis(hy, Ny))  # rows = iy
+    xyz = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], axis=1)
+    jy, jx = np.meshgrid(np.arange(Ny), np.arange(Nx), indexing="ij")
+    v1 = (jy * (Nx + 1) + jx + 1).ravel()
+    v2, v3, v4 = v1 + 1, v1 + Nx + 2, v1 + Nx + 1
+    tri = np.empty((2 * Nx * Ny, 3), dtype=np.int64)
+    tri[0::2] = np.stack([v1, v2, v3], axis=1)
+    tri[1::2] = np.stack([v1, v3, v4], axis=1)
+    return TriMesh(xyz, tri, ctx)
+
+
+def make_icosphere(radius=0.05, subdivisions=2, ctx=None):
+    """Icosahedron refined by edge midpoints projected to the sphere, scaled by
+    radius (the reference's test helper, test/runtests.jl:373-427)."""
+    phi = (1 + math.sqrt(5.0)) / 2
+    verts = [(-1, phi, 0), (1, phi, 0), (-1, -phi, 0), (1, -phi, 0), (0, -1, phi), (0, 1, phi),
+             (0, -1, -phi), (0, 1, -phi), (phi, 0, -1), (phi, 0, 1), (-phi, 0, -1), (-phi, 0, 1)]
+    verts = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(1, 12, 6), (1, 6, 2), (1, 2, 8), (1, 8, 11), (1, 11, 12), (2, 6, 10), (6, 12, 5), (12, 11, 3),
+             (11, 8, 7), (8, 2, 9), (4, 10, 5), (4, 5, 3), (4, 3, 7), (4, 7, 9), (4, 9, 10), (5, 10, 6),
+             (3, 5, 12), (7, 3, 11), (9, 7, 8), (10, 9, 2)]
+    for _ in range(int(subdivisions)):
+        mid = {}
+        nf = []
+
+        def midpoint(i, j):
+            key = (min(i, j), max(i, j))
+            if key in mid:
+                return mid[key]
+            m = (verts[i - 1] + verts[j - 1]) / 2
+            m = m / math.sqrt(float(m @ m))
+            verts.append(m)
+            mid[key] = len(verts)
+            return len(verts)
+
+        for (i, j, k) in faces:
+            a, b, c = midpoint(i, j), midpoint(j, k), midpoint(k, i)
+            nf += [(i, a, c), (j, b, a), (k, c, b), (a, b, c)]
+        faces = nf
+    xyz = float(radius) * np.array(verts)
+    return TriMesh(xyz, np.array(faces, dtype=np.int64), ctx)
+
+
+# ----------------------------------------------------------------------- RWG
+class RWGData:
+    """RWG basis arrays (src/Types.jl:138-152), indices 1-based, any order."""
+
+    def __init__(self, mesh, handle, arrays):
+        self.mesh = mesh
+        self.h = handle
+        for k, v in arrays.items():
+            setattr(self, k, v)
+
+    @property
+    def nedges(self):
+        return len(self.tplus)
+
+    def adjacent_pair_count(self):
+        c = C.c_int64()
+        _check(lib().dm_rwg_adjacent_pairs(self.h, C.byref(c)))
+        return c.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().dm_rwg_free(self.h)
+            self.h = None
+
+
+def _export_rwg(mesh, h):
+    n = C.c_int64()
+    _check(lib().dm_rwg_size(h, C.byref(n)))
+    n = n.value
+    a = dict(tplus=np.zeros(n, np.int64), tminus=np.zeros(n, np.int64), evert=np.zeros((n, 2), np.int64),
+             vplus_opp=np.zeros(n, np.int64), vminus_opp=np.zeros(n, np.int64), len=np.zeros(n),
+             area_plus=np.zeros(n), area_minus=np.zeros(n))
+    _check(lib().dm_rwg_export(h, *[_ptr(a[k]) for k in ("tplus", "tminus", "evert", "vplus_opp", "vminus_opp",
+                                                       "len", "area_plus", "area_minus")]))
+    a["coeff_plus"] = np.ones(n)
+    a["coeff_minus"] = np.ones(n)
+    return a
+
+
+def build_rwg(mesh, precheck=True, allow_boundary=True, require_closed=False, area_tol_rel=1e-12):
+    """One basis per interior edge, T+ = lower-index triangle (src/basis/RWG.jl:102-143).
+    Basis order: sorted (v1 < v2) edge keys."""
+    h = C.c_void_p()
+    _check(lib().dm_build_rwg(mesh.h, int(precheck), int(allow_boundary), int(require_closed),
+                              float(area_tol_rel), C.byref(h)))
+    return RWGData(mesh, h, _export_rwg(mesh, h))
+
+
+def rwg_from_arrays(mesh, tplus, tminus, vplus_opp, vminus_opp, length, area_plus, area_minus,
+                    evert=None, coeff_plus=None, coeff_minus=None):
+    """RWGData as a caller holds it (e.g. a Julia RWGData in Dict order)."""
+    arr = dict(tplus=np.ascontiguousarray(tplus, np.int64), tminus=np.ascontiguousarray(tminus, np.int64),
+               vplus_opp=np.ascontiguousarray(vplus_opp, np.int64),
+               vminus_opp=np.ascontiguousarray(vminus_opp, np.int64), len=np.ascontiguousarray(length, np.float64),
+               area_plus=np.ascontiguousarray(area_plus, np.float64),
+               area_minus=np.ascontiguousarray(area_minus, np.float64))
+    n = len(arr["tplus"])
+    arr["evert"] = np.ascontiguousarray(evert, np.int64) if evert is not None else np.zeros((n, 2), np.int64)
+    arr["coeff_plus"] = np.ascontiguousarray(coeff_plus, np.float64) if coeff_plus is not None else np.ones(n)
+    arr["coeff_minus"] = np.ascontiguousarray(coeff_minus, np.float64) if coeff_minus is not None else np.ones(n)
+    h = C.c_void_p()
+    _check(lib().dm_rwg_create(mesh.h, n, _ptr(arr["tplus"]), _ptr(arr["tminus"]), _ptr(arr["evert"]),
+                               _ptr(arr["vplus_opp"]), _ptr(arr["vminus_opp"]), _ptr(arr["len"]),
+                               _ptr(arr["area_plus"]), _ptr(arr["area_minus"]), _ptr(arr["coeff_plus"]),
+                               _ptr(arr["coeff_minus"]), C.byref(h)))
+    return RWGData(mesh, h, arr)
+
+
+def _check_pair(mesh, rwg):
+    if rwg.mesh is not mesh:
+        raise ValueError("RWG data must be built from the supplied mesh object.")
+
+
+def assert_mesh_quality(mesh, allow_boundary=True, require_closed=False, area_tol_rel=1e-12):
+    _check(lib().dm_mesh_check(mesh.h, int(allow_boundary), int(require_closed), float(area_tol_rel)))
+
+
+# ------------------------------------------------------------------ matrices
+class ZMatrix:
+    """Device-resident EFIE matrix (complex128 and/or complex64 forms)."""
+
+    def __init__(self, rwg, handle, forms, k, eta0, quad_order):
+        self.rwg = rwg
+        self.h = handle
+        self.forms = forms
+        self.k = k
+        self.eta0 = eta0
+        self.quad_order = quad_order
+        self.N = rwg.nedges
+        self.ctx = rwg.mesh.ctx
+
+    @property
+    def shape(self):
+        return (self.N, self.N)
+
+    def _form(self, form):
+        return form if form is not None else (DM_Z64 if self.forms & DM_Z64 else DM_Z32)
+
+    def to_numpy(self, form=None):
+        out = np.zeros((self.N, self.N), dtype=np.complex128, order="F")
+        _check(lib().dm_zmat_get(self.h, self._form(form), out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def entries(self, m, n, form=None):
+        m = np.ascontiguousarray(m, np.int64)
+        n = np.ascontiguousarray(n, np.int64)
+        out = np.zeros(len(m), np.complex128)
+        _check(lib().dm_zmat_entries(self.h, self._form(form), _ptr(m), _ptr(n), len(m), _ptr(out)))
+        return out
+
+    def matmul(self, X, form=None, adjoint=False):
+        X = np.asarray(X, dtype=np.complex128)
+        vec = X.ndim == 1
+        Xf = np.asfortranarray(X.reshape(self.N, -1))
+        Y = np.zeros_like(Xf, order="F")
+        _check(lib().dm_apply(self.ctx.h, self.h, self._form(form), int(adjoint), Xf.ctypes.data_as(C.c_void_p),
+                              self.N, Xf.shape[1], Y.ctypes.data_as(C.c_void_p), self.N))
+        return Y[:, 0] if vec else Y
+
+    def __matmul__(self, X):
+        return self.matmul(X)
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().dm_zmat_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _k_parts(k):
+    k = complex(k)
+    return k.real, k.imag
+
+
+def assemble_Z_efie(mesh, rwg, k, quad_order=3, eta0=ETA0, mesh_precheck=True, allow_boundary=True,
+                    require_closed=False, area_tol_rel=1e-12, max_output_bytes=_DEFAULT_MAX_DENSE_PAYLOAD_BYTES,
+                    forms=DM_Z64, device=False):
+    """Dense EFIE matrix (src/assembly/EFIE.jl:699-753).  Returns a complex128
+    numpy matrix like the reference, or the device-resident ZMatrix when
+    device=True (forms selects complex128 and/or complex64 storage)."""
+    _check_pair(mesh, rwg)
+    N = rwg.nedges
+    if max_output_bytes is not None and N * N * 16 > max_output_bytes:
+        raise ValueError(f"dense EFIE matrix requires {N * N * 16} bytes, exceeding max_output_bytes={max_output_bytes}")
+    if mesh_precheck:
+        assert_mesh_quality(mesh, allow_boundary, require_closed, area_tol_rel)
+    kr, ki = _k_parts(k)
+    h = C.c_void_p()
+    _check(lib().dm_assemble_z(mesh.ctx.h, rwg.h, kr, ki, float(eta0), int(quad_order), int(forms),
+                               int(max_output_bytes or 0), C.byref(h)))
+    Z = ZMatrix(rwg, h, forms, complex(k), float(eta0), quad_order)
+    if device:
+        return Z
+    out = Z.to_numpy()
+    Z.free()
+    return out
+
+
+class MatrixFreeEFIEOperator:
+    """Matrix-free EFIE operator (src/assembly/EFIE.jl:759-1007): every product
+    recomputes all entries on the GPU in FP64; nothing of size N^2 is stored."""
+
+    def __init__(self, rwg, h, k, eta0, quad_order, adjoint=False):
+        self.rwg, self.h, self.k, self.eta0, self.quad_order = rwg, h, k, eta0, quad_order
+        self.N = rwg.nedges
+        self._adjoint = adjoint
+        self.ctx = rwg.mesh.ctx
+
+    @property
+    def shape(self):
+        return (self.N, self.N)
+
+    def __matmul__(self, x):
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        if x.shape != (self.N,):
+            raise DimensionMismatch(f"x length {x.shape} != {self.N}")
+        y = np.zeros(self.N, np.complex128)
+        _check(lib().dm_mf_apply(self.ctx.h, self.h, int(self._adjoint), _ptr(x), _ptr(y)))
+        return y
+
+    def adjoint(self):
+        return MatrixFreeEFIEOperator(self.rwg, self.h, self.k, self.eta0, self.quad_order, not self._adjoint)
+
+    @property
+    def H(self):
+        return self.adjoint()
+
+
+def matrixfree_efie_operator(mesh, rwg, k, quad_order=3, eta0=ETA0, mesh_precheck=True, allow_boundary=True,
+                             require_closed=False, area_tol_rel=1e-12):
+    _check_pair(mesh, rwg)
+    if mesh_precheck:
+        assert_mesh_quality(mesh, allow_boundary, require_closed, area_tol_rel)
+    kr, ki = _k_parts(k)
+    h = C.c_void_p()
+    _check(lib().dm_mf_create(mesh.ctx.h, rwg.h, kr, ki, float(eta0), int(quad_order), C.byref(h)))
+    return MatrixFreeEFIEOperator(rwg, h, complex(k), float(eta0), quad_order)
+
+
+# ---------------------------------------------------------------- excitation
+@dataclass
+class PlaneWaveExcitation:
+    """src/assembly/Excitation.jl:86-101"""
+    k_vec: np.ndarray
+    E0: float
+    pol: np.ndarray
+
+
+def make_plane_wave(k_vec, E0, pol):
+    return PlaneWaveExcitation(np.asarray(k_vec, float), float(E0), np.asarray(pol, float))
+
+
+def assemble_multiple_excitations(mesh, rwg, excitations, quad_order=3):
+    _check_pair(mesh, rwg)
+    M = len(excitations)
+    kv = np.ascontiguousarray(np.array([e.k_vec for e in excitations], float).reshape(M, 3))
+    pol = np.ascontiguousarray(np.array([e.pol for e in excitations], float).reshape(M, 3))
+    E0 = np.ascontiguousarray([float(e.E0) for e in excitations])
+    V = np.zeros((rwg.nedges, M), dtype=np.complex128, order="F")
+    _check(lib().dm_excitation_planewave(mesh.ctx.h, rwg.h, _ptr(kv), _ptr(pol), _ptr(E0), M, int(quad_order),
+                                         V.ctypes.data_as(C.c_void_p)))
+    return V
+
+
+def assemble_v_plane_wave(mesh, rwg, k_vec, E0, pol, quad_order=3):
+    return assemble_multiple_excitations(mesh, rwg, [make_plane_wave(k_vec, E0, pol)], quad_order)[:, 0]
+
+
+def assemble_excitation(mesh, rwg, excitation, quad_order=3):
+    if not isinstance(excitation, PlaneWaveExcitation):
+        raise NotImplementedError("only plane-wave excitations are on the GPU path")
+    return assemble_v_plane_wave(mesh, rwg, excitation.k_vec, excitation.E0, excitation.pol, quad_order)
+
+
+# ------------------------------------------------------------------- solvers
+@dataclass
+class NearFieldPreconditionerData:
+    """Near-field preconditioner spec (src/solver/NearFieldPreconditioner.jl:84-110).
+    On the GPU the near field is taken as dense blocks of spatially consecutive
+    bases (block_size each) factorised by batched inversion."""
+    cutoff: float
+    factorization: str = "lu"
+    block_size: int = 128
+
+
+def build_nearfield_preconditioner(Z, mesh, rwg, cutoff, factorization="lu", block_size=128):
+    if factorization not in ("lu", "diag"):
+        raise ValueError(f"unsupported factorization {factorization!r} on the GPU path (use 'lu' or 'diag')")
+    if not (cutoff >= 0):
+        raise ValueError("cutoff must be nonnegative")
+    return NearFieldPreconditionerData(float(cutoff), factorization, int(block_size))
+
+
+@dataclass
+class SolveStats:
+    niter: int
+    solved: bool
+    residual: float
+    outer: int = 0
+    precond_time_s: float = 0.0
+    solve_time_s: float = 0.0
+    residuals: list = field(default_factory=list)
+
+
+def _solve(Z, B, method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, check=True,
+           inner_tol=0.0, max_outer=10):
+    if not isinstance(Z, ZMatrix):
+        raise TypeError("GPU solves take the device matrix (assemble_Z_efie(..., device=True))")
+    B = np.asarray(B, dtype=np.complex128)
+    vec = B.ndim == 1
+    Bf = np.asfortranarray(B.reshape(Z.N, -1))
+    if Bf.shape[0] != Z.N:
+        raise DimensionMismatch(f"rhs length {Bf.shape[0]} != {Z.N}")
+    X = np.zeros_like(Bf, order="F")
+    o = dm_solve_opts()
+    lib().dm_solve_opts_default(C.byref(o))
+    o.method = method
+    o.tol = float(tol)
+    o.maxiter = int(maxiter)
+    o.restart = int(restart)
+    o.check_convergence = int(check)
+    o.precond_side = 1 if side == "right" else 0
+    o.inner_tol = float(inner_tol)
+    o.max_outer = int(max_outer)
+    if precond is None:
+        o.precond = DM_PRECOND_NONE
+    elif isinstance(precond, NearFieldPreconditionerData):
+        o.precond = DM_PRECOND_DIAG if precond.factorization == "diag" else DM_PRECOND_NEARFIELD
+        o.block_size = precond.block_size
+        o.nf_cutoff = precond.cutoff
+    elif precond == "diag":
+        o.precond = DM_PRECOND_DIAG
+    else:
+        raise ValueError(f"unknown preconditioner {precond!r}")
+    st = dm_solve_stats()
+    _check(lib().dm_solve(Z.ctx.h, Z.h, Bf.ctypes.data_as(C.c_void_p), Z.N, Bf.shape[1], C.byref(o),
+                          X.ctypes.data_as(C.c_void_p), Z.N, C.byref(st)))
+    stats = SolveStats(st.iterations, bool(st.solved), st.residual, st.outer_iterations, st.precond_time_s,
+                       st.solve_time_s)
+    return (X[:, 0] if vec else X), stats
+
+
+def solve_forward(Z, v, solver="direct", **kw):
+    """src/solver/Solve.jl:1342-1382: dense LU (:direct) or GMRES with a true-residual guard."""
+    if solver == "direct":
+        x, _ = _solve(Z, v, DM_METHOD_LU)
+        return x
+    if solver == "gmres":
+        x, st = solve_gmres(Z, v, **kw)
+        if not st.residual <= 100 * kw.get("tol", 1e-8):
+            raise ConvergenceError(f"forward GMRES true residual too large: relative_residual={st.residual}")
+        return x
+    raise ValueError(f"unknown solver {solver!r}")
+
+
+def solve_gmres(Z, rhs, preconditioner=None, precond_side="left", tol=1e-8, maxiter=200, memory=20,
+                verbose=False, check_gmres_convergence=True, restart=0):
+    """src/solver/IterativeSolve.jl:341-392 (full GMRES unless restart > 0)."""
+    if not (math.isfinite(tol) and tol > 0):
+        raise ValueError(f"GMRES tol must be finite and positive, got {tol}")
+    if maxiter < 1:
+        raise ValueError(f"GMRES maxiter must be at least 1, got {maxiter}")
+    if memory < 1:
+        raise ValueError(f"GMRES memory must be at least 1, got {memory}")
+    if precond_side not in ("left", "right"):
+        raise ValueError(f"Invalid precond_side: {precond_side}")
+    return _solve(Z, rhs, DM_METHOD_GMRES, preconditioner, precond_side, tol, maxiter, restart,
+                  check_gmres_convergence)
+
+
+def solve_mixed(Z, rhs, preconditioner=None, tol=1e-6, maxiter=300, inner_tol=0.0, max_outer=10,
+                check_convergence=True, restart=0):
+    """Mixed precision: FGMRES on the complex64 matrix with FP64 Arnoldi, FP64
+    iterative refinement on the complex128 matrix (no reference counterpart)."""
+    return _solve(Z, rhs, DM_METHOD_MIXED, preconditioner, "right", tol, maxiter, restart, check_convergence,
+                  inner_tol, max_outer)
+
+
+# --------------------------------------------------------------- far field
+@dataclass
+class SphGrid:
+    rhat: np.ndarray  # (NΩ, 3)
+    theta: np.ndarray
+    phi: np.ndarray
+    w: np.ndarray
+
+
+def make_sph_grid(Ntheta, Nphi):
+    """Uniform (θ, φ) grid, src/postprocessing/FarField.jl:216-277."""
+    if Ntheta <= 0 or Nphi <= 0:
+        raise ValueError("Ntheta and Nphi must be positive")
+    dth, dph = math.pi / Ntheta, 2 * math.pi / Nphi
+    th = np.repeat((np.arange(1, Ntheta + 1) - 0.5) * dth, Nphi)
+    ph = np.tile((np.arange(1, Nphi + 1) - 0.5) * dph, Ntheta)
+    rhat = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], axis=1)
+    return SphGrid(np.ascontiguousarray(rhat), th, ph, np.sin(th) * dth * dph)
+
+
+def radiation_vectors(mesh, rwg, grid, k, quad_order=3, eta0=ETA0):
+    """G (3NΩ x N): E∞ = G I (src/postprocessing/FarField.jl:425-574), on the GPU."""
+    _check_pair(mesh, rwg)
+    rhat = np.ascontiguousarray(grid.rhat if isinstance(grid, SphGrid) else grid, dtype=np.float64)
+    no = rhat.shape[0]
+    G = np.zeros((3 * no, rwg.nedges), dtype=np.complex128, order="F")
+    _check(lib().dm_radiation_vectors(mesh.ctx.h, rwg.h, _ptr(rhat), no, float(k), float(eta0), int(quad_order),
+                                      G.ctypes.data_as(C.c_void_p)))
+    return G
+
+
+def compute_farfield(G, I, NΩ):
+    """E∞ (3, NΩ) = G I (src/postprocessing/FarField.jl:584-605)."""
+    if G.shape[0] != 3 * NΩ:
+        raise DimensionMismatch(f"G has {G.shape[0]} rows, expected {3 * NΩ}")
+    return (G @ np.asarray(I, np.complex128)).reshape((3, NΩ), order="F")
+
+
+def bistatic_rcs(E_ff, E0=1.0):
+    """σ = 4π |E∞|² / |E0|² (src/postprocessing/Diagnostics.jl:460-473)."""
+    return 4 * math.pi * np.sum(np.abs(E_ff) ** 2, axis=0) / abs(E0) ** 2
+
+
+# -------------------------------------------------------------- workflow
+@dataclass
+class ScatteringResult:
+    """src/Types.jl:1297-1308"""
+    I_coeffs: np.ndarray
+    method: str
+    N: int
+    assembly_time_s: float
+    solve_time_s: float
+    preconditioner_time_s: float
+    gmres_iters: int
+    gmres_residual: float
+    mesh_report: dict
+    warnings: list
+
+
+def mesh_resolution_report(mesh, freq_hz, points_per_wavelength=10.0, c0=C0_DEFAULT):
+    """src/geometry/Mesh.jl:3418-3471"""
+    lam = c0 / freq_hz
+    t = mesh.tri - 1
+    e = np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]])
+    e = np.unique(np.sort(e, axis=1), axis=0)
+    lens = np.sort(np.linalg.norm(mesh.xyz[e[:, 0]] - mesh.xyz[e[:, 1]], axis=1))
+    target = lam / points_per_wavelength
+    return dict(wavelength_m=lam, target_max_edge_m=target, edge_min_m=lens[0], edge_max_m=lens[-1],
+                edge_median_m=float(np.median(lens)), edge_max_over_lambda=lens[-1] / lam,
+                meets_target=bool(lens[-1] <= target))
+
+
+def solve_scattering(mesh, freq_hz, excitation, method="auto", dense_direct_limit=2000, dense_gmres_limit=10000,
+                     mlfma_threshold=50000, check_resolution=True, points_per_wavelength=10.0,
+                     error_on_underresolved=False, gmres_tol=1e-6, gmres_maxiter=300, check_gmres_convergence=True,
+                     nf_cutoff_lambda=1.0, preconditioner="auto", verbose=False, quad_order=3, c0=C0_DEFAULT,
+                     max_dense_matrix_bytes=_DEFAULT_MAX_DENSE_PAYLOAD_BYTES, precision="fp64", nf_block_size=128):
+    """src/Workflow.jl:67-317 for the dense methods (:dense_direct, :dense_gmres).
+    ``precision="mixed"`` selects the complex64 + FP64-refinement solve.  The
+    ACA/MLFMA branches of the reference are not part of this library."""
+    f = float(freq_hz)
+    if not (math.isfinite(f) and f > 0):
+        raise ValueError(f"solve_scattering: freq_hz must be finite and positive, got {freq_hz}")
+    if method not in ("auto", "dense_direct", "dense_gmres", "aca_gmres", "mlfma"):
+        raise ValueError("solve_scattering: method must be :auto, :dense_direct, :dense_gmres, :aca_gmres, or :mlfma")
+    if preconditioner not in ("auto", "lu", "ilu", "diag", "none"):
+        raise ValueError("solve_scattering: preconditioner must be :auto, :lu, :ilu, :diag, or :none")
+    warnings = []
+    lam = c0 / f
+    k = 2 * math.pi * f / c0
+    if isinstance(excitation, PlaneWaveExcitation):
+        kn = float(np.linalg.norm(excitation.k_vec))
+        if not abs(kn - k) <= 1e-8 * k:
+            raise ValueError("solve_scattering: plane-wave |k_vec| does not match the frequency")
+    report = mesh_resolution_report(mesh, f, points_per_wavelength, c0)
+    if check_resolution and not report["meets_target"]:
+        msg = (f"Mesh under-resolved: edge_max/lambda={report['edge_max_over_lambda']:.3f}, "
+               f"target <= {1.0 / points_per_wavelength:.3f}. Results may be inaccurate.")
+        warnings.append(msg)
+        if error_on_underresolved:
+            raise RuntimeError(f"solve_scattering: {msg}")
+    rwg = build_rwg(mesh)
+    N = rwg.nedges
+    sel = method
+    if method == "auto":
+        sel = ("dense_direct" if N <= dense_direct_limit else "dense_gmres" if N <= dense_gmres_limit
+               else "aca_gmres" if N <= mlfma_threshold else "mlfma")
+    if sel in ("aca_gmres", "mlfma"):
+        raise NotImplementedError(f"solve_scattering: method {sel} (N={N}) is outside the dense GPU path; "
+                                  "force method='dense_gmres'")
+    if preconditioner == "ilu":
+        raise ValueError("solve_scattering: preconditioner=:ilu is supported only by the MLFMA path")
+    if N * N * 16 > max_dense_matrix_bytes:
+        raise ValueError("solve_scattering dense EFIE matrix exceeds max_dense_matrix_bytes")
+    if isinstance(excitation, PlaneWaveExcitation):
+        v = assemble_excitation(mesh, rwg, excitation, quad_order)
+    else:
+        v = np.asarray(excitation, dtype=np.complex128)
+        if v.shape != (N,):
+            raise DimensionMismatch(f"solve_scattering: excitation length {v.shape} != N={N}")
+    if not np.all(np.isfinite(v)):
+        raise ValueError("solve_scattering: excitation vector must contain only finite values")
+    forms = DM_Z64 | DM_Z32 if (precision == "mixed" and sel == "dense_gmres") else DM_Z64
+    t0 = time.perf_counter()
+    Z = assemble_Z_efie(mesh, rwg, k, quad_order=quad_order, mesh_precheck=False, max_output_bytes=None,
+                        forms=forms, device=True)
+    mesh.ctx.sync()
+    t_asm = time.perf_counter() - t0
+    iters, resid, t_pre = -1, float("nan"), 0.0
+    if sel == "dense_direct":
+        t1 = time.perf_counter()
+        I = solve_forward(Z, v, solver="direct")
+        t_solve = time.perf_counter() - t1
+    else:
+        P = None
+        if preconditioner in ("auto", "lu"):
+            P = build_nearfield_preconditioner(Z, mesh, rwg, nf_cutoff_lambda * lam, "lu", nf_block_size)
+        elif preconditioner == "diag":
+            P = build_nearfield_preconditioner(Z, mesh, rwg, nf_cutoff_lambda * lam, "diag")
+        t1 = time.perf_counter()
+        if precision == "mixed":
+            I, st = solve_mixed(Z, v, P, tol=gmres_tol, maxiter=gmres_maxiter,
+                                check_convergence=check_gmres_convergence)
+        else:
+            I, st = solve_gmres(Z, v, P, precond_side="right", tol=gmres_tol, maxiter=gmres_maxiter,
+                                check_gmres_convergence=check_gmres_convergence)
+        t_solve = time.perf_counter() - t1
+        iters, resid, t_pre = st.niter, st.residual, st.precond_time_s
+    Z.free()
+    return ScatteringResult(I, sel, N, t_asm, t_solve, t_pre, iters, resid, report, warnings)

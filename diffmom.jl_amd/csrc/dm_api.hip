@@ -1,0 +1,958 @@
+// dm_api.hip — the C ABI of libdiffmom (include/diffmom.h) and the host-side
+// orchestration of assembly, products and Krylov solves.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <memory>
+
+#include "dm_common.h"
+#include "k_linalg.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return DM_OK;
+  } catch (const DmError& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return DM_ERR_MEMORY;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return DM_ERR_DEVICE;
+  }
+}
+
+struct CtxScope {
+  std::lock_guard<std::mutex> lk;
+  explicit CtxScope(dm_ctx* c) : lk(c->mu) { HIP_OK(hipSetDevice(c->device)); }
+};
+
+typedef std::complex<double> cd;
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// _validated_efie_prefactors: src/assembly/EFIE.jl:297-330
+void validate_prefactors(double kr, double ki, double eta0) {
+  DM_REQUIRE(std::isfinite(kr) && std::isfinite(ki) && (kr != 0 || ki != 0), DM_ERR_ARGUMENT,
+             "EFIE wavenumber must be finite and nonzero");
+  DM_REQUIRE(std::isfinite(eta0) && eta0 != 0, DM_ERR_ARGUMENT, "EFIE eta0 must be numeric, finite, and nonzero");
+  cd k(kr, ki);
+  cd k2 = k * k;
+  DM_REQUIRE(std::isfinite(k2.real()) && std::isfinite(k2.imag()) && std::abs(k2) != 0, DM_ERR_OVERFLOW,
+             "EFIE wavenumber squared is not representable");
+  cd inv = 1.0 / k2;
+  DM_REQUIRE(std::isfinite(inv.real()) && std::isfinite(inv.imag()) && std::abs(inv) != 0, DM_ERR_OVERFLOW,
+             "EFIE inverse wavenumber squared is not representable");
+  cd om = k * eta0;
+  DM_REQUIRE(std::isfinite(om.real()) && std::isfinite(om.imag()) && std::abs(om) != 0, DM_ERR_OVERFLOW,
+             "EFIE k*eta0 prefactor is not representable");
+}
+
+struct NearTables {
+  DBuf<double> jself, jadj;
+  DBuf<int> flag;
+};
+
+AsmArgs make_args(dm_rwg& r, const TriTables& T, double kr, double ki, double eta0, i64 ld, NearTables& nt) {
+  AsmArgs a;
+  a.n = r.n;
+  a.ld = ld;
+  a.nt = r.mesh->nt;
+  a.nblk = r.nblk;
+  a.TB = r.TB;
+  a.blk_off = r.d_blk_off.p;
+  a.blk_tri = r.d_blk_tri.p;
+  a.hrwg = r.d_hrwg.p;
+  a.hsig = r.d_hsig.p;
+  a.nb = r.d_nb.p;
+  a.ext = r.d_ext.p;
+  a.len = r.d_len.p;
+  a.cen = T.cen.p;
+  a.uq = T.uq.p;
+  a.av = T.av.p;
+  double xi[14], w[7];
+  int nq = dmh::quad_rule(T.quad_order, xi, w);
+  for (int i = 0; i < 7; ++i) a.w[i] = i < nq ? w[i] : 0.0;
+  a.nq = nq;
+  a.k_re = kr;
+  a.k_im = ki;
+  a.eta0 = eta0;
+  hipStream_t s = r.ctx->stream;
+  nt.jself.alloc(a.nt * 18);
+  nt.jadj.alloc(a.nt * 54);
+  nt.flag.alloc(1);
+  HIP_OK(hipMemsetAsync(nt.flag.p, 0, sizeof(int), s));
+  a.jself = nt.jself.p;
+  a.jadj = nt.jadj.p;
+  a.flag = nt.flag.p;
+  launch_near_tables(a, T, nt.jself.p, nt.jadj.p, s);
+  return a;
+}
+
+void check_flag(NearTables& nt, hipStream_t s) {
+  int f = 0;
+  HIP_OK(hipMemcpyAsync(&f, nt.flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  DM_REQUIRE(f == 0, DM_ERR_OVERFLOW, "EFIE entry is outside the representable ComplexF64 range.");
+}
+
+i64 round_ld(i64 n) { return (n + 63) / 64 * 64; }
+
+// ------------------------------------------------------------ device operators
+struct DevOp {
+  virtual ~DevOp() = default;
+  virtual void apply(const double* x, double* y, hipStream_t s) = 0;
+};
+struct OpZ64 : DevOp {
+  const dm_zmat* z;
+  explicit OpZ64(const dm_zmat* z_) : z(z_) {}
+  void apply(const double* x, double* y, hipStream_t s) override { launch_gemv_z64(z->n, z->ld, z->z64.p, x, y, s); }
+};
+struct OpZ32 : DevOp {
+  const dm_zmat* z;
+  explicit OpZ32(const dm_zmat* z_) : z(z_) {}
+  void apply(const double* x, double* y, hipStream_t s) override { launch_gemv_z32(z->n, z->ld, z->z32.p, x, y, s); }
+};
+
+struct DevPrec {
+  virtual ~DevPrec() = default;
+  virtual void apply(const double* x, double* y, hipStream_t s) = 0;
+};
+struct PrecDiag : DevPrec {
+  i64 n;
+  DBuf<double> d;
+  void apply(const double* x, double* y, hipStream_t s) override { dml::diag_apply(n, d.p, x, y, s); }
+};
+struct PrecBlock : DevPrec {
+  i64 n;
+  int bs, nb;
+  DBuf<double> inv;
+  void apply(const double* x, double* y, hipStream_t s) override { dml::block_apply(n, bs, nb, inv.p, x, y, s); }
+};
+
+std::unique_ptr<DevPrec> build_precond(const dm_zmat* z, bool use32, const dm_solve_opts& o, hipStream_t s) {
+  i64 n = z->n;
+  if (o.precond == DM_PRECOND_NONE) return nullptr;
+  if (o.precond == DM_PRECOND_DIAG) {
+    auto p = std::make_unique<PrecDiag>();
+    p->n = n;
+    p->d.alloc(2 * n);
+    if (!use32) {
+      dml::diag_inv(n, z->ld, z->z64.p, p->d.p, s);
+    } else {
+      // widen the diagonal of Z32 through a one-block extraction per element
+      DBuf<double> tmp;
+      tmp.alloc(2 * n);
+      std::vector<i64> idx(n);
+      for (i64 i = 0; i < n; ++i) idx[i] = i;
+      DBuf<i64> di;
+      di.upload(idx, s);
+      std::vector<int> iden(n);
+      for (i64 i = 0; i < n; ++i) iden[i] = (int)i;
+      DBuf<int> dm;
+      dm.upload(iden, s);
+      dml::entries32(z->ld, n, dm.p, di.p, di.p, z->z32.p, tmp.p, s);
+      // d = 1 / diag
+      std::vector<double> h(2 * n);
+      HIP_OK(hipMemcpyAsync(h.data(), tmp.p, 16 * n, hipMemcpyDeviceToHost, s));
+      HIP_OK(hipStreamSynchronize(s));
+      for (i64 i = 0; i < n; ++i) {
+        cd v(h[2 * i], h[2 * i + 1]);
+        DM_REQUIRE(std::abs(v) > 0, DM_ERR_ARGUMENT, "zero diagonal entry in diagonal preconditioner");
+        cd iv = 1.0 / v;
+        h[2 * i] = iv.real();
+        h[2 * i + 1] = iv.imag();
+      }
+      p->d.upload(h, s);
+      HIP_OK(hipStreamSynchronize(s));
+    }
+    return p;
+  }
+  if (o.precond == DM_PRECOND_NEARFIELD) {
+    auto p = std::make_unique<PrecBlock>();
+    p->n = n;
+    p->bs = o.block_size > 0 ? o.block_size : 128;
+    DM_REQUIRE(p->bs <= 1024, DM_ERR_ARGUMENT, "near-field block size must be <= 1024");
+    p->bs = (int)std::min<i64>(p->bs, n);
+    p->nb = (int)((n + p->bs - 1) / p->bs);
+    p->inv.alloc((size_t)p->nb * p->bs * p->bs * 2);
+    if (use32)
+      dml::extract_blocks32(n, z->ld, p->bs, p->nb, z->z32.p, p->inv.p, s);
+    else
+      dml::extract_blocks64(n, z->ld, p->bs, p->nb, z->z64.p, p->inv.p, s);
+    DBuf<int> st;
+    st.alloc(p->nb);
+    HIP_OK(hipMemsetAsync(st.p, 0, sizeof(int) * p->nb, s));
+    dml::block_inverse(p->bs, p->nb, p->inv.p, st.p, s);
+    std::vector<int> hs(p->nb);
+    HIP_OK(hipMemcpyAsync(hs.data(), st.p, sizeof(int) * p->nb, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    for (int b = 0; b < p->nb; ++b)
+      DM_REQUIRE(hs[b] == 0, DM_ERR_ARGUMENT, "near-field block " + std::to_string(b) + " is singular");
+    return p;
+  }
+  dm_throw(DM_ERR_ARGUMENT, "preconditioner kind not available for this solve");
+}
+
+// ------------------------------------------------------------------- GMRES
+struct Krylov {
+  i64 n = 0;
+  int m = 0;  // basis capacity
+  DBuf<double> V, Zs, w, t, hbuf, ybuf;
+  void reserve(i64 n_, int m_, bool flexible) {
+    n = n_;
+    m = m_;
+    V.alloc((size_t)2 * n * (m + 1));
+    if (flexible) Zs.alloc((size_t)2 * n * m);
+    w.alloc(2 * n);
+    t.alloc(2 * n);
+    hbuf.alloc(2 * (2 * (m + 1) + 1));
+    ybuf.alloc(2 * (m + 1));
+  }
+};
+
+struct GmresOut {
+  int iters = 0;
+  bool solved = false;
+  double est = 0;
+};
+
+// Restarted (restart = 0: full) GMRES with classical Gram-Schmidt applied
+// twice (CGS2) for the Arnoldi step, FP64 throughout; right preconditioning is
+// flexible (FGMRES: the preconditioned directions are stored), left
+// preconditioning follows Krylov.jl's semantics (stop on ||M r|| <= rtol ||M b||).
+// x holds the initial guess on entry.
+GmresOut gmres(i64 n, DevOp& A, DevPrec* M, int side, const double* b, double* x, double rtol, int maxiter,
+               int restart, Krylov& K, hipStream_t s) {
+  GmresOut out;
+  int mcap = restart > 0 ? std::min(restart, maxiter) : maxiter;
+  bool right = M && side == 1;
+  bool left = M && side == 0;
+  K.reserve(n, mcap, right);
+  double* V = K.V.p;
+  const i64 ldv = 2 * n;  // doubles per column
+  auto col = [&](double* base, int j) { return base + (size_t)ldv * j; };
+  // eps from the initial (preconditioned) residual with x0
+  auto residual = [&](double* r) {
+    A.apply(x, K.t.p, s);
+    HIP_OK(hipMemcpyAsync(r, b, 16 * n, hipMemcpyDeviceToDevice, s));
+    dml::axpby(n, -1, 0, K.t.p, 1, 0, r, s);
+    if (left) {
+      M->apply(r, K.t.p, s);
+      HIP_OK(hipMemcpyAsync(r, K.t.p, 16 * n, hipMemcpyDeviceToDevice, s));
+    }
+  };
+  auto nrm = [&](const double* v) {
+    double h;
+    dml::norm2(n, v, K.hbuf.p, s);
+    HIP_OK(hipMemcpyAsync(&h, K.hbuf.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    return std::sqrt(h);
+  };
+  double* r = col(V, 0);
+  residual(r);
+  double beta = nrm(r);
+  double eps = rtol * beta;
+  if (!(beta > 0)) {
+    out.solved = true;
+    return out;
+  }
+  std::vector<cd> H((size_t)(mcap + 1) * mcap), g(mcap + 1), sn(mcap);
+  std::vector<double> cs(mcap);
+  std::vector<double> hh(2 * (2 * (mcap + 1) + 1));
+  int total = 0;
+  while (true) {
+    dml::axpby(n, 1.0 / beta, 0, r, 0, 0, r, s);
+    std::fill(g.begin(), g.end(), cd(0));
+    g[0] = beta;
+    int j = 0;
+    double est = beta;
+    for (; j < mcap && total < maxiter; ++j) {
+      double* vj = col(V, j);
+      if (right) {
+        double* zj = col(K.Zs.p, j);
+        M->apply(vj, zj, s);
+        A.apply(zj, K.w.p, s);
+      } else if (left) {
+        A.apply(vj, K.t.p, s);
+        M->apply(K.t.p, K.w.p, s);
+      } else {
+        A.apply(vj, K.w.p, s);
+      }
+      int k = j + 1;
+      double* h1 = K.hbuf.p;
+      double* h2 = K.hbuf.p + 2 * k;
+      dml::dots(n, ldv / 2, k, V, K.w.p, h1, s);
+      dml::update(n, ldv / 2, k, V, h1, K.w.p, s);
+      dml::dots(n, ldv / 2, k, V, K.w.p, h2, s);
+      dml::update(n, ldv / 2, k, V, h2, K.w.p, s);
+      dml::norm2(n, K.w.p, K.hbuf.p + 4 * k, s);
+      HIP_OK(hipMemcpyAsync(hh.data(), K.hbuf.p, sizeof(double) * (4 * k + 1), hipMemcpyDeviceToHost, s));
+      HIP_OK(hipStreamSynchronize(s));
+      cd* Hj = &H[(size_t)j * (mcap + 1)];
+      for (int i = 0; i < k; ++i) Hj[i] = cd(hh[2 * i] + hh[2 * k + 2 * i], hh[2 * i + 1] + hh[2 * k + 2 * i + 1]);
+      double hn = std::sqrt(hh[4 * k]);
+      Hj[k] = hn;
+      if (hn > 0 && k <= mcap) dml::axpby(n, 1.0 / hn, 0, K.w.p, 0, 0, col(V, k), s);
+      for (int i = 0; i < j; ++i) {
+        cd t = cs[i] * Hj[i] + sn[i] * Hj[i + 1];
+        Hj[i + 1] = -std::conj(sn[i]) * Hj[i] + cs[i] * Hj[i + 1];
+        Hj[i] = t;
+      }
+      cd a = Hj[j], bb = Hj[j + 1];
+      double rho = std::hypot(std::abs(a), std::abs(bb));
+      if (std::abs(a) == 0) {
+        cs[j] = 0;
+        sn[j] = std::abs(bb) > 0 ? std::conj(bb) / std::abs(bb) : cd(1);
+      } else {
+        cs[j] = std::abs(a) / rho;
+        sn[j] = (a / std::abs(a)) * std::conj(bb) / rho;
+      }
+      Hj[j] = cs[j] * a + sn[j] * bb;
+      Hj[j + 1] = 0;
+      g[j + 1] = -std::conj(sn[j]) * g[j];
+      g[j] = cs[j] * g[j];
+      est = std::abs(g[j + 1]);
+      ++total;
+      if (est <= eps || hn == 0) {
+        ++j;
+        break;
+      }
+    }
+    // y = H^-1 g, x += V y (or Zs y)
+    std::vector<cd> y(j);
+    for (int i = j - 1; i >= 0; --i) {
+      cd acc = g[i];
+      for (int l = i + 1; l < j; ++l) acc -= H[(size_t)l * (mcap + 1) + i] * y[l];
+      y[i] = acc / H[(size_t)i * (mcap + 1) + i];
+    }
+    if (j > 0) {
+      HIP_OK(hipMemcpyAsync(K.ybuf.p, y.data(), 16 * j, hipMemcpyHostToDevice, s));
+      dml::combine(n, ldv / 2, j, right ? K.Zs.p : V, K.ybuf.p, x, s);
+      HIP_OK(hipStreamSynchronize(s));
+    }
+    out.iters = total;
+    out.est = est;
+    if (est <= eps) {
+      out.solved = true;
+      break;
+    }
+    if (total >= maxiter) break;
+    // restart from the true (preconditioned) residual
+    residual(r);
+    beta = nrm(r);
+    if (beta <= eps) {
+      out.solved = true;
+      out.est = beta;
+      break;
+    }
+  }
+  return out;
+}
+
+double true_residual(DevOp& A, i64 n, const double* b, const double* x, double* tmp, double* hbuf, hipStream_t s) {
+  A.apply(x, tmp, s);
+  dml::axpby(n, 1, 0, b, -1, 0, tmp, s);
+  double h[2];
+  dml::norm2(n, tmp, hbuf, s);
+  dml::norm2(n, b, hbuf + 1, s);
+  HIP_OK(hipMemcpyAsync(h, hbuf, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  if (h[1] == 0) return h[0] == 0 ? 0.0 : INFINITY;
+  return std::sqrt(h[0] / h[1]);
+}
+
+void check_ctx(const void* a, const char* what) { DM_REQUIRE(a != nullptr, DM_ERR_ARGUMENT, std::string(what) + " is NULL"); }
+
+}  // namespace
+
+extern "C" {
+
+const char* dm_last_error(void) { return g_err.c_str(); }
+int dm_version(void) { return 100; }
+
+int dm_create(int device, dm_ctx** out) {
+  return guard([&] {
+    check_ctx(out, "output handle");
+    int count = 0;
+    HIP_OK(hipGetDeviceCount(&count));
+    DM_REQUIRE(device >= 0 && device < count, DM_ERR_ARGUMENT, "device index out of range");
+    HIP_OK(hipSetDevice(device));
+    auto c = std::make_unique<dm_ctx>();
+    c->device = device;
+    HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_OK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    HIP_OK(hipGetDeviceProperties(&prop, device));
+    c->cu_count = prop.multiProcessorCount;
+    *out = c.release();
+  });
+}
+
+int dm_destroy(dm_ctx* ctx) {
+  return guard([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream2);
+    delete ctx;
+  });
+}
+
+int dm_sync(dm_ctx* ctx) {
+  return guard([&] {
+    check_ctx(ctx, "context");
+    CtxScope sc(ctx);
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream2));
+  });
+}
+
+int dm_mesh_create(dm_ctx* ctx, const double* xyz, int64_t nv, const int64_t* tri, int64_t nt, dm_mesh** out) {
+  return guard([&] {
+    check_ctx(ctx, "context");
+    check_ctx(out, "output handle");
+    DM_REQUIRE(nv >= 0 && nt >= 0, DM_ERR_ARGUMENT, "negative mesh size");
+    DM_REQUIRE((nv == 0 || xyz) && (nt == 0 || tri), DM_ERR_ARGUMENT, "mesh arrays are NULL");
+    auto m = std::make_unique<dm_mesh>();
+    m->ctx = ctx;
+    m->nv = nv;
+    m->nt = nt;
+    m->xyz.assign(xyz, xyz + 3 * nv);
+    m->tri.assign(tri, tri + 3 * nt);
+    for (i64 i = 0; i < 3 * nt; ++i)
+      DM_REQUIRE(m->tri[i] >= 1 && m->tri[i] <= nv, DM_ERR_ARGUMENT, "triangle vertex index out of range");
+    *out = m.release();
+  });
+}
+
+int dm_mesh_free(dm_mesh* mesh) {
+  return guard([&] { delete mesh; });
+}
+
+int dm_mesh_check(const dm_mesh* mesh, int allow_boundary, int require_closed, double area_tol_rel) {
+  return guard([&] {
+    check_ctx(mesh, "mesh");
+    dmh::mesh_check(*mesh, allow_boundary, require_closed, area_tol_rel);
+  });
+}
+
+int dm_mesh_areas(const dm_mesh* mesh, double* areas) {
+  return guard([&] {
+    check_ctx(mesh, "mesh");
+    for (i64 t = 0; t < mesh->nt; ++t) areas[t] = dmh::triangle_area(*mesh, t);
+  });
+}
+
+int dm_build_rwg(const dm_mesh* mesh, int precheck, int allow_boundary, int require_closed, double area_tol_rel,
+                 dm_rwg** out) {
+  return guard([&] {
+    check_ctx(mesh, "mesh");
+    check_ctx(out, "output handle");
+    CtxScope sc(mesh->ctx);
+    if (precheck) dmh::mesh_check(*mesh, allow_boundary, require_closed, area_tol_rel);
+    auto r = std::make_unique<dm_rwg>();
+    r->mesh = mesh;
+    r->ctx = mesh->ctx;
+    dmh::build_rwg_canonical(*mesh, *r);
+    DM_REQUIRE(r->n >= 1, DM_ERR_ARGUMENT,
+               "mesh produces no RWG unknowns; at least two triangles sharing an edge are required");
+    dmh::finalize_rwg(*r);
+    *out = r.release();
+  });
+}
+
+int dm_rwg_create(const dm_mesh* mesh, int64_t n, const int64_t* tplus, const int64_t* tminus, const int64_t* evert,
+                  const int64_t* vplus_opp, const int64_t* vminus_opp, const double* len, const double* area_plus,
+                  const double* area_minus, const double* coeff_plus, const double* coeff_minus, dm_rwg** out) {
+  return guard([&] {
+    check_ctx(mesh, "mesh");
+    check_ctx(out, "output handle");
+    DM_REQUIRE(n >= 1, DM_ERR_ARGUMENT, "RWG data has no basis functions");
+    DM_REQUIRE(tplus && tminus && vplus_opp && vminus_opp && len && area_plus && area_minus, DM_ERR_ARGUMENT,
+               "RWG arrays are NULL");
+    CtxScope sc(mesh->ctx);
+    auto r = std::make_unique<dm_rwg>();
+    r->mesh = mesh;
+    r->ctx = mesh->ctx;
+    r->n = n;
+    r->tplus.assign(tplus, tplus + n);
+    r->tminus.assign(tminus, tminus + n);
+    if (evert) r->evert.assign(evert, evert + 2 * n);
+    r->vplus_opp.assign(vplus_opp, vplus_opp + n);
+    r->vminus_opp.assign(vminus_opp, vminus_opp + n);
+    r->len.assign(len, len + n);
+    r->area_p.assign(area_plus, area_plus + n);
+    r->area_m.assign(area_minus, area_minus + n);
+    if (coeff_plus) r->coeff_p.assign(coeff_plus, coeff_plus + n); else r->coeff_p.assign(n, 1.0);
+    if (coeff_minus) r->coeff_m.assign(coeff_minus, coeff_minus + n); else r->coeff_m.assign(n, 1.0);
+    dmh::finalize_rwg(*r);
+    *out = r.release();
+  });
+}
+
+int dm_rwg_size(const dm_rwg* rwg, int64_t* n) {
+  return guard([&] {
+    check_ctx(rwg, "rwg");
+    *n = rwg->n;
+  });
+}
+
+int dm_rwg_export(const dm_rwg* r, int64_t* tplus, int64_t* tminus, int64_t* evert, int64_t* vplus_opp,
+                  int64_t* vminus_opp, double* len, double* area_plus, double* area_minus) {
+  return guard([&] {
+    check_ctx(r, "rwg");
+    i64 n = r->n;
+    if (tplus) std::copy(r->tplus.begin(), r->tplus.end(), tplus);
+    if (tminus) std::copy(r->tminus.begin(), r->tminus.end(), tminus);
+    if (evert) {
+      if ((i64)r->evert.size() == 2 * n) std::copy(r->evert.begin(), r->evert.end(), evert);
+      else std::fill(evert, evert + 2 * n, 0);
+    }
+    if (vplus_opp) std::copy(r->vplus_opp.begin(), r->vplus_opp.end(), vplus_opp);
+    if (vminus_opp) std::copy(r->vminus_opp.begin(), r->vminus_opp.end(), vminus_opp);
+    if (len) std::copy(r->len.begin(), r->len.end(), len);
+    if (area_plus) std::copy(r->area_p.begin(), r->area_p.end(), area_plus);
+    if (area_minus) std::copy(r->area_m.begin(), r->area_m.end(), area_minus);
+  });
+}
+
+int dm_rwg_free(dm_rwg* rwg) {
+  return guard([&] {
+    if (!rwg) return;
+    std::lock_guard<std::mutex> lk(rwg->ctx->mu);
+    (void)hipSetDevice(rwg->ctx->device);
+    delete rwg;
+  });
+}
+
+int dm_rwg_adjacent_pairs(const dm_rwg* rwg, int64_t* count) {
+  return guard([&] {
+    check_ctx(rwg, "rwg");
+    *count = rwg->adj_pairs;
+  });
+}
+
+int dm_assemble_z(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double k_im, double eta0, int quad_order, int forms,
+                  int64_t max_output_bytes, dm_zmat** out) {
+  return guard([&] {
+    check_ctx(ctx, "context");
+    check_ctx(rwg_c, "rwg");
+    check_ctx(out, "output handle");
+    DM_REQUIRE(rwg_c->ctx == ctx, DM_ERR_ARGUMENT, "RWG data belongs to another context");
+    DM_REQUIRE(forms > 0 && (forms & ~(DM_Z64 | DM_Z32)) == 0, DM_ERR_ARGUMENT, "forms must be a DM_Z64/DM_Z32 mask");
+    CtxScope sc(ctx);
+    dm_rwg& r = const_cast<dm_rwg&>(*rwg_c);
+    i64 n = r.n;
+    // byte gate on the dense complex128 payload (src/assembly/EFIE.jl:711-716)
+    if (max_output_bytes > 0) {
+      long double bytes = (long double)n * (long double)n * 16.0L;
+      DM_REQUIRE(bytes <= (long double)max_output_bytes, DM_ERR_ARGUMENT,
+                 "dense EFIE matrix requires " + std::to_string((unsigned long long)bytes) +
+                     " bytes, exceeding max_output_bytes=" + std::to_string(max_output_bytes));
+    }
+    validate_prefactors(k_re, k_im, eta0);
+    const TriTables& T = dmh::tables(r, quad_order);
+    auto z = std::make_unique<dm_zmat>();
+    z->ctx = ctx;
+    z->rwg = rwg_c;
+    z->n = n;
+    z->ld = round_ld(n);
+    z->forms = forms;
+    z->k_re = k_re;
+    z->k_im = k_im;
+    z->eta0 = eta0;
+    z->quad_order = quad_order;
+    hipStream_t s = ctx->stream;
+    NearTables nt;
+    AsmArgs a = make_args(r, T, k_re, k_im, eta0, z->ld, nt);
+    if (forms & DM_Z64) {
+      z->z64.alloc((size_t)z->ld * n * 2);
+      launch_assemble_z64(a, z->z64.p, r.max_blk_tris, s);
+    }
+    if (forms & DM_Z32) {
+      z->z32.alloc((size_t)z->ld * n * 2);
+      launch_assemble_z32(a, z->z32.p, r.max_blk_tris, s);
+    }
+    check_flag(nt, s);
+    *out = z.release();
+  });
+}
+
+int dm_zmat_size(const dm_zmat* z, int64_t* n, int* forms) {
+  return guard([&] {
+    check_ctx(z, "zmat");
+    if (n) *n = z->n;
+    if (forms) *forms = z->forms;
+  });
+}
+
+static void need_form(const dm_zmat* z, int form) {
+  DM_REQUIRE(form == DM_Z64 || form == DM_Z32, DM_ERR_ARGUMENT, "form must be DM_Z64 or DM_Z32");
+  DM_REQUIRE(z->forms & form, DM_ERR_ARGUMENT, "requested form was not assembled");
+}
+
+int dm_zmat_get(const dm_zmat* z, int form, double* out) {
+  return guard([&] {
+    check_ctx(z, "zmat");
+    check_ctx(out, "output buffer");
+    need_form(z, form);
+    CtxScope sc(z->ctx);
+    hipStream_t s = z->ctx->stream;
+    i64 n = z->n;
+    // column chunks of at most ~1 GiB of staging
+    i64 cols = std::max<i64>(1, std::min<i64>(n, (1LL << 26) / std::max<i64>(n, 1)));
+    DBuf<double> stage;
+    stage.alloc((size_t)2 * n * cols);
+    for (i64 c0 = 0; c0 < n; c0 += cols) {
+      i64 nc = std::min(cols, n - c0);
+      if (form == DM_Z64)
+        dml::unpermute64(n, z->ld, c0, nc, z->rwg->d_iofe.p, z->z64.p, stage.p, s);
+      else
+        dml::unpermute32(n, z->ld, c0, nc, z->rwg->d_iofe.p, z->z32.p, stage.p, s);
+      HIP_OK(hipMemcpyAsync(out + 2 * n * c0, stage.p, 16 * n * nc, hipMemcpyDeviceToHost, s));
+    }
+    HIP_OK(hipStreamSynchronize(s));
+  });
+}
+
+int dm_zmat_entries(const dm_zmat* z, int form, const int64_t* m, const int64_t* nidx, int64_t count, double* out) {
+  return guard([&] {
+    check_ctx(z, "zmat");
+    need_form(z, form);
+    if (count <= 0) return;
+    for (i64 i = 0; i < count; ++i)
+      DM_REQUIRE(m[i] >= 0 && m[i] < z->n && nidx[i] >= 0 && nidx[i] < z->n, DM_ERR_ARGUMENT, "entry index out of range");
+    CtxScope sc(z->ctx);
+    hipStream_t s = z->ctx->stream;
+    DBuf<i64> dmi, dni;
+    DBuf<double> o;
+    dmi.upload(m, count, s);
+    dni.upload(nidx, count, s);
+    o.alloc(2 * count);
+    if (form == DM_Z64)
+      dml::entries64(z->ld, count, z->rwg->d_iofe.p, dmi.p, dni.p, z->z64.p, o.p, s);
+    else
+      dml::entries32(z->ld, count, z->rwg->d_iofe.p, dmi.p, dni.p, z->z32.p, o.p, s);
+    HIP_OK(hipMemcpyAsync(out, o.p, 16 * count, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+  });
+}
+
+int dm_zmat_free(dm_zmat* z) {
+  return guard([&] {
+    if (!z) return;
+    std::lock_guard<std::mutex> lk(z->ctx->mu);
+    (void)hipSetDevice(z->ctx->device);
+    delete z;
+  });
+}
+
+int dm_apply(dm_ctx* ctx, const dm_zmat* z, int form, int adjoint, const double* X, int64_t ldx, int64_t nrhs,
+             double* Y, int64_t ldy) {
+  return guard([&] {
+    check_ctx(ctx, "context");
+    check_ctx(z, "zmat");
+    need_form(z, form);
+    i64 n = z->n;
+    DM_REQUIRE(nrhs >= 0 && ldx >= n && ldy >= n, DM_ERR_DIMENSION, "leading dimension smaller than N");
+    CtxScope sc(ctx);
+    hipStream_t s = ctx->stream;
+    DBuf<double> xe, xi, yi;
+    xe.alloc(2 * n);
+    xi.alloc(2 * n);
+    yi.alloc(2 * n);
+    for (i64 j = 0; j < nrhs; ++j) {
+      HIP_OK(hipMemcpyAsync(xe.p, X + 2 * ldx * j, 16 * n, hipMemcpyHostToDevice, s));
+      if (adjoint) dml::conj(n, xe.p, s);
+      launch_permute(n, z->rwg->d_ext.p, xe.p, xi.p, 1, s);
+      if (form == DM_Z64) launch_gemv_z64(n, z->ld, z->z64.p, xi.p, yi.p, s);
+      else launch_gemv_z32(n, z->ld, z->z32.p, xi.p, yi.p, s);
+      launch_permute(n, z->rwg->d_ext.p, yi.p, xe.p, 0, s);
+      if (adjoint) dml::conj(n, xe.p, s);
+      HIP_OK(hipMemcpyAsync(Y + 2 * ldy * j, xe.p, 16 * n, hipMemcpyDeviceToHost, s));
+    }
+    HIP_OK(hipStreamSynchronize(s));
+  });
+}
+
+int dm_mf_create(dm_ctx* ctx, const dm_rwg* rwg, double k_re, double k_im, double eta0, int quad_order, dm_mfop** out) {
+  return guard([&] {
+    check_ctx(ctx, "context");
+    check_ctx(rwg, "rwg");
+    check_ctx(out, "output handle");
+    validate_prefactors(k_re, k_im, eta0);
+    CtxScope sc(ctx);
+    dmh::tables(const_cast<dm_rwg&>(*rwg), quad_order);
+    auto m = std::make_unique<dm_mfop>();
+    m->ctx = ctx;
+    m->rwg = rwg;
+    m->k_re = k_re;
+    m->k_im = k_im;
+    m->eta0 = eta0;
+    m->quad_order = quad_order;
+    *out = m.release();
+  });
+}
+
+int dm_mf_apply(dm_ctx* ctx, const dm_mfop* op, int adjoint, const double* x, double* y) {
+  return guard([&] {
+    check_ctx(ctx, "context");
+    check_ctx(op, "operator");
+    CtxScope sc(ctx);
+    dm_rwg& r = const_cast<dm_rwg&>(*op->rwg);
+    const TriTables& T = dmh::tables(r, op->quad_order);
+    hipStream_t s = ctx->stream;
+    i64 n = r.n;
+    NearTables nt;
+    AsmArgs a = make_args(r, T, op->k_re, op->k_im, op->eta0, 0, nt);
+    DBuf<double> xe, xi, yi;
+    xe.alloc(2 * n);
+    xi.alloc(2 * n);
+    yi.alloc(2 * n);
+    HIP_OK(hipMemcpyAsync(xe.p, x, 16 * n, hipMemcpyHostToDevice, s));
+    if (adjoint) dml::conj(n, xe.p, s);
+    launch_permute(n, r.d_ext.p, xe.p, xi.p, 1, s);
+    HIP_OK(hipMemsetAsync(yi.p, 0, 16 * n, s));
+    launch_mf_apply(a, xi.p, yi.p, r.max_blk_tris, s);
+    launch_permute(n, r.d_ext.p, yi.p, xe.p, 0, s);
+    if (adjoint) dml::conj(n, xe.p, s);
+    HIP_OK(hipMemcpyAsync(y, xe.p, 16 * n, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+  });
+}
+
+int dm_mf_free(dm_mfop* op) {
+  return guard([&] { delete op; });
+}
+
+// Plane-wave parameter checks: src/assembly/Excitation.jl:1549-1598
+static void validate_planewave(const double* k, const double* p, double E0) {
+  for (int c = 0; c < 3; ++c)
+    DM_REQUIRE(std::isfinite(k[c]) && std::isfinite(p[c]), DM_ERR_ARGUMENT, "plane-wave k_vec and polarization must be finite.");
+  double kn = std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+  DM_REQUIRE(kn > 0 && std::isfinite(kn), DM_ERR_ARGUMENT, "plane-wave k_vec must have a finite, nonzero norm.");
+  DM_REQUIRE(std::isfinite(E0), DM_ERR_ARGUMENT, "plane-wave amplitude E0 must be finite.");
+  double pn = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+  DM_REQUIRE(pn > 0, DM_ERR_ARGUMENT, "plane-wave polarization must have a finite, nonzero norm.");
+  DM_REQUIRE(std::fabs(pn - 1.0) <= 1e-12 + 1e-10 * std::max(pn, 1.0), DM_ERR_ARGUMENT,
+             "plane-wave polarization must be a unit vector; norm=" + std::to_string(pn) + ".");
+  double tr = std::fabs((k[0] * p[0] + k[1] * p[1] + k[2] * p[2]) / (kn * pn));
+  DM_REQUIRE(tr <= 1e-10, DM_ERR_ARGUMENT, "plane-wave polarization must be transverse to k_vec.");
+}
+
+int dm_excitation_planewave(dm_ctx* ctx, const dm_rwg* rwg, const double* kvec, const double* pol, const double* E0,
+                            int64_t M, int quad_order, double* V) {
+  return guard([&] {
+    check_ctx(ctx, "context");
+    check_ctx(rwg, "rwg");
+    DM_REQUIRE(M >= 0, DM_ERR_ARGUMENT, "negative excitation count");
+    for (i64 m = 0; m < M; ++m) validate_planewave(kvec + 3 * m, pol + 3 * m, E0[m]);
+    if (M == 0) return;
+    CtxScope sc(ctx);
+    dm_rwg& r = const_cast<dm_rwg&>(*rwg);
+    const TriTables& T = dmh::tables(r, quad_order);
+    hipStream_t s = ctx->stream;
+    DBuf<double> dk, dp, de, dv;
+    dk.upload(kvec, 3 * M, s);
+    dp.upload(pol, 3 * M, s);
+    de.upload(E0, M, s);
+    dv.alloc((size_t)2 * r.n * M);
+    dmf::planewave(r, T, dk.p, dp.p, de.p, (int)M, r.d_ext.p, dv.p, r.n, s);
+    HIP_OK(hipMemcpyAsync(V, dv.p, 16 * r.n * M, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+  });
+}
+
+int dm_radiation_vectors(dm_ctx* ctx, const dm_rwg* rwg, const double* rhat, int64_t nomega, double k, double eta0,
+                         int quad_order, double* G) {
+  return guard([&] {
+    check_ctx(ctx, "context");
+    check_ctx(rwg, "rwg");
+    DM_REQUIRE(nomega > 0, DM_ERR_ARGUMENT, "NΩ must be positive");
+    DM_REQUIRE(std::isfinite(k) && std::isfinite(eta0), DM_ERR_ARGUMENT, "radiation_vectors wavenumber and eta0 must be finite");
+    for (i64 q = 0; q < nomega; ++q) {
+      double nn = rhat[3 * q] * rhat[3 * q] + rhat[3 * q + 1] * rhat[3 * q + 1] + rhat[3 * q + 2] * rhat[3 * q + 2];
+      DM_REQUIRE(std::isfinite(nn) && nn > 0, DM_ERR_ARGUMENT, "far-field direction must be finite and nonzero");
+    }
+    CtxScope sc(ctx);
+    dm_rwg& r = const_cast<dm_rwg&>(*rwg);
+    const TriTables& T = dmh::tables(r, quad_order);
+    hipStream_t s = ctx->stream;
+    DBuf<double> dr, dg;
+    dr.upload(rhat, 3 * nomega, s);
+    dg.alloc((size_t)2 * 3 * nomega * r.n);
+    dmf::radiation_vectors(r, T, dr.p, nomega, k, eta0, dg.p, s);
+    HIP_OK(hipMemcpyAsync(G, dg.p, dg.bytes(), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+  });
+}
+
+void dm_solve_opts_default(dm_solve_opts* o) {
+  if (!o) return;
+  o->method = DM_METHOD_GMRES;
+  o->precond = DM_PRECOND_NEARFIELD;
+  o->precond_side = 1;
+  o->maxiter = 300;
+  o->tol = 1e-6;
+  o->nf_cutoff = 0;
+  o->restart = 0;
+  o->check_convergence = 1;
+  o->inner_tol = 0;
+  o->max_outer = 10;
+  o->block_size = 128;
+}
+
+int dm_solve(dm_ctx* ctx, const dm_zmat* z, const double* B, int64_t ldb, int64_t nrhs, const dm_solve_opts* opts,
+             double* X, int64_t ldx, dm_solve_stats* stats) {
+  bool unconverged = false;
+  int rc = guard([&] {
+    check_ctx(ctx, "context");
+    check_ctx(z, "zmat");
+    dm_solve_opts o;
+    dm_solve_opts_default(&o);
+    if (opts) o = *opts;
+    i64 n = z->n;
+    DM_REQUIRE(nrhs >= 0 && ldb >= n && ldx >= n, DM_ERR_DIMENSION, "leading dimension smaller than N");
+    DM_REQUIRE(std::isfinite(o.tol) && o.tol > 0, DM_ERR_ARGUMENT, "GMRES tol must be finite and positive");
+    DM_REQUIRE(o.maxiter >= 1 && o.maxiter <= 1000, DM_ERR_ARGUMENT, "maxiter must be in [1, 1000]");
+    DM_REQUIRE(o.precond_side == 0 || o.precond_side == 1, DM_ERR_ARGUMENT, "precond_side must be 0 (left) or 1 (right)");
+    DM_REQUIRE(o.method >= DM_METHOD_LU && o.method <= DM_METHOD_MIXED, DM_ERR_ARGUMENT, "unknown solve method");
+    for (i64 j = 0; j < nrhs; ++j)
+      for (i64 i = 0; i < 2 * n; ++i)
+        DM_REQUIRE(std::isfinite(B[2 * ldb * j + i]), DM_ERR_ARGUMENT, "right-hand side must be finite");
+    CtxScope sc(ctx);
+    hipStream_t s = ctx->stream;
+    const dm_rwg& r = *z->rwg;
+    dm_solve_stats st{};
+    DBuf<double> be, b, x, tmp, hb;
+    be.alloc(2 * n);
+    b.alloc(2 * n);
+    x.alloc(2 * n);
+    tmp.alloc(2 * n);
+    hb.alloc(4);
+    double t0 = now();
+    if (o.method == DM_METHOD_LU) {
+      DM_REQUIRE(z->forms & DM_Z64, DM_ERR_ARGUMENT, "dense LU needs the complex128 matrix");
+      DBuf<double> A, bb;
+      DBuf<int> ipiv, status;
+      A.alloc((size_t)2 * z->ld * n);
+      HIP_OK(hipMemcpyAsync(A.p, z->z64.p, A.bytes(), hipMemcpyDeviceToDevice, s));
+      ipiv.alloc(n);
+      status.alloc(1);
+      HIP_OK(hipMemsetAsync(status.p, 0, sizeof(int), s));
+      dml::lu_factor(n, z->ld, A.p, ipiv.p, status.p, s);
+      int sv = 0;
+      HIP_OK(hipMemcpyAsync(&sv, status.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIP_OK(hipStreamSynchronize(s));
+      DM_REQUIRE(sv == 0, DM_ERR_ARGUMENT, "dense LU: matrix is singular (zero pivot " + std::to_string(sv) + ")");
+      st.precond_time_s = now() - t0;
+      OpZ64 A64(z);
+      double worst = 0;
+      for (i64 j = 0; j < nrhs; ++j) {
+        HIP_OK(hipMemcpyAsync(be.p, B + 2 * ldb * j, 16 * n, hipMemcpyHostToDevice, s));
+        launch_permute(n, r.d_ext.p, be.p, b.p, 1, s);
+        HIP_OK(hipMemcpyAsync(x.p, b.p, 16 * n, hipMemcpyDeviceToDevice, s));
+        dml::lu_solve(n, z->ld, A.p, ipiv.p, x.p, n, 1, s);
+        worst = std::max(worst, true_residual(A64, n, b.p, x.p, tmp.p, hb.p, s));
+        launch_permute(n, r.d_ext.p, x.p, be.p, 0, s);
+        HIP_OK(hipMemcpyAsync(X + 2 * ldx * j, be.p, 16 * n, hipMemcpyDeviceToHost, s));
+      }
+      HIP_OK(hipStreamSynchronize(s));
+      st.solved = 1;
+      st.residual = worst;
+      st.solve_time_s = now() - t0 - st.precond_time_s;
+      if (stats) *stats = st;
+      return;
+    }
+    bool have64 = z->forms & DM_Z64, have32 = z->forms & DM_Z32;
+    if (o.method == DM_METHOD_MIXED)
+      DM_REQUIRE(have64 && have32, DM_ERR_ARGUMENT, "mixed-precision solve needs both DM_Z64 and DM_Z32 forms");
+    bool prec32 = o.method == DM_METHOD_MIXED || !have64;
+    auto M = build_precond(z, prec32, o, s);
+    HIP_OK(hipStreamSynchronize(s));
+    st.precond_time_s = now() - t0;
+    double t1 = now();
+    std::unique_ptr<DevOp> A64, A32;
+    if (have64) A64 = std::make_unique<OpZ64>(z);
+    if (have32) A32 = std::make_unique<OpZ32>(z);
+    DevOp& Aref = have64 ? *A64 : *A32;
+    Krylov K;
+    bool all_ok = true;
+    double worst = 0;
+    for (i64 j = 0; j < nrhs; ++j) {
+      HIP_OK(hipMemcpyAsync(be.p, B + 2 * ldb * j, 16 * n, hipMemcpyHostToDevice, s));
+      launch_permute(n, r.d_ext.p, be.p, b.p, 1, s);
+      HIP_OK(hipMemsetAsync(x.p, 0, 16 * n, s));
+      bool ok = false;
+      double res = 0;
+      if (o.method == DM_METHOD_GMRES) {
+        GmresOut g = gmres(n, Aref, M.get(), o.precond_side, b.p, x.p, o.tol, o.maxiter, o.restart, K, s);
+        st.iterations += g.iters;
+        res = true_residual(Aref, n, b.p, x.p, tmp.p, hb.p, s);
+        ok = g.solved;
+      } else {
+        // Mixed-precision iterative refinement: inner FGMRES on the complex64
+        // matrix with FP64 Arnoldi, outer residual r = b - Z64 x in FP64.
+        DBuf<double> rr, d;
+        rr.alloc(2 * n);
+        d.alloc(2 * n);
+        HIP_OK(hipMemcpyAsync(rr.p, b.p, 16 * n, hipMemcpyDeviceToDevice, s));
+        double rel = 1.0;
+        double tau = o.inner_tol > 0 ? o.inner_tol : 0.5 * o.tol;
+        int left = o.maxiter;
+        for (int outer = 0; outer < std::max(1, o.max_outer) && left > 0; ++outer) {
+          HIP_OK(hipMemsetAsync(d.p, 0, 16 * n, s));
+          GmresOut g = gmres(n, *A32, M.get(), 1, rr.p, d.p, tau, left, o.restart, K, s);
+          st.iterations += g.iters;
+          left -= g.iters;
+          st.outer_iterations += 1;
+          dml::axpby(n, 1, 0, d.p, 1, 0, x.p, s);
+          // rr = b - Z64 x
+          A64->apply(x.p, tmp.p, s);
+          HIP_OK(hipMemcpyAsync(rr.p, b.p, 16 * n, hipMemcpyDeviceToDevice, s));
+          dml::axpby(n, -1, 0, tmp.p, 1, 0, rr.p, s);
+          double h[2];
+          dml::norm2(n, rr.p, hb.p, s);
+          dml::norm2(n, b.p, hb.p + 1, s);
+          HIP_OK(hipMemcpyAsync(h, hb.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+          HIP_OK(hipStreamSynchronize(s));
+          rel = h[1] > 0 ? std::sqrt(h[0] / h[1]) : 0.0;
+          if (rel <= o.tol) break;
+          tau = std::min(0.5, std::max(1e-12, 0.5 * o.tol / rel));
+        }
+        res = rel;
+        ok = rel <= o.tol;
+      }
+      worst = std::max(worst, res);
+      all_ok = all_ok && ok;
+      launch_permute(n, r.d_ext.p, x.p, be.p, 0, s);
+      HIP_OK(hipMemcpyAsync(X + 2 * ldx * j, be.p, 16 * n, hipMemcpyDeviceToHost, s));
+    }
+    HIP_OK(hipStreamSynchronize(s));
+    st.solved = all_ok ? 1 : 0;
+    st.residual = worst;
+    st.solve_time_s = now() - t1;
+    if (stats) *stats = st;
+    if (!all_ok && o.check_convergence) unconverged = true;
+  });
+  if (rc == DM_OK && unconverged) {
+    g_err = "GMRES did not converge within maxiter";
+    return DM_ERR_CONVERGENCE;
+  }
+  return rc;
+}
+
+}  // extern "C"

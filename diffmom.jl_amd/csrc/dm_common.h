@@ -1,0 +1,188 @@
+// dm_common.h — shared internals of libdiffmom (host + device).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/diffmom.h"
+
+typedef int64_t i64;
+
+// ------------------------------------------------------------------ errors
+struct DmError : std::runtime_error {
+  int code;
+  DmError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] inline void dm_throw(int code, const std::string& msg) { throw DmError(code, msg); }
+
+#define HIP_OK(expr)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      dm_throw(e_ == hipErrorOutOfMemory ? DM_ERR_MEMORY : DM_ERR_DEVICE,                \
+               std::string(#expr) + ": " + hipGetErrorString(e_));                       \
+  } while (0)
+
+#define DM_REQUIRE(cond, code, msg) \
+  do {                              \
+    if (!(cond)) dm_throw(code, msg); \
+  } while (0)
+
+// --------------------------------------------------------------- device buffers
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count == 0) return;
+    HIP_OK(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+  }
+  void upload(const T* host, size_t count, hipStream_t s) {
+    alloc(count);
+    if (count) HIP_OK(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// ------------------------------------------------------------------ handles
+struct dm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;   // main stream: every public call runs here
+  hipStream_t stream2 = nullptr;  // second stream for overlapped work
+  std::mutex mu;
+  int cu_count = 256;
+};
+
+struct dm_mesh {
+  dm_ctx* ctx = nullptr;
+  i64 nv = 0, nt = 0;
+  std::vector<double> xyz;  // 3 x nv
+  std::vector<i64> tri;     // 3 x nt, 1-based
+};
+
+// Per-quadrature-order triangle tables (device), built lazily per RWG handle.
+struct TriTables {
+  int quad_order = 0, nq = 0;
+  DBuf<double> cen;   // nt*3 centroid
+  DBuf<double> uq;    // nt*nq*3 quad points relative to the centroid
+  DBuf<double> av;    // nt*9 vertex offsets from the centroid
+  DBuf<double> rq;    // nt*nq*3 absolute quad points
+  DBuf<double> rq7;   // nt*7*3 absolute 7-point rule points
+  DBuf<double> vtx;   // nt*9 absolute vertices
+  DBuf<double> area;  // nt
+  std::vector<double> h_rq, h_area;
+};
+
+struct dm_rwg {
+  const dm_mesh* mesh = nullptr;
+  dm_ctx* ctx = nullptr;
+  i64 n = 0;
+  // caller-order arrays (1-based triangle / vertex ids)
+  std::vector<i64> tplus, tminus, evert, vplus_opp, vminus_opp;
+  std::vector<double> len, area_p, area_m, coeff_p, coeff_m;
+  // derived
+  std::vector<int> eplus, eminus;     // local vertex index (0..2) opposite the edge
+  std::vector<int> ext_of_int;        // internal index -> caller index (0-based)
+  std::vector<int> int_of_ext;        // caller index -> internal index
+  std::vector<int> nb;                // nt*3 edge neighbours (0-based, -1 none)
+  i64 adj_pairs = 0;
+  // tiles (internal order): blocks of TB consecutive internal RWGs
+  int TB = 64;
+  int nblk = 0;
+  int max_blk_tris = 0;
+  std::vector<int> blk_off, blk_tri;  // CSR: triangles touched by each block
+  // device copies
+  DBuf<int> d_ext;          // n
+  DBuf<int> d_iofe;         // n: caller index -> internal index
+  DBuf<double> d_len;       // n (internal order)
+  DBuf<int> d_hrwg;         // nt*3 internal RWG of half (t, e), -1 if none
+  DBuf<double> d_hsig;      // nt*3 sign/coefficient of half (t, e)
+  DBuf<int> d_nb;           // nt*3
+  DBuf<int> d_blk_off, d_blk_tri;
+  DBuf<int> d_tplus_i, d_tminus_i, d_eplus_i, d_eminus_i;  // internal order, 0-based
+  TriTables tabs[8];        // indexed by quad order (1,3,4,7)
+  std::mutex tab_mu;
+};
+
+struct dm_zmat {
+  dm_ctx* ctx = nullptr;
+  const dm_rwg* rwg = nullptr;
+  i64 n = 0, ld = 0;
+  int forms = 0;
+  double k_re = 0, k_im = 0, eta0 = 0;
+  int quad_order = 3;
+  DBuf<double> z64;  // ld x n complex128 (interleaved), internal order
+  DBuf<float> z32;   // ld x n complex64, internal order
+};
+
+struct dm_mfop {
+  dm_ctx* ctx = nullptr;
+  const dm_rwg* rwg = nullptr;
+  double k_re = 0, k_im = 0, eta0 = 0;
+  int quad_order = 3;
+};
+
+// ------------------------------------------------------------ host geometry
+namespace dmh {
+void build_rwg_canonical(const dm_mesh& m, dm_rwg& r);
+void finalize_rwg(dm_rwg& r);  // validates, orders, tiles, uploads
+void mesh_check(const dm_mesh& m, int allow_boundary, int require_closed, double area_tol_rel);
+double triangle_area(const dm_mesh& m, i64 t0);
+const TriTables& tables(dm_rwg& r, int quad_order);
+int quad_rule(int order, double* xi, double* w);
+}  // namespace dmh
+
+// ------------------------------------------------------------ kernel launchers
+struct AsmArgs {
+  i64 n, ld, nt;
+  int nblk, TB;
+  const int* blk_off;
+  const int* blk_tri;
+  const int* hrwg;
+  const double* hsig;
+  const int* nb;
+  const int* ext;
+  const double* len;
+  const double* cen;
+  const double* uq;
+  const double* av;
+  const double* jself;  // nt*9 complex
+  const double* jadj;   // nt*3*9 complex
+  double w[7];
+  double k_re, k_im, eta0;
+  int nq;
+  int* flag;  // set to 1 when an assembled entry is non-finite
+};
+
+void launch_near_tables(const AsmArgs& a, const TriTables& t, double* jself, double* jadj,
+                        hipStream_t s);
+void launch_assemble_z64(const AsmArgs& a, double* z, int max_tris, hipStream_t s);
+void launch_assemble_z32(const AsmArgs& a, float* z, int max_tris, hipStream_t s);
+// matrix-free y += Z x in internal order (FP64), x and y complex128
+void launch_mf_apply(const AsmArgs& a, const double* x, double* y, int max_tris, hipStream_t s);
+
+// dense products (internal order, column-major, complex interleaved)
+void launch_gemv_z64(i64 n, i64 ld, const double* z, const double* x, double* y, hipStream_t s);
+void launch_gemv_z32(i64 n, i64 ld, const float* z, const double* x, double* y, hipStream_t s);
+
+// small vector kernels
+void launch_permute(i64 n, const int* idx, const double* src, double* dst, int gather, hipStream_t s);
+void launch_cast_c128_to_c64_matrix(i64 n, i64 ld, const double* src, float* dst, hipStream_t s);

@@ -1,0 +1,667 @@
+// k_assembly.hip — EFIE impedance-matrix assembly kernels for gfx950.
+//
+// Formulation.  For a basis m with support triangles T_m^+ / T_m^- write each
+// half as f = sigma * len/(2A) * (r - v_e), v_e the vertex opposite the edge
+// (src/basis/RWG.jl:819-856).  For a triangle pair (t, t') and opposite-vertex
+// indices (e, e') let
+//   J[e][e'] = sum_{q,q'} w_q w_q' G(r_q, r'_q') [ (r_q - v_e).(r'_q' - v'_e') - 4/k^2 ]
+// Then Z_mn = -i k eta0 * len_m len_n * sum over the four support pairs of
+// sigma_m sigma_n J[e_m][e_n]  — exactly _efie_entry of
+// src/assembly/EFIE.jl:439-512 with the area factors cancelled.  One J block
+// serves the 3 x 3 basis pairs of a triangle pair, so a triangle pair's nine
+// Green evaluations are shared by nine matrix entries instead of being redone
+// for each (the reference evaluates 36 Green values per entry).
+//
+// Self (t == t') and edge-adjacent pairs use the reference's singularity
+// extraction (src/assembly/EFIE.jl:534-684): they are precomputed in FP64 into
+// per-triangle tables by k_near_self / k_near_adj and looked up by the tile
+// kernel.  The adjacent-cell integral is not symmetric in (outer, inner); the
+// reference evaluates entry (min(m,n), max(m,n)) with the lower-index basis's
+// triangle as the outer one, and the tile kernel follows the caller's basis
+// numbering to pick the same table.
+//
+// Tiling.  Bases are in a spatial (Morton) order; block b = TB consecutive
+// bases, touching the triangle list blk_tri[blk_off[b]..blk_off[b+1]).  One
+// workgroup owns tile (bi <= bj): it evaluates every (row triangle, column
+// triangle) pair of the two lists, accumulates sigma*sigma*J into an LDS tile
+// with ds_add atomics, and writes the tile and its mirror (Z = Z^T) to HBM.
+#include <hip/hip_runtime.h>
+
+#include "dm_common.h"
+
+namespace {
+
+constexpr double INV4PI = 0.07957747154594767;  // 1/(4 pi)
+
+struct c2 {
+  double re, im;
+};
+__device__ inline c2 cmul(c2 a, c2 b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ inline c2 cadd(c2 a, c2 b) { return {a.re + b.re, a.im + b.im}; }
+__device__ inline c2 csub(c2 a, c2 b) { return {a.re - b.re, a.im - b.im}; }
+__device__ inline c2 cscale(c2 a, double s) { return {a.re * s, a.im * s}; }
+
+// ------------------------------------------------------------------ FP64 near
+// int_T 1/|P - r'| dS' (Wilton/Graglia closed form), unchecked form of
+// src/assembly/SingularIntegrals.jl:146-203.
+__device__ double int_1overR(const double* P, const double* A0, const double* B0, const double* C0) {
+  double e1[3] = {B0[0] - A0[0], B0[1] - A0[1], B0[2] - A0[2]};
+  double e2[3] = {C0[0] - A0[0], C0[1] - A0[1], C0[2] - A0[2]};
+  double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+  double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  if (nn == 0.0) return 0.0;
+  n[0] /= nn; n[1] /= nn; n[2] /= nn;
+  double h = (P[0] - A0[0]) * n[0] + (P[1] - A0[1]) * n[1] + (P[2] - A0[2]) * n[2];
+  double ah = fabs(h);
+  double xi[3] = {P[0] - h * n[0], P[1] - h * n[1], P[2] - h * n[2]};
+  const double* As[3] = {A0, B0, C0};
+  const double* Bs[3] = {B0, C0, A0};
+  double log_sum = 0.0, atan_sum = 0.0;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const double* A = As[e];
+    const double* B = Bs[e];
+    double ev[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+    double el = sqrt(ev[0] * ev[0] + ev[1] * ev[1] + ev[2] * ev[2]);
+    if (el == 0.0) continue;
+    double l[3] = {ev[0] / el, ev[1] / el, ev[2] / el};
+    double nh[3] = {l[1] * n[2] - l[2] * n[1], l[2] * n[0] - l[0] * n[2], l[0] * n[1] - l[1] * n[0]};
+    double ax[3] = {A[0] - xi[0], A[1] - xi[1], A[2] - xi[2]};
+    double bx[3] = {B[0] - xi[0], B[1] - xi[1], B[2] - xi[2]};
+    double d = ax[0] * nh[0] + ax[1] * nh[1] + ax[2] * nh[2];
+    double sm = ax[0] * l[0] + ax[1] * l[1] + ax[2] * l[2];
+    double sp = bx[0] * l[0] + bx[1] * l[1] + bx[2] * l[2];
+    double pa[3] = {P[0] - A[0], P[1] - A[1], P[2] - A[2]};
+    double pb[3] = {P[0] - B[0], P[1] - B[1], P[2] - B[2]};
+    double Rm = sqrt(pa[0] * pa[0] + pa[1] * pa[1] + pa[2] * pa[2]);
+    double Rp = sqrt(pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2]);
+    double R0sq = d * d + h * h;
+    if (d != 0.0) {
+      double den = sm + Rm, num = sp + Rp;
+      if (den > 0.0 && num > 0.0) log_sum += d * log(num / den);
+    }
+    if (ah != 0.0 && d != 0.0) {
+      double ap = atan(d * sp / (R0sq + ah * Rp));
+      double am = atan(d * sm / (R0sq + ah * Rm));
+      atan_sum += ap - am;
+    }
+  }
+  return log_sum - ah * atan_sum;
+}
+
+__device__ inline double dist3(const double* a, const double* b) {
+  double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return hypot(hypot(dx, dy), dz);
+}
+
+// G_smooth = (exp(-ikR) - 1)/(4 pi R): src/basis/Greens.jl:189-201,381-400.
+// q = (Im k, -Re k); R = 0 -> q/(4 pi); |qR| <= 0.5 -> (q/4pi) * exprel(qR) by the
+// Kahan-compensated 18-term series; otherwise expm1(qR)/(4 pi R).
+__device__ c2 greens_smooth(double R, c2 q) {
+  if (R == 0.0) return cscale(q, INV4PI);
+  c2 z = cscale(q, R);
+  double az = hypot(z.re, z.im);
+  if (az <= 0.5) {
+    c2 term = {1.0, 0.0}, total = {1.0, 0.0}, corr = {0.0, 0.0};
+    for (int order = 1; order <= 18; ++order) {
+      term = cscale(cmul(term, z), 1.0 / double(order + 1));
+      c2 ct = csub(term, corr);
+      c2 upd = cadd(total, ct);
+      corr = csub(csub(upd, total), ct);
+      total = upd;
+    }
+    return cmul(cscale(q, INV4PI), total);
+  }
+  // complex expm1 for finite z: (expm1(x) - 2 e^x sin^2(y/2)) + i e^x sin(y)
+  double erm1 = expm1(z.re);
+  c2 em1;
+  if (z.im == 0.0) {
+    em1 = {erm1, 0.0};
+  } else {
+    double er = erm1 + 1.0;
+    double s = sin(0.5 * z.im);
+    em1 = {erm1 - 2.0 * er * (s * s), er * sin(z.im)};
+  }
+  return cscale(em1, INV4PI / R);
+}
+
+struct NearArgs {
+  i64 nt;
+  int nq;
+  const double* rq;   // nt*nq*3
+  const double* rq7;  // nt*21
+  const double* vtx;  // nt*9
+  const double* area; // nt
+  const int* nb;      // nt*3
+  double w[7], w7[7];
+  c2 q;               // (Im k, -Re k)
+  c2 four_ik2;        // 4 / k^2
+};
+
+// Self cell (src/assembly/EFIE.jl:534-606), all nine (e, e') pairs of one
+// triangle, normalised by (2A)^2:
+//   J = sum_ab (f_e(a).f_e'(b) - 4/k^2) Gs_ab w_a w_b
+//     + sum_a w_a [ (f_e(a).f_e'(a) - 4/k^2) S_a/(4 pi 2A) + sum_b f_e(a).(r_b - r_a) w_b/(4 pi R_ab) ]
+// with f_e(r) = r - v_e.
+__global__ __launch_bounds__(64) void k_near_self(NearArgs a, double* jself) {
+  i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (t >= a.nt) return;
+  const double* P = a.rq7 + t * 21;
+  const double* V = a.vtx + t * 9;
+  double A = a.area[t];
+  c2 J[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) J[i] = {0.0, 0.0};
+  // smooth part, 7 x 7 with G_smooth
+  for (int ia = 0; ia < 7; ++ia) {
+    const double* ra = P + 3 * ia;
+    double fa[3][3];
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) fa[e][c] = ra[c] - V[3 * e + c];
+    for (int ib = 0; ib < 7; ++ib) {
+      const double* rb = P + 3 * ib;
+      c2 Gs = greens_smooth(dist3(ra, rb), a.q);
+      double wt = a.w7[ia] * a.w7[ib];
+#pragma unroll
+      for (int e = 0; e < 3; ++e)
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+          double dd = fa[e][0] * (rb[0] - V[3 * f]) + fa[e][1] * (rb[1] - V[3 * f + 1]) + fa[e][2] * (rb[2] - V[3 * f + 2]);
+          c2 coef = {dd - a.four_ik2.re, -a.four_ik2.im};
+          J[3 * e + f] = cadd(J[3 * e + f], cscale(cmul(coef, Gs), wt));
+        }
+    }
+  }
+  // singular part: analytic inner integral + bounded remainder
+  for (int ia = 0; ia < 7; ++ia) {
+    const double* ra = P + 3 * ia;
+    double S = int_1overR(ra, V, V + 3, V + 6);
+    double sc = a.w7[ia] * S * INV4PI / (2.0 * A);
+    double rem[3] = {0, 0, 0};
+    for (int ib = 0; ib < 7; ++ib) {
+      const double* rb = P + 3 * ib;
+      double R = dist3(ra, rb);
+      if (R == 0.0) continue;
+      double k = a.w7[ia] * a.w7[ib] * INV4PI / R;
+#pragma unroll
+      for (int e = 0; e < 3; ++e)
+        rem[e] += ((ra[0] - V[3 * e]) * (rb[0] - ra[0]) + (ra[1] - V[3 * e + 1]) * (rb[1] - ra[1]) +
+                   (ra[2] - V[3 * e + 2]) * (rb[2] - ra[2])) * k;
+    }
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+#pragma unroll
+      for (int f = 0; f < 3; ++f) {
+        double dd = (ra[0] - V[3 * e]) * (ra[0] - V[3 * f]) + (ra[1] - V[3 * e + 1]) * (ra[1] - V[3 * f + 1]) +
+                    (ra[2] - V[3 * e + 2]) * (ra[2] - V[3 * f + 2]);
+        c2 coef = {dd - a.four_ik2.re, -a.four_ik2.im};
+        J[3 * e + f] = cadd(J[3 * e + f], cscale(coef, sc));
+        J[3 * e + f].re += rem[e];
+      }
+  }
+  c2* out = reinterpret_cast<c2*>(jself) + t * 9;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) out[i] = J[i];
+}
+
+// Adjacent cell (src/assembly/EFIE.jl:611-684), outer triangle t, inner
+// triangle u = nb[t][j], normalised by (2A_t)(2A_u):
+//   J = sum_ab^{nq} (f_e(a).f'_e'(b) - 4/k^2) Gs_ab w_a w_b
+//     + sum_a^7 w_a [ sum_b^7 f_e(a).f'_e'(b) w_b/(4 pi R_ab) - 4/k^2 S_u(a)/(4 pi 2A_u) ]
+__global__ __launch_bounds__(64) void k_near_adj(NearArgs a, double* jadj) {
+  i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (idx >= a.nt * 3) return;
+  i64 t = idx / 3;
+  int u = a.nb[idx];
+  c2* out = reinterpret_cast<c2*>(jadj) + idx * 9;
+  if (u < 0) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) out[i] = {0.0, 0.0};
+    return;
+  }
+  const double* Vt = a.vtx + t * 9;
+  const double* Vu = a.vtx + (i64)u * 9;
+  double Au = a.area[u];
+  c2 J[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) J[i] = {0.0, 0.0};
+  const int nq = a.nq;
+  for (int ia = 0; ia < nq; ++ia) {
+    const double* ra = a.rq + (t * nq + ia) * 3;
+    for (int ib = 0; ib < nq; ++ib) {
+      const double* rb = a.rq + ((i64)u * nq + ib) * 3;
+      c2 Gs = greens_smooth(dist3(ra, rb), a.q);
+      double wt = a.w[ia] * a.w[ib];
+#pragma unroll
+      for (int e = 0; e < 3; ++e)
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+          double dd = (ra[0] - Vt[3 * e]) * (rb[0] - Vu[3 * f]) + (ra[1] - Vt[3 * e + 1]) * (rb[1] - Vu[3 * f + 1]) +
+                      (ra[2] - Vt[3 * e + 2]) * (rb[2] - Vu[3 * f + 2]);
+          c2 coef = {dd - a.four_ik2.re, -a.four_ik2.im};
+          J[3 * e + f] = cadd(J[3 * e + f], cscale(cmul(coef, Gs), wt));
+        }
+    }
+  }
+  for (int ia = 0; ia < 7; ++ia) {
+    const double* ra = a.rq7 + t * 21 + 3 * ia;
+    double S = int_1overR(ra, Vu, Vu + 3, Vu + 6);
+    double sc = a.w7[ia] * S * INV4PI / (2.0 * Au);
+    double vs[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) vs[i] = 0.0;
+    for (int ib = 0; ib < 7; ++ib) {
+      const double* rb = a.rq7 + (i64)u * 21 + 3 * ib;
+      double R = dist3(ra, rb);
+      if (R == 0.0) continue;
+      double k = a.w7[ia] * a.w7[ib] * INV4PI / R;
+#pragma unroll
+      for (int e = 0; e < 3; ++e)
+#pragma unroll
+        for (int f = 0; f < 3; ++f)
+          vs[3 * e + f] += ((ra[0] - Vt[3 * e]) * (rb[0] - Vu[3 * f]) + (ra[1] - Vt[3 * e + 1]) * (rb[1] - Vu[3 * f + 1]) +
+                            (ra[2] - Vt[3 * e + 2]) * (rb[2] - Vu[3 * f + 2])) * k;
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      J[i].re += vs[i] - sc * a.four_ik2.re;
+      J[i].im -= sc * a.four_ik2.im;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) out[i] = J[i];
+}
+
+// ------------------------------------------------------------------ far tiles
+template <typename T>
+struct TriLds;
+
+// LDS image of one tile triangle (structure of arrays, `cap` triangles).
+template <typename T, int NQ>
+struct TileSide {
+  double* c;    // cap*3 centroid (FP64: the centroid difference is formed in FP64)
+  T* u;         // cap*NQ*3 quad points - centroid
+  T* av;        // cap*9 vertices - centroid
+  double* sig;  // cap*3 half sign (0 when the half's basis is outside this block)
+  int* slot;    // cap*3 local basis index in this block or -1
+  int* gid;     // cap
+  int* nb;      // cap*3 edge neighbours
+};
+
+template <typename T, int NQ>
+__device__ inline char* carve_side(char* p, int cap, TileSide<T, NQ>& s) {
+  s.c = reinterpret_cast<double*>(p); p += sizeof(double) * cap * 3;
+  s.sig = reinterpret_cast<double*>(p); p += sizeof(double) * cap * 3;
+  s.u = reinterpret_cast<T*>(p); p += sizeof(T) * cap * NQ * 3;
+  s.av = reinterpret_cast<T*>(p); p += sizeof(T) * cap * 9;
+  p = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(p) + 15) & ~uintptr_t(15));
+  s.slot = reinterpret_cast<int*>(p); p += sizeof(int) * cap * 3;
+  s.gid = reinterpret_cast<int*>(p); p += sizeof(int) * cap;
+  s.nb = reinterpret_cast<int*>(p); p += sizeof(int) * cap * 3;
+  p = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(p) + 15) & ~uintptr_t(15));
+  return p;
+}
+
+template <typename T, int NQ>
+__host__ __device__ inline size_t side_bytes(int cap) {
+  size_t b = sizeof(double) * cap * 6 + sizeof(T) * cap * (NQ * 3 + 9);
+  b = (b + 15) & ~size_t(15);
+  b += sizeof(int) * cap * 7;
+  b = (b + 15) & ~size_t(15);
+  return b;
+}
+
+template <typename T, int NQ>
+__device__ inline void load_side(const AsmArgs& a, int blk, int nt_side, TileSide<T, NQ>& s, int tid, int nthr) {
+  const int* tl = a.blk_tri + a.blk_off[blk];
+  const int lo = blk * a.TB;
+  for (int i = tid; i < nt_side; i += nthr) {
+    int g = tl[i];
+    s.gid[i] = g;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s.c[3 * i + c] = a.cen[3 * (i64)g + c];
+#pragma unroll
+    for (int k = 0; k < NQ * 3; ++k) s.u[i * NQ * 3 + k] = (T)a.uq[(i64)g * NQ * 3 + k];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) s.av[9 * i + k] = (T)a.av[9 * (i64)g + k];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      int r = a.hrwg[3 * (i64)g + e];
+      int loc = (r >= lo && r < lo + a.TB) ? r - lo : -1;
+      s.slot[3 * i + e] = loc;
+      s.sig[3 * i + e] = loc >= 0 ? a.hsig[3 * (i64)g + e] : 0.0;
+      s.nb[3 * i + e] = a.nb[3 * (i64)g + e];
+    }
+  }
+}
+
+// Far-pair J block in compute type T (moment form in centroid-relative
+// coordinates, so FP32 keeps its precision for nearby pairs).
+template <typename T, int NQ>
+__device__ inline void far_block(const TileSide<T, NQ>& R, int i, const TileSide<T, NQ>& C, int j,
+                                 const T* ww, T kr, T ki, T fik_re, T fik_im, T Jr[9], T Ji[9]) {
+  T dc[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) dc[c] = (T)(R.c[3 * i + c] - C.c[3 * j + c]);
+  T ur[NQ][3], uc[NQ][3];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      ur[q][c] = R.u[(i * NQ + q) * 3 + c];
+      uc[q][c] = C.u[(j * NQ + q) * 3 + c];
+    }
+  T m0r = 0, m0i = 0, muvr = 0, muvi = 0;
+  T mur[3] = {0, 0, 0}, mui[3] = {0, 0, 0}, mvr[3] = {0, 0, 0}, mvi[3] = {0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+#pragma unroll
+    for (int p = 0; p < NQ; ++p) {
+      T dx = dc[0] + (ur[q][0] - uc[p][0]);
+      T dy = dc[1] + (ur[q][1] - uc[p][1]);
+      T dz = dc[2] + (ur[q][2] - uc[p][2]);
+      T R2 = dx * dx + dy * dy + dz * dz;
+      T Rd = sqrt(R2);
+      T g = ww[q * NQ + p] / Rd;
+      T sn, cs;
+      if constexpr (sizeof(T) == 8) {
+        sincos(kr * Rd, &sn, &cs);
+      } else {
+        sincosf(kr * Rd, &sn, &cs);
+      }
+      if (ki != (T)0) g *= exp(ki * Rd);
+      T gr = g * cs, gi = -g * sn;
+      T uv = ur[q][0] * uc[p][0] + ur[q][1] * uc[p][1] + ur[q][2] * uc[p][2];
+      m0r += gr; m0i += gi;
+      muvr += gr * uv; muvi += gi * uv;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        mur[c] += gr * ur[q][c]; mui[c] += gi * ur[q][c];
+        mvr[c] += gr * uc[p][c]; mvi[c] += gi * uc[p][c];
+      }
+    }
+  }
+  // (u - a).(u' - a') = u.u' - a.u' - u.a' + a.a'
+  T bvr[3], bvi[3];  // a'_f . Mu
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    const T* af = C.av + 9 * j + 3 * f;
+    bvr[f] = af[0] * mur[0] + af[1] * mur[1] + af[2] * mur[2];
+    bvi[f] = af[0] * mui[0] + af[1] * mui[1] + af[2] * mui[2];
+  }
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const T* ae = R.av + 9 * i + 3 * e;
+    T avr = ae[0] * mvr[0] + ae[1] * mvr[1] + ae[2] * mvr[2];
+    T avi = ae[0] * mvi[0] + ae[1] * mvi[1] + ae[2] * mvi[2];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const T* af = C.av + 9 * j + 3 * f;
+      T aa = ae[0] * af[0] + ae[1] * af[1] + ae[2] * af[2];
+      T cr = aa - fik_re, ci = -fik_im;
+      Jr[3 * e + f] = muvr - avr - bvr[f] + (cr * m0r - ci * m0i);
+      Ji[3 * e + f] = muvi - avi - bvi[f] + (cr * m0i + ci * m0r);
+    }
+  }
+}
+
+__device__ inline void tile_coords(int nblk, int& bi, int& bj, bool& valid) {
+  // XCD-aware order: workgroup w runs on XCD w % 8; give each XCD a contiguous
+  // range of tiles so tiles sharing a column block share that XCD's L2.
+  i64 T = (i64)nblk * (nblk + 1) / 2;
+  i64 per = (T + 7) / 8;
+  i64 w = blockIdx.x;
+  i64 l = (w % 8) * per + w / 8;
+  valid = l < T;
+  if (!valid) return;
+  // l = bj (bj + 1) / 2 + bi, 0 <= bi <= bj
+  i64 b = (i64)((sqrt(8.0 * (double)l + 1.0) - 1.0) * 0.5);
+  while (b * (b + 1) / 2 > l) --b;
+  while ((b + 1) * (b + 2) / 2 <= l) ++b;
+  bj = (int)b;
+  bi = (int)(l - b * (b + 1) / 2);
+}
+
+// MODE 0: write Z (AccT = storage scalar), MODE 1: matrix-free y += Z x.
+template <typename T, typename AccT, int NQ, int NTHR, int MODE>
+__global__ __launch_bounds__(NTHR) void k_tile(AsmArgs a, int cap, AccT* __restrict__ zout,
+                                                const double* __restrict__ x, double* __restrict__ y) {
+  extern __shared__ __align__(16) char smem[];
+  int bi, bj;
+  bool valid;
+  tile_coords(a.nblk, bi, bj, valid);
+  if (!valid) return;
+  const int TB = a.TB;
+  const int ldt = TB + 1;
+  AccT* zt = reinterpret_cast<AccT*>(smem);
+  char* p = smem + sizeof(AccT) * 2 * TB * ldt;
+  p = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(p) + 15) & ~uintptr_t(15));
+  TileSide<T, NQ> R, C;
+  p = carve_side<T, NQ>(p, cap, R);
+  p = carve_side<T, NQ>(p, cap, C);
+  int* extr = reinterpret_cast<int*>(p);
+  int* extc = extr + TB;
+  T* ww = reinterpret_cast<T*>(extc + TB + 2);
+  const int tid = threadIdx.x;
+  const int nI = a.blk_off[bi + 1] - a.blk_off[bi];
+  const int nJ = a.blk_off[bj + 1] - a.blk_off[bj];
+  for (int k = tid; k < 2 * TB * ldt; k += NTHR) zt[k] = (AccT)0;
+  load_side<T, NQ>(a, bi, nI, R, tid, NTHR);
+  load_side<T, NQ>(a, bj, nJ, C, tid, NTHR);
+  for (int k = tid; k < TB; k += NTHR) {
+    i64 gi = (i64)bi * TB + k, gj = (i64)bj * TB + k;
+    extr[k] = gi < a.n ? a.ext[gi] : 0x7fffffff;
+    extc[k] = gj < a.n ? a.ext[gj] : 0x7fffffff;
+  }
+  if (tid < NQ * NQ) ww[tid] = (T)(a.w[tid / NQ] * a.w[tid % NQ] * INV4PI);
+  __syncthreads();
+
+  const T kr = (T)a.k_re, ki = (T)a.k_im;
+  // 4/k^2
+  double kk_re = a.k_re * a.k_re - a.k_im * a.k_im, kk_im = 2.0 * a.k_re * a.k_im;
+  double kd = kk_re * kk_re + kk_im * kk_im;
+  const double f4r = 4.0 * kk_re / kd, f4i = -4.0 * kk_im / kd;
+  const c2* jself = reinterpret_cast<const c2*>(a.jself);
+  const c2* jadj = reinterpret_cast<const c2*>(a.jadj);
+
+  const int npairs = nI * nJ;
+  for (int pidx = tid; pidx < npairs; pidx += NTHR) {
+    const int i = pidx / nJ;
+    const int j = pidx - i * nJ;
+    const int gi = R.gid[i], gj = C.gid[j];
+    int jn = -1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (R.nb[3 * i + k] == gj) jn = k;
+    if (gi == gj || jn >= 0) {
+      // self / edge-adjacent: FP64 table lookup with the caller-order rule
+      int jn2 = 0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if (C.nb[3 * j + k] == gi) jn2 = k;
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        int rs = R.slot[3 * i + e];
+        if (rs < 0) continue;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+          int cs = C.slot[3 * j + f];
+          if (cs < 0) continue;
+          bool fwd = extr[rs] <= extc[cs];
+          c2 v;
+          if (gi == gj)
+            v = fwd ? jself[(i64)gi * 9 + 3 * e + f] : jself[(i64)gi * 9 + 3 * f + e];
+          else
+            v = fwd ? jadj[((i64)gi * 3 + jn) * 9 + 3 * e + f] : jadj[((i64)gj * 3 + jn2) * 9 + 3 * f + e];
+          double s = R.sig[3 * i + e] * C.sig[3 * j + f];
+          AccT* dst = zt + 2 * (rs * ldt + cs);
+          atomicAdd(dst, (AccT)(v.re * s));
+          atomicAdd(dst + 1, (AccT)(v.im * s));
+        }
+      }
+      continue;
+    }
+    T Jr[9], Ji[9];
+    far_block<T, NQ>(R, i, C, j, ww, kr, ki, (T)f4r, (T)f4i, Jr, Ji);
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      int rs = R.slot[3 * i + e];
+      if (rs < 0) continue;
+      T sr = (T)R.sig[3 * i + e];
+#pragma unroll
+      for (int f = 0; f < 3; ++f) {
+        int cs = C.slot[3 * j + f];
+        if (cs < 0) continue;
+        T s = sr * (T)C.sig[3 * j + f];
+        AccT* dst = zt + 2 * (rs * ldt + cs);
+        atomicAdd(dst, (AccT)(Jr[3 * e + f] * s));
+        atomicAdd(dst + 1, (AccT)(Ji[3 * e + f] * s));
+      }
+    }
+  }
+  __syncthreads();
+
+  // Z_mn = -i k eta0 len_m len_n * tile ; P = -i k eta0 = (k_im eta0, -k_re eta0)
+  const double Pr = a.k_im * a.eta0, Pi = -a.k_re * a.eta0;
+  const i64 r0 = (i64)bi * TB, c0 = (i64)bj * TB;
+  const bool diag = bi == bj;
+  if constexpr (MODE == 0) {
+    int bad = 0;
+    for (int k = tid; k < TB * TB; k += NTHR) {
+      int r = k % TB, c = k / TB;
+      i64 gr = r0 + r, gc = c0 + c;
+      if (gr >= a.n || gc >= a.n) continue;
+      int rr = r, cc = c;
+      if (diag && r > c) { rr = c; cc = r; }
+      double tr = (double)zt[2 * (rr * ldt + cc)], ti = (double)zt[2 * (rr * ldt + cc) + 1];
+      double l2 = a.len[gr] * a.len[gc];
+      double vr = (Pr * tr - Pi * ti) * l2, vi = (Pr * ti + Pi * tr) * l2;
+      bad |= !isfinite(vr) || !isfinite(vi);
+      AccT* o = zout + 2 * (gr + gc * a.ld);
+      o[0] = (AccT)vr;
+      o[1] = (AccT)vi;
+    }
+    if (!diag) {
+      for (int k = tid; k < TB * TB; k += NTHR) {
+        int c = k % TB, r = k / TB;
+        i64 gr = r0 + r, gc = c0 + c;
+        if (gr >= a.n || gc >= a.n) continue;
+        double tr = (double)zt[2 * (r * ldt + c)], ti = (double)zt[2 * (r * ldt + c) + 1];
+        double l2 = a.len[gr] * a.len[gc];
+        double vr = (Pr * tr - Pi * ti) * l2, vi = (Pr * ti + Pi * tr) * l2;
+        AccT* o = zout + 2 * (gc + gr * a.ld);
+        o[0] = (AccT)vr;
+        o[1] = (AccT)vi;
+      }
+    }
+    if (bad) atomicOr(a.flag, 1);
+  } else {
+    // y[rows] += Z_tile x[cols]; y[cols] += Z_tile^T x[rows] (off-diagonal tiles)
+    for (int k = tid; k < 2 * TB; k += NTHR) {
+      bool rows = k < TB;
+      int r = rows ? k : k - TB;
+      if (!rows && diag) continue;
+      i64 go = rows ? r0 + r : c0 + r;
+      if (go >= a.n) continue;
+      double accr = 0, acci = 0;
+      for (int c = 0; c < TB; ++c) {
+        i64 gi2 = rows ? c0 + c : r0 + c;
+        if (gi2 >= a.n) break;
+        int rr = rows ? r : c, cc = rows ? c : r;
+        if (diag && rr > cc) { int t2 = rr; rr = cc; cc = t2; }
+        double tr = (double)zt[2 * (rr * ldt + cc)], ti = (double)zt[2 * (rr * ldt + cc) + 1];
+        double l2 = a.len[go] * a.len[gi2];
+        double vr = (Pr * tr - Pi * ti) * l2, vi = (Pr * ti + Pi * tr) * l2;
+        double xr = x[2 * gi2], xi = x[2 * gi2 + 1];
+        accr += vr * xr - vi * xi;
+        acci += vr * xi + vi * xr;
+      }
+      atomicAdd(y + 2 * go, accr);
+      atomicAdd(y + 2 * go + 1, acci);
+    }
+  }
+}
+
+template <typename T, typename AccT>
+size_t tile_smem(int nq, int TB, int cap) {
+  size_t b = sizeof(AccT) * 2 * TB * (TB + 1);
+  b = (b + 15) & ~size_t(15);
+  size_t sb = 0;
+  switch (nq) {
+    case 1: sb = side_bytes<T, 1>(cap); break;
+    case 3: sb = side_bytes<T, 3>(cap); break;
+    case 4: sb = side_bytes<T, 4>(cap); break;
+    default: sb = side_bytes<T, 7>(cap); break;
+  }
+  b += 2 * sb;
+  b += sizeof(int) * (2 * TB + 2) + sizeof(T) * 49;
+  return b;
+}
+
+template <typename T, typename AccT, int MODE>
+void launch_tile(const AsmArgs& a, int cap, AccT* z, const double* x, double* y, hipStream_t s) {
+  constexpr int NTHR = 512;
+  size_t smem = tile_smem<T, AccT>(a.nq, a.TB, cap);
+  DM_REQUIRE(smem <= 160 * 1024, DM_ERR_ARGUMENT, "tile working set exceeds LDS");
+  i64 T_ = (i64)a.nblk * (a.nblk + 1) / 2;
+  i64 grid = ((T_ + 7) / 8) * 8;
+  DM_REQUIRE(grid < (1LL << 31), DM_ERR_ARGUMENT, "tile grid too large");
+  switch (a.nq) {
+#define DM_CASE(NQ)                                                                              \
+  case NQ:                                                                                       \
+    HIP_OK(hipFuncSetAttribute((const void*)k_tile<T, AccT, NQ, NTHR, MODE>,                     \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));          \
+    k_tile<T, AccT, NQ, NTHR, MODE><<<dim3((unsigned)grid), dim3(NTHR), smem, s>>>(a, cap, z, x, y); \
+    break;
+    DM_CASE(1)
+    DM_CASE(3)
+    DM_CASE(4)
+    DM_CASE(7)
+#undef DM_CASE
+    default:
+      dm_throw(DM_ERR_ARGUMENT, "unsupported quadrature size");
+  }
+  HIP_OK(hipGetLastError());
+}
+
+}  // namespace
+
+static NearArgs near_args(const AsmArgs& a, const TriTables& t) {
+  NearArgs n;
+  n.nt = a.nt;
+  n.nq = t.nq;
+  n.rq = t.rq.p;
+  n.rq7 = t.rq7.p;
+  n.vtx = t.vtx.p;
+  n.area = t.area.p;
+  n.nb = a.nb;
+  for (int i = 0; i < 7; ++i) n.w[i] = a.w[i];
+  double xi[14], w7[7];
+  dmh::quad_rule(7, xi, w7);
+  for (int i = 0; i < 7; ++i) n.w7[i] = w7[i];
+  n.q = {a.k_im, -a.k_re};
+  double kk_re = a.k_re * a.k_re - a.k_im * a.k_im, kk_im = 2.0 * a.k_re * a.k_im;
+  double kd = kk_re * kk_re + kk_im * kk_im;
+  n.four_ik2 = {4.0 * kk_re / kd, -4.0 * kk_im / kd};
+  return n;
+}
+
+void launch_near_tables(const AsmArgs& a, const TriTables& t, double* jself, double* jadj, hipStream_t s) {
+  NearArgs n = near_args(a, t);
+  unsigned g1 = (unsigned)((a.nt + 63) / 64), g2 = (unsigned)((3 * a.nt + 63) / 64);
+  k_near_self<<<g1, 64, 0, s>>>(n, jself);
+  k_near_adj<<<g2, 64, 0, s>>>(n, jadj);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_assemble_z64(const AsmArgs& a, double* z, int cap, hipStream_t s) {
+  launch_tile<double, double, 0>(a, cap, z, nullptr, nullptr, s);
+}
+void launch_assemble_z32(const AsmArgs& a, float* z, int cap, hipStream_t s) {
+  launch_tile<float, float, 0>(a, cap, z, nullptr, nullptr, s);
+}
+void launch_mf_apply(const AsmArgs& a, const double* x, double* y, int cap, hipStream_t s) {
+  launch_tile<double, double, 1>(a, cap, nullptr, x, y, s);
+}

@@ -1,0 +1,35 @@
+// k_linalg.h — launchers of the Krylov / preconditioner / LU kernels.
+#pragma once
+#include "dm_common.h"
+
+namespace dml {
+void dots(i64 n, i64 ldv, int k, const double* V, const double* w, double* h, hipStream_t s);
+void update(i64 n, i64 ldv, int k, const double* V, const double* h, double* w, hipStream_t s);
+void combine(i64 n, i64 ldv, int k, const double* V, const double* y, double* x, hipStream_t s);
+void norm2(i64 n, const double* w, double* out, hipStream_t s);
+void axpby(i64 n, double ar, double ai, const double* x, double br, double bi, double* y, hipStream_t s);
+void extract_blocks64(i64 n, i64 ld, int bs, int nb, const double* z, double* blocks, hipStream_t s);
+void extract_blocks32(i64 n, i64 ld, int bs, int nb, const float* z, double* blocks, hipStream_t s);
+void block_inverse(int bs, int nb, double* blocks, int* status, hipStream_t s);
+void block_apply(i64 n, int bs, int nb, const double* inv, const double* x, double* y, hipStream_t s);
+void diag_inv(i64 n, i64 ld, const double* z, double* d, hipStream_t s);
+void diag_apply(i64 n, const double* d, const double* x, double* y, hipStream_t s);
+void lu_factor(i64 n, i64 ld, double* A, int* ipiv, int* status, hipStream_t s);
+void lu_solve(i64 n, i64 ld, const double* A, const int* ipiv, double* B, i64 ldb, int nrhs, hipStream_t s);
+void unpermute64(i64 n, i64 ld, i64 c0, i64 nc, const int* iofe, const double* z, double* out, hipStream_t s);
+void unpermute32(i64 n, i64 ld, i64 c0, i64 nc, const int* iofe, const float* z, double* out, hipStream_t s);
+void entries64(i64 ld, i64 count, const int* iofe, const i64* m, const i64* n, const double* z, double* out, hipStream_t s);
+void entries32(i64 ld, i64 count, const int* iofe, const i64* m, const i64* n, const float* z, double* out, hipStream_t s);
+void finite64(i64 n, i64 ld, const double* z, int* flag, hipStream_t s);
+void finite32(i64 n, i64 ld, const float* z, int* flag, hipStream_t s);
+void conj(i64 n, double* v, hipStream_t s);
+}  // namespace dml
+
+namespace dmf {
+// Plane-wave RHS for M waves into V (n x M, ldv), entry of internal basis i
+// written at row out_idx[i] (identity or caller order).
+void planewave(const dm_rwg& r, const TriTables& t, const double* kvec, const double* pol, const double* E0, int M,
+               const int* out_idx, double* V, i64 ldv, hipStream_t s);
+void radiation_vectors(const dm_rwg& r, const TriTables& t, const double* rhat, i64 nomega, double k, double eta0,
+                       double* G, hipStream_t s);
+}  // namespace dmf

@@ -1,0 +1,624 @@
+// k_linalg.hip — dense EFIE products, Krylov building blocks, preconditioner
+// and direct-solve kernels (all in the internal spatial basis order).
+//
+// Z is symmetric (Z = Z^T, src/assembly/EFIE.jl:737-742), so y = Z x is formed
+// as column dot products y_n = sum_m Z[m, n] x_m: every wave streams whole
+// columns of the column-major matrix with 16-byte loads, which is the HBM-bound
+// access pattern the single-right-hand-side product needs.
+#include <hip/hip_runtime.h>
+
+#include "dm_common.h"
+#include "k_linalg.h"
+
+#include <algorithm>
+
+namespace {
+
+typedef double nd2 __attribute__((ext_vector_type(2)));
+typedef float nf4 __attribute__((ext_vector_type(4)));
+// Streaming loads: the matrix is read once per product, so bypass cache retention.
+__device__ inline double2 ld_stream(const double2* p) {
+  nd2 v = __builtin_nontemporal_load(reinterpret_cast<const nd2*>(p));
+  return make_double2(v.x, v.y);
+}
+__device__ inline float4 ld_stream(const float4* p) {
+  nf4 v = __builtin_nontemporal_load(reinterpret_cast<const nf4*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// y = Z x, complex128 Z.  Each wave owns NC consecutive columns; lane l reads
+// rows l, l+64, ...; x is shared by the NC columns from registers.
+template <int NC>
+__global__ __launch_bounds__(256) void k_gemv_z64(i64 n, i64 ld, const double2* __restrict__ z,
+                                                  const double2* __restrict__ x, double2* __restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const i64 c0 = ((i64)blockIdx.x * 4 + (threadIdx.x >> 6)) * NC;
+  if (c0 >= n) return;
+  double ar[NC], ai[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) ar[c] = ai[c] = 0.0;
+  const double2* zc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) zc[c] = z + (c0 + c < n ? c0 + c : c0) * ld;
+  i64 m = lane;
+  for (; m + 64 < n; m += 128) {
+    double2 x0 = x[m], x1 = x[m + 64];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      double2 a0 = ld_stream(zc[c] + m);
+      double2 a1 = ld_stream(zc[c] + m + 64);
+      ar[c] += a0.x * x0.x - a0.y * x0.y + a1.x * x1.x - a1.y * x1.y;
+      ai[c] += a0.x * x0.y + a0.y * x0.x + a1.x * x1.y + a1.y * x1.x;
+    }
+  }
+  for (; m < n; m += 64) {
+    double2 x0 = x[m];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      double2 a0 = zc[c][m];
+      ar[c] += a0.x * x0.x - a0.y * x0.y;
+      ai[c] += a0.x * x0.y + a0.y * x0.x;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    double sr = wave_sum(ar[c]), si = wave_sum(ai[c]);
+    if (lane == 0 && c0 + c < n) y[c0 + c] = make_double2(sr, si);
+  }
+}
+
+// y = Z x with complex64 storage and FP64 arithmetic (the product is HBM
+// bound, so FP64 accumulation costs nothing).  Lane reads two consecutive
+// entries (16 bytes) per column.
+template <int NC>
+__global__ __launch_bounds__(256) void k_gemv_z32(i64 n, i64 ld, const float4* __restrict__ z4,
+                                                  const double2* __restrict__ x, double2* __restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const i64 c0 = ((i64)blockIdx.x * 4 + (threadIdx.x >> 6)) * NC;
+  if (c0 >= n) return;
+  double ar[NC], ai[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) ar[c] = ai[c] = 0.0;
+  const i64 ld2 = ld >> 1;  // ld is even
+  const i64 n2 = n >> 1;
+  for (i64 m2 = lane; m2 < n2; m2 += 64) {
+    double2 x0 = x[2 * m2], x1 = x[2 * m2 + 1];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      i64 col = c0 + c < n ? c0 + c : c0;
+      float4 a = ld_stream(z4 + col * ld2 + m2);
+      ar[c] += (double)a.x * x0.x - (double)a.y * x0.y + (double)a.z * x1.x - (double)a.w * x1.y;
+      ai[c] += (double)a.x * x0.y + (double)a.y * x0.x + (double)a.z * x1.y + (double)a.w * x1.x;
+    }
+  }
+  if ((n & 1) && lane == 0) {
+    i64 m = n - 1;
+    const float2* z2 = reinterpret_cast<const float2*>(z4);
+    double2 x0 = x[m];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      i64 col = c0 + c < n ? c0 + c : c0;
+      float2 a = z2[col * ld + m];
+      ar[c] += (double)a.x * x0.x - (double)a.y * x0.y;
+      ai[c] += (double)a.x * x0.y + (double)a.y * x0.x;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    double sr = wave_sum(ar[c]), si = wave_sum(ai[c]);
+    if (lane == 0 && c0 + c < n) y[c0 + c] = make_double2(sr, si);
+  }
+}
+
+__global__ void k_permute(i64 n, const int* __restrict__ idx, const double2* __restrict__ src,
+                          double2* __restrict__ dst, int gather) {
+  i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (gather) dst[i] = src[idx[i]];
+  else dst[idx[i]] = src[i];
+}
+
+// ---------------------------------------------------------------- Krylov ops
+// h[i] = sum_n conj(V[n, i]) w[n] for i < k; one workgroup per basis vector.
+__global__ __launch_bounds__(256) void k_dots(i64 n, i64 ldv, const double2* __restrict__ V,
+                                              const double2* __restrict__ w, double2* __restrict__ h) {
+  const int i = blockIdx.x;
+  const double2* v = V + (i64)i * ldv;
+  double ar = 0, ai = 0;
+  for (i64 m = threadIdx.x; m < n; m += 256) {
+    double2 a = v[m], b = w[m];
+    ar += a.x * b.x + a.y * b.y;
+    ai += a.x * b.y - a.y * b.x;
+  }
+  __shared__ double sr[4], si[4];
+  ar = wave_sum(ar);
+  ai = wave_sum(ai);
+  if ((threadIdx.x & 63) == 0) { sr[threadIdx.x >> 6] = ar; si[threadIdx.x >> 6] = ai; }
+  __syncthreads();
+  if (threadIdx.x == 0) h[i] = make_double2(sr[0] + sr[1] + sr[2] + sr[3], si[0] + si[1] + si[2] + si[3]);
+}
+
+// w[n] -= sum_{i<k} V[n, i] h[i]  (acc_h: h_acc[i] += h[i] for CGS2 bookkeeping)
+__global__ __launch_bounds__(256) void k_update(i64 n, i64 ldv, int k, const double2* __restrict__ V,
+                                                const double2* __restrict__ h, double2* __restrict__ w) {
+  __shared__ double2 hs[1024];
+  for (int i = threadIdx.x; i < k; i += 256) hs[i] = h[i];
+  __syncthreads();
+  i64 m = blockIdx.x * (i64)256 + threadIdx.x;
+  if (m >= n) return;
+  double ar = 0, ai = 0;
+  for (int i = 0; i < k; ++i) {
+    double2 a = V[(i64)i * ldv + m], b = hs[i];
+    ar += a.x * b.x - a.y * b.y;
+    ai += a.x * b.y + a.y * b.x;
+  }
+  double2 o = w[m];
+  w[m] = make_double2(o.x - ar, o.y - ai);
+}
+
+// out[0] = ||w||^2 (single workgroup of 1024 threads)
+__global__ __launch_bounds__(1024) void k_norm2(i64 n, const double2* __restrict__ w, double* out) {
+  double s = 0;
+  for (i64 m = threadIdx.x; m < n; m += 1024) {
+    double2 a = w[m];
+    s += a.x * a.x + a.y * a.y;
+  }
+  s = wave_sum(s);
+  __shared__ double part[16];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int i = 0; i < 16; ++i) t += part[i];
+    out[0] = t;
+  }
+}
+
+// y = a*x + b*y (complex a, b)
+__global__ void k_axpby(i64 n, double2 a, const double2* __restrict__ x, double2 b, double2* __restrict__ y) {
+  i64 m = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  double2 xv = x[m], yv = y[m];
+  y[m] = make_double2(a.x * xv.x - a.y * xv.y + b.x * yv.x - b.y * yv.y,
+                      a.x * xv.y + a.y * xv.x + b.x * yv.y + b.y * yv.x);
+}
+
+// x += V y (k columns, y on device)
+__global__ __launch_bounds__(256) void k_combine(i64 n, i64 ldv, int k, const double2* __restrict__ V,
+                                                 const double2* __restrict__ yv, double2* __restrict__ x) {
+  __shared__ double2 hs[1024];
+  for (int i = threadIdx.x; i < k; i += 256) hs[i] = yv[i];
+  __syncthreads();
+  i64 m = blockIdx.x * (i64)256 + threadIdx.x;
+  if (m >= n) return;
+  double ar = 0, ai = 0;
+  for (int i = 0; i < k; ++i) {
+    double2 a = V[(i64)i * ldv + m], b = hs[i];
+    ar += a.x * b.x - a.y * b.y;
+    ai += a.x * b.y + a.y * b.x;
+  }
+  double2 o = x[m];
+  x[m] = make_double2(o.x + ar, o.y + ai);
+}
+
+__global__ void k_cast_matrix(i64 n, i64 ld, const double2* __restrict__ src, float2* __restrict__ dst) {
+  i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (idx >= ld * n) return;
+  double2 v = src[idx];
+  dst[idx] = make_float2((float)v.x, (float)v.y);
+}
+
+// ------------------------------------------------------- block preconditioner
+// Extract diagonal blocks of Z (internal order): blocks[b] = Z[b*bs .. , b*bs ..],
+// identity-padded past n.
+template <typename ZT>
+__global__ void k_extract_blocks(i64 n, i64 ld, int bs, const ZT* __restrict__ z, double2* __restrict__ blocks) {
+  int b = blockIdx.y;
+  i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (idx >= (i64)bs * bs) return;
+  int r = idx % bs, c = idx / bs;
+  i64 gr = (i64)b * bs + r, gc = (i64)b * bs + c;
+  double2 v;
+  if (gr < n && gc < n) {
+    ZT e = z[gr + gc * ld];
+    v = make_double2((double)e.x, (double)e.y);
+  } else {
+    v = make_double2(r == c ? 1.0 : 0.0, 0.0);
+  }
+  blocks[(i64)b * bs * bs + idx] = v;
+}
+
+// In-place Gauss-Jordan inverse with partial pivoting of each bs x bs block
+// (column-major), one workgroup per block.  The singular-pivot flag is set in
+// status[b] (the host turns it into an error).
+__global__ __launch_bounds__(256) void k_block_inverse(int bs, double2* __restrict__ blocks, int* status) {
+  double2* A = blocks + (i64)blockIdx.x * bs * bs;
+  extern __shared__ double2 sh[];
+  double2* rowk = sh;         // bs
+  double2* colk = sh + bs;    // bs
+  __shared__ int piv;
+  __shared__ double pmax;
+  // perm tracks row swaps to un-permute the columns at the end
+  int* perm = reinterpret_cast<int*>(sh + 2 * bs);
+  for (int i = threadIdx.x; i < bs; i += blockDim.x) perm[i] = i;
+  __syncthreads();
+  for (int k = 0; k < bs; ++k) {
+    if (threadIdx.x == 0) {
+      int p = k;
+      double m = -1;
+      for (int i = k; i < bs; ++i) {
+        double2 v = A[i + (i64)k * bs];
+        double a = v.x * v.x + v.y * v.y;
+        if (a > m) { m = a; p = i; }
+      }
+      piv = p;
+      pmax = m;
+    }
+    __syncthreads();
+    if (pmax == 0.0) {
+      if (threadIdx.x == 0) status[blockIdx.x] = 1;
+      return;
+    }
+    int p = piv;
+    if (p != k) {
+      for (int c = threadIdx.x; c < bs; c += blockDim.x) {
+        double2 t = A[k + (i64)c * bs];
+        A[k + (i64)c * bs] = A[p + (i64)c * bs];
+        A[p + (i64)c * bs] = t;
+      }
+      if (threadIdx.x == 0) { int t = perm[k]; perm[k] = perm[p]; perm[p] = t; }
+    }
+    __syncthreads();
+    double2 d = A[k + (i64)k * bs];
+    double dn = d.x * d.x + d.y * d.y;
+    double2 inv = make_double2(d.x / dn, -d.y / dn);
+    // row k scaled by 1/d (with the pivot column replaced by 1/d)
+    for (int c = threadIdx.x; c < bs; c += blockDim.x) {
+      double2 v = c == k ? make_double2(1.0, 0.0) : A[k + (i64)c * bs];
+      rowk[c] = make_double2(v.x * inv.x - v.y * inv.y, v.x * inv.y + v.y * inv.x);
+    }
+    for (int r = threadIdx.x; r < bs; r += blockDim.x) colk[r] = r == k ? make_double2(0.0, 0.0) : A[r + (i64)k * bs];
+    __syncthreads();
+    for (i64 idx = threadIdx.x; idx < (i64)bs * bs; idx += blockDim.x) {
+      int r = idx % bs, c = idx / bs;
+      if (r == k) {
+        A[idx] = rowk[c];
+      } else {
+        double2 base = c == k ? make_double2(0.0, 0.0) : A[idx];
+        double2 f = colk[r], g = rowk[c];
+        A[idx] = make_double2(base.x - (f.x * g.x - f.y * g.y), base.y - (f.x * g.y + f.y * g.x));
+      }
+    }
+    __syncthreads();
+  }
+  // undo the row permutation of the input: inverse columns are permuted
+  // (A P)^-1 handling: Gauss-Jordan on P A gives (P A)^-1 = A^-1 P^T, so
+  // A^-1 = (P A)^-1 P: column j of A^-1 is column perm^-1(j) of the result.
+  for (int r = 0; r < bs; ++r) {
+    for (int c = threadIdx.x; c < bs; c += blockDim.x) rowk[perm[c]] = A[r + (i64)c * bs];
+    __syncthreads();
+    for (int c = threadIdx.x; c < bs; c += blockDim.x) A[r + (i64)c * bs] = rowk[c];
+    __syncthreads();
+  }
+}
+
+// y_b = inv_b x_b for every block (block-diagonal product).
+__global__ __launch_bounds__(256) void k_block_apply(i64 n, int bs, const double2* __restrict__ inv,
+                                                     const double2* __restrict__ x, double2* __restrict__ y) {
+  int b = blockIdx.y;
+  int r = blockIdx.x * 256 + threadIdx.x;
+  extern __shared__ double2 xs[];
+  for (int c = threadIdx.x; c < bs; c += 256) {
+    i64 g = (i64)b * bs + c;
+    xs[c] = g < n ? x[g] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  if (r >= bs) return;
+  i64 gr = (i64)b * bs + r;
+  if (gr >= n) return;
+  const double2* A = inv + (i64)b * bs * bs;
+  double ar = 0, ai = 0;
+  for (int c = 0; c < bs; ++c) {
+    double2 a = A[r + (i64)c * bs], v = xs[c];
+    ar += a.x * v.x - a.y * v.y;
+    ai += a.x * v.y + a.y * v.x;
+  }
+  y[gr] = make_double2(ar, ai);
+}
+
+__global__ void k_diag_inv(i64 n, i64 ld, const double2* __restrict__ z, double2* __restrict__ d) {
+  i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double2 v = z[i + i * ld];
+  double m = v.x * v.x + v.y * v.y;
+  d[i] = make_double2(v.x / m, -v.y / m);
+}
+
+__global__ void k_diag_apply(i64 n, const double2* __restrict__ d, const double2* __restrict__ x, double2* __restrict__ y) {
+  i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double2 a = d[i], v = x[i];
+  y[i] = make_double2(a.x * v.x - a.y * v.y, a.x * v.y + a.y * v.x);
+}
+
+// ------------------------------------------------------------ dense LU
+// Right-looking LU with partial pivoting, one column step per pair of
+// launches (pivot+swap+scale, then rank-1 update).  Used for the dense direct
+// method (solve_forward :direct), sized for N up to a few thousand.
+__global__ __launch_bounds__(1024) void k_lu_pivot(i64 n, i64 ld, int k, double2* __restrict__ A, int* __restrict__ ipiv,
+                                                   int* status) {
+  __shared__ double best[1024];
+  __shared__ int bidx[1024];
+  double m = -1;
+  int p = k;
+  for (i64 i = k + threadIdx.x; i < n; i += 1024) {
+    double2 v = A[i + (i64)k * ld];
+    double a = v.x * v.x + v.y * v.y;
+    if (a > m) { m = a; p = (int)i; }
+  }
+  best[threadIdx.x] = m;
+  bidx[threadIdx.x] = p;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      if (best[threadIdx.x + s] > best[threadIdx.x] ||
+          (best[threadIdx.x + s] == best[threadIdx.x] && bidx[threadIdx.x + s] < bidx[threadIdx.x])) {
+        best[threadIdx.x] = best[threadIdx.x + s];
+        bidx[threadIdx.x] = bidx[threadIdx.x + s];
+      }
+    }
+    __syncthreads();
+  }
+  int piv = bidx[0];
+  if (best[0] <= 0.0) {
+    if (threadIdx.x == 0) status[0] = k + 1;
+    return;
+  }
+  if (threadIdx.x == 0) ipiv[k] = piv;
+  if (piv != k) {
+    for (i64 c = threadIdx.x; c < n; c += 1024) {
+      double2 t = A[k + c * ld];
+      A[k + c * ld] = A[piv + c * ld];
+      A[piv + c * ld] = t;
+    }
+  }
+  __syncthreads();
+  double2 d = A[k + (i64)k * ld];
+  double dn = d.x * d.x + d.y * d.y;
+  double2 inv = make_double2(d.x / dn, -d.y / dn);
+  for (i64 i = k + 1 + threadIdx.x; i < n; i += 1024) {
+    double2 v = A[i + (i64)k * ld];
+    A[i + (i64)k * ld] = make_double2(v.x * inv.x - v.y * inv.y, v.x * inv.y + v.y * inv.x);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_lu_update(i64 n, i64 ld, int k, double2* __restrict__ A) {
+  i64 r = k + 1 + blockIdx.x * (i64)256 + threadIdx.x;
+  i64 c = k + 1 + blockIdx.y;
+  if (r >= n || c >= n) return;
+  double2 l = A[r + (i64)k * ld], u = A[k + c * ld], a = A[r + c * ld];
+  A[r + c * ld] = make_double2(a.x - (l.x * u.x - l.y * u.y), a.y - (l.x * u.y + l.y * u.x));
+}
+
+// Solve with the LU factors for one right-hand side per workgroup (in place).
+__global__ __launch_bounds__(1024) void k_lu_solve(i64 n, i64 ld, const double2* __restrict__ A, const int* __restrict__ ipiv,
+                                                   double2* __restrict__ B, i64 ldb) {
+  double2* b = B + (i64)blockIdx.x * ldb;
+  if (threadIdx.x == 0)
+    for (i64 k = 0; k < n; ++k) {
+      int p = ipiv[k];
+      if (p != k) { double2 t = b[k]; b[k] = b[p]; b[p] = t; }
+    }
+  __syncthreads();
+  for (i64 k = 0; k < n; ++k) {  // L y = b (unit diagonal)
+    double2 bk = b[k];
+    for (i64 i = k + 1 + threadIdx.x; i < n; i += 1024) {
+      double2 l = A[i + k * ld], v = b[i];
+      b[i] = make_double2(v.x - (l.x * bk.x - l.y * bk.y), v.y - (l.x * bk.y + l.y * bk.x));
+    }
+    __syncthreads();
+  }
+  for (i64 k = n - 1; k >= 0; --k) {  // U x = y
+    if (threadIdx.x == 0) {
+      double2 d = A[k + k * ld], v = b[k];
+      double dn = d.x * d.x + d.y * d.y;
+      b[k] = make_double2((v.x * d.x + v.y * d.y) / dn, (v.y * d.x - v.x * d.y) / dn);
+    }
+    __syncthreads();
+    double2 xk = b[k];
+    for (i64 i = threadIdx.x; i < k; i += 1024) {
+      double2 u = A[i + k * ld], v = b[i];
+      b[i] = make_double2(v.x - (u.x * xk.x - u.y * xk.y), v.y - (u.x * xk.y + u.y * xk.x));
+    }
+    __syncthreads();
+  }
+}
+
+inline unsigned nblocks(i64 n, int t) { return (unsigned)((n + t - 1) / t); }
+// Caller-order copy of a block of columns: out[r + (c - c0)*n] = Z[int(r) + int(c)*ld]
+template <typename ZT>
+__global__ void k_unpermute(i64 n, i64 ld, i64 c0, i64 nc, const int* __restrict__ iofe, const ZT* __restrict__ z,
+                            double2* __restrict__ out) {
+  i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (idx >= n * nc) return;
+  i64 r = idx % n, c = c0 + idx / n;
+  ZT v = z[(i64)iofe[r] + (i64)iofe[c] * ld];
+  out[idx] = make_double2((double)v.x, (double)v.y);
+}
+
+template <typename ZT>
+__global__ void k_entries(i64 ld, i64 count, const int* __restrict__ iofe, const i64* __restrict__ m,
+                          const i64* __restrict__ nn, const ZT* __restrict__ z, double2* __restrict__ out) {
+  i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  ZT v = z[(i64)iofe[m[i]] + (i64)iofe[nn[i]] * ld];
+  out[i] = make_double2((double)v.x, (double)v.y);
+}
+
+// flag[0] |= 1 when any stored entry is non-finite (grid-stride over columns).
+template <typename ZT>
+__global__ void k_finite(i64 n, i64 ld, const ZT* __restrict__ z, int* flag) {
+  int bad = 0;
+  for (i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x; idx < n * n; idx += (i64)gridDim.x * blockDim.x) {
+    i64 r = idx % n, c = idx / n;
+    ZT v = z[r + c * ld];
+    bad |= !isfinite((double)v.x) || !isfinite((double)v.y);
+  }
+  if (bad) atomicOr(flag, 1);
+}
+
+__global__ void k_conj(i64 n, double2* v) {
+  i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (i < n) v[i].y = -v[i].y;
+}
+
+
+}  // namespace
+
+void launch_gemv_z64(i64 n, i64 ld, const double* z, const double* x, double* y, hipStream_t s) {
+  constexpr int NC = 4;
+  i64 cols_per_block = 4 * NC;
+  k_gemv_z64<NC><<<nblocks(n, (int)cols_per_block), 256, 0, s>>>(
+      n, ld, reinterpret_cast<const double2*>(z), reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y));
+  HIP_OK(hipGetLastError());
+}
+
+void launch_gemv_z32(i64 n, i64 ld, const float* z, const double* x, double* y, hipStream_t s) {
+  constexpr int NC = 4;
+  DM_REQUIRE((ld & 1) == 0, DM_ERR_ARGUMENT, "complex64 matrix leading dimension must be even");
+  k_gemv_z32<NC><<<nblocks(n, 4 * NC), 256, 0, s>>>(
+      n, ld, reinterpret_cast<const float4*>(z), reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y));
+  HIP_OK(hipGetLastError());
+}
+
+void launch_permute(i64 n, const int* idx, const double* src, double* dst, int gather, hipStream_t s) {
+  k_permute<<<nblocks(n, 256), 256, 0, s>>>(n, idx, reinterpret_cast<const double2*>(src),
+                                             reinterpret_cast<double2*>(dst), gather);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_cast_c128_to_c64_matrix(i64 n, i64 ld, const double* src, float* dst, hipStream_t s) {
+  k_cast_matrix<<<nblocks(ld * n, 256), 256, 0, s>>>(n, ld, reinterpret_cast<const double2*>(src),
+                                                      reinterpret_cast<float2*>(dst));
+  HIP_OK(hipGetLastError());
+}
+
+namespace dml {
+
+void dots(i64 n, i64 ldv, int k, const double* V, const double* w, double* h, hipStream_t s) {
+  if (k <= 0) return;
+  k_dots<<<k, 256, 0, s>>>(n, ldv, reinterpret_cast<const double2*>(V), reinterpret_cast<const double2*>(w),
+                           reinterpret_cast<double2*>(h));
+  HIP_OK(hipGetLastError());
+}
+void update(i64 n, i64 ldv, int k, const double* V, const double* h, double* w, hipStream_t s) {
+  if (k <= 0) return;
+  DM_REQUIRE(k <= 1024, DM_ERR_ARGUMENT, "Krylov basis larger than 1024 vectors");
+  k_update<<<nblocks(n, 256), 256, 0, s>>>(n, ldv, k, reinterpret_cast<const double2*>(V),
+                                            reinterpret_cast<const double2*>(h), reinterpret_cast<double2*>(w));
+  HIP_OK(hipGetLastError());
+}
+void combine(i64 n, i64 ldv, int k, const double* V, const double* y, double* x, hipStream_t s) {
+  if (k <= 0) return;
+  DM_REQUIRE(k <= 1024, DM_ERR_ARGUMENT, "Krylov basis larger than 1024 vectors");
+  k_combine<<<nblocks(n, 256), 256, 0, s>>>(n, ldv, k, reinterpret_cast<const double2*>(V),
+                                             reinterpret_cast<const double2*>(y), reinterpret_cast<double2*>(x));
+  HIP_OK(hipGetLastError());
+}
+void norm2(i64 n, const double* w, double* out, hipStream_t s) {
+  k_norm2<<<1, 1024, 0, s>>>(n, reinterpret_cast<const double2*>(w), out);
+  HIP_OK(hipGetLastError());
+}
+void axpby(i64 n, double ar, double ai, const double* x, double br, double bi, double* y, hipStream_t s) {
+  k_axpby<<<nblocks(n, 256), 256, 0, s>>>(n, make_double2(ar, ai), reinterpret_cast<const double2*>(x),
+                                           make_double2(br, bi), reinterpret_cast<double2*>(y));
+  HIP_OK(hipGetLastError());
+}
+void extract_blocks64(i64 n, i64 ld, int bs, int nb, const double* z, double* blocks, hipStream_t s) {
+  dim3 g(nblocks((i64)bs * bs, 256), nb);
+  k_extract_blocks<double2><<<g, 256, 0, s>>>(n, ld, bs, reinterpret_cast<const double2*>(z),
+                                               reinterpret_cast<double2*>(blocks));
+  HIP_OK(hipGetLastError());
+}
+void extract_blocks32(i64 n, i64 ld, int bs, int nb, const float* z, double* blocks, hipStream_t s) {
+  dim3 g(nblocks((i64)bs * bs, 256), nb);
+  k_extract_blocks<float2><<<g, 256, 0, s>>>(n, ld, bs, reinterpret_cast<const float2*>(z),
+                                              reinterpret_cast<double2*>(blocks));
+  HIP_OK(hipGetLastError());
+}
+void block_inverse(int bs, int nb, double* blocks, int* status, hipStream_t s) {
+  size_t sm = sizeof(double2) * 2 * bs + sizeof(int) * bs;
+  k_block_inverse<<<nb, 256, sm, s>>>(bs, reinterpret_cast<double2*>(blocks), status);
+  HIP_OK(hipGetLastError());
+}
+void block_apply(i64 n, int bs, int nb, const double* inv, const double* x, double* y, hipStream_t s) {
+  dim3 g(nblocks(bs, 256), nb);
+  k_block_apply<<<g, 256, sizeof(double2) * bs, s>>>(n, bs, reinterpret_cast<const double2*>(inv),
+                                                      reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y));
+  HIP_OK(hipGetLastError());
+}
+void diag_inv(i64 n, i64 ld, const double* z, double* d, hipStream_t s) {
+  k_diag_inv<<<nblocks(n, 256), 256, 0, s>>>(n, ld, reinterpret_cast<const double2*>(z), reinterpret_cast<double2*>(d));
+  HIP_OK(hipGetLastError());
+}
+void diag_apply(i64 n, const double* d, const double* x, double* y, hipStream_t s) {
+  k_diag_apply<<<nblocks(n, 256), 256, 0, s>>>(n, reinterpret_cast<const double2*>(d), reinterpret_cast<const double2*>(x),
+                                                reinterpret_cast<double2*>(y));
+  HIP_OK(hipGetLastError());
+}
+void lu_factor(i64 n, i64 ld, double* A, int* ipiv, int* status, hipStream_t s) {
+  for (i64 k = 0; k < n; ++k) {
+    k_lu_pivot<<<1, 1024, 0, s>>>(n, ld, (int)k, reinterpret_cast<double2*>(A), ipiv, status);
+    if (k + 1 < n) {
+      dim3 g(nblocks(n - k - 1, 256), (unsigned)(n - k - 1));
+      k_lu_update<<<g, 256, 0, s>>>(n, ld, (int)k, reinterpret_cast<double2*>(A));
+    }
+  }
+  HIP_OK(hipGetLastError());
+}
+void lu_solve(i64 n, i64 ld, const double* A, const int* ipiv, double* B, i64 ldb, int nrhs, hipStream_t s) {
+  k_lu_solve<<<nrhs, 1024, 0, s>>>(n, ld, reinterpret_cast<const double2*>(A), ipiv, reinterpret_cast<double2*>(B), ldb);
+  HIP_OK(hipGetLastError());
+}
+
+void unpermute64(i64 n, i64 ld, i64 c0, i64 nc, const int* iofe, const double* z, double* out, hipStream_t s) {
+  k_unpermute<double2><<<nblocks(n * nc, 256), 256, 0, s>>>(n, ld, c0, nc, iofe, reinterpret_cast<const double2*>(z),
+                                                             reinterpret_cast<double2*>(out));
+  HIP_OK(hipGetLastError());
+}
+void unpermute32(i64 n, i64 ld, i64 c0, i64 nc, const int* iofe, const float* z, double* out, hipStream_t s) {
+  k_unpermute<float2><<<nblocks(n * nc, 256), 256, 0, s>>>(n, ld, c0, nc, iofe, reinterpret_cast<const float2*>(z),
+                                                            reinterpret_cast<double2*>(out));
+  HIP_OK(hipGetLastError());
+}
+void entries64(i64 ld, i64 count, const int* iofe, const i64* m, const i64* n, const double* z, double* out, hipStream_t s) {
+  k_entries<double2><<<nblocks(count, 256), 256, 0, s>>>(ld, count, iofe, m, n, reinterpret_cast<const double2*>(z),
+                                                          reinterpret_cast<double2*>(out));
+  HIP_OK(hipGetLastError());
+}
+void entries32(i64 ld, i64 count, const int* iofe, const i64* m, const i64* n, const float* z, double* out, hipStream_t s) {
+  k_entries<float2><<<nblocks(count, 256), 256, 0, s>>>(ld, count, iofe, m, n, reinterpret_cast<const float2*>(z),
+                                                         reinterpret_cast<double2*>(out));
+  HIP_OK(hipGetLastError());
+}
+void finite64(i64 n, i64 ld, const double* z, int* flag, hipStream_t s) {
+  unsigned g = (unsigned)std::min<i64>(nblocks(n * n, 256), 4096);
+  k_finite<double2><<<g, 256, 0, s>>>(n, ld, reinterpret_cast<const double2*>(z), flag);
+  HIP_OK(hipGetLastError());
+}
+void finite32(i64 n, i64 ld, const float* z, int* flag, hipStream_t s) {
+  unsigned g = (unsigned)std::min<i64>(nblocks(n * n, 256), 4096);
+  k_finite<float2><<<g, 256, 0, s>>>(n, ld, reinterpret_cast<const float2*>(z), flag);
+  HIP_OK(hipGetLastError());
+}
+void conj(i64 n, double* v, hipStream_t s) {
+  k_conj<<<nblocks(n, 256), 256, 0, s>>>(n, reinterpret_cast<double2*>(v));
+  HIP_OK(hipGetLastError());
+}
+}  // namespace dml

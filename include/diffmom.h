@@ -1,0 +1,183 @@
+/*
+ * diffmom.h — C ABI of libdiffmom, the MI355X (gfx950) implementation of
+ * DiffMoM.jl's surface-EFIE path (RWG assembly, EFIE products, dense / GMRES
+ * solves, plane-wave excitation, radiation vectors).
+ *
+ * The reference exposes no FFI: its "operator API" is Julia's AbstractMatrix
+ * protocol plus the functions cited below.  Each entry point names the
+ * reference function it replaces (path:line in DiffMoM.jl).  A Julia host binds
+ * these with `ccall` (one `ccall` per function).
+ *
+ * Conventions (identical to Julia's in-memory layout, so Julia arrays pass
+ * through unchanged):
+ *   - indices are 1-based int64 (vertex ids in `tri`, triangle ids in RWG arrays)
+ *     except where a function says 0-based;
+ *   - matrices are column-major; `xyz` is 3 x Nv, `tri` 3 x Nt;
+ *   - complex numbers are interleaved (re, im) doubles = Julia ComplexF64
+ *     = C `double _Complex`;
+ *   - host buffers are only read or written during the call, never retained.
+ * Every function returns 0 on success or a DM_ERR_* code; `dm_last_error()`
+ * (thread-local) gives the message.  Codes map one-to-one onto the reference's
+ * exception classes so a Julia shim can rethrow the same type.
+ * Thread safety: calls on one context are serialised by a per-context lock;
+ * distinct contexts run concurrently.  Handles from one context must not be
+ * used with another.
+ */
+#ifndef DIFFMOM_H
+#define DIFFMOM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DM_OK 0
+#define DM_ERR_ARGUMENT 1    /* Julia ArgumentError */
+#define DM_ERR_OVERFLOW 2    /* Julia OverflowError */
+#define DM_ERR_DIMENSION 3   /* Julia DimensionMismatch */
+#define DM_ERR_CONVERGENCE 4 /* Julia ErrorException: iterative solve did not converge */
+#define DM_ERR_DEVICE 5      /* HIP runtime failure */
+#define DM_ERR_MEMORY 6      /* device allocation failed */
+#define DM_ERR_DOMAIN 7      /* Julia DomainError (e.g. zero-length RWG edge) */
+
+/* Which stored forms of Z an assembly produces (bit mask). */
+#define DM_Z64 1 /* complex128 from FP64 quadrature (the reference's arithmetic) */
+#define DM_Z32 2 /* complex64 from FP32 far-pair quadrature + FP64 self/adjacent terms */
+
+typedef struct dm_ctx dm_ctx;
+typedef struct dm_mesh dm_mesh;
+typedef struct dm_rwg dm_rwg;
+typedef struct dm_zmat dm_zmat;
+typedef struct dm_mfop dm_mfop;
+
+/* ------------------------------------------------------------------ context */
+int dm_create(int device, dm_ctx** out);
+int dm_destroy(dm_ctx* ctx);
+const char* dm_last_error(void);
+int dm_version(void);
+/* Device synchronisation (for host timing around asynchronous work). */
+int dm_sync(dm_ctx* ctx);
+
+/* --------------------------------------------------------------------- mesh */
+/* TriMesh(xyz, tri): src/Types.jl:123-129.  Copies the arrays. */
+int dm_mesh_create(dm_ctx* ctx, const double* xyz, int64_t nv, const int64_t* tri, int64_t nt,
+                   dm_mesh** out);
+int dm_mesh_free(dm_mesh* mesh);
+/* assert_mesh_quality: src/geometry/Mesh.jl:1093-1255 (DM_ERR_ARGUMENT with the
+ * report in the message when the mesh fails). */
+int dm_mesh_check(const dm_mesh* mesh, int allow_boundary, int require_closed,
+                  double area_tol_rel);
+/* triangle_area for all triangles: src/geometry/Mesh.jl:3205-3232 */
+int dm_mesh_areas(const dm_mesh* mesh, double* areas);
+
+/* ---------------------------------------------------------------------- RWG */
+/* build_rwg(mesh; precheck, allow_boundary, require_closed, area_tol_rel):
+ * src/basis/RWG.jl:102-143.  T+ is the lower-index triangle of each interior
+ * edge.  Basis order: interior edges sorted by their (v1 < v2) vertex key (the
+ * reference's order is Julia Dict iteration order, which no other language
+ * reproduces; callers holding a Julia RWGData use dm_rwg_create instead). */
+int dm_build_rwg(const dm_mesh* mesh, int precheck, int allow_boundary, int require_closed,
+                 double area_tol_rel, dm_rwg** out);
+/* RWGData as the caller holds it (src/Types.jl:138-152), any basis order.
+ * evert may be NULL; coeff_plus / coeff_minus may be NULL (unit coefficients). */
+int dm_rwg_create(const dm_mesh* mesh, int64_t n, const int64_t* tplus, const int64_t* tminus,
+                  const int64_t* evert, const int64_t* vplus_opp, const int64_t* vminus_opp,
+                  const double* len, const double* area_plus, const double* area_minus,
+                  const double* coeff_plus, const double* coeff_minus, dm_rwg** out);
+int dm_rwg_size(const dm_rwg* rwg, int64_t* n);
+/* Copies the basis arrays back (evert is 2 x n). */
+int dm_rwg_export(const dm_rwg* rwg, int64_t* tplus, int64_t* tminus, int64_t* evert,
+                  int64_t* vplus_opp, int64_t* vminus_opp, double* len, double* area_plus,
+                  double* area_minus);
+int dm_rwg_free(dm_rwg* rwg);
+/* Number of edge-adjacent triangle pairs (_build_triangle_adjacency,
+ * src/assembly/EFIE.jl:140-231). */
+int dm_rwg_adjacent_pairs(const dm_rwg* rwg, int64_t* count);
+
+/* ----------------------------------------------------------------- assembly */
+/* assemble_Z_efie(mesh, rwg, k; quad_order, eta0, max_output_bytes):
+ * src/assembly/EFIE.jl:699-753.  The matrix stays on the device.  `forms` is a
+ * DM_Z64 / DM_Z32 mask.  k may be complex (k_im != 0).  max_output_bytes bounds
+ * N*N*16 as the reference's byte gate does (<= 0: no gate). */
+int dm_assemble_z(dm_ctx* ctx, const dm_rwg* rwg, double k_re, double k_im, double eta0,
+                  int quad_order, int forms, int64_t max_output_bytes, dm_zmat** out);
+int dm_zmat_size(const dm_zmat* z, int64_t* n, int* forms);
+/* Copy Z (caller basis order, column-major N x N, complex128) to host.
+ * form = DM_Z64 or DM_Z32 (complex64 values widened). */
+int dm_zmat_get(const dm_zmat* z, int form, double* out);
+/* Entries Z[m_i, n_i] (0-based caller indices) of one stored form. */
+int dm_zmat_entries(const dm_zmat* z, int form, const int64_t* m, const int64_t* n,
+                    int64_t count, double* out);
+int dm_zmat_free(dm_zmat* z);
+
+/* Y = op(Z) X for nrhs columns (mul!(y, Z, x) / zgemv inside Krylov.gmres,
+ * src/solver/IterativeSolve.jl:364-384).  op = Z (adjoint = 0) or Z^H (1).
+ * Host buffers, caller basis order, complex128; form as in dm_zmat_get. */
+int dm_apply(dm_ctx* ctx, const dm_zmat* z, int form, int adjoint, const double* X, int64_t ldx,
+             int64_t nrhs, double* Y, int64_t ldy);
+
+/* ------------------------------------------------------- matrix-free operator */
+/* matrixfree_efie_operator / mul! / adjoint: src/assembly/EFIE.jl:759-1007.
+ * Each product recomputes every entry on the device (FP64), never storing Z. */
+int dm_mf_create(dm_ctx* ctx, const dm_rwg* rwg, double k_re, double k_im, double eta0,
+                 int quad_order, dm_mfop** out);
+int dm_mf_apply(dm_ctx* ctx, const dm_mfop* op, int adjoint, const double* x, double* y);
+int dm_mf_free(dm_mfop* op);
+
+/* --------------------------------------------------------------- excitation */
+/* assemble_plane_wave / assemble_multiple_excitations for plane waves:
+ * src/assembly/Excitation.jl:2738-2797,2866-2896.  kvec, pol: 3 x M, E0: M.
+ * Output V: N x M complex128 column-major (caller basis order).  Validates
+ * |pol| = 1 and pol . k = 0 as src/assembly/Excitation.jl:1549-1598 does. */
+int dm_excitation_planewave(dm_ctx* ctx, const dm_rwg* rwg, const double* kvec, const double* pol,
+                            const double* E0, int64_t M, int quad_order, double* V);
+
+/* ------------------------------------------------------------------- solves */
+#define DM_METHOD_LU 0        /* dense LU (solve_forward :direct, src/solver/Solve.jl:1124-1198) */
+#define DM_METHOD_GMRES 1     /* FP64 GMRES on Z64 (solve_gmres, src/solver/IterativeSolve.jl:341-392) */
+#define DM_METHOD_MIXED 2     /* FGMRES on Z32 with FP64 Arnoldi + FP64 iterative refinement on Z64 */
+
+#define DM_PRECOND_NONE 0
+#define DM_PRECOND_DIAG 1     /* Jacobi (factorization=:diag, src/solver/NearFieldPreconditioner.jl:623-668) */
+#define DM_PRECOND_NEARFIELD 2 /* near-field block LU (cutoff-defined spatial blocks) */
+#define DM_PRECOND_LEARNED 3  /* message-passing network, flexible (needs a trained model) */
+
+typedef struct dm_solve_opts {
+  int method;          /* DM_METHOD_* */
+  int precond;         /* DM_PRECOND_* */
+  int precond_side;    /* 0 = left, 1 = right (solve_gmres precond_side) */
+  int maxiter;         /* total Krylov iterations allowed */
+  double tol;          /* relative tolerance on the (true FP64) residual */
+  double nf_cutoff;    /* near-field cutoff in metres (<= 0: automatic block size) */
+  int restart;         /* Krylov basis size before restart (0 = no restart) */
+  int check_convergence; /* 1: DM_ERR_CONVERGENCE when not converged */
+  double inner_tol;    /* MIXED: relative tolerance of the first inner solve (<= 0: tol/2) */
+  int max_outer;       /* MIXED: refinement steps allowed */
+  int block_size;      /* DM_PRECOND_NEARFIELD: bases per spatial block (0: 128) */
+} dm_solve_opts;
+
+typedef struct dm_solve_stats {
+  int iterations;        /* Krylov iterations (sum over refinement steps) */
+  int outer_iterations;  /* refinement steps (MIXED) */
+  int solved;            /* 1 if the tolerance was met */
+  double residual;       /* true FP64 relative residual ||b - Z64 x|| / ||b|| */
+  double precond_time_s;
+  double solve_time_s;
+} dm_solve_stats;
+
+void dm_solve_opts_default(dm_solve_opts* opts);
+/* Solve Z X = B for nrhs right-hand sides (host buffers, caller order). */
+int dm_solve(dm_ctx* ctx, const dm_zmat* z, const double* B, int64_t ldb, int64_t nrhs,
+             const dm_solve_opts* opts, double* X, int64_t ldx, dm_solve_stats* stats);
+
+/* ---------------------------------------------------------------- far field */
+/* radiation_vectors(mesh, rwg, grid, k; quad_order, eta0):
+ * src/postprocessing/FarField.jl:425-574.  rhat: 3 x NΩ; G: (3 NΩ) x N. */
+int dm_radiation_vectors(dm_ctx* ctx, const dm_rwg* rwg, const double* rhat, int64_t nomega,
+                         double k, double eta0, int quad_order, double* G);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DIFFMOM_H */

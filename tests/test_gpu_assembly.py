@@ -1,0 +1,172 @@
+"""GPU parity of assembly, products, excitation and radiation vectors against the
+CPU oracle (FP64 bar: relative Frobenius <= 1e-10; complex64 path: <= 1e-5)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, oracle_rwg, rel_fro
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(ROOT, "tests", "golden")
+K0 = 2 * math.pi / 0.1
+
+
+def _oracle_Z(O, mesh, rwg, k, quad_order=3):
+    return O.assemble_Z_efie(oracle_rwg(O, mesh, rwg), k, quad_order=quad_order)
+
+
+def test_c0_plate_z64_matches_fixture_and_oracle(dm, O):
+    npz = np.load(os.path.join(GOLD, "c0_plate.npz"))
+    m = dm.make_rect_plate(0.1, 0.1, 6, 6)
+    r = dm.build_rwg(m)
+    assert r.nedges == 96
+    assert np.array_equal(r.tplus, npz["tplus"]) and np.array_equal(r.vminus_opp, npz["vminus_opp"])
+    Z = dm.assemble_Z_efie(m, r, K0)
+    assert rel_fro(Z, npz["Z"]) <= 1e-10
+    assert rel_fro(Z, _oracle_Z(O, m, r, K0)) <= 1e-10
+    assert np.array_equal(Z, Z.T)
+
+
+def test_c0_plate_z32_within_bar(dm, O):
+    m = dm.make_rect_plate(0.1, 0.1, 6, 6)
+    r = dm.build_rwg(m)
+    Zd = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64 | dm.DM_Z32, device=True)
+    Zo = _oracle_Z(O, m, r, K0)
+    e32 = rel_fro(Zd.to_numpy(dm.DM_Z32), Zo)
+    e64 = rel_fro(Zd.to_numpy(dm.DM_Z64), Zo)
+    print(f"rel Frobenius: fp64 {e64:.2e}  fp32 {e32:.2e}")
+    assert e64 <= 1e-10 and e32 <= 1e-5
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_caller_basis_order(dm, O, seed):
+    """A caller's RWGData in arbitrary order (Julia Dict order) is honoured,
+    including the order-dependent adjacent-cell outer/inner choice."""
+    m = dm.make_rect_plate(0.12, 0.1, 7, 5)
+    base = dm.build_rwg(m)
+    perm = np.random.default_rng(seed).permutation(base.nedges)
+    r = dm.rwg_from_arrays(m, base.tplus[perm], base.tminus[perm], base.vplus_opp[perm], base.vminus_opp[perm],
+                           base.len[perm], base.area_plus[perm], base.area_minus[perm])
+    Z = dm.assemble_Z_efie(m, r, K0)
+    assert rel_fro(Z, _oracle_Z(O, m, r, K0)) <= 1e-10
+
+
+@pytest.mark.parametrize("quad_order", [1, 4, 7])
+def test_quad_orders(dm, O, quad_order):
+    m = dm.make_rect_plate(0.1, 0.1, 5, 5)
+    r = dm.build_rwg(m)
+    Z = dm.assemble_Z_efie(m, r, K0, quad_order=quad_order)
+    assert rel_fro(Z, _oracle_Z(O, m, r, K0, quad_order)) <= 1e-10
+
+
+def test_complex_wavenumber(dm, O):
+    m = dm.make_rect_plate(0.1, 0.1, 5, 4)
+    r = dm.build_rwg(m)
+    k = K0 * (1 - 0.05j)
+    Z = dm.assemble_Z_efie(m, r, k)
+    assert rel_fro(Z, _oracle_Z(O, m, r, k)) <= 1e-10
+
+
+@pytest.mark.parametrize("subdiv", [2, 3])
+def test_sphere_full_matrix(dm, O, subdiv):
+    m = dm.make_icosphere(0.1, subdiv)
+    r = dm.build_rwg(m)
+    Zd = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64 | dm.DM_Z32, device=True)
+    Zo = _oracle_Z(O, m, r, K0)
+    e64 = rel_fro(Zd.to_numpy(dm.DM_Z64), Zo)
+    e32 = rel_fro(Zd.to_numpy(dm.DM_Z32), Zo)
+    print(f"sphere s={subdiv} N={r.nedges}: fp64 {e64:.2e} fp32 {e32:.2e}")
+    assert e64 <= 1e-10 and e32 <= 1e-5
+
+
+def test_plate_30x30_full_matrix(dm, O):
+    m = dm.make_rect_plate(0.25, 0.25, 30, 30)
+    r = dm.build_rwg(m)
+    Zd = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64 | dm.DM_Z32, device=True)
+    Zo = _oracle_Z(O, m, r, K0)
+    assert rel_fro(Zd.to_numpy(dm.DM_Z64), Zo) <= 1e-10
+    assert rel_fro(Zd.to_numpy(dm.DM_Z32), Zo) <= 1e-5
+
+
+def test_c1_plate_sampled_entries(dm, O):
+    """configs[1] plate (0.5 m, 60x60, N=10680): 2000 sampled entries including
+    the diagonal and near entries, FP64 and complex64."""
+    m = dm.make_rect_plate(0.5, 0.5, 60, 60)
+    r = dm.build_rwg(m)
+    assert r.nedges == 10680
+    Zd = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64 | dm.DM_Z32, device=True)
+    rng = np.random.default_rng(11)
+    M = rng.integers(0, r.nedges, 2000)
+    Nn = np.concatenate([rng.integers(0, r.nedges, 1000), M[1000:1500], np.clip(M[1500:] + rng.integers(-3, 4, 500), 0, r.nedges - 1)])
+    zo = O.EFIE(oracle_rwg(O, m, r), K0).entries(M, Nn)
+    z64 = Zd.entries(M, Nn, dm.DM_Z64)
+    z32 = Zd.entries(M, Nn, dm.DM_Z32)
+    assert np.linalg.norm(z64 - zo) / np.linalg.norm(zo) <= 1e-10
+    assert np.max(np.abs(z64 - zo) / np.abs(zo)) <= 1e-9
+    assert np.linalg.norm(z32 - zo) / np.linalg.norm(zo) <= 1e-5
+
+
+def test_products_and_matrix_free(dm, O):
+    m = dm.make_rect_plate(0.12, 0.12, 10, 9)
+    r = dm.build_rwg(m)
+    Zd = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64 | dm.DM_Z32, device=True)
+    Z = Zd.to_numpy()
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((r.nedges, 3)) + 1j * rng.standard_normal((r.nedges, 3))
+    Y = Zd.matmul(X)
+    assert np.linalg.norm(Y - Z @ X) / np.linalg.norm(Z @ X) < 1e-12
+    Ya = Zd.matmul(X, adjoint=True)
+    assert np.linalg.norm(Ya - Z.conj().T @ X) / np.linalg.norm(Ya) < 1e-12
+    Y32 = Zd.matmul(X, form=dm.DM_Z32)
+    assert np.linalg.norm(Y32 - Z @ X) / np.linalg.norm(Z @ X) < 1e-5
+    A = dm.matrixfree_efie_operator(m, r, K0)
+    x = X[:, 0]
+    y = A @ x
+    assert np.linalg.norm(y - Z @ x) / np.linalg.norm(y) < 1e-10  # test/runtests.jl:8748
+    ya = A.H @ x
+    assert np.linalg.norm(ya - Z.conj().T @ x) / np.linalg.norm(ya) < 1e-10
+
+
+def test_excitation_parity(dm, O):
+    for mk in (lambda: dm.make_rect_plate(0.1, 0.1, 6, 6), lambda: dm.make_icosphere(0.1, 2)):
+        m = mk()
+        r = dm.build_rwg(m)
+        orr = oracle_rwg(O, m, r)
+        th = math.radians(30)
+        kv = K0 * np.array([math.sin(th), 0, math.cos(th)])
+        pol = np.array([math.cos(th), 0, -math.sin(th)])
+        v = dm.assemble_v_plane_wave(m, r, kv, 1.5, pol)
+        vo = O.assemble_v_plane_wave(orr, kv, 1.5, pol)
+        assert np.linalg.norm(v - vo) / np.linalg.norm(vo) < 1e-13
+    with pytest.raises(ValueError):
+        dm.assemble_v_plane_wave(m, r, kv, 1.0, [1.0, 0, 0.1])  # not unit / not transverse
+
+
+def test_radiation_vectors_parity(dm, O):
+    m = dm.make_icosphere(0.05, 2)
+    r = dm.build_rwg(m)
+    grid = dm.make_sph_grid(9, 8)
+    G = dm.radiation_vectors(m, r, grid, K0)
+    Go = O.radiation_vectors(oracle_rwg(O, m, r), grid.rhat, K0)
+    assert np.linalg.norm(G - Go) / np.linalg.norm(Go) < 1e-13
+
+
+def test_error_paths(dm):
+    m = dm.make_rect_plate(0.1, 0.1, 3, 3)
+    r = dm.build_rwg(m)
+    with pytest.raises(ValueError):
+        dm.assemble_Z_efie(m, r, float("nan"))
+    with pytest.raises(ValueError):
+        dm.assemble_Z_efie(m, r, K0, eta0=0.0)
+    with pytest.raises(ValueError):
+        dm.assemble_Z_efie(m, r, K0, max_output_bytes=r.nedges ** 2 * 16 - 1)
+    with pytest.raises(ValueError):
+        dm.assemble_Z_efie(m, r, K0, quad_order=5)
+    other = dm.make_rect_plate(0.1, 0.1, 3, 3)
+    with pytest.raises(ValueError):
+        dm.assemble_Z_efie(other, r, K0)  # mesh/RWG pair mismatch (src/Types.jl:154-159)
+    bad = dm.TriMesh(np.array([[0, 0, 0], [1, 0, 0], [2, 0, 0.0]]), np.array([[1, 2, 3]]))
+    with pytest.raises(ValueError):
+        dm.build_rwg(bad)  # degenerate triangle fails the precheck
