@@ -44,7 +44,7 @@ DM_OK, DM_ERR_ARGUMENT, DM_ERR_OVERFLOW, DM_ERR_DIMENSION = 0, 1, 2, 3
 DM_ERR_CONVERGENCE, DM_ERR_DEVICE, DM_ERR_MEMORY, DM_ERR_DOMAIN = 4, 5, 6, 7
 DM_Z64, DM_Z32 = 1, 2
 DM_METHOD_LU, DM_METHOD_GMRES, DM_METHOD_MIXED = 0, 1, 2
-DM_PRECOND_NONE, DM_PRECOND_DIAG, DM_PRECOND_NEARFIELD, DM_PRECOND_LEARNED = 0, 1, 2, 3
+DM_PRECOND_NONE, DM_PRECOND_DIAG, DM_PRECOND_NEARFIELD, DM_PRECOND_LEARNED, DM_PRECOND_BLOCK = 0, 1, 2, 3, 4
 
 ETA0 = 376.730313668
 C0_DEFAULT = 299792458.0
@@ -120,6 +120,12 @@ def lib():
         "dm_solve_opts_default": ([C.POINTER(dm_solve_opts)], None),
         "dm_solve": ([vp, vp, vp, i64, i64, C.POINTER(dm_solve_opts), vp, i64, C.POINTER(dm_solve_stats)], i32),
         "dm_radiation_vectors": ([vp, vp, vp, i64, d, d, i32, vp], i32),
+        "dm_solve_scattering_pw": ([vp, vp, d, d, d, i32, vp, vp, vp, i64, C.POINTER(dm_solve_opts), vp,
+                                    C.POINTER(dm_solve_stats), C.POINTER(d), pp], i32),
+        "dm_precond_create": ([vp, vp, i32, d, i32, i32, pp], i32), "dm_precond_apply": ([vp, vp, vp, vp], i32),
+        "dm_precond_free": ([vp], i32),
+        "dm_profile_enable": ([vp, i32], i32), "dm_profile_reset": ([vp], i32),
+        "dm_profile_read": ([vp, C.c_char_p, C.POINTER(i64), C.POINTER(d)], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -150,6 +156,19 @@ class Context:
 
     def sync(self):
         _check(lib().dm_sync(self.h))
+
+    def profile(self, on=True):
+        """Bracket tracked kernel launches with HIP events on the library's stream."""
+        _check(lib().dm_profile_enable(self.h, int(on)))
+
+    def profile_reset(self):
+        _check(lib().dm_profile_reset(self.h))
+
+    def profile_read(self, name):
+        """(launch count, total milliseconds) of one tracked kernel name."""
+        c, t = C.c_int64(), C.c_double()
+        _check(lib().dm_profile_read(self.h, name.encode(), C.byref(c), C.byref(t)))
+        return c.value, t.value
 
     def close(self):
         if getattr(self, "h", None):
@@ -517,19 +536,51 @@ def assemble_excitation(mesh, rwg, excitation, quad_order=3):
 @dataclass
 class NearFieldPreconditionerData:
     """Near-field preconditioner spec (src/solver/NearFieldPreconditioner.jl:84-110).
-    On the GPU the near field is taken as dense blocks of spatially consecutive
-    bases (block_size each) factorised by batched inversion."""
+
+    factorization="lu": the reference factors the near-field matrix (entries with
+    RWG-centre distance <= cutoff) with a sparse LU.  On the GPU the same
+    near-field coupling is inverted approximately: a sparse approximate inverse
+    M minimising ||Z M - I||_F over the cutoff pattern (at most 80 nearest bases
+    per column), applied as one SpMV per iteration.
+    factorization="diag": Jacobi (src/solver/NearFieldPreconditioner.jl:623-668).
+    factorization="block": block Jacobi over block_size spatially consecutive bases."""
     cutoff: float
     factorization: str = "lu"
     block_size: int = 128
 
 
 def build_nearfield_preconditioner(Z, mesh, rwg, cutoff, factorization="lu", block_size=128):
-    if factorization not in ("lu", "diag"):
-        raise ValueError(f"unsupported factorization {factorization!r} on the GPU path (use 'lu' or 'diag')")
+    if factorization not in ("lu", "diag", "block"):
+        raise ValueError(f"unsupported factorization {factorization!r} on the GPU path (use 'lu', 'diag' or 'block')")
     if not (cutoff >= 0):
         raise ValueError("cutoff must be nonnegative")
     return NearFieldPreconditionerData(float(cutoff), factorization, int(block_size))
+
+
+class NearFieldOperator:
+    """The preconditioner as an operator, y = M x (NearFieldOperator's mul!,
+    src/solver/NearFieldPreconditioner.jl:1034-1060), built on the GPU from the
+    device matrix Z (form DM_Z64 or DM_Z32)."""
+
+    def __init__(self, P, Z, form=None):
+        kind = {"diag": DM_PRECOND_DIAG, "block": DM_PRECOND_BLOCK}.get(P.factorization, DM_PRECOND_NEARFIELD)
+        h = C.c_void_p()
+        _check(lib().dm_precond_create(Z.ctx.h, Z.h, kind, float(P.cutoff), int(P.block_size), Z._form(form),
+                                       C.byref(h)))
+        self.h, self.Z, self.N = h, Z, Z.N
+
+    def __matmul__(self, x):
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        if x.shape != (self.N,):
+            raise DimensionMismatch(f"x length {x.shape} != {self.N}")
+        y = np.zeros(self.N, np.complex128)
+        _check(lib().dm_precond_apply(self.Z.ctx.h, self.h, _ptr(x), _ptr(y)))
+        return y
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().dm_precond_free(self.h)
+            self.h = None
 
 
 @dataclass
@@ -566,7 +617,8 @@ def _solve(Z, B, method, precond=None, side="right", tol=1e-6, maxiter=300, rest
     if precond is None:
         o.precond = DM_PRECOND_NONE
     elif isinstance(precond, NearFieldPreconditionerData):
-        o.precond = DM_PRECOND_DIAG if precond.factorization == "diag" else DM_PRECOND_NEARFIELD
+        o.precond = {"diag": DM_PRECOND_DIAG, "block": DM_PRECOND_BLOCK}.get(precond.factorization,
+                                                                             DM_PRECOND_NEARFIELD)
         o.block_size = precond.block_size
         o.nf_cutoff = precond.cutoff
     elif precond == "diag":
@@ -579,6 +631,58 @@ def _solve(Z, B, method, precond=None, side="right", tol=1e-6, maxiter=300, rest
     stats = SolveStats(st.iterations, bool(st.solved), st.residual, st.outer_iterations, st.precond_time_s,
                        st.solve_time_s)
     return (X[:, 0] if vec else X), stats
+
+
+def _opts(method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, check=True, inner_tol=0.0,
+          max_outer=10):
+    o = dm_solve_opts()
+    lib().dm_solve_opts_default(C.byref(o))
+    o.method, o.tol, o.maxiter, o.restart = method, float(tol), int(maxiter), int(restart)
+    o.check_convergence, o.inner_tol, o.max_outer = int(check), float(inner_tol), int(max_outer)
+    o.precond_side = 1 if side == "right" else 0
+    if precond is None or precond == "none":
+        o.precond = DM_PRECOND_NONE
+    elif isinstance(precond, NearFieldPreconditionerData):
+        o.precond = {"diag": DM_PRECOND_DIAG, "block": DM_PRECOND_BLOCK}.get(precond.factorization,
+                                                                             DM_PRECOND_NEARFIELD)
+        o.block_size, o.nf_cutoff = precond.block_size, precond.cutoff
+    elif precond == "diag":
+        o.precond = DM_PRECOND_DIAG
+    else:
+        raise ValueError(f"unknown preconditioner {precond!r}")
+    return o
+
+
+def solve_planewaves(mesh, rwg, k, excitations, method="mixed", preconditioner=None, tol=1e-6, maxiter=300,
+                     quad_order=3, eta0=ETA0, check_convergence=True, keep_matrix=False):
+    """End-to-end device solve for plane waves (the dense branches of
+    solve_scattering, src/Workflow.jl:67-317): assembly, excitation,
+    preconditioner and Krylov solve in one call.  method: "direct" (dense LU),
+    "fp64" (FP64 GMRES on the complex128 matrix) or "mixed" (FGMRES on the
+    complex64 matrix with FP64 Arnoldi and FP64 refinement; the complex128
+    matrix is assembled on a second stream while the solve runs).
+    Returns (X, stats, assembly_time_s[, ZMatrix])."""
+    _check_pair(mesh, rwg)
+    M = len(excitations)
+    kv = np.ascontiguousarray(np.array([e.k_vec for e in excitations], float).reshape(M, 3))
+    pol = np.ascontiguousarray(np.array([e.pol for e in excitations], float).reshape(M, 3))
+    E0 = np.ascontiguousarray([float(e.E0) for e in excitations])
+    meth = {"direct": DM_METHOD_LU, "fp64": DM_METHOD_GMRES, "mixed": DM_METHOD_MIXED}[method]
+    o = _opts(meth, preconditioner, "right", tol, maxiter, check=check_convergence)
+    X = np.zeros((rwg.nedges, M), dtype=np.complex128, order="F")
+    st = dm_solve_stats()
+    t_asm = C.c_double()
+    kr, ki = _k_parts(k)
+    zh = C.c_void_p()
+    _check(lib().dm_solve_scattering_pw(mesh.ctx.h, rwg.h, kr, ki, float(eta0), int(quad_order), _ptr(kv), _ptr(pol),
+                                        _ptr(E0), M, C.byref(o), X.ctypes.data_as(C.c_void_p), C.byref(st),
+                                        C.byref(t_asm), C.byref(zh) if keep_matrix else None))
+    stats = SolveStats(st.iterations, bool(st.solved), st.residual, st.outer_iterations, st.precond_time_s,
+                       st.solve_time_s)
+    if keep_matrix:
+        forms = DM_Z64 | DM_Z32 if method == "mixed" else DM_Z64
+        return X, stats, t_asm.value, ZMatrix(rwg, zh, forms, complex(k), float(eta0), quad_order)
+    return X, stats, t_asm.value
 
 
 def solve_forward(Z, v, solver="direct", **kw):
@@ -739,31 +843,38 @@ def solve_scattering(mesh, freq_hz, excitation, method="auto", dense_direct_limi
             raise DimensionMismatch(f"solve_scattering: excitation length {v.shape} != N={N}")
     if not np.all(np.isfinite(v)):
         raise ValueError("solve_scattering: excitation vector must contain only finite values")
-    forms = DM_Z64 | DM_Z32 if (precision == "mixed" and sel == "dense_gmres") else DM_Z64
-    t0 = time.perf_counter()
-    Z = assemble_Z_efie(mesh, rwg, k, quad_order=quad_order, mesh_precheck=False, max_output_bytes=None,
-                        forms=forms, device=True)
-    mesh.ctx.sync()
-    t_asm = time.perf_counter() - t0
-    iters, resid, t_pre = -1, float("nan"), 0.0
-    if sel == "dense_direct":
-        t1 = time.perf_counter()
-        I = solve_forward(Z, v, solver="direct")
-        t_solve = time.perf_counter() - t1
-    else:
-        P = None
+    P = None
+    if sel == "dense_gmres":
         if preconditioner in ("auto", "lu"):
-            P = build_nearfield_preconditioner(Z, mesh, rwg, nf_cutoff_lambda * lam, "lu", nf_block_size)
+            P = build_nearfield_preconditioner(None, mesh, rwg, 0.0 if nf_cutoff_lambda >= 1.0 else nf_cutoff_lambda * lam)
         elif preconditioner == "diag":
-            P = build_nearfield_preconditioner(Z, mesh, rwg, nf_cutoff_lambda * lam, "diag")
-        t1 = time.perf_counter()
-        if precision == "mixed":
+            P = build_nearfield_preconditioner(None, mesh, rwg, nf_cutoff_lambda * lam, "diag")
+    method_gpu = "direct" if sel == "dense_direct" else ("mixed" if precision == "mixed" else "fp64")
+    t0 = time.perf_counter()
+    if isinstance(excitation, PlaneWaveExcitation):
+        X, st, t_asm = solve_planewaves(mesh, rwg, k, [excitation], method_gpu, P, tol=gmres_tol,
+                                        maxiter=gmres_maxiter, quad_order=quad_order,
+                                        check_convergence=check_gmres_convergence)
+        I = X[:, 0]
+    else:
+        forms = DM_Z64 | DM_Z32 if method_gpu == "mixed" else DM_Z64
+        Z = assemble_Z_efie(mesh, rwg, k, quad_order=quad_order, mesh_precheck=False, max_output_bytes=None,
+                            forms=forms, device=True)
+        mesh.ctx.sync()
+        t_asm = time.perf_counter() - t0
+        if method_gpu == "direct":
+            I = solve_forward(Z, v, solver="direct")
+            st = SolveStats(-1, True, float("nan"))
+        elif method_gpu == "mixed":
             I, st = solve_mixed(Z, v, P, tol=gmres_tol, maxiter=gmres_maxiter,
                                 check_convergence=check_gmres_convergence)
         else:
             I, st = solve_gmres(Z, v, P, precond_side="right", tol=gmres_tol, maxiter=gmres_maxiter,
                                 check_gmres_convergence=check_gmres_convergence)
-        t_solve = time.perf_counter() - t1
-        iters, resid, t_pre = st.niter, st.residual, st.precond_time_s
-    Z.free()
+        Z.free()
+    t_total = time.perf_counter() - t0
+    iters = st.niter if sel == "dense_gmres" else -1
+    resid = st.residual if sel == "dense_gmres" else float("nan")
+    t_pre = st.precond_time_s
+    t_solve = t_total - t_asm - t_pre
     return ScatteringResult(I, sel, N, t_asm, t_solve, t_pre, iters, resid, report, warnings)

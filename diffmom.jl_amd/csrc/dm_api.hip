@@ -78,6 +78,7 @@ AsmArgs make_args(dm_rwg& r, const TriTables& T, double kr, double ki, double et
   a.nb = r.d_nb.p;
   a.ext = r.d_ext.p;
   a.len = r.d_len.p;
+  a.tplus_i = r.d_tplus_i.p;
   a.cen = T.cen.p;
   a.uq = T.uq.p;
   a.av = T.av.p;
@@ -117,12 +118,18 @@ struct DevOp {
 struct OpZ64 : DevOp {
   const dm_zmat* z;
   explicit OpZ64(const dm_zmat* z_) : z(z_) {}
-  void apply(const double* x, double* y, hipStream_t s) override { launch_gemv_z64(z->n, z->ld, z->z64.p, x, y, s); }
+  void apply(const double* x, double* y, hipStream_t s) override {
+    ProfScope p(z->ctx, "gemv_z64", s);
+    launch_gemv_z64(z->n, z->ld, z->z64.p, x, y, s);
+  }
 };
 struct OpZ32 : DevOp {
   const dm_zmat* z;
   explicit OpZ32(const dm_zmat* z_) : z(z_) {}
-  void apply(const double* x, double* y, hipStream_t s) override { launch_gemv_z32(z->n, z->ld, z->z32.p, x, y, s); }
+  void apply(const double* x, double* y, hipStream_t s) override {
+    ProfScope p(z->ctx, "gemv_z32", s);
+    launch_gemv_z32(z->n, z->ld, z->z32.p, x, y, s);
+  }
 };
 
 struct DevPrec {
@@ -133,6 +140,14 @@ struct PrecDiag : DevPrec {
   i64 n;
   DBuf<double> d;
   void apply(const double* x, double* y, hipStream_t s) override { dml::diag_apply(n, d.p, x, y, s); }
+};
+struct PrecSpai : DevPrec {
+  i64 n;
+  const SpaiPattern* pat;
+  DBuf<double> val;
+  void apply(const double* x, double* y, hipStream_t s) override {
+    dmp::spmv(n, pat->d_rowptr.p, pat->d_colidx.p, val.p, x, y, s);
+  }
 };
 struct PrecBlock : DevPrec {
   i64 n;
@@ -180,6 +195,29 @@ std::unique_ptr<DevPrec> build_precond(const dm_zmat* z, bool use32, const dm_so
     return p;
   }
   if (o.precond == DM_PRECOND_NEARFIELD) {
+    dm_rwg& r = const_cast<dm_rwg&>(*z->rwg);
+    double cutJ = o.nf_cutoff > 0 ? o.nf_cutoff : 2.5 * r.mean_len;
+    const SpaiPattern& P = dmh::spai_pattern(r, cutJ, dmp::spai_max_cols());
+    auto p = std::make_unique<PrecSpai>();
+    p->n = n;
+    p->pat = &P;
+    DBuf<double> cscv;
+    cscv.alloc(2 * P.nnz);
+    p->val.alloc(2 * P.nnz);
+    DBuf<int> st;
+    st.alloc(1);
+    HIP_OK(hipMemsetAsync(st.p, 0, sizeof(int), s));
+    dmp::spai_build(n, z->ld, use32 ? nullptr : z->z64.p, use32 ? z->z32.p : nullptr, P.d_jptr.p, P.d_jidx.p,
+                    P.d_iptr.p, P.d_iidx.p, cscv.p, st.p, s);
+    dmp::csr_from_csc(P.nnz, P.d_map.p, cscv.p, p->val.p, s);
+    int sv = 0;
+    HIP_OK(hipMemcpyAsync(&sv, st.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    DM_REQUIRE(sv == 0, DM_ERR_ARGUMENT,
+               "near-field preconditioner: normal equations of column " + std::to_string(sv) + " are not positive definite");
+    return p;
+  }
+  if (o.precond == DM_PRECOND_BLOCK) {
     auto p = std::make_unique<PrecBlock>();
     p->n = n;
     p->bs = o.block_size > 0 ? o.block_size : 128;
@@ -204,6 +242,16 @@ std::unique_ptr<DevPrec> build_precond(const dm_zmat* z, bool use32, const dm_so
   }
   dm_throw(DM_ERR_ARGUMENT, "preconditioner kind not available for this solve");
 }
+
+}  // namespace
+
+struct dm_precond {
+  dm_ctx* ctx = nullptr;
+  const dm_zmat* z = nullptr;
+  std::unique_ptr<DevPrec> p;
+};
+
+namespace {
 
 // ------------------------------------------------------------------- GMRES
 struct Krylov {
@@ -233,8 +281,8 @@ struct GmresOut {
 // flexible (FGMRES: the preconditioned directions are stored), left
 // preconditioning follows Krylov.jl's semantics (stop on ||M r|| <= rtol ||M b||).
 // x holds the initial guess on entry.
-GmresOut gmres(i64 n, DevOp& A, DevPrec* M, int side, const double* b, double* x, double rtol, int maxiter,
-               int restart, Krylov& K, hipStream_t s) {
+GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double* b, double* x, double rtol,
+               int maxiter, int restart, Krylov& K, hipStream_t s) {
   GmresOut out;
   int mcap = restart > 0 ? std::min(restart, maxiter) : maxiter;
   bool right = M && side == 1;
@@ -282,10 +330,14 @@ GmresOut gmres(i64 n, DevOp& A, DevPrec* M, int side, const double* b, double* x
       double* vj = col(V, j);
       if (right) {
         double* zj = col(K.Zs.p, j);
-        M->apply(vj, zj, s);
+        {
+          ProfScope p(ctx, "precond", s);
+          M->apply(vj, zj, s);
+        }
         A.apply(zj, K.w.p, s);
       } else if (left) {
         A.apply(vj, K.t.p, s);
+        ProfScope p(ctx, "precond", s);
         M->apply(K.t.p, K.w.p, s);
       } else {
         A.apply(vj, K.w.p, s);
@@ -293,11 +345,14 @@ GmresOut gmres(i64 n, DevOp& A, DevPrec* M, int side, const double* b, double* x
       int k = j + 1;
       double* h1 = K.hbuf.p;
       double* h2 = K.hbuf.p + 2 * k;
-      dml::dots(n, ldv / 2, k, V, K.w.p, h1, s);
-      dml::update(n, ldv / 2, k, V, h1, K.w.p, s);
-      dml::dots(n, ldv / 2, k, V, K.w.p, h2, s);
-      dml::update(n, ldv / 2, k, V, h2, K.w.p, s);
-      dml::norm2(n, K.w.p, K.hbuf.p + 4 * k, s);
+      {
+        ProfScope p(ctx, "arnoldi", s);
+        dml::dots(n, ldv / 2, k, V, K.w.p, h1, s);
+        dml::update(n, ldv / 2, k, V, h1, K.w.p, s);
+        dml::dots(n, ldv / 2, k, V, K.w.p, h2, s);
+        dml::update(n, ldv / 2, k, V, h2, K.w.p, s);
+        dml::norm2(n, K.w.p, K.hbuf.p + 4 * k, s);
+      }
       HIP_OK(hipMemcpyAsync(hh.data(), K.hbuf.p, sizeof(double) * (4 * k + 1), hipMemcpyDeviceToHost, s));
       HIP_OK(hipStreamSynchronize(s));
       cd* Hj = &H[(size_t)j * (mcap + 1)];
@@ -373,6 +428,68 @@ double true_residual(DevOp& A, i64 n, const double* b, const double* x, double* 
   return std::sqrt(h[0] / h[1]);
 }
 
+// One right-hand side on the device (internal basis order): FP64 GMRES on the
+// matrix `A`, or (mixed) FGMRES on the complex64 matrix with FP64 Arnoldi and
+// FP64 iterative refinement r = b - Z64 x.  `z64_ready`, when given, is waited
+// on before the first complex128 product (the FP64 matrix may still be in
+// assembly on the second stream).  Returns the true relative residual.
+double solve_one(dm_ctx* ctx, const dm_solve_opts& o, DevOp* A64, DevOp* A32, DevPrec* M, const double* b,
+                 double* x, i64 n, Krylov& K, dm_solve_stats& st, bool& ok, hipEvent_t z64_ready) {
+  hipStream_t s = ctx->stream;
+  DBuf<double> tmp, hb;
+  tmp.alloc(2 * n);
+  hb.alloc(4);
+  HIP_OK(hipMemsetAsync(x, 0, 16 * n, s));
+  if (o.method == DM_METHOD_GMRES) {
+    DevOp& A = A64 ? *A64 : *A32;
+    if (A64 && z64_ready) HIP_OK(hipStreamWaitEvent(s, z64_ready, 0));
+    GmresOut g = gmres(ctx, n, A, M, o.precond_side, b, x, o.tol, o.maxiter, o.restart, K, s);
+    st.iterations += g.iters;
+    ok = g.solved;
+    return true_residual(A, n, b, x, tmp.p, hb.p, s);
+  }
+  DBuf<double> rr, d;
+  rr.alloc(2 * n);
+  d.alloc(2 * n);
+  HIP_OK(hipMemcpyAsync(rr.p, b, 16 * n, hipMemcpyDeviceToDevice, s));
+  double rel = 1.0;
+  double tau = o.inner_tol > 0 ? o.inner_tol : 0.5 * o.tol;
+  int left = o.maxiter;
+  bool waited = false;
+  for (int outer = 0; outer < std::max(1, o.max_outer) && left > 0; ++outer) {
+    HIP_OK(hipMemsetAsync(d.p, 0, 16 * n, s));
+    GmresOut g = gmres(ctx, n, *A32, M, 1, rr.p, d.p, tau, left, o.restart, K, s);
+    st.iterations += g.iters;
+    left -= g.iters;
+    st.outer_iterations += 1;
+    dml::axpby(n, 1, 0, d.p, 1, 0, x, s);
+    if (z64_ready && !waited) {
+      HIP_OK(hipStreamWaitEvent(s, z64_ready, 0));
+      waited = true;
+    }
+    A64->apply(x, tmp.p, s);  // rr = b - Z64 x
+    HIP_OK(hipMemcpyAsync(rr.p, b, 16 * n, hipMemcpyDeviceToDevice, s));
+    dml::axpby(n, -1, 0, tmp.p, 1, 0, rr.p, s);
+    double h[2];
+    dml::norm2(n, rr.p, hb.p, s);
+    dml::norm2(n, b, hb.p + 1, s);
+    HIP_OK(hipMemcpyAsync(h, hb.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    rel = h[1] > 0 ? std::sqrt(h[0] / h[1]) : 0.0;
+    if (rel <= o.tol) break;
+    tau = std::min(0.5, std::max(1e-12, 0.5 * o.tol / rel));
+  }
+  ok = rel <= o.tol;
+  return rel;
+}
+
+void validate_opts(const dm_solve_opts& o) {
+  DM_REQUIRE(std::isfinite(o.tol) && o.tol > 0, DM_ERR_ARGUMENT, "GMRES tol must be finite and positive");
+  DM_REQUIRE(o.maxiter >= 1 && o.maxiter <= 1000, DM_ERR_ARGUMENT, "maxiter must be in [1, 1000]");
+  DM_REQUIRE(o.precond_side == 0 || o.precond_side == 1, DM_ERR_ARGUMENT, "precond_side must be 0 (left) or 1 (right)");
+  DM_REQUIRE(o.method >= DM_METHOD_LU && o.method <= DM_METHOD_MIXED, DM_ERR_ARGUMENT, "unknown solve method");
+}
+
 void check_ctx(const void* a, const char* what) { DM_REQUIRE(a != nullptr, DM_ERR_ARGUMENT, std::string(what) + " is NULL"); }
 
 }  // namespace
@@ -417,6 +534,46 @@ int dm_sync(dm_ctx* ctx) {
     CtxScope sc(ctx);
     HIP_OK(hipStreamSynchronize(ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream2));
+  });
+}
+
+int dm_profile_enable(dm_ctx* ctx, int on) {
+  return guard([&] {
+    check_ctx(ctx, "context");
+    CtxScope sc(ctx);
+    ctx->prof = on != 0;
+  });
+}
+
+int dm_profile_reset(dm_ctx* ctx) {
+  return guard([&] {
+    check_ctx(ctx, "context");
+    CtxScope sc(ctx);
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    for (auto& r : ctx->recs) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    ctx->recs.clear();
+  });
+}
+
+int dm_profile_read(dm_ctx* ctx, const char* name, int64_t* count, double* total_ms) {
+  return guard([&] {
+    check_ctx(ctx, "context");
+    CtxScope sc(ctx);
+    i64 c = 0;
+    double t = 0;
+    for (auto& r : ctx->recs) {
+      if (r.name != name) continue;
+      HIP_OK(hipEventSynchronize(r.b));
+      float ms = 0;
+      HIP_OK(hipEventElapsedTime(&ms, r.a, r.b));
+      t += ms;
+      ++c;
+    }
+    *count = c;
+    *total_ms = t;
   });
 }
 
@@ -577,13 +734,19 @@ int dm_assemble_z(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double k_im, do
     z->quad_order = quad_order;
     hipStream_t s = ctx->stream;
     NearTables nt;
-    AsmArgs a = make_args(r, T, k_re, k_im, eta0, z->ld, nt);
+    AsmArgs a;
+    {
+      ProfScope p(ctx, "near_tables", s);
+      a = make_args(r, T, k_re, k_im, eta0, z->ld, nt);
+    }
     if (forms & DM_Z64) {
       z->z64.alloc((size_t)z->ld * n * 2);
+      ProfScope p(ctx, "assemble_z64", s);
       launch_assemble_z64(a, z->z64.p, r.max_blk_tris, s);
     }
     if (forms & DM_Z32) {
       z->z32.alloc((size_t)z->ld * n * 2);
+      ProfScope p(ctx, "assemble_z32", s);
       launch_assemble_z32(a, z->z32.p, r.max_blk_tris, s);
     }
     check_flag(nt, s);
@@ -800,6 +963,59 @@ int dm_radiation_vectors(dm_ctx* ctx, const dm_rwg* rwg, const double* rhat, int
   });
 }
 
+int dm_precond_create(dm_ctx* ctx, const dm_zmat* z, int kind, double nf_cutoff, int block_size, int form,
+                      dm_precond** out) {
+  return guard([&] {
+    check_ctx(ctx, "context");
+    check_ctx(z, "zmat");
+    check_ctx(out, "output handle");
+    need_form(z, form);
+    DM_REQUIRE(kind == DM_PRECOND_DIAG || kind == DM_PRECOND_NEARFIELD || kind == DM_PRECOND_BLOCK, DM_ERR_ARGUMENT,
+               "preconditioner kind must be DM_PRECOND_DIAG, DM_PRECOND_NEARFIELD or DM_PRECOND_BLOCK");
+    CtxScope sc(ctx);
+    dm_solve_opts o;
+    dm_solve_opts_default(&o);
+    o.precond = kind;
+    o.nf_cutoff = nf_cutoff;
+    o.block_size = block_size;
+    auto h = std::make_unique<dm_precond>();
+    h->ctx = ctx;
+    h->z = z;
+    h->p = build_precond(z, form == DM_Z32, o, ctx->stream);
+    *out = h.release();
+  });
+}
+
+int dm_precond_apply(dm_ctx* ctx, const dm_precond* P, const double* x, double* y) {
+  return guard([&] {
+    check_ctx(ctx, "context");
+    check_ctx(P, "preconditioner");
+    CtxScope sc(ctx);
+    hipStream_t s = ctx->stream;
+    const dm_rwg& r = *P->z->rwg;
+    i64 n = r.n;
+    DBuf<double> xe, xi, yi;
+    xe.alloc(2 * n);
+    xi.alloc(2 * n);
+    yi.alloc(2 * n);
+    HIP_OK(hipMemcpyAsync(xe.p, x, 16 * n, hipMemcpyHostToDevice, s));
+    launch_permute(n, r.d_ext.p, xe.p, xi.p, 1, s);
+    P->p->apply(xi.p, yi.p, s);
+    launch_permute(n, r.d_ext.p, yi.p, xe.p, 0, s);
+    HIP_OK(hipMemcpyAsync(y, xe.p, 16 * n, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+  });
+}
+
+int dm_precond_free(dm_precond* P) {
+  return guard([&] {
+    if (!P) return;
+    std::lock_guard<std::mutex> lk(P->ctx->mu);
+    (void)hipSetDevice(P->ctx->device);
+    delete P;
+  });
+}
+
 void dm_solve_opts_default(dm_solve_opts* o) {
   if (!o) return;
   o->method = DM_METHOD_GMRES;
@@ -815,6 +1031,148 @@ void dm_solve_opts_default(dm_solve_opts* o) {
   o->block_size = 128;
 }
 
+int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double k_im, double eta0, int quad_order,
+                           const double* kvec, const double* pol, const double* E0, int64_t nrhs,
+                           const dm_solve_opts* opts, double* X, dm_solve_stats* stats, double* assembly_time_s,
+                           dm_zmat** zout) {
+  bool unconverged = false;
+  int rc = guard([&] {
+    check_ctx(ctx, "context");
+    check_ctx(rwg_c, "rwg");
+    DM_REQUIRE(rwg_c->ctx == ctx, DM_ERR_ARGUMENT, "RWG data belongs to another context");
+    dm_solve_opts o;
+    dm_solve_opts_default(&o);
+    if (opts) o = *opts;
+    validate_opts(o);
+    DM_REQUIRE(o.method != DM_METHOD_LU || nrhs >= 0, DM_ERR_ARGUMENT, "bad RHS count");
+    DM_REQUIRE(nrhs >= 1, DM_ERR_ARGUMENT, "at least one plane wave is required");
+    validate_prefactors(k_re, k_im, eta0);
+    for (i64 m = 0; m < nrhs; ++m) validate_planewave(kvec + 3 * m, pol + 3 * m, E0[m]);
+    CtxScope sc(ctx);
+    dm_rwg& r = const_cast<dm_rwg&>(*rwg_c);
+    i64 n = r.n;
+    const TriTables& T = dmh::tables(r, quad_order);
+    hipStream_t s = ctx->stream, s2 = ctx->stream2;
+    double t0 = now();
+    auto z = std::make_unique<dm_zmat>();
+    z->ctx = ctx;
+    z->rwg = rwg_c;
+    z->n = n;
+    z->ld = round_ld(n);
+    z->forms = o.method == DM_METHOD_MIXED ? (DM_Z64 | DM_Z32) : DM_Z64;
+    z->k_re = k_re;
+    z->k_im = k_im;
+    z->eta0 = eta0;
+    z->quad_order = quad_order;
+    NearTables nt;
+    AsmArgs a;
+    {
+      ProfScope p(ctx, "near_tables", s);
+      a = make_args(r, T, k_re, k_im, eta0, z->ld, nt);
+    }
+    hipEvent_t e_near = nullptr, e_z64 = nullptr;
+    HIP_OK(hipEventCreateWithFlags(&e_near, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&e_z64, hipEventDisableTiming));
+    struct EvGuard {
+      hipEvent_t a, b;
+      ~EvGuard() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+      }
+    } evg{e_near, e_z64};
+    HIP_OK(hipEventRecord(e_near, s));
+    z->z64.alloc((size_t)z->ld * n * 2);
+    if (o.method == DM_METHOD_MIXED) {
+      z->z32.alloc((size_t)z->ld * n * 2);
+      {
+        ProfScope p(ctx, "assemble_z32", s);
+        launch_assemble_z32(a, z->z32.p, r.max_blk_tris, s);
+      }
+      // FP64 matrix on the second stream, concurrent with the complex64 solve
+      HIP_OK(hipStreamWaitEvent(s2, e_near, 0));
+      {
+        ProfScope p(ctx, "assemble_z64", s2);
+        launch_assemble_z64(a, z->z64.p, r.max_blk_tris, s2);
+      }
+      HIP_OK(hipEventRecord(e_z64, s2));
+    } else {
+      ProfScope p(ctx, "assemble_z64", s);
+      launch_assemble_z64(a, z->z64.p, r.max_blk_tris, s);
+    }
+    DBuf<double> dk, dp, de, V, x, xe;
+    dk.upload(kvec, 3 * nrhs, s);
+    dp.upload(pol, 3 * nrhs, s);
+    de.upload(E0, nrhs, s);
+    V.alloc((size_t)2 * n * nrhs);
+    x.alloc(2 * n);
+    xe.alloc(2 * n);
+    dmf::planewave(r, T, dk.p, dp.p, de.p, (int)nrhs, nullptr, V.p, n, s);
+    HIP_OK(hipStreamSynchronize(s));
+    double t_asm = now() - t0;
+    dm_solve_stats st{};
+    std::unique_ptr<DevOp> A64 = std::make_unique<OpZ64>(z.get()), A32;
+    if (o.method == DM_METHOD_MIXED) A32 = std::make_unique<OpZ32>(z.get());
+    double t1 = now();
+    double worst = 0;
+    bool all_ok = true;
+    if (o.method == DM_METHOD_LU) {
+      DBuf<double> A;
+      DBuf<int> ipiv, status;
+      A.alloc((size_t)2 * z->ld * n);
+      HIP_OK(hipMemcpyAsync(A.p, z->z64.p, A.bytes(), hipMemcpyDeviceToDevice, s));
+      ipiv.alloc(n);
+      status.alloc(1);
+      HIP_OK(hipMemsetAsync(status.p, 0, sizeof(int), s));
+      dml::lu_factor(n, z->ld, A.p, ipiv.p, status.p, s);
+      int sv = 0;
+      HIP_OK(hipMemcpyAsync(&sv, status.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIP_OK(hipStreamSynchronize(s));
+      DM_REQUIRE(sv == 0, DM_ERR_ARGUMENT, "dense LU: matrix is singular");
+      st.precond_time_s = now() - t1;
+      DBuf<double> tmp, hb;
+      tmp.alloc(2 * n);
+      hb.alloc(4);
+      for (i64 j = 0; j < nrhs; ++j) {
+        double* bj = V.p + 2 * n * j;
+        HIP_OK(hipMemcpyAsync(x.p, bj, 16 * n, hipMemcpyDeviceToDevice, s));
+        dml::lu_solve(n, z->ld, A.p, ipiv.p, x.p, n, 1, s);
+        worst = std::max(worst, true_residual(*A64, n, bj, x.p, tmp.p, hb.p, s));
+        launch_permute(n, r.d_ext.p, x.p, xe.p, 0, s);
+        HIP_OK(hipMemcpyAsync(X + 2 * n * j, xe.p, 16 * n, hipMemcpyDeviceToHost, s));
+      }
+    } else {
+      auto M = build_precond(z.get(), o.method == DM_METHOD_MIXED, o, s);
+      HIP_OK(hipStreamSynchronize(s));
+      st.precond_time_s = now() - t1;
+      Krylov K;
+      for (i64 j = 0; j < nrhs; ++j) {
+        bool ok = false;
+        double res = solve_one(ctx, o, A64.get(), A32.get(), M.get(), V.p + 2 * n * j, x.p, n, K, st, ok,
+                               o.method == DM_METHOD_MIXED ? e_z64 : nullptr);
+        worst = std::max(worst, res);
+        all_ok = all_ok && ok;
+        launch_permute(n, r.d_ext.p, x.p, xe.p, 0, s);
+        HIP_OK(hipMemcpyAsync(X + 2 * n * j, xe.p, 16 * n, hipMemcpyDeviceToHost, s));
+      }
+    }
+    HIP_OK(hipStreamSynchronize(s));
+    HIP_OK(hipStreamSynchronize(s2));
+    check_flag(nt, s);
+    st.solved = all_ok ? 1 : 0;
+    st.residual = worst;
+    st.solve_time_s = now() - t1 - st.precond_time_s;
+    if (stats) *stats = st;
+    if (assembly_time_s) *assembly_time_s = t_asm;
+    if (zout) *zout = z.release();
+    if (!all_ok && o.check_convergence) unconverged = true;
+  });
+  if (rc == DM_OK && unconverged) {
+    g_err = "GMRES did not converge within maxiter";
+    return DM_ERR_CONVERGENCE;
+  }
+  return rc;
+}
+
 int dm_solve(dm_ctx* ctx, const dm_zmat* z, const double* B, int64_t ldb, int64_t nrhs, const dm_solve_opts* opts,
              double* X, int64_t ldx, dm_solve_stats* stats) {
   bool unconverged = false;
@@ -826,10 +1184,7 @@ int dm_solve(dm_ctx* ctx, const dm_zmat* z, const double* B, int64_t ldb, int64_
     if (opts) o = *opts;
     i64 n = z->n;
     DM_REQUIRE(nrhs >= 0 && ldb >= n && ldx >= n, DM_ERR_DIMENSION, "leading dimension smaller than N");
-    DM_REQUIRE(std::isfinite(o.tol) && o.tol > 0, DM_ERR_ARGUMENT, "GMRES tol must be finite and positive");
-    DM_REQUIRE(o.maxiter >= 1 && o.maxiter <= 1000, DM_ERR_ARGUMENT, "maxiter must be in [1, 1000]");
-    DM_REQUIRE(o.precond_side == 0 || o.precond_side == 1, DM_ERR_ARGUMENT, "precond_side must be 0 (left) or 1 (right)");
-    DM_REQUIRE(o.method >= DM_METHOD_LU && o.method <= DM_METHOD_MIXED, DM_ERR_ARGUMENT, "unknown solve method");
+    validate_opts(o);
     for (i64 j = 0; j < nrhs; ++j)
       for (i64 i = 0; i < 2 * n; ++i)
         DM_REQUIRE(std::isfinite(B[2 * ldb * j + i]), DM_ERR_ARGUMENT, "right-hand side must be finite");
@@ -888,54 +1243,14 @@ int dm_solve(dm_ctx* ctx, const dm_zmat* z, const double* B, int64_t ldb, int64_
     std::unique_ptr<DevOp> A64, A32;
     if (have64) A64 = std::make_unique<OpZ64>(z);
     if (have32) A32 = std::make_unique<OpZ32>(z);
-    DevOp& Aref = have64 ? *A64 : *A32;
     Krylov K;
     bool all_ok = true;
     double worst = 0;
     for (i64 j = 0; j < nrhs; ++j) {
       HIP_OK(hipMemcpyAsync(be.p, B + 2 * ldb * j, 16 * n, hipMemcpyHostToDevice, s));
       launch_permute(n, r.d_ext.p, be.p, b.p, 1, s);
-      HIP_OK(hipMemsetAsync(x.p, 0, 16 * n, s));
       bool ok = false;
-      double res = 0;
-      if (o.method == DM_METHOD_GMRES) {
-        GmresOut g = gmres(n, Aref, M.get(), o.precond_side, b.p, x.p, o.tol, o.maxiter, o.restart, K, s);
-        st.iterations += g.iters;
-        res = true_residual(Aref, n, b.p, x.p, tmp.p, hb.p, s);
-        ok = g.solved;
-      } else {
-        // Mixed-precision iterative refinement: inner FGMRES on the complex64
-        // matrix with FP64 Arnoldi, outer residual r = b - Z64 x in FP64.
-        DBuf<double> rr, d;
-        rr.alloc(2 * n);
-        d.alloc(2 * n);
-        HIP_OK(hipMemcpyAsync(rr.p, b.p, 16 * n, hipMemcpyDeviceToDevice, s));
-        double rel = 1.0;
-        double tau = o.inner_tol > 0 ? o.inner_tol : 0.5 * o.tol;
-        int left = o.maxiter;
-        for (int outer = 0; outer < std::max(1, o.max_outer) && left > 0; ++outer) {
-          HIP_OK(hipMemsetAsync(d.p, 0, 16 * n, s));
-          GmresOut g = gmres(n, *A32, M.get(), 1, rr.p, d.p, tau, left, o.restart, K, s);
-          st.iterations += g.iters;
-          left -= g.iters;
-          st.outer_iterations += 1;
-          dml::axpby(n, 1, 0, d.p, 1, 0, x.p, s);
-          // rr = b - Z64 x
-          A64->apply(x.p, tmp.p, s);
-          HIP_OK(hipMemcpyAsync(rr.p, b.p, 16 * n, hipMemcpyDeviceToDevice, s));
-          dml::axpby(n, -1, 0, tmp.p, 1, 0, rr.p, s);
-          double h[2];
-          dml::norm2(n, rr.p, hb.p, s);
-          dml::norm2(n, b.p, hb.p + 1, s);
-          HIP_OK(hipMemcpyAsync(h, hb.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-          HIP_OK(hipStreamSynchronize(s));
-          rel = h[1] > 0 ? std::sqrt(h[0] / h[1]) : 0.0;
-          if (rel <= o.tol) break;
-          tau = std::min(0.5, std::max(1e-12, 0.5 * o.tol / rel));
-        }
-        res = rel;
-        ok = rel <= o.tol;
-      }
+      double res = solve_one(ctx, o, A64.get(), A32.get(), M.get(), b.p, x.p, n, K, st, ok, nullptr);
       worst = std::max(worst, res);
       all_ok = all_ok && ok;
       launch_permute(n, r.d_ext.p, x.p, be.p, 0, s);

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -63,12 +64,39 @@ struct DBuf {
 };
 
 // ------------------------------------------------------------------ handles
+struct ProfRec {
+  std::string name;
+  hipEvent_t a, b;
+};
+
 struct dm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;   // main stream: every public call runs here
   hipStream_t stream2 = nullptr;  // second stream for overlapped work
   std::mutex mu;
   int cu_count = 256;
+  bool prof = false;              // record HIP events around tracked launches
+  std::vector<ProfRec> recs;
+};
+
+// Brackets one tracked launch with HIP events on its stream when profiling is on.
+struct ProfScope {
+  dm_ctx* c;
+  hipStream_t s;
+  ProfRec rec;
+  bool on;
+  ProfScope(dm_ctx* c_, const char* name, hipStream_t s_) : c(c_), s(s_), on(c_ && c_->prof) {
+    if (!on) return;
+    rec.name = name;
+    HIP_OK(hipEventCreate(&rec.a));
+    HIP_OK(hipEventCreate(&rec.b));
+    HIP_OK(hipEventRecord(rec.a, s));
+  }
+  ~ProfScope() {
+    if (!on) return;
+    (void)hipEventRecord(rec.b, s);
+    c->recs.push_back(rec);
+  }
 };
 
 struct dm_mesh {
@@ -89,6 +117,15 @@ struct TriTables {
   DBuf<double> vtx;   // nt*9 absolute vertices
   DBuf<double> area;  // nt
   std::vector<double> h_rq, h_area;
+};
+
+// Near-field sparse-approximate-inverse pattern (internal basis order).
+struct SpaiPattern {
+  double cutJ = 0, cutI = 0;
+  i64 nnz = 0;
+  std::vector<int> jptr, jidx, iptr, iidx;   // per column: support J_j, rows I_j
+  std::vector<int> rowptr, colidx, map;      // CSR of M and CSR position -> CSC position
+  DBuf<int> d_jptr, d_jidx, d_iptr, d_iidx, d_rowptr, d_colidx, d_map;
 };
 
 struct dm_rwg {
@@ -120,6 +157,9 @@ struct dm_rwg {
   DBuf<int> d_tplus_i, d_tminus_i, d_eplus_i, d_eminus_i;  // internal order, 0-based
   TriTables tabs[8];        // indexed by quad order (1,3,4,7)
   std::mutex tab_mu;
+  std::vector<double> cen_i;  // internal-order basis centres (3 per basis)
+  double mean_len = 0;
+  std::unique_ptr<SpaiPattern> spai;
 };
 
 struct dm_zmat {
@@ -148,7 +188,16 @@ void mesh_check(const dm_mesh& m, int allow_boundary, int require_closed, double
 double triangle_area(const dm_mesh& m, i64 t0);
 const TriTables& tables(dm_rwg& r, int quad_order);
 int quad_rule(int order, double* xi, double* w);
+const SpaiPattern& spai_pattern(dm_rwg& r, double cutJ, int maxJ);
 }  // namespace dmh
+
+namespace dmp {
+int spai_max_cols();
+void spai_build(i64 n, i64 ld, const double* z64, const float* z32, const int* jptr, const int* jidx, const int* iptr,
+                const int* iidx, double* vals, int* status, hipStream_t s);
+void csr_from_csc(i64 nnz, const int* map, const double* cscval, double* csrval, hipStream_t s);
+void spmv(i64 n, const int* rowptr, const int* colidx, const double* val, const double* x, double* y, hipStream_t s);
+}  // namespace dmp
 
 // ------------------------------------------------------------ kernel launchers
 struct AsmArgs {
@@ -161,6 +210,7 @@ struct AsmArgs {
   const int* nb;
   const int* ext;
   const double* len;
+  const int* tplus_i;  // internal basis -> T+ triangle (0-based)
   const double* cen;
   const double* uq;
   const double* av;

@@ -346,6 +346,16 @@ void finalize_rwg(dm_rwg& r) {
   std::vector<int> hrwg(nt * 3, -1);
   for (i64 h = 0; h < nt * 3; ++h)
     if (hrwg_ext[h] >= 0) hrwg[h] = r.int_of_ext[hrwg_ext[h]];
+  r.cen_i.assign(3 * n, 0.0);
+  double lsum = 0;
+  for (i64 i = 0; i < n; ++i) {
+    int e = r.ext_of_int[i];
+    r.cen_i[3 * i] = cx[e];
+    r.cen_i[3 * i + 1] = cy[e];
+    r.cen_i[3 * i + 2] = cz[e];
+    lsum += r.len[e];
+  }
+  r.mean_len = lsum / n;
   std::vector<double> len_i(n);
   std::vector<int> tp(n), tm(n), ep(n), em(n);
   for (i64 i = 0; i < n; ++i) {
@@ -370,6 +380,102 @@ void finalize_rwg(dm_rwg& r) {
   r.d_eplus_i.upload(ep, s);
   r.d_eminus_i.upload(em, s);
   HIP_OK(hipStreamSynchronize(s));
+}
+
+// Near-field pattern for the sparse approximate inverse: J_j = the (at most
+// maxJ nearest) bases with centre distance <= cutJ from basis j, I_j = bases
+// within 1.6 x the radius actually covered by J_j (at most 512 nearest).
+// Centres are the midpoints of the two triangle centroids
+// (rwg_centers, src/solver/NearFieldPreconditioner.jl:247-257).
+const SpaiPattern& spai_pattern(dm_rwg& r, double cutJ, int maxJ) {
+  std::lock_guard<std::mutex> lk(r.tab_mu);
+  if (r.spai && r.spai->cutJ == cutJ) return *r.spai;
+  auto P = std::make_unique<SpaiPattern>();
+  P->cutJ = cutJ;
+  i64 n = r.n;
+  const double* c = r.cen_i.data();
+  double lo[3] = {INFINITY, INFINITY, INFINITY};
+  for (i64 i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) lo[k] = std::min(lo[k], c[3 * i + k]);
+  // hash grid with cells of the row cutoff (upper bound 1.6 cutJ)
+  double cell = 1.6 * cutJ;
+  std::map<std::array<long, 3>, std::vector<int>> grid;
+  auto key = [&](i64 i) {
+    return std::array<long, 3>{(long)std::floor((c[3 * i] - lo[0]) / cell), (long)std::floor((c[3 * i + 1] - lo[1]) / cell),
+                               (long)std::floor((c[3 * i + 2] - lo[2]) / cell)};
+  };
+  for (i64 i = 0; i < n; ++i) grid[key(i)].push_back((int)i);
+  P->jptr.assign(n + 1, 0);
+  P->iptr.assign(n + 1, 0);
+  std::vector<std::pair<double, int>> cand;
+  for (i64 j = 0; j < n; ++j) {
+    cand.clear();
+    auto kj = key(j);
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dz = -1; dz <= 1; ++dz) {
+          auto it = grid.find({kj[0] + dx, kj[1] + dy, kj[2] + dz});
+          if (it == grid.end()) continue;
+          for (int m : it->second) {
+            double ddx = c[3 * m] - c[3 * j], ddy = c[3 * m + 1] - c[3 * j + 1], ddz = c[3 * m + 2] - c[3 * j + 2];
+            double d = std::sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+            if (d <= cell) cand.emplace_back(d, m);
+          }
+        }
+    std::sort(cand.begin(), cand.end());
+    int nj = 0;
+    double rJ = 0;
+    std::vector<int> J;
+    for (auto& pr : cand) {
+      if (pr.first > cutJ || nj >= maxJ) break;
+      J.push_back(pr.second);
+      rJ = pr.first;
+      ++nj;
+    }
+    if (J.empty() || J[0] != (int)j) {  // the basis itself is always in its own support
+      J.erase(std::remove(J.begin(), J.end(), (int)j), J.end());
+      J.insert(J.begin(), (int)j);
+      if ((int)J.size() > maxJ) J.resize(maxJ);
+    }
+    double rI = std::max(1.6 * rJ, 1e-300);
+    std::vector<int> I;
+    for (auto& pr : cand) {
+      if (pr.first > rI || (int)I.size() >= 512) break;
+      I.push_back(pr.second);
+    }
+    std::sort(J.begin(), J.end());
+    std::sort(I.begin(), I.end());
+    P->jidx.insert(P->jidx.end(), J.begin(), J.end());
+    P->iidx.insert(P->iidx.end(), I.begin(), I.end());
+    P->jptr[j + 1] = (int)P->jidx.size();
+    P->iptr[j + 1] = (int)P->iidx.size();
+  }
+  P->nnz = (i64)P->jidx.size();
+  // CSR of M by transposing the CSC support lists
+  P->rowptr.assign(n + 1, 0);
+  for (int rr : P->jidx) P->rowptr[rr + 1]++;
+  for (i64 i = 0; i < n; ++i) P->rowptr[i + 1] += P->rowptr[i];
+  P->colidx.resize(P->nnz);
+  P->map.resize(P->nnz);
+  std::vector<int> fill(P->rowptr.begin(), P->rowptr.end() - 1);
+  for (i64 j = 0; j < n; ++j)
+    for (int p = P->jptr[j]; p < P->jptr[j + 1]; ++p) {
+      int rr = P->jidx[p];
+      int q = fill[rr]++;
+      P->colidx[q] = (int)j;
+      P->map[q] = p;
+    }
+  hipStream_t s = r.ctx->stream;
+  P->d_jptr.upload(P->jptr, s);
+  P->d_jidx.upload(P->jidx, s);
+  P->d_iptr.upload(P->iptr, s);
+  P->d_iidx.upload(P->iidx, s);
+  P->d_rowptr.upload(P->rowptr, s);
+  P->d_colidx.upload(P->colidx, s);
+  P->d_map.upload(P->map, s);
+  HIP_OK(hipStreamSynchronize(s));
+  r.spai = std::move(P);
+  return *r.spai;
 }
 
 // Per-triangle tables for one quadrature order: quad points via the affine map

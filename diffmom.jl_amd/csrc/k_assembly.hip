@@ -27,6 +27,9 @@
 // with ds_add atomics, and writes the tile and its mirror (Z = Z^T) to HBM.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdlib>
+
 #include "dm_common.h"
 
 namespace {
@@ -337,11 +340,55 @@ __device__ inline void load_side(const AsmArgs& a, int blk, int nt_side, TileSid
   }
 }
 
+// ------------------------------------------------------------ fast kernels
+// sin/cos of a phase kR (|kR| << 2^20) in FP64: Cody-Waite reduction by pi/2
+// with a two-term split applied through FMA, then Taylor polynomials on
+// |r| <= pi/4 (sin to r^15, cos to r^16: truncation < 5e-17).  The split is
+// pi/2 rounded to double plus the double nearest the remainder, both computed
+// from the exact value of pi.
+__device__ inline void sincos_phase(double x, double* s, double* c) {
+  const double PIO2_HI = 1.5707963267948966, PIO2_LO = 6.123233995736766e-17, TWO_OVER_PI = 0.6366197723675814;
+  double n = rint(x * TWO_OVER_PI);
+  double r = fma(-n, PIO2_HI, x);
+  r = fma(-n, PIO2_LO, r);
+  double z = r * r;
+  double ps = fma(z, -7.647163731819816e-13, 1.6059043836821613e-10);
+  ps = fma(z, ps, -2.505210838544172e-08);
+  ps = fma(z, ps, 2.7557319223985893e-06);
+  ps = fma(z, ps, -0.0001984126984126984);
+  ps = fma(z, ps, 0.008333333333333333);
+  ps = fma(z, ps, -0.16666666666666666);
+  double sr = fma(r * z, ps, r);
+  double pc = fma(z, 4.779477332387385e-14, -1.1470745597729725e-11);
+  pc = fma(z, pc, 2.08767569878681e-09);
+  pc = fma(z, pc, -2.755731922398589e-07);
+  pc = fma(z, pc, 2.48015873015873e-05);
+  pc = fma(z, pc, -0.001388888888888889);
+  pc = fma(z, pc, 0.041666666666666664);
+  pc = fma(z, pc, -0.5);
+  double cr = fma(z, pc, 1.0);
+  int q = (int)n & 3;
+  double ss = (q & 1) ? cr : sr;
+  double cc = (q & 1) ? sr : cr;
+  *s = (q & 2) ? -ss : ss;
+  *c = ((q + 1) & 2) ? -cc : cc;
+}
+
+// 1/sqrt in FP64: hardware estimate refined by two Newton steps.
+__device__ inline double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
+}
+
 // Far-pair J block in compute type T (moment form in centroid-relative
 // coordinates, so FP32 keeps its precision for nearby pairs).
 template <typename T, int NQ>
 __device__ inline void far_block(const TileSide<T, NQ>& R, int i, const TileSide<T, NQ>& C, int j,
                                  const T* ww, T kr, T ki, T fik_re, T fik_im, T Jr[9], T Ji[9]) {
+  const T kr_turns = kr * (T)0.15915494309189535;  // k / (2 pi)
   T dc[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) dc[c] = (T)(R.c[3 * i + c] - C.c[3 * j + c]);
@@ -363,13 +410,20 @@ __device__ inline void far_block(const TileSide<T, NQ>& R, int i, const TileSide
       T dy = dc[1] + (ur[q][1] - uc[p][1]);
       T dz = dc[2] + (ur[q][2] - uc[p][2]);
       T R2 = dx * dx + dy * dy + dz * dz;
-      T Rd = sqrt(R2);
-      T g = ww[q * NQ + p] / Rd;
-      T sn, cs;
+      T sn, cs, Rd, g;
       if constexpr (sizeof(T) == 8) {
-        sincos(kr * Rd, &sn, &cs);
+        T iR = rsqrt_nr(R2);
+        Rd = R2 * iR;
+        g = ww[q * NQ + p] * iR;
+        sincos_phase(kr * Rd, &sn, &cs);
       } else {
-        sincosf(kr * Rd, &sn, &cs);
+        // hardware rsq / sin / cos; the phase goes in revolutions (turns)
+        T iR = __builtin_amdgcn_rsqf(R2);
+        Rd = R2 * iR;
+        g = ww[q * NQ + p] * iR;
+        T turns = __builtin_amdgcn_fractf(kr_turns * Rd);
+        sn = __builtin_amdgcn_sinf(turns);
+        cs = __builtin_amdgcn_cosf(turns);
       }
       if (ki != (T)0) g *= exp(ki * Rd);
       T gr = g * cs, gi = -g * sn;
@@ -496,9 +550,8 @@ __global__ __launch_bounds__(NTHR) void k_tile(AsmArgs a, int cap, AccT* __restr
           else
             v = fwd ? jadj[((i64)gi * 3 + jn) * 9 + 3 * e + f] : jadj[((i64)gj * 3 + jn2) * 9 + 3 * f + e];
           double s = R.sig[3 * i + e] * C.sig[3 * j + f];
-          AccT* dst = zt + 2 * (rs * ldt + cs);
-          atomicAdd(dst, (AccT)(v.re * s));
-          atomicAdd(dst + 1, (AccT)(v.im * s));
+          atomicAdd(zt + rs * ldt + cs, (AccT)(v.re * s));
+          atomicAdd(zt + TB * ldt + rs * ldt + cs, (AccT)(v.im * s));
         }
       }
       continue;
@@ -515,9 +568,8 @@ __global__ __launch_bounds__(NTHR) void k_tile(AsmArgs a, int cap, AccT* __restr
         int cs = C.slot[3 * j + f];
         if (cs < 0) continue;
         T s = sr * (T)C.sig[3 * j + f];
-        AccT* dst = zt + 2 * (rs * ldt + cs);
-        atomicAdd(dst, (AccT)(Jr[3 * e + f] * s));
-        atomicAdd(dst + 1, (AccT)(Ji[3 * e + f] * s));
+        atomicAdd(zt + rs * ldt + cs, (AccT)(Jr[3 * e + f] * s));
+        atomicAdd(zt + TB * ldt + rs * ldt + cs, (AccT)(Ji[3 * e + f] * s));
       }
     }
   }
@@ -535,7 +587,7 @@ __global__ __launch_bounds__(NTHR) void k_tile(AsmArgs a, int cap, AccT* __restr
       if (gr >= a.n || gc >= a.n) continue;
       int rr = r, cc = c;
       if (diag && r > c) { rr = c; cc = r; }
-      double tr = (double)zt[2 * (rr * ldt + cc)], ti = (double)zt[2 * (rr * ldt + cc) + 1];
+      double tr = (double)zt[rr * ldt + cc], ti = (double)zt[TB * ldt + rr * ldt + cc];
       double l2 = a.len[gr] * a.len[gc];
       double vr = (Pr * tr - Pi * ti) * l2, vi = (Pr * ti + Pi * tr) * l2;
       bad |= !isfinite(vr) || !isfinite(vi);
@@ -548,7 +600,7 @@ __global__ __launch_bounds__(NTHR) void k_tile(AsmArgs a, int cap, AccT* __restr
         int c = k % TB, r = k / TB;
         i64 gr = r0 + r, gc = c0 + c;
         if (gr >= a.n || gc >= a.n) continue;
-        double tr = (double)zt[2 * (r * ldt + c)], ti = (double)zt[2 * (r * ldt + c) + 1];
+        double tr = (double)zt[r * ldt + c], ti = (double)zt[TB * ldt + r * ldt + c];
         double l2 = a.len[gr] * a.len[gc];
         double vr = (Pr * tr - Pi * ti) * l2, vi = (Pr * ti + Pi * tr) * l2;
         AccT* o = zout + 2 * (gc + gr * a.ld);
@@ -571,7 +623,7 @@ __global__ __launch_bounds__(NTHR) void k_tile(AsmArgs a, int cap, AccT* __restr
         if (gi2 >= a.n) break;
         int rr = rows ? r : c, cc = rows ? c : r;
         if (diag && rr > cc) { int t2 = rr; rr = cc; cc = t2; }
-        double tr = (double)zt[2 * (rr * ldt + cc)], ti = (double)zt[2 * (rr * ldt + cc) + 1];
+        double tr = (double)zt[rr * ldt + cc], ti = (double)zt[TB * ldt + rr * ldt + cc];
         double l2 = a.len[go] * a.len[gi2];
         double vr = (Pr * tr - Pi * ti) * l2, vi = (Pr * ti + Pi * tr) * l2;
         double xr = x[2 * gi2], xi = x[2 * gi2 + 1];

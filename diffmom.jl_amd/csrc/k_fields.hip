@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void k_planewave(PwArgs a, double2* __restrict
     acci += s * hi;
   }
   double l = a.len[i];
-  V[(i64)a.out_idx[i] + (i64)m * ldv] = make_double2(-l * accr, -l * acci);
+  V[(a.out_idx ? (i64)a.out_idx[i] : i) + (i64)m * ldv] = make_double2(-l * accr, -l * acci);
 }
 
 struct RadArgs {

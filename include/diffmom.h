@@ -59,6 +59,14 @@ int dm_version(void);
 /* Device synchronisation (for host timing around asynchronous work). */
 int dm_sync(dm_ctx* ctx);
 
+/* Kernel timing on the library's own streams (no reference counterpart; for
+ * benchmarks): when enabled, tracked launches ("assemble_z64", "assemble_z32",
+ * "near_tables", "gemv_z64", "gemv_z32", "arnoldi", "precond") are bracketed
+ * with HIP events; dm_profile_read sums the elapsed times of one name. */
+int dm_profile_enable(dm_ctx* ctx, int on);
+int dm_profile_reset(dm_ctx* ctx);
+int dm_profile_read(dm_ctx* ctx, const char* name, int64_t* count, double* total_ms);
+
 /* --------------------------------------------------------------------- mesh */
 /* TriMesh(xyz, tri): src/Types.jl:123-129.  Copies the arrays. */
 int dm_mesh_create(dm_ctx* ctx, const double* xyz, int64_t nv, const int64_t* tri, int64_t nt,
@@ -140,8 +148,9 @@ int dm_excitation_planewave(dm_ctx* ctx, const dm_rwg* rwg, const double* kvec, 
 
 #define DM_PRECOND_NONE 0
 #define DM_PRECOND_DIAG 1     /* Jacobi (factorization=:diag, src/solver/NearFieldPreconditioner.jl:623-668) */
-#define DM_PRECOND_NEARFIELD 2 /* near-field block LU (cutoff-defined spatial blocks) */
+#define DM_PRECOND_NEARFIELD 2 /* near-field sparse approximate inverse over the cutoff pattern */
 #define DM_PRECOND_LEARNED 3  /* message-passing network, flexible (needs a trained model) */
+#define DM_PRECOND_BLOCK 4    /* block Jacobi over spatially consecutive bases */
 
 typedef struct dm_solve_opts {
   int method;          /* DM_METHOD_* */
@@ -149,12 +158,12 @@ typedef struct dm_solve_opts {
   int precond_side;    /* 0 = left, 1 = right (solve_gmres precond_side) */
   int maxiter;         /* total Krylov iterations allowed */
   double tol;          /* relative tolerance on the (true FP64) residual */
-  double nf_cutoff;    /* near-field cutoff in metres (<= 0: automatic block size) */
+  double nf_cutoff;    /* near-field cutoff in metres (<= 0: 2.5 x mean edge length) */
   int restart;         /* Krylov basis size before restart (0 = no restart) */
   int check_convergence; /* 1: DM_ERR_CONVERGENCE when not converged */
   double inner_tol;    /* MIXED: relative tolerance of the first inner solve (<= 0: tol/2) */
   int max_outer;       /* MIXED: refinement steps allowed */
-  int block_size;      /* DM_PRECOND_NEARFIELD: bases per spatial block (0: 128) */
+  int block_size;      /* DM_PRECOND_BLOCK: bases per spatial block (0: 128) */
 } dm_solve_opts;
 
 typedef struct dm_solve_stats {
@@ -167,9 +176,34 @@ typedef struct dm_solve_stats {
 } dm_solve_stats;
 
 void dm_solve_opts_default(dm_solve_opts* opts);
+
+/* Preconditioner as an operator: build_nearfield_preconditioner +
+ * NearFieldOperator's mul! (src/solver/NearFieldPreconditioner.jl:732-759,
+ * 1034-1060).  kind: DM_PRECOND_NEARFIELD (sparse approximate inverse of the
+ * near field within nf_cutoff metres, <= 0: 2.5 x mean edge), DM_PRECOND_DIAG or
+ * DM_PRECOND_BLOCK; built from the stored form `form` of Z.  apply: y = M x
+ * (host vectors, caller basis order). */
+typedef struct dm_precond dm_precond;
+int dm_precond_create(dm_ctx* ctx, const dm_zmat* z, int kind, double nf_cutoff, int block_size, int form,
+                      dm_precond** out);
+int dm_precond_apply(dm_ctx* ctx, const dm_precond* p, const double* x, double* y);
+int dm_precond_free(dm_precond* p);
 /* Solve Z X = B for nrhs right-hand sides (host buffers, caller order). */
 int dm_solve(dm_ctx* ctx, const dm_zmat* z, const double* B, int64_t ldb, int64_t nrhs,
              const dm_solve_opts* opts, double* X, int64_t ldx, dm_solve_stats* stats);
+
+/* solve_scattering(mesh, freq, excitation; method=:dense_direct|:dense_gmres)
+ * for plane waves, end to end on the device (src/Workflow.jl:67-317): assembly,
+ * excitation, preconditioner and solve of nrhs waves (kvec, pol: 3 x nrhs).
+ * opts->method DM_METHOD_LU | DM_METHOD_GMRES | DM_METHOD_MIXED; with MIXED the
+ * complex128 matrix (needed only by the refinement residual) is assembled on a
+ * second stream while the complex64 solve runs.  X: N x nrhs complex128,
+ * caller basis order.  assembly_time_s (optional): wall time until the matrix
+ * the solver iterates on is ready.  zout (optional): receives the matrix. */
+int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg, double k_re, double k_im, double eta0, int quad_order,
+                           const double* kvec, const double* pol, const double* E0, int64_t nrhs,
+                           const dm_solve_opts* opts, double* X, dm_solve_stats* stats, double* assembly_time_s,
+                           dm_zmat** zout);
 
 /* ---------------------------------------------------------------- far field */
 /* radiation_vectors(mesh, rwg, grid, k; quad_order, eta0):

@@ -630,6 +630,24 @@ void or_assemble_z_rows(void* h, i64 row0, i64 row1, double* Z, i64 ldz) {
     }
 }
 
+// CPU-baseline sample: entries (m, n >= m) for rows [row0, row1), reduced to a
+// checksum (no storage).  Returns the number of entries evaluated.
+i64 or_assemble_rows_sum(void* h, i64 row0, i64 row1, double* checksum) {
+  const Cache& c = ((OrCtx*)h)->c;
+  i64 N = c.rwg.n;
+  double s = 0;
+  i64 cnt = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : s, cnt)
+  for (i64 m = row0; m < row1; ++m)
+    for (i64 n = m; n < N; ++n) {
+      cd z = efie_entry(c, m, n);
+      s += z.real() + z.imag();
+      ++cnt;
+    }
+  *checksum = s;
+  return cnt;
+}
+
 // Matrix-free product y = A x recomputing every entry: src/assembly/EFIE.jl:867-909
 // (the exact-reduction BigFloat fallback is not restated).
 void or_mf_apply(void* h, const double* x, double* y) {

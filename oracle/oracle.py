@@ -72,6 +72,8 @@ def lib():
     L.or_efie_entries.argtypes = [vp, i64p, i64p, C.c_int64, f64p]
     L.or_assemble_z_rows.argtypes = [vp, C.c_int64, C.c_int64, f64p, C.c_int64]
     L.or_mf_apply.argtypes = [vp, f64p, f64p]
+    L.or_assemble_rows_sum.argtypes = [vp, C.c_int64, C.c_int64, f64p]
+    L.or_assemble_rows_sum.restype = C.c_int64
     L.or_plane_wave.argtypes = rwg_args + [f64p, C.c_double, f64p, C.c_int, f64p]
     L.or_radiation_vectors.argtypes = rwg_args + [f64p, C.c_int64, C.c_double, C.c_double,
                                                   C.c_int, f64p]
@@ -222,6 +224,13 @@ class EFIE:
         buf = Z.reshape(-1, order="F").view(np.float64)
         lib().or_assemble_z_rows(self.h, int(row0), int(row1), buf, N)
         return Z
+
+    def rows_sum(self, row0, row1):
+        """Evaluate entries (m, n >= m) of rows [row0, row1) without storing them;
+        returns (entry count, checksum).  Used to time the CPU baseline."""
+        cs = np.zeros(1)
+        cnt = lib().or_assemble_rows_sum(self.h, int(row0), int(row1), cs)
+        return cnt, float(cs[0])
 
     def mf_apply(self, x):
         x = np.ascontiguousarray(x, dtype=np.complex128)

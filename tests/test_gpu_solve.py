@@ -33,7 +33,7 @@ def test_dense_direct(dm, c0):
     assert np.linalg.norm(I - npz["I"]) / np.linalg.norm(npz["I"]) < 1e-9
 
 
-@pytest.mark.parametrize("precond", [None, "diag", "nf"])
+@pytest.mark.parametrize("precond", [None, "diag", "nf", "block"])
 @pytest.mark.parametrize("side", ["left", "right"])
 def test_gmres_matches_direct(dm, c0, precond, side):
     m, r, Z, v = c0
@@ -42,7 +42,9 @@ def test_gmres_matches_direct(dm, c0, precond, side):
     if precond == "diag":
         P = dm.build_nearfield_preconditioner(Z, m, r, 0.0, factorization="diag")
     elif precond == "nf":
-        P = dm.build_nearfield_preconditioner(Z, m, r, 0.1, block_size=32)
+        P = dm.build_nearfield_preconditioner(Z, m, r, 0.03)
+    elif precond == "block":
+        P = dm.build_nearfield_preconditioner(Z, m, r, 0.0, factorization="block", block_size=32)
     x, st = dm.solve_gmres(Z, v, preconditioner=P, precond_side=side, tol=1e-8, maxiter=200)
     assert st.solved
     assert np.linalg.norm(x - Id) / np.linalg.norm(Id) < 1e-6
@@ -62,7 +64,7 @@ def test_gmres_restarted_and_nonconvergence(dm, c0):
 def test_mixed_precision_refinement(dm, c0):
     m, r, Z, v = c0
     Id = dm.solve_forward(Z, v, solver="direct")
-    P = dm.build_nearfield_preconditioner(Z, m, r, 0.1, block_size=32)
+    P = dm.build_nearfield_preconditioner(Z, m, r, 0.03)
     x, st = dm.solve_mixed(Z, v, P, tol=1e-10, maxiter=300)
     assert st.solved and st.residual <= 1e-10
     assert np.linalg.norm(x - Id) / np.linalg.norm(Id) < 1e-8
@@ -88,8 +90,11 @@ def test_plate_10k_gmres_true_residual(dm):
     th = math.radians(30)
     v = dm.assemble_v_plane_wave(m, r, K0 * np.array([math.sin(th), 0, math.cos(th)]), 1.0,
                                  [math.cos(th), 0, -math.sin(th)])
-    P = dm.build_nearfield_preconditioner(Z, m, r, 0.1)
+    P = dm.build_nearfield_preconditioner(Z, m, r, 0.0)
     x, st = dm.solve_gmres(Z, v, preconditioner=P, precond_side="right", tol=1e-8, maxiter=500)
+    x0, st0 = dm.solve_gmres(Z, v, precond_side="right", tol=1e-8, maxiter=500)
+    print(f"plate 30x30: {st.niter} iterations with the near-field preconditioner, {st0.niter} without")
+    assert st.niter < st0.niter
     assert st.solved and st.residual < 1e-7
     Id = dm.solve_forward(Z, v, solver="direct")
     assert np.linalg.norm(x - Id) / np.linalg.norm(Id) < 1e-6
@@ -127,3 +132,58 @@ def test_solve_scattering_sphere_mie_gate(dm, O):
     for prec in ("fp64", "mixed"):
         rg = dm.solve_scattering(m, f, pw, method="dense_gmres", gmres_tol=1e-8, precision=prec)
         assert np.linalg.norm(rg.I_coeffs - Io) / np.linalg.norm(Io) < 1e-6
+
+
+def test_nearfield_preconditioner_columns_and_gain(dm, O):
+    """Each column of the near-field sparse approximate inverse solves the same
+    least-squares problem numpy solves on the oracle's matrix, and the
+    preconditioner cuts GMRES iterations on a sphere."""
+    m = dm.make_icosphere(0.1, 3)
+    r = dm.build_rwg(m)
+    Z = dm.assemble_Z_efie(m, r, K0, device=True)
+    v = dm.assemble_v_plane_wave(m, r, [0, 0, -K0], 1.0, [1, 0, 0])
+    P = dm.build_nearfield_preconditioner(Z, m, r, 0.0)
+    x1, st1 = dm.solve_gmres(Z, v, P, precond_side="right", tol=1e-6, maxiter=500)
+    x0, st0 = dm.solve_gmres(Z, v, None, precond_side="right", tol=1e-6, maxiter=500)
+    print(f"sphere N={r.nedges}: {st1.niter} iterations with the near-field preconditioner, {st0.niter} without")
+    assert st1.solved and st1.niter * 2 < st0.niter
+    assert np.linalg.norm(x1 - x0) / np.linalg.norm(x0) < 1e-5
+    # columns of M against numpy least squares over the same pattern on the oracle's Z
+    Zo = O.assemble_Z_efie(oracle_rwg(O, m, r), K0)
+    cen = O.rwg_centers(oracle_rwg(O, m, r))
+    cutJ = 2.5 * float(np.mean(r.len))
+    op = dm.NearFieldOperator(P, Z)
+    for j in (0, 17, 999, r.nedges - 1):
+        e = np.zeros(r.nedges, complex)
+        e[j] = 1
+        col = op @ e
+        d = np.linalg.norm(cen - cen[j], axis=1)
+        J = np.where(d <= cutJ)[0]
+        assert len(J) < 80
+        I = np.where(d <= 1.6 * d[J].max())[0]
+        mj, *_ = np.linalg.lstsq(Zo[np.ix_(I, J)], (I == j).astype(complex), rcond=None)
+        ref = np.zeros(r.nedges, complex)
+        ref[J] = mj
+        assert set(np.nonzero(col)[0]) == set(J)
+        assert np.linalg.norm(col - ref) / np.linalg.norm(ref) < 1e-8
+
+
+@pytest.mark.parametrize("method", ["direct", "fp64", "mixed"])
+def test_end_to_end_planewaves(dm, O, method):
+    """One-call device solve of several plane waves (assembly on two streams in
+    the mixed method) against the oracle's FP64 dense solution."""
+    m = dm.make_icosphere(0.1, 2)
+    r = dm.build_rwg(m)
+    exc = [dm.make_plane_wave(K0 * np.array([0, 0, -1.0]), 1.0, [1.0, 0, 0]),
+           dm.make_plane_wave(K0 * np.array([1.0, 0, 0]), 2.0, [0, 0.6, 0.8])]
+    P = None if method == "direct" else dm.build_nearfield_preconditioner(None, m, r, 0.0)
+    X, st, t_asm = dm.solve_planewaves(m, r, K0, exc, method, P, tol=1e-9, maxiter=500)
+    orr = oracle_rwg(O, m, r)
+    Zo = O.assemble_Z_efie(orr, K0)
+    for j, e in enumerate(exc):
+        vo = O.assemble_v_plane_wave(orr, e.k_vec, e.E0, e.pol)
+        Io = O.solve_direct(Zo, vo)
+        assert np.linalg.norm(X[:, j] - Io) / np.linalg.norm(Io) < 1e-7
+    assert st.solved and t_asm > 0
+    if method == "mixed":
+        assert st.outer >= 1
