@@ -110,6 +110,12 @@ void check_flag(NearTables& nt, hipStream_t s) {
 
 i64 round_ld(i64 n) { return (n + 63) / 64 * 64; }
 
+// dm_apply switches from one GEMV per vector to the MFMA Z X product at this
+// many right-hand sides, processed in blocks of at most kMfmaMaxRhs columns
+// (bounds the split-K workspace: 8 chunks x 256 x N complex128 at N = 30k).
+constexpr i64 kMfmaMinRhs = 8;
+constexpr i64 kMfmaMaxRhs = 256;
+
 // ------------------------------------------------------------ device operators
 struct DevOp {
   virtual ~DevOp() = default;
@@ -833,6 +839,34 @@ int dm_apply(dm_ctx* ctx, const dm_zmat* z, int form, int adjoint, const double*
     DM_REQUIRE(nrhs >= 0 && ldx >= n && ldy >= n, DM_ERR_DIMENSION, "leading dimension smaller than N");
     CtxScope sc(ctx);
     hipStream_t s = ctx->stream;
+    if (nrhs >= kMfmaMinRhs) {
+      // many right-hand sides: one Z X product on the matrix cores, Z read once
+      // per block of 64 right-hand sides instead of once per vector
+      for (i64 j0 = 0; j0 < nrhs; j0 += kMfmaMaxRhs) {
+        int R = (int)std::min<i64>(kMfmaMaxRhs, nrhs - j0);
+        DBuf<double> xe, xi, yi;
+        DBuf<char> ws;
+        xe.alloc(2 * n * R);
+        xi.alloc(2 * n * R);
+        yi.alloc(2 * n * R);
+        ws.alloc(dml::zx_workspace_bytes(n, R));
+        HIP_OK(hipMemcpy2DAsync(xe.p, 16 * n, X + 2 * ldx * j0, 16 * ldx, 16 * n, R, hipMemcpyHostToDevice, s));
+        if (adjoint) dml::conj(n * R, xe.p, s);
+        for (int j = 0; j < R; ++j) launch_permute(n, z->rwg->d_ext.p, xe.p + 2 * n * j, xi.p + 2 * n * j, 1, s);
+        if (form == DM_Z64) {
+          ProfScope p(ctx, "zx_z64", s);
+          dml::zx_mfma64(n, z->ld, z->z64.p, xi.p, n, R, yi.p, n, ws.p, s);
+        } else {
+          ProfScope p(ctx, "zx_z32", s);
+          dml::zx_mfma(n, z->ld, z->z32.p, xi.p, n, R, yi.p, n, ws.p, s);
+        }
+        for (int j = 0; j < R; ++j) launch_permute(n, z->rwg->d_ext.p, yi.p + 2 * n * j, xe.p + 2 * n * j, 0, s);
+        if (adjoint) dml::conj(n * R, xe.p, s);
+        HIP_OK(hipMemcpy2DAsync(Y + 2 * ldy * j0, 16 * ldy, xe.p, 16 * n, 16 * n, R, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+      }
+      return;
+    }
     DBuf<double> xe, xi, yi;
     xe.alloc(2 * n);
     xi.alloc(2 * n);

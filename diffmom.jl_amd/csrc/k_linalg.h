@@ -23,6 +23,11 @@ void entries32(i64 ld, i64 count, const int* iofe, const i64* m, const i64* n, c
 void finite64(i64 n, i64 ld, const double* z, int* flag, hipStream_t s);
 void finite32(i64 n, i64 ld, const float* z, int* flag, hipStream_t s);
 void conj(i64 n, double* v, hipStream_t s);
+size_t zx_workspace_bytes(i64 n, int R);
+void zx_mfma(i64 n, i64 ld, const float* z32, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
+             hipStream_t s);
+void zx_mfma64(i64 n, i64 ld, const double* z64, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
+               hipStream_t s);
 }  // namespace dml
 
 namespace dmf {

@@ -1,0 +1,257 @@
+// k_zx.hip — multi-right-hand-side product Y = Z X on MFMA (complex64 Z).
+//
+// The complex64 matrix is column-major with interleaved (re, im) parts, i.e.
+// it already is the real 2N x N matrix A' with rows (Re z_m, Im z_m).  With
+// P = A' Re(X) and Q = A' Im(X) (two real products sharing every A' tile):
+//   Re Y_m = P_{2m} - Q_{2m+1},   Im Y_m = Q_{2m} + P_{2m+1}
+// so the complex product costs exactly 8 N^2 R flops with no repacking of Z.
+// Tiles: a workgroup (4 waves) owns 64 complex rows x 64 right-hand sides for
+// one K chunk; each wave owns 32 real rows x 64 columns of P and Q as 2x4x2
+// v_mfma_f32_16x16x4f32 accumulators.  A' and X tiles are staged through LDS.
+// K chunks of KC columns accumulate in FP32 and are summed in FP64 by a second
+// kernel, which bounds the FP32 accumulation length.
+#include <hip/hip_runtime.h>
+
+#include "dm_common.h"
+#include "k_linalg.h"
+
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int ZX_BM = 64;   // complex rows per workgroup
+constexpr int ZX_BN = 64;   // right-hand sides per workgroup
+constexpr int ZX_BK = 16;   // complex columns of Z per LDS stage
+constexpr int ZX_KC = 4096; // columns per FP32 accumulation chunk
+
+// Xr, Xi: n x R (ldx) real planes of the right-hand sides (FP32)
+// part: [kchunk][R][n] complex128 partial products
+__global__ __launch_bounds__(256) void k_zx_mfma(i64 n, i64 ld, const float* __restrict__ Z, const float* __restrict__ Xr,
+                                                 const float* __restrict__ Xi, i64 ldx, int R, double2* __restrict__ part) {
+  __shared__ float As[ZX_BK][2 * ZX_BM + 4];
+  __shared__ float Br[ZX_BK][ZX_BN + 4];
+  __shared__ float Bi[ZX_BK][ZX_BN + 4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const i64 m0 = (i64)blockIdx.x * ZX_BM;   // complex row base
+  const int c0 = blockIdx.y * ZX_BN;        // rhs base
+  const i64 k_begin = (i64)blockIdx.z * ZX_KC;
+  const i64 k_end = k_begin + ZX_KC < n ? k_begin + ZX_KC : n;
+  v4f accP[2][4], accQ[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      accP[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+      accQ[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+    }
+  for (i64 k0 = k_begin; k0 < k_end; k0 += ZX_BK) {
+    // A' tile: ZX_BK columns x 128 real rows (columns of Z are contiguous)
+    for (int e = tid; e < ZX_BK * 2 * ZX_BM; e += 256) {
+      int kk = e / (2 * ZX_BM), rr = e % (2 * ZX_BM);
+      i64 col = k0 + kk, row2 = 2 * m0 + rr;
+      As[kk][rr] = (col < k_end && row2 < 2 * n) ? Z[col * 2 * ld + row2] : 0.f;
+    }
+    // X tiles: ZX_BK rows x 64 rhs
+    for (int e = tid; e < ZX_BK * ZX_BN; e += 256) {
+      int kk = e % ZX_BK, jj = e / ZX_BK;
+      i64 row = k0 + kk;
+      int col = c0 + jj;
+      bool ok = row < k_end && col < R;
+      Br[kk][jj] = ok ? Xr[row + (i64)col * ldx] : 0.f;
+      Bi[kk][jj] = ok ? Xi[row + (i64)col * ldx] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kq = 0; kq < ZX_BK / 4; ++kq) {
+      const int kk = 4 * kq + (lane >> 4);
+      float a[2];
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) a[rb] = As[kk][32 * wave + 16 * rb + (lane & 15)];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        float br = Br[kk][16 * cb + (lane & 15)];
+        float bi = Bi[kk][16 * cb + (lane & 15)];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+          accP[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb], br, accP[rb][cb], 0, 0, 0);
+          accQ[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb], bi, accQ[rb][cb], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // lane holds real rows 4*(lane/16) + r (r = 0..3) of each 16-row block and
+  // column lane%16: rows (0,1) and (2,3) are the (re, im) rows of two complex rows
+  double2* out = part + (i64)blockIdx.z * (i64)R * n;
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+      const int col = c0 + 16 * cb + (lane & 15);
+      if (col >= R) continue;
+      v4f p = accP[rb][cb], q = accQ[rb][cb];
+      const i64 real_row = 32 * wave + 16 * rb + 4 * (lane >> 4);  // even
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        i64 m = m0 + (real_row >> 1) + h;
+        if (m >= n) continue;
+        float pr = h ? p[2] : p[0], pi = h ? p[3] : p[1];
+        float qr = h ? q[2] : q[0], qi = h ? q[3] : q[1];
+        out[m + (i64)col * n] = make_double2((double)pr - (double)qi, (double)qr + (double)pi);
+      }
+    }
+}
+
+__global__ void k_zx_reduce(i64 n, int R, int nk, const double2* __restrict__ part, double2* __restrict__ Y, i64 ldy) {
+  i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (idx >= n * R) return;
+  i64 m = idx % n, j = idx / n;
+  double sr = 0, si = 0;
+  for (int c = 0; c < nk; ++c) {
+    double2 v = part[(i64)c * R * n + idx];
+    sr += v.x;
+    si += v.y;
+  }
+  Y[m + j * ldy] = make_double2(sr, si);
+}
+
+// complex128 n x R (ldx) -> FP32 real / imaginary planes (n x R)
+__global__ void k_split_planes(i64 n, int R, const double2* __restrict__ X, i64 ldx, float* __restrict__ Xr,
+                               float* __restrict__ Xi) {
+  i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (idx >= n * R) return;
+  i64 m = idx % n, j = idx / n;
+  double2 v = X[m + j * ldx];
+  Xr[idx] = (float)v.x;
+  Xi[idx] = (float)v.y;
+}
+
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// FP64 variant (complex128 Z, v_mfma_f64_16x16x4f64): the refinement residual
+// of many right-hand sides.  Same tiling; one K chunk (FP64 accumulation).
+__global__ __launch_bounds__(256) void k_zx_mfma64(i64 n, i64 ld, const double* __restrict__ Z,
+                                                   const double* __restrict__ Xr, const double* __restrict__ Xi, i64 ldx,
+                                                   int R, double2* __restrict__ Y, i64 ldy) {
+  __shared__ double As[ZX_BK][2 * ZX_BM + 2];
+  __shared__ double Br[ZX_BK][ZX_BN + 2];
+  __shared__ double Bi[ZX_BK][ZX_BN + 2];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const i64 m0 = (i64)blockIdx.x * ZX_BM;
+  const int c0 = blockIdx.y * ZX_BN;
+  v4d accP[2][4], accQ[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      accP[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+      accQ[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+    }
+  for (i64 k0 = 0; k0 < n; k0 += ZX_BK) {
+    for (int e = tid; e < ZX_BK * 2 * ZX_BM; e += 256) {
+      int kk = e / (2 * ZX_BM), rr = e % (2 * ZX_BM);
+      i64 col = k0 + kk, row2 = 2 * m0 + rr;
+      As[kk][rr] = (col < n && row2 < 2 * n) ? Z[col * 2 * ld + row2] : 0.0;
+    }
+    for (int e = tid; e < ZX_BK * ZX_BN; e += 256) {
+      int kk = e % ZX_BK, jj = e / ZX_BK;
+      i64 row = k0 + kk;
+      int col = c0 + jj;
+      bool ok = row < n && col < R;
+      Br[kk][jj] = ok ? Xr[row + (i64)col * ldx] : 0.0;
+      Bi[kk][jj] = ok ? Xi[row + (i64)col * ldx] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kq = 0; kq < ZX_BK / 4; ++kq) {
+      const int kk = 4 * kq + (lane >> 4);
+      double a[2];
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) a[rb] = As[kk][32 * wave + 16 * rb + (lane & 15)];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        double br = Br[kk][16 * cb + (lane & 15)];
+        double bi = Bi[kk][16 * cb + (lane & 15)];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+          accP[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], br, accP[rb][cb], 0, 0, 0);
+          accQ[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], bi, accQ[rb][cb], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+      const int col = c0 + 16 * cb + (lane & 15);
+      if (col >= R) continue;
+      v4d p = accP[rb][cb], q = accQ[rb][cb];
+      const i64 real_row = 32 * wave + 16 * rb + 4 * (lane >> 4);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        i64 m = m0 + (real_row >> 1) + h;
+        if (m >= n) continue;
+        double pr = h ? p[2] : p[0], pi = h ? p[3] : p[1];
+        double qr = h ? q[2] : q[0], qi = h ? q[3] : q[1];
+        Y[m + (i64)col * ldy] = make_double2(pr - qi, qr + pi);
+      }
+    }
+}
+
+__global__ void k_split_planes64(i64 n, int R, const double2* __restrict__ X, i64 ldx, double* __restrict__ Xr,
+                                 double* __restrict__ Xi) {
+  i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (idx >= n * R) return;
+  i64 m = idx % n, j = idx / n;
+  double2 v = X[m + j * ldx];
+  Xr[idx] = v.x;
+  Xi[idx] = v.y;
+}
+
+}  // namespace
+
+namespace dml {
+
+size_t zx_workspace_bytes(i64 n, int R) {
+  i64 nk = (n + ZX_KC - 1) / ZX_KC;
+  size_t a = (size_t)nk * R * n * sizeof(double2) + 2 * (size_t)n * R * sizeof(float);
+  size_t b = 2 * (size_t)n * R * sizeof(double);
+  return a > b ? a : b;
+}
+
+// Y = Z64 X with FP64 MFMA (complex128 everything).
+void zx_mfma64(i64 n, i64 ld, const double* z64, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
+               hipStream_t s) {
+  double* Xr = reinterpret_cast<double*>(ws);
+  double* Xi = Xr + n * R;
+  i64 tot = n * R;
+  k_split_planes64<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, R, reinterpret_cast<const double2*>(X), ldx, Xr,
+                                                                  Xi);
+  dim3 grid((unsigned)((n + ZX_BM - 1) / ZX_BM), (unsigned)((R + ZX_BN - 1) / ZX_BN), 1);
+  DM_REQUIRE(grid.y <= 65535, DM_ERR_ARGUMENT, "too many right-hand sides for one product");
+  k_zx_mfma64<<<grid, 256, 0, s>>>(n, ld, z64, Xr, Xi, n, R, reinterpret_cast<double2*>(Y), ldy);
+  HIP_OK(hipGetLastError());
+}
+
+// Y (n x R, ldy, complex128) = Z32 X (X: n x R, ldx, complex128).  ws: at
+// least zx_workspace_bytes(n, R).
+void zx_mfma(i64 n, i64 ld, const float* z32, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
+             hipStream_t s) {
+  DM_REQUIRE((ld & 1) == 0, DM_ERR_ARGUMENT, "complex64 matrix leading dimension must be even");
+  i64 nk = (n + ZX_KC - 1) / ZX_KC;
+  double2* part = reinterpret_cast<double2*>(ws);
+  float* Xr = reinterpret_cast<float*>(part + nk * R * n);
+  float* Xi = Xr + n * R;
+  i64 tot = n * R;
+  k_split_planes<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, R, reinterpret_cast<const double2*>(X), ldx, Xr, Xi);
+  dim3 grid((unsigned)((n + ZX_BM - 1) / ZX_BM), (unsigned)((R + ZX_BN - 1) / ZX_BN), (unsigned)nk);
+  DM_REQUIRE(grid.y <= 65535 && grid.z <= 65535, DM_ERR_ARGUMENT, "too many right-hand sides for one product");
+  k_zx_mfma<<<grid, 256, 0, s>>>(n, ld, z32, Xr, Xi, n, R, part);
+  k_zx_reduce<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, R, (int)nk, part, reinterpret_cast<double2*>(Y), ldy);
+  HIP_OK(hipGetLastError());
+}
+
+}  // namespace dml

@@ -129,6 +129,32 @@ def test_products_and_matrix_free(dm, O):
     assert np.linalg.norm(ya - Z.conj().T @ x) / np.linalg.norm(ya) < 1e-10
 
 
+@pytest.mark.parametrize("subdiv,nrhs", [(2, 8), (2, 70), (4, 72)])
+def test_multi_rhs_products_on_matrix_cores(dm, subdiv, nrhs):
+    """dm_apply with >= 8 right-hand sides runs Z X on MFMA (complex64: FP32
+    products, FP64 sum of 4096-column chunks; complex128: FP64 MFMA).  subdiv 4
+    (N = 7680) spans two K chunks; 70 and 72 leave partial 64-column blocks."""
+    m = dm.make_icosphere(0.1, subdiv)
+    r = dm.build_rwg(m)
+    Zd = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64 | dm.DM_Z32, device=True)
+    Z = Zd.to_numpy()
+    rng = np.random.default_rng(nrhs)
+    X = rng.standard_normal((r.nedges, nrhs)) + 1j * rng.standard_normal((r.nedges, nrhs))
+    ref = Z @ X
+    Y64 = Zd.matmul(X, form=dm.DM_Z64)
+    assert np.linalg.norm(Y64 - ref) / np.linalg.norm(ref) < 1e-12
+    Y32 = Zd.matmul(X, form=dm.DM_Z32)
+    assert np.linalg.norm(Y32 - ref) / np.linalg.norm(ref) < 1e-5
+    # every column, not only the aggregate, and against the one-vector GEMV path
+    col_err = np.linalg.norm(Y32 - ref, axis=0) / np.linalg.norm(ref, axis=0)
+    assert col_err.max() < 1e-5
+    y_gemv = Zd.matmul(X[:, -1], form=dm.DM_Z32)
+    assert np.linalg.norm(Y32[:, -1] - y_gemv) / np.linalg.norm(y_gemv) < 1e-5
+    Ya = Zd.matmul(X, form=dm.DM_Z64, adjoint=True)
+    refa = Z.conj().T @ X
+    assert np.linalg.norm(Ya - refa) / np.linalg.norm(refa) < 1e-12
+
+
 def test_excitation_parity(dm, O):
     for mk in (lambda: dm.make_rect_plate(0.1, 0.1, 6, 6), lambda: dm.make_icosphere(0.1, 2)):
         m = mk()
