@@ -79,7 +79,7 @@ class dm_solve_opts(C.Structure):
     _fields_ = [("method", C.c_int), ("precond", C.c_int), ("precond_side", C.c_int), ("maxiter", C.c_int),
                 ("tol", C.c_double), ("nf_cutoff", C.c_double), ("restart", C.c_int),
                 ("check_convergence", C.c_int), ("inner_tol", C.c_double), ("max_outer", C.c_int),
-                ("block_size", C.c_int)]
+                ("block_size", C.c_int), ("batch", C.c_int)]
 
 
 class dm_solve_stats(C.Structure):
@@ -595,7 +595,7 @@ class SolveStats:
 
 
 def _solve(Z, B, method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, check=True,
-           inner_tol=0.0, max_outer=10):
+           inner_tol=0.0, max_outer=10, batch=0):
     if not isinstance(Z, ZMatrix):
         raise TypeError("GPU solves take the device matrix (assemble_Z_efie(..., device=True))")
     B = np.asarray(B, dtype=np.complex128)
@@ -604,27 +604,7 @@ def _solve(Z, B, method, precond=None, side="right", tol=1e-6, maxiter=300, rest
     if Bf.shape[0] != Z.N:
         raise DimensionMismatch(f"rhs length {Bf.shape[0]} != {Z.N}")
     X = np.zeros_like(Bf, order="F")
-    o = dm_solve_opts()
-    lib().dm_solve_opts_default(C.byref(o))
-    o.method = method
-    o.tol = float(tol)
-    o.maxiter = int(maxiter)
-    o.restart = int(restart)
-    o.check_convergence = int(check)
-    o.precond_side = 1 if side == "right" else 0
-    o.inner_tol = float(inner_tol)
-    o.max_outer = int(max_outer)
-    if precond is None:
-        o.precond = DM_PRECOND_NONE
-    elif isinstance(precond, NearFieldPreconditionerData):
-        o.precond = {"diag": DM_PRECOND_DIAG, "block": DM_PRECOND_BLOCK}.get(precond.factorization,
-                                                                             DM_PRECOND_NEARFIELD)
-        o.block_size = precond.block_size
-        o.nf_cutoff = precond.cutoff
-    elif precond == "diag":
-        o.precond = DM_PRECOND_DIAG
-    else:
-        raise ValueError(f"unknown preconditioner {precond!r}")
+    o = _opts(method, precond, side, tol, maxiter, restart, check, inner_tol, max_outer, batch)
     st = dm_solve_stats()
     _check(lib().dm_solve(Z.ctx.h, Z.h, Bf.ctypes.data_as(C.c_void_p), Z.N, Bf.shape[1], C.byref(o),
                           X.ctypes.data_as(C.c_void_p), Z.N, C.byref(st)))
@@ -634,10 +614,13 @@ def _solve(Z, B, method, precond=None, side="right", tol=1e-6, maxiter=300, rest
 
 
 def _opts(method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, check=True, inner_tol=0.0,
-          max_outer=10):
+          max_outer=10, batch=0):
+    """dm_solve_opts.  batch: 0 = lockstep batches of right-hand sides when there
+    are >= 8 of them, 1 = always, -1 = one right-hand side at a time."""
     o = dm_solve_opts()
     lib().dm_solve_opts_default(C.byref(o))
     o.method, o.tol, o.maxiter, o.restart = method, float(tol), int(maxiter), int(restart)
+    o.batch = int(batch)
     o.check_convergence, o.inner_tol, o.max_outer = int(check), float(inner_tol), int(max_outer)
     o.precond_side = 1 if side == "right" else 0
     if precond is None or precond == "none":
@@ -654,7 +637,7 @@ def _opts(method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, 
 
 
 def solve_planewaves(mesh, rwg, k, excitations, method="mixed", preconditioner=None, tol=1e-6, maxiter=300,
-                     quad_order=3, eta0=ETA0, check_convergence=True, keep_matrix=False):
+                     quad_order=3, eta0=ETA0, check_convergence=True, keep_matrix=False, batch=0, restart=0):
     """End-to-end device solve for plane waves (the dense branches of
     solve_scattering, src/Workflow.jl:67-317): assembly, excitation,
     preconditioner and Krylov solve in one call.  method: "direct" (dense LU),
@@ -668,7 +651,7 @@ def solve_planewaves(mesh, rwg, k, excitations, method="mixed", preconditioner=N
     pol = np.ascontiguousarray(np.array([e.pol for e in excitations], float).reshape(M, 3))
     E0 = np.ascontiguousarray([float(e.E0) for e in excitations])
     meth = {"direct": DM_METHOD_LU, "fp64": DM_METHOD_GMRES, "mixed": DM_METHOD_MIXED}[method]
-    o = _opts(meth, preconditioner, "right", tol, maxiter, check=check_convergence)
+    o = _opts(meth, preconditioner, "right", tol, maxiter, restart=restart, check=check_convergence, batch=batch)
     X = np.zeros((rwg.nedges, M), dtype=np.complex128, order="F")
     st = dm_solve_stats()
     t_asm = C.c_double()
@@ -699,8 +682,10 @@ def solve_forward(Z, v, solver="direct", **kw):
 
 
 def solve_gmres(Z, rhs, preconditioner=None, precond_side="left", tol=1e-8, maxiter=200, memory=20,
-                verbose=False, check_gmres_convergence=True, restart=0):
-    """src/solver/IterativeSolve.jl:341-392 (full GMRES unless restart > 0)."""
+                verbose=False, check_gmres_convergence=True, restart=0, batch=0):
+    """src/solver/IterativeSolve.jl:341-392 (full GMRES unless restart > 0).
+    Several right-hand sides with right or no preconditioning run as lockstep
+    batches (one Z X product per step) unless batch=-1."""
     if not (math.isfinite(tol) and tol > 0):
         raise ValueError(f"GMRES tol must be finite and positive, got {tol}")
     if maxiter < 1:
@@ -710,15 +695,16 @@ def solve_gmres(Z, rhs, preconditioner=None, precond_side="left", tol=1e-8, maxi
     if precond_side not in ("left", "right"):
         raise ValueError(f"Invalid precond_side: {precond_side}")
     return _solve(Z, rhs, DM_METHOD_GMRES, preconditioner, precond_side, tol, maxiter, restart,
-                  check_gmres_convergence)
+                  check_gmres_convergence, batch=batch)
 
 
 def solve_mixed(Z, rhs, preconditioner=None, tol=1e-6, maxiter=300, inner_tol=0.0, max_outer=10,
-                check_convergence=True, restart=0):
+                check_convergence=True, restart=0, batch=0):
     """Mixed precision: FGMRES on the complex64 matrix with FP64 Arnoldi, FP64
-    iterative refinement on the complex128 matrix (no reference counterpart)."""
+    iterative refinement on the complex128 matrix (no reference counterpart).
+    Several right-hand sides run as lockstep batches (Z X on MFMA) unless batch=-1."""
     return _solve(Z, rhs, DM_METHOD_MIXED, preconditioner, "right", tol, maxiter, restart, check_convergence,
-                  inner_tol, max_outer)
+                  inner_tol, max_outer, batch)
 
 
 # --------------------------------------------------------------- far field

@@ -197,6 +197,9 @@ void spai_build(i64 n, i64 ld, const double* z64, const float* z32, const int* j
                 const int* iidx, double* vals, int* status, hipStream_t s);
 void csr_from_csc(i64 nnz, const int* map, const double* cscval, double* csrval, hipStream_t s);
 void spmv(i64 n, const int* rowptr, const int* colidx, const double* val, const double* x, double* y, hipStream_t s);
+// Y = S X for R columns of length n (leading dimension n)
+void spmm(i64 n, int R, const int* rowptr, const int* colidx, const double* val, const double* X, double* Y,
+          hipStream_t s);
 }  // namespace dmp
 
 // ------------------------------------------------------------ kernel launchers

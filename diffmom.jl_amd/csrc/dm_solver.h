@@ -1,0 +1,144 @@
+// dm_solver.h — device operators, preconditioners and the Krylov solvers behind
+// dm_solve / dm_solve_scattering_pw: one right-hand side at a time (GEMV
+// products) or a lockstep batch of right-hand sides (Z X products on MFMA).
+#pragma once
+#include <complex>
+#include <memory>
+#include <vector>
+
+#include "dm_common.h"
+#include "k_linalg.h"
+
+namespace dms {
+
+typedef std::complex<double> cd;
+
+double now();
+
+// Products switch from one GEMV per vector to the MFMA Z X kernel at this many
+// right-hand sides; batches hold at most kMfmaMaxRhs columns (bounds the
+// split-K workspace: 8 chunks x 256 x N complex128 at N = 30k).
+constexpr i64 kMfmaMinRhs = 8;
+constexpr i64 kMfmaMaxRhs = 256;
+
+// ------------------------------------------------------------ device operators
+// apply: y = A x (internal basis order, complex128 vectors of length n).
+// apply_batch: Y = A X for R columns (leading dimension n); ws holds at least
+// batch_ws(R) bytes.
+struct DevOp {
+  virtual ~DevOp() = default;
+  virtual void apply(const double* x, double* y, hipStream_t s) = 0;
+  virtual void apply_batch(const double* X, double* Y, int R, void* ws, hipStream_t s) = 0;
+  virtual size_t batch_ws(int R) const = 0;
+};
+struct OpZ64 : DevOp {
+  const dm_zmat* z;
+  explicit OpZ64(const dm_zmat* z_) : z(z_) {}
+  void apply(const double* x, double* y, hipStream_t s) override {
+    ProfScope p(z->ctx, "gemv_z64", s);
+    launch_gemv_z64(z->n, z->ld, z->z64.p, x, y, s);
+  }
+  void apply_batch(const double* X, double* Y, int R, void* ws, hipStream_t s) override {
+    if (R < kMfmaMinRhs) {
+      for (int j = 0; j < R; ++j) apply(X + 2 * z->n * j, Y + 2 * z->n * j, s);
+      return;
+    }
+    ProfScope p(z->ctx, "zx_z64", s);
+    dml::zx_mfma64(z->n, z->ld, z->z64.p, X, z->n, R, Y, z->n, ws, s);
+  }
+  size_t batch_ws(int R) const override { return dml::zx_workspace_bytes(z->n, R); }
+};
+struct OpZ32 : DevOp {
+  const dm_zmat* z;
+  explicit OpZ32(const dm_zmat* z_) : z(z_) {}
+  void apply(const double* x, double* y, hipStream_t s) override {
+    ProfScope p(z->ctx, "gemv_z32", s);
+    launch_gemv_z32(z->n, z->ld, z->z32.p, x, y, s);
+  }
+  void apply_batch(const double* X, double* Y, int R, void* ws, hipStream_t s) override {
+    if (R < kMfmaMinRhs) {
+      for (int j = 0; j < R; ++j) apply(X + 2 * z->n * j, Y + 2 * z->n * j, s);
+      return;
+    }
+    ProfScope p(z->ctx, "zx_z32", s);
+    dml::zx_mfma(z->n, z->ld, z->z32.p, X, z->n, R, Y, z->n, ws, s);
+  }
+  size_t batch_ws(int R) const override { return dml::zx_workspace_bytes(z->n, R); }
+};
+
+// y = M x; apply_batch: R columns of length n (leading dimension n)
+struct DevPrec {
+  i64 n = 0;
+  virtual ~DevPrec() = default;
+  virtual void apply(const double* x, double* y, hipStream_t s) = 0;
+  virtual void apply_batch(const double* X, double* Y, int R, hipStream_t s) {
+    for (int j = 0; j < R; ++j) apply(X + 2 * n * j, Y + 2 * n * j, s);
+  }
+};
+struct PrecDiag : DevPrec {
+  DBuf<double> d;
+  void apply(const double* x, double* y, hipStream_t s) override { dml::diag_apply(n, d.p, x, y, s); }
+};
+struct PrecSpai : DevPrec {
+  const SpaiPattern* pat;
+  DBuf<double> val;
+  void apply(const double* x, double* y, hipStream_t s) override {
+    dmp::spmv(n, pat->d_rowptr.p, pat->d_colidx.p, val.p, x, y, s);
+  }
+  void apply_batch(const double* X, double* Y, int R, hipStream_t s) override {
+    dmp::spmm(n, R, pat->d_rowptr.p, pat->d_colidx.p, val.p, X, Y, s);
+  }
+};
+struct PrecBlock : DevPrec {
+  int bs, nb;
+  DBuf<double> inv;
+  void apply(const double* x, double* y, hipStream_t s) override { dml::block_apply(n, bs, nb, inv.p, x, y, s); }
+};
+
+// Preconditioner of kind o.precond built from the complex64 (use32) or
+// complex128 stored form of z.  Returns null for DM_PRECOND_NONE.
+std::unique_ptr<DevPrec> build_precond(const dm_zmat* z, bool use32, const dm_solve_opts& o, hipStream_t s);
+
+// ------------------------------------------------------------------- GMRES
+struct Krylov {
+  i64 n = 0;
+  int m = 0;  // basis capacity
+  DBuf<double> V, Zs, w, t, hbuf, ybuf;
+  void reserve(i64 n_, int m_, bool flexible) {
+    n = n_;
+    m = m_;
+    V.alloc((size_t)2 * n * (m + 1));
+    if (flexible) Zs.alloc((size_t)2 * n * m);
+    w.alloc(2 * n);
+    t.alloc(2 * n);
+    hbuf.alloc(2 * (2 * (m + 1) + 1));
+    ybuf.alloc(2 * (m + 1));
+  }
+};
+
+struct GmresOut {
+  int iters = 0;
+  bool solved = false;
+  double est = 0;
+};
+
+GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double* b, double* x, double rtol,
+               int maxiter, int restart, Krylov& K, hipStream_t s);
+double true_residual(DevOp& A, i64 n, const double* b, const double* x, double* tmp, double* hbuf, hipStream_t s);
+double solve_one(dm_ctx* ctx, const dm_solve_opts& o, DevOp* A64, DevOp* A32, DevPrec* M, const double* b,
+                 double* x, i64 n, Krylov& K, dm_solve_stats& st, bool& ok, hipEvent_t z64_ready);
+
+// ------------------------------------------------------- lockstep batches
+// Solve Z X = B for R right-hand sides at once (device, internal basis order,
+// n x R with leading dimension n).  Every Krylov step applies the operator to
+// all R columns with one product; each column keeps its own Arnoldi basis,
+// Hessenberg matrix and stopping test, and drops out of the Arnoldi kernels
+// once converged.  Method GMRES: FGMRES (right preconditioning) on A64 (or A32
+// when A64 is null); MIXED: FGMRES on A32 with FP64 refinement against A64.
+// Per-column iteration counts and true relative residuals are returned in
+// iters / res; `ok` is set when every column met the tolerance.
+void solve_batch(dm_ctx* ctx, const dm_solve_opts& o, DevOp* A64, DevOp* A32, DevPrec* M, const double* B,
+                 double* X, i64 n, int R, std::vector<int>& iters, std::vector<double>& res, int& outer, bool& ok,
+                 hipEvent_t z64_ready);
+
+}  // namespace dms

@@ -1,0 +1,310 @@
+// dm_solver.hip — preconditioner set-up and the single-right-hand-side Krylov
+// solvers (GMRES / FGMRES with CGS2 Arnoldi, mixed-precision refinement).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <string>
+
+#include "dm_solver.h"
+
+namespace dms {
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+std::unique_ptr<DevPrec> build_precond(const dm_zmat* z, bool use32, const dm_solve_opts& o, hipStream_t s) {
+  i64 n = z->n;
+  if (o.precond == DM_PRECOND_NONE) return nullptr;
+  if (o.precond == DM_PRECOND_DIAG) {
+    auto p = std::make_unique<PrecDiag>();
+    p->n = n;
+    p->d.alloc(2 * n);
+    if (!use32) {
+      dml::diag_inv(n, z->ld, z->z64.p, p->d.p, s);
+    } else {
+      // widen the diagonal of Z32 through a one-block extraction per element
+      DBuf<double> tmp;
+      tmp.alloc(2 * n);
+      std::vector<i64> idx(n);
+      for (i64 i = 0; i < n; ++i) idx[i] = i;
+      DBuf<i64> di;
+      di.upload(idx, s);
+      std::vector<int> iden(n);
+      for (i64 i = 0; i < n; ++i) iden[i] = (int)i;
+      DBuf<int> dm;
+      dm.upload(iden, s);
+      dml::entries32(z->ld, n, dm.p, di.p, di.p, z->z32.p, tmp.p, s);
+      // d = 1 / diag
+      std::vector<double> h(2 * n);
+      HIP_OK(hipMemcpyAsync(h.data(), tmp.p, 16 * n, hipMemcpyDeviceToHost, s));
+      HIP_OK(hipStreamSynchronize(s));
+      for (i64 i = 0; i < n; ++i) {
+        cd v(h[2 * i], h[2 * i + 1]);
+        DM_REQUIRE(std::abs(v) > 0, DM_ERR_ARGUMENT, "zero diagonal entry in diagonal preconditioner");
+        cd iv = 1.0 / v;
+        h[2 * i] = iv.real();
+        h[2 * i + 1] = iv.imag();
+      }
+      p->d.upload(h, s);
+      HIP_OK(hipStreamSynchronize(s));
+    }
+    return p;
+  }
+  if (o.precond == DM_PRECOND_NEARFIELD) {
+    dm_rwg& r = const_cast<dm_rwg&>(*z->rwg);
+    double cutJ = o.nf_cutoff > 0 ? o.nf_cutoff : 2.5 * r.mean_len;
+    const SpaiPattern& P = dmh::spai_pattern(r, cutJ, dmp::spai_max_cols());
+    auto p = std::make_unique<PrecSpai>();
+    p->n = n;
+    p->pat = &P;
+    DBuf<double> cscv;
+    cscv.alloc(2 * P.nnz);
+    p->val.alloc(2 * P.nnz);
+    DBuf<int> st;
+    st.alloc(1);
+    HIP_OK(hipMemsetAsync(st.p, 0, sizeof(int), s));
+    dmp::spai_build(n, z->ld, use32 ? nullptr : z->z64.p, use32 ? z->z32.p : nullptr, P.d_jptr.p, P.d_jidx.p,
+                    P.d_iptr.p, P.d_iidx.p, cscv.p, st.p, s);
+    dmp::csr_from_csc(P.nnz, P.d_map.p, cscv.p, p->val.p, s);
+    int sv = 0;
+    HIP_OK(hipMemcpyAsync(&sv, st.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    DM_REQUIRE(sv == 0, DM_ERR_ARGUMENT,
+               "near-field preconditioner: normal equations of column " + std::to_string(sv) + " are not positive definite");
+    return p;
+  }
+  if (o.precond == DM_PRECOND_BLOCK) {
+    auto p = std::make_unique<PrecBlock>();
+    p->n = n;
+    p->bs = o.block_size > 0 ? o.block_size : 128;
+    DM_REQUIRE(p->bs <= 1024, DM_ERR_ARGUMENT, "near-field block size must be <= 1024");
+    p->bs = (int)std::min<i64>(p->bs, n);
+    p->nb = (int)((n + p->bs - 1) / p->bs);
+    p->inv.alloc((size_t)p->nb * p->bs * p->bs * 2);
+    if (use32)
+      dml::extract_blocks32(n, z->ld, p->bs, p->nb, z->z32.p, p->inv.p, s);
+    else
+      dml::extract_blocks64(n, z->ld, p->bs, p->nb, z->z64.p, p->inv.p, s);
+    DBuf<int> st;
+    st.alloc(p->nb);
+    HIP_OK(hipMemsetAsync(st.p, 0, sizeof(int) * p->nb, s));
+    dml::block_inverse(p->bs, p->nb, p->inv.p, st.p, s);
+    std::vector<int> hs(p->nb);
+    HIP_OK(hipMemcpyAsync(hs.data(), st.p, sizeof(int) * p->nb, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    for (int b = 0; b < p->nb; ++b)
+      DM_REQUIRE(hs[b] == 0, DM_ERR_ARGUMENT, "near-field block " + std::to_string(b) + " is singular");
+    return p;
+  }
+  dm_throw(DM_ERR_ARGUMENT, "preconditioner kind not available for this solve");
+}
+
+// Restarted (restart = 0: full) GMRES with classical Gram-Schmidt applied
+// twice (CGS2) for the Arnoldi step, FP64 throughout; right preconditioning is
+// flexible (FGMRES: the preconditioned directions are stored), left
+// preconditioning follows Krylov.jl's semantics (stop on ||M r|| <= rtol ||M b||).
+// x holds the initial guess on entry.
+GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double* b, double* x, double rtol,
+               int maxiter, int restart, Krylov& K, hipStream_t s) {
+  GmresOut out;
+  int mcap = restart > 0 ? std::min(restart, maxiter) : maxiter;
+  bool right = M && side == 1;
+  bool left = M && side == 0;
+  K.reserve(n, mcap, right);
+  double* V = K.V.p;
+  const i64 ldv = 2 * n;  // doubles per column
+  auto col = [&](double* base, int j) { return base + (size_t)ldv * j; };
+  // eps from the initial (preconditioned) residual with x0
+  auto residual = [&](double* r) {
+    A.apply(x, K.t.p, s);
+    HIP_OK(hipMemcpyAsync(r, b, 16 * n, hipMemcpyDeviceToDevice, s));
+    dml::axpby(n, -1, 0, K.t.p, 1, 0, r, s);
+    if (left) {
+      M->apply(r, K.t.p, s);
+      HIP_OK(hipMemcpyAsync(r, K.t.p, 16 * n, hipMemcpyDeviceToDevice, s));
+    }
+  };
+  auto nrm = [&](const double* v) {
+    double h;
+    dml::norm2(n, v, K.hbuf.p, s);
+    HIP_OK(hipMemcpyAsync(&h, K.hbuf.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    return std::sqrt(h);
+  };
+  double* r = col(V, 0);
+  residual(r);
+  double beta = nrm(r);
+  double eps = rtol * beta;
+  if (!(beta > 0)) {
+    out.solved = true;
+    return out;
+  }
+  std::vector<cd> H((size_t)(mcap + 1) * mcap), g(mcap + 1), sn(mcap);
+  std::vector<double> cs(mcap);
+  std::vector<double> hh(2 * (2 * (mcap + 1) + 1));
+  int total = 0;
+  while (true) {
+    dml::axpby(n, 1.0 / beta, 0, r, 0, 0, r, s);
+    std::fill(g.begin(), g.end(), cd(0));
+    g[0] = beta;
+    int j = 0;
+    double est = beta;
+    for (; j < mcap && total < maxiter; ++j) {
+      double* vj = col(V, j);
+      if (right) {
+        double* zj = col(K.Zs.p, j);
+        {
+          ProfScope p(ctx, "precond", s);
+          M->apply(vj, zj, s);
+        }
+        A.apply(zj, K.w.p, s);
+      } else if (left) {
+        A.apply(vj, K.t.p, s);
+        ProfScope p(ctx, "precond", s);
+        M->apply(K.t.p, K.w.p, s);
+      } else {
+        A.apply(vj, K.w.p, s);
+      }
+      int k = j + 1;
+      double* h1 = K.hbuf.p;
+      double* h2 = K.hbuf.p + 2 * k;
+      {
+        ProfScope p(ctx, "arnoldi", s);
+        dml::dots(n, ldv / 2, k, V, K.w.p, h1, s);
+        dml::update(n, ldv / 2, k, V, h1, K.w.p, s);
+        dml::dots(n, ldv / 2, k, V, K.w.p, h2, s);
+        dml::update(n, ldv / 2, k, V, h2, K.w.p, s);
+        dml::norm2(n, K.w.p, K.hbuf.p + 4 * k, s);
+      }
+      HIP_OK(hipMemcpyAsync(hh.data(), K.hbuf.p, sizeof(double) * (4 * k + 1), hipMemcpyDeviceToHost, s));
+      HIP_OK(hipStreamSynchronize(s));
+      cd* Hj = &H[(size_t)j * (mcap + 1)];
+      for (int i = 0; i < k; ++i) Hj[i] = cd(hh[2 * i] + hh[2 * k + 2 * i], hh[2 * i + 1] + hh[2 * k + 2 * i + 1]);
+      double hn = std::sqrt(hh[4 * k]);
+      Hj[k] = hn;
+      if (hn > 0 && k <= mcap) dml::axpby(n, 1.0 / hn, 0, K.w.p, 0, 0, col(V, k), s);
+      for (int i = 0; i < j; ++i) {
+        cd t = cs[i] * Hj[i] + sn[i] * Hj[i + 1];
+        Hj[i + 1] = -std::conj(sn[i]) * Hj[i] + cs[i] * Hj[i + 1];
+        Hj[i] = t;
+      }
+      cd a = Hj[j], bb = Hj[j + 1];
+      double rho = std::hypot(std::abs(a), std::abs(bb));
+      if (std::abs(a) == 0) {
+        cs[j] = 0;
+        sn[j] = std::abs(bb) > 0 ? std::conj(bb) / std::abs(bb) : cd(1);
+      } else {
+        cs[j] = std::abs(a) / rho;
+        sn[j] = (a / std::abs(a)) * std::conj(bb) / rho;
+      }
+      Hj[j] = cs[j] * a + sn[j] * bb;
+      Hj[j + 1] = 0;
+      g[j + 1] = -std::conj(sn[j]) * g[j];
+      g[j] = cs[j] * g[j];
+      est = std::abs(g[j + 1]);
+      ++total;
+      if (est <= eps || hn == 0) {
+        ++j;
+        break;
+      }
+    }
+    // y = H^-1 g, x += V y (or Zs y)
+    std::vector<cd> y(j);
+    for (int i = j - 1; i >= 0; --i) {
+      cd acc = g[i];
+      for (int l = i + 1; l < j; ++l) acc -= H[(size_t)l * (mcap + 1) + i] * y[l];
+      y[i] = acc / H[(size_t)i * (mcap + 1) + i];
+    }
+    if (j > 0) {
+      HIP_OK(hipMemcpyAsync(K.ybuf.p, y.data(), 16 * j, hipMemcpyHostToDevice, s));
+      dml::combine(n, ldv / 2, j, right ? K.Zs.p : V, K.ybuf.p, x, s);
+      HIP_OK(hipStreamSynchronize(s));
+    }
+    out.iters = total;
+    out.est = est;
+    if (est <= eps) {
+      out.solved = true;
+      break;
+    }
+    if (total >= maxiter) break;
+    // restart from the true (preconditioned) residual
+    residual(r);
+    beta = nrm(r);
+    if (beta <= eps) {
+      out.solved = true;
+      out.est = beta;
+      break;
+    }
+  }
+  return out;
+}
+
+double true_residual(DevOp& A, i64 n, const double* b, const double* x, double* tmp, double* hbuf, hipStream_t s) {
+  A.apply(x, tmp, s);
+  dml::axpby(n, 1, 0, b, -1, 0, tmp, s);
+  double h[2];
+  dml::norm2(n, tmp, hbuf, s);
+  dml::norm2(n, b, hbuf + 1, s);
+  HIP_OK(hipMemcpyAsync(h, hbuf, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  if (h[1] == 0) return h[0] == 0 ? 0.0 : INFINITY;
+  return std::sqrt(h[0] / h[1]);
+}
+
+// One right-hand side on the device (internal basis order): FP64 GMRES on the
+// matrix `A`, or (mixed) FGMRES on the complex64 matrix with FP64 Arnoldi and
+// FP64 iterative refinement r = b - Z64 x.  `z64_ready`, when given, is waited
+// on before the first complex128 product (the FP64 matrix may still be in
+// assembly on the second stream).  Returns the true relative residual.
+double solve_one(dm_ctx* ctx, const dm_solve_opts& o, DevOp* A64, DevOp* A32, DevPrec* M, const double* b,
+                 double* x, i64 n, Krylov& K, dm_solve_stats& st, bool& ok, hipEvent_t z64_ready) {
+  hipStream_t s = ctx->stream;
+  DBuf<double> tmp, hb;
+  tmp.alloc(2 * n);
+  hb.alloc(4);
+  HIP_OK(hipMemsetAsync(x, 0, 16 * n, s));
+  if (o.method == DM_METHOD_GMRES) {
+    DevOp& A = A64 ? *A64 : *A32;
+    if (A64 && z64_ready) HIP_OK(hipStreamWaitEvent(s, z64_ready, 0));
+    GmresOut g = gmres(ctx, n, A, M, o.precond_side, b, x, o.tol, o.maxiter, o.restart, K, s);
+    st.iterations += g.iters;
+    ok = g.solved;
+    return true_residual(A, n, b, x, tmp.p, hb.p, s);
+  }
+  DBuf<double> rr, d;
+  rr.alloc(2 * n);
+  d.alloc(2 * n);
+  HIP_OK(hipMemcpyAsync(rr.p, b, 16 * n, hipMemcpyDeviceToDevice, s));
+  double rel = 1.0;
+  double tau = o.inner_tol > 0 ? o.inner_tol : 0.5 * o.tol;
+  int left = o.maxiter;
+  bool waited = false;
+  for (int outer = 0; outer < std::max(1, o.max_outer) && left > 0; ++outer) {
+    HIP_OK(hipMemsetAsync(d.p, 0, 16 * n, s));
+    GmresOut g = gmres(ctx, n, *A32, M, 1, rr.p, d.p, tau, left, o.restart, K, s);
+    st.iterations += g.iters;
+    left -= g.iters;
+    st.outer_iterations += 1;
+    dml::axpby(n, 1, 0, d.p, 1, 0, x, s);
+    if (z64_ready && !waited) {
+      HIP_OK(hipStreamWaitEvent(s, z64_ready, 0));
+      waited = true;
+    }
+    A64->apply(x, tmp.p, s);  // rr = b - Z64 x
+    HIP_OK(hipMemcpyAsync(rr.p, b, 16 * n, hipMemcpyDeviceToDevice, s));
+    dml::axpby(n, -1, 0, tmp.p, 1, 0, rr.p, s);
+    double h[2];
+    dml::norm2(n, rr.p, hb.p, s);
+    dml::norm2(n, b, hb.p + 1, s);
+    HIP_OK(hipMemcpyAsync(h, hb.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    rel = h[1] > 0 ? std::sqrt(h[0] / h[1]) : 0.0;
+    if (rel <= o.tol) break;
+    tau = std::min(0.5, std::max(1e-12, 0.5 * o.tol / rel));
+  }
+  ok = rel <= o.tol;
+  return rel;
+}
+
+}  // namespace dms

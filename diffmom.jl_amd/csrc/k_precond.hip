@@ -142,6 +142,9 @@ __global__ __launch_bounds__(256) void k_spai_build(i64 ld, const ZT* __restrict
 __global__ __launch_bounds__(256) void k_spmv(i64 n, const int* __restrict__ rowptr, const int* __restrict__ colidx,
                                               const double2* __restrict__ val, const double2* __restrict__ x,
                                               double2* __restrict__ y) {
+  // blockIdx.y selects one of several right-hand sides (columns of length n)
+  x += (i64)blockIdx.y * n;
+  y += (i64)blockIdx.y * n;
   i64 r = blockIdx.x * (i64)blockDim.x + threadIdx.x;
   if (r >= n) return;
   double ar = 0, ai = 0;
@@ -185,6 +188,16 @@ void csr_from_csc(i64 nnz, const int* map, const double* cscval, double* csrval,
 void spmv(i64 n, const int* rowptr, const int* colidx, const double* val, const double* x, double* y, hipStream_t s) {
   k_spmv<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, rowptr, colidx, reinterpret_cast<const double2*>(val),
                                                       reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y));
+  HIP_OK(hipGetLastError());
+}
+
+void spmm(i64 n, int R, const int* rowptr, const int* colidx, const double* val, const double* X, double* Y,
+          hipStream_t s) {
+  if (R <= 0) return;
+  DM_REQUIRE(R <= 65535, DM_ERR_ARGUMENT, "too many right-hand sides for one sparse product");
+  dim3 g((unsigned)((n + 255) / 256), (unsigned)R);
+  k_spmv<<<g, 256, 0, s>>>(n, rowptr, colidx, reinterpret_cast<const double2*>(val),
+                           reinterpret_cast<const double2*>(X), reinterpret_cast<double2*>(Y));
   HIP_OK(hipGetLastError());
 }
 

@@ -164,6 +164,8 @@ typedef struct dm_solve_opts {
   double inner_tol;    /* MIXED: relative tolerance of the first inner solve (<= 0: tol/2) */
   int max_outer;       /* MIXED: refinement steps allowed */
   int block_size;      /* DM_PRECOND_BLOCK: bases per spatial block (0: 128) */
+  int batch;           /* several right-hand sides: 0 = lockstep batches when nrhs >= 8,
+                          1 = always, -1 = one right-hand side at a time */
 } dm_solve_opts;
 
 typedef struct dm_solve_stats {

@@ -187,3 +187,97 @@ def test_end_to_end_planewaves(dm, O, method):
     assert st.solved and t_asm > 0
     if method == "mixed":
         assert st.outer >= 1
+
+
+def _random_planewaves(dm, rng, k, count):
+    """Directions normalised N(0, I3) draws, polarisations random unit vectors
+    projected perpendicular to k (the many-angle workload's recipe)."""
+    out = []
+    for _ in range(count):
+        d = rng.standard_normal(3)
+        d /= np.linalg.norm(d)
+        p = rng.standard_normal(3)
+        p -= d * (p @ d)
+        p /= np.linalg.norm(p)
+        out.append(dm.make_plane_wave(k * d, 1.0, p))
+    return out
+
+
+@pytest.fixture(scope="module")
+def sphere2(dm):
+    m = dm.make_icosphere(0.1, 2)
+    r = dm.build_rwg(m)
+    Z = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64 | dm.DM_Z32, device=True)
+    rng = np.random.default_rng(20261202)
+    pws = _random_planewaves(dm, rng, K0, 12)
+    V = dm.assemble_multiple_excitations(m, r, pws)
+    return m, r, Z, V
+
+
+@pytest.mark.parametrize("method", ["gmres", "mixed"])
+@pytest.mark.parametrize("precond", [None, "nf"])
+def test_batched_solve_matches_per_column(dm, sphere2, method, precond):
+    """Lockstep batch (one Z X product per step, per-column Arnoldi/Givens) vs
+    the one-column solver: same true residual bar per column, same currents to
+    the solve tolerance, and iteration counts that agree column for column up
+    to rounding (the batch reduces dot products in another order)."""
+    m, r, Z, V = sphere2
+    P = dm.build_nearfield_preconditioner(Z, m, r, 2.0 * float(np.mean(r.len))) if precond else None
+    tol = 1e-8 if method == "gmres" else 1e-7
+    f = dm.solve_gmres if method == "gmres" else dm.solve_mixed
+    kw = dict(preconditioner=P, tol=tol, maxiter=300)
+    if method == "gmres":
+        kw["precond_side"] = "right"
+    Xb, sb = f(Z, V, batch=1, **kw)
+    Xs, ss = f(Z, V, batch=-1, **kw)
+    Zh = Z.to_numpy()
+    res = np.linalg.norm(Zh @ Xb - V, axis=0) / np.linalg.norm(V, axis=0)
+    bar = tol * (1.05 if method == "gmres" else 1.0)
+    assert sb.solved and res.max() <= max(bar, sb.residual * 1.01)
+    assert sb.residual <= (tol if method == "mixed" else 10 * tol)
+    assert np.linalg.norm(Xb - Xs) / np.linalg.norm(Xs) < 1e-5
+    # the batch's complex64 products use FP32 inputs and accumulation (4096-term
+    # chunks), the one-column GEMV FP64 accumulation: the inner solves differ by
+    # rounding, so mixed-precision counts may drift slightly more
+    slack = 0.03 if method == "gmres" else 0.10
+    assert abs(sb.niter - ss.niter) <= max(2, slack * ss.niter)
+    Id = np.linalg.solve(Zh, V)
+    assert np.linalg.norm(Xb - Id) / np.linalg.norm(Id) < 1e-4
+
+
+def test_batched_restart_single_column_and_nonconvergence(dm, sphere2):
+    m, r, Z, V = sphere2
+    Zh = Z.to_numpy()
+    Id = np.linalg.solve(Zh, V[:, :9])
+    X, st = dm.solve_gmres(Z, V[:, :9], precond_side="right", tol=1e-8, maxiter=600, restart=25)
+    assert st.solved and np.linalg.norm(X - Id) / np.linalg.norm(Id) < 1e-6
+    # a batch of one reproduces the one-column solver
+    x1, s1 = dm.solve_gmres(Z, V[:, 0], precond_side="right", tol=1e-8, maxiter=300, batch=1)
+    x0, s0 = dm.solve_gmres(Z, V[:, 0], precond_side="right", tol=1e-8, maxiter=300, batch=-1)
+    assert abs(s1.niter - s0.niter) <= 1
+    assert np.linalg.norm(x1 - x0) / np.linalg.norm(x0) < 1e-7
+    with pytest.raises(dm.ConvergenceError):
+        dm.solve_gmres(Z, V, precond_side="right", tol=1e-12, maxiter=3)
+    X, st = dm.solve_gmres(Z, V, precond_side="right", tol=1e-12, maxiter=3, check_gmres_convergence=False)
+    assert not st.solved and st.niter == 3 * V.shape[1]
+
+
+def test_batched_planewaves_end_to_end(dm):
+    """64 plane waves on a 1920-unknown sphere in one call: mixed precision,
+    SPAI, lockstep batches; every column's true FP64 residual <= 1e-6 and the
+    currents match the dense direct solve."""
+    m = dm.make_icosphere(0.1, 3)
+    r = dm.build_rwg(m)
+    rng = np.random.default_rng(7)
+    pws = _random_planewaves(dm, rng, K0, 64)
+    P = dm.build_nearfield_preconditioner(None, m, r, 2.0 * float(np.mean(r.len)))
+    X, st, t_asm, Z = dm.solve_planewaves(m, r, K0, pws, "mixed", P, tol=1e-6, maxiter=300, keep_matrix=True)
+    assert st.solved and st.residual <= 1e-6
+    Zh = Z.to_numpy()
+    V = dm.assemble_multiple_excitations(m, r, pws)
+    res = np.linalg.norm(Zh @ X - V, axis=0) / np.linalg.norm(V, axis=0)
+    assert res.max() <= 1e-6 * 1.001
+    Id = np.linalg.solve(Zh, V)
+    assert np.linalg.norm(X - Id) / np.linalg.norm(Id) < 1e-4
+    Xs, ss, _ = dm.solve_planewaves(m, r, K0, pws[:3], "mixed", P, tol=1e-6, maxiter=300, batch=-1)
+    assert np.linalg.norm(Xs - X[:, :3]) / np.linalg.norm(X[:, :3]) < 1e-4
