@@ -182,21 +182,28 @@ __global__ __launch_bounds__(256) void k_zx_mfma64(i64 n, i64 ld, const double* 
     }
     __syncthreads();
   }
+  // FP64 MFMA output layout: lane l holds real rows (l / 16) + 4 r (r = 0..3)
+  // of each 16-row block, column l % 16.  The (re, im) rows 2m, 2m + 1 of one
+  // complex row sit in lanes l and l ^ 16: swap, then the even lane writes.
+  const int g = lane >> 4;
 #pragma unroll
   for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) {
       const int col = c0 + 16 * cb + (lane & 15);
-      if (col >= R) continue;
       v4d p = accP[rb][cb], q = accQ[rb][cb];
-      const i64 real_row = 32 * wave + 16 * rb + 4 * (lane >> 4);
+      double po[4], qo[4];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        i64 m = m0 + (real_row >> 1) + h;
+      for (int r = 0; r < 4; ++r) {
+        po[r] = __shfl_xor(p[r], 16, 64);
+        qo[r] = __shfl_xor(q[r], 16, 64);
+      }
+      if (col >= R || (g & 1)) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const i64 m = m0 + ((32 * wave + 16 * rb + g + 4 * r) >> 1);
         if (m >= n) continue;
-        double pr = h ? p[2] : p[0], pi = h ? p[3] : p[1];
-        double qr = h ? q[2] : q[0], qi = h ? q[3] : q[1];
-        Y[m + (i64)col * ldy] = make_double2(pr - qi, qr + pi);
+        Y[m + (i64)col * ldy] = make_double2(p[r] - qo[r], q[r] + po[r]);
       }
     }
 }
