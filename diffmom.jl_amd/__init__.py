@@ -43,6 +43,8 @@ LIB_PATH = os.path.join(_HERE, "libdiffmom.so")
 DM_OK, DM_ERR_ARGUMENT, DM_ERR_OVERFLOW, DM_ERR_DIMENSION = 0, 1, 2, 3
 DM_ERR_CONVERGENCE, DM_ERR_DEVICE, DM_ERR_MEMORY, DM_ERR_DOMAIN = 4, 5, 6, 7
 DM_Z64, DM_Z32 = 1, 2
+# packed symmetric storage of the same matrices (upper triangle of 128 x 128 tiles)
+DM_Z64P, DM_Z32P = 4, 8
 DM_METHOD_LU, DM_METHOD_GMRES, DM_METHOD_MIXED = 0, 1, 2
 DM_PRECOND_NONE, DM_PRECOND_DIAG, DM_PRECOND_NEARFIELD, DM_PRECOND_LEARNED, DM_PRECOND_BLOCK = 0, 1, 2, 3, 4
 
@@ -79,7 +81,7 @@ class dm_solve_opts(C.Structure):
     _fields_ = [("method", C.c_int), ("precond", C.c_int), ("precond_side", C.c_int), ("maxiter", C.c_int),
                 ("tol", C.c_double), ("nf_cutoff", C.c_double), ("restart", C.c_int),
                 ("check_convergence", C.c_int), ("inner_tol", C.c_double), ("max_outer", C.c_int),
-                ("block_size", C.c_int), ("batch", C.c_int)]
+                ("block_size", C.c_int), ("batch", C.c_int), ("learned", C.c_void_p)]
 
 
 class dm_solve_stats(C.Structure):
@@ -126,6 +128,14 @@ def lib():
         "dm_precond_free": ([vp], i32),
         "dm_profile_enable": ([vp, i32], i32), "dm_profile_reset": ([vp], i32),
         "dm_profile_read": ([vp, C.c_char_p, C.POINTER(i64), C.POINTER(d)], i32),
+        "dm_mpnn_param_count": ([], i64), "dm_mpnn_create": ([vp, vp, pp], i32),
+        "dm_mpnn_get_params": ([vp, vp], i32), "dm_mpnn_free": ([vp], i32),
+        "dm_rwg_graph": ([vp, d, vp, vp, vp], i32),
+        "dm_mpnn_data_create": ([vp, i64, vp, vp, vp, vp, vp, vp, i64, vp, pp], i32),
+        "dm_mpnn_data_free": ([vp], i32),
+        "dm_mpnn_loss_grad": ([vp, vp, vp, i64, i64, C.POINTER(d), vp], i32),
+        "dm_mpnn_train": ([vp, vp, vp, i32, i64, d, C.c_uint64, vp], i32),
+        "dm_mpnn_apply": ([vp, vp, vp, d, vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -389,7 +399,12 @@ class ZMatrix:
         return (self.N, self.N)
 
     def _form(self, form):
-        return form if form is not None else (DM_Z64 if self.forms & DM_Z64 else DM_Z32)
+        if form is not None:
+            return form
+        for f in (DM_Z64, DM_Z64P, DM_Z32, DM_Z32P):
+            if self.forms & f:
+                return f
+        raise ValueError("matrix holds no stored form")
 
     def to_numpy(self, form=None):
         out = np.zeros((self.N, self.N), dtype=np.complex128, order="F")
@@ -631,6 +646,12 @@ def _opts(method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, 
         o.block_size, o.nf_cutoff = precond.block_size, precond.cutoff
     elif precond == "diag":
         o.precond = DM_PRECOND_DIAG
+    elif isinstance(precond, LearnedPreconditioner):
+        if side != "right":
+            raise ValueError("the learned preconditioner is nonlinear: use precond_side='right'")
+        o.precond = DM_PRECOND_LEARNED
+        o.nf_cutoff = precond.cutoff
+        o.learned = precond.net.h
     else:
         raise ValueError(f"unknown preconditioner {precond!r}")
     return o
@@ -663,8 +684,10 @@ def solve_planewaves(mesh, rwg, k, excitations, method="mixed", preconditioner=N
     stats = SolveStats(st.iterations, bool(st.solved), st.residual, st.outer_iterations, st.precond_time_s,
                        st.solve_time_s)
     if keep_matrix:
-        forms = DM_Z64 | DM_Z32 if method == "mixed" else DM_Z64
-        return X, stats, t_asm.value, ZMatrix(rwg, zh, forms, complex(k), float(eta0), quad_order)
+        n = C.c_int64()
+        forms = C.c_int()
+        _check(lib().dm_zmat_size(zh, C.byref(n), C.byref(forms)))
+        return X, stats, t_asm.value, ZMatrix(rwg, zh, forms.value, complex(k), float(eta0), quad_order)
     return X, stats, t_asm.value
 
 
@@ -864,3 +887,190 @@ def solve_scattering(mesh, freq_hz, excitation, method="auto", dense_direct_limi
     t_pre = st.precond_time_s
     t_solve = t_total - t_asm - t_pre
     return ScatteringResult(I, sel, N, t_asm, t_solve, t_pre, iters, resid, report, warnings)
+
+
+# ------------------------------------------------------ learned preconditioner
+# No reference counterpart.  A 4-layer message-passing network over the RWG
+# edge-adjacency graph corrects the near-field SPAI inside FGMRES:
+# M r = S r + network(S r, r).  Training runs on the GPU (forward, backward and
+# Adam in the library) on (residual, error) pairs from the library's own GMRES
+# solves; convergence is always judged by the true FP64 residual.
+
+def mpnn_param_count():
+    return int(lib().dm_mpnn_param_count())
+
+
+def mpnn_init_params(seed=0):
+    """Initial parameters: W_e ~ N(0, 1/6), W_s, W_n ~ N(0, 1/(4*64)), b = 0 and
+    W_d = 0 (so an untrained network leaves the SPAI output unchanged)."""
+    rng = np.random.default_rng(seed)
+    H, F, Ls = 64, 6, 4
+    parts = [rng.standard_normal(H * F) / math.sqrt(F)]
+    for _ in range(Ls):
+        parts += [0.5 * rng.standard_normal(H * H) / math.sqrt(H), 0.5 * rng.standard_normal(H * H) / math.sqrt(H),
+                  np.zeros(H)]
+    parts.append(np.zeros(2 * H))
+    out = np.concatenate(parts).astype(np.float32)
+    assert out.size == mpnn_param_count()
+    return out
+
+
+def rwg_graph(rwg, k):
+    """(nbr (N, 4) int32 0-based caller indices with -1 padding, sgn (N, 4)
+    float32, geom (N, 2) float32) of the network for one mesh."""
+    N = rwg.nedges
+    nbr = np.zeros((N, 4), np.int32)
+    sgn = np.zeros((N, 4), np.float32)
+    geom = np.zeros((N, 2), np.float32)
+    _check(lib().dm_rwg_graph(rwg.h, float(k), _ptr(nbr), _ptr(sgn), _ptr(geom)))
+    return nbr, sgn, geom
+
+
+class MPNN:
+    """The network on the device (FP32 master parameters, bfloat16 copy used by
+    the kernels, Adam state)."""
+
+    def __init__(self, params=None, ctx=None, seed=0):
+        self.ctx = ctx or default_context()
+        p = mpnn_init_params(seed) if params is None else np.ascontiguousarray(params, np.float32)
+        if p.shape != (mpnn_param_count(),):
+            raise ValueError(f"expected {mpnn_param_count()} parameters, got {p.shape}")
+        h = C.c_void_p()
+        _check(lib().dm_mpnn_create(self.ctx.h, _ptr(p), C.byref(h)))
+        self.h = h
+
+    def params(self):
+        out = np.zeros(mpnn_param_count(), np.float32)
+        _check(lib().dm_mpnn_get_params(self.h, _ptr(out)))
+        return out
+
+    def loss_grad(self, data, s0=0, s1=None):
+        """Mean loss over samples [s0, s1) of `data` and its gradient."""
+        s1 = data.nsample if s1 is None else s1
+        loss = C.c_double()
+        g = np.zeros(mpnn_param_count(), np.float32)
+        _check(lib().dm_mpnn_loss_grad(self.ctx.h, self.h, data.h, int(s0), int(s1), C.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def train(self, data, epochs=10, batch_samples=32, lr=1e-3, seed=0):
+        """Adam over shuffled mini-batches; returns the mean loss of each epoch."""
+        hist = np.zeros(int(epochs))
+        _check(lib().dm_mpnn_train(self.ctx.h, self.h, data.h, int(epochs), int(batch_samples), float(lr),
+                                   int(seed), _ptr(hist)))
+        return hist
+
+    def apply(self, rwg, k, z, r):
+        """z + network(z, r) for one mesh (caller order)."""
+        z = np.ascontiguousarray(z, np.complex128)
+        r = np.ascontiguousarray(r, np.complex128)
+        out = np.zeros_like(z)
+        _check(lib().dm_mpnn_apply(self.ctx.h, self.h, rwg.h, float(k), _ptr(z), _ptr(r), _ptr(out)))
+        return out
+
+    def save(self, path):
+        np.savez(path, params=self.params())
+
+    @classmethod
+    def load(cls, path, ctx=None):
+        with np.load(path, allow_pickle=False) as f:
+            return cls(f["params"], ctx=ctx)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().dm_mpnn_free(self.h)
+            self.h = None
+
+
+class MPNNData:
+    """Training samples on the device.  Each sample is a dict with z, r, e
+    (complex, N) and nbr, sgn, geom of its mesh (rwg_graph)."""
+
+    def __init__(self, samples, ctx=None):
+        self.ctx = ctx or default_context()
+        sizes = [len(s["z"]) for s in samples]
+        ptr = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        nbr = np.concatenate([np.where(s["nbr"] >= 0, s["nbr"] + off, -1) for s, off in zip(samples, ptr[:-1])])
+        cat = lambda key, dt: np.ascontiguousarray(np.concatenate([s[key] for s in samples]), dt)  # noqa: E731
+        nbr = np.ascontiguousarray(nbr, np.int32)
+        sgn, geom = cat("sgn", np.float32), cat("geom", np.float32)
+        z, r, e = cat("z", np.complex128), cat("r", np.complex128), cat("e", np.complex128)
+        h = C.c_void_p()
+        _check(lib().dm_mpnn_data_create(self.ctx.h, int(ptr[-1]), _ptr(nbr), _ptr(sgn), _ptr(geom), _ptr(z), _ptr(r),
+                                         _ptr(e), len(samples), _ptr(ptr), C.byref(h)))
+        self.h, self.nsample, self.nnode = h, len(samples), int(ptr[-1])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().dm_mpnn_data_free(self.h)
+            self.h = None
+
+
+@dataclass
+class LearnedPreconditioner:
+    """FGMRES preconditioner M r = S r + network(S r, r); S is the near-field
+    SPAI over `cutoff` metres (<= 0: the library default)."""
+    net: MPNN
+    cutoff: float = 0.0
+
+
+def make_training_meshes(count, seed=0, wavelength=0.1, jitter=0.1, max_sphere_subdiv=3, ctx=None):
+    """Seeded random plates (sides U[0.05, 0.3] m, 4..30 cells per side, vertex
+    jitter in the plane) and icospheres (2..max_sphere_subdiv subdivisions,
+    radius U[0.3, 1.5] wavelengths, 3-D jitter); jitter is the standard
+    deviation in mean edge lengths.  Meshes that fail the quality check are
+    redrawn."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < count:
+        if rng.random() < 0.5:
+            Lx, Ly = rng.uniform(0.05, 0.3, 2)
+            Nx, Ny = rng.integers(4, 31, 2)
+            base = make_rect_plate(Lx, Ly, int(Nx), int(Ny), ctx=ctx)
+            h = 0.5 * (Lx / Nx + Ly / Ny)
+            xyz = base.xyz.copy()
+            xyz[:, :2] += rng.normal(0.0, jitter * h, (xyz.shape[0], 2))
+        else:
+            sub = int(rng.integers(2, max_sphere_subdiv + 1))
+            base = make_icosphere(float(rng.uniform(0.3, 1.5)) * wavelength, sub, ctx=ctx)
+            e = base.xyz[base.tri[:, :2] - 1]
+            h = float(np.mean(np.linalg.norm(e[:, 0] - e[:, 1], axis=1)))
+            xyz = base.xyz + rng.normal(0.0, jitter * h, base.xyz.shape)
+        m = TriMesh(xyz, base.tri, ctx=ctx)
+        try:
+            assert_mesh_quality(m)
+        except Exception:
+            continue
+        out.append(m)
+    return out
+
+
+def generate_training_samples(meshes, k, iters=(1, 2, 4, 8, 16, 32), cutoff_edges=2.0, seed=0, tol=1e-10):
+    """(residual, error) pairs from the library's GMRES on each mesh: a random
+    plane wave b, the converged x* (right SPAI-preconditioned GMRES to tol),
+    and the iterates x_k after k = iters steps: r = b - Z x_k, e = x* - x_k,
+    z = S r."""
+    rng = np.random.default_rng(seed)
+    samples = []
+    for m in meshes:
+        rwg = build_rwg(m)
+        Z = assemble_Z_efie(m, rwg, k, forms=DM_Z64P, device=True, mesh_precheck=False, max_output_bytes=None)
+        P = build_nearfield_preconditioner(None, m, rwg, cutoff_edges * float(np.mean(rwg.len)))
+        S = NearFieldOperator(P, Z)
+        d = rng.standard_normal(3)
+        d /= np.linalg.norm(d)
+        pol = rng.standard_normal(3)
+        pol -= d * (pol @ d)
+        pol /= np.linalg.norm(pol)
+        b = assemble_v_plane_wave(m, rwg, k * d, 1.0, pol)
+        xs, _ = solve_gmres(Z, b, preconditioner=P, precond_side="right", tol=tol, maxiter=1000)
+        nbr, sgn, geom = rwg_graph(rwg, k)
+        for it in iters:
+            xk, _ = solve_gmres(Z, b, preconditioner=P, precond_side="right", tol=1e-15, maxiter=int(it),
+                                check_gmres_convergence=False)
+            r = b - Z.matmul(xk)
+            if not np.linalg.norm(r) > 1e-12 * np.linalg.norm(b):
+                continue
+            samples.append(dict(z=S @ r, r=r, e=xs - xk, nbr=nbr, sgn=sgn, geom=geom))
+        del S
+        Z.free()
+    return samples

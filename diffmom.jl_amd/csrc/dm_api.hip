@@ -15,8 +15,12 @@
 using namespace dms;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
+
+void dm_set_error(const std::string& msg) { g_err = msg; }
+
+namespace {
 
 template <class F>
 int guard(F&& f) {
@@ -109,6 +113,25 @@ void check_flag(NearTables& nt, hipStream_t s) {
 
 i64 round_ld(i64 n) { return (n + 63) / 64 * 64; }
 
+constexpr int kAllForms = DM_Z64 | DM_Z32 | DM_Z64P | DM_Z32P;
+
+// Packed symmetric forms requested in `forms` (storage zeroed first: entries
+// past n stay zero).
+void assemble_packed(dm_zmat* z, const AsmArgs& a, const dm_rwg& r, int forms, hipStream_t s) {
+  if (forms & DM_Z64P) {
+    z->z64p.alloc(packed_scalars(z->n));
+    HIP_OK(hipMemsetAsync(z->z64p.p, 0, z->z64p.bytes(), s));
+    ProfScope p(z->ctx, "assemble_z64", s);
+    launch_assemble_z64p(a, z->z64p.p, r.max_blk_tris, s);
+  }
+  if (forms & DM_Z32P) {
+    z->z32p.alloc(packed_scalars(z->n));
+    HIP_OK(hipMemsetAsync(z->z32p.p, 0, z->z32p.bytes(), s));
+    ProfScope p(z->ctx, "assemble_z32", s);
+    launch_assemble_z32p(a, z->z32p.p, r.max_blk_tris, s);
+  }
+}
+
 
 }  // namespace
 
@@ -135,10 +158,27 @@ int batch_width(i64 n, i64 nrhs, int maxiter, int restart) {
   return (int)w;
 }
 
-bool use_batch(const dm_solve_opts& o, i64 nrhs, const DevPrec* M) {
-  if (o.method == DM_METHOD_LU || o.batch < 0) return false;
-  if (M && o.precond_side != 1) return false;  // batches use flexible right preconditioning
+// Lockstep batches serve several right-hand sides with right (flexible) or no
+// preconditioning.
+bool batched_solve(const dm_solve_opts& o, i64 nrhs) {
+  if (o.method == DM_METHOD_LU || o.batch < 0 || nrhs < 1) return false;
+  if (o.precond != DM_PRECOND_NONE && o.precond_side != 1) return false;
   return o.batch > 0 || nrhs >= kMfmaMinRhs;
+}
+
+bool use_batch(const dm_solve_opts& o, i64 nrhs, const DevPrec* M) {
+  return batched_solve(o, nrhs) && (M == nullptr || o.precond_side == 1);
+}
+
+// Operator form of one precision held by z: the packed symmetric form for
+// one-column products (half the bytes), the full form for lockstep batches
+// (the MFMA product reads full storage).  0 when neither is stored.
+int pick_form(const dm_zmat* z, bool f32, bool batched) {
+  const int full = f32 ? DM_Z32 : DM_Z64, packed = f32 ? DM_Z32P : DM_Z64P;
+  const bool hf = (z->forms & full) != 0, hp = (z->forms & packed) != 0;
+  if (!hf && !hp) return 0;
+  if (batched) return hf ? full : packed;
+  return hp ? packed : full;
 }
 
 // Solve the nrhs columns of Bd (device, internal basis order, n x nrhs) and
@@ -413,7 +453,8 @@ int dm_assemble_z(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double k_im, do
     check_ctx(rwg_c, "rwg");
     check_ctx(out, "output handle");
     DM_REQUIRE(rwg_c->ctx == ctx, DM_ERR_ARGUMENT, "RWG data belongs to another context");
-    DM_REQUIRE(forms > 0 && (forms & ~(DM_Z64 | DM_Z32)) == 0, DM_ERR_ARGUMENT, "forms must be a DM_Z64/DM_Z32 mask");
+    DM_REQUIRE(forms > 0 && (forms & ~kAllForms) == 0, DM_ERR_ARGUMENT,
+               "forms must be a mask of DM_Z64, DM_Z32, DM_Z64P, DM_Z32P");
     CtxScope sc(ctx);
     dm_rwg& r = const_cast<dm_rwg&>(*rwg_c);
     i64 n = r.n;
@@ -453,6 +494,7 @@ int dm_assemble_z(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double k_im, do
       ProfScope p(ctx, "assemble_z32", s);
       launch_assemble_z32(a, z->z32.p, r.max_blk_tris, s);
     }
+    assemble_packed(z.get(), a, r, forms, s);
     check_flag(nt, s);
     *out = z.release();
   });
@@ -467,7 +509,8 @@ int dm_zmat_size(const dm_zmat* z, int64_t* n, int* forms) {
 }
 
 static void need_form(const dm_zmat* z, int form) {
-  DM_REQUIRE(form == DM_Z64 || form == DM_Z32, DM_ERR_ARGUMENT, "form must be DM_Z64 or DM_Z32");
+  DM_REQUIRE(form == DM_Z64 || form == DM_Z32 || form == DM_Z64P || form == DM_Z32P, DM_ERR_ARGUMENT,
+             "form must be one of DM_Z64, DM_Z32, DM_Z64P, DM_Z32P");
   DM_REQUIRE(z->forms & form, DM_ERR_ARGUMENT, "requested form was not assembled");
 }
 
@@ -485,10 +528,7 @@ int dm_zmat_get(const dm_zmat* z, int form, double* out) {
     stage.alloc((size_t)2 * n * cols);
     for (i64 c0 = 0; c0 < n; c0 += cols) {
       i64 nc = std::min(cols, n - c0);
-      if (form == DM_Z64)
-        dml::unpermute64(n, z->ld, c0, nc, z->rwg->d_iofe.p, z->z64.p, stage.p, s);
-      else
-        dml::unpermute32(n, z->ld, c0, nc, z->rwg->d_iofe.p, z->z32.p, stage.p, s);
+      dml::unpermute(n, z->ref(form), c0, nc, z->rwg->d_iofe.p, stage.p, s);
       HIP_OK(hipMemcpyAsync(out + 2 * n * c0, stage.p, 16 * n * nc, hipMemcpyDeviceToHost, s));
     }
     HIP_OK(hipStreamSynchronize(s));
@@ -509,10 +549,7 @@ int dm_zmat_entries(const dm_zmat* z, int form, const int64_t* m, const int64_t*
     dmi.upload(m, count, s);
     dni.upload(nidx, count, s);
     o.alloc(2 * count);
-    if (form == DM_Z64)
-      dml::entries64(z->ld, count, z->rwg->d_iofe.p, dmi.p, dni.p, z->z64.p, o.p, s);
-    else
-      dml::entries32(z->ld, count, z->rwg->d_iofe.p, dmi.p, dni.p, z->z32.p, o.p, s);
+    dml::entries(z->ref(form), count, z->rwg->d_iofe.p, dmi.p, dni.p, o.p, s);
     HIP_OK(hipMemcpyAsync(out, o.p, 16 * count, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
   });
@@ -537,49 +574,27 @@ int dm_apply(dm_ctx* ctx, const dm_zmat* z, int form, int adjoint, const double*
     DM_REQUIRE(nrhs >= 0 && ldx >= n && ldy >= n, DM_ERR_DIMENSION, "leading dimension smaller than N");
     CtxScope sc(ctx);
     hipStream_t s = ctx->stream;
-    if (nrhs >= kMfmaMinRhs) {
-      // many right-hand sides: one Z X product on the matrix cores, Z read once
-      // per block of 64 right-hand sides instead of once per vector
-      for (i64 j0 = 0; j0 < nrhs; j0 += kMfmaMaxRhs) {
-        int R = (int)std::min<i64>(kMfmaMaxRhs, nrhs - j0);
-        DBuf<double> xe, xi, yi;
-        DBuf<char> ws;
-        xe.alloc(2 * n * R);
-        xi.alloc(2 * n * R);
-        yi.alloc(2 * n * R);
-        ws.alloc(dml::zx_workspace_bytes(n, R));
-        HIP_OK(hipMemcpy2DAsync(xe.p, 16 * n, X + 2 * ldx * j0, 16 * ldx, 16 * n, R, hipMemcpyHostToDevice, s));
-        if (adjoint) dml::conj(n * R, xe.p, s);
-        for (int j = 0; j < R; ++j) launch_permute(n, z->rwg->d_ext.p, xe.p + 2 * n * j, xi.p + 2 * n * j, 1, s);
-        if (form == DM_Z64) {
-          ProfScope p(ctx, "zx_z64", s);
-          dml::zx_mfma64(n, z->ld, z->z64.p, xi.p, n, R, yi.p, n, ws.p, s);
-        } else {
-          ProfScope p(ctx, "zx_z32", s);
-          dml::zx_mfma(n, z->ld, z->z32.p, xi.p, n, R, yi.p, n, ws.p, s);
-        }
-        for (int j = 0; j < R; ++j) launch_permute(n, z->rwg->d_ext.p, yi.p + 2 * n * j, xe.p + 2 * n * j, 0, s);
-        if (adjoint) dml::conj(n * R, xe.p, s);
-        HIP_OK(hipMemcpy2DAsync(Y + 2 * ldy * j0, 16 * ldy, xe.p, 16 * n, 16 * n, R, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
-      }
-      return;
+    // >= 8 right-hand sides on a full form: one Z X product on the matrix
+    // cores per block of columns (Z streamed once per 64 columns); otherwise
+    // one GEMV (full forms) or symmetric product (packed forms) per column
+    auto op = make_op(z, form);
+    for (i64 j0 = 0; j0 < nrhs; j0 += kMfmaMaxRhs) {
+      int R = (int)std::min<i64>(kMfmaMaxRhs, nrhs - j0);
+      DBuf<double> xe, xi, yi;
+      DBuf<char> ws;
+      xe.alloc(2 * n * R);
+      xi.alloc(2 * n * R);
+      yi.alloc(2 * n * R);
+      ws.alloc(std::max<size_t>(op->batch_ws(R), 1));
+      HIP_OK(hipMemcpy2DAsync(xe.p, 16 * n, X + 2 * ldx * j0, 16 * ldx, 16 * n, R, hipMemcpyHostToDevice, s));
+      if (adjoint) dml::conj(n * R, xe.p, s);
+      for (int j = 0; j < R; ++j) launch_permute(n, z->rwg->d_ext.p, xe.p + 2 * n * j, xi.p + 2 * n * j, 1, s);
+      op->apply_batch(xi.p, yi.p, R, ws.p, s);
+      for (int j = 0; j < R; ++j) launch_permute(n, z->rwg->d_ext.p, yi.p + 2 * n * j, xe.p + 2 * n * j, 0, s);
+      if (adjoint) dml::conj(n * R, xe.p, s);
+      HIP_OK(hipMemcpy2DAsync(Y + 2 * ldy * j0, 16 * ldy, xe.p, 16 * n, 16 * n, R, hipMemcpyDeviceToHost, s));
+      HIP_OK(hipStreamSynchronize(s));
     }
-    DBuf<double> xe, xi, yi;
-    xe.alloc(2 * n);
-    xi.alloc(2 * n);
-    yi.alloc(2 * n);
-    for (i64 j = 0; j < nrhs; ++j) {
-      HIP_OK(hipMemcpyAsync(xe.p, X + 2 * ldx * j, 16 * n, hipMemcpyHostToDevice, s));
-      if (adjoint) dml::conj(n, xe.p, s);
-      launch_permute(n, z->rwg->d_ext.p, xe.p, xi.p, 1, s);
-      if (form == DM_Z64) launch_gemv_z64(n, z->ld, z->z64.p, xi.p, yi.p, s);
-      else launch_gemv_z32(n, z->ld, z->z32.p, xi.p, yi.p, s);
-      launch_permute(n, z->rwg->d_ext.p, yi.p, xe.p, 0, s);
-      if (adjoint) dml::conj(n, xe.p, s);
-      HIP_OK(hipMemcpyAsync(Y + 2 * ldy * j, xe.p, 16 * n, hipMemcpyDeviceToHost, s));
-    }
-    HIP_OK(hipStreamSynchronize(s));
   });
 }
 
@@ -713,7 +728,7 @@ int dm_precond_create(dm_ctx* ctx, const dm_zmat* z, int kind, double nf_cutoff,
     auto h = std::make_unique<dm_precond>();
     h->ctx = ctx;
     h->z = z;
-    h->p = build_precond(z, form == DM_Z32, o, ctx->stream);
+    h->p = build_precond(z, form, o, ctx->stream);
     *out = h.release();
   });
 }
@@ -762,6 +777,7 @@ void dm_solve_opts_default(dm_solve_opts* o) {
   o->max_outer = 10;
   o->block_size = 128;
   o->batch = 0;
+  o->learned = nullptr;
 }
 
 int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double k_im, double eta0, int quad_order,
@@ -792,7 +808,13 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
     z->rwg = rwg_c;
     z->n = n;
     z->ld = round_ld(n);
-    z->forms = o.method == DM_METHOD_MIXED ? (DM_Z64 | DM_Z32) : DM_Z64;
+    // mixed precision: packed symmetric forms for one-column solves, full forms
+    // for lockstep batches (MFMA); the plain FP64 path keeps full storage
+    const bool batched = batched_solve(o, nrhs);
+    if (o.method == DM_METHOD_MIXED)
+      z->forms = batched ? (DM_Z64 | DM_Z32) : (DM_Z64P | DM_Z32P);
+    else
+      z->forms = DM_Z64;
     z->k_re = k_re;
     z->k_im = k_im;
     z->eta0 = eta0;
@@ -814,21 +836,34 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
       }
     } evg{e_near, e_z64};
     HIP_OK(hipEventRecord(e_near, s));
-    z->z64.alloc((size_t)z->ld * n * 2);
     if (o.method == DM_METHOD_MIXED) {
-      z->z32.alloc((size_t)z->ld * n * 2);
       {
         ProfScope p(ctx, "assemble_z32", s);
-        launch_assemble_z32(a, z->z32.p, r.max_blk_tris, s);
+        if (batched) {
+          z->z32.alloc((size_t)z->ld * n * 2);
+          launch_assemble_z32(a, z->z32.p, r.max_blk_tris, s);
+        } else {
+          z->z32p.alloc(packed_scalars(n));
+          HIP_OK(hipMemsetAsync(z->z32p.p, 0, z->z32p.bytes(), s));
+          launch_assemble_z32p(a, z->z32p.p, r.max_blk_tris, s);
+        }
       }
       // FP64 matrix on the second stream, concurrent with the complex64 solve
       HIP_OK(hipStreamWaitEvent(s2, e_near, 0));
       {
         ProfScope p(ctx, "assemble_z64", s2);
-        launch_assemble_z64(a, z->z64.p, r.max_blk_tris, s2);
+        if (batched) {
+          z->z64.alloc((size_t)z->ld * n * 2);
+          launch_assemble_z64(a, z->z64.p, r.max_blk_tris, s2);
+        } else {
+          z->z64p.alloc(packed_scalars(n));
+          HIP_OK(hipMemsetAsync(z->z64p.p, 0, z->z64p.bytes(), s2));
+          launch_assemble_z64p(a, z->z64p.p, r.max_blk_tris, s2);
+        }
       }
       HIP_OK(hipEventRecord(e_z64, s2));
     } else {
+      z->z64.alloc((size_t)z->ld * n * 2);
       ProfScope p(ctx, "assemble_z64", s);
       launch_assemble_z64(a, z->z64.p, r.max_blk_tris, s);
     }
@@ -843,8 +878,8 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
     HIP_OK(hipStreamSynchronize(s));
     double t_asm = now() - t0;
     dm_solve_stats st{};
-    std::unique_ptr<DevOp> A64 = std::make_unique<OpZ64>(z.get()), A32;
-    if (o.method == DM_METHOD_MIXED) A32 = std::make_unique<OpZ32>(z.get());
+    std::unique_ptr<DevOp> A64 = make_op(z.get(), pick_form(z.get(), false, batched)), A32;
+    if (o.method == DM_METHOD_MIXED) A32 = make_op(z.get(), pick_form(z.get(), true, batched));
     double t1 = now();
     double worst = 0;
     bool all_ok = true;
@@ -874,7 +909,7 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
         HIP_OK(hipMemcpyAsync(X + 2 * n * j, xe.p, 16 * n, hipMemcpyDeviceToHost, s));
       }
     } else {
-      auto M = build_precond(z.get(), o.method == DM_METHOD_MIXED, o, s);
+      auto M = build_precond(z.get(), pick_form(z.get(), o.method == DM_METHOD_MIXED, batched), o, s);
       HIP_OK(hipStreamSynchronize(s));
       st.precond_time_s = now() - t1;
       solve_columns(ctx, o, r, A64.get(), A32.get(), M.get(), V.p, n, nrhs, X, n, st, worst, all_ok,
@@ -957,17 +992,18 @@ int dm_solve(dm_ctx* ctx, const dm_zmat* z, const double* B, int64_t ldb, int64_
       if (stats) *stats = st;
       return;
     }
-    bool have64 = z->forms & DM_Z64, have32 = z->forms & DM_Z32;
+    const bool batched = batched_solve(o, nrhs);
+    const int f64 = pick_form(z, false, batched), f32 = pick_form(z, true, batched);
     if (o.method == DM_METHOD_MIXED)
-      DM_REQUIRE(have64 && have32, DM_ERR_ARGUMENT, "mixed-precision solve needs both DM_Z64 and DM_Z32 forms");
-    bool prec32 = o.method == DM_METHOD_MIXED || !have64;
-    auto M = build_precond(z, prec32, o, s);
+      DM_REQUIRE(f64 && f32, DM_ERR_ARGUMENT, "mixed-precision solve needs a complex128 and a complex64 form");
+    const int pform = (o.method == DM_METHOD_MIXED || !f64) ? f32 : f64;
+    auto M = build_precond(z, pform, o, s);
     HIP_OK(hipStreamSynchronize(s));
     st.precond_time_s = now() - t0;
     double t1 = now();
     std::unique_ptr<DevOp> A64, A32;
-    if (have64) A64 = std::make_unique<OpZ64>(z);
-    if (have32) A32 = std::make_unique<OpZ32>(z);
+    if (f64) A64 = make_op(z, f64);
+    if (f32) A32 = make_op(z, f32);
     bool all_ok = true;
     double worst = 0;
     DBuf<double> Bd;

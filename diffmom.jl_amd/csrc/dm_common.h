@@ -99,6 +99,9 @@ struct ProfScope {
   }
 };
 
+// Message returned by dm_last_error() for the calling thread.
+void dm_set_error(const std::string& msg);
+
 struct dm_mesh {
   dm_ctx* ctx = nullptr;
   i64 nv = 0, nt = 0;
@@ -162,6 +165,44 @@ struct dm_rwg {
   std::unique_ptr<SpaiPattern> spai;
 };
 
+// ------------------------------------------------------------ Z storage
+// Full storage: column-major ld x n, interleaved (re, im).  Packed symmetric
+// storage: tiles of kPT x kPT bases (I <= J) at tile index J (J + 1) / 2 + I,
+// each tile column-major; entries past n are zero.
+constexpr int kPT = 128;
+inline i64 packed_tiles(i64 n) {
+  i64 nb = (n + kPT - 1) / kPT;
+  return nb * (nb + 1) / 2;
+}
+inline size_t packed_scalars(i64 n) { return (size_t)packed_tiles(n) * kPT * kPT * 2; }
+
+struct ZRef {
+  const void* p = nullptr;
+  i64 ld = 0;
+  int f32 = 0;     // complex64 storage
+  int packed = 0;  // packed symmetric tiles
+};
+
+__device__ __host__ inline i64 packed_index(i64 r, i64 c) {
+  if (r > c) {
+    i64 t = r;
+    r = c;
+    c = t;
+  }
+  i64 I = r / kPT, J = c / kPT;
+  i64 l = J * (J + 1) / 2 + I;
+  return (l * kPT + (c - J * kPT)) * kPT + (r - I * kPT);
+}
+
+__device__ inline double2 zat(const ZRef& z, i64 r, i64 c) {
+  i64 idx = z.packed ? packed_index(r, c) : r + c * z.ld;
+  if (z.f32) {
+    float2 v = reinterpret_cast<const float2*>(z.p)[idx];
+    return make_double2((double)v.x, (double)v.y);
+  }
+  return reinterpret_cast<const double2*>(z.p)[idx];
+}
+
 struct dm_zmat {
   dm_ctx* ctx = nullptr;
   const dm_rwg* rwg = nullptr;
@@ -169,8 +210,21 @@ struct dm_zmat {
   int forms = 0;
   double k_re = 0, k_im = 0, eta0 = 0;
   int quad_order = 3;
-  DBuf<double> z64;  // ld x n complex128 (interleaved), internal order
-  DBuf<float> z32;   // ld x n complex64, internal order
+  DBuf<double> z64;   // ld x n complex128 (interleaved), internal order
+  DBuf<float> z32;    // ld x n complex64, internal order
+  DBuf<double> z64p;  // packed symmetric complex128
+  DBuf<float> z32p;   // packed symmetric complex64
+  ZRef ref(int form) const {
+    ZRef r;
+    r.ld = ld;
+    r.f32 = (form == DM_Z32 || form == DM_Z32P) ? 1 : 0;
+    r.packed = (form == DM_Z64P || form == DM_Z32P) ? 1 : 0;
+    r.p = form == DM_Z64 ? (const void*)z64.p
+          : form == DM_Z32 ? (const void*)z32.p
+          : form == DM_Z64P ? (const void*)z64p.p
+                            : (const void*)z32p.p;
+    return r;
+  }
 };
 
 struct dm_mfop {
@@ -193,8 +247,8 @@ const SpaiPattern& spai_pattern(dm_rwg& r, double cutJ, int maxJ);
 
 namespace dmp {
 int spai_max_cols();
-void spai_build(i64 n, i64 ld, const double* z64, const float* z32, const int* jptr, const int* jidx, const int* iptr,
-                const int* iidx, double* vals, int* status, hipStream_t s);
+void spai_build(i64 n, const ZRef& z, const int* jptr, const int* jidx, const int* iptr, const int* iidx,
+                double* vals, int* status, hipStream_t s);
 void csr_from_csc(i64 nnz, const int* map, const double* cscval, double* csrval, hipStream_t s);
 void spmv(i64 n, const int* rowptr, const int* colidx, const double* val, const double* x, double* y, hipStream_t s);
 // Y = S X for R columns of length n (leading dimension n)
@@ -229,6 +283,9 @@ void launch_near_tables(const AsmArgs& a, const TriTables& t, double* jself, dou
                         hipStream_t s);
 void launch_assemble_z64(const AsmArgs& a, double* z, int max_tris, hipStream_t s);
 void launch_assemble_z32(const AsmArgs& a, float* z, int max_tris, hipStream_t s);
+// packed symmetric storage (buffer of packed_scalars(n), zeroed beforehand)
+void launch_assemble_z64p(const AsmArgs& a, double* z, int max_tris, hipStream_t s);
+void launch_assemble_z32p(const AsmArgs& a, float* z, int max_tris, hipStream_t s);
 // matrix-free y += Z x in internal order (FP64), x and y complex128
 void launch_mf_apply(const AsmArgs& a, const double* x, double* y, int max_tris, hipStream_t s);
 

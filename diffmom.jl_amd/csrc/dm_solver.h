@@ -66,6 +66,30 @@ struct OpZ32 : DevOp {
   size_t batch_ws(int R) const override { return dml::zx_workspace_bytes(z->n, R); }
 };
 
+// Packed symmetric storage (DM_Z64P / DM_Z32P): one symmetric product per
+// vector; batches loop over columns.
+struct OpPacked : DevOp {
+  const dm_zmat* z;
+  int form;
+  const char* tag;
+  DBuf<char> ws;
+  OpPacked(const dm_zmat* z_, int form_) : z(z_), form(form_) {
+    tag = form == DM_Z32P ? "symv_z32" : "symv_z64";
+    ws.alloc(dml::symv_workspace_bytes(z->n));
+  }
+  void apply(const double* x, double* y, hipStream_t s) override {
+    ProfScope p(z->ctx, tag, s);
+    dml::symv_packed(z->n, z->ref(form), x, y, ws.p, s);
+  }
+  void apply_batch(const double* X, double* Y, int R, void*, hipStream_t s) override {
+    for (int j = 0; j < R; ++j) apply(X + 2 * z->n * j, Y + 2 * z->n * j, s);
+  }
+  size_t batch_ws(int) const override { return 0; }
+};
+
+// The operator of one stored form.
+std::unique_ptr<DevOp> make_op(const dm_zmat* z, int form);
+
 // y = M x; apply_batch: R columns of length n (leading dimension n)
 struct DevPrec {
   i64 n = 0;
@@ -95,9 +119,12 @@ struct PrecBlock : DevPrec {
   void apply(const double* x, double* y, hipStream_t s) override { dml::block_apply(n, bs, nb, inv.p, x, y, s); }
 };
 
-// Preconditioner of kind o.precond built from the complex64 (use32) or
-// complex128 stored form of z.  Returns null for DM_PRECOND_NONE.
-std::unique_ptr<DevPrec> build_precond(const dm_zmat* z, bool use32, const dm_solve_opts& o, hipStream_t s);
+// Preconditioner of kind o.precond built from the stored form `form` of z
+// (any of DM_Z64, DM_Z32, DM_Z64P, DM_Z32P).  Returns null for DM_PRECOND_NONE.
+std::unique_ptr<DevPrec> build_precond(const dm_zmat* z, int form, const dm_solve_opts& o, hipStream_t s);
+// DM_PRECOND_LEARNED: near-field SPAI S plus the trained network o.learned,
+// M r = S r + network(S r, r) (nonlinear: right preconditioning only).
+std::unique_ptr<DevPrec> make_learned_precond(const dm_zmat* z, int form, const dm_solve_opts& o, hipStream_t s);
 
 // ------------------------------------------------------------------- GMRES
 struct Krylov {

@@ -13,42 +13,24 @@ double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-std::unique_ptr<DevPrec> build_precond(const dm_zmat* z, bool use32, const dm_solve_opts& o, hipStream_t s) {
+std::unique_ptr<DevOp> make_op(const dm_zmat* z, int form) {
+  DM_REQUIRE(z->forms & form, DM_ERR_ARGUMENT, "requested form was not assembled");
+  if (form == DM_Z64) return std::make_unique<OpZ64>(z);
+  if (form == DM_Z32) return std::make_unique<OpZ32>(z);
+  return std::make_unique<OpPacked>(z, form);
+}
+
+std::unique_ptr<DevPrec> build_precond(const dm_zmat* z, int form, const dm_solve_opts& o, hipStream_t s) {
   i64 n = z->n;
+  DM_REQUIRE(z->forms & form, DM_ERR_ARGUMENT, "preconditioner source form was not assembled");
+  const ZRef zr = z->ref(form);
   if (o.precond == DM_PRECOND_NONE) return nullptr;
+  if (o.precond == DM_PRECOND_LEARNED) return make_learned_precond(z, form, o, s);
   if (o.precond == DM_PRECOND_DIAG) {
     auto p = std::make_unique<PrecDiag>();
     p->n = n;
     p->d.alloc(2 * n);
-    if (!use32) {
-      dml::diag_inv(n, z->ld, z->z64.p, p->d.p, s);
-    } else {
-      // widen the diagonal of Z32 through a one-block extraction per element
-      DBuf<double> tmp;
-      tmp.alloc(2 * n);
-      std::vector<i64> idx(n);
-      for (i64 i = 0; i < n; ++i) idx[i] = i;
-      DBuf<i64> di;
-      di.upload(idx, s);
-      std::vector<int> iden(n);
-      for (i64 i = 0; i < n; ++i) iden[i] = (int)i;
-      DBuf<int> dm;
-      dm.upload(iden, s);
-      dml::entries32(z->ld, n, dm.p, di.p, di.p, z->z32.p, tmp.p, s);
-      // d = 1 / diag
-      std::vector<double> h(2 * n);
-      HIP_OK(hipMemcpyAsync(h.data(), tmp.p, 16 * n, hipMemcpyDeviceToHost, s));
-      HIP_OK(hipStreamSynchronize(s));
-      for (i64 i = 0; i < n; ++i) {
-        cd v(h[2 * i], h[2 * i + 1]);
-        DM_REQUIRE(std::abs(v) > 0, DM_ERR_ARGUMENT, "zero diagonal entry in diagonal preconditioner");
-        cd iv = 1.0 / v;
-        h[2 * i] = iv.real();
-        h[2 * i + 1] = iv.imag();
-      }
-      p->d.upload(h, s);
-      HIP_OK(hipStreamSynchronize(s));
-    }
+    dml::diag_inv(n, zr, p->d.p, s);
     return p;
   }
   if (o.precond == DM_PRECOND_NEARFIELD) {
@@ -64,8 +46,7 @@ std::unique_ptr<DevPrec> build_precond(const dm_zmat* z, bool use32, const dm_so
     DBuf<int> st;
     st.alloc(1);
     HIP_OK(hipMemsetAsync(st.p, 0, sizeof(int), s));
-    dmp::spai_build(n, z->ld, use32 ? nullptr : z->z64.p, use32 ? z->z32.p : nullptr, P.d_jptr.p, P.d_jidx.p,
-                    P.d_iptr.p, P.d_iidx.p, cscv.p, st.p, s);
+    dmp::spai_build(n, zr, P.d_jptr.p, P.d_jidx.p, P.d_iptr.p, P.d_iidx.p, cscv.p, st.p, s);
     dmp::csr_from_csc(P.nnz, P.d_map.p, cscv.p, p->val.p, s);
     int sv = 0;
     HIP_OK(hipMemcpyAsync(&sv, st.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -82,10 +63,7 @@ std::unique_ptr<DevPrec> build_precond(const dm_zmat* z, bool use32, const dm_so
     p->bs = (int)std::min<i64>(p->bs, n);
     p->nb = (int)((n + p->bs - 1) / p->bs);
     p->inv.alloc((size_t)p->nb * p->bs * p->bs * 2);
-    if (use32)
-      dml::extract_blocks32(n, z->ld, p->bs, p->nb, z->z32.p, p->inv.p, s);
-    else
-      dml::extract_blocks64(n, z->ld, p->bs, p->nb, z->z64.p, p->inv.p, s);
+    dml::extract_blocks(n, zr, p->bs, p->nb, p->inv.p, s);
     DBuf<int> st;
     st.alloc(p->nb);
     HIP_OK(hipMemsetAsync(st.p, 0, sizeof(int) * p->nb, s));

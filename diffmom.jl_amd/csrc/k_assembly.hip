@@ -609,6 +609,35 @@ __global__ __launch_bounds__(NTHR) void k_tile(AsmArgs a, int cap, AccT* __restr
       }
     }
     if (bad) atomicOr(a.flag, 1);
+  } else if constexpr (MODE == 2) {
+    // packed symmetric storage: assembly tile (bi, bj) is the (bi & 1, bj & 1)
+    // quarter of storage tile (bi / 2, bj / 2); a storage tile on the diagonal
+    // also receives the mirror of its off-diagonal quarter
+    int bad = 0;
+    const int BI = bi >> 1, BJ = bj >> 1;
+    const int ro = (bi & 1) * TB, co = (bj & 1) * TB;
+    AccT* tile = zout + 2 * (((i64)BJ * (BJ + 1) / 2 + BI) * kPT * kPT);
+    const bool mirror = BI == BJ && !diag;
+    for (int k = tid; k < TB * TB; k += NTHR) {
+      int r = k % TB, c = k / TB;
+      i64 gr = r0 + r, gc = c0 + c;
+      if (gr >= a.n || gc >= a.n) continue;
+      int rr = r, cc = c;
+      if (diag && r > c) { rr = c; cc = r; }
+      double tr = (double)zt[rr * ldt + cc], ti = (double)zt[TB * ldt + rr * ldt + cc];
+      double l2 = a.len[gr] * a.len[gc];
+      double vr = (Pr * tr - Pi * ti) * l2, vi = (Pr * ti + Pi * tr) * l2;
+      bad |= !isfinite(vr) || !isfinite(vi);
+      AccT* o = tile + 2 * ((i64)(co + c) * kPT + ro + r);
+      o[0] = (AccT)vr;
+      o[1] = (AccT)vi;
+      if (mirror) {
+        AccT* m = tile + 2 * ((i64)(ro + r) * kPT + co + c);
+        m[0] = (AccT)vr;
+        m[1] = (AccT)vi;
+      }
+    }
+    if (bad) atomicOr(a.flag, 1);
   } else {
     // y[rows] += Z_tile x[cols]; y[cols] += Z_tile^T x[rows] (off-diagonal tiles)
     for (int k = tid; k < 2 * TB; k += NTHR) {
@@ -713,6 +742,14 @@ void launch_assemble_z64(const AsmArgs& a, double* z, int cap, hipStream_t s) {
 }
 void launch_assemble_z32(const AsmArgs& a, float* z, int cap, hipStream_t s) {
   launch_tile<float, float, 0>(a, cap, z, nullptr, nullptr, s);
+}
+void launch_assemble_z64p(const AsmArgs& a, double* z, int cap, hipStream_t s) {
+  DM_REQUIRE(2 * a.TB == kPT, DM_ERR_ARGUMENT, "packed storage needs assembly tiles of half the storage tile");
+  launch_tile<double, double, 2>(a, cap, z, nullptr, nullptr, s);
+}
+void launch_assemble_z32p(const AsmArgs& a, float* z, int cap, hipStream_t s) {
+  DM_REQUIRE(2 * a.TB == kPT, DM_ERR_ARGUMENT, "packed storage needs assembly tiles of half the storage tile");
+  launch_tile<float, float, 2>(a, cap, z, nullptr, nullptr, s);
 }
 void launch_mf_apply(const AsmArgs& a, const double* x, double* y, int cap, hipStream_t s) {
   launch_tile<double, double, 1>(a, cap, nullptr, x, y, s);

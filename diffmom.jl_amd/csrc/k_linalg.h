@@ -8,21 +8,20 @@ void update(i64 n, i64 ldv, int k, const double* V, const double* h, double* w, 
 void combine(i64 n, i64 ldv, int k, const double* V, const double* y, double* x, hipStream_t s);
 void norm2(i64 n, const double* w, double* out, hipStream_t s);
 void axpby(i64 n, double ar, double ai, const double* x, double br, double bi, double* y, hipStream_t s);
-void extract_blocks64(i64 n, i64 ld, int bs, int nb, const double* z, double* blocks, hipStream_t s);
-void extract_blocks32(i64 n, i64 ld, int bs, int nb, const float* z, double* blocks, hipStream_t s);
+void extract_blocks(i64 n, const ZRef& z, int bs, int nb, double* blocks, hipStream_t s);
 void block_inverse(int bs, int nb, double* blocks, int* status, hipStream_t s);
 void block_apply(i64 n, int bs, int nb, const double* inv, const double* x, double* y, hipStream_t s);
-void diag_inv(i64 n, i64 ld, const double* z, double* d, hipStream_t s);
+void diag_inv(i64 n, const ZRef& z, double* d, hipStream_t s);
 void diag_apply(i64 n, const double* d, const double* x, double* y, hipStream_t s);
 void lu_factor(i64 n, i64 ld, double* A, int* ipiv, int* status, hipStream_t s);
 void lu_solve(i64 n, i64 ld, const double* A, const int* ipiv, double* B, i64 ldb, int nrhs, hipStream_t s);
-void unpermute64(i64 n, i64 ld, i64 c0, i64 nc, const int* iofe, const double* z, double* out, hipStream_t s);
-void unpermute32(i64 n, i64 ld, i64 c0, i64 nc, const int* iofe, const float* z, double* out, hipStream_t s);
-void entries64(i64 ld, i64 count, const int* iofe, const i64* m, const i64* n, const double* z, double* out, hipStream_t s);
-void entries32(i64 ld, i64 count, const int* iofe, const i64* m, const i64* n, const float* z, double* out, hipStream_t s);
-void finite64(i64 n, i64 ld, const double* z, int* flag, hipStream_t s);
-void finite32(i64 n, i64 ld, const float* z, int* flag, hipStream_t s);
+// entries of one stored form; rows / columns of out in caller order via iofe
+void unpermute(i64 n, const ZRef& z, i64 c0, i64 nc, const int* iofe, double* out, hipStream_t s);
+void entries(const ZRef& z, i64 count, const int* iofe, const i64* m, const i64* n, double* out, hipStream_t s);
 void conj(i64 n, double* v, hipStream_t s);
+// y = Z x from packed symmetric storage; ws >= symv_workspace_bytes(n)
+size_t symv_workspace_bytes(i64 n);
+void symv_packed(i64 n, const ZRef& z, const double* x, double* y, void* ws, hipStream_t s);
 size_t zx_workspace_bytes(i64 n, int R);
 void zx_mfma(i64 n, i64 ld, const float* z32, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
              hipStream_t s);

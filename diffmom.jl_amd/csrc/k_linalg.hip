@@ -217,21 +217,13 @@ __global__ void k_cast_matrix(i64 n, i64 ld, const double2* __restrict__ src, fl
 // ------------------------------------------------------- block preconditioner
 // Extract diagonal blocks of Z (internal order): blocks[b] = Z[b*bs .. , b*bs ..],
 // identity-padded past n.
-template <typename ZT>
-__global__ void k_extract_blocks(i64 n, i64 ld, int bs, const ZT* __restrict__ z, double2* __restrict__ blocks) {
+__global__ void k_extract_blocks(i64 n, ZRef z, int bs, double2* __restrict__ blocks) {
   int b = blockIdx.y;
   i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
   if (idx >= (i64)bs * bs) return;
   int r = idx % bs, c = idx / bs;
   i64 gr = (i64)b * bs + r, gc = (i64)b * bs + c;
-  double2 v;
-  if (gr < n && gc < n) {
-    ZT e = z[gr + gc * ld];
-    v = make_double2((double)e.x, (double)e.y);
-  } else {
-    v = make_double2(r == c ? 1.0 : 0.0, 0.0);
-  }
-  blocks[(i64)b * bs * bs + idx] = v;
+  blocks[(i64)b * bs * bs + idx] = (gr < n && gc < n) ? zat(z, gr, gc) : make_double2(r == c ? 1.0 : 0.0, 0.0);
 }
 
 // In-place Gauss-Jordan inverse with partial pivoting of each bs x bs block
@@ -332,10 +324,10 @@ __global__ __launch_bounds__(256) void k_block_apply(i64 n, int bs, const double
   y[gr] = make_double2(ar, ai);
 }
 
-__global__ void k_diag_inv(i64 n, i64 ld, const double2* __restrict__ z, double2* __restrict__ d) {
+__global__ void k_diag_inv(i64 n, ZRef z, double2* __restrict__ d) {
   i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double2 v = z[i + i * ld];
+  double2 v = zat(z, i, i);
   double m = v.x * v.x + v.y * v.y;
   d[i] = make_double2(v.x / m, -v.y / m);
 }
@@ -442,35 +434,18 @@ __global__ __launch_bounds__(1024) void k_lu_solve(i64 n, i64 ld, const double2*
 
 inline unsigned nblocks(i64 n, int t) { return (unsigned)((n + t - 1) / t); }
 // Caller-order copy of a block of columns: out[r + (c - c0)*n] = Z[int(r) + int(c)*ld]
-template <typename ZT>
-__global__ void k_unpermute(i64 n, i64 ld, i64 c0, i64 nc, const int* __restrict__ iofe, const ZT* __restrict__ z,
-                            double2* __restrict__ out) {
+__global__ void k_unpermute(i64 n, ZRef z, i64 c0, i64 nc, const int* __restrict__ iofe, double2* __restrict__ out) {
   i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
   if (idx >= n * nc) return;
   i64 r = idx % n, c = c0 + idx / n;
-  ZT v = z[(i64)iofe[r] + (i64)iofe[c] * ld];
-  out[idx] = make_double2((double)v.x, (double)v.y);
+  out[idx] = zat(z, iofe[r], iofe[c]);
 }
 
-template <typename ZT>
-__global__ void k_entries(i64 ld, i64 count, const int* __restrict__ iofe, const i64* __restrict__ m,
-                          const i64* __restrict__ nn, const ZT* __restrict__ z, double2* __restrict__ out) {
+__global__ void k_entries(ZRef z, i64 count, const int* __restrict__ iofe, const i64* __restrict__ m,
+                          const i64* __restrict__ nn, double2* __restrict__ out) {
   i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x;
   if (i >= count) return;
-  ZT v = z[(i64)iofe[m[i]] + (i64)iofe[nn[i]] * ld];
-  out[i] = make_double2((double)v.x, (double)v.y);
-}
-
-// flag[0] |= 1 when any stored entry is non-finite (grid-stride over columns).
-template <typename ZT>
-__global__ void k_finite(i64 n, i64 ld, const ZT* __restrict__ z, int* flag) {
-  int bad = 0;
-  for (i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x; idx < n * n; idx += (i64)gridDim.x * blockDim.x) {
-    i64 r = idx % n, c = idx / n;
-    ZT v = z[r + c * ld];
-    bad |= !isfinite((double)v.x) || !isfinite((double)v.y);
-  }
-  if (bad) atomicOr(flag, 1);
+  out[i] = zat(z, iofe[m[i]], iofe[nn[i]]);
 }
 
 __global__ void k_conj(i64 n, double2* v) {
@@ -540,16 +515,9 @@ void axpby(i64 n, double ar, double ai, const double* x, double br, double bi, d
                                            make_double2(br, bi), reinterpret_cast<double2*>(y));
   HIP_OK(hipGetLastError());
 }
-void extract_blocks64(i64 n, i64 ld, int bs, int nb, const double* z, double* blocks, hipStream_t s) {
+void extract_blocks(i64 n, const ZRef& z, int bs, int nb, double* blocks, hipStream_t s) {
   dim3 g(nblocks((i64)bs * bs, 256), nb);
-  k_extract_blocks<double2><<<g, 256, 0, s>>>(n, ld, bs, reinterpret_cast<const double2*>(z),
-                                               reinterpret_cast<double2*>(blocks));
-  HIP_OK(hipGetLastError());
-}
-void extract_blocks32(i64 n, i64 ld, int bs, int nb, const float* z, double* blocks, hipStream_t s) {
-  dim3 g(nblocks((i64)bs * bs, 256), nb);
-  k_extract_blocks<float2><<<g, 256, 0, s>>>(n, ld, bs, reinterpret_cast<const float2*>(z),
-                                              reinterpret_cast<double2*>(blocks));
+  k_extract_blocks<<<g, 256, 0, s>>>(n, z, bs, reinterpret_cast<double2*>(blocks));
   HIP_OK(hipGetLastError());
 }
 void block_inverse(int bs, int nb, double* blocks, int* status, hipStream_t s) {
@@ -563,8 +531,8 @@ void block_apply(i64 n, int bs, int nb, const double* inv, const double* x, doub
                                                       reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y));
   HIP_OK(hipGetLastError());
 }
-void diag_inv(i64 n, i64 ld, const double* z, double* d, hipStream_t s) {
-  k_diag_inv<<<nblocks(n, 256), 256, 0, s>>>(n, ld, reinterpret_cast<const double2*>(z), reinterpret_cast<double2*>(d));
+void diag_inv(i64 n, const ZRef& z, double* d, hipStream_t s) {
+  k_diag_inv<<<nblocks(n, 256), 256, 0, s>>>(n, z, reinterpret_cast<double2*>(d));
   HIP_OK(hipGetLastError());
 }
 void diag_apply(i64 n, const double* d, const double* x, double* y, hipStream_t s) {
@@ -587,34 +555,12 @@ void lu_solve(i64 n, i64 ld, const double* A, const int* ipiv, double* B, i64 ld
   HIP_OK(hipGetLastError());
 }
 
-void unpermute64(i64 n, i64 ld, i64 c0, i64 nc, const int* iofe, const double* z, double* out, hipStream_t s) {
-  k_unpermute<double2><<<nblocks(n * nc, 256), 256, 0, s>>>(n, ld, c0, nc, iofe, reinterpret_cast<const double2*>(z),
-                                                             reinterpret_cast<double2*>(out));
+void unpermute(i64 n, const ZRef& z, i64 c0, i64 nc, const int* iofe, double* out, hipStream_t s) {
+  k_unpermute<<<nblocks(n * nc, 256), 256, 0, s>>>(n, z, c0, nc, iofe, reinterpret_cast<double2*>(out));
   HIP_OK(hipGetLastError());
 }
-void unpermute32(i64 n, i64 ld, i64 c0, i64 nc, const int* iofe, const float* z, double* out, hipStream_t s) {
-  k_unpermute<float2><<<nblocks(n * nc, 256), 256, 0, s>>>(n, ld, c0, nc, iofe, reinterpret_cast<const float2*>(z),
-                                                            reinterpret_cast<double2*>(out));
-  HIP_OK(hipGetLastError());
-}
-void entries64(i64 ld, i64 count, const int* iofe, const i64* m, const i64* n, const double* z, double* out, hipStream_t s) {
-  k_entries<double2><<<nblocks(count, 256), 256, 0, s>>>(ld, count, iofe, m, n, reinterpret_cast<const double2*>(z),
-                                                          reinterpret_cast<double2*>(out));
-  HIP_OK(hipGetLastError());
-}
-void entries32(i64 ld, i64 count, const int* iofe, const i64* m, const i64* n, const float* z, double* out, hipStream_t s) {
-  k_entries<float2><<<nblocks(count, 256), 256, 0, s>>>(ld, count, iofe, m, n, reinterpret_cast<const float2*>(z),
-                                                         reinterpret_cast<double2*>(out));
-  HIP_OK(hipGetLastError());
-}
-void finite64(i64 n, i64 ld, const double* z, int* flag, hipStream_t s) {
-  unsigned g = (unsigned)std::min<i64>(nblocks(n * n, 256), 4096);
-  k_finite<double2><<<g, 256, 0, s>>>(n, ld, reinterpret_cast<const double2*>(z), flag);
-  HIP_OK(hipGetLastError());
-}
-void finite32(i64 n, i64 ld, const float* z, int* flag, hipStream_t s) {
-  unsigned g = (unsigned)std::min<i64>(nblocks(n * n, 256), 4096);
-  k_finite<float2><<<g, 256, 0, s>>>(n, ld, reinterpret_cast<const float2*>(z), flag);
+void entries(const ZRef& z, i64 count, const int* iofe, const i64* m, const i64* n, double* out, hipStream_t s) {
+  k_entries<<<nblocks(count, 256), 256, 0, s>>>(z, count, iofe, m, n, reinterpret_cast<double2*>(out));
   HIP_OK(hipGetLastError());
 }
 void conj(i64 n, double* v, hipStream_t s) {

@@ -23,14 +23,7 @@ namespace {
 constexpr int SPAI_MAXJ = 80;   // columns per least-squares problem (LDS bound)
 constexpr int SPAI_CHUNK = 16;  // rows of A staged per pass
 
-template <typename ZT>
-__device__ inline double2 zload(const ZT* z, i64 idx) {
-  ZT v = z[idx];
-  return make_double2((double)v.x, (double)v.y);
-}
-
-template <typename ZT>
-__global__ __launch_bounds__(256) void k_spai_build(i64 ld, const ZT* __restrict__ z, const int* __restrict__ jptr,
+__global__ __launch_bounds__(256) void k_spai_build(ZRef z, const int* __restrict__ jptr,
                                                     const int* __restrict__ jidx, const int* __restrict__ iptr,
                                                     const int* __restrict__ iidx, double2* __restrict__ vals,
                                                     int* __restrict__ status) {
@@ -49,7 +42,7 @@ __global__ __launch_bounds__(256) void k_spai_build(i64 ld, const ZT* __restrict
   __syncthreads();
   // rhs = A^H e_j = conj(Z[j, J]) (row j of Z restricted to the pattern)
   for (int a = tid; a < nJ; a += 256) {
-    double2 v = zload(z, (i64)j + (i64)cols[a] * ld);
+    double2 v = zat(z, j, cols[a]);
     rhs[a] = make_double2(v.x, -v.y);
   }
   // G = A^H A accumulated over row chunks (lower triangle incl. diagonal)
@@ -58,7 +51,7 @@ __global__ __launch_bounds__(256) void k_spai_build(i64 ld, const ZT* __restrict
     __syncthreads();
     for (int e = tid; e < nr * nJ; e += 256) {
       int ii = e / nJ, a = e - ii * nJ;
-      Ach[ii * SPAI_MAXJ + a] = zload(z, (i64)iidx[i0 + r0 + ii] + (i64)cols[a] * ld);
+      Ach[ii * SPAI_MAXJ + a] = zat(z, iidx[i0 + r0 + ii], cols[a]);
     }
     __syncthreads();
     for (int e = tid; e < nJ * nJ; e += 256) {
@@ -168,14 +161,9 @@ namespace dmp {
 
 int spai_max_cols() { return SPAI_MAXJ; }
 
-void spai_build(i64 n, i64 ld, const double* z64, const float* z32, const int* jptr, const int* jidx, const int* iptr,
-                const int* iidx, double* vals, int* status, hipStream_t s) {
-  if (z32)
-    k_spai_build<float2><<<(unsigned)n, 256, 0, s>>>(ld, reinterpret_cast<const float2*>(z32), jptr, jidx, iptr, iidx,
-                                                      reinterpret_cast<double2*>(vals), status);
-  else
-    k_spai_build<double2><<<(unsigned)n, 256, 0, s>>>(ld, reinterpret_cast<const double2*>(z64), jptr, jidx, iptr,
-                                                       iidx, reinterpret_cast<double2*>(vals), status);
+void spai_build(i64 n, const ZRef& z, const int* jptr, const int* jidx, const int* iptr, const int* iidx,
+                double* vals, int* status, hipStream_t s) {
+  k_spai_build<<<(unsigned)n, 256, 0, s>>>(z, jptr, jidx, iptr, iidx, reinterpret_cast<double2*>(vals), status);
   HIP_OK(hipGetLastError());
 }
 

@@ -44,6 +44,11 @@ extern "C" {
 /* Which stored forms of Z an assembly produces (bit mask). */
 #define DM_Z64 1 /* complex128 from FP64 quadrature (the reference's arithmetic) */
 #define DM_Z32 2 /* complex64 from FP32 far-pair quadrature + FP64 self/adjacent terms */
+/* The same two matrices in packed symmetric storage: only the upper triangle of
+ * 128 x 128 tiles is kept (Z = Z^T exactly), half the bytes of the full forms;
+ * products stream each stored tile once and use it for Z x and Z^T x. */
+#define DM_Z64P 4
+#define DM_Z32P 8
 
 typedef struct dm_ctx dm_ctx;
 typedef struct dm_mesh dm_mesh;
@@ -106,13 +111,13 @@ int dm_rwg_adjacent_pairs(const dm_rwg* rwg, int64_t* count);
 /* ----------------------------------------------------------------- assembly */
 /* assemble_Z_efie(mesh, rwg, k; quad_order, eta0, max_output_bytes):
  * src/assembly/EFIE.jl:699-753.  The matrix stays on the device.  `forms` is a
- * DM_Z64 / DM_Z32 mask.  k may be complex (k_im != 0).  max_output_bytes bounds
+ * mask of DM_Z64, DM_Z32, DM_Z64P, DM_Z32P.  k may be complex (k_im != 0).  max_output_bytes bounds
  * N*N*16 as the reference's byte gate does (<= 0: no gate). */
 int dm_assemble_z(dm_ctx* ctx, const dm_rwg* rwg, double k_re, double k_im, double eta0,
                   int quad_order, int forms, int64_t max_output_bytes, dm_zmat** out);
 int dm_zmat_size(const dm_zmat* z, int64_t* n, int* forms);
 /* Copy Z (caller basis order, column-major N x N, complex128) to host.
- * form = DM_Z64 or DM_Z32 (complex64 values widened). */
+ * form = one stored form (complex64 values widened). */
 int dm_zmat_get(const dm_zmat* z, int form, double* out);
 /* Entries Z[m_i, n_i] (0-based caller indices) of one stored form. */
 int dm_zmat_entries(const dm_zmat* z, int form, const int64_t* m, const int64_t* n,
@@ -152,6 +157,8 @@ int dm_excitation_planewave(dm_ctx* ctx, const dm_rwg* rwg, const double* kvec, 
 #define DM_PRECOND_LEARNED 3  /* message-passing network, flexible (needs a trained model) */
 #define DM_PRECOND_BLOCK 4    /* block Jacobi over spatially consecutive bases */
 
+typedef struct dm_mpnn dm_mpnn; /* trained learned-preconditioner network */
+
 typedef struct dm_solve_opts {
   int method;          /* DM_METHOD_* */
   int precond;         /* DM_PRECOND_* */
@@ -166,6 +173,7 @@ typedef struct dm_solve_opts {
   int block_size;      /* DM_PRECOND_BLOCK: bases per spatial block (0: 128) */
   int batch;           /* several right-hand sides: 0 = lockstep batches when nrhs >= 8,
                           1 = always, -1 = one right-hand side at a time */
+  const dm_mpnn* learned; /* DM_PRECOND_LEARNED: the network (SPAI over nf_cutoff + network) */
 } dm_solve_opts;
 
 typedef struct dm_solve_stats {
@@ -193,6 +201,42 @@ int dm_precond_free(dm_precond* p);
 /* Solve Z X = B for nrhs right-hand sides (host buffers, caller order). */
 int dm_solve(dm_ctx* ctx, const dm_zmat* z, const double* B, int64_t ldb, int64_t nrhs,
              const dm_solve_opts* opts, double* X, int64_t ldx, dm_solve_stats* stats);
+
+/* ------------------------------------------------- learned preconditioner */
+/* No reference counterpart: a 4-layer message-passing network over the RWG
+ * edge-adjacency graph (bases sharing a triangle), hidden width 64, bfloat16
+ * weights with FP32 accumulation, that corrects the near-field SPAI inside
+ * FGMRES: M r = S r + network(S r, r) (opts.precond = DM_PRECOND_LEARNED,
+ * opts.learned = the network).  Trained on the GPU by the library's own
+ * forward / backward kernels and Adam on (residual, error) pairs. */
+int64_t dm_mpnn_param_count(void);
+int dm_mpnn_create(dm_ctx* ctx, const float* params, dm_mpnn** out);
+int dm_mpnn_get_params(const dm_mpnn* net, float* out);
+int dm_mpnn_free(dm_mpnn* net);
+/* The network's graph and geometry features of one mesh in caller order:
+ * nbr (N x 4, 0-based, -1 padded), sgn (N x 4), geom (N x 2). */
+int dm_rwg_graph(const dm_rwg* rwg, double k, int32_t* nbr, float* sgn, float* geom);
+/* Training samples resident on the device: nsample meshes' nodes concatenated
+ * (sample s owns nodes sample_ptr[s] .. sample_ptr[s+1]-1; nbr holds global
+ * node ids), z = S r, residual r and error e = Z^-1 r (complex128, N). */
+typedef struct dm_mpnn_data dm_mpnn_data;
+int dm_mpnn_data_create(dm_ctx* ctx, int64_t nnode, const int32_t* nbr, const float* sgn, const float* geom,
+                        const double* z, const double* r, const double* e, int64_t nsample,
+                        const int64_t* sample_ptr, dm_mpnn_data** out);
+int dm_mpnn_data_free(dm_mpnn_data* data);
+/* Mean loss over samples [s0, s1) and its gradient w.r.t. the bfloat16-rounded
+ * parameters (grad may be NULL). */
+int dm_mpnn_loss_grad(dm_ctx* ctx, dm_mpnn* net, const dm_mpnn_data* data, int64_t s0, int64_t s1,
+                      double* loss, float* grad);
+/* Adam training: `epochs` passes over mini-batches of batch_samples samples
+ * in a seeded shuffled order; epoch_loss (may be NULL) receives the mean loss
+ * of each epoch. */
+int dm_mpnn_train(dm_ctx* ctx, dm_mpnn* net, const dm_mpnn_data* data, int epochs, int64_t batch_samples,
+                  double lr, uint64_t seed, double* epoch_loss);
+/* The preconditioner's correction for one mesh: out = z + network(z, r)
+ * (caller order, complex128). */
+int dm_mpnn_apply(dm_ctx* ctx, const dm_mpnn* net, const dm_rwg* rwg, double k, const double* z,
+                  const double* r, double* out);
 
 /* solve_scattering(mesh, freq, excitation; method=:dense_direct|:dense_gmres)
  * for plane waves, end to end on the device (src/Workflow.jl:67-317): assembly,

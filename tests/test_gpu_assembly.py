@@ -196,3 +196,43 @@ def test_error_paths(dm):
     bad = dm.TriMesh(np.array([[0, 0, 0], [1, 0, 0], [2, 0, 0.0]]), np.array([[1, 2, 3]]))
     with pytest.raises(ValueError):
         dm.build_rwg(bad)  # degenerate triangle fails the precheck
+
+
+@pytest.mark.parametrize("mk", ["plate", "sphere"])
+def test_packed_symmetric_storage_and_product(dm, mk):
+    """Packed symmetric forms hold the entries of the full forms (upper triangle
+    of 128 x 128 tiles; separate assemblies agree to the rounding of the tile
+    accumulation, whose LDS atomics have no fixed order); the symmetric product
+    (one read per stored tile, used for Z x and Z^T x) matches the dense product
+    of the same stored entries to FP64 rounding."""
+    m = dm.make_rect_plate(0.3, 0.25, 30, 22) if mk == "plate" else dm.make_icosphere(0.1, 3)
+    r = dm.build_rwg(m)
+    allf = dm.DM_Z64 | dm.DM_Z32 | dm.DM_Z64P | dm.DM_Z32P
+    Zd = dm.assemble_Z_efie(m, r, K0, forms=allf, device=True)
+    Z64, Z32 = Zd.to_numpy(dm.DM_Z64), Zd.to_numpy(dm.DM_Z32)
+    P64, P32 = Zd.to_numpy(dm.DM_Z64P), Zd.to_numpy(dm.DM_Z32P)
+    assert np.array_equal(P64, P64.T) and np.array_equal(P32, P32.T)
+    assert np.max(np.abs(P64 - Z64)) <= 1e-13 * np.max(np.abs(Z64))
+    assert rel_fro(P64, Z64) <= 1e-14
+    assert rel_fro(P32, Z32) <= 1e-6
+    rng = np.random.default_rng(11)
+    idx = rng.integers(0, r.nedges, (2, 500))
+    assert np.array_equal(Zd.entries(idx[0], idx[1], dm.DM_Z64P), P64[idx[0], idx[1]])
+    X = rng.standard_normal((r.nedges, 3)) + 1j * rng.standard_normal((r.nedges, 3))
+    ref = P64 @ X
+    Y = Zd.matmul(X, form=dm.DM_Z64P)
+    assert np.linalg.norm(Y - ref) / np.linalg.norm(ref) < 1e-13
+    assert np.max(np.abs(Y - ref)) <= 1e-12 * np.max(np.abs(ref))
+    Y32 = Zd.matmul(X, form=dm.DM_Z32P)
+    assert np.linalg.norm(Y32 - P32 @ X) / np.linalg.norm(ref) < 1e-13  # complex64 entries, FP64 sums
+    assert np.linalg.norm(Y32 - ref) / np.linalg.norm(ref) < 1e-5
+    Ya = Zd.matmul(X, form=dm.DM_Z64P, adjoint=True)
+    assert np.linalg.norm(Ya - P64.conj().T @ X) / np.linalg.norm(ref) < 1e-13
+    # diagonal, block and near-field preconditioners read packed storage too
+    for fact, cut in (("diag", 0.0), ("block", 0.0), ("lu", 2.0 * float(np.mean(r.len)))):
+        P = dm.build_nearfield_preconditioner(Zd, m, r, cut, factorization=fact, block_size=64)
+        for f_full, f_packed, tol in ((dm.DM_Z64, dm.DM_Z64P, 1e-11), (dm.DM_Z32, dm.DM_Z32P, 1e-4)):
+            x = X[:, 0]
+            yf = dm.NearFieldOperator(P, Zd, f_full) @ x
+            yp = dm.NearFieldOperator(P, Zd, f_packed) @ x
+            assert np.linalg.norm(yp - yf) / np.linalg.norm(yf) < tol, (fact, f_packed)

@@ -281,3 +281,26 @@ def test_batched_planewaves_end_to_end(dm):
     assert np.linalg.norm(X - Id) / np.linalg.norm(Id) < 1e-4
     Xs, ss, _ = dm.solve_planewaves(m, r, K0, pws[:3], "mixed", P, tol=1e-6, maxiter=300, batch=-1)
     assert np.linalg.norm(Xs - X[:, :3]) / np.linalg.norm(X[:, :3]) < 1e-4
+
+
+@pytest.mark.parametrize("method", ["gmres", "mixed"])
+def test_solves_on_packed_forms(dm, method):
+    """GMRES / mixed-precision FGMRES on packed symmetric storage match the
+    same solves on full storage (same entries, products to FP64 rounding)."""
+    m = dm.make_icosphere(0.1, 3)
+    r = dm.build_rwg(m)
+    v = dm.assemble_v_plane_wave(m, r, [0, 0, -K0], 1.0, [1, 0, 0])
+    Zf = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64 | dm.DM_Z32, device=True)
+    Zp = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64P | dm.DM_Z32P, device=True)
+    P = dm.build_nearfield_preconditioner(None, m, r, 2.0 * float(np.mean(r.len)))
+    if method == "gmres":
+        xf, sf = dm.solve_gmres(Zf, v, preconditioner=P, precond_side="right", tol=1e-8)
+        xp, sp = dm.solve_gmres(Zp, v, preconditioner=P, precond_side="right", tol=1e-8)
+    else:
+        xf, sf = dm.solve_mixed(Zf, v, preconditioner=P, tol=1e-7)
+        xp, sp = dm.solve_mixed(Zp, v, preconditioner=P, tol=1e-7)
+    assert sf.solved and sp.solved
+    assert abs(sf.niter - sp.niter) <= 2
+    assert np.linalg.norm(xp - xf) / np.linalg.norm(xf) < 1e-5
+    Z = Zf.to_numpy()
+    assert np.linalg.norm(Z @ xp - v) / np.linalg.norm(v) <= (1.05e-8 if method == "gmres" else 1e-7)
