@@ -1,19 +1,26 @@
 #!/usr/bin/env python3
 """EFIE time-to-solve at 1e-6 true FP64 relative residual, N = 30720 sphere.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--subdiv S]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
+                    [--subdiv S] [--precond spai|learned]
 
-Workload: PEC icosphere r = 0.1 m, 5 subdivisions (Nt = 20480,
-N = 30720 RWG), k = 2 pi / 0.1, quad order 3, plane wave k_vec = (0, 0, -k),
-x-polarised, E0 = 1 (the configs[0] excitation).  Synthetic mesh, no data files.
+Workload (the mixed-precision sphere case): PEC icosphere r = 0.1 m, 5
+subdivisions (Nt = 20480, N = 30720 RWG), k = 2 pi / 0.1, quad order 3, plane
+wave k_vec = (0, 0, -k), x-polarised, E0 = 1.  Synthetic mesh, no data files.
 
 One step = one complete solve of the chosen form starting from the
-device-resident mesh + RWG handle: assembly of both stored forms (complex64
-from FP32 far-pair quadrature + FP64 singular terms; complex128 from FP64
-quadrature for the refinement residual), preconditioner set-up, FGMRES on the
-complex64 matrix with FP64 Arnoldi, FP64 iterative refinement until
-||b - Z64 x|| / ||b|| <= 1e-6.  The plain FP64 path (FP64 assembly, complex128
-products, FP64 GMRES to the same true residual) is timed in the same run.
+device-resident mesh + RWG handle: assembly of both stored forms in packed
+symmetric storage (complex64 from FP32 far-pair quadrature + FP64 singular
+terms; complex128 from FP64 quadrature, on a second stream, for the
+refinement residual), preconditioner set-up (near-field SPAI, or SPAI plus the
+trained message-passing network), FGMRES on the complex64 matrix with FP64
+Arnoldi, FP64 iterative refinement until ||b - Z64 x|| / ||b|| <= 1e-6.  The
+plain FP64 path (FP64 assembly, full complex128 storage and GEMV, FP64 GMRES
+to the same true residual, with no preconditioner or the SPAI, whichever is
+faster) is timed in the same run.
+
+Also measured (not the headline value): the many-angle case, 256 seeded plane
+waves on the same sphere solved in lockstep batches (Z X on MFMA).
 
 Prints one JSON line (time-like metric: value = seconds, lower is better).
 """
@@ -98,6 +105,21 @@ def cpu_baseline(mesh, rwg, k, plain_iters, budget_s=12.0):
                 assembly_s=round(t_asm, 3), matvec_ms=round(t_mv * 1e3, 2))
 
 
+def random_planewaves(dm, k, count, seed=20261202):
+    """Directions: normalised N(0, I3) draws; polarisations: random unit vectors
+    projected perpendicular to k (the many-angle workload's recipe)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        d = rng.standard_normal(3)
+        d /= np.linalg.norm(d)
+        p = rng.standard_normal(3)
+        p -= d * (p @ d)
+        p /= np.linalg.norm(p)
+        out.append(dm.make_plane_wave(k * d, 1.0, p))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +127,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--subdiv", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-plane-wave batched measurement")
+    ap.add_argument("--batch-rhs", type=int, default=256)
+    ap.add_argument("--precond", choices=["spai", "learned"], default="spai",
+                    help="chosen form's preconditioner (learned needs the trained weights)")
     ap.add_argument("--nf-cutoff", type=float, default=2.0, help="near-field cutoff in mean edge lengths")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -119,60 +145,101 @@ def main():
     N = rwg.nedges
     ctx = mesh.ctx
     khat, pol = np.array([0.0, 0.0, -1.0]), np.array([1.0, 0.0, 0.0])
-    v = dm.assemble_v_plane_wave(mesh, rwg, k * khat, 1.0, pol)
     log(f"sphere N={N} Nt={mesh.nt}")
 
     h = float(np.mean(rwg.len))
     nf = dm.build_nearfield_preconditioner(None, mesh, rwg, a.nf_cutoff * h)
+    weights = os.path.join(ROOT, "diffmom.jl_amd", "weights", "learned_precond.npz")
+    learned = None
+    if os.path.exists(weights):
+        learned = dm.LearnedPreconditioner(dm.MPNN.load(weights, ctx=ctx), a.nf_cutoff * h)
+    if a.precond == "learned" and learned is None:
+        raise SystemExit(f"bench.py: --precond learned needs {weights} (python diffmom.jl_amd/train_learned.py)")
+    chosen_P = learned if a.precond == "learned" else nf
     pw = [dm.make_plane_wave(k * khat, 1.0, pol)]
 
-    def solve(method, P):
+    def solve(method, P, waves=pw, maxiter=1000):
         ctx.sync()
         t0 = time.perf_counter()
-        X, st, t_asm = dm.solve_planewaves(mesh, rwg, k, pw, method, P, tol=1e-6, maxiter=1000)
+        X, st, t_asm = dm.solve_planewaves(mesh, rwg, k, waves, method, P, tol=1e-6, maxiter=maxiter)
         ctx.sync()
-        return time.perf_counter() - t0, X[:, 0], st, t_asm
+        return time.perf_counter() - t0, X, st, t_asm
 
     for i in range(a.warmup):
-        dt, _, st, _ = solve("mixed", nf)
+        dt, _, st, _ = solve("mixed", chosen_P)
         log(f"warmup {i}: {dt:.3f} s, {st.niter} it")
     ctx.profile_reset()
     ctx.profile(True)
     times = []
     for i in range(a.steps):
-        dt, x_c, st_c, tasm_c = solve("mixed", nf)
+        dt, X, st_c, tasm_c = solve("mixed", chosen_P)
+        x_c = X[:, 0]
         times.append(dt)
         log(f"step {i}: {dt:.3f} s (matrix ready {tasm_c * 1e3:.1f} ms, precond {st_c.precond_time_s * 1e3:.1f} ms), "
             f"{st_c.niter} it, {st_c.outer} refinements, residual {st_c.residual:.2e}")
     ctx.profile(False)
     kern = {}
-    for name in ("assemble_z64", "assemble_z32", "near_tables", "gemv_z32", "gemv_z64", "arnoldi", "precond"):
+    for name in ("assemble_z64", "assemble_z32", "near_tables", "symv_z32", "symv_z64", "gemv_z32", "gemv_z64",
+                 "arnoldi", "precond", "learned"):
         c, ms = ctx.profile_read(name)
         if c:
             kern[name] = dict(launches=c, total_ms=round(ms, 3), avg_ms=round(ms / c, 4))
     ctx.profile_reset()
+    variants = {}
+    other = ("learned", learned) if a.precond == "spai" else ("spai", nf)
+    if other[1] is not None:
+        solve("mixed", other[1])
+        dt, X, st_o, _ = solve("mixed", other[1])
+        variants[f"mixed_{other[0]}"] = dict(s=round(dt, 4), iterations=st_o.niter, refinements=st_o.outer,
+                                             true_residual=st_o.residual)
+        log(f"mixed ({other[0]}): {dt:.3f} s, {st_o.niter} it, residual {st_o.residual:.2e}")
     plain = {}
+    ctx.profile(True)
     for label, P in (("none", None), ("nearfield", nf)):
         solve("fp64", P)  # warm
         pt = []
         for i in range(max(1, min(a.steps, 2))):
-            dt, x_p, st_p, _ = solve("fp64", P)
+            dt, X, st_p, _ = solve("fp64", P)
             pt.append(dt)
             log(f"plain fp64 ({label}) {i}: {dt:.3f} s, {st_p.niter} it, residual {st_p.residual:.2e}")
-        plain[label] = dict(s=float(np.median(pt)), iterations=st_p.niter, true_residual=st_p.residual, x=x_p)
+        plain[label] = dict(s=float(np.median(pt)), iterations=st_p.niter, true_residual=st_p.residual, x=X[:, 0])
+    ctx.profile(False)
+    for name in ("gemv_z64", "assemble_z64"):
+        c, ms = ctx.profile_read(name)
+        if c:
+            kern["plain_" + name] = dict(launches=c, total_ms=round(ms, 3), avg_ms=round(ms / c, 4))
+    ctx.profile_reset()
     best = min(plain, key=lambda l: plain[l]["s"])
     t_c, t_p = float(np.median(times)), plain[best]["s"]
-    x_p, st_p = plain[best]["x"], plain[best]
+    x_p = plain[best]["x"]
     dx = float(np.linalg.norm(x_c - x_p) / np.linalg.norm(x_p))
 
+    batched = None
+    if not a.no_batch and a.batch_rhs > 0:
+        waves = random_planewaves(dm, k, a.batch_rhs)
+        dt, X, st_b, tb = solve("mixed", nf, waves, maxiter=300)
+        batched = dict(rhs=a.batch_rhs, s=round(dt, 3), s_per_rhs=round(dt / a.batch_rhs, 5),
+                       iterations_total=st_b.niter, iterations_mean=round(st_b.niter / a.batch_rhs, 1),
+                       refinements=st_b.outer, worst_true_residual=st_b.residual, matrix_ready_s=round(tb, 4),
+                       single_rhs_chosen_s=round(t_c, 4),
+                       speedup_per_rhs_vs_single_chosen=round(t_c * a.batch_rhs / dt, 2),
+                       speedup_per_rhs_vs_single_plain=round(t_p * a.batch_rhs / dt, 2),
+                       note="plain path not re-run per wave; per-RHS comparisons use the single-RHS times")
+        log(f"batched {a.batch_rhs} plane waves: {dt:.2f} s ({dt / a.batch_rhs * 1e3:.1f} ms per RHS), "
+            f"{st_b.niter} it total, worst residual {st_b.residual:.2e}")
+
     # roofline of the dominant kernel of the chosen form
-    steps = a.steps
     nq = 3
     pf = far_pairs(mesh, rwg)
-    alg = {"gemv_z32": ("hbm", 8.0 * N * N + 32.0 * N, HBM_PEAK_GBS, "GB/s"),
+    packed = 4.0 * N * (N + 1)  # upper triangle, complex64
+    alg = {"symv_z32": ("hbm", packed + 32.0 * N, HBM_PEAK_GBS, "GB/s"),
+           "symv_z64": ("hbm", 2 * packed + 32.0 * N, HBM_PEAK_GBS, "GB/s"),
+           "gemv_z32": ("hbm", 8.0 * N * N + 32.0 * N, HBM_PEAK_GBS, "GB/s"),
            "gemv_z64": ("hbm", 16.0 * N * N + 32.0 * N, HBM_PEAK_GBS, "GB/s"),
+           "plain_gemv_z64": ("hbm", 16.0 * N * N + 32.0 * N, HBM_PEAK_GBS, "GB/s"),
            "assemble_z32": ("valu", 52.0 * nq * nq * pf, FP32_PEAK_TFLOPS, "TFLOP/s"),
-           "assemble_z64": ("valu", 52.0 * nq * nq * pf, FP64_PEAK_TFLOPS, "TFLOP/s")}
+           "assemble_z64": ("valu", 52.0 * nq * nq * pf, FP64_PEAK_TFLOPS, "TFLOP/s"),
+           "plain_assemble_z64": ("valu", 52.0 * nq * nq * pf, FP64_PEAK_TFLOPS, "TFLOP/s")}
     fracs = {}
     for name, (bound, work, peak, unit) in alg.items():
         if name in kern:
@@ -180,26 +247,33 @@ def main():
             ach = work / avg_s / (1e9 if unit == "GB/s" else 1e12)
             fracs[name] = dict(bound=bound, achieved=round(ach, 2), peak=peak, unit=unit, frac=round(ach / peak, 4),
                                work_per_launch=work, avg_ms=kern[name]["avg_ms"])
-    dom = max(kern, key=lambda n: kern[n]["total_ms"])
-    roof = fracs.get(dom) or fracs.get("gemv_z32")
-    roofline = dict(kernel=dom if dom in fracs else "gemv_z32", bound=roof["bound"], achieved=roof["achieved"],
-                    peak=roof["peak"], unit=roof["unit"], frac=roof["frac"], traffic=None)
+    chosen_k = {n: v for n, v in kern.items() if not n.startswith("plain_")}
+    dom = max(chosen_k, key=lambda n: chosen_k[n]["total_ms"])
+    if dom not in fracs:
+        dom = "symv_z32" if "symv_z32" in fracs else "gemv_z32"
+    roof = fracs[dom]
+    roofline = dict(kernel=dom, bound=roof["bound"], achieved=roof["achieved"], peak=roof["peak"], unit=roof["unit"],
+                    frac=roof["frac"], traffic=None)
     cpu = None if a.no_cpu else cpu_baseline(mesh, rwg, k, plain["none"]["iterations"])
     out = {
-        "metric": METRIC, "value": round(t_c, 4), "unit": "s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
+        "metric": METRIC, "value": round(t_c, 4), "unit": "s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(t_c * 1e3, 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64+f64", "data": "synthetic",
         "config": {"workload": f"PEC icosphere r=0.1 m, {a.subdiv} subdivisions, N={N}, quad 3, k=2pi/0.1, "
                                "normal-incidence x-pol plane wave, tol 1e-6 true FP64 residual",
                    "N": N, "Nt": mesh.nt, "tol": 1e-6},
         "plain_fp64_s": round(t_p, 4), "speedup_vs_plain_fp64": round(t_p / t_c, 3),
-        "chosen": {"iterations": st_c.niter, "refinements": st_c.outer, "true_residual": st_c.residual,
-                   "precond": f"near-field SPAI cutoff {a.nf_cutoff} x mean edge", "matrix_ready_s": tasm_c,
-                   "precond_s": st_c.precond_time_s},
+        "chosen": {"form": "mixed precision, packed symmetric storage, FGMRES + FP64 refinement",
+                   "precond": (f"near-field SPAI cutoff {a.nf_cutoff} x mean edge" if a.precond == "spai" else
+                               f"SPAI ({a.nf_cutoff} x mean edge) + learned message-passing network"),
+                   "iterations": st_c.niter, "refinements": st_c.outer, "true_residual": st_c.residual,
+                   "matrix_ready_s": tasm_c, "precond_s": st_c.precond_time_s},
+        "variants": variants,
         "plain": {l: {"s": round(v["s"], 4), "iterations": v["iterations"], "true_residual": v["true_residual"]}
                   for l, v in plain.items()},
         "plain_best": best,
         "currents_rel_diff_chosen_vs_plain": dx,
+        "batched": batched,
         "kernels": kern, "kernel_roofline": fracs,
         "roofline": roofline, "cpu_baseline": cpu,
     }

@@ -81,7 +81,7 @@ class dm_solve_opts(C.Structure):
     _fields_ = [("method", C.c_int), ("precond", C.c_int), ("precond_side", C.c_int), ("maxiter", C.c_int),
                 ("tol", C.c_double), ("nf_cutoff", C.c_double), ("restart", C.c_int),
                 ("check_convergence", C.c_int), ("inner_tol", C.c_double), ("max_outer", C.c_int),
-                ("block_size", C.c_int), ("batch", C.c_int), ("learned", C.c_void_p)]
+                ("block_size", C.c_int), ("batch", C.c_int), ("learned", C.c_void_p), ("adjoint", C.c_int)]
 
 
 class dm_solve_stats(C.Structure):
@@ -112,6 +112,7 @@ def lib():
         "dm_rwg_size": ([vp, C.POINTER(i64)], i32), "dm_rwg_export": ([vp] + [vp] * 8, i32),
         "dm_rwg_free": ([vp], i32), "dm_rwg_adjacent_pairs": ([vp, C.POINTER(i64)], i32),
         "dm_assemble_z": ([vp, vp, d, d, d, i32, i32, i64, pp], i32),
+        "dm_assemble_z_multi": ([vp, i64, vp, d, d, d, i32, i32, vp], i32),
         "dm_zmat_size": ([vp, C.POINTER(i64), C.POINTER(i32)], i32),
         "dm_zmat_get": ([vp, i32, vp], i32), "dm_zmat_entries": ([vp, i32, vp, vp, i64, vp], i32),
         "dm_zmat_free": ([vp], i32),
@@ -501,6 +502,23 @@ class MatrixFreeEFIEOperator:
         return self.adjoint()
 
 
+def assemble_Z_efie_multi(meshes, rwgs, k, quad_order=3, eta0=ETA0, forms=DM_Z64):
+    """Device matrices of many meshes from one persistent assembly launch per
+    stored form (all workgroups share one tile queue over every mesh; no
+    reference counterpart).  Returns a list of ZMatrix."""
+    if len(meshes) != len(rwgs) or not meshes:
+        raise ValueError("need one RWG set per mesh")
+    for m, r in zip(meshes, rwgs):
+        _check_pair(m, r)
+    ctx = meshes[0].ctx
+    hs = (C.c_void_p * len(rwgs))(*[r.h.value for r in rwgs])
+    outs = (C.c_void_p * len(rwgs))()
+    kr, ki = _k_parts(k)
+    _check(lib().dm_assemble_z_multi(ctx.h, len(rwgs), C.cast(hs, C.c_void_p), kr, ki, float(eta0), int(quad_order),
+                                     int(forms), C.cast(outs, C.c_void_p)))
+    return [ZMatrix(r, C.c_void_p(h), forms, complex(k), float(eta0), quad_order) for r, h in zip(rwgs, outs)]
+
+
 def matrixfree_efie_operator(mesh, rwg, k, quad_order=3, eta0=ETA0, mesh_precheck=True, allow_boundary=True,
                              require_closed=False, area_tol_rel=1e-12):
     _check_pair(mesh, rwg)
@@ -610,7 +628,7 @@ class SolveStats:
 
 
 def _solve(Z, B, method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, check=True,
-           inner_tol=0.0, max_outer=10, batch=0):
+           inner_tol=0.0, max_outer=10, batch=0, adjoint=False):
     if not isinstance(Z, ZMatrix):
         raise TypeError("GPU solves take the device matrix (assemble_Z_efie(..., device=True))")
     B = np.asarray(B, dtype=np.complex128)
@@ -620,6 +638,7 @@ def _solve(Z, B, method, precond=None, side="right", tol=1e-6, maxiter=300, rest
         raise DimensionMismatch(f"rhs length {Bf.shape[0]} != {Z.N}")
     X = np.zeros_like(Bf, order="F")
     o = _opts(method, precond, side, tol, maxiter, restart, check, inner_tol, max_outer, batch)
+    o.adjoint = int(adjoint)
     st = dm_solve_stats()
     _check(lib().dm_solve(Z.ctx.h, Z.h, Bf.ctypes.data_as(C.c_void_p), Z.N, Bf.shape[1], C.byref(o),
                           X.ctypes.data_as(C.c_void_p), Z.N, C.byref(st)))
@@ -719,6 +738,21 @@ def solve_gmres(Z, rhs, preconditioner=None, precond_side="left", tol=1e-8, maxi
         raise ValueError(f"Invalid precond_side: {precond_side}")
     return _solve(Z, rhs, DM_METHOD_GMRES, preconditioner, precond_side, tol, maxiter, restart,
                   check_gmres_convergence, batch=batch)
+
+
+def solve_gmres_adjoint(Z, rhs, preconditioner=None, precond_side="left", tol=1e-8, maxiter=200, memory=20,
+                        verbose=False, check_gmres_convergence=True, restart=0, batch=0):
+    """Z^H x = rhs (src/solver/IterativeSolve.jl:409-460).  Z is exactly
+    symmetric, so the library solves Z conj(x) = conj(rhs) with the forward
+    operator and preconditioner and conjugates the result."""
+    if not (math.isfinite(tol) and tol > 0):
+        raise ValueError(f"GMRES tol must be finite and positive, got {tol}")
+    if maxiter < 1:
+        raise ValueError(f"GMRES maxiter must be at least 1, got {maxiter}")
+    if precond_side not in ("left", "right"):
+        raise ValueError(f"Invalid precond_side: {precond_side}")
+    return _solve(Z, rhs, DM_METHOD_GMRES, preconditioner, precond_side, tol, maxiter, restart,
+                  check_gmres_convergence, batch=batch, adjoint=True)
 
 
 def solve_mixed(Z, rhs, preconditioner=None, tol=1e-6, maxiter=300, inner_tol=0.0, max_outer=10,

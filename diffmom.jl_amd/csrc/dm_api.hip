@@ -185,7 +185,7 @@ int pick_form(const dm_zmat* z, bool f32, bool batched) {
 // write the currents in caller order to host X (leading dimension ldx).
 void solve_columns(dm_ctx* ctx, const dm_solve_opts& o, const dm_rwg& r, DevOp* A64, DevOp* A32, DevPrec* M,
                    const double* Bd, i64 n, i64 nrhs, double* X, i64 ldx, dm_solve_stats& st, double& worst,
-                   bool& all_ok, hipEvent_t z64_ready) {
+                   bool& all_ok, hipEvent_t z64_ready, bool conj_out = false) {
   hipStream_t s = ctx->stream;
   if (nrhs <= 0) return;
   if (!use_batch(o, nrhs, M)) {
@@ -199,6 +199,7 @@ void solve_columns(dm_ctx* ctx, const dm_solve_opts& o, const dm_rwg& r, DevOp* 
       worst = std::max(worst, res);
       all_ok = all_ok && ok;
       launch_permute(n, r.d_ext.p, x.p, xe.p, 0, s);
+      if (conj_out) dml::conj(n, xe.p, s);
       HIP_OK(hipMemcpyAsync(X + 2 * ldx * j, xe.p, 16 * n, hipMemcpyDeviceToHost, s));
     }
     HIP_OK(hipStreamSynchronize(s));
@@ -222,6 +223,7 @@ void solve_columns(dm_ctx* ctx, const dm_solve_opts& o, const dm_rwg& r, DevOp* 
     }
     st.outer_iterations = std::max(st.outer_iterations, outer);
     all_ok = all_ok && ok;
+    if (conj_out) dml::conj(n * R, xe.p, s);
     HIP_OK(hipMemcpy2DAsync(X + 2 * ldx * j0, 16 * ldx, xe.p, 16 * n, 16 * n, R, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
   }
@@ -497,6 +499,82 @@ int dm_assemble_z(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double k_im, do
     assemble_packed(z.get(), a, r, forms, s);
     check_flag(nt, s);
     *out = z.release();
+  });
+}
+
+int dm_assemble_z_multi(dm_ctx* ctx, int64_t nmesh, const dm_rwg* const* rwgs, double k_re, double k_im,
+                        double eta0, int quad_order, int forms, dm_zmat** outs) {
+  return guard([&] {
+    check_ctx(ctx, "context");
+    check_ctx(rwgs, "rwg list");
+    check_ctx(outs, "output handles");
+    DM_REQUIRE(nmesh >= 1 && nmesh <= 65536, DM_ERR_ARGUMENT, "mesh count out of range");
+    DM_REQUIRE(forms > 0 && (forms & ~kAllForms) == 0, DM_ERR_ARGUMENT,
+               "forms must be a mask of DM_Z64, DM_Z32, DM_Z64P, DM_Z32P");
+    for (i64 m = 0; m < nmesh; ++m) {
+      check_ctx(rwgs[m], "rwg");
+      DM_REQUIRE(rwgs[m]->ctx == ctx, DM_ERR_ARGUMENT, "RWG data belongs to another context");
+    }
+    validate_prefactors(k_re, k_im, eta0);
+    CtxScope sc(ctx);
+    hipStream_t s = ctx->stream;
+    std::vector<std::unique_ptr<dm_zmat>> zs(nmesh);
+    std::vector<NearTables> nts(nmesh);
+    std::vector<AsmArgs> args(nmesh);
+    int cap = 0;
+    {
+      ProfScope p(ctx, "near_tables", s);
+      for (i64 m = 0; m < nmesh; ++m) {
+        dm_rwg& r = const_cast<dm_rwg&>(*rwgs[m]);
+        const TriTables& T = dmh::tables(r, quad_order);
+        auto z = std::make_unique<dm_zmat>();
+        z->ctx = ctx;
+        z->rwg = rwgs[m];
+        z->n = r.n;
+        z->ld = round_ld(r.n);
+        z->forms = forms;
+        z->k_re = k_re;
+        z->k_im = k_im;
+        z->eta0 = eta0;
+        z->quad_order = quad_order;
+        args[m] = make_args(r, T, k_re, k_im, eta0, z->ld, nts[m]);
+        cap = std::max(cap, r.max_blk_tris);
+        zs[m] = std::move(z);
+      }
+    }
+    // work list: largest meshes first
+    std::vector<i64> order(nmesh);
+    for (i64 m = 0; m < nmesh; ++m) order[m] = m;
+    std::stable_sort(order.begin(), order.end(), [&](i64 a, i64 b) { return args[a].nblk > args[b].nblk; });
+    std::vector<AsmArgs> oargs(nmesh);
+    for (i64 q = 0; q < nmesh; ++q) oargs[q] = args[order[q]];
+    for (int form : {DM_Z64, DM_Z32, DM_Z64P, DM_Z32P}) {
+      if (!(forms & form)) continue;
+      std::vector<void*> ptrs(nmesh);
+      for (i64 q = 0; q < nmesh; ++q) {
+        dm_zmat& z = *zs[order[q]];
+        const i64 n = z.n;
+        if (form == DM_Z64) {
+          z.z64.alloc((size_t)z.ld * n * 2);
+          ptrs[q] = z.z64.p;
+        } else if (form == DM_Z32) {
+          z.z32.alloc((size_t)z.ld * n * 2);
+          ptrs[q] = z.z32.p;
+        } else if (form == DM_Z64P) {
+          z.z64p.alloc(packed_scalars(n));
+          HIP_OK(hipMemsetAsync(z.z64p.p, 0, z.z64p.bytes(), s));
+          ptrs[q] = z.z64p.p;
+        } else {
+          z.z32p.alloc(packed_scalars(n));
+          HIP_OK(hipMemsetAsync(z.z32p.p, 0, z.z32p.bytes(), s));
+          ptrs[q] = z.z32p.p;
+        }
+      }
+      ProfScope p(ctx, "assemble_multi", s);
+      launch_assemble_multi(oargs, ptrs, form, cap, s);
+    }
+    for (i64 m = 0; m < nmesh; ++m) check_flag(nts[m], s);
+    for (i64 m = 0; m < nmesh; ++m) outs[m] = zs[m].release();
   });
 }
 
@@ -778,6 +856,7 @@ void dm_solve_opts_default(dm_solve_opts* o) {
   o->block_size = 128;
   o->batch = 0;
   o->learned = nullptr;
+  o->adjoint = 0;
 }
 
 int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double k_im, double eta0, int quad_order,
@@ -978,11 +1057,13 @@ int dm_solve(dm_ctx* ctx, const dm_zmat* z, const double* B, int64_t ldb, int64_
       double worst = 0;
       for (i64 j = 0; j < nrhs; ++j) {
         HIP_OK(hipMemcpyAsync(be.p, B + 2 * ldb * j, 16 * n, hipMemcpyHostToDevice, s));
+        if (o.adjoint) dml::conj(n, be.p, s);
         launch_permute(n, r.d_ext.p, be.p, b.p, 1, s);
         HIP_OK(hipMemcpyAsync(x.p, b.p, 16 * n, hipMemcpyDeviceToDevice, s));
         dml::lu_solve(n, z->ld, A.p, ipiv.p, x.p, n, 1, s);
         worst = std::max(worst, true_residual(A64, n, b.p, x.p, tmp.p, hb.p, s));
         launch_permute(n, r.d_ext.p, x.p, be.p, 0, s);
+        if (o.adjoint) dml::conj(n, be.p, s);
         HIP_OK(hipMemcpyAsync(X + 2 * ldx * j, be.p, 16 * n, hipMemcpyDeviceToHost, s));
       }
       HIP_OK(hipStreamSynchronize(s));
@@ -1010,9 +1091,11 @@ int dm_solve(dm_ctx* ctx, const dm_zmat* z, const double* B, int64_t ldb, int64_
     Bd.alloc((size_t)2 * n * std::max<i64>(nrhs, 1));
     for (i64 j = 0; j < nrhs; ++j) {
       HIP_OK(hipMemcpyAsync(be.p, B + 2 * ldb * j, 16 * n, hipMemcpyHostToDevice, s));
+      if (o.adjoint) dml::conj(n, be.p, s);
       launch_permute(n, r.d_ext.p, be.p, Bd.p + 2 * n * j, 1, s);
     }
-    solve_columns(ctx, o, r, A64.get(), A32.get(), M.get(), Bd.p, n, nrhs, X, ldx, st, worst, all_ok, nullptr);
+    solve_columns(ctx, o, r, A64.get(), A32.get(), M.get(), Bd.p, n, nrhs, X, ldx, st, worst, all_ok, nullptr,
+                  o.adjoint != 0);
     st.solved = all_ok ? 1 : 0;
     st.residual = worst;
     st.solve_time_s = now() - t1;

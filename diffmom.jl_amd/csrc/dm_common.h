@@ -286,6 +286,10 @@ void launch_assemble_z32(const AsmArgs& a, float* z, int max_tris, hipStream_t s
 // packed symmetric storage (buffer of packed_scalars(n), zeroed beforehand)
 void launch_assemble_z64p(const AsmArgs& a, double* z, int max_tris, hipStream_t s);
 void launch_assemble_z32p(const AsmArgs& a, float* z, int max_tris, hipStream_t s);
+// Several meshes' matrices (one stored form) in one persistent launch;
+// packed forms need zeroed buffers.  Synchronises the stream on return.
+void launch_assemble_multi(const std::vector<AsmArgs>& args, const std::vector<void*>& z, int form, int max_tris,
+                           hipStream_t s);
 // matrix-free y += Z x in internal order (FP64), x and y complex128
 void launch_mf_apply(const AsmArgs& a, const double* x, double* y, int max_tris, hipStream_t s);
 

@@ -479,14 +479,11 @@ __device__ inline void tile_coords(int nblk, int& bi, int& bj, bool& valid) {
 }
 
 // MODE 0: write Z (AccT = storage scalar), MODE 1: matrix-free y += Z x.
+// One tile (bi <= bj) of one mesh: the whole workgroup's work.
 template <typename T, typename AccT, int NQ, int NTHR, int MODE>
-__global__ __launch_bounds__(NTHR) void k_tile(AsmArgs a, int cap, AccT* __restrict__ zout,
-                                                const double* __restrict__ x, double* __restrict__ y) {
+__device__ inline void tile_body(const AsmArgs& a, int cap, AccT* __restrict__ zout, const double* __restrict__ x,
+                                 double* __restrict__ y, int bi, int bj) {
   extern __shared__ __align__(16) char smem[];
-  int bi, bj;
-  bool valid;
-  tile_coords(a.nblk, bi, bj, valid);
-  if (!valid) return;
   const int TB = a.TB;
   const int ldt = TB + 1;
   AccT* zt = reinterpret_cast<AccT*>(smem);
@@ -665,6 +662,46 @@ __global__ __launch_bounds__(NTHR) void k_tile(AsmArgs a, int cap, AccT* __restr
   }
 }
 
+template <typename T, typename AccT, int NQ, int NTHR, int MODE>
+__global__ __launch_bounds__(NTHR) void k_tile(AsmArgs a, int cap, AccT* __restrict__ zout,
+                                                const double* __restrict__ x, double* __restrict__ y) {
+  int bi, bj;
+  bool valid;
+  tile_coords(a.nblk, bi, bj, valid);
+  if (!valid) return;
+  tile_body<T, AccT, NQ, NTHR, MODE>(a, cap, zout, x, y, bi, bj);
+}
+
+// Persistent assembly of several meshes in one launch: every workgroup takes
+// the next tile from a global counter over the concatenated tile lists
+// (meshes in the caller's order, largest first for balance) until all
+// cum[nmesh] tiles are done.
+template <typename T, typename AccT, int NQ, int NTHR, int MODE>
+__global__ __launch_bounds__(NTHR) void k_tile_multi(const AsmArgs* __restrict__ args, AccT* const* __restrict__ zouts,
+                                                      const i64* __restrict__ cum, int nmesh, int cap,
+                                                      int* __restrict__ counter) {
+  __shared__ int item;
+  const i64 total = cum[nmesh];
+  for (;;) {
+    if (threadIdx.x == 0) item = atomicAdd(counter, 1);
+    __syncthreads();
+    const i64 w = item;
+    __syncthreads();
+    if (w >= total) break;
+    int m = 0;
+    while (m + 1 < nmesh && cum[m + 1] <= w) ++m;
+    const i64 l = w - cum[m];
+    i64 b = (i64)((sqrt(8.0 * (double)l + 1.0) - 1.0) * 0.5);
+    while (b * (b + 1) / 2 > l) --b;
+    while ((b + 1) * (b + 2) / 2 <= l) ++b;
+    const AsmArgs a = args[m];
+    tile_body<T, AccT, NQ, NTHR, MODE>(a, cap, zouts[m], nullptr, nullptr, (int)(l - b * (b + 1) / 2), (int)b);
+    __syncthreads();
+  }
+}
+
+__global__ void k_fill_int(int* p, int v) { *p = v; }
+
 template <typename T, typename AccT>
 size_t tile_smem(int nq, int TB, int cap) {
   size_t b = sizeof(AccT) * 2 * TB * (TB + 1);
@@ -705,6 +742,53 @@ void launch_tile(const AsmArgs& a, int cap, AccT* z, const double* x, double* y,
       dm_throw(DM_ERR_ARGUMENT, "unsupported quadrature size");
   }
   HIP_OK(hipGetLastError());
+}
+
+template <typename T, typename AccT, int MODE>
+void launch_tile_multi(const std::vector<AsmArgs>& args, const std::vector<AccT*>& zouts, int cap, hipStream_t s) {
+  constexpr int NTHR = 512;
+  const int nmesh = (int)args.size();
+  DM_REQUIRE(nmesh >= 1, DM_ERR_ARGUMENT, "no meshes");
+  const int nq = args[0].nq;
+  for (const auto& a : args) DM_REQUIRE(a.nq == nq && a.TB == args[0].TB, DM_ERR_ARGUMENT, "meshes differ in quadrature");
+  size_t smem = tile_smem<T, AccT>(nq, args[0].TB, cap);
+  DM_REQUIRE(smem <= 160 * 1024, DM_ERR_ARGUMENT, "tile working set exceeds LDS");
+  std::vector<i64> cum(nmesh + 1, 0);
+  for (int m = 0; m < nmesh; ++m) cum[m + 1] = cum[m] + (i64)args[m].nblk * (args[m].nblk + 1) / 2;
+  DM_REQUIRE(cum[nmesh] < (1LL << 31), DM_ERR_ARGUMENT, "too many tiles for one launch");
+  DBuf<AsmArgs> dargs;
+  DBuf<AccT*> dz;
+  DBuf<i64> dcum;
+  DBuf<int> counter;
+  dargs.upload(args, s);
+  dz.upload(zouts, s);
+  dcum.upload(cum, s);
+  counter.alloc(1);
+  k_fill_int<<<1, 1, 0, s>>>(counter.p, 0);
+  int dev = 0, ncu = 0;
+  HIP_OK(hipGetDevice(&dev));
+  HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  switch (nq) {
+#define DM_CASE(NQ)                                                                                        \
+  case NQ: {                                                                                               \
+    auto kfn = k_tile_multi<T, AccT, NQ, NTHR, MODE>;                                                      \
+    HIP_OK(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)); \
+    int per_cu = 0;                                                                                        \
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, NTHR, smem));                        \
+    i64 grid = std::min<i64>(cum[nmesh], (i64)std::max(per_cu, 1) * ncu);                                  \
+    kfn<<<dim3((unsigned)grid), dim3(NTHR), smem, s>>>(dargs.p, dz.p, dcum.p, nmesh, cap, counter.p);      \
+    break;                                                                                                 \
+  }
+    DM_CASE(1)
+    DM_CASE(3)
+    DM_CASE(4)
+    DM_CASE(7)
+#undef DM_CASE
+    default:
+      dm_throw(DM_ERR_ARGUMENT, "unsupported quadrature size");
+  }
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(s));  // the argument tables are freed on return
 }
 
 }  // namespace
@@ -753,4 +837,23 @@ void launch_assemble_z32p(const AsmArgs& a, float* z, int cap, hipStream_t s) {
 }
 void launch_mf_apply(const AsmArgs& a, const double* x, double* y, int cap, hipStream_t s) {
   launch_tile<double, double, 1>(a, cap, nullptr, x, y, s);
+}
+
+void launch_assemble_multi(const std::vector<AsmArgs>& args, const std::vector<void*>& z, int form, int max_tris,
+                           hipStream_t s) {
+  if (form == DM_Z64 || form == DM_Z64P) {
+    std::vector<double*> zz;
+    for (void* p : z) zz.push_back(static_cast<double*>(p));
+    if (form == DM_Z64)
+      launch_tile_multi<double, double, 0>(args, zz, max_tris, s);
+    else
+      launch_tile_multi<double, double, 2>(args, zz, max_tris, s);
+  } else {
+    std::vector<float*> zz;
+    for (void* p : z) zz.push_back(static_cast<float*>(p));
+    if (form == DM_Z32)
+      launch_tile_multi<float, float, 0>(args, zz, max_tris, s);
+    else
+      launch_tile_multi<float, float, 2>(args, zz, max_tris, s);
+  }
 }

@@ -115,6 +115,12 @@ int dm_rwg_adjacent_pairs(const dm_rwg* rwg, int64_t* count);
  * N*N*16 as the reference's byte gate does (<= 0: no gate). */
 int dm_assemble_z(dm_ctx* ctx, const dm_rwg* rwg, double k_re, double k_im, double eta0,
                   int quad_order, int forms, int64_t max_output_bytes, dm_zmat** out);
+/* Many meshes in one persistent launch per stored form (no reference
+ * counterpart; batches of small scatterers): every workgroup pulls the next
+ * tile from one queue over all meshes' tiles, largest meshes first.  outs[m]
+ * receives mesh m's matrix, exactly what dm_assemble_z would produce. */
+int dm_assemble_z_multi(dm_ctx* ctx, int64_t nmesh, const dm_rwg* const* rwgs, double k_re, double k_im,
+                        double eta0, int quad_order, int forms, dm_zmat** outs);
 int dm_zmat_size(const dm_zmat* z, int64_t* n, int* forms);
 /* Copy Z (caller basis order, column-major N x N, complex128) to host.
  * form = one stored form (complex64 values widened). */
@@ -174,6 +180,9 @@ typedef struct dm_solve_opts {
   int batch;           /* several right-hand sides: 0 = lockstep batches when nrhs >= 8,
                           1 = always, -1 = one right-hand side at a time */
   const dm_mpnn* learned; /* DM_PRECOND_LEARNED: the network (SPAI over nf_cutoff + network) */
+  int adjoint;         /* dm_solve: 1 solves Z^H X = B (solve_gmres_adjoint,
+                          src/solver/IterativeSolve.jl:409-460); Z = Z^T, so Z^H x = b
+                          is solved as Z conj(x) = conj(b) */
 } dm_solve_opts;
 
 typedef struct dm_solve_stats {

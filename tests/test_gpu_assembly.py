@@ -236,3 +236,21 @@ def test_packed_symmetric_storage_and_product(dm, mk):
             yf = dm.NearFieldOperator(P, Zd, f_full) @ x
             yp = dm.NearFieldOperator(P, Zd, f_packed) @ x
             assert np.linalg.norm(yp - yf) / np.linalg.norm(yf) < tol, (fact, f_packed)
+
+
+def test_multi_mesh_persistent_assembly(dm, O):
+    """One persistent launch over 12 seeded random plates (0.05-0.3 m sides,
+    4-30 cells) and a sphere: each matrix equals its own single-mesh assembly
+    (to the rounding of the tile accumulation order) and the oracle."""
+    rng = np.random.default_rng(20261202)
+    meshes = [dm.make_rect_plate(*rng.uniform(0.05, 0.3, 2), *map(int, rng.integers(4, 31, 2))) for _ in range(12)]
+    meshes.append(dm.make_icosphere(0.1, 2))
+    rwgs = [dm.build_rwg(m) for m in meshes]
+    Zs = dm.assemble_Z_efie_multi(meshes, rwgs, K0, forms=dm.DM_Z64 | dm.DM_Z32P)
+    for i, (m, r, Z) in enumerate(zip(meshes, rwgs, Zs)):
+        single = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64 | dm.DM_Z32P, device=True)
+        A, B = Z.to_numpy(dm.DM_Z64), single.to_numpy(dm.DM_Z64)
+        assert rel_fro(A, B) <= 1e-14, i
+        assert rel_fro(Z.to_numpy(dm.DM_Z32P), single.to_numpy(dm.DM_Z32P)) <= 1e-6, i
+        if i in (0, 12):
+            assert rel_fro(A, _oracle_Z(O, m, r, K0)) <= 1e-10

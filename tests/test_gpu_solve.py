@@ -304,3 +304,18 @@ def test_solves_on_packed_forms(dm, method):
     assert np.linalg.norm(xp - xf) / np.linalg.norm(xf) < 1e-5
     Z = Zf.to_numpy()
     assert np.linalg.norm(Z @ xp - v) / np.linalg.norm(v) <= (1.05e-8 if method == "gmres" else 1e-7)
+
+
+def test_adjoint_solve(dm, c0):
+    """solve_gmres_adjoint: Z^H x = b (src/solver/IterativeSolve.jl:409-460)."""
+    m, r, Z, v = c0
+    Zh = Z.to_numpy()
+    x, st = dm.solve_gmres_adjoint(Z, v, tol=1e-10, maxiter=200)
+    assert st.solved
+    assert np.linalg.norm(Zh.conj().T @ x - v) / np.linalg.norm(v) < 1e-9
+    xd = np.linalg.solve(Zh.conj().T, v)
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8
+    X, st = dm.solve_gmres_adjoint(Z, np.stack([v, 1j * v, v.conj()], 1), precond_side="right", tol=1e-10, batch=1)
+    assert np.linalg.norm(Zh.conj().T @ X[:, 2] - v.conj()) / np.linalg.norm(v) < 1e-9
+    xl = dm._solve(Z, v, dm.DM_METHOD_LU, adjoint=True)[0]
+    assert np.linalg.norm(xl - xd) / np.linalg.norm(xd) < 1e-10
