@@ -927,20 +927,6 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
           launch_assemble_z32p(a, z->z32p.p, r.max_blk_tris, s);
         }
       }
-      // FP64 matrix on the second stream, concurrent with the complex64 solve
-      HIP_OK(hipStreamWaitEvent(s2, e_near, 0));
-      {
-        ProfScope p(ctx, "assemble_z64", s2);
-        if (batched) {
-          z->z64.alloc((size_t)z->ld * n * 2);
-          launch_assemble_z64(a, z->z64.p, r.max_blk_tris, s2);
-        } else {
-          z->z64p.alloc(packed_scalars(n));
-          HIP_OK(hipMemsetAsync(z->z64p.p, 0, z->z64p.bytes(), s2));
-          launch_assemble_z64p(a, z->z64p.p, r.max_blk_tris, s2);
-        }
-      }
-      HIP_OK(hipEventRecord(e_z64, s2));
     } else {
       z->z64.alloc((size_t)z->ld * n * 2);
       ProfScope p(ctx, "assemble_z64", s);
@@ -991,6 +977,24 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
       auto M = build_precond(z.get(), pick_form(z.get(), o.method == DM_METHOD_MIXED, batched), o, s);
       HIP_OK(hipStreamSynchronize(s));
       st.precond_time_s = now() - t1;
+      if (o.method == DM_METHOD_MIXED) {
+        // FP64 matrix on the second stream, concurrent with the complex64 solve
+        // (started after the preconditioner build, which it would slow down;
+        // the solve waits for it only at its first FP64 residual)
+        HIP_OK(hipStreamWaitEvent(s2, e_near, 0));
+        {
+          ProfScope p(ctx, "assemble_z64", s2);
+          if (batched) {
+            z->z64.alloc((size_t)z->ld * n * 2);
+            launch_assemble_z64(a, z->z64.p, r.max_blk_tris, s2);
+          } else {
+            z->z64p.alloc(packed_scalars(n));
+            HIP_OK(hipMemsetAsync(z->z64p.p, 0, z->z64p.bytes(), s2));
+            launch_assemble_z64p(a, z->z64p.p, r.max_blk_tris, s2);
+          }
+        }
+        HIP_OK(hipEventRecord(e_z64, s2));
+      }
       solve_columns(ctx, o, r, A64.get(), A32.get(), M.get(), V.p, n, nrhs, X, n, st, worst, all_ok,
                     o.method == DM_METHOD_MIXED ? e_z64 : nullptr);
     }

@@ -130,10 +130,11 @@ std::unique_ptr<DevPrec> make_learned_precond(const dm_zmat* z, int form, const 
 struct Krylov {
   i64 n = 0;
   int m = 0;  // basis capacity
-  DBuf<double> V, Zs, w, t, hbuf, ybuf;
+  DBuf<double> V, Zs, w, t, hbuf, ybuf, part;
   void reserve(i64 n_, int m_, bool flexible) {
     n = n_;
     m = m_;
+    part.alloc(2 * dml::dots_part_count(n, m + 1));
     V.alloc((size_t)2 * n * (m + 1));
     if (flexible) Zs.alloc((size_t)2 * n * m);
     w.alloc(2 * n);

@@ -149,9 +149,9 @@ GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double*
       double* h2 = K.hbuf.p + 2 * k;
       {
         ProfScope p(ctx, "arnoldi", s);
-        dml::dots(n, ldv / 2, k, V, K.w.p, h1, s);
+        dml::dots(n, ldv / 2, k, V, K.w.p, h1, K.part.p, s);
         dml::update(n, ldv / 2, k, V, h1, K.w.p, s);
-        dml::dots(n, ldv / 2, k, V, K.w.p, h2, s);
+        dml::dots(n, ldv / 2, k, V, K.w.p, h2, K.part.p, s);
         dml::update(n, ldv / 2, k, V, h2, K.w.p, s);
         dml::norm2(n, K.w.p, K.hbuf.p + 4 * k, s);
       }

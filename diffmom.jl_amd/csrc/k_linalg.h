@@ -3,7 +3,9 @@
 #include "dm_common.h"
 
 namespace dml {
-void dots(i64 n, i64 ldv, int k, const double* V, const double* w, double* h, hipStream_t s);
+// h[i] = V_i^H w for i < k; part holds dots_part_count(n, k) complex partials
+size_t dots_part_count(i64 n, int k);
+void dots(i64 n, i64 ldv, int k, const double* V, const double* w, double* h, double* part, hipStream_t s);
 void update(i64 n, i64 ldv, int k, const double* V, const double* h, double* w, hipStream_t s);
 void combine(i64 n, i64 ldv, int k, const double* V, const double* y, double* x, hipStream_t s);
 void norm2(i64 n, const double* w, double* out, hipStream_t s);

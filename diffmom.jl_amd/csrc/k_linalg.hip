@@ -125,23 +125,50 @@ __global__ void k_permute(i64 n, const int* __restrict__ idx, const double2* __r
 }
 
 // ---------------------------------------------------------------- Krylov ops
-// h[i] = sum_n conj(V[n, i]) w[n] for i < k; one workgroup per basis vector.
-__global__ __launch_bounds__(256) void k_dots(i64 n, i64 ldv, const double2* __restrict__ V,
-                                              const double2* __restrict__ w, double2* __restrict__ h) {
-  const int i = blockIdx.x;
-  const double2* v = V + (i64)i * ldv;
-  double ar = 0, ai = 0;
-  for (i64 m = threadIdx.x; m < n; m += 256) {
-    double2 a = v[m], b = w[m];
-    ar += a.x * b.x + a.y * b.y;
-    ai += a.x * b.y - a.y * b.x;
+// Row-chunked dot products for the Arnoldi step: workgroup (c, g) forms the
+// partial sums of vectors g*kDotVG .. over rows c*256 .. c*256+255 (w read
+// once per group, every V entry once), reduced over chunks by k_dots_reduce.
+constexpr int kDotVG = 8;
+
+__global__ __launch_bounds__(256) void k_dots_part(i64 n, i64 ldv, int k, const double2* __restrict__ V,
+                                                   const double2* __restrict__ w, double2* __restrict__ part,
+                                                   int nch) {
+  __shared__ double2 red[4][kDotVG];
+  const int c = blockIdx.x, i0 = blockIdx.y * kDotVG;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const i64 m = (i64)c * 256 + threadIdx.x;
+  const double2 wm = m < n ? w[m] : make_double2(0.0, 0.0);
+  const int cnt = min(kDotVG, k - i0);
+  for (int q = 0; q < cnt; ++q) {
+    double ar = 0, ai = 0;
+    if (m < n) {
+      double2 a = V[(i64)(i0 + q) * ldv + m];
+      ar = a.x * wm.x + a.y * wm.y;
+      ai = a.x * wm.y - a.y * wm.x;
+    }
+    ar = wave_sum(ar);
+    ai = wave_sum(ai);
+    if (lane == 0) red[wave][q] = make_double2(ar, ai);
   }
-  __shared__ double sr[4], si[4];
-  ar = wave_sum(ar);
-  ai = wave_sum(ai);
-  if ((threadIdx.x & 63) == 0) { sr[threadIdx.x >> 6] = ar; si[threadIdx.x >> 6] = ai; }
   __syncthreads();
-  if (threadIdx.x == 0) h[i] = make_double2(sr[0] + sr[1] + sr[2] + sr[3], si[0] + si[1] + si[2] + si[3]);
+  if (threadIdx.x < cnt) {
+    const int q = threadIdx.x;
+    part[(i64)(i0 + q) * nch + c] = make_double2(red[0][q].x + red[1][q].x + red[2][q].x + red[3][q].x,
+                                                 red[0][q].y + red[1][q].y + red[2][q].y + red[3][q].y);
+  }
+}
+
+__global__ void k_dots_reduce(int nch, const double2* __restrict__ part, double2* __restrict__ h) {
+  const int i = blockIdx.x;
+  double sr = 0, si = 0;
+  for (int c = threadIdx.x; c < nch; c += 64) {
+    double2 v = part[(i64)i * nch + c];
+    sr += v.x;
+    si += v.y;
+  }
+  sr = wave_sum(sr);
+  si = wave_sum(si);
+  if (threadIdx.x == 0) h[i] = make_double2(sr, si);
 }
 
 // w[n] -= sum_{i<k} V[n, i] h[i]  (acc_h: h_acc[i] += h[i] for CGS2 bookkeeping)
@@ -486,10 +513,15 @@ void launch_cast_c128_to_c64_matrix(i64 n, i64 ld, const double* src, float* dst
 
 namespace dml {
 
-void dots(i64 n, i64 ldv, int k, const double* V, const double* w, double* h, hipStream_t s) {
+size_t dots_part_count(i64 n, int k) { return (size_t)k * (size_t)nblocks(n, 256); }
+
+void dots(i64 n, i64 ldv, int k, const double* V, const double* w, double* h, double* part, hipStream_t s) {
   if (k <= 0) return;
-  k_dots<<<k, 256, 0, s>>>(n, ldv, reinterpret_cast<const double2*>(V), reinterpret_cast<const double2*>(w),
-                           reinterpret_cast<double2*>(h));
+  const int nch = (int)nblocks(n, 256);
+  dim3 g((unsigned)nch, (unsigned)((k + kDotVG - 1) / kDotVG));
+  k_dots_part<<<g, 256, 0, s>>>(n, ldv, k, reinterpret_cast<const double2*>(V), reinterpret_cast<const double2*>(w),
+                                reinterpret_cast<double2*>(part), nch);
+  k_dots_reduce<<<k, 64, 0, s>>>(nch, reinterpret_cast<const double2*>(part), reinterpret_cast<double2*>(h));
   HIP_OK(hipGetLastError());
 }
 void update(i64 n, i64 ldv, int k, const double* V, const double* h, double* w, hipStream_t s) {
