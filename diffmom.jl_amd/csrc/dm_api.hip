@@ -101,6 +101,26 @@ AsmArgs make_args(dm_rwg& r, const TriTables& T, double kr, double ki, double et
   a.jadj = nt.jadj.p;
   a.flag = nt.flag.p;
   launch_near_tables(a, T, nt.jself.p, nt.jadj.p, s);
+  // tile-side images (depend on the RWG and quadrature order only; kept on the RWG)
+  a.side_cap = r.max_blk_tris;
+  {
+    std::lock_guard<std::mutex> lk(r.tab_mu);
+    for (int f = 0; f < 2; ++f) {
+      DBuf<char>& img = r.side_img[T.quad_order][f];
+      const size_t stride = side_image_bytes(nq, f == 1, r.max_blk_tris);
+      if (!img.p) {
+        img.alloc(stride * r.nblk);
+        launch_build_sides(a, r.max_blk_tris, f == 1, img.p, s);
+      }
+      if (f == 1) {
+        a.side_img64 = img.p;
+        a.side_stride64 = (i64)stride;
+      } else {
+        a.side_img32 = img.p;
+        a.side_stride32 = (i64)stride;
+      }
+    }
+  }
   return a;
 }
 
@@ -548,6 +568,27 @@ int dm_assemble_z_multi(dm_ctx* ctx, int64_t nmesh, const dm_rwg* const* rwgs, d
     std::stable_sort(order.begin(), order.end(), [&](i64 a, i64 b) { return args[a].nblk > args[b].nblk; });
     std::vector<AsmArgs> oargs(nmesh);
     for (i64 q = 0; q < nmesh; ++q) oargs[q] = args[order[q]];
+    // the persistent kernel lays out LDS for the largest block; meshes with
+    // smaller blocks get side images built for that layout
+    std::vector<DBuf<char>> imgs(2 * nmesh);
+    for (i64 q = 0; q < nmesh; ++q) {
+      AsmArgs& a = oargs[q];
+      if (a.side_cap == cap) continue;
+      for (int f = 0; f < 2; ++f) {
+        const size_t stride = side_image_bytes(a.nq, f == 1, cap);
+        DBuf<char>& img = imgs[2 * q + f];
+        img.alloc(stride * a.nblk);
+        launch_build_sides(a, cap, f == 1, img.p, s);
+        if (f == 1) {
+          a.side_img64 = img.p;
+          a.side_stride64 = (i64)stride;
+        } else {
+          a.side_img32 = img.p;
+          a.side_stride32 = (i64)stride;
+        }
+      }
+      a.side_cap = cap;
+    }
     for (int form : {DM_Z64, DM_Z32, DM_Z64P, DM_Z32P}) {
       if (!(forms & form)) continue;
       std::vector<void*> ptrs(nmesh);

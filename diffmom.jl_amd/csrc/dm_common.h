@@ -163,6 +163,8 @@ struct dm_rwg {
   std::vector<double> cen_i;  // internal-order basis centres (3 per basis)
   double mean_len = 0;
   std::unique_ptr<SpaiPattern> spai;
+  // tile-side images per quadrature order and precision (built lazily)
+  DBuf<char> side_img[8][2];
 };
 
 // ------------------------------------------------------------ Z storage
@@ -277,10 +279,19 @@ struct AsmArgs {
   double k_re, k_im, eta0;
   int nq;
   int* flag;  // set to 1 when an assembled entry is non-finite
+  // per-block LDS images of the tile sides (built once per RWG, quadrature
+  // order and precision): block b at img + b * stride
+  const char* side_img32;
+  const char* side_img64;
+  i64 side_stride32, side_stride64;
+  int side_cap;
 };
 
 void launch_near_tables(const AsmArgs& a, const TriTables& t, double* jself, double* jadj,
                         hipStream_t s);
+// Bytes of one block's tile-side image and the kernel that fills nblk of them.
+size_t side_image_bytes(int nq, bool f64, int cap);
+void launch_build_sides(const AsmArgs& a, int cap, bool f64, char* img, hipStream_t s);
 void launch_assemble_z64(const AsmArgs& a, double* z, int max_tris, hipStream_t s);
 void launch_assemble_z32(const AsmArgs& a, float* z, int max_tris, hipStream_t s);
 // packed symmetric storage (buffer of packed_scalars(n), zeroed beforehand)

@@ -499,8 +499,20 @@ __device__ inline void tile_body(const AsmArgs& a, int cap, AccT* __restrict__ z
   const int nI = a.blk_off[bi + 1] - a.blk_off[bi];
   const int nJ = a.blk_off[bj + 1] - a.blk_off[bj];
   for (int k = tid; k < 2 * TB * ldt; k += NTHR) zt[k] = (AccT)0;
-  load_side<T, NQ>(a, bi, nI, R, tid, NTHR);
-  load_side<T, NQ>(a, bj, nJ, C, tid, NTHR);
+  {
+    // both sides' images are contiguous in LDS (R then C): 16-byte copies
+    const i64 stride = sizeof(T) == 8 ? a.side_stride64 : a.side_stride32;
+    const char* img = sizeof(T) == 8 ? a.side_img64 : a.side_img32;
+    const int4* sr = reinterpret_cast<const int4*>(img + (i64)bi * stride);
+    const int4* sc = reinterpret_cast<const int4*>(img + (i64)bj * stride);
+    int4* dr = reinterpret_cast<int4*>(R.c);
+    int4* dc = reinterpret_cast<int4*>(reinterpret_cast<char*>(R.c) + stride);
+    const int nv = (int)(stride / 16);
+    for (int k = tid; k < nv; k += NTHR) {
+      dr[k] = sr[k];
+      dc[k] = sc[k];
+    }
+  }
   for (int k = tid; k < TB; k += NTHR) {
     i64 gi = (i64)bi * TB + k, gj = (i64)bj * TB + k;
     extr[k] = gi < a.n ? a.ext[gi] : 0x7fffffff;
@@ -702,6 +714,16 @@ __global__ __launch_bounds__(NTHR) void k_tile_multi(const AsmArgs* __restrict__
 
 __global__ void k_fill_int(int* p, int v) { *p = v; }
 
+// One block's tile-side image in global memory, laid out exactly as the
+// tile kernel's LDS (carve_side), so tiles copy it with 16-byte loads.
+template <typename T, int NQ>
+__global__ void k_build_sides(AsmArgs a, int cap, char* img, i64 stride) {
+  const int b = blockIdx.x;
+  TileSide<T, NQ> S;
+  carve_side<T, NQ>(img + (i64)b * stride, cap, S);
+  load_side<T, NQ>(a, b, a.blk_off[b + 1] - a.blk_off[b], S, threadIdx.x, blockDim.x);
+}
+
 template <typename T, typename AccT>
 size_t tile_smem(int nq, int TB, int cap) {
   size_t b = sizeof(AccT) * 2 * TB * (TB + 1);
@@ -856,4 +878,35 @@ void launch_assemble_multi(const std::vector<AsmArgs>& args, const std::vector<v
     else
       launch_tile_multi<float, float, 2>(args, zz, max_tris, s);
   }
+}
+
+size_t side_image_bytes(int nq, bool f64, int cap) {
+  switch (nq) {
+    case 1: return f64 ? side_bytes<double, 1>(cap) : side_bytes<float, 1>(cap);
+    case 3: return f64 ? side_bytes<double, 3>(cap) : side_bytes<float, 3>(cap);
+    case 4: return f64 ? side_bytes<double, 4>(cap) : side_bytes<float, 4>(cap);
+    case 7: return f64 ? side_bytes<double, 7>(cap) : side_bytes<float, 7>(cap);
+    default: dm_throw(DM_ERR_ARGUMENT, "unsupported quadrature size");
+  }
+}
+
+void launch_build_sides(const AsmArgs& a, int cap, bool f64, char* img, hipStream_t s) {
+  const i64 stride = (i64)side_image_bytes(a.nq, f64, cap);
+  switch (a.nq) {
+#define DM_CASE(NQ)                                                                          \
+  case NQ:                                                                                   \
+    if (f64)                                                                                 \
+      k_build_sides<double, NQ><<<a.nblk, 64, 0, s>>>(a, cap, img, stride);                  \
+    else                                                                                     \
+      k_build_sides<float, NQ><<<a.nblk, 64, 0, s>>>(a, cap, img, stride);                   \
+    break;
+    DM_CASE(1)
+    DM_CASE(3)
+    DM_CASE(4)
+    DM_CASE(7)
+#undef DM_CASE
+    default:
+      dm_throw(DM_ERR_ARGUMENT, "unsupported quadrature size");
+  }
+  HIP_OK(hipGetLastError());
 }

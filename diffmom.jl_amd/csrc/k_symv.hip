@@ -195,7 +195,19 @@ __global__ void k_symv_reduce(i64 n, int nb, const double2* __restrict__ rowpart
   if (g >= n) return;
   const int R = (int)(g / kPT), r = (int)(g % kPT);
   double sr = 0, si = 0;
-  for (int J = R; J < nb; ++J) {
+  // eight independent loads in flight per step (the sum order stays fixed)
+  int J = R;
+  for (; J + 8 <= nb; J += 8) {
+    double2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = rowpart[((i64)(J + u) * (J + u + 1) / 2 + R) * kPT + r];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      sr += v[u].x;
+      si += v[u].y;
+    }
+  }
+  for (; J < nb; ++J) {
     double2 v = rowpart[((i64)J * (J + 1) / 2 + R) * kPT + r];
     sr += v.x;
     si += v.y;
