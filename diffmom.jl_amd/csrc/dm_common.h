@@ -162,7 +162,7 @@ struct dm_rwg {
   std::mutex tab_mu;
   std::vector<double> cen_i;  // internal-order basis centres (3 per basis)
   double mean_len = 0;
-  std::unique_ptr<SpaiPattern> spai;
+  std::shared_ptr<SpaiPattern> spai;  // latest pattern; preconditioners keep their own reference
   // tile-side images per quadrature order and precision (built lazily)
   DBuf<char> side_img[8][2];
 };
@@ -244,7 +244,7 @@ void mesh_check(const dm_mesh& m, int allow_boundary, int require_closed, double
 double triangle_area(const dm_mesh& m, i64 t0);
 const TriTables& tables(dm_rwg& r, int quad_order);
 int quad_rule(int order, double* xi, double* w);
-const SpaiPattern& spai_pattern(dm_rwg& r, double cutJ, int maxJ);
+std::shared_ptr<const SpaiPattern> spai_pattern(dm_rwg& r, double cutJ, int maxJ);
 }  // namespace dmh
 
 namespace dmp {

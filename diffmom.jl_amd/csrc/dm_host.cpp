@@ -387,10 +387,10 @@ void finalize_rwg(dm_rwg& r) {
 // within 1.6 x the radius actually covered by J_j (at most 512 nearest).
 // Centres are the midpoints of the two triangle centroids
 // (rwg_centers, src/solver/NearFieldPreconditioner.jl:247-257).
-const SpaiPattern& spai_pattern(dm_rwg& r, double cutJ, int maxJ) {
+std::shared_ptr<const SpaiPattern> spai_pattern(dm_rwg& r, double cutJ, int maxJ) {
   std::lock_guard<std::mutex> lk(r.tab_mu);
-  if (r.spai && r.spai->cutJ == cutJ) return *r.spai;
-  auto P = std::make_unique<SpaiPattern>();
+  if (r.spai && r.spai->cutJ == cutJ) return r.spai;
+  auto P = std::make_shared<SpaiPattern>();
   P->cutJ = cutJ;
   i64 n = r.n;
   const double* c = r.cen_i.data();
@@ -474,8 +474,8 @@ const SpaiPattern& spai_pattern(dm_rwg& r, double cutJ, int maxJ) {
   P->d_colidx.upload(P->colidx, s);
   P->d_map.upload(P->map, s);
   HIP_OK(hipStreamSynchronize(s));
-  r.spai = std::move(P);
-  return *r.spai;
+  r.spai = P;
+  return P;
 }
 
 // Per-triangle tables for one quadrature order: quad points via the affine map

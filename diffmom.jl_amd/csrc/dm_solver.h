@@ -104,7 +104,7 @@ struct PrecDiag : DevPrec {
   void apply(const double* x, double* y, hipStream_t s) override { dml::diag_apply(n, d.p, x, y, s); }
 };
 struct PrecSpai : DevPrec {
-  const SpaiPattern* pat;
+  std::shared_ptr<const SpaiPattern> pat;
   DBuf<double> val;
   void apply(const double* x, double* y, hipStream_t s) override {
     dmp::spmv(n, pat->d_rowptr.p, pat->d_colidx.p, val.p, x, y, s);

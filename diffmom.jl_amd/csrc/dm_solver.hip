@@ -36,10 +36,11 @@ std::unique_ptr<DevPrec> build_precond(const dm_zmat* z, int form, const dm_solv
   if (o.precond == DM_PRECOND_NEARFIELD) {
     dm_rwg& r = const_cast<dm_rwg&>(*z->rwg);
     double cutJ = o.nf_cutoff > 0 ? o.nf_cutoff : 2.5 * r.mean_len;
-    const SpaiPattern& P = dmh::spai_pattern(r, cutJ, dmp::spai_max_cols());
+    std::shared_ptr<const SpaiPattern> pat = dmh::spai_pattern(r, cutJ, dmp::spai_max_cols());
+    const SpaiPattern& P = *pat;
     auto p = std::make_unique<PrecSpai>();
     p->n = n;
-    p->pat = &P;
+    p->pat = pat;
     DBuf<double> cscv;
     cscv.alloc(2 * P.nnz);
     p->val.alloc(2 * P.nnz);

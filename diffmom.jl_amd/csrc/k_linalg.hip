@@ -126,9 +126,10 @@ __global__ void k_permute(i64 n, const int* __restrict__ idx, const double2* __r
 
 // ---------------------------------------------------------------- Krylov ops
 // Row-chunked dot products for the Arnoldi step: workgroup (c, g) forms the
-// partial sums of vectors g*kDotVG .. over rows c*256 .. c*256+255 (w read
-// once per group, every V entry once), reduced over chunks by k_dots_reduce.
-constexpr int kDotVG = 8;
+// partial sums of vectors g*kDotVG .. over 1024 rows (w read once per group,
+// every V entry once), reduced over chunks by k_dots_reduce.
+constexpr int kDotVG = 4;
+constexpr int kDotRows = 4;  // rows per thread
 
 __global__ __launch_bounds__(256) void k_dots_part(i64 n, i64 ldv, int k, const double2* __restrict__ V,
                                                    const double2* __restrict__ w, double2* __restrict__ part,
@@ -136,15 +137,25 @@ __global__ __launch_bounds__(256) void k_dots_part(i64 n, i64 ldv, int k, const 
   __shared__ double2 red[4][kDotVG];
   const int c = blockIdx.x, i0 = blockIdx.y * kDotVG;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const i64 m = (i64)c * 256 + threadIdx.x;
-  const double2 wm = m < n ? w[m] : make_double2(0.0, 0.0);
+  const i64 m0 = (i64)c * 256 * kDotRows + threadIdx.x;
+  double2 wm[kDotRows];
+#pragma unroll
+  for (int u = 0; u < kDotRows; ++u) {
+    const i64 m = m0 + 256 * u;
+    wm[u] = m < n ? w[m] : make_double2(0.0, 0.0);
+  }
   const int cnt = min(kDotVG, k - i0);
   for (int q = 0; q < cnt; ++q) {
+    const double2* v = V + (i64)(i0 + q) * ldv;
     double ar = 0, ai = 0;
-    if (m < n) {
-      double2 a = V[(i64)(i0 + q) * ldv + m];
-      ar = a.x * wm.x + a.y * wm.y;
-      ai = a.x * wm.y - a.y * wm.x;
+#pragma unroll
+    for (int u = 0; u < kDotRows; ++u) {
+      const i64 m = m0 + 256 * u;
+      if (m < n) {
+        double2 a = v[m];
+        ar += a.x * wm[u].x + a.y * wm[u].y;
+        ai += a.x * wm[u].y - a.y * wm[u].x;
+      }
     }
     ar = wave_sum(ar);
     ai = wave_sum(ai);
@@ -207,13 +218,19 @@ __global__ __launch_bounds__(1024) void k_norm2(i64 n, const double2* __restrict
   }
 }
 
-// y = a*x + b*y (complex a, b)
-__global__ void k_axpby(i64 n, double2 a, const double2* __restrict__ x, double2 b, double2* __restrict__ y) {
+// y = a*x + b*y (complex a, b).  With b == 0 the old y is not read: it may be
+// uninitialised memory (a fresh Krylov column), whose NaN bits 0 * y would keep.
+__global__ void k_axpby(i64 n, double2 a, const double2* x, double2 b, double2* y) {
   i64 m = blockIdx.x * (i64)blockDim.x + threadIdx.x;
   if (m >= n) return;
-  double2 xv = x[m], yv = y[m];
-  y[m] = make_double2(a.x * xv.x - a.y * xv.y + b.x * yv.x - b.y * yv.y,
-                      a.x * xv.y + a.y * xv.x + b.x * yv.y + b.y * yv.x);
+  double2 xv = x[m];
+  double2 r = make_double2(a.x * xv.x - a.y * xv.y, a.x * xv.y + a.y * xv.x);
+  if (b.x != 0.0 || b.y != 0.0) {
+    double2 yv = y[m];
+    r.x += b.x * yv.x - b.y * yv.y;
+    r.y += b.x * yv.y + b.y * yv.x;
+  }
+  y[m] = r;
 }
 
 // x += V y (k columns, y on device)
@@ -513,11 +530,11 @@ void launch_cast_c128_to_c64_matrix(i64 n, i64 ld, const double* src, float* dst
 
 namespace dml {
 
-size_t dots_part_count(i64 n, int k) { return (size_t)k * (size_t)nblocks(n, 256); }
+size_t dots_part_count(i64 n, int k) { return (size_t)k * (size_t)nblocks(n, 256 * kDotRows); }
 
 void dots(i64 n, i64 ldv, int k, const double* V, const double* w, double* h, double* part, hipStream_t s) {
   if (k <= 0) return;
-  const int nch = (int)nblocks(n, 256);
+  const int nch = (int)nblocks(n, 256 * kDotRows);
   dim3 g((unsigned)nch, (unsigned)((k + kDotVG - 1) / kDotVG));
   k_dots_part<<<g, 256, 0, s>>>(n, ldv, k, reinterpret_cast<const double2*>(V), reinterpret_cast<const double2*>(w),
                                 reinterpret_cast<double2*>(part), nch);

@@ -36,11 +36,12 @@ __host__ __device__ inline i64 seg_base(i64 J) {
   return J + kSeg * q * (q - 1) / 2 + q * rem;
 }
 
-// Chunks of CC columns x kPT rows are staged in LDS with coalesced 16-byte
-// loads; rows are then summed by threads owning a row (Z_IJ x_J) and columns by
-// threads owning a column (Z_IJ^T x_I), so neither product needs cross-lane
-// reductions.  The next chunk's loads are issued into registers before the
-// current chunk is consumed.
+// Chunks of CC columns x kPT rows are read with coalesced 16-byte loads into
+// registers.  Each thread's loads cover fixed rows, so the row sums (Z_IJ x_J)
+// are formed from the registers; the chunk is then staged in LDS, where
+// threads owning a column form the column sums (Z_IJ^T x_I) without cross-lane
+// reductions.  Two chunks are in flight per workgroup (registers of chunk
+// g + 2 are reloaded as soon as chunk g is in LDS).
 template <int F32>
 struct SymvCfg {
   typedef typename std::conditional<F32 != 0, float2, double2>::type ET;
@@ -90,9 +91,17 @@ __global__ __launch_bounds__(256) void k_symv(i64 n, int nb, const void* __restr
                                               double2* __restrict__ colpart) {
   using C = SymvCfg<F32>;
   using ET = typename C::ET;
+  // thread t's 16-byte load k covers column t / RPL + (256 / RPL) k of the
+  // chunk and rows (t % RPL) * EPL .. + EPL - 1 (EPL complex per 16 bytes)
+  constexpr int EPL = F32 ? 2 : 1;          // complex entries per 16-byte load
+  constexpr int RPL = kPT / EPL;            // threads covering one column
+  constexpr int RGRP = 256 / RPL;           // thread groups sharing the same rows
+  constexpr i64 kChunkF4 = (i64)C::CC * kPT * sizeof(ET) / 16;
+  constexpr i64 kTileF4 = (i64)kPT * kPT * sizeof(ET) / 16;
   __shared__ ET Zs[C::CC * C::LD];
   __shared__ double2 xs[2][kPT];
-  __shared__ double2 red[2][kPT];
+  __shared__ double2 red[RGRP][kPT];
+  __shared__ double2 cacc[C::G * kPT];  // column sums of (row group, column), one owner thread each
   __shared__ int segJ, segI0;
   const int tid = threadIdx.x;
   if (tid == 0) {
@@ -107,82 +116,102 @@ __global__ __launch_bounds__(256) void k_symv(i64 n, int nb, const void* __restr
   __syncthreads();
   const int J = segJ, I0 = segI0;
   const int I1 = min(I0 + kSeg, J + 1);
+  const i64 l0 = (i64)J * (J + 1) / 2 + I0;  // first tile of the segment (tiles are contiguous)
+  const int nchunk = (I1 - I0) * C::NCH;     // even: NCH is
   for (int c = tid; c < kPT; c += 256) {
     i64 g = (i64)J * kPT + c;
     xs[0][c] = g < n ? x[g] : make_double2(0.0, 0.0);
   }
-  // row phase: thread owns row rr of half hh of each chunk's columns
-  const int rr = tid & (kPT - 1), hh = tid >> 7;
-  // column phase: thread owns column cc of each chunk, rows of group gg
+  for (int e = tid; e < C::G * kPT; e += 256) cacc[e] = make_double2(0.0, 0.0);
+  const int row0 = (tid % RPL) * EPL, rgrp = tid / RPL;
   const int cc = tid % C::CC, gg = tid / C::CC;
-  double colr[C::NCH], coli[C::NCH];
+  double ar[EPL], ai[EPL];
+  // two chunks in flight: A holds even chunks, B odd ones
+  f4v va[8], vb[8];
+  const i64 base = l0 * kTileF4;
+  chunk_load<F32>(zp, base, tid, va);
+  chunk_load<F32>(zp, base + kChunkF4, tid, vb);
+  auto step = [&](int g, f4v(&v)[8]) {
+    const int ch = g % C::NCH;
+    const int I = I0 + g / C::NCH;
+    const i64 l = l0 + g / C::NCH;
+    const int c0 = ch * C::CC;
+    if (ch == 0) {
+      __syncthreads();
+      for (int r = tid; r < kPT; r += 256) {
+        i64 gr = (i64)I * kPT + r;
+        xs[1][r] = gr < n ? x[gr] : make_double2(0.0, 0.0);
+      }
 #pragma unroll
-  for (int k = 0; k < C::NCH; ++k) {
-    colr[k] = 0.0;
-    coli[k] = 0.0;
-  }
-  // 16-byte units per chunk: CC * kPT complex = 4096 (complex64) / 2048 (complex128) -> 2048 float4
-  constexpr i64 kChunkF4 = (i64)C::CC * kPT * sizeof(ET) / 16;
-  f4v v[8];
-  chunk_load<F32>(zp, ((i64)J * (J + 1) / 2 + I0) * kPT * kPT * sizeof(ET) / 16, tid, v);
-  for (int I = I0; I < I1; ++I) {
-    const i64 l = (i64)J * (J + 1) / 2 + I;
-    const bool offdiag = I != J;
-    __syncthreads();
-    for (int r = tid; r < kPT; r += 256) {
-      i64 g = (i64)I * kPT + r;
-      xs[1][r] = g < n ? x[g] : make_double2(0.0, 0.0);
+      for (int e = 0; e < EPL; ++e) {
+        ar[e] = 0.0;
+        ai[e] = 0.0;
+      }
     }
-    double ar = 0, ai = 0;
-    for (int ch = 0; ch < C::NCH; ++ch) {
-      __syncthreads();
-      chunk_store<F32>(Zs, tid, v);
-      __syncthreads();
-      // prefetch the next chunk (of this tile or the next one in the segment)
-      const bool more = ch + 1 < C::NCH || I + 1 < I1;
-      if (more) {
-        const i64 nl = ch + 1 < C::NCH ? l : l + 1;
-        const int nch = ch + 1 < C::NCH ? ch + 1 : 0;
-        chunk_load<F32>(zp, nl * kPT * kPT * sizeof(ET) / 16 + nch * kChunkF4, tid, v);
-      }
-      const int c0 = ch * C::CC;
-#pragma unroll 4
-      for (int c = hh * (C::CC / 2); c < (hh + 1) * (C::CC / 2); ++c) {
-        const double2 z = widen(Zs[c * C::LD + rr]);
-        const double2 xc = xs[0][c0 + c];
-        ar += z.x * xc.x - z.y * xc.y;
-        ai += z.x * xc.y + z.y * xc.x;
-      }
-      if (offdiag) {
-        double sr = 0, si = 0;
-#pragma unroll 4
-        for (int r = gg * C::RG; r < (gg + 1) * C::RG; ++r) {
-          const double2 z = widen(Zs[cc * C::LD + r]);
-          const double2 xr = xs[1][r];
-          sr += z.x * xr.x - z.y * xr.y;
-          si += z.x * xr.y + z.y * xr.x;
+    // row sums straight from the loaded registers
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const double2 xc = xs[0][c0 + rgrp + RGRP * k];
+#pragma unroll
+      for (int e = 0; e < EPL; ++e) {
+        double zr, zi;
+        if constexpr (F32 != 0) {
+          zr = (double)v[k][2 * e];
+          zi = (double)v[k][2 * e + 1];
+        } else {
+          double2 d;
+          __builtin_memcpy(&d, &v[k], sizeof(d));
+          zr = d.x;
+          zi = d.y;
         }
-        colr[ch] += sr;
-        coli[ch] += si;
+        ar[e] += zr * xc.x - zi * xc.y;
+        ai[e] += zr * xc.y + zi * xc.x;
       }
     }
-    red[hh][rr] = make_double2(ar, ai);
     __syncthreads();
-    if (tid < kPT) {
-      rowpart[l * kPT + tid] = make_double2(red[0][tid].x + red[1][tid].x, red[0][tid].y + red[1][tid].y);
+    chunk_store<F32>(Zs, tid, v);
+    __syncthreads();
+    if (g + 2 < nchunk) chunk_load<F32>(zp, base + (i64)(g + 2) * kChunkF4, tid, v);
+    if (I != J) {
+      double sr = 0, si = 0;
+#pragma unroll 4
+      for (int r = gg * C::RG; r < (gg + 1) * C::RG; ++r) {
+        const double2 z = widen(Zs[cc * C::LD + r]);
+        const double2 xr = xs[1][r];
+        sr += z.x * xr.x - z.y * xr.y;
+        si += z.x * xr.y + z.y * xr.x;
+      }
+      double2& acc = cacc[gg * kPT + c0 + cc];
+      acc.x += sr;
+      acc.y += si;
     }
-  }
-  // column sums: reduce the G row groups of every column through LDS
-  __syncthreads();
-  double2* cred = reinterpret_cast<double2*>(Zs);  // G x kPT double2 fits in the chunk buffer
+    if (ch == C::NCH - 1) {
 #pragma unroll
-  for (int k = 0; k < C::NCH; ++k) cred[gg * kPT + k * C::CC + cc] = make_double2(colr[k], coli[k]);
+      for (int e = 0; e < EPL; ++e) red[rgrp][row0 + e] = make_double2(ar[e], ai[e]);
+      __syncthreads();
+      if (tid < kPT) {
+        double sr = 0, si = 0;
+#pragma unroll
+        for (int q = 0; q < RGRP; ++q) {
+          sr += red[q][tid].x;
+          si += red[q][tid].y;
+        }
+        rowpart[l * kPT + tid] = make_double2(sr, si);
+      }
+    }
+  };
+#pragma unroll 1
+  for (int g = 0; g < nchunk; g += 2) {
+    step(g, va);
+    step(g + 1, vb);
+  }
+  // column sums: reduce the G row groups of every column
   __syncthreads();
   if (tid < kPT) {
     double sr = 0, si = 0;
     for (int q = 0; q < C::G; ++q) {
-      sr += cred[q * kPT + tid].x;
-      si += cred[q * kPT + tid].y;
+      sr += cacc[q * kPT + tid].x;
+      si += cacc[q * kPT + tid].y;
     }
     colpart[(i64)blockIdx.x * kPT + tid] = make_double2(sr, si);
   }
