@@ -120,6 +120,60 @@ def random_planewaves(dm, k, count, seed=20261202):
     return out
 
 
+COPY_BYTES = 1 << 30
+TRAFFIC_FILES = {("symv_z32", 5): "symv_z32_c2_sphere_traffic.md"}
+
+
+def measured_traffic(kernel, subdiv):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this kernel and workload (scaled by the copy calibration), or
+    None when no such summary exists."""
+    f = TRAFFIC_FILES.get((kernel, subdiv))
+    path = os.path.join(ROOT, "profiles", f) if f else None
+    if not path or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_product:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def traffic_probe(dm, subdiv):
+    """Dispatches for the HBM-traffic counters: the symmetric product of the
+    packed complex64 matrix (after the tuner's timing launches) and a plain
+    device-to-device copy of COPY_BYTES (the runtime's copy kernel reads and
+    writes COPY_BYTES each), whose counter readings give the scale factor."""
+    import ctypes as C
+    k = 2 * math.pi / 0.1
+    mesh = dm.make_icosphere(0.1, subdiv)
+    rwg = dm.build_rwg(mesh)
+    Z = dm.assemble_Z_efie(mesh, rwg, k, forms=dm.DM_Z32P, device=True, max_output_bytes=None, mesh_precheck=False)
+    x = np.random.default_rng(0).standard_normal(rwg.nedges) + 0j
+    Z.matmul(x, form=dm.DM_Z32P)  # tuner timing launches happen here
+    mesh.ctx.sync()
+    print(f"tuned: {mesh.ctx.tune_get()}", flush=True)
+    for _ in range(5):
+        Z.matmul(x, form=dm.DM_Z32P)
+    mesh.ctx.sync()
+    print("symmetric products done", flush=True)
+    try:
+        hip = C.CDLL("libamdhip64.so.7")  # the runtime the library already loaded
+    except OSError:
+        hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(COPY_BYTES)) != 0:
+            raise SystemExit("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(COPY_BYTES))
+    for _ in range(3):
+        if hip.hipMemcpyDtoD(b, a, C.c_size_t(COPY_BYTES)) != 0:
+            raise SystemExit("hipMemcpyDtoD failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"copies done ({COPY_BYTES} bytes each)", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +186,9 @@ def main():
     ap.add_argument("--precond", choices=["spai", "learned"], default="spai",
                     help="chosen form's preconditioner (learned needs the trained weights)")
     ap.add_argument("--nf-cutoff", type=float, default=2.0, help="near-field cutoff in mean edge lengths")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="no timing: 5 symmetric products of the packed complex64 matrix, then 3 plain 1 GiB "
+                         "device-to-device copies (counter calibration), for rocprofv3 --pmc runs")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -139,6 +196,8 @@ def main():
               "(the sharding design is described in DESIGN.md).", file=sys.stderr)
         sys.exit(2)
     dm = load_pkg()
+    if a.traffic_probe:
+        return traffic_probe(dm, a.subdiv)
     k = 2 * math.pi / 0.1
     mesh = dm.make_icosphere(0.1, a.subdiv)
     rwg = dm.build_rwg(mesh)
@@ -253,7 +312,7 @@ def main():
         dom = "symv_z32" if "symv_z32" in fracs else "gemv_z32"
     roof = fracs[dom]
     roofline = dict(kernel=dom, bound=roof["bound"], achieved=roof["achieved"], peak=roof["peak"], unit=roof["unit"],
-                    frac=roof["frac"], traffic=None)
+                    frac=roof["frac"], traffic=measured_traffic(dom, a.subdiv))
     cpu = None if a.no_cpu else cpu_baseline(mesh, rwg, k, plain["none"]["iterations"])
     out = {
         "metric": METRIC, "value": round(t_c, 4), "unit": "s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -274,7 +333,7 @@ def main():
         "plain_best": best,
         "currents_rel_diff_chosen_vs_plain": dx,
         "batched": batched,
-        "kernels": kern, "kernel_roofline": fracs,
+        "kernels": kern, "kernel_roofline": fracs, "tuned": ctx.tune_get(),
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

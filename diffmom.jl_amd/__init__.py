@@ -129,6 +129,7 @@ def lib():
         "dm_precond_free": ([vp], i32),
         "dm_profile_enable": ([vp, i32], i32), "dm_profile_reset": ([vp], i32),
         "dm_profile_read": ([vp, C.c_char_p, C.POINTER(i64), C.POINTER(d)], i32),
+        "dm_tune_get": ([vp, C.c_char_p, i64, C.POINTER(i64)], i32), "dm_tune_set": ([vp, C.c_char_p], i32),
         "dm_mpnn_param_count": ([], i64), "dm_mpnn_create": ([vp, vp, pp], i32),
         "dm_mpnn_get_params": ([vp, vp], i32), "dm_mpnn_free": ([vp], i32),
         "dm_rwg_graph": ([vp, d, vp, vp, vp], i32),
@@ -174,6 +175,20 @@ class Context:
 
     def profile_reset(self):
         _check(lib().dm_profile_reset(self.h))
+
+    def tune_get(self):
+        """Launch shapes chosen by the library's tuner so far, as a dict."""
+        n = C.c_int64()
+        _check(lib().dm_tune_get(self.h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        _check(lib().dm_tune_get(self.h, buf, n.value + 1, C.byref(n)))
+        import json
+        return json.loads(buf.value.decode())
+
+    def tune_set(self, choices):
+        """Load tuner choices (e.g. saved from tune_get in an earlier process)."""
+        import json
+        _check(lib().dm_tune_set(self.h, json.dumps({str(k): int(v) for k, v in choices.items()}).encode()))
 
     def profile_read(self, name):
         """(launch count, total milliseconds) of one tracked kernel name."""

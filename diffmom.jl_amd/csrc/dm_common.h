@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -77,6 +78,7 @@ struct dm_ctx {
   int cu_count = 256;
   bool prof = false;              // record HIP events around tracked launches
   std::vector<ProfRec> recs;
+  std::map<std::string, int> tune;  // launch parameters chosen by the tuner, by shape key
 };
 
 // Brackets one tracked launch with HIP events on its stream when profiling is on.

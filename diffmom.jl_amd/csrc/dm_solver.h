@@ -68,18 +68,26 @@ struct OpZ32 : DevOp {
 
 // Packed symmetric storage (DM_Z64P / DM_Z32P): one symmetric product per
 // vector; batches loop over columns.
+}  // namespace dms
+namespace dmt {
+// Tuned segment length of the symmetric product (timed on first use per shape).
+int symv_seg(dm_ctx* ctx, const dm_zmat* z, int form, void* ws, hipStream_t s);
+}  // namespace dmt
+namespace dms {
 struct OpPacked : DevOp {
   const dm_zmat* z;
   int form;
   const char* tag;
+  int seg = 0;  // chosen on the first product (the matrix must be assembled by then)
   DBuf<char> ws;
   OpPacked(const dm_zmat* z_, int form_) : z(z_), form(form_) {
     tag = form == DM_Z32P ? "symv_z32" : "symv_z64";
     ws.alloc(dml::symv_workspace_bytes(z->n));
   }
   void apply(const double* x, double* y, hipStream_t s) override {
+    if (!seg) seg = dmt::symv_seg(z->ctx, z, form, ws.p, s);
     ProfScope p(z->ctx, tag, s);
-    dml::symv_packed(z->n, z->ref(form), x, y, ws.p, s);
+    dml::symv_packed(z->n, z->ref(form), x, y, ws.p, seg, s);
   }
   void apply_batch(const double* X, double* Y, int R, void*, hipStream_t s) override {
     for (int j = 0; j < R; ++j) apply(X + 2 * z->n * j, Y + 2 * z->n * j, s);

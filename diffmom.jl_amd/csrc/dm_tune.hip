@@ -1,0 +1,128 @@
+// dm_tune.hip — the library's own launch-shape tuner.
+//
+// A tunable kernel asks for its parameter with a key naming the shape (kernel,
+// precision, problem size).  On the first request in a context the candidates
+// are timed with HIP events on the calling stream (one warm-up and three timed
+// launches each) and the fastest is kept in the context; later requests reuse
+// it.  Results can be exported to / imported from a flat JSON object so a
+// caller can persist them across processes (dm_tune_get / dm_tune_set).
+// DM_TUNE=0 in the environment disables timing and uses the defaults.
+#include <cstdlib>
+#include <cstring>
+#include <sstream>
+#include <string>
+
+#include "dm_solver.h"
+
+namespace dmt {
+
+namespace {
+
+bool tuning_enabled() {
+  const char* e = std::getenv("DM_TUNE");
+  return !(e && std::strcmp(e, "0") == 0);
+}
+
+template <class Launch>
+double time_ms(Launch&& run, hipStream_t s) {
+  hipEvent_t a, b;
+  HIP_OK(hipEventCreate(&a));
+  HIP_OK(hipEventCreate(&b));
+  run();  // warm-up
+  HIP_OK(hipEventRecord(a, s));
+  for (int r = 0; r < 3; ++r) run();
+  HIP_OK(hipEventRecord(b, s));
+  HIP_OK(hipEventSynchronize(b));
+  float ms = 0.f;
+  HIP_OK(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms / 3.0;
+}
+
+}  // namespace
+
+int symv_seg(dm_ctx* ctx, const dm_zmat* z, int form, void* ws, hipStream_t s) {
+  const int fallback = 8;
+  const std::string key = std::string("symv/") + (form == DM_Z32P ? "c64/" : "c128/") + std::to_string(z->n);
+  auto it = ctx->tune.find(key);
+  if (it != ctx->tune.end()) return it->second;
+  if (!tuning_enabled()) return fallback;
+  DBuf<double> x, y;
+  x.alloc(2 * z->n);
+  y.alloc(2 * z->n);
+  HIP_OK(hipMemsetAsync(x.p, 0, x.bytes(), s));
+  const ZRef zr = z->ref(form);
+  int best = fallback;
+  double best_ms = 1e300;
+  for (int seg : {2, 4, 8, 16, 32}) {
+    double ms = time_ms([&] { dml::symv_packed(z->n, zr, x.p, y.p, ws, seg, s); }, s);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = seg;
+    }
+  }
+  ctx->tune[key] = best;
+  return best;
+}
+
+}  // namespace dmt
+
+extern "C" {
+
+int dm_tune_get(dm_ctx* ctx, char* buf, int64_t cap, int64_t* len) {
+  if (!ctx || !len) {
+    dm_set_error("NULL argument");
+    return DM_ERR_ARGUMENT;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  std::ostringstream o;
+  o << "{";
+  bool first = true;
+  for (auto& kv : ctx->tune) {
+    o << (first ? "" : ", ") << "\"" << kv.first << "\": " << kv.second;
+    first = false;
+  }
+  o << "}";
+  const std::string js = o.str();
+  *len = (int64_t)js.size();
+  if (buf && cap > 0) {
+    const size_t m = std::min<size_t>((size_t)cap - 1, js.size());
+    std::memcpy(buf, js.data(), m);
+    buf[m] = 0;
+  }
+  return DM_OK;
+}
+
+// Flat {"key": int, ...} objects only (what dm_tune_get writes).
+int dm_tune_set(dm_ctx* ctx, const char* json) {
+  if (!ctx || !json) {
+    dm_set_error("NULL argument");
+    return DM_ERR_ARGUMENT;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  const std::string s(json);
+  size_t p = 0;
+  std::map<std::string, int> got;
+  while ((p = s.find('"', p)) != std::string::npos) {
+    size_t q = s.find('"', p + 1);
+    size_t c = q == std::string::npos ? q : s.find(':', q);
+    if (c == std::string::npos) {
+      dm_set_error("malformed tuning record");
+      return DM_ERR_ARGUMENT;
+    }
+    const std::string key = s.substr(p + 1, q - p - 1);
+    char* end = nullptr;
+    long v = std::strtol(s.c_str() + c + 1, &end, 10);
+    if (end == s.c_str() + c + 1) {
+      dm_set_error("malformed tuning record");
+      return DM_ERR_ARGUMENT;
+    }
+    got[key] = (int)v;
+    p = (size_t)(end - s.c_str());
+  }
+  for (auto& kv : got) ctx->tune[kv.first] = kv.second;
+  return DM_OK;
+}
+
+}  // extern "C"

@@ -23,7 +23,8 @@ void entries(const ZRef& z, i64 count, const int* iofe, const i64* m, const i64*
 void conj(i64 n, double* v, hipStream_t s);
 // y = Z x from packed symmetric storage; ws >= symv_workspace_bytes(n)
 size_t symv_workspace_bytes(i64 n);
-void symv_packed(i64 n, const ZRef& z, const double* x, double* y, void* ws, hipStream_t s);
+// seg: tiles per workgroup (2..64; chosen by the tuner)
+void symv_packed(i64 n, const ZRef& z, const double* x, double* y, void* ws, int seg, hipStream_t s);
 size_t zx_workspace_bytes(i64 n, int R);
 void zx_mfma(i64 n, i64 ld, const float* z32, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
              hipStream_t s);

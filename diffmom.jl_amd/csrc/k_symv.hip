@@ -5,7 +5,7 @@
 // instead of N^2 (4 N^2 bytes for complex64).  Arithmetic is FP64 on
 // widened entries.
 //
-// A workgroup walks a segment of up to kSeg consecutive tiles of one tile
+// A workgroup walks a segment of up to `seg` consecutive tiles of one tile
 // column J (contiguous in memory), staging each tile through LDS in chunks of
 // columns.  Row sums Z_IJ x_J are stored per tile; column sums Z_IJ^T x_I are
 // kept per thread over the whole segment and reduced through LDS once.  A
@@ -21,7 +21,7 @@
 
 namespace {
 
-constexpr int kSeg = 8;  // tiles per workgroup
+constexpr int kSegMin = 2;  // shortest segment (tiles per workgroup) the workspace allows
 
 
 __device__ inline double wsum64(double v) {
@@ -30,10 +30,10 @@ __device__ inline double wsum64(double v) {
   return v;
 }
 
-// number of segments of tile columns 0 .. J-1
-__host__ __device__ inline i64 seg_base(i64 J) {
-  i64 q = J / kSeg, rem = J % kSeg;
-  return J + kSeg * q * (q - 1) / 2 + q * rem;
+// number of segments of tile columns 0 .. J-1 (column J has J / seg + 1)
+__host__ __device__ inline i64 seg_base(i64 J, int seg) {
+  i64 q = J / seg, rem = J % seg;
+  return J + seg * q * (q - 1) / 2 + q * rem;
 }
 
 // Chunks of CC columns x kPT rows are read with coalesced 16-byte loads into
@@ -86,7 +86,7 @@ __device__ inline double2 widen(ET v) {
 }
 
 template <int F32>
-__global__ __launch_bounds__(256) void k_symv(i64 n, int nb, const void* __restrict__ zp,
+__global__ __launch_bounds__(256) void k_symv(i64 n, int nb, int seg, const void* __restrict__ zp,
                                               const double2* __restrict__ x, double2* __restrict__ rowpart,
                                               double2* __restrict__ colpart) {
   using C = SymvCfg<F32>;
@@ -106,16 +106,16 @@ __global__ __launch_bounds__(256) void k_symv(i64 n, int nb, const void* __restr
   const int tid = threadIdx.x;
   if (tid == 0) {
     int b = blockIdx.x, J = 0;
-    while (b >= J / kSeg + 1) {
-      b -= J / kSeg + 1;
+    while (b >= J / seg + 1) {
+      b -= J / seg + 1;
       ++J;
     }
     segJ = J;
-    segI0 = b * kSeg;
+    segI0 = b * seg;
   }
   __syncthreads();
   const int J = segJ, I0 = segI0;
-  const int I1 = min(I0 + kSeg, J + 1);
+  const int I1 = min(I0 + seg, J + 1);
   const i64 l0 = (i64)J * (J + 1) / 2 + I0;  // first tile of the segment (tiles are contiguous)
   const int nchunk = (I1 - I0) * C::NCH;     // even: NCH is
   for (int c = tid; c < kPT; c += 256) {
@@ -218,8 +218,8 @@ __global__ __launch_bounds__(256) void k_symv(i64 n, int nb, const void* __restr
 }
 
 // y[R kPT + r] = sum_{J >= R} rowpart[tile (R, J)][r] + sum_{segments s of column R} colpart[s][r]
-__global__ void k_symv_reduce(i64 n, int nb, const double2* __restrict__ rowpart, const double2* __restrict__ colpart,
-                              double2* __restrict__ y) {
+__global__ void k_symv_reduce(i64 n, int nb, int seg, const double2* __restrict__ rowpart,
+                              const double2* __restrict__ colpart, double2* __restrict__ y) {
   const i64 g = blockIdx.x * (i64)blockDim.x + threadIdx.x;
   if (g >= n) return;
   const int R = (int)(g / kPT), r = (int)(g % kPT);
@@ -241,8 +241,8 @@ __global__ void k_symv_reduce(i64 n, int nb, const double2* __restrict__ rowpart
     sr += v.x;
     si += v.y;
   }
-  const i64 sb = seg_base(R);
-  for (int s = 0; s <= R / kSeg; ++s) {
+  const i64 sb = seg_base(R, seg);
+  for (int s = 0; s <= R / seg; ++s) {
     double2 v = colpart[(sb + s) * kPT + r];
     sr += v.x;
     si += v.y;
@@ -256,22 +256,24 @@ namespace dml {
 
 size_t symv_workspace_bytes(i64 n) {
   const i64 nb = (n + kPT - 1) / kPT;
-  return (size_t)(packed_tiles(n) + seg_base(nb)) * kPT * sizeof(double2);
+  return (size_t)(packed_tiles(n) + seg_base(nb, kSegMin)) * kPT * sizeof(double2);
 }
 
-void symv_packed(i64 n, const ZRef& z, const double* x, double* y, void* ws, hipStream_t s) {
+void symv_packed(i64 n, const ZRef& z, const double* x, double* y, void* ws, int seg, hipStream_t s) {
   DM_REQUIRE(z.packed, DM_ERR_ARGUMENT, "symmetric product needs packed storage");
+  DM_REQUIRE(seg >= kSegMin && seg <= 64, DM_ERR_ARGUMENT, "segment length out of range");
   const i64 nb = (n + kPT - 1) / kPT;
-  const i64 nseg = seg_base(nb);
+  const i64 nseg = seg_base(nb, seg);
   DM_REQUIRE(nseg < (1LL << 31), DM_ERR_ARGUMENT, "matrix too large for the packed product");
   double2* rowpart = reinterpret_cast<double2*>(ws);
   double2* colpart = rowpart + packed_tiles(n) * kPT;
   const double2* xv = reinterpret_cast<const double2*>(x);
   if (z.f32)
-    k_symv<1><<<(unsigned)nseg, 256, 0, s>>>(n, (int)nb, z.p, xv, rowpart, colpart);
+    k_symv<1><<<(unsigned)nseg, 256, 0, s>>>(n, (int)nb, seg, z.p, xv, rowpart, colpart);
   else
-    k_symv<0><<<(unsigned)nseg, 256, 0, s>>>(n, (int)nb, z.p, xv, rowpart, colpart);
-  k_symv_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, (int)nb, rowpart, colpart, reinterpret_cast<double2*>(y));
+    k_symv<0><<<(unsigned)nseg, 256, 0, s>>>(n, (int)nb, seg, z.p, xv, rowpart, colpart);
+  k_symv_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, (int)nb, seg, rowpart, colpart,
+                                                              reinterpret_cast<double2*>(y));
   HIP_OK(hipGetLastError());
 }
 

@@ -211,6 +211,16 @@ int dm_precond_free(dm_precond* p);
 int dm_solve(dm_ctx* ctx, const dm_zmat* z, const double* B, int64_t ldb, int64_t nrhs,
              const dm_solve_opts* opts, double* X, int64_t ldx, dm_solve_stats* stats);
 
+/* ------------------------------------------------------------------ tuning */
+/* No reference counterpart.  Launch shapes (the symmetric product's tiles per
+ * workgroup, ...) are timed on first use per shape and kept in the context.
+ * dm_tune_get writes them as a flat JSON object {"key": value, ...} (len: its
+ * length; buf may be NULL to query it); dm_tune_set loads such an object, so a
+ * caller can persist the choices across processes.  DM_TUNE=0 in the
+ * environment skips timing and uses defaults. */
+int dm_tune_get(dm_ctx* ctx, char* buf, int64_t cap, int64_t* len);
+int dm_tune_set(dm_ctx* ctx, const char* json);
+
 /* ------------------------------------------------- learned preconditioner */
 /* No reference counterpart: a 4-layer message-passing network over the RWG
  * edge-adjacency graph (bases sharing a triangle), hidden width 64, bfloat16

@@ -254,3 +254,30 @@ def test_multi_mesh_persistent_assembly(dm, O):
         assert rel_fro(Z.to_numpy(dm.DM_Z32P), single.to_numpy(dm.DM_Z32P)) <= 1e-6, i
         if i in (0, 12):
             assert rel_fro(A, _oracle_Z(O, m, r, K0)) <= 1e-10
+
+
+def test_tuned_symmetric_product_segments(dm):
+    """The tuner times the symmetric product's tiles-per-workgroup on first use
+    and records the choice; every allowed choice gives the same product (FP64
+    sums in another order), and choices loaded with tune_set are used as is."""
+    m = dm.make_icosphere(0.1, 3)
+    r = dm.build_rwg(m)
+    Zd = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64P | dm.DM_Z32P, device=True)
+    P64 = Zd.to_numpy(dm.DM_Z64P)
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((r.nedges, 2)) + 1j * rng.standard_normal((r.nedges, 2))
+    ref = P64 @ X
+    ctx = dm.default_context()
+    Zd.matmul(X, form=dm.DM_Z64P)
+    key = f"symv/c128/{r.nedges}"
+    got = ctx.tune_get()
+    assert got[key] in (2, 4, 8, 16, 32)
+    for seg in (2, 3, 16, 64):
+        ctx.tune_set({key: seg})
+        assert ctx.tune_get()[key] == seg
+        Y = Zd.matmul(X, form=dm.DM_Z64P)
+        assert np.linalg.norm(Y - ref) / np.linalg.norm(ref) < 1e-13, seg
+    ctx.tune_set({key: 1})  # outside the kernel's range: rejected at launch
+    with pytest.raises(Exception):
+        Zd.matmul(X, form=dm.DM_Z64P)
+    ctx.tune_set(got)
