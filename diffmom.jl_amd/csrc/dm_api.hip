@@ -140,13 +140,13 @@ constexpr int kAllForms = DM_Z64 | DM_Z32 | DM_Z64P | DM_Z32P;
 void assemble_packed(dm_zmat* z, const AsmArgs& a, const dm_rwg& r, int forms, hipStream_t s) {
   if (forms & DM_Z64P) {
     z->z64p.alloc(packed_scalars(z->n));
-    HIP_OK(hipMemsetAsync(z->z64p.p, 0, z->z64p.bytes(), s));
+    zero_packed_padding(z->n, z->z64p.p, sizeof(double), s);
     ProfScope p(z->ctx, "assemble_z64", s);
     launch_assemble_z64p(a, z->z64p.p, r.max_blk_tris, s);
   }
   if (forms & DM_Z32P) {
     z->z32p.alloc(packed_scalars(z->n));
-    HIP_OK(hipMemsetAsync(z->z32p.p, 0, z->z32p.bytes(), s));
+    zero_packed_padding(z->n, z->z32p.p, sizeof(float), s);
     ProfScope p(z->ctx, "assemble_z32", s);
     launch_assemble_z32p(a, z->z32p.p, r.max_blk_tris, s);
   }
@@ -603,11 +603,11 @@ int dm_assemble_z_multi(dm_ctx* ctx, int64_t nmesh, const dm_rwg* const* rwgs, d
           ptrs[q] = z.z32.p;
         } else if (form == DM_Z64P) {
           z.z64p.alloc(packed_scalars(n));
-          HIP_OK(hipMemsetAsync(z.z64p.p, 0, z.z64p.bytes(), s));
+          zero_packed_padding(z.n, z.z64p.p, sizeof(double), s);
           ptrs[q] = z.z64p.p;
         } else {
           z.z32p.alloc(packed_scalars(n));
-          HIP_OK(hipMemsetAsync(z.z32p.p, 0, z.z32p.bytes(), s));
+          zero_packed_padding(z.n, z.z32p.p, sizeof(float), s);
           ptrs[q] = z.z32p.p;
         }
       }
@@ -921,7 +921,7 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
     dm_rwg& r = const_cast<dm_rwg&>(*rwg_c);
     i64 n = r.n;
     const TriTables& T = dmh::tables(r, quad_order);
-    hipStream_t s = ctx->stream, s2 = ctx->stream2;
+    hipStream_t s = ctx->stream;
     double t0 = now();
     auto z = std::make_unique<dm_zmat>();
     z->ctx = ctx;
@@ -945,29 +945,24 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
       ProfScope p(ctx, "near_tables", s);
       a = make_args(r, T, k_re, k_im, eta0, z->ld, nt);
     }
-    hipEvent_t e_near = nullptr, e_z64 = nullptr;
-    HIP_OK(hipEventCreateWithFlags(&e_near, hipEventDisableTiming));
-    HIP_OK(hipEventCreateWithFlags(&e_z64, hipEventDisableTiming));
-    struct EvGuard {
-      hipEvent_t a, b;
-      ~EvGuard() {
-        if (a) (void)hipEventDestroy(a);
-        if (b) (void)hipEventDestroy(b);
-      }
-    } evg{e_near, e_z64};
-    HIP_OK(hipEventRecord(e_near, s));
     if (o.method == DM_METHOD_MIXED) {
-      {
-        ProfScope p(ctx, "assemble_z32", s);
-        if (batched) {
-          z->z32.alloc((size_t)z->ld * n * 2);
-          launch_assemble_z32(a, z->z32.p, r.max_blk_tris, s);
-        } else {
-          z->z32p.alloc(packed_scalars(n));
-          HIP_OK(hipMemsetAsync(z->z32p.p, 0, z->z32p.bytes(), s));
-          launch_assemble_z32p(a, z->z32p.p, r.max_blk_tris, s);
-        }
+      // one FP64 assembly writes both forms: the complex128 matrix for the
+      // refinement residual and its complex64 rounding for the inner solve
+      ProfScope p(ctx, "assemble_z64_z32", s);
+      if (batched) {
+        z->z64.alloc((size_t)z->ld * n * 2);
+        z->z32.alloc((size_t)z->ld * n * 2);
+        a.z32_dual = z->z32.p;
+        launch_assemble_z64(a, z->z64.p, r.max_blk_tris, s);
+      } else {
+        z->z64p.alloc(packed_scalars(n));
+        z->z32p.alloc(packed_scalars(n));
+        zero_packed_padding(n, z->z64p.p, sizeof(double), s);
+        zero_packed_padding(n, z->z32p.p, sizeof(float), s);
+        a.z32_dual = z->z32p.p;
+        launch_assemble_z64p(a, z->z64p.p, r.max_blk_tris, s);
       }
+      a.z32_dual = nullptr;
     } else {
       z->z64.alloc((size_t)z->ld * n * 2);
       ProfScope p(ctx, "assemble_z64", s);
@@ -1018,29 +1013,9 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
       auto M = build_precond(z.get(), pick_form(z.get(), o.method == DM_METHOD_MIXED, batched), o, s);
       HIP_OK(hipStreamSynchronize(s));
       st.precond_time_s = now() - t1;
-      if (o.method == DM_METHOD_MIXED) {
-        // FP64 matrix on the second stream, concurrent with the complex64 solve
-        // (started after the preconditioner build, which it would slow down;
-        // the solve waits for it only at its first FP64 residual)
-        HIP_OK(hipStreamWaitEvent(s2, e_near, 0));
-        {
-          ProfScope p(ctx, "assemble_z64", s2);
-          if (batched) {
-            z->z64.alloc((size_t)z->ld * n * 2);
-            launch_assemble_z64(a, z->z64.p, r.max_blk_tris, s2);
-          } else {
-            z->z64p.alloc(packed_scalars(n));
-            HIP_OK(hipMemsetAsync(z->z64p.p, 0, z->z64p.bytes(), s2));
-            launch_assemble_z64p(a, z->z64p.p, r.max_blk_tris, s2);
-          }
-        }
-        HIP_OK(hipEventRecord(e_z64, s2));
-      }
-      solve_columns(ctx, o, r, A64.get(), A32.get(), M.get(), V.p, n, nrhs, X, n, st, worst, all_ok,
-                    o.method == DM_METHOD_MIXED ? e_z64 : nullptr);
+      solve_columns(ctx, o, r, A64.get(), A32.get(), M.get(), V.p, n, nrhs, X, n, st, worst, all_ok, nullptr);
     }
     HIP_OK(hipStreamSynchronize(s));
-    HIP_OK(hipStreamSynchronize(s2));
     check_flag(nt, s);
     st.solved = all_ok ? 1 : 0;
     st.residual = worst;

@@ -251,7 +251,8 @@ std::shared_ptr<const SpaiPattern> spai_pattern(dm_rwg& r, double cutJ, int maxJ
 
 namespace dmp {
 int spai_max_cols();
-void spai_build(i64 n, const ZRef& z, const int* jptr, const int* jidx, const int* iptr, const int* iidx,
+// maxJ: widest column support of the pattern (<= spai_max_cols())
+void spai_build(i64 n, const ZRef& z, int maxJ, const int* jptr, const int* jidx, const int* iptr, const int* iidx,
                 double* vals, int* status, hipStream_t s);
 void csr_from_csc(i64 nnz, const int* map, const double* cscval, double* csrval, hipStream_t s);
 void spmv(i64 n, const int* rowptr, const int* colidx, const double* val, const double* x, double* y, hipStream_t s);
@@ -287,6 +288,9 @@ struct AsmArgs {
   const char* side_img64;
   i64 side_stride32, side_stride64;
   int side_cap;
+  // FP64 assembly only: when set, every entry is also stored rounded to
+  // complex64 at the same position of this second matrix (same form)
+  float* z32_dual = nullptr;
 };
 
 void launch_near_tables(const AsmArgs& a, const TriTables& t, double* jself, double* jadj,
@@ -296,9 +300,13 @@ size_t side_image_bytes(int nq, bool f64, int cap);
 void launch_build_sides(const AsmArgs& a, int cap, bool f64, char* img, hipStream_t s);
 void launch_assemble_z64(const AsmArgs& a, double* z, int max_tris, hipStream_t s);
 void launch_assemble_z32(const AsmArgs& a, float* z, int max_tris, hipStream_t s);
-// packed symmetric storage (buffer of packed_scalars(n), zeroed beforehand)
+// packed symmetric storage (buffer of packed_scalars(n); entries past n must be
+// zeroed beforehand: zero_packed_padding)
 void launch_assemble_z64p(const AsmArgs& a, double* z, int max_tris, hipStream_t s);
 void launch_assemble_z32p(const AsmArgs& a, float* z, int max_tris, hipStream_t s);
+// Zero the entries of packed storage that lie past n (only the last tile
+// column holds any); every other entry is written by the assembly.
+void zero_packed_padding(i64 n, void* z, size_t scalar_bytes, hipStream_t s);
 // Several meshes' matrices (one stored form) in one persistent launch;
 // packed forms need zeroed buffers.  Synchronises the stream on return.
 void launch_assemble_multi(const std::vector<AsmArgs>& args, const std::vector<void*>& z, int form, int max_tris,

@@ -47,7 +47,9 @@ std::unique_ptr<DevPrec> build_precond(const dm_zmat* z, int form, const dm_solv
     DBuf<int> st;
     st.alloc(1);
     HIP_OK(hipMemsetAsync(st.p, 0, sizeof(int), s));
-    dmp::spai_build(n, zr, P.d_jptr.p, P.d_jidx.p, P.d_iptr.p, P.d_iidx.p, cscv.p, st.p, s);
+    int maxJ = 1;
+    for (i64 c = 0; c < n; ++c) maxJ = std::max(maxJ, P.jptr[c + 1] - P.jptr[c]);
+    dmp::spai_build(n, zr, maxJ, P.d_jptr.p, P.d_jidx.p, P.d_iptr.p, P.d_iidx.p, cscv.p, st.p, s);
     dmp::csr_from_csc(P.nnz, P.d_map.p, cscv.p, p->val.p, s);
     int sv = 0;
     HIP_OK(hipMemcpyAsync(&sv, st.p, sizeof(int), hipMemcpyDeviceToHost, s));

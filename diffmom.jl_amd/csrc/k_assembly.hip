@@ -461,6 +461,13 @@ __device__ inline void far_block(const TileSide<T, NQ>& R, int i, const TileSide
   }
 }
 
+__device__ inline void store_c(double* o, double vr, double vi) {
+  *reinterpret_cast<double2*>(o) = make_double2(vr, vi);
+}
+__device__ inline void store_c(float* o, double vr, double vi) {
+  *reinterpret_cast<float2*>(o) = make_float2((float)vr, (float)vi);
+}
+
 __device__ inline void tile_coords(int nblk, int& bi, int& bj, bool& valid) {
   // XCD-aware order: workgroup w runs on XCD w % 8; give each XCD a contiguous
   // range of tiles so tiles sharing a column block share that XCD's L2.
@@ -587,6 +594,7 @@ __device__ inline void tile_body(const AsmArgs& a, int cap, AccT* __restrict__ z
   // Z_mn = -i k eta0 len_m len_n * tile ; P = -i k eta0 = (k_im eta0, -k_re eta0)
   const double Pr = a.k_im * a.eta0, Pi = -a.k_re * a.eta0;
   const i64 r0 = (i64)bi * TB, c0 = (i64)bj * TB;
+  float* const dual = sizeof(AccT) == 8 ? a.z32_dual : nullptr;
   const bool diag = bi == bj;
   if constexpr (MODE == 0) {
     int bad = 0;
@@ -600,9 +608,9 @@ __device__ inline void tile_body(const AsmArgs& a, int cap, AccT* __restrict__ z
       double l2 = a.len[gr] * a.len[gc];
       double vr = (Pr * tr - Pi * ti) * l2, vi = (Pr * ti + Pi * tr) * l2;
       bad |= !isfinite(vr) || !isfinite(vi);
-      AccT* o = zout + 2 * (gr + gc * a.ld);
-      o[0] = (AccT)vr;
-      o[1] = (AccT)vi;
+      const i64 at = gr + gc * a.ld;
+      store_c(zout + 2 * at, vr, vi);
+      if (dual) store_c(dual + 2 * at, vr, vi);
     }
     if (!diag) {
       for (int k = tid; k < TB * TB; k += NTHR) {
@@ -612,9 +620,9 @@ __device__ inline void tile_body(const AsmArgs& a, int cap, AccT* __restrict__ z
         double tr = (double)zt[r * ldt + c], ti = (double)zt[TB * ldt + r * ldt + c];
         double l2 = a.len[gr] * a.len[gc];
         double vr = (Pr * tr - Pi * ti) * l2, vi = (Pr * ti + Pi * tr) * l2;
-        AccT* o = zout + 2 * (gc + gr * a.ld);
-        o[0] = (AccT)vr;
-        o[1] = (AccT)vi;
+        const i64 at = gc + gr * a.ld;
+        store_c(zout + 2 * at, vr, vi);
+        if (dual) store_c(dual + 2 * at, vr, vi);
       }
     }
     if (bad) atomicOr(a.flag, 1);
@@ -625,7 +633,7 @@ __device__ inline void tile_body(const AsmArgs& a, int cap, AccT* __restrict__ z
     int bad = 0;
     const int BI = bi >> 1, BJ = bj >> 1;
     const int ro = (bi & 1) * TB, co = (bj & 1) * TB;
-    AccT* tile = zout + 2 * (((i64)BJ * (BJ + 1) / 2 + BI) * kPT * kPT);
+    const i64 tbase = ((i64)BJ * (BJ + 1) / 2 + BI) * kPT * kPT;
     const bool mirror = BI == BJ && !diag;
     for (int k = tid; k < TB * TB; k += NTHR) {
       int r = k % TB, c = k / TB;
@@ -637,13 +645,13 @@ __device__ inline void tile_body(const AsmArgs& a, int cap, AccT* __restrict__ z
       double l2 = a.len[gr] * a.len[gc];
       double vr = (Pr * tr - Pi * ti) * l2, vi = (Pr * ti + Pi * tr) * l2;
       bad |= !isfinite(vr) || !isfinite(vi);
-      AccT* o = tile + 2 * ((i64)(co + c) * kPT + ro + r);
-      o[0] = (AccT)vr;
-      o[1] = (AccT)vi;
+      const i64 at = tbase + (i64)(co + c) * kPT + ro + r;
+      store_c(zout + 2 * at, vr, vi);
+      if (dual) store_c(dual + 2 * at, vr, vi);
       if (mirror) {
-        AccT* m = tile + 2 * ((i64)(ro + r) * kPT + co + c);
-        m[0] = (AccT)vr;
-        m[1] = (AccT)vi;
+        const i64 am = tbase + (i64)(ro + r) * kPT + co + c;
+        store_c(zout + 2 * am, vr, vi);
+        if (dual) store_c(dual + 2 * am, vr, vi);
       }
     }
     if (bad) atomicOr(a.flag, 1);
@@ -841,6 +849,14 @@ void launch_near_tables(const AsmArgs& a, const TriTables& t, double* jself, dou
   k_near_self<<<g1, 64, 0, s>>>(n, jself);
   k_near_adj<<<g2, 64, 0, s>>>(n, jadj);
   HIP_OK(hipGetLastError());
+}
+
+void zero_packed_padding(i64 n, void* z, size_t scalar_bytes, hipStream_t s) {
+  const i64 nb = (n + kPT - 1) / kPT;
+  if (n % kPT == 0 || nb == 0) return;
+  const size_t tile_bytes = (size_t)kPT * kPT * 2 * scalar_bytes;
+  const size_t first = (size_t)((nb - 1) * nb / 2) * tile_bytes;  // tiles (I, nb - 1), I = 0 .. nb - 1
+  HIP_OK(hipMemsetAsync(static_cast<char*>(z) + first, 0, (size_t)nb * tile_bytes, s));
 }
 
 void launch_assemble_z64(const AsmArgs& a, double* z, int cap, hipStream_t s) {

@@ -23,22 +23,30 @@ namespace {
 constexpr int SPAI_MAXJ = 80;   // columns per least-squares problem (LDS bound)
 constexpr int SPAI_CHUNK = 16;  // rows of A staged per pass
 
-__global__ __launch_bounds__(256) void k_spai_build(ZRef z, const int* __restrict__ jptr,
+// One column per workgroup.  LDS holds, for the widest support mJ of the
+// pattern (not the cap), the Gram matrix's lower triangle packed by rows
+// (entry (a, b), b <= a, at a (a + 1) / 2 + b), SPAI_CHUNK staged rows of A,
+// the right-hand side and the support's column ids: about 28 KB at mJ = 44,
+// so several columns share a CU.
+__device__ inline int tri(int a, int b) { return a * (a + 1) / 2 + b; }
+
+__global__ __launch_bounds__(256) void k_spai_build(ZRef z, int mJ, const int* __restrict__ jptr,
                                                     const int* __restrict__ jidx, const int* __restrict__ iptr,
                                                     const int* __restrict__ iidx, double2* __restrict__ vals,
                                                     int* __restrict__ status) {
   const int j = blockIdx.x;
   const int j0 = jptr[j], nJ = jptr[j + 1] - j0;
   const int i0 = iptr[j], nI = iptr[j + 1] - i0;
-  __shared__ double2 G[SPAI_MAXJ * SPAI_MAXJ];
-  __shared__ double2 Ach[SPAI_CHUNK * SPAI_MAXJ];
-  __shared__ double2 rhs[SPAI_MAXJ];
-  __shared__ int cols[SPAI_MAXJ];
+  extern __shared__ __align__(16) char smem[];
+  double2* G = reinterpret_cast<double2*>(smem);
+  double2* Ach = G + mJ * (mJ + 1) / 2;
+  double2* rhs = Ach + SPAI_CHUNK * mJ;
+  int* cols = reinterpret_cast<int*>(rhs + mJ);
   __shared__ int bad;
   const int tid = threadIdx.x;
   if (tid == 0) bad = 0;
   for (int a = tid; a < nJ; a += 256) cols[a] = jidx[j0 + a];
-  for (int e = tid; e < nJ * nJ; e += 256) G[e] = make_double2(0.0, 0.0);
+  for (int e = tid; e < nJ * (nJ + 1) / 2; e += 256) G[e] = make_double2(0.0, 0.0);
   __syncthreads();
   // rhs = A^H e_j = conj(Z[j, J]) (row j of Z restricted to the pattern)
   for (int a = tid; a < nJ; a += 256) {
@@ -46,67 +54,71 @@ __global__ __launch_bounds__(256) void k_spai_build(ZRef z, const int* __restric
     rhs[a] = make_double2(v.x, -v.y);
   }
   // G = A^H A accumulated over row chunks (lower triangle incl. diagonal)
+  const int ntri = nJ * (nJ + 1) / 2;
   for (int r0 = 0; r0 < nI; r0 += SPAI_CHUNK) {
     const int nr = min(SPAI_CHUNK, nI - r0);
     __syncthreads();
     for (int e = tid; e < nr * nJ; e += 256) {
       int ii = e / nJ, a = e - ii * nJ;
-      Ach[ii * SPAI_MAXJ + a] = zat(z, iidx[i0 + r0 + ii], cols[a]);
+      Ach[ii * mJ + a] = zat(z, iidx[i0 + r0 + ii], cols[a]);
     }
     __syncthreads();
-    for (int e = tid; e < nJ * nJ; e += 256) {
-      int a = e / nJ, b = e - a * nJ;
-      if (b > a) continue;
+    for (int e = tid; e < ntri; e += 256) {
+      // e = a (a + 1) / 2 + b
+      int a = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+      while (a * (a + 1) / 2 > e) --a;
+      while ((a + 1) * (a + 2) / 2 <= e) ++a;
+      const int b = e - a * (a + 1) / 2;
       double sr = 0, si = 0;
       for (int ii = 0; ii < nr; ++ii) {
-        double2 p = Ach[ii * SPAI_MAXJ + a], q = Ach[ii * SPAI_MAXJ + b];
+        double2 p = Ach[ii * mJ + a], q = Ach[ii * mJ + b];
         // conj(p) * q
         sr += p.x * q.x + p.y * q.y;
         si += p.x * q.y - p.y * q.x;
       }
-      G[a * nJ + b].x += sr;
-      G[a * nJ + b].y += si;
+      G[e].x += sr;
+      G[e].y += si;
     }
   }
   __syncthreads();
   // Cholesky G = L L^H in place (lower triangle), right-looking
   for (int k = 0; k < nJ; ++k) {
     if (tid == 0) {
-      double d = G[k * nJ + k].x;
+      double d = G[tri(k, k)].x;
       if (!(d > 0.0)) {
         bad = 1;
         d = 1e-300;
       }
-      G[k * nJ + k] = make_double2(sqrt(d), 0.0);
+      G[tri(k, k)] = make_double2(sqrt(d), 0.0);
     }
     __syncthreads();
-    double lkk = G[k * nJ + k].x;
+    const double lkk = G[tri(k, k)].x;
     for (int i = k + 1 + tid; i < nJ; i += 256) {
-      double2 v = G[i * nJ + k];
-      G[i * nJ + k] = make_double2(v.x / lkk, v.y / lkk);
+      double2 v = G[tri(i, k)];
+      G[tri(i, k)] = make_double2(v.x / lkk, v.y / lkk);
     }
     __syncthreads();
-    int m = nJ - k - 1;
+    const int m = nJ - k - 1;
     for (int e = tid; e < m * m; e += 256) {
       int i = k + 1 + e / m, l = k + 1 + e % m;
       if (l > i) continue;
-      double2 a = G[i * nJ + k], b = G[l * nJ + k];
+      double2 a = G[tri(i, k)], b = G[tri(l, k)];
       // G[i][l] -= L[i][k] conj(L[l][k])
-      G[i * nJ + l].x -= a.x * b.x + a.y * b.y;
-      G[i * nJ + l].y -= a.y * b.x - a.x * b.y;
+      G[tri(i, l)].x -= a.x * b.x + a.y * b.y;
+      G[tri(i, l)].y -= a.y * b.x - a.x * b.y;
     }
     __syncthreads();
   }
   // forward: L y = rhs
   for (int k = 0; k < nJ; ++k) {
     if (tid == 0) {
-      double d = G[k * nJ + k].x;
+      double d = G[tri(k, k)].x;
       rhs[k] = make_double2(rhs[k].x / d, rhs[k].y / d);
     }
     __syncthreads();
     double2 yk = rhs[k];
     for (int i = k + 1 + tid; i < nJ; i += 256) {
-      double2 l = G[i * nJ + k];
+      double2 l = G[tri(i, k)];
       rhs[i].x -= l.x * yk.x - l.y * yk.y;
       rhs[i].y -= l.x * yk.y + l.y * yk.x;
     }
@@ -115,13 +127,13 @@ __global__ __launch_bounds__(256) void k_spai_build(ZRef z, const int* __restric
   // backward: L^H m = y
   for (int k = nJ - 1; k >= 0; --k) {
     if (tid == 0) {
-      double d = G[k * nJ + k].x;
+      double d = G[tri(k, k)].x;
       rhs[k] = make_double2(rhs[k].x / d, rhs[k].y / d);
     }
     __syncthreads();
     double2 mk = rhs[k];
     for (int i = tid; i < k; i += 256) {
-      double2 l = G[k * nJ + i];  // (L^H)[i][k] = conj(L[k][i])
+      double2 l = G[tri(k, i)];  // (L^H)[i][k] = conj(L[k][i])
       rhs[i].x -= l.x * mk.x + l.y * mk.y;
       rhs[i].y -= l.x * mk.y - l.y * mk.x;
     }
@@ -173,9 +185,14 @@ namespace dmp {
 
 int spai_max_cols() { return SPAI_MAXJ; }
 
-void spai_build(i64 n, const ZRef& z, const int* jptr, const int* jidx, const int* iptr, const int* iidx,
+void spai_build(i64 n, const ZRef& z, int maxJ, const int* jptr, const int* jidx, const int* iptr, const int* iidx,
                 double* vals, int* status, hipStream_t s) {
-  k_spai_build<<<(unsigned)n, 256, 0, s>>>(z, jptr, jidx, iptr, iidx, reinterpret_cast<double2*>(vals), status);
+  DM_REQUIRE(maxJ >= 1 && maxJ <= SPAI_MAXJ, DM_ERR_ARGUMENT, "near-field preconditioner: support width out of range");
+  const size_t smem = sizeof(double2) * ((size_t)maxJ * (maxJ + 1) / 2 + (size_t)SPAI_CHUNK * maxJ + maxJ) +
+                      sizeof(int) * maxJ;
+  HIP_OK(hipFuncSetAttribute((const void*)k_spai_build, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+  k_spai_build<<<(unsigned)n, 256, smem, s>>>(z, maxJ, jptr, jidx, iptr, iidx, reinterpret_cast<double2*>(vals),
+                                              status);
   HIP_OK(hipGetLastError());
 }
 

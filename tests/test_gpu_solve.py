@@ -8,7 +8,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import ROOT, oracle_rwg
+from conftest import ROOT, oracle_rwg, rel_fro
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(ROOT, "tests", "golden")
@@ -281,6 +281,36 @@ def test_batched_planewaves_end_to_end(dm):
     assert np.linalg.norm(X - Id) / np.linalg.norm(Id) < 1e-4
     Xs, ss, _ = dm.solve_planewaves(m, r, K0, pws[:3], "mixed", P, tol=1e-6, maxiter=300, batch=-1)
     assert np.linalg.norm(Xs - X[:, :3]) / np.linalg.norm(X[:, :3]) < 1e-4
+
+
+@pytest.mark.parametrize("nw", [1, 12])
+def test_mixed_pipeline_complex64_is_rounded_complex128(dm, O, nw):
+    """The mixed solve assembles once in FP64 and stores the complex64 matrix as
+    the rounding of the complex128 one (packed forms for one wave, full forms
+    for lockstep batches); the complex128 matrix meets the FP64 bar against the
+    oracle.  A plate (N = 338, not a multiple of the 128-wide storage tile)
+    exercises the zeroed padding of packed storage."""
+    m = dm.make_rect_plate(0.12, 0.1, 12, 10)
+    r = dm.build_rwg(m)
+    assert r.nedges == 338
+    pws = _random_planewaves(dm, np.random.default_rng(3), K0, nw)
+    P = dm.build_nearfield_preconditioner(None, m, r, 2.0 * float(np.mean(r.len)))
+    X, st, _, Z = dm.solve_planewaves(m, r, K0, pws, "mixed", P, tol=1e-6, maxiter=300, keep_matrix=True)
+    assert st.solved and st.residual <= 1e-6
+    f64, f32 = (dm.DM_Z64P, dm.DM_Z32P) if nw == 1 else (dm.DM_Z64, dm.DM_Z32)
+    assert Z.forms == f64 | f32
+    Z64, Z32 = Z.to_numpy(f64), Z.to_numpy(f32)
+    assert np.array_equal(Z32, Z64.astype(np.complex64))
+    assert rel_fro(Z64, O.assemble_Z_efie(oracle_rwg(O, m, r), K0, quad_order=3)) <= 1e-10
+    V = dm.assemble_multiple_excitations(m, r, pws)
+    res = np.linalg.norm(Z64 @ X - V, axis=0) / np.linalg.norm(V, axis=0)
+    assert res.max() <= 1e-6 * 1.001
+    # padding entries are zero: the symmetric product of the packed forms
+    # matches the dense product
+    if nw == 1:
+        x = X[:, 0]
+        y = Z.matmul(x, form=dm.DM_Z32P)
+        assert np.linalg.norm(y - Z32.astype(np.complex128) @ x) / np.linalg.norm(y) < 1e-12
 
 
 @pytest.mark.parametrize("method", ["gmres", "mixed"])
