@@ -186,6 +186,9 @@ def main():
     ap.add_argument("--precond", choices=["spai", "learned"], default="spai",
                     help="chosen form's preconditioner (learned needs the trained weights)")
     ap.add_argument("--nf-cutoff", type=float, default=2.0, help="near-field cutoff in mean edge lengths")
+    ap.add_argument("--compress-tol", type=float, default=5e-8,
+                    help="chosen form: cross-approximate the complex64 matrix to this relative Frobenius error "
+                         "(0 = dense complex64)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="no timing: 5 symmetric products of the packed complex64 matrix, then 3 plain 1 GiB "
                          "device-to-device copies (counter calibration), for rocprofv3 --pmc runs")
@@ -217,38 +220,52 @@ def main():
     chosen_P = learned if a.precond == "learned" else nf
     pw = [dm.make_plane_wave(k * khat, 1.0, pol)]
 
-    def solve(method, P, waves=pw, maxiter=1000):
+    def solve(method, P, waves=pw, maxiter=1000, ctol=0.0):
         ctx.sync()
         t0 = time.perf_counter()
-        X, st, t_asm = dm.solve_planewaves(mesh, rwg, k, waves, method, P, tol=1e-6, maxiter=maxiter)
+        X, st, t_asm = dm.solve_planewaves(mesh, rwg, k, waves, method, P, tol=1e-6, maxiter=maxiter,
+                                           compress_tol=ctol, batch=-1 if ctol > 0 else 0)
         ctx.sync()
         return time.perf_counter() - t0, X, st, t_asm
 
+    ctol = a.compress_tol
     for i in range(a.warmup):
-        dt, _, st, _ = solve("mixed", chosen_P)
+        dt, _, st, _ = solve("mixed", chosen_P, ctol=ctol)
         log(f"warmup {i}: {dt:.3f} s, {st.niter} it")
     ctx.profile_reset()
     ctx.profile(True)
     times = []
     for i in range(a.steps):
-        dt, X, st_c, tasm_c = solve("mixed", chosen_P)
+        dt, X, st_c, tasm_c = solve("mixed", chosen_P, ctol=ctol)
         x_c = X[:, 0]
         times.append(dt)
         log(f"step {i}: {dt:.3f} s (matrix ready {tasm_c * 1e3:.1f} ms, precond {st_c.precond_time_s * 1e3:.1f} ms), "
             f"{st_c.niter} it, {st_c.outer} refinements, residual {st_c.residual:.2e}")
     ctx.profile(False)
     kern = {}
-    for name in ("assemble_z64_z32", "assemble_z64", "assemble_z32", "near_tables", "symv_z32", "symv_z64", "gemv_z32", "gemv_z64",
-                 "arnoldi", "precond", "learned"):
+    for name in ("assemble_z64_z32", "assemble_z64", "assemble_z32", "near_tables", "compress", "symv_z32h", "symv_z32",
+                 "symv_z64", "gemv_z32", "gemv_z64", "arnoldi", "precond", "learned"):
         c, ms = ctx.profile_read(name)
         if c:
             kern[name] = dict(launches=c, total_ms=round(ms, 3), avg_ms=round(ms / c, 4))
     ctx.profile_reset()
     variants = {}
+    comp = None
+    if ctol > 0:
+        # compression statistics of the chosen form (same deterministic ranks)
+        _, _, _, Zk = dm.solve_planewaves(mesh, rwg, k, pw, "mixed", chosen_P, tol=1e-6, maxiter=1000,
+                                          compress_tol=ctol, batch=-1, keep_matrix=True)
+        comp = Zk.compress(eps=ctol)
+        del Zk
+        solve("mixed", chosen_P)
+        dt, X, st_d, _ = solve("mixed", chosen_P)
+        variants["mixed_dense_complex64"] = dict(s=round(dt, 4), iterations=st_d.niter, refinements=st_d.outer,
+                                                 true_residual=st_d.residual)
+        log(f"mixed (dense complex64): {dt:.3f} s, {st_d.niter} it, residual {st_d.residual:.2e}")
     other = ("learned", learned) if a.precond == "spai" else ("spai", nf)
     if other[1] is not None:
-        solve("mixed", other[1])
-        dt, X, st_o, _ = solve("mixed", other[1])
+        solve("mixed", other[1], ctol=ctol)
+        dt, X, st_o, _ = solve("mixed", other[1], ctol=ctol)
         variants[f"mixed_{other[0]}"] = dict(s=round(dt, 4), iterations=st_o.niter, refinements=st_o.outer,
                                              true_residual=st_o.residual)
         log(f"mixed ({other[0]}): {dt:.3f} s, {st_o.niter} it, residual {st_o.residual:.2e}")
@@ -292,6 +309,8 @@ def main():
     pf = far_pairs(mesh, rwg)
     packed = 4.0 * N * (N + 1)  # upper triangle, complex64
     alg = {"symv_z32": ("hbm", packed + 32.0 * N, HBM_PEAK_GBS, "GB/s"),
+           "symv_z32h": ("hbm", (comp["bytes_ratio"] * comp["tiles"] * 128 * 128 * 8 if comp else packed) + 32.0 * N,
+                         HBM_PEAK_GBS, "GB/s"),
            "symv_z64": ("hbm", 2 * packed + 32.0 * N, HBM_PEAK_GBS, "GB/s"),
            "gemv_z32": ("hbm", 8.0 * N * N + 32.0 * N, HBM_PEAK_GBS, "GB/s"),
            "gemv_z64": ("hbm", 16.0 * N * N + 32.0 * N, HBM_PEAK_GBS, "GB/s"),
@@ -323,7 +342,10 @@ def main():
                                "normal-incidence x-pol plane wave, tol 1e-6 true FP64 residual",
                    "N": N, "Nt": mesh.nt, "tol": 1e-6},
         "plain_fp64_s": round(t_p, 4), "speedup_vs_plain_fp64": round(t_p / t_c, 3),
-        "chosen": {"form": "mixed precision, packed symmetric storage, FGMRES + FP64 refinement",
+        "chosen": {"form": ("mixed precision, packed symmetric storage" +
+                            (f", complex64 tiles cross-approximated to {ctol:g} relative Frobenius error"
+                             if ctol > 0 else "") + ", FGMRES + FP64 refinement against the dense complex128 matrix"),
+                   "compression": comp,
                    "precond": (f"near-field SPAI cutoff {a.nf_cutoff} x mean edge" if a.precond == "spai" else
                                f"SPAI ({a.nf_cutoff} x mean edge) + learned message-passing network"),
                    "iterations": st_c.niter, "refinements": st_c.outer, "true_residual": st_c.residual,

@@ -45,6 +45,7 @@ DM_ERR_CONVERGENCE, DM_ERR_DEVICE, DM_ERR_MEMORY, DM_ERR_DOMAIN = 4, 5, 6, 7
 DM_Z64, DM_Z32 = 1, 2
 # packed symmetric storage of the same matrices (upper triangle of 128 x 128 tiles)
 DM_Z64P, DM_Z32P = 4, 8
+DM_Z32H = 16  # compressed complex64 (products and solves only)
 DM_METHOD_LU, DM_METHOD_GMRES, DM_METHOD_MIXED = 0, 1, 2
 DM_PRECOND_NONE, DM_PRECOND_DIAG, DM_PRECOND_NEARFIELD, DM_PRECOND_LEARNED, DM_PRECOND_BLOCK = 0, 1, 2, 3, 4
 
@@ -81,7 +82,8 @@ class dm_solve_opts(C.Structure):
     _fields_ = [("method", C.c_int), ("precond", C.c_int), ("precond_side", C.c_int), ("maxiter", C.c_int),
                 ("tol", C.c_double), ("nf_cutoff", C.c_double), ("restart", C.c_int),
                 ("check_convergence", C.c_int), ("inner_tol", C.c_double), ("max_outer", C.c_int),
-                ("block_size", C.c_int), ("batch", C.c_int), ("learned", C.c_void_p), ("adjoint", C.c_int)]
+                ("block_size", C.c_int), ("batch", C.c_int), ("learned", C.c_void_p), ("adjoint", C.c_int),
+                ("compress_tol", C.c_double)]
 
 
 class dm_solve_stats(C.Structure):
@@ -117,6 +119,7 @@ def lib():
         "dm_zmat_get": ([vp, i32, vp], i32), "dm_zmat_entries": ([vp, i32, vp, vp, i64, vp], i32),
         "dm_zmat_free": ([vp], i32),
         "dm_apply": ([vp, vp, i32, i32, vp, i64, i64, vp, i64], i32),
+        "dm_zmat_compress": ([vp, C.c_double, i32, vp], i32),
         "dm_mf_create": ([vp, vp, d, d, d, i32, pp], i32), "dm_mf_apply": ([vp, vp, i32, vp, vp], i32),
         "dm_mf_free": ([vp], i32),
         "dm_excitation_planewave": ([vp, vp, vp, vp, vp, i64, i32, vp], i32),
@@ -422,6 +425,16 @@ class ZMatrix:
                 return f
         raise ValueError("matrix holds no stored form")
 
+    def compress(self, eps=5e-8, rmax=48):
+        """Cross-approximate the off-diagonal tiles of the packed complex64 form
+        (dm_zmat_compress): ||Z32P - Z32H||_F <= eps ||Z32P||_F.  Adds DM_Z32H,
+        which mixed-precision solves then use for the inner products.  Returns
+        the tile statistics."""
+        st = np.zeros(4)
+        _check(lib().dm_zmat_compress(self.h, float(eps), int(rmax), _ptr(st)))
+        self.forms |= DM_Z32H
+        return dict(tiles=int(st[0]), lowrank_tiles=int(st[1]), mean_rank=float(st[2]), bytes_ratio=float(st[3]))
+
     def to_numpy(self, form=None):
         out = np.zeros((self.N, self.N), dtype=np.complex128, order="F")
         _check(lib().dm_zmat_get(self.h, self._form(form), out.ctypes.data_as(C.c_void_p)))
@@ -663,13 +676,15 @@ def _solve(Z, B, method, precond=None, side="right", tol=1e-6, maxiter=300, rest
 
 
 def _opts(method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, check=True, inner_tol=0.0,
-          max_outer=10, batch=0):
+          max_outer=10, batch=0, compress_tol=0.0):
     """dm_solve_opts.  batch: 0 = lockstep batches of right-hand sides when there
-    are >= 8 of them, 1 = always, -1 = one right-hand side at a time."""
+    are >= 8 of them, 1 = always, -1 = one right-hand side at a time.
+    compress_tol > 0: the mixed solve compresses its complex64 matrix."""
     o = dm_solve_opts()
     lib().dm_solve_opts_default(C.byref(o))
     o.method, o.tol, o.maxiter, o.restart = method, float(tol), int(maxiter), int(restart)
     o.batch = int(batch)
+    o.compress_tol = float(compress_tol)
     o.check_convergence, o.inner_tol, o.max_outer = int(check), float(inner_tol), int(max_outer)
     o.precond_side = 1 if side == "right" else 0
     if precond is None or precond == "none":
@@ -692,13 +707,16 @@ def _opts(method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, 
 
 
 def solve_planewaves(mesh, rwg, k, excitations, method="mixed", preconditioner=None, tol=1e-6, maxiter=300,
-                     quad_order=3, eta0=ETA0, check_convergence=True, keep_matrix=False, batch=0, restart=0):
+                     quad_order=3, eta0=ETA0, check_convergence=True, keep_matrix=False, batch=0, restart=0,
+                     compress_tol=0.0):
     """End-to-end device solve for plane waves (the dense branches of
     solve_scattering, src/Workflow.jl:67-317): assembly, excitation,
     preconditioner and Krylov solve in one call.  method: "direct" (dense LU),
     "fp64" (FP64 GMRES on the complex128 matrix) or "mixed" (FGMRES on the
-    complex64 matrix with FP64 Arnoldi and FP64 refinement; the complex128
-    matrix is assembled on a second stream while the solve runs).
+    complex64 matrix with FP64 Arnoldi and FP64 refinement against the
+    complex128 matrix; one FP64 assembly writes both).  compress_tol > 0 (mixed,
+    one wave at a time): the complex64 matrix is compressed first
+    (ZMatrix.compress) and the inner products use the compressed form.
     Returns (X, stats, assembly_time_s[, ZMatrix])."""
     _check_pair(mesh, rwg)
     M = len(excitations)
@@ -706,7 +724,8 @@ def solve_planewaves(mesh, rwg, k, excitations, method="mixed", preconditioner=N
     pol = np.ascontiguousarray(np.array([e.pol for e in excitations], float).reshape(M, 3))
     E0 = np.ascontiguousarray([float(e.E0) for e in excitations])
     meth = {"direct": DM_METHOD_LU, "fp64": DM_METHOD_GMRES, "mixed": DM_METHOD_MIXED}[method]
-    o = _opts(meth, preconditioner, "right", tol, maxiter, restart=restart, check=check_convergence, batch=batch)
+    o = _opts(meth, preconditioner, "right", tol, maxiter, restart=restart, check=check_convergence, batch=batch,
+              compress_tol=compress_tol)
     X = np.zeros((rwg.nedges, M), dtype=np.complex128, order="F")
     st = dm_solve_stats()
     t_asm = C.c_double()

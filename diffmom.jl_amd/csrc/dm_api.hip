@@ -201,6 +201,60 @@ int pick_form(const dm_zmat* z, bool f32, bool batched) {
   return hp ? packed : full;
 }
 
+// Operator form of the complex64 side of a mixed solve: the compressed form
+// when it exists and columns are solved one at a time.
+int op_form(const dm_zmat* z, bool batched) {
+  if (!batched && (z->forms & DM_Z32H)) return DM_Z32H;
+  return pick_form(z, true, batched);
+}
+
+constexpr int kAcaRankCap = 48;
+
+// Cross approximation of the packed complex64 form (see dm_zmat_compress).
+void compress_z32p(dm_zmat* z, double eps, int rmax, double* stats, hipStream_t s) {
+  DM_REQUIRE(z->forms & DM_Z32P, DM_ERR_ARGUMENT, "compression needs the packed complex64 form (DM_Z32P)");
+  DM_REQUIRE(eps > 0 && std::isfinite(eps), DM_ERR_ARGUMENT, "compression tolerance must be positive");
+  DM_REQUIRE(rmax >= 1 && rmax <= 64, DM_ERR_ARGUMENT, "rank cap must be in 1..64");
+  const i64 n = z->n, nb = (n + kPT - 1) / kPT, T = packed_tiles(n);
+  DBuf<double> nrm;
+  nrm.alloc(T);
+  dml::tile_norm2(n, z->z32p.p, nrm.p, s);
+  std::vector<double> h(T);
+  HIP_OK(hipMemcpyAsync(h.data(), nrm.p, sizeof(double) * T, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  double fro2 = 0;
+  for (i64 J = 0; J < nb; ++J)
+    for (i64 I = 0; I <= J; ++I) fro2 += (I == J ? 1.0 : 2.0) * h[J * (J + 1) / 2 + I];
+  const i64 toff = T - nb;
+  const double tau2 = toff > 0 ? eps * eps * fro2 / (2.0 * (double)toff) : 0.0;
+  z->hrank.alloc(T);
+  z->hfac.alloc((size_t)T * 2 * kPT * rmax * 2);
+  z->hrmax = rmax;
+  dml::aca_compress(n, z->z32p.p, tau2, rmax, z->hrank.p, z->hfac.p, s);
+  std::vector<int> rk(T);
+  HIP_OK(hipMemcpyAsync(rk.data(), z->hrank.p, sizeof(int) * T, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  i64 low = 0;
+  double rsum = 0, bytes = 0;
+  for (i64 t = 0; t < T; ++t) {
+    if (rk[t] >= 0) {
+      ++low;
+      rsum += rk[t];
+      bytes += 2.0 * kPT * rk[t] * 8;
+    } else {
+      bytes += (double)kPT * kPT * 8;
+    }
+  }
+  z->hbytes = bytes;
+  z->forms |= DM_Z32H;
+  if (stats) {
+    stats[0] = (double)T;
+    stats[1] = (double)low;
+    stats[2] = low ? rsum / low : 0.0;
+    stats[3] = bytes / ((double)T * kPT * kPT * 8);
+  }
+}
+
 // Solve the nrhs columns of Bd (device, internal basis order, n x nrhs) and
 // write the currents in caller order to host X (leading dimension ldx).
 void solve_columns(dm_ctx* ctx, const dm_solve_opts& o, const dm_rwg& r, DevOp* A64, DevOp* A32, DevPrec* M,
@@ -254,6 +308,8 @@ void validate_opts(const dm_solve_opts& o) {
   DM_REQUIRE(o.maxiter >= 1 && o.maxiter <= 1000, DM_ERR_ARGUMENT, "maxiter must be in [1, 1000]");
   DM_REQUIRE(o.precond_side == 0 || o.precond_side == 1, DM_ERR_ARGUMENT, "precond_side must be 0 (left) or 1 (right)");
   DM_REQUIRE(o.method >= DM_METHOD_LU && o.method <= DM_METHOD_MIXED, DM_ERR_ARGUMENT, "unknown solve method");
+  DM_REQUIRE(std::isfinite(o.compress_tol) && o.compress_tol >= 0 && o.compress_tol < 1, DM_ERR_ARGUMENT,
+             "compress_tol must be in [0, 1)");
 }
 
 void check_ctx(const void* a, const char* what) { DM_REQUIRE(a != nullptr, DM_ERR_ARGUMENT, std::string(what) + " is NULL"); }
@@ -288,6 +344,12 @@ int dm_destroy(dm_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream2);
+    for (auto& r : ctx->recs) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (hipEvent_t e : ctx->ev_free) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     (void)hipStreamDestroy(ctx->stream2);
     delete ctx;
@@ -316,9 +378,10 @@ int dm_profile_reset(dm_ctx* ctx) {
     check_ctx(ctx, "context");
     CtxScope sc(ctx);
     HIP_OK(hipStreamSynchronize(ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream2));
     for (auto& r : ctx->recs) {
-      (void)hipEventDestroy(r.a);
-      (void)hipEventDestroy(r.b);
+      ctx->ev_free.push_back(r.a);
+      ctx->ev_free.push_back(r.b);
     }
     ctx->recs.clear();
   });
@@ -633,6 +696,14 @@ static void need_form(const dm_zmat* z, int form) {
   DM_REQUIRE(z->forms & form, DM_ERR_ARGUMENT, "requested form was not assembled");
 }
 
+int dm_zmat_compress(dm_zmat* z, double eps, int rmax, double* stats) {
+  return guard([&] {
+    check_ctx(z, "zmat");
+    CtxScope sc(z->ctx);
+    compress_z32p(z, eps, rmax, stats, z->ctx->stream);
+  });
+}
+
 int dm_zmat_get(const dm_zmat* z, int form, double* out) {
   return guard([&] {
     check_ctx(z, "zmat");
@@ -688,7 +759,10 @@ int dm_apply(dm_ctx* ctx, const dm_zmat* z, int form, int adjoint, const double*
   return guard([&] {
     check_ctx(ctx, "context");
     check_ctx(z, "zmat");
-    need_form(z, form);
+    if (form == DM_Z32H)
+      DM_REQUIRE(z->forms & DM_Z32H, DM_ERR_ARGUMENT, "requested form was not assembled");
+    else
+      need_form(z, form);
     i64 n = z->n;
     DM_REQUIRE(nrhs >= 0 && ldx >= n && ldy >= n, DM_ERR_DIMENSION, "leading dimension smaller than N");
     CtxScope sc(ctx);
@@ -885,6 +959,7 @@ int dm_precond_free(dm_precond* P) {
 void dm_solve_opts_default(dm_solve_opts* o) {
   if (!o) return;
   o->method = DM_METHOD_GMRES;
+  o->compress_tol = 0.0;
   o->precond = DM_PRECOND_NEARFIELD;
   o->precond_side = 1;
   o->maxiter = 300;
@@ -968,6 +1043,10 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
       ProfScope p(ctx, "assemble_z64", s);
       launch_assemble_z64(a, z->z64.p, r.max_blk_tris, s);
     }
+    if (o.method == DM_METHOD_MIXED && !batched && o.compress_tol > 0) {
+      ProfScope pc(ctx, "compress", s);
+      compress_z32p(z.get(), o.compress_tol, kAcaRankCap, nullptr, s);
+    }
     DBuf<double> dk, dp, de, V, x, xe;
     dk.upload(kvec, 3 * nrhs, s);
     dp.upload(pol, 3 * nrhs, s);
@@ -980,7 +1059,7 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
     double t_asm = now() - t0;
     dm_solve_stats st{};
     std::unique_ptr<DevOp> A64 = make_op(z.get(), pick_form(z.get(), false, batched)), A32;
-    if (o.method == DM_METHOD_MIXED) A32 = make_op(z.get(), pick_form(z.get(), true, batched));
+    if (o.method == DM_METHOD_MIXED) A32 = make_op(z.get(), op_form(z.get(), batched));
     double t1 = now();
     double worst = 0;
     bool all_ok = true;
@@ -1104,7 +1183,7 @@ int dm_solve(dm_ctx* ctx, const dm_zmat* z, const double* B, int64_t ldb, int64_
     double t1 = now();
     std::unique_ptr<DevOp> A64, A32;
     if (f64) A64 = make_op(z, f64);
-    if (f32) A32 = make_op(z, f32);
+    if (f32) A32 = make_op(z, o.method == DM_METHOD_MIXED ? op_form(z, batched) : f32);
     bool all_ok = true;
     double worst = 0;
     DBuf<double> Bd;

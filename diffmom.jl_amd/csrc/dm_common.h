@@ -78,7 +78,18 @@ struct dm_ctx {
   int cu_count = 256;
   bool prof = false;              // record HIP events around tracked launches
   std::vector<ProfRec> recs;
+  std::vector<hipEvent_t> ev_free;  // recycled profiling events
   std::map<std::string, int> tune;  // launch parameters chosen by the tuner, by shape key
+  hipEvent_t ev_get() {
+    if (ev_free.empty()) {
+      hipEvent_t e;
+      HIP_OK(hipEventCreate(&e));
+      return e;
+    }
+    hipEvent_t e = ev_free.back();
+    ev_free.pop_back();
+    return e;
+  }
 };
 
 // Brackets one tracked launch with HIP events on its stream when profiling is on.
@@ -90,8 +101,8 @@ struct ProfScope {
   ProfScope(dm_ctx* c_, const char* name, hipStream_t s_) : c(c_), s(s_), on(c_ && c_->prof) {
     if (!on) return;
     rec.name = name;
-    HIP_OK(hipEventCreate(&rec.a));
-    HIP_OK(hipEventCreate(&rec.b));
+    rec.a = c->ev_get();
+    rec.b = c->ev_get();
     HIP_OK(hipEventRecord(rec.a, s));
   }
   ~ProfScope() {
@@ -218,6 +229,12 @@ struct dm_zmat {
   DBuf<float> z32;    // ld x n complex64, internal order
   DBuf<double> z64p;  // packed symmetric complex128
   DBuf<float> z32p;   // packed symmetric complex64
+  // compressed complex64 form over z32p: per stored tile its rank (-1: dense)
+  // and a slot of 2 x hrmax columns of 128 complex64 (U then V)
+  DBuf<int> hrank;
+  DBuf<float> hfac;
+  int hrmax = 0;
+  double hbytes = 0;  // bytes one product reads
   ZRef ref(int form) const {
     ZRef r;
     r.ld = ld;

@@ -70,24 +70,41 @@ struct OpZ32 : DevOp {
 // vector; batches loop over columns.
 }  // namespace dms
 namespace dmt {
-// Tuned segment length of the symmetric product (timed on first use per shape).
-int symv_seg(dm_ctx* ctx, const dm_zmat* z, int form, void* ws, hipStream_t s);
+// Tuned launch shape of the symmetric product (timed on first use per size).
+int symv_shape(dm_ctx* ctx, const dm_zmat* z, int form, void* ws, hipStream_t s);
+// Tuned tiles per wave of the compressed product.
+int symv_hier_shape(dm_ctx* ctx, const dm_zmat* z, void* ws, hipStream_t s);
 }  // namespace dmt
 namespace dms {
+struct OpHier : DevOp {
+  const dm_zmat* z;
+  int tpw = 0;
+  DBuf<char> ws;
+  explicit OpHier(const dm_zmat* z_) : z(z_) { ws.alloc(dml::symv_workspace_bytes(z->n)); }
+  void apply(const double* x, double* y, hipStream_t s) override {
+    if (!tpw) tpw = dmt::symv_hier_shape(z->ctx, z, ws.p, s);
+    ProfScope p(z->ctx, "symv_z32h", s);
+    dml::symv_hier(z->n, z->z32p.p, z->hrank.p, z->hfac.p, z->hrmax, x, y, ws.p, tpw, s);
+  }
+  void apply_batch(const double* X, double* Y, int R, void*, hipStream_t s) override {
+    for (int j = 0; j < R; ++j) apply(X + 2 * z->n * j, Y + 2 * z->n * j, s);
+  }
+  size_t batch_ws(int) const override { return 0; }
+};
 struct OpPacked : DevOp {
   const dm_zmat* z;
   int form;
   const char* tag;
-  int seg = 0;  // chosen on the first product (the matrix must be assembled by then)
+  int shape = 0;  // chosen on the first product (the matrix must be assembled by then)
   DBuf<char> ws;
   OpPacked(const dm_zmat* z_, int form_) : z(z_), form(form_) {
     tag = form == DM_Z32P ? "symv_z32" : "symv_z64";
     ws.alloc(dml::symv_workspace_bytes(z->n));
   }
   void apply(const double* x, double* y, hipStream_t s) override {
-    if (!seg) seg = dmt::symv_seg(z->ctx, z, form, ws.p, s);
+    if (!shape) shape = dmt::symv_shape(z->ctx, z, form, ws.p, s);
     ProfScope p(z->ctx, tag, s);
-    dml::symv_packed(z->n, z->ref(form), x, y, ws.p, seg, s);
+    dml::symv_packed(z->n, z->ref(form), x, y, ws.p, shape, s);
   }
   void apply_batch(const double* X, double* Y, int R, void*, hipStream_t s) override {
     for (int j = 0; j < R; ++j) apply(X + 2 * z->n * j, Y + 2 * z->n * j, s);
@@ -139,9 +156,19 @@ struct Krylov {
   i64 n = 0;
   int m = 0;  // basis capacity
   DBuf<double> V, Zs, w, t, hbuf, ybuf, part;
+  // device Givens state: H ((m + 1) x m complex), rotations, rotated rhs,
+  // per-step (estimate, subdiagonal), 1 / subdiagonal
+  DBuf<double> Hd, csd, snd, gd, stt, scd;
+  double g0[2] = {0.0, 0.0};
   void reserve(i64 n_, int m_, bool flexible) {
     n = n_;
     m = m_;
+    Hd.alloc((size_t)2 * (m + 1) * m);
+    csd.alloc(m);
+    snd.alloc(2 * (size_t)m);
+    gd.alloc(2 * (size_t)(m + 1));
+    stt.alloc(2 * (size_t)m);
+    scd.alloc(1);
     part.alloc(2 * dml::dots_part_count(n, m + 1));
     V.alloc((size_t)2 * n * (m + 1));
     if (flexible) Zs.alloc((size_t)2 * n * m);

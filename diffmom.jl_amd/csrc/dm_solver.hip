@@ -17,6 +17,7 @@ std::unique_ptr<DevOp> make_op(const dm_zmat* z, int form) {
   DM_REQUIRE(z->forms & form, DM_ERR_ARGUMENT, "requested form was not assembled");
   if (form == DM_Z64) return std::make_unique<OpZ64>(z);
   if (form == DM_Z32) return std::make_unique<OpZ32>(z);
+  if (form == DM_Z32H) return std::make_unique<OpHier>(z);
   return std::make_unique<OpPacked>(z, form);
 }
 
@@ -121,17 +122,26 @@ GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double*
     out.solved = true;
     return out;
   }
-  std::vector<cd> H((size_t)(mcap + 1) * mcap), g(mcap + 1), sn(mcap);
-  std::vector<double> cs(mcap);
-  std::vector<double> hh(2 * (2 * (mcap + 1) + 1));
+  // The Givens rotations run on the device (k_givens after each Arnoldi step),
+  // so iterations are queued without a host round trip; the host reads the
+  // residual estimates only at check points predicted from their decay rate
+  // (every iteration near convergence).  Steps queued past the stopping point
+  // are discarded: the leading columns of H and g do not change after their
+  // own step.
+  std::vector<double> sth(2 * (size_t)mcap);
   int total = 0;
+  double est = beta;
   while (true) {
     dml::axpby(n, 1.0 / beta, 0, r, 0, 0, r, s);
-    std::fill(g.begin(), g.end(), cd(0));
-    g[0] = beta;
-    int j = 0;
-    double est = beta;
-    for (; j < mcap && total < maxiter; ++j) {
+    K.g0[0] = beta;
+    K.g0[1] = 0.0;
+    HIP_OK(hipMemsetAsync(K.gd.p, 0, K.gd.bytes(), s));
+    HIP_OK(hipMemcpyAsync(K.gd.p, K.g0, sizeof(K.g0), hipMemcpyHostToDevice, s));
+    int j = 0, jstop = -1, seen = 0, step = 1;
+    int jprev = 0;
+    double eprev = beta;
+    est = beta;
+    while (j < mcap && total < maxiter) {
       double* vj = col(V, j);
       if (right) {
         double* zj = col(K.Zs.p, j);
@@ -147,59 +157,51 @@ GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double*
       } else {
         A.apply(vj, K.w.p, s);
       }
-      int k = j + 1;
-      double* h1 = K.hbuf.p;
-      double* h2 = K.hbuf.p + 2 * k;
+      const int k = j + 1;
       {
         ProfScope p(ctx, "arnoldi", s);
+        double* h1 = K.hbuf.p;
+        double* h2 = K.hbuf.p + 2 * k;
         dml::dots(n, ldv / 2, k, V, K.w.p, h1, K.part.p, s);
         dml::update(n, ldv / 2, k, V, h1, K.w.p, s);
         dml::dots(n, ldv / 2, k, V, K.w.p, h2, K.part.p, s);
         dml::update(n, ldv / 2, k, V, h2, K.w.p, s);
         dml::norm2(n, K.w.p, K.hbuf.p + 4 * k, s);
+        dml::givens(j, mcap + 1, K.hbuf.p, K.Hd.p, K.csd.p, K.snd.p, K.gd.p, K.stt.p, K.scd.p, s);
+        dml::scale_dev(n, K.scd.p, K.w.p, col(V, k), s);
       }
-      HIP_OK(hipMemcpyAsync(hh.data(), K.hbuf.p, sizeof(double) * (4 * k + 1), hipMemcpyDeviceToHost, s));
-      HIP_OK(hipStreamSynchronize(s));
-      cd* Hj = &H[(size_t)j * (mcap + 1)];
-      for (int i = 0; i < k; ++i) Hj[i] = cd(hh[2 * i] + hh[2 * k + 2 * i], hh[2 * i + 1] + hh[2 * k + 2 * i + 1]);
-      double hn = std::sqrt(hh[4 * k]);
-      Hj[k] = hn;
-      if (hn > 0 && k <= mcap) dml::axpby(n, 1.0 / hn, 0, K.w.p, 0, 0, col(V, k), s);
-      for (int i = 0; i < j; ++i) {
-        cd t = cs[i] * Hj[i] + sn[i] * Hj[i + 1];
-        Hj[i + 1] = -std::conj(sn[i]) * Hj[i] + cs[i] * Hj[i + 1];
-        Hj[i] = t;
-      }
-      cd a = Hj[j], bb = Hj[j + 1];
-      double rho = std::hypot(std::abs(a), std::abs(bb));
-      if (std::abs(a) == 0) {
-        cs[j] = 0;
-        sn[j] = std::abs(bb) > 0 ? std::conj(bb) / std::abs(bb) : cd(1);
-      } else {
-        cs[j] = std::abs(a) / rho;
-        sn[j] = (a / std::abs(a)) * std::conj(bb) / rho;
-      }
-      Hj[j] = cs[j] * a + sn[j] * bb;
-      Hj[j + 1] = 0;
-      g[j + 1] = -std::conj(sn[j]) * g[j];
-      g[j] = cs[j] * g[j];
-      est = std::abs(g[j + 1]);
+      ++j;
       ++total;
-      if (est <= eps || hn == 0) {
-        ++j;
-        break;
+      if (j - seen < step && j < mcap && total < maxiter) continue;
+      HIP_OK(hipMemcpyAsync(sth.data() + 2 * seen, K.stt.p + 2 * seen, sizeof(double) * 2 * (j - seen),
+                            hipMemcpyDeviceToHost, s));
+      HIP_OK(hipStreamSynchronize(s));
+      for (int i = seen; i < j; ++i) {
+        est = sth[2 * i];
+        if (est <= eps || sth[2 * i + 1] == 0) {
+          jstop = i + 1;
+          break;
+        }
       }
+      seen = j;
+      if (jstop >= 0) break;
+      // next check: ~70 % of the steps the recent decay rate says are left
+      const double rate = j > jprev && eprev > 0 ? std::pow(est / eprev, 1.0 / (j - jprev)) : 1.0;
+      if (rate > 0 && rate < 0.999 && est > eps) {
+        const double left_steps = std::log(eps / est) / std::log(rate);
+        step = std::max(1, std::min(16, (int)(0.7 * left_steps)));
+      } else {
+        step = std::min(16, 2 * step);
+      }
+      jprev = j;
+      eprev = est;
     }
+    if (jstop < 0) jstop = j;
+    total -= j - jstop;  // queued steps past the stopping point do not count
     // y = H^-1 g, x += V y (or Zs y)
-    std::vector<cd> y(j);
-    for (int i = j - 1; i >= 0; --i) {
-      cd acc = g[i];
-      for (int l = i + 1; l < j; ++l) acc -= H[(size_t)l * (mcap + 1) + i] * y[l];
-      y[i] = acc / H[(size_t)i * (mcap + 1) + i];
-    }
-    if (j > 0) {
-      HIP_OK(hipMemcpyAsync(K.ybuf.p, y.data(), 16 * j, hipMemcpyHostToDevice, s));
-      dml::combine(n, ldv / 2, j, right ? K.Zs.p : V, K.ybuf.p, x, s);
+    if (jstop > 0) {
+      dml::backsolve(jstop, mcap + 1, K.Hd.p, K.gd.p, K.ybuf.p, s);
+      dml::combine(n, ldv / 2, jstop, right ? K.Zs.p : V, K.ybuf.p, x, s);
       HIP_OK(hipStreamSynchronize(s));
     }
     out.iters = total;

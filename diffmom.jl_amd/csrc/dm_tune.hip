@@ -42,7 +42,7 @@ double time_ms(Launch&& run, hipStream_t s) {
 
 }  // namespace
 
-int symv_seg(dm_ctx* ctx, const dm_zmat* z, int form, void* ws, hipStream_t s) {
+int symv_shape(dm_ctx* ctx, const dm_zmat* z, int form, void* ws, hipStream_t s) {
   const int fallback = 8;
   const std::string key = std::string("symv/") + (form == DM_Z32P ? "c64/" : "c128/") + std::to_string(z->n);
   auto it = ctx->tune.find(key);
@@ -55,11 +55,35 @@ int symv_seg(dm_ctx* ctx, const dm_zmat* z, int form, void* ws, hipStream_t s) {
   const ZRef zr = z->ref(form);
   int best = fallback;
   double best_ms = 1e300;
-  for (int seg : {2, 4, 8, 16, 32}) {
-    double ms = time_ms([&] { dml::symv_packed(z->n, zr, x.p, y.p, ws, seg, s); }, s);
+  for (int shape : dml::symv_shapes(z->n, form == DM_Z32P)) {
+    double ms = time_ms([&] { dml::symv_packed(z->n, zr, x.p, y.p, ws, shape, s); }, s);
     if (ms < best_ms) {
       best_ms = ms;
-      best = seg;
+      best = shape;
+    }
+  }
+  ctx->tune[key] = best;
+  return best;
+}
+
+int symv_hier_shape(dm_ctx* ctx, const dm_zmat* z, void* ws, hipStream_t s) {
+  const int fallback = 2;
+  const std::string key = "symvh/c64/" + std::to_string(z->n);
+  auto it = ctx->tune.find(key);
+  if (it != ctx->tune.end()) return it->second;
+  if (!tuning_enabled()) return fallback;
+  DBuf<double> x, y;
+  x.alloc(2 * z->n);
+  y.alloc(2 * z->n);
+  HIP_OK(hipMemsetAsync(x.p, 0, x.bytes(), s));
+  int best = fallback;
+  double best_ms = 1e300;
+  for (int tpw : dml::symv_hier_shapes(z->n)) {
+    double ms = time_ms(
+        [&] { dml::symv_hier(z->n, z->z32p.p, z->hrank.p, z->hfac.p, z->hrmax, x.p, y.p, ws, tpw, s); }, s);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = tpw;
     }
   }
   ctx->tune[key] = best;

@@ -1,0 +1,192 @@
+// k_aca.hip — compression of the packed complex64 matrix: every off-diagonal
+// stored tile (128 x 128) whose numerical rank is small is replaced by a cross
+// approximation Z_IJ ~ U V^T.
+//
+// The reference compresses far blocks with adaptive cross approximation over
+// entries computed on demand (src/fast/ACA.jl:558-688).  Here the dense tile
+// is already in HBM after assembly, so the cross approximation runs with full
+// pivoting on the whole tile held in registers (512 threads x 32 entries):
+// each step picks the largest residual entry (i*, j*), takes U[:, k] =
+// R[:, j*] and V[:, k] = R[i*, :] / R[i*, j*], and subtracts U[:, k] V[:, k]^T
+// from the residual.  The Frobenius norm of the residual is summed exactly at
+// every step (FP64 sum of FP32 entries), so the stopping test ||R||_F <= tau is
+// a bound on the tile's approximation error (up to the FP32 rounding of the
+// updates, ~1e-7 of the tile's norm), not an estimate.  With
+// tau^2 = eps^2 ||Z||_F^2 / (2 T_off) for T_off stored off-diagonal tiles, the
+// whole matrix (where each such tile appears twice) satisfies
+// ||Z - Z_H||_F <= eps ||Z||_F.  Tiles that need more than rmax steps stay
+// dense (rank -1), as do the diagonal tiles.
+#include <hip/hip_runtime.h>
+
+#include "dm_common.h"
+#include "k_linalg.h"
+
+namespace {
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+constexpr int kAcaThreads = 512;
+
+// Sum of |z|^2 over each stored tile (FP64), one workgroup per tile.
+__global__ __launch_bounds__(256) void k_tile_norm2(const float2* __restrict__ zp, double* __restrict__ out) {
+  const i64 t = blockIdx.x;
+  const f4v* tile = reinterpret_cast<const f4v*>(zp + t * kPT * kPT);
+  double s = 0;
+  for (int k = threadIdx.x; k < kPT * kPT / 2; k += 256) {
+    const f4v v = __builtin_nontemporal_load(tile + k);
+    s += (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2] + (double)v[3] * v[3];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  __shared__ double ws[4];
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[t] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+}
+
+__device__ inline float2 cdivf(float2 a, float2 b) {
+  const float d = b.x * b.x + b.y * b.y;
+  return make_float2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+
+// Thread t owns column t / 4, rows 32 (t % 4) .. + 31 of the tile.
+__global__ __launch_bounds__(kAcaThreads) void k_aca(int nb, const float2* __restrict__ zp, double tau2, int rmax,
+                                                     int* __restrict__ rank, float2* __restrict__ fac) {
+  const i64 t = blockIdx.x;
+  int J = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((i64)J * (J + 1) / 2 > t) --J;
+  while ((i64)(J + 1) * (J + 2) / 2 <= t) ++J;
+  const int I = (int)(t - (i64)J * (J + 1) / 2);
+  const int tid = threadIdx.x;
+  if (I == J) {
+    if (tid == 0) rank[t] = -1;
+    return;
+  }
+  const int c = tid >> 2, r0 = (tid & 3) * 32, lane = tid & 63, w = tid >> 6;
+  float Rr[32], Ri[32];
+  {
+    const f4v* src = reinterpret_cast<const f4v*>(zp + t * kPT * kPT + (i64)c * kPT + r0);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const f4v v = src[q];
+      Rr[2 * q] = v[0];
+      Ri[2 * q] = v[1];
+      Rr[2 * q + 1] = v[2];
+      Ri[2 * q + 1] = v[3];
+    }
+  }
+  __shared__ float2 colb[kPT], rowb[kPT];
+  __shared__ float wmax[kAcaThreads / 64];
+  __shared__ int warg[kAcaThreads / 64];
+  __shared__ double wsum[kAcaThreads / 64];
+  __shared__ int s_piv;
+  __shared__ double s_sum;
+  float2* U = fac + t * 2 * kPT * (i64)rmax;
+  float2* V = U + (i64)rmax * kPT;
+  int k = 0;
+  for (;; ++k) {
+    // largest residual entry (ties: lowest index) and the residual norm
+    float best = -1.f;
+    int bq = 0;
+    double sum = 0;
+#pragma unroll
+    for (int q = 0; q < 32; ++q) {
+      const float a = Rr[q] * Rr[q] + Ri[q] * Ri[q];
+      sum += (double)a;
+      if (a > best) {
+        best = a;
+        bq = q;
+      }
+    }
+    int gidx = c * kPT + r0 + bq;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const float ob = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(gidx, o, 64);
+      if (ob > best || (ob == best && oi < gidx)) {
+        best = ob;
+        gidx = oi;
+      }
+      sum += __shfl_xor(sum, o, 64);
+    }
+    if (lane == 0) {
+      wmax[w] = best;
+      warg[w] = gidx;
+      wsum[w] = sum;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      float b = wmax[0];
+      int g = warg[0];
+      double s = wsum[0];
+      for (int q = 1; q < kAcaThreads / 64; ++q) {
+        if (wmax[q] > b || (wmax[q] == b && warg[q] < g)) {
+          b = wmax[q];
+          g = warg[q];
+        }
+        s += wsum[q];
+      }
+      s_piv = g;
+      s_sum = s;
+    }
+    __syncthreads();
+    if (s_sum <= tau2) break;  // rank k reaches the tolerance
+    if (k == rmax) {
+      k = -1;  // not compressible within rmax: stays dense
+      break;
+    }
+    const int pj = s_piv / kPT, pi = s_piv % kPT;
+    if (c == pj) {
+#pragma unroll
+      for (int q = 0; q < 32; ++q) colb[r0 + q] = make_float2(Rr[q], Ri[q]);
+    }
+    if ((pi >> 5) == (tid & 3)) {
+      const int qq = pi & 31;
+      float vr = 0.f, vi = 0.f;
+#pragma unroll
+      for (int q = 0; q < 32; ++q)
+        if (q == qq) {
+          vr = Rr[q];
+          vi = Ri[q];
+        }
+      rowb[c] = make_float2(vr, vi);
+    }
+    __syncthreads();
+    const float2 piv = colb[pi];
+    if (tid < kPT)
+      U[(i64)k * kPT + tid] = colb[tid];
+    else if (tid < 2 * kPT)
+      V[(i64)k * kPT + tid - kPT] = cdivf(rowb[tid - kPT], piv);
+    const float2 vc = cdivf(rowb[c], piv);
+#pragma unroll
+    for (int q = 0; q < 32; ++q) {
+      const float2 u = colb[r0 + q];
+      Rr[q] -= u.x * vc.x - u.y * vc.y;
+      Ri[q] -= u.x * vc.y + u.y * vc.x;
+    }
+    __syncthreads();  // colb / rowb / s_piv are rewritten in the next step
+  }
+  if (tid == 0) rank[t] = k;
+}
+
+}  // namespace
+
+namespace dml {
+
+void tile_norm2(i64 n, const float* z32p, double* out, hipStream_t s) {
+  const i64 T = packed_tiles(n);
+  DM_REQUIRE(T < (1LL << 31), DM_ERR_ARGUMENT, "matrix too large");
+  k_tile_norm2<<<(unsigned)T, 256, 0, s>>>(reinterpret_cast<const float2*>(z32p), out);
+  HIP_OK(hipGetLastError());
+}
+
+void aca_compress(i64 n, const float* z32p, double tau2, int rmax, int* rank, float* fac, hipStream_t s) {
+  const i64 nb = (n + kPT - 1) / kPT;
+  const i64 T = packed_tiles(n);
+  DM_REQUIRE(rmax >= 1 && rmax <= kPT / 2, DM_ERR_ARGUMENT, "cross approximation rank cap out of range");
+  DM_REQUIRE(T < (1LL << 31), DM_ERR_ARGUMENT, "matrix too large");
+  k_aca<<<(unsigned)T, kAcaThreads, 0, s>>>((int)nb, reinterpret_cast<const float2*>(z32p), tau2, rmax, rank,
+                                            reinterpret_cast<float2*>(fac));
+  HIP_OK(hipGetLastError());
+}
+
+}  // namespace dml

@@ -1,5 +1,7 @@
 // k_linalg.h — launchers of the Krylov / preconditioner / LU kernels.
 #pragma once
+#include <vector>
+
 #include "dm_common.h"
 
 namespace dml {
@@ -21,10 +23,31 @@ void lu_solve(i64 n, i64 ld, const double* A, const int* ipiv, double* B, i64 ld
 void unpermute(i64 n, const ZRef& z, i64 c0, i64 nc, const int* iofe, double* out, hipStream_t s);
 void entries(const ZRef& z, i64 count, const int* iofe, const i64* m, const i64* n, double* out, hipStream_t s);
 void conj(i64 n, double* v, hipStream_t s);
+// GMRES on the device: Givens step of Arnoldi column j (H: ldh x m complex,
+// column-major; cs, sn, g: rotations and rotated right-hand side; st[2j],
+// st[2j+1] = residual estimate, subdiagonal; sc[0] = 1 / subdiagonal),
+// triangular solve y = R^{-1} g of the leading j columns, y = a[0] x.
+void givens(int j, int ldh, const double* hbuf, double* H, double* cs, double* sn, double* g, double* st,
+            double* sc, hipStream_t s);
+void backsolve(int j, int ldh, const double* H, const double* g, double* y, hipStream_t s);
+void scale_dev(i64 n, const double* a, const double* x, double* y, hipStream_t s);
 // y = Z x from packed symmetric storage; ws >= symv_workspace_bytes(n)
 size_t symv_workspace_bytes(i64 n);
-// seg: tiles per workgroup (2..64; chosen by the tuner)
-void symv_packed(i64 n, const ZRef& z, const double* x, double* y, void* ws, int seg, hipStream_t s);
+// shape: 2..64 = LDS-staged form with that many tiles per workgroup segment;
+// 1000 + t (t = 1..1024) = wave form with t stored tiles per wave (chosen by
+// the tuner)
+void symv_packed(i64 n, const ZRef& z, const double* x, double* y, void* ws, int shape, hipStream_t s);
+// launch shapes worth timing for this size and precision
+std::vector<int> symv_shapes(i64 n, bool f32);
+// Compressed packed complex64 form: per-tile Frobenius norms^2, the cross
+// approximation of the off-diagonal tiles (rank[t] = -1: dense; fac: per tile
+// U then V, rmax columns of 128 complex64 each) and its symmetric product with
+// tpw stored tiles per wave (workspace: symv_workspace_bytes(n)).
+void tile_norm2(i64 n, const float* z32p, double* out, hipStream_t s);
+void aca_compress(i64 n, const float* z32p, double tau2, int rmax, int* rank, float* fac, hipStream_t s);
+void symv_hier(i64 n, const float* z32p, const int* rank, const float* fac, int rmax, const double* x, double* y,
+               void* ws, int tpw, hipStream_t s);
+std::vector<int> symv_hier_shapes(i64 n);
 size_t zx_workspace_bytes(i64 n, int R);
 void zx_mfma(i64 n, i64 ld, const float* z32, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
              hipStream_t s);

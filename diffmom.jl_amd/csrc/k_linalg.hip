@@ -476,6 +476,88 @@ __global__ __launch_bounds__(1024) void k_lu_solve(i64 n, i64 ld, const double2*
   }
 }
 
+// ------------------------------------------------ GMRES Givens on the device
+// Column j of the Hessenberg matrix from the two CGS2 passes and the norm of
+// the orthogonalised vector (hbuf = [h1 (k complex), h2 (k complex), ||w||^2],
+// k = j + 1), the previous rotations applied, a new rotation chosen so that
+// H[j+1][j] = 0, and g (the rotated right-hand side beta e_1) updated.
+// st[2j], st[2j+1] = |g[j+1]| (the residual estimate) and the subdiagonal
+// (0 = breakdown); sc[0] = 1 / subdiagonal (0 at breakdown) scales v_{j+1}.
+__global__ void k_givens(int j, int ldh, const double* __restrict__ hbuf, double2* __restrict__ H,
+                         double* __restrict__ cs, double2* __restrict__ sn, double2* __restrict__ g,
+                         double* __restrict__ st, double* __restrict__ sc) {
+  const int k = j + 1;
+  double2* Hj = H + (i64)j * ldh;
+  for (int i = threadIdx.x; i < k; i += blockDim.x)
+    Hj[i] = make_double2(hbuf[2 * i] + hbuf[2 * k + 2 * i], hbuf[2 * i + 1] + hbuf[2 * k + 2 * i + 1]);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double hn = sqrt(hbuf[4 * k]);
+  // previous rotations: [c s; -conj(s) c]
+  for (int i = 0; i < j; ++i) {
+    const double c = cs[i];
+    const double2 s = sn[i], a = Hj[i], b = Hj[i + 1];
+    // t = c a + s b ; b' = -conj(s) a + c b
+    const double2 t = make_double2(c * a.x + s.x * b.x - s.y * b.y, c * a.y + s.x * b.y + s.y * b.x);
+    Hj[i + 1] = make_double2(-(s.x * a.x + s.y * a.y) + c * b.x, -(s.x * a.y - s.y * a.x) + c * b.y);
+    Hj[i] = t;
+  }
+  const double2 a = Hj[j];
+  const double bb = hn;  // H[j+1][j] before the rotation (real, >= 0)
+  const double aa = hypot(a.x, a.y);
+  const double rho = hypot(aa, bb);
+  double c;
+  double2 s;
+  if (aa == 0) {
+    c = 0;
+    s = make_double2(1.0, 0.0);
+  } else {
+    c = aa / rho;
+    // s = (a / |a|) conj(b) / rho with b real
+    s = make_double2(a.x / aa * bb / rho, a.y / aa * bb / rho);
+  }
+  cs[j] = c;
+  sn[j] = s;
+  // H[j][j] = c a + s b
+  Hj[j] = make_double2(c * a.x + s.x * bb, c * a.y + s.y * bb);
+  Hj[k] = make_double2(0.0, 0.0);
+  const double2 gj = g[j];
+  // g[j+1] = -conj(s) g[j] ; g[j] = c g[j]
+  const double2 gk = make_double2(-(s.x * gj.x + s.y * gj.y), -(s.x * gj.y - s.y * gj.x));
+  g[k] = gk;
+  g[j] = make_double2(c * gj.x, c * gj.y);
+  st[2 * j] = hypot(gk.x, gk.y);
+  st[2 * j + 1] = hn;
+  sc[0] = hn > 0 ? 1.0 / hn : 0.0;
+}
+
+// y = R^{-1} g for the leading j x j upper triangle of the rotated H
+// (column-major, leading dimension ldh), one thread (j is small).
+__global__ void k_backsolve(int j, int ldh, const double2* __restrict__ H, const double2* __restrict__ g,
+                            double2* __restrict__ y) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int i = j - 1; i >= 0; --i) {
+    double2 acc = g[i];
+    for (int l = i + 1; l < j; ++l) {
+      const double2 h = H[(i64)l * ldh + i], yl = y[l];
+      acc.x -= h.x * yl.x - h.y * yl.y;
+      acc.y -= h.x * yl.y + h.y * yl.x;
+    }
+    const double2 d = H[(i64)i * ldh + i];
+    const double dd = d.x * d.x + d.y * d.y;
+    y[i] = make_double2((acc.x * d.x + acc.y * d.y) / dd, (acc.y * d.x - acc.x * d.y) / dd);
+  }
+}
+
+__global__ void k_scale_dev(i64 n, const double* __restrict__ a, const double2* __restrict__ x,
+                            double2* __restrict__ y) {
+  const i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double s = a[0];
+  const double2 v = x[i];
+  y[i] = make_double2(s * v.x, s * v.y);
+}
+
 inline unsigned nblocks(i64 n, int t) { return (unsigned)((n + t - 1) / t); }
 // Caller-order copy of a block of columns: out[r + (c - c0)*n] = Z[int(r) + int(c)*ld]
 __global__ void k_unpermute(i64 n, ZRef z, i64 c0, i64 nc, const int* __restrict__ iofe, double2* __restrict__ out) {
@@ -555,6 +637,25 @@ void combine(i64 n, i64 ldv, int k, const double* V, const double* y, double* x,
                                              reinterpret_cast<const double2*>(y), reinterpret_cast<double2*>(x));
   HIP_OK(hipGetLastError());
 }
+void givens(int j, int ldh, const double* hbuf, double* H, double* cs, double* sn, double* g, double* st,
+            double* sc, hipStream_t s) {
+  k_givens<<<1, 256, 0, s>>>(j, ldh, hbuf, reinterpret_cast<double2*>(H), cs, reinterpret_cast<double2*>(sn),
+                             reinterpret_cast<double2*>(g), st, sc);
+  HIP_OK(hipGetLastError());
+}
+
+void backsolve(int j, int ldh, const double* H, const double* g, double* y, hipStream_t s) {
+  k_backsolve<<<1, 64, 0, s>>>(j, ldh, reinterpret_cast<const double2*>(H), reinterpret_cast<const double2*>(g),
+                               reinterpret_cast<double2*>(y));
+  HIP_OK(hipGetLastError());
+}
+
+void scale_dev(i64 n, const double* a, const double* x, double* y, hipStream_t s) {
+  k_scale_dev<<<nblocks(n, 256), 256, 0, s>>>(n, a, reinterpret_cast<const double2*>(x),
+                                              reinterpret_cast<double2*>(y));
+  HIP_OK(hipGetLastError());
+}
+
 void norm2(i64 n, const double* w, double* out, hipStream_t s) {
   k_norm2<<<1, 1024, 0, s>>>(n, reinterpret_cast<const double2*>(w), out);
   HIP_OK(hipGetLastError());

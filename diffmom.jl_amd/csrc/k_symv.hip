@@ -14,7 +14,9 @@
 // (deterministic).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
+#include <vector>
 
 #include "dm_common.h"
 #include "k_linalg.h"
@@ -250,19 +252,445 @@ __global__ void k_symv_reduce(i64 n, int nb, int seg, const double2* __restrict_
   y[g] = make_double2(sr, si);
 }
 
+// ---------------------------------------------------------------- wave form
+// Every wave (64 lanes) takes `tpw` consecutive stored tiles in memory order
+// (crossing tile columns), so all waves carry the same number of bytes.  A
+// tile is read column by column, one coalesced 1 KB (complex64) or 2 KB
+// (complex128) column per step; lane l holds rows rows_e(l) of the column:
+//   row sums    y_I += Z_IJ x_J : per lane, in registers, x_J[c] wave-uniform;
+//   column sums y_J += Z_IJ^T x_I : per lane a partial over its two rows, then
+//               a transposing butterfly over the 64 lanes for 8 columns at a
+//               time (xor 32/16/8 halve the values each step, xor 4/2/1 finish
+//               the last one), which leaves column 8 b + (l >> 3) of batch b
+//               in lanes 8 (l >> 3) .. + 7; lane l keeps batches l & 7 and
+//               (l & 7) + 8.
+// Row sums are stored per tile.  Column sums are accumulated over the wave's
+// run of tiles in one tile column and stored at the run's last tile (the end
+// of the wave's range, or the diagonal tile, whose own column sums are not
+// added: diagonal tiles are stored in full).
+constexpr int kWaveShape = 1000;  // shape >= this: wave form with shape - 1000 tiles per wave
+
+template <int F32>
+__device__ inline int wrow(int lane, int e) {
+  return F32 ? 2 * lane + e : lane + 64 * e;
+}
+
+__device__ inline void cadd(double2& a, double2 b) {
+  a.x += b.x;
+  a.y += b.y;
+}
+
+__device__ inline double2 shfl2(double2 v, int o) {
+  return make_double2(__shfl_xor(v.x, o, 64), __shfl_xor(v.y, o, 64));
+}
+
+// Z column c of the tile (2 entries per lane, widened to FP64).
+// Transposing butterfly over the 64 lanes for 8 per-lane partial sums: after
+// xor 32 / 16 / 8 (each halving the values kept) and xor 4 / 2 / 1, p[0] of
+// lane l holds the sum over all lanes of value (l >> 3).
+__device__ inline void butterfly8(double2 (&p)[8], int lane) {
+  const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double2 keep = h5 ? p[i + 4] : p[i], send = h5 ? p[i] : p[i + 4];
+    cadd(keep, shfl2(send, 32));
+    p[i] = keep;
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    double2 keep = h4 ? p[i + 2] : p[i], send = h4 ? p[i] : p[i + 2];
+    cadd(keep, shfl2(send, 16));
+    p[i] = keep;
+  }
+  {
+    double2 keep = h3 ? p[1] : p[0], send = h3 ? p[0] : p[1];
+    cadd(keep, shfl2(send, 8));
+    p[0] = keep;
+  }
+  cadd(p[0], shfl2(p[0], 4));
+  cadd(p[0], shfl2(p[0], 2));
+  cadd(p[0], shfl2(p[0], 1));
+}
+
+// The same for 4 values: p[0] of lane l holds the sum of value (l >> 4).
+__device__ inline void butterfly4(double2 (&p)[4], int lane) {
+  const bool h5 = lane & 32, h4 = lane & 16;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    double2 keep = h5 ? p[i + 2] : p[i], send = h5 ? p[i] : p[i + 2];
+    cadd(keep, shfl2(send, 32));
+    p[i] = keep;
+  }
+  {
+    double2 keep = h4 ? p[1] : p[0], send = h4 ? p[0] : p[1];
+    cadd(keep, shfl2(send, 16));
+    p[0] = keep;
+  }
+  cadd(p[0], shfl2(p[0], 8));
+  cadd(p[0], shfl2(p[0], 4));
+  cadd(p[0], shfl2(p[0], 2));
+  cadd(p[0], shfl2(p[0], 1));
+}
+
+template <int F32>
+__device__ inline void col_load(const f4v* tile, int c, int lane, f4v (&v)[2]) {
+  if constexpr (F32 != 0) {
+    v[0] = __builtin_nontemporal_load(tile + c * (kPT / 2) + lane);
+  } else {
+    v[0] = __builtin_nontemporal_load(tile + c * kPT + lane);
+    v[1] = __builtin_nontemporal_load(tile + c * kPT + 64 + lane);
+  }
+}
+
+template <int F32>
+__device__ inline void col_entries(const f4v (&v)[2], double2 (&z)[2]) {
+  if constexpr (F32 != 0) {
+    z[0] = make_double2((double)v[0][0], (double)v[0][1]);
+    z[1] = make_double2((double)v[0][2], (double)v[0][3]);
+  } else {
+    __builtin_memcpy(&z[0], &v[0], sizeof(double2));
+    __builtin_memcpy(&z[1], &v[1], sizeof(double2));
+  }
+}
+
+// x_J[col] from the lane that holds it (wave-uniform col)
+__device__ inline double rl(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int F32>
+__device__ inline double2 bcast(const double2 (&xJ)[2], int col) {
+  // row col is entry e of lane l with wrow(l, e) = col
+  const int e = F32 ? (col & 1) : (col >> 6);
+  const int l = F32 ? (col >> 1) : (col & 63);
+  const double2 v = e ? xJ[1] : xJ[0];
+  return make_double2(rl(v.x, l), rl(v.y, l));
+}
+
+// One dense stored tile: accR (this lane's rows) += Z x_J; for an
+// off-diagonal tile also the column partials Z^T x_I, reduced by the
+// butterfly into accC0 / accC1 (column 8 b + (lane >> 3) of batch b = lane & 7
+// and (lane & 7) + 8).
+template <int F32>
+__device__ inline void dense_tile(const f4v* tile, int lane, bool offdiag, const double2 (&xI)[2],
+                                  const double2 (&xJ)[2], double2 (&accR)[2], double2& accC0, double2& accC1) {
+  auto batch = [&](int b, f4v(&v)[8][2]) {
+    double2 p[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      double2 z[2];
+      col_entries<F32>(v[c], z);
+      const int col = 8 * b + c;
+      const double2 xc = bcast<F32>(xJ, col);
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        accR[e].x += z[e].x * xc.x - z[e].y * xc.y;
+        accR[e].y += z[e].x * xc.y + z[e].y * xc.x;
+      }
+      p[c].x = z[0].x * xI[0].x - z[0].y * xI[0].y + z[1].x * xI[1].x - z[1].y * xI[1].y;
+      p[c].y = z[0].x * xI[0].y + z[0].y * xI[0].x + z[1].x * xI[1].y + z[1].y * xI[1].x;
+    }
+    if (offdiag) {
+      butterfly8(p, lane);
+      if ((b & 7) == (lane & 7)) {
+        if (b < 8)
+          cadd(accC0, p[0]);
+        else
+          cadd(accC1, p[0]);
+      }
+    }
+  };
+#pragma unroll 1
+  for (int b = 0; b < kPT / 8; ++b) {
+    f4v v[8][2];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) col_load<F32>(tile, 8 * b + c, lane, v[c]);
+    batch(b, v);
+  }
+}
+
+// tile t0 = J (J + 1) / 2 + I
+__device__ inline void tile_ij(i64 t0, int& I, int& J) {
+  J = (int)((sqrt(8.0 * (double)t0 + 1.0) - 1.0) * 0.5);
+  while ((i64)J * (J + 1) / 2 > t0) --J;
+  while ((i64)(J + 1) * (J + 2) / 2 <= t0) ++J;
+  I = (int)(t0 - (i64)J * (J + 1) / 2);
+}
+
+template <int F32>
+__global__ __launch_bounds__(256) void k_symv_w(i64 n, int nb, int tpw, const void* __restrict__ zp,
+                                                const double2* __restrict__ x, double2* __restrict__ rowpart,
+                                                double2* __restrict__ colpart) {
+  using ET = typename SymvCfg<F32>::ET;
+  constexpr i64 kTileF4 = (i64)kPT * kPT * sizeof(ET) / 16;
+  const int lane = threadIdx.x & 63;
+  const i64 wave = (i64)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const i64 T = (i64)nb * (nb + 1) / 2;
+  const i64 t0 = wave * tpw;
+  if (t0 >= T) return;
+  const i64 t1 = min(t0 + (i64)tpw, T);
+  int I, J;
+  tile_ij(t0, I, J);
+  double2 accC0 = make_double2(0.0, 0.0), accC1 = make_double2(0.0, 0.0);
+  for (i64 t = t0; t < t1; ++t) {
+    const f4v* tile = reinterpret_cast<const f4v*>(zp) + t * kTileF4;
+    // x_I and x_J for this lane's rows; x_J[c] is broadcast with readlane
+    double2 xI[2], xJ[2], accR[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const i64 gi = (i64)I * kPT + wrow<F32>(lane, e), gj = (i64)J * kPT + wrow<F32>(lane, e);
+      xI[e] = gi < n ? x[gi] : make_double2(0.0, 0.0);
+      xJ[e] = gj < n ? x[gj] : make_double2(0.0, 0.0);
+    }
+    dense_tile<F32>(tile, lane, I != J, xI, xJ, accR, accC0, accC1);
+#pragma unroll
+    for (int e = 0; e < 2; ++e) rowpart[t * kPT + wrow<F32>(lane, e)] = accR[e];
+    // end of this column run: store the accumulated column sums at tile t
+    if (I == J || t + 1 == t1) {
+      colpart[t * kPT + 8 * (lane & 7) + (lane >> 3)] = accC0;
+      colpart[t * kPT + 64 + 8 * (lane & 7) + (lane >> 3)] = accC1;
+      accC0 = accC1 = make_double2(0.0, 0.0);
+    }
+    if (I == J) {
+      ++J;
+      I = 0;
+    } else {
+      ++I;
+    }
+  }
+}
+
+// ------------------------------------------------ compressed (hierarchical) form
+// Packed complex64 storage in which off-diagonal tiles may be replaced by a
+// cross approximation Z_IJ ~ U V^T (rank[t] >= 0; U, V: 128 x rank, slot t of
+// fac holds U then V, each rmax columns of 128 complex64).  Dense tiles
+// (rank[t] < 0, always the diagonal ones) are read from the packed matrix.
+// For a low-rank tile, with s = U^T x_I and r = V^T x_J:
+//   y_I += U r,  y_J += V s,
+// the factor columns taken 4 at a time (one coalesced 1 KB read each), the
+// 4 + 4 dot products reduced by the butterfly and broadcast with readlane.
+// Column sums are kept in this lane's row layout (columns 2 lane, 2 lane + 1);
+// the dense tiles' butterfly-layout sums are moved there with one gather at
+// each flush.
+__device__ inline double2 cfma(double2 acc, double2 a, double2 b) {
+  acc.x += a.x * b.x - a.y * b.y;
+  acc.y += a.x * b.y + a.y * b.x;
+  return acc;
+}
+
+__device__ inline double2 bcast_lane(double2 v, int l) { return make_double2(rl(v.x, l), rl(v.y, l)); }
+
+__global__ __launch_bounds__(256, 3) void k_symv_h(i64 n, int nb, int tpw, const void* __restrict__ zp,
+                                                const int* __restrict__ rank, const float2* __restrict__ fac,
+                                                int rmax, const double2* __restrict__ x,
+                                                double2* __restrict__ rowpart, double2* __restrict__ colpart) {
+  constexpr i64 kTileF4 = (i64)kPT * kPT * sizeof(float2) / 16;
+  const int lane = threadIdx.x & 63;
+  const i64 wave = (i64)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const i64 T = (i64)nb * (nb + 1) / 2;
+  const i64 t0 = wave * tpw;
+  if (t0 >= T) return;
+  const i64 t1 = min(t0 + (i64)tpw, T);
+  int I, J;
+  tile_ij(t0, I, J);
+  double2 accC0 = make_double2(0.0, 0.0), accC1 = make_double2(0.0, 0.0);
+  double2 accJ[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+  for (i64 t = t0; t < t1; ++t) {
+    double2 xI[2], xJ[2], accR[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const i64 gi = (i64)I * kPT + 2 * lane + e, gj = (i64)J * kPT + 2 * lane + e;
+      xI[e] = gi < n ? x[gi] : make_double2(0.0, 0.0);
+      xJ[e] = gj < n ? x[gj] : make_double2(0.0, 0.0);
+    }
+    const int rk = rank[t];
+    if (rk < 0) {
+      dense_tile<1>(reinterpret_cast<const f4v*>(zp) + t * kTileF4, lane, I != J, xI, xJ, accR, accC0, accC1);
+    } else {
+      const f4v* U = reinterpret_cast<const f4v*>(fac + t * 2 * kPT * (i64)rmax);
+      const f4v* V = U + (i64)rmax * kPT / 2;
+#pragma unroll 1
+      for (int k0 = 0; k0 < rk; k0 += 4) {
+        f4v u[4], v[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          if (k0 + c < rk) {
+            u[c] = __builtin_nontemporal_load(U + (k0 + c) * (kPT / 2) + lane);
+            v[c] = __builtin_nontemporal_load(V + (k0 + c) * (kPT / 2) + lane);
+          } else {
+            u[c] = f4v{0.f, 0.f, 0.f, 0.f};
+            v[c] = f4v{0.f, 0.f, 0.f, 0.f};
+          }
+        }
+        double2 ps[4], pr[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const double2 u0 = make_double2(u[c][0], u[c][1]), u1 = make_double2(u[c][2], u[c][3]);
+          const double2 v0 = make_double2(v[c][0], v[c][1]), v1 = make_double2(v[c][2], v[c][3]);
+          ps[c] = cfma(cfma(make_double2(0.0, 0.0), u0, xI[0]), u1, xI[1]);
+          pr[c] = cfma(cfma(make_double2(0.0, 0.0), v0, xJ[0]), v1, xJ[1]);
+        }
+        butterfly4(ps, lane);
+        butterfly4(pr, lane);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const double2 sc = bcast_lane(ps[0], 16 * c), rc = bcast_lane(pr[0], 16 * c);
+          accR[0] = cfma(accR[0], make_double2(u[c][0], u[c][1]), rc);
+          accR[1] = cfma(accR[1], make_double2(u[c][2], u[c][3]), rc);
+          accJ[0] = cfma(accJ[0], make_double2(v[c][0], v[c][1]), sc);
+          accJ[1] = cfma(accJ[1], make_double2(v[c][2], v[c][3]), sc);
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 2; ++e) rowpart[t * kPT + 2 * lane + e] = accR[e];
+    if (I == J || t + 1 == t1) {
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int c = 2 * lane + e, cm = c & 63;
+        const int src = ((cm & 7) << 3) | (cm >> 3);
+        const double2 g0 = make_double2(__shfl(accC0.x, src, 64), __shfl(accC0.y, src, 64));
+        const double2 g1 = make_double2(__shfl(accC1.x, src, 64), __shfl(accC1.y, src, 64));
+        double2 v = c < 64 ? g0 : g1;
+        cadd(v, accJ[e]);
+        colpart[t * kPT + c] = v;
+      }
+      accC0 = accC1 = make_double2(0.0, 0.0);
+      accJ[0] = accJ[1] = make_double2(0.0, 0.0);
+    }
+    if (I == J) {
+      ++J;
+      I = 0;
+    } else {
+      ++I;
+    }
+  }
+}
+
+// y[R kPT + r] = sum_{J >= R} rowpart[tile (R, J)][r]
+//              + sum over the column runs of tile column R of their colpart
+// (a run ends at tile t = R (R + 1) / 2 + I when t + 1 is a multiple of tpw
+// or I = R); fixed order, deterministic.
+__global__ void k_symv_w_reduce(i64 n, int nb, int tpw, const double2* __restrict__ rowpart,
+                                const double2* __restrict__ colpart, double2* __restrict__ y) {
+  const i64 g = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const int R = (int)(g / kPT), r = (int)(g % kPT);
+  double sr = 0, si = 0;
+  int J = R;
+  for (; J + 8 <= nb; J += 8) {
+    double2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = rowpart[((i64)(J + u) * (J + u + 1) / 2 + R) * kPT + r];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      sr += v[u].x;
+      si += v[u].y;
+    }
+  }
+  for (; J < nb; ++J) {
+    double2 v = rowpart[((i64)J * (J + 1) / 2 + R) * kPT + r];
+    sr += v.x;
+    si += v.y;
+  }
+  const i64 c0 = (i64)R * (R + 1) / 2;
+  for (int I = 0; I <= R; ++I) {
+    const i64 t = c0 + I;
+    if (I == R || (t + 1) % tpw == 0) {
+      double2 v = colpart[t * kPT + r];
+      sr += v.x;
+      si += v.y;
+    }
+  }
+  y[g] = make_double2(sr, si);
+}
+
 }  // namespace
 
 namespace dml {
 
-size_t symv_workspace_bytes(i64 n) {
+void symv_hier(i64 n, const float* z32p, const int* rank, const float* fac, int rmax, const double* x, double* y,
+               void* ws, int tpw, hipStream_t s) {
+  DM_REQUIRE(tpw >= 1 && tpw <= 1024, DM_ERR_ARGUMENT, "tiles per wave out of range");
   const i64 nb = (n + kPT - 1) / kPT;
-  return (size_t)(packed_tiles(n) + seg_base(nb, kSegMin)) * kPT * sizeof(double2);
+  const i64 T = packed_tiles(n);
+  const i64 blocks = ((T + tpw - 1) / tpw + 3) / 4;
+  DM_REQUIRE(blocks < (1LL << 31), DM_ERR_ARGUMENT, "matrix too large for the packed product");
+  double2* rowpart = reinterpret_cast<double2*>(ws);
+  double2* colpart = rowpart + T * kPT;
+  k_symv_h<<<(unsigned)blocks, 256, 0, s>>>(n, (int)nb, tpw, z32p, rank, reinterpret_cast<const float2*>(fac), rmax,
+                                            reinterpret_cast<const double2*>(x), rowpart, colpart);
+  k_symv_w_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, (int)nb, tpw, rowpart, colpart,
+                                                              reinterpret_cast<double2*>(y));
+  HIP_OK(hipGetLastError());
 }
 
-void symv_packed(i64 n, const ZRef& z, const double* x, double* y, void* ws, int seg, hipStream_t s) {
-  DM_REQUIRE(z.packed, DM_ERR_ARGUMENT, "symmetric product needs packed storage");
-  DM_REQUIRE(seg >= kSegMin && seg <= 64, DM_ERR_ARGUMENT, "segment length out of range");
+std::vector<int> symv_hier_shapes(i64 n) {
+  std::vector<int> c = {1, 2, 4};
+  int dev = 0, ncu = 0, per_cu = 0;
+  HIP_OK(hipGetDevice(&dev));
+  HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_symv_h, 256, 0));
+  const i64 slots = (i64)std::max(per_cu, 1) * ncu * 4;
+  const i64 T = packed_tiles(n);
+  for (int k = 1; k <= 4; k *= 2) {
+    const i64 t = (T + k * slots - 1) / (k * slots);
+    if (t > 4 && t <= 1024) c.push_back((int)t);
+  }
+  return c;
+}
+
+std::vector<int> symv_shapes(i64 n, bool f32) {
+  std::vector<int> c = {4, 8, 16, kWaveShape + 1, kWaveShape + 2, kWaveShape + 4};
+  // wave form sized so that every resident wave slot gets one or two equal ranges
+  int dev = 0, ncu = 0, per_cu = 0;
+  HIP_OK(hipGetDevice(&dev));
+  HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const void* fn = f32 ? (const void*)k_symv_w<1> : (const void*)k_symv_w<0>;
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0));
+  const i64 slots = (i64)std::max(per_cu, 1) * ncu * 4;
+  const i64 T = packed_tiles(n);
+  for (int k = 1; k <= 2; ++k) {
+    const i64 t = (T + k * slots - 1) / (k * slots);
+    if (t > 4 && t <= 1024) c.push_back(kWaveShape + (int)t);
+  }
+  return c;
+}
+
+size_t symv_workspace_bytes(i64 n) {
   const i64 nb = (n + kPT - 1) / kPT;
+  const i64 lds_form = packed_tiles(n) + seg_base(nb, kSegMin);
+  const i64 wave_form = 2 * packed_tiles(n);
+  return (size_t)std::max(lds_form, wave_form) * kPT * sizeof(double2);
+}
+
+void symv_packed(i64 n, const ZRef& z, const double* x, double* y, void* ws, int shape, hipStream_t s) {
+  DM_REQUIRE(z.packed, DM_ERR_ARGUMENT, "symmetric product needs packed storage");
+  const i64 nb = (n + kPT - 1) / kPT;
+  if (shape >= kWaveShape) {
+    const int tpw = shape - kWaveShape;
+    DM_REQUIRE(tpw >= 1 && tpw <= 1024, DM_ERR_ARGUMENT, "tiles per wave out of range");
+    const i64 T = packed_tiles(n);
+    const i64 waves = (T + tpw - 1) / tpw;
+    const i64 blocks = (waves + 3) / 4;
+    DM_REQUIRE(blocks < (1LL << 31), DM_ERR_ARGUMENT, "matrix too large for the packed product");
+    double2* rowpart = reinterpret_cast<double2*>(ws);
+    double2* colpart = rowpart + T * kPT;
+    const double2* xv = reinterpret_cast<const double2*>(x);
+    if (z.f32)
+      k_symv_w<1><<<(unsigned)blocks, 256, 0, s>>>(n, (int)nb, tpw, z.p, xv, rowpart, colpart);
+    else
+      k_symv_w<0><<<(unsigned)blocks, 256, 0, s>>>(n, (int)nb, tpw, z.p, xv, rowpart, colpart);
+    k_symv_w_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, (int)nb, tpw, rowpart, colpart,
+                                                                reinterpret_cast<double2*>(y));
+    HIP_OK(hipGetLastError());
+    return;
+  }
+  const int seg = shape;
+  DM_REQUIRE(seg >= kSegMin && seg <= 64, DM_ERR_ARGUMENT, "segment length out of range");
   const i64 nseg = seg_base(nb, seg);
   DM_REQUIRE(nseg < (1LL << 31), DM_ERR_ARGUMENT, "matrix too large for the packed product");
   double2* rowpart = reinterpret_cast<double2*>(ws);

@@ -49,6 +49,9 @@ extern "C" {
  * products stream each stored tile once and use it for Z x and Z^T x. */
 #define DM_Z64P 4
 #define DM_Z32P 8
+/* complex64 packed storage whose off-diagonal tiles may be cross-approximated
+ * (dm_zmat_compress); usable for products and solves, not for entry reads */
+#define DM_Z32H 16
 
 typedef struct dm_ctx dm_ctx;
 typedef struct dm_mesh dm_mesh;
@@ -183,6 +186,10 @@ typedef struct dm_solve_opts {
   int adjoint;         /* dm_solve: 1 solves Z^H X = B (solve_gmres_adjoint,
                           src/solver/IterativeSolve.jl:409-460); Z = Z^T, so Z^H x = b
                           is solved as Z conj(x) = conj(b) */
+  double compress_tol; /* MIXED, dm_solve_scattering_pw, one right-hand side at a time:
+                          > 0 compresses the complex64 matrix (dm_zmat_compress with
+                          this eps) for the inner solve; the refinement residual
+                          stays the dense complex128 product.  0 = off */
 } dm_solve_opts;
 
 typedef struct dm_solve_stats {
@@ -210,6 +217,18 @@ int dm_precond_free(dm_precond* p);
 /* Solve Z X = B for nrhs right-hand sides (host buffers, caller order). */
 int dm_solve(dm_ctx* ctx, const dm_zmat* z, const double* B, int64_t ldb, int64_t nrhs,
              const dm_solve_opts* opts, double* X, int64_t ldx, dm_solve_stats* stats);
+
+/* ------------------------------------------------------------ compression */
+/* No reference counterpart as an entry point; the reference's ACA
+ * (src/fast/ACA.jl:558-688) compresses far blocks from entries computed on
+ * demand.  Here the assembled packed complex64 form (DM_Z32P, required) is
+ * compressed tile by tile: every off-diagonal 128 x 128 tile whose cross
+ * approximation reaches the per-tile bound within rmax terms (1..64) is stored
+ * as U V^T, so that ||Z32P - Z32H||_F <= eps ||Z32P||_F (plus the FP32
+ * rounding of the residual updates, a few 1e-8 ||Z||_F).  Adds DM_Z32H to the
+ * forms.  stats (may be NULL): [0] stored tiles, [1] low-rank tiles, [2] mean
+ * rank of the low-rank tiles, [3] bytes a product reads relative to DM_Z32P. */
+int dm_zmat_compress(dm_zmat* z, double eps, int rmax, double* stats);
 
 /* ------------------------------------------------------------------ tuning */
 /* No reference counterpart.  Launch shapes (the symmetric product's tiles per

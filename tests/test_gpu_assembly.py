@@ -256,28 +256,89 @@ def test_multi_mesh_persistent_assembly(dm, O):
             assert rel_fro(A, _oracle_Z(O, m, r, K0)) <= 1e-10
 
 
-def test_tuned_symmetric_product_segments(dm):
-    """The tuner times the symmetric product's tiles-per-workgroup on first use
-    and records the choice; every allowed choice gives the same product (FP64
-    sums in another order), and choices loaded with tune_set are used as is."""
-    m = dm.make_icosphere(0.1, 3)
+@pytest.mark.parametrize("mk", ["plate", "sphere"])
+def test_tuned_symmetric_product_shapes(dm, mk):
+    """The tuner times the symmetric product's launch shapes on first use and
+    records the choice.  Every allowed shape gives the same product (FP64 sums
+    in another order): the LDS-staged form with 2..64 tiles per workgroup and
+    the wave form (1000 + tiles per wave, runs of tiles crossing tile columns);
+    shapes loaded with tune_set are used as is, out-of-range ones are refused."""
+    m = dm.make_rect_plate(0.3, 0.25, 30, 22) if mk == "plate" else dm.make_icosphere(0.1, 3)
     r = dm.build_rwg(m)
     Zd = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64P | dm.DM_Z32P, device=True)
-    P64 = Zd.to_numpy(dm.DM_Z64P)
     rng = np.random.default_rng(5)
     X = rng.standard_normal((r.nedges, 2)) + 1j * rng.standard_normal((r.nedges, 2))
-    ref = P64 @ X
     ctx = dm.default_context()
-    Zd.matmul(X, form=dm.DM_Z64P)
-    key = f"symv/c128/{r.nedges}"
-    got = ctx.tune_get()
-    assert got[key] in (2, 4, 8, 16, 32)
-    for seg in (2, 3, 16, 64):
-        ctx.tune_set({key: seg})
-        assert ctx.tune_get()[key] == seg
-        Y = Zd.matmul(X, form=dm.DM_Z64P)
-        assert np.linalg.norm(Y - ref) / np.linalg.norm(ref) < 1e-13, seg
-    ctx.tune_set({key: 1})  # outside the kernel's range: rejected at launch
+    for form, tag in ((dm.DM_Z64P, "c128"), (dm.DM_Z32P, "c64")):
+        P = Zd.to_numpy(form).astype(np.complex128)
+        ref = P @ X
+        Zd.matmul(X, form=form)
+        key = f"symv/{tag}/{r.nedges}"
+        got = ctx.tune_get()
+        assert 2 <= got[key] <= 64 or 1001 <= got[key] <= 2024
+        for shape in (2, 3, 16, 64, 1001, 1002, 1003, 1007, 1100):
+            ctx.tune_set({key: shape})
+            assert ctx.tune_get()[key] == shape
+            Y = Zd.matmul(X, form=form)
+            assert np.linalg.norm(Y - ref) / np.linalg.norm(ref) < 1e-13, (tag, shape)
+        for bad in (1, 65, 1000, 2025):
+            ctx.tune_set({key: bad})
+            with pytest.raises(Exception):
+                Zd.matmul(X, form=form)
+        ctx.tune_set(got)
+
+
+@pytest.mark.parametrize("mk", ["plate", "sphere"])
+def test_compressed_form_error_bound(dm, mk):
+    """dm_zmat_compress: off-diagonal stored tiles become cross approximations
+    U V^T with ||Z32P - Z32H||_F <= eps ||Z32P||_F up to FP32 rounding of the
+    updates (checked on the whole matrix, recovered column by column through
+    the compressed product), the diagonal tiles stay dense, and the product
+    matches the dense product of the same complex64 entries to that bound."""
+    m = dm.make_rect_plate(0.3, 0.25, 30, 22) if mk == "plate" else dm.make_icosphere(0.1, 3)
+    r = dm.build_rwg(m)
+    N = r.nedges
+    Zd = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64P | dm.DM_Z32P, device=True)
+    P = Zd.to_numpy(dm.DM_Z32P).astype(np.complex128)
+    for eps in (1e-3, 1e-6):
+        st = Zd.compress(eps=eps, rmax=48)
+        nb = -(-N // 128)
+        assert st["tiles"] == nb * (nb + 1) // 2
+        assert 0 < st["lowrank_tiles"] <= st["tiles"] - nb
+        assert 0 < st["bytes_ratio"] < 1
+        H = Zd.matmul(np.eye(N, dtype=np.complex128), form=dm.DM_Z32H)
+        err = np.linalg.norm(H - P) / np.linalg.norm(P)
+        print(f"{mk} N={N} eps {eps}: {st}, error {err:.2e}")
+        # the bound holds for the tracked FP32 residual; rounding of the
+        # FP32 updates adds a few 1e-8 ||Z||_F at most
+        assert err <= eps + 1e-7
+        assert err > 0 or st["lowrank_tiles"] == 0
+        # Z_IJ ~ U V^T is used as Z_JI ~ V U^T: the product stays symmetric
+        assert np.linalg.norm(H - H.T) / np.linalg.norm(P) <= 1e-12
+    x = np.random.default_rng(2).standard_normal(N) + 0j
+    assert np.linalg.norm(Zd.matmul(x, form=dm.DM_Z32H) - P @ x) / np.linalg.norm(P @ x) < 1e-5
     with pytest.raises(Exception):
-        Zd.matmul(X, form=dm.DM_Z64P)
+        Zd.compress(eps=0.0)
+    with pytest.raises(Exception):
+        Zd.compress(eps=1e-6, rmax=65)
+
+
+def test_compressed_product_shapes(dm):
+    """Every tiles-per-wave shape of the compressed product gives the same
+    result (FP64 sums in another order)."""
+    m = dm.make_icosphere(0.1, 3)
+    r = dm.build_rwg(m)
+    Zd = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z32P, device=True)
+    Zd.compress(eps=1e-6)
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(r.nedges) + 1j * rng.standard_normal(r.nedges)
+    ctx = dm.default_context()
+    ref = Zd.matmul(x, form=dm.DM_Z32H)
+    key = f"symvh/c64/{r.nedges}"
+    got = ctx.tune_get()
+    assert 1 <= got[key] <= 1024
+    for tpw in (1, 2, 3, 7, 40):
+        ctx.tune_set({key: tpw})
+        y = Zd.matmul(x, form=dm.DM_Z32H)
+        assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-13, tpw
     ctx.tune_set(got)

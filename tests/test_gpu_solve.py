@@ -349,3 +349,32 @@ def test_adjoint_solve(dm, c0):
     assert np.linalg.norm(Zh.conj().T @ X[:, 2] - v.conj()) / np.linalg.norm(v) < 1e-9
     xl = dm._solve(Z, v, dm.DM_METHOD_LU, adjoint=True)[0]
     assert np.linalg.norm(xl - xd) / np.linalg.norm(xd) < 1e-10
+
+
+def test_mixed_solve_on_compressed_matrix(dm):
+    """Mixed-precision solve with the compressed complex64 matrix for the inner
+    products: true FP64 residual <= 1e-6 against the dense complex128 matrix,
+    currents within 1e-5 of the uncompressed mixed solve; a loose compression
+    (1e-3) still converges through extra refinement steps."""
+    m = dm.make_icosphere(0.1, 3)
+    r = dm.build_rwg(m)
+    pws = _random_planewaves(dm, np.random.default_rng(12), K0, 2)
+    P = dm.build_nearfield_preconditioner(None, m, r, 2.0 * float(np.mean(r.len)))
+    X0, s0, _ = dm.solve_planewaves(m, r, K0, pws, "mixed", P, tol=1e-6, maxiter=300, batch=-1)
+    X1, s1, _, Z = dm.solve_planewaves(m, r, K0, pws, "mixed", P, tol=1e-6, maxiter=300, batch=-1,
+                                       compress_tol=5e-8, keep_matrix=True)
+    assert Z.forms & dm.DM_Z32H
+    assert s1.solved and s1.residual <= 1e-6
+    Z64 = Z.to_numpy(dm.DM_Z64P)
+    V = dm.assemble_multiple_excitations(m, r, pws)
+    res = np.linalg.norm(Z64 @ X1 - V, axis=0) / np.linalg.norm(V, axis=0)
+    assert res.max() <= 1e-6 * 1.001
+    assert np.linalg.norm(X1 - X0) / np.linalg.norm(X0) < 1e-5
+    X2, s2, _ = dm.solve_planewaves(m, r, K0, pws, "mixed", P, tol=1e-6, maxiter=300, batch=-1, compress_tol=1e-3)
+    assert s2.solved and s2.residual <= 1e-6 and s2.outer >= s1.outer
+    # dm_solve with a compressed matrix: the mixed method uses it
+    Zc = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64P | dm.DM_Z32P, device=True)
+    Zc.compress(eps=5e-8)
+    x, st = dm.solve_mixed(Zc, V[:, 0], preconditioner=P, tol=1e-7)
+    assert st.solved
+    assert np.linalg.norm(Z64 @ x - V[:, 0]) / np.linalg.norm(V[:, 0]) <= 1e-7 * 1.001
