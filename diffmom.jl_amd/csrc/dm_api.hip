@@ -210,27 +210,26 @@ int op_form(const dm_zmat* z, bool batched) {
 
 constexpr int kAcaRankCap = 48;
 
-// Cross approximation of the packed complex64 form (see dm_zmat_compress).
-void compress_z32p(dm_zmat* z, double eps, int rmax, double* stats, hipStream_t s) {
+// Cross approximation of the packed complex64 form (see dm_zmat_compress):
+// queued on stream s without a host round trip; compression_stats reads the
+// ranks back.
+void compress_z32p(dm_zmat* z, double eps, int rmax, hipStream_t s) {
   DM_REQUIRE(z->forms & DM_Z32P, DM_ERR_ARGUMENT, "compression needs the packed complex64 form (DM_Z32P)");
-  DM_REQUIRE(eps > 0 && std::isfinite(eps), DM_ERR_ARGUMENT, "compression tolerance must be positive");
+  DM_REQUIRE(eps > 0 && eps < 1, DM_ERR_ARGUMENT, "compression tolerance must be in (0, 1)");
   DM_REQUIRE(rmax >= 1 && rmax <= 64, DM_ERR_ARGUMENT, "rank cap must be in 1..64");
-  const i64 n = z->n, nb = (n + kPT - 1) / kPT, T = packed_tiles(n);
-  DBuf<double> nrm;
-  nrm.alloc(T);
-  dml::tile_norm2(n, z->z32p.p, nrm.p, s);
-  std::vector<double> h(T);
-  HIP_OK(hipMemcpyAsync(h.data(), nrm.p, sizeof(double) * T, hipMemcpyDeviceToHost, s));
-  HIP_OK(hipStreamSynchronize(s));
-  double fro2 = 0;
-  for (i64 J = 0; J < nb; ++J)
-    for (i64 I = 0; I <= J; ++I) fro2 += (I == J ? 1.0 : 2.0) * h[J * (J + 1) / 2 + I];
-  const i64 toff = T - nb;
-  const double tau2 = toff > 0 ? eps * eps * fro2 / (2.0 * (double)toff) : 0.0;
+  const i64 n = z->n, T = packed_tiles(n);
+  z->hwork.alloc(T + 1);
   z->hrank.alloc(T);
   z->hfac.alloc((size_t)T * 2 * kPT * rmax * 2);
   z->hrmax = rmax;
-  dml::aca_compress(n, z->z32p.p, tau2, rmax, z->hrank.p, z->hfac.p, s);
+  dml::aca_compress(n, z->z32p.p, eps, rmax, z->hwork.p, z->hrank.p, z->hfac.p, s);
+  z->forms |= DM_Z32H;
+}
+
+// [0] stored tiles, [1] low-rank tiles, [2] their mean rank, [3] bytes a
+// product reads relative to the dense packed form.
+void compression_stats(dm_zmat* z, double* stats, hipStream_t s) {
+  const i64 T = packed_tiles(z->n);
   std::vector<int> rk(T);
   HIP_OK(hipMemcpyAsync(rk.data(), z->hrank.p, sizeof(int) * T, hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
@@ -246,13 +245,10 @@ void compress_z32p(dm_zmat* z, double eps, int rmax, double* stats, hipStream_t 
     }
   }
   z->hbytes = bytes;
-  z->forms |= DM_Z32H;
-  if (stats) {
-    stats[0] = (double)T;
-    stats[1] = (double)low;
-    stats[2] = low ? rsum / low : 0.0;
-    stats[3] = bytes / ((double)T * kPT * kPT * 8);
-  }
+  stats[0] = (double)T;
+  stats[1] = (double)low;
+  stats[2] = low ? rsum / low : 0.0;
+  stats[3] = bytes / ((double)T * kPT * kPT * 8);
 }
 
 // Solve the nrhs columns of Bd (device, internal basis order, n x nrhs) and
@@ -700,7 +696,10 @@ int dm_zmat_compress(dm_zmat* z, double eps, int rmax, double* stats) {
   return guard([&] {
     check_ctx(z, "zmat");
     CtxScope sc(z->ctx);
-    compress_z32p(z, eps, rmax, stats, z->ctx->stream);
+    compress_z32p(z, eps, rmax, z->ctx->stream);
+    double st[4];
+    compression_stats(z, st, z->ctx->stream);
+    if (stats) std::copy(st, st + 4, stats);
   });
 }
 
@@ -1043,9 +1042,28 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
       ProfScope p(ctx, "assemble_z64", s);
       launch_assemble_z64(a, z->z64.p, r.max_blk_tris, s);
     }
-    if (o.method == DM_METHOD_MIXED && !batched && o.compress_tol > 0) {
-      ProfScope pc(ctx, "compress", s);
-      compress_z32p(z.get(), o.compress_tol, kAcaRankCap, nullptr, s);
+    // compression of the complex64 matrix on the second stream, overlapped
+    // with the preconditioner build (both only read the packed matrix)
+    hipEvent_t e_asm = nullptr, e_h = nullptr;
+    struct EvGuard {
+      hipEvent_t& a;
+      hipEvent_t& b;
+      ~EvGuard() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+      }
+    } evg{e_asm, e_h};
+    const bool compress = o.method == DM_METHOD_MIXED && !batched && o.compress_tol > 0;
+    if (compress) {
+      HIP_OK(hipEventCreateWithFlags(&e_asm, hipEventDisableTiming));
+      HIP_OK(hipEventCreateWithFlags(&e_h, hipEventDisableTiming));
+      HIP_OK(hipEventRecord(e_asm, s));
+      HIP_OK(hipStreamWaitEvent(ctx->stream2, e_asm, 0));
+      {
+        ProfScope pc(ctx, "compress", ctx->stream2);
+        compress_z32p(z.get(), o.compress_tol, kAcaRankCap, ctx->stream2);
+      }
+      HIP_OK(hipEventRecord(e_h, ctx->stream2));
     }
     DBuf<double> dk, dp, de, V, x, xe;
     dk.upload(kvec, 3 * nrhs, s);
@@ -1092,6 +1110,7 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
       auto M = build_precond(z.get(), pick_form(z.get(), o.method == DM_METHOD_MIXED, batched), o, s);
       HIP_OK(hipStreamSynchronize(s));
       st.precond_time_s = now() - t1;
+      if (compress) HIP_OK(hipStreamWaitEvent(s, e_h, 0));
       solve_columns(ctx, o, r, A64.get(), A32.get(), M.get(), V.p, n, nrhs, X, n, st, worst, all_ok, nullptr);
     }
     HIP_OK(hipStreamSynchronize(s));

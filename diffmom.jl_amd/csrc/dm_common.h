@@ -233,6 +233,7 @@ struct dm_zmat {
   // and a slot of 2 x hrmax columns of 128 complex64 (U then V)
   DBuf<int> hrank;
   DBuf<float> hfac;
+  DBuf<double> hwork;  // tile norms and the tolerance
   int hrmax = 0;
   double hbytes = 0;  // bytes one product reads
   ZRef ref(int form) const {

@@ -12,7 +12,9 @@
 // every step (FP64 sum of FP32 entries), so the stopping test ||R||_F <= tau is
 // a bound on the tile's approximation error (up to the FP32 rounding of the
 // updates, ~1e-7 of the tile's norm), not an estimate.  With
-// tau^2 = eps^2 ||Z||_F^2 / (2 T_off) for T_off stored off-diagonal tiles, the
+// tau^2 = eps^2 ||Z||_F^2 / (2 T_off) for T_off stored off-diagonal tiles
+// (computed on the device from the tiles' norms, so the whole compression is
+// queued without a host round trip), the
 // whole matrix (where each such tile appears twice) satisfies
 // ||Z - Z_H||_F <= eps ||Z||_F.  Tiles that need more than rmax steps stay
 // dense (rank -1), as do the diagonal tiles.
@@ -43,15 +45,42 @@ __global__ __launch_bounds__(256) void k_tile_norm2(const float2* __restrict__ z
   if (threadIdx.x == 0) out[t] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
 }
 
+// tau^2 = eps^2 ||Z||_F^2 / (2 T_off) from the stored tiles' norms (diagonal
+// tiles count once, off-diagonal ones twice); one workgroup, fixed order.
+__global__ __launch_bounds__(256) void k_tau2(i64 T, int nb, const double* __restrict__ norms, double eps2,
+                                              double* __restrict__ tau2) {
+  double s = 0;
+  for (i64 t = threadIdx.x; t < T; t += 256) {
+    // diagonal tiles sit at t = J (J + 3) / 2
+    int J = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((i64)J * (J + 1) / 2 > t) --J;
+    while ((i64)(J + 1) * (J + 2) / 2 <= t) ++J;
+    const bool diag = t == (i64)J * (J + 1) / 2 + J;
+    s += (diag ? 1.0 : 2.0) * norms[t];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  __shared__ double ws[4];
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double fro2 = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+    const i64 toff = T - nb;
+    tau2[0] = toff > 0 ? eps2 * fro2 / (2.0 * (double)toff) : 0.0;
+  }
+}
+
 __device__ inline float2 cdivf(float2 a, float2 b) {
   const float d = b.x * b.x + b.y * b.y;
   return make_float2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
 }
 
 // Thread t owns column t / 4, rows 32 (t % 4) .. + 31 of the tile.
-__global__ __launch_bounds__(kAcaThreads) void k_aca(int nb, const float2* __restrict__ zp, double tau2, int rmax,
-                                                     int* __restrict__ rank, float2* __restrict__ fac) {
+__global__ __launch_bounds__(kAcaThreads) void k_aca(int nb, const float2* __restrict__ zp,
+                                                     const double* __restrict__ tau2p, int rmax, int* __restrict__ rank,
+                                                     float2* __restrict__ fac) {
   const i64 t = blockIdx.x;
+  const double tau2 = tau2p[0];
   int J = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
   while ((i64)J * (J + 1) / 2 > t) --J;
   while ((i64)(J + 1) * (J + 2) / 2 <= t) ++J;
@@ -172,19 +201,15 @@ __global__ __launch_bounds__(kAcaThreads) void k_aca(int nb, const float2* __res
 
 namespace dml {
 
-void tile_norm2(i64 n, const float* z32p, double* out, hipStream_t s) {
-  const i64 T = packed_tiles(n);
-  DM_REQUIRE(T < (1LL << 31), DM_ERR_ARGUMENT, "matrix too large");
-  k_tile_norm2<<<(unsigned)T, 256, 0, s>>>(reinterpret_cast<const float2*>(z32p), out);
-  HIP_OK(hipGetLastError());
-}
-
-void aca_compress(i64 n, const float* z32p, double tau2, int rmax, int* rank, float* fac, hipStream_t s) {
+void aca_compress(i64 n, const float* z32p, double eps, int rmax, double* work, int* rank, float* fac,
+                  hipStream_t s) {
   const i64 nb = (n + kPT - 1) / kPT;
   const i64 T = packed_tiles(n);
   DM_REQUIRE(rmax >= 1 && rmax <= kPT / 2, DM_ERR_ARGUMENT, "cross approximation rank cap out of range");
   DM_REQUIRE(T < (1LL << 31), DM_ERR_ARGUMENT, "matrix too large");
-  k_aca<<<(unsigned)T, kAcaThreads, 0, s>>>((int)nb, reinterpret_cast<const float2*>(z32p), tau2, rmax, rank,
+  k_tile_norm2<<<(unsigned)T, 256, 0, s>>>(reinterpret_cast<const float2*>(z32p), work);
+  k_tau2<<<1, 256, 0, s>>>(T, (int)nb, work, eps * eps, work + T);
+  k_aca<<<(unsigned)T, kAcaThreads, 0, s>>>((int)nb, reinterpret_cast<const float2*>(z32p), work + T, rmax, rank,
                                             reinterpret_cast<float2*>(fac));
   HIP_OK(hipGetLastError());
 }

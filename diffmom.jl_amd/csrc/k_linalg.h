@@ -39,12 +39,13 @@ size_t symv_workspace_bytes(i64 n);
 void symv_packed(i64 n, const ZRef& z, const double* x, double* y, void* ws, int shape, hipStream_t s);
 // launch shapes worth timing for this size and precision
 std::vector<int> symv_shapes(i64 n, bool f32);
-// Compressed packed complex64 form: per-tile Frobenius norms^2, the cross
-// approximation of the off-diagonal tiles (rank[t] = -1: dense; fac: per tile
-// U then V, rmax columns of 128 complex64 each) and its symmetric product with
-// tpw stored tiles per wave (workspace: symv_workspace_bytes(n)).
-void tile_norm2(i64 n, const float* z32p, double* out, hipStream_t s);
-void aca_compress(i64 n, const float* z32p, double tau2, int rmax, int* rank, float* fac, hipStream_t s);
+// Compressed packed complex64 form: the cross approximation of the
+// off-diagonal tiles to ||Z - Z_H||_F <= eps ||Z||_F (rank[t] = -1: dense;
+// fac: per tile U then V, rmax columns of 128 complex64 each; work: T + 1
+// doubles) and its symmetric product with tpw stored tiles per wave
+// (workspace: symv_workspace_bytes(n)).
+void aca_compress(i64 n, const float* z32p, double eps, int rmax, double* work, int* rank, float* fac,
+                  hipStream_t s);
 void symv_hier(i64 n, const float* z32p, const int* rank, const float* fac, int rmax, const double* x, double* y,
                void* ws, int tpw, hipStream_t s);
 std::vector<int> symv_hier_shapes(i64 n);

@@ -488,64 +488,87 @@ __global__ void k_givens(int j, int ldh, const double* __restrict__ hbuf, double
                          double* __restrict__ st, double* __restrict__ sc) {
   const int k = j + 1;
   double2* Hj = H + (i64)j * ldh;
+  // the column and the previous rotations are staged in LDS by all threads;
+  // the rotation chain then runs in one thread without global-memory latency
+  extern __shared__ double2 sh[];
+  double2* col = sh;          // k + 1
+  double2* rs = sh + k + 1;   // j rotations' s
+  double* rc = reinterpret_cast<double*>(rs + j);  // j rotations' c
   for (int i = threadIdx.x; i < k; i += blockDim.x)
-    Hj[i] = make_double2(hbuf[2 * i] + hbuf[2 * k + 2 * i], hbuf[2 * i + 1] + hbuf[2 * k + 2 * i + 1]);
+    col[i] = make_double2(hbuf[2 * i] + hbuf[2 * k + 2 * i], hbuf[2 * i + 1] + hbuf[2 * k + 2 * i + 1]);
+  for (int i = threadIdx.x; i < j; i += blockDim.x) {
+    rs[i] = sn[i];
+    rc[i] = cs[i];
+  }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  const double hn = sqrt(hbuf[4 * k]);
-  // previous rotations: [c s; -conj(s) c]
-  for (int i = 0; i < j; ++i) {
-    const double c = cs[i];
-    const double2 s = sn[i], a = Hj[i], b = Hj[i + 1];
-    // t = c a + s b ; b' = -conj(s) a + c b
-    const double2 t = make_double2(c * a.x + s.x * b.x - s.y * b.y, c * a.y + s.x * b.y + s.y * b.x);
-    Hj[i + 1] = make_double2(-(s.x * a.x + s.y * a.y) + c * b.x, -(s.x * a.y - s.y * a.x) + c * b.y);
-    Hj[i] = t;
+  if (threadIdx.x == 0) {
+    const double hn = sqrt(hbuf[4 * k]);
+    // previous rotations: [c s; -conj(s) c]
+    double2 a = col[0];
+    for (int i = 0; i < j; ++i) {
+      const double c = rc[i];
+      const double2 s = rs[i], b = col[i + 1];
+      // t = c a + s b ; b' = -conj(s) a + c b
+      col[i] = make_double2(c * a.x + s.x * b.x - s.y * b.y, c * a.y + s.x * b.y + s.y * b.x);
+      a = make_double2(-(s.x * a.x + s.y * a.y) + c * b.x, -(s.x * a.y - s.y * a.x) + c * b.y);
+    }
+    const double bb = hn;  // H[j+1][j] before the rotation (real, >= 0)
+    const double aa = hypot(a.x, a.y);
+    const double rho = hypot(aa, bb);
+    double c;
+    double2 s;
+    if (aa == 0) {
+      c = 0;
+      s = make_double2(1.0, 0.0);
+    } else {
+      c = aa / rho;
+      // s = (a / |a|) conj(b) / rho with b real
+      s = make_double2(a.x / aa * bb / rho, a.y / aa * bb / rho);
+    }
+    cs[j] = c;
+    sn[j] = s;
+    // H[j][j] = c a + s b
+    col[j] = make_double2(c * a.x + s.x * bb, c * a.y + s.y * bb);
+    col[k] = make_double2(0.0, 0.0);
+    const double2 gj = g[j];
+    // g[j+1] = -conj(s) g[j] ; g[j] = c g[j]
+    const double2 gk = make_double2(-(s.x * gj.x + s.y * gj.y), -(s.x * gj.y - s.y * gj.x));
+    g[k] = gk;
+    g[j] = make_double2(c * gj.x, c * gj.y);
+    st[2 * j] = hypot(gk.x, gk.y);
+    st[2 * j + 1] = hn;
+    sc[0] = hn > 0 ? 1.0 / hn : 0.0;
   }
-  const double2 a = Hj[j];
-  const double bb = hn;  // H[j+1][j] before the rotation (real, >= 0)
-  const double aa = hypot(a.x, a.y);
-  const double rho = hypot(aa, bb);
-  double c;
-  double2 s;
-  if (aa == 0) {
-    c = 0;
-    s = make_double2(1.0, 0.0);
-  } else {
-    c = aa / rho;
-    // s = (a / |a|) conj(b) / rho with b real
-    s = make_double2(a.x / aa * bb / rho, a.y / aa * bb / rho);
-  }
-  cs[j] = c;
-  sn[j] = s;
-  // H[j][j] = c a + s b
-  Hj[j] = make_double2(c * a.x + s.x * bb, c * a.y + s.y * bb);
-  Hj[k] = make_double2(0.0, 0.0);
-  const double2 gj = g[j];
-  // g[j+1] = -conj(s) g[j] ; g[j] = c g[j]
-  const double2 gk = make_double2(-(s.x * gj.x + s.y * gj.y), -(s.x * gj.y - s.y * gj.x));
-  g[k] = gk;
-  g[j] = make_double2(c * gj.x, c * gj.y);
-  st[2 * j] = hypot(gk.x, gk.y);
-  st[2 * j + 1] = hn;
-  sc[0] = hn > 0 ? 1.0 / hn : 0.0;
+  __syncthreads();
+  for (int i = threadIdx.x; i <= k; i += blockDim.x) Hj[i] = col[i];
 }
 
 // y = R^{-1} g for the leading j x j upper triangle of the rotated H
-// (column-major, leading dimension ldh), one thread (j is small).
-__global__ void k_backsolve(int j, int ldh, const double2* __restrict__ H, const double2* __restrict__ g,
-                            double2* __restrict__ y) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// (column-major, leading dimension ldh): column-oriented back substitution,
+// the right-hand side in LDS, one column update per step across the block.
+__global__ __launch_bounds__(256) void k_backsolve(int j, int ldh, const double2* __restrict__ H,
+                                                   const double2* __restrict__ g, double2* __restrict__ y) {
+  extern __shared__ double2 rhs[];
+  __shared__ double2 yi_s;
+  for (int i = threadIdx.x; i < j; i += blockDim.x) rhs[i] = g[i];
+  __syncthreads();
   for (int i = j - 1; i >= 0; --i) {
-    double2 acc = g[i];
-    for (int l = i + 1; l < j; ++l) {
-      const double2 h = H[(i64)l * ldh + i], yl = y[l];
-      acc.x -= h.x * yl.x - h.y * yl.y;
-      acc.y -= h.x * yl.y + h.y * yl.x;
+    if (threadIdx.x == 0) {
+      const double2 d = H[(i64)i * ldh + i], acc = rhs[i];
+      const double dd = d.x * d.x + d.y * d.y;
+      const double2 yi = make_double2((acc.x * d.x + acc.y * d.y) / dd, (acc.y * d.x - acc.x * d.y) / dd);
+      yi_s = yi;
+      y[i] = yi;
     }
-    const double2 d = H[(i64)i * ldh + i];
-    const double dd = d.x * d.x + d.y * d.y;
-    y[i] = make_double2((acc.x * d.x + acc.y * d.y) / dd, (acc.y * d.x - acc.x * d.y) / dd);
+    __syncthreads();
+    const double2 yi = yi_s;
+    const double2* hc = H + (i64)i * ldh;
+    for (int l = threadIdx.x; l < i; l += blockDim.x) {
+      const double2 h = hc[l];
+      rhs[l].x -= h.x * yi.x - h.y * yi.y;
+      rhs[l].y -= h.x * yi.y + h.y * yi.x;
+    }
+    __syncthreads();
   }
 }
 
@@ -639,14 +662,18 @@ void combine(i64 n, i64 ldv, int k, const double* V, const double* y, double* x,
 }
 void givens(int j, int ldh, const double* hbuf, double* H, double* cs, double* sn, double* g, double* st,
             double* sc, hipStream_t s) {
-  k_givens<<<1, 256, 0, s>>>(j, ldh, hbuf, reinterpret_cast<double2*>(H), cs, reinterpret_cast<double2*>(sn),
-                             reinterpret_cast<double2*>(g), st, sc);
+  const size_t smem = sizeof(double2) * (2 * (size_t)j + 2) + sizeof(double) * j;
+  DM_REQUIRE(smem <= 64 * 1024, DM_ERR_ARGUMENT, "Krylov basis too large for the device Givens step");
+  k_givens<<<1, 256, smem, s>>>(j, ldh, hbuf, reinterpret_cast<double2*>(H), cs, reinterpret_cast<double2*>(sn),
+                                reinterpret_cast<double2*>(g), st, sc);
   HIP_OK(hipGetLastError());
 }
 
 void backsolve(int j, int ldh, const double* H, const double* g, double* y, hipStream_t s) {
-  k_backsolve<<<1, 64, 0, s>>>(j, ldh, reinterpret_cast<const double2*>(H), reinterpret_cast<const double2*>(g),
-                               reinterpret_cast<double2*>(y));
+  const size_t smem = sizeof(double2) * (size_t)std::max(j, 1);
+  DM_REQUIRE(smem <= 64 * 1024, DM_ERR_ARGUMENT, "Krylov basis too large for the device back substitution");
+  k_backsolve<<<1, 256, smem, s>>>(j, ldh, reinterpret_cast<const double2*>(H),
+                                   reinterpret_cast<const double2*>(g), reinterpret_cast<double2*>(y));
   HIP_OK(hipGetLastError());
 }
 

@@ -574,38 +574,57 @@ __global__ __launch_bounds__(256, 3) void k_symv_h(i64 n, int nb, int tpw, const
 //              + sum over the column runs of tile column R of their colpart
 // (a run ends at tile t = R (R + 1) / 2 + I when t + 1 is a multiple of tpw
 // or I = R); fixed order, deterministic.
-__global__ void k_symv_w_reduce(i64 n, int nb, int tpw, const double2* __restrict__ rowpart,
-                                const double2* __restrict__ colpart, double2* __restrict__ y) {
-  const i64 g = blockIdx.x * (i64)blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  const int R = (int)(g / kPT), r = (int)(g % kPT);
+__global__ __launch_bounds__(256) void k_symv_w_reduce(i64 n, int nb, int tpw, const double2* __restrict__ rowpart,
+                                                       const double2* __restrict__ colpart, double2* __restrict__ y) {
+  // 32 rows per block, 8 groups sharing the tiles of a row block round robin;
+  // the groups' sums are added in a fixed order
+  const int rl = threadIdx.x & 31, grp = threadIdx.x >> 5;
+  const i64 g = blockIdx.x * (i64)32 + rl;
+  const int R = (int)min((i64)(nb - 1), (blockIdx.x * (i64)32) / kPT);
+  const int r = (int)(g % kPT);
   double sr = 0, si = 0;
-  int J = R;
-  for (; J + 8 <= nb; J += 8) {
-    double2 v[8];
+  if (g < n) {
+    int J = R + grp;
+    for (; J + 24 < nb; J += 32) {
+      double2 v[4];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = rowpart[((i64)(J + u) * (J + u + 1) / 2 + R) * kPT + r];
+      for (int u = 0; u < 4; ++u) {
+        const i64 Ju = J + 8 * u;
+        v[u] = rowpart[(Ju * (Ju + 1) / 2 + R) * kPT + r];
+      }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      sr += v[u].x;
-      si += v[u].y;
+      for (int u = 0; u < 4; ++u) {
+        sr += v[u].x;
+        si += v[u].y;
+      }
     }
-  }
-  for (; J < nb; ++J) {
-    double2 v = rowpart[((i64)J * (J + 1) / 2 + R) * kPT + r];
-    sr += v.x;
-    si += v.y;
-  }
-  const i64 c0 = (i64)R * (R + 1) / 2;
-  for (int I = 0; I <= R; ++I) {
-    const i64 t = c0 + I;
-    if (I == R || (t + 1) % tpw == 0) {
-      double2 v = colpart[t * kPT + r];
+    for (; J < nb; J += 8) {
+      const double2 v = rowpart[((i64)J * (J + 1) / 2 + R) * kPT + r];
       sr += v.x;
       si += v.y;
     }
+    const i64 c0 = (i64)R * (R + 1) / 2;
+    for (int I = grp; I <= R; I += 8) {
+      const i64 t = c0 + I;
+      if (I == R || (t + 1) % tpw == 0) {
+        const double2 v = colpart[t * kPT + r];
+        sr += v.x;
+        si += v.y;
+      }
+    }
   }
-  y[g] = make_double2(sr, si);
+  __shared__ double2 part[8][32];
+  part[grp][rl] = make_double2(sr, si);
+  __syncthreads();
+  if (grp == 0 && g < n) {
+    double2 acc = part[0][rl];
+#pragma unroll
+    for (int q = 1; q < 8; ++q) {
+      acc.x += part[q][rl].x;
+      acc.y += part[q][rl].y;
+    }
+    y[g] = acc;
+  }
 }
 
 }  // namespace
@@ -623,8 +642,8 @@ void symv_hier(i64 n, const float* z32p, const int* rank, const float* fac, int 
   double2* colpart = rowpart + T * kPT;
   k_symv_h<<<(unsigned)blocks, 256, 0, s>>>(n, (int)nb, tpw, z32p, rank, reinterpret_cast<const float2*>(fac), rmax,
                                             reinterpret_cast<const double2*>(x), rowpart, colpart);
-  k_symv_w_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, (int)nb, tpw, rowpart, colpart,
-                                                              reinterpret_cast<double2*>(y));
+  k_symv_w_reduce<<<(unsigned)((n + 31) / 32), 256, 0, s>>>(n, (int)nb, tpw, rowpart, colpart,
+                                                            reinterpret_cast<double2*>(y));
   HIP_OK(hipGetLastError());
 }
 
@@ -684,8 +703,8 @@ void symv_packed(i64 n, const ZRef& z, const double* x, double* y, void* ws, int
       k_symv_w<1><<<(unsigned)blocks, 256, 0, s>>>(n, (int)nb, tpw, z.p, xv, rowpart, colpart);
     else
       k_symv_w<0><<<(unsigned)blocks, 256, 0, s>>>(n, (int)nb, tpw, z.p, xv, rowpart, colpart);
-    k_symv_w_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, (int)nb, tpw, rowpart, colpart,
-                                                                reinterpret_cast<double2*>(y));
+    k_symv_w_reduce<<<(unsigned)((n + 31) / 32), 256, 0, s>>>(n, (int)nb, tpw, rowpart, colpart,
+                                                              reinterpret_cast<double2*>(y));
     HIP_OK(hipGetLastError());
     return;
   }
