@@ -147,15 +147,21 @@ def traffic_probe(dm, subdiv):
     k = 2 * math.pi / 0.1
     mesh = dm.make_icosphere(0.1, subdiv)
     rwg = dm.build_rwg(mesh)
-    Z = dm.assemble_Z_efie(mesh, rwg, k, forms=dm.DM_Z32P, device=True, max_output_bytes=None, mesh_precheck=False)
+    Z = dm.assemble_Z_efie(mesh, rwg, k, forms=dm.DM_Z64P | dm.DM_Z32P | dm.DM_Z32_ROUNDED, device=True,
+                           max_output_bytes=None, mesh_precheck=False)
     x = np.random.default_rng(0).standard_normal(rwg.nedges) + 0j
     Z.matmul(x, form=dm.DM_Z32P)  # tuner timing launches happen here
     mesh.ctx.sync()
-    print(f"tuned: {mesh.ctx.tune_get()}", flush=True)
     for _ in range(5):
         Z.matmul(x, form=dm.DM_Z32P)
     mesh.ctx.sync()
     print("symmetric products done", flush=True)
+    st = Z.compress(eps=5e-8)
+    Z.matmul(x, form=dm.DM_Z32H)  # tuner timing launches happen here
+    for _ in range(5):
+        Z.matmul(x, form=dm.DM_Z32H)
+    mesh.ctx.sync()
+    print(f"compressed products done: {st}; tuned: {mesh.ctx.tune_get()}", flush=True)
     try:
         hip = C.CDLL("libamdhip64.so.7")  # the runtime the library already loaded
     except OSError:
@@ -190,8 +196,9 @@ def main():
                     help="chosen form: cross-approximate the complex64 matrix to this relative Frobenius error "
                          "(0 = dense complex64)")
     ap.add_argument("--traffic-probe", action="store_true",
-                    help="no timing: 5 symmetric products of the packed complex64 matrix, then 3 plain 1 GiB "
-                         "device-to-device copies (counter calibration), for rocprofv3 --pmc runs")
+                    help="no timing: 5 symmetric products of the packed complex64 matrix, its compression "
+                         "(5e-8) and 5 compressed products, then 3 plain 1 GiB device-to-device copies (counter "
+                         "calibration), for rocprofv3 --pmc runs")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:

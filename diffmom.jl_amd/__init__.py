@@ -46,6 +46,7 @@ DM_Z64, DM_Z32 = 1, 2
 # packed symmetric storage of the same matrices (upper triangle of 128 x 128 tiles)
 DM_Z64P, DM_Z32P = 4, 8
 DM_Z32H = 16  # compressed complex64 (products and solves only)
+DM_Z32_ROUNDED = 32  # assemble_Z_efie flag: complex64 forms = rounded FP64 assembly
 DM_METHOD_LU, DM_METHOD_GMRES, DM_METHOD_MIXED = 0, 1, 2
 DM_PRECOND_NONE, DM_PRECOND_DIAG, DM_PRECOND_NEARFIELD, DM_PRECOND_LEARNED, DM_PRECOND_BLOCK = 0, 1, 2, 3, 4
 
@@ -481,7 +482,9 @@ def assemble_Z_efie(mesh, rwg, k, quad_order=3, eta0=ETA0, mesh_precheck=True, a
                     forms=DM_Z64, device=False):
     """Dense EFIE matrix (src/assembly/EFIE.jl:699-753).  Returns a complex128
     numpy matrix like the reference, or the device-resident ZMatrix when
-    device=True (forms selects complex128 and/or complex64 storage)."""
+    device=True (forms selects complex128 and/or complex64 storage; with
+    DM_Z32_ROUNDED the complex64 forms are the rounding of the FP64 assembly,
+    which compresses far better than the FP32-quadrature ones)."""
     _check_pair(mesh, rwg)
     N = rwg.nedges
     if max_output_bytes is not None and N * N * 16 > max_output_bytes:
@@ -492,7 +495,7 @@ def assemble_Z_efie(mesh, rwg, k, quad_order=3, eta0=ETA0, mesh_precheck=True, a
     h = C.c_void_p()
     _check(lib().dm_assemble_z(mesh.ctx.h, rwg.h, kr, ki, float(eta0), int(quad_order), int(forms),
                                int(max_output_bytes or 0), C.byref(h)))
-    Z = ZMatrix(rwg, h, forms, complex(k), float(eta0), quad_order)
+    Z = ZMatrix(rwg, h, int(forms) & ~DM_Z32_ROUNDED, complex(k), float(eta0), quad_order)
     if device:
         return Z
     out = Z.to_numpy()

@@ -534,8 +534,13 @@ int dm_assemble_z(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double k_im, do
     check_ctx(rwg_c, "rwg");
     check_ctx(out, "output handle");
     DM_REQUIRE(rwg_c->ctx == ctx, DM_ERR_ARGUMENT, "RWG data belongs to another context");
+    const bool rounded = (forms & DM_Z32_ROUNDED) != 0;
+    forms &= ~DM_Z32_ROUNDED;
     DM_REQUIRE(forms > 0 && (forms & ~kAllForms) == 0, DM_ERR_ARGUMENT,
-               "forms must be a mask of DM_Z64, DM_Z32, DM_Z64P, DM_Z32P");
+               "forms must be a mask of DM_Z64, DM_Z32, DM_Z64P, DM_Z32P (and DM_Z32_ROUNDED)");
+    if (rounded)
+      DM_REQUIRE(((forms & DM_Z32) == 0 || (forms & DM_Z64)) && ((forms & DM_Z32P) == 0 || (forms & DM_Z64P)),
+                 DM_ERR_ARGUMENT, "DM_Z32_ROUNDED needs the complex128 form of the same storage");
     CtxScope sc(ctx);
     dm_rwg& r = const_cast<dm_rwg&>(*rwg_c);
     i64 n = r.n;
@@ -565,17 +570,30 @@ int dm_assemble_z(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double k_im, do
       ProfScope p(ctx, "near_tables", s);
       a = make_args(r, T, k_re, k_im, eta0, z->ld, nt);
     }
+    if (forms & DM_Z32) z->z32.alloc((size_t)z->ld * n * 2);
     if (forms & DM_Z64) {
       z->z64.alloc((size_t)z->ld * n * 2);
       ProfScope p(ctx, "assemble_z64", s);
+      a.z32_dual = rounded && (forms & DM_Z32) ? z->z32.p : nullptr;
       launch_assemble_z64(a, z->z64.p, r.max_blk_tris, s);
+      a.z32_dual = nullptr;
     }
-    if (forms & DM_Z32) {
-      z->z32.alloc((size_t)z->ld * n * 2);
+    if ((forms & DM_Z32) && !rounded) {
       ProfScope p(ctx, "assemble_z32", s);
       launch_assemble_z32(a, z->z32.p, r.max_blk_tris, s);
     }
-    assemble_packed(z.get(), a, r, forms, s);
+    if (rounded && (forms & DM_Z32P)) {
+      z->z64p.alloc(packed_scalars(n));
+      z->z32p.alloc(packed_scalars(n));
+      zero_packed_padding(n, z->z64p.p, sizeof(double), s);
+      zero_packed_padding(n, z->z32p.p, sizeof(float), s);
+      ProfScope p(ctx, "assemble_z64_z32", s);
+      a.z32_dual = z->z32p.p;
+      launch_assemble_z64p(a, z->z64p.p, r.max_blk_tris, s);
+      a.z32_dual = nullptr;
+    } else {
+      assemble_packed(z.get(), a, r, forms, s);
+    }
     check_flag(nt, s);
     *out = z.release();
   });

@@ -107,13 +107,11 @@ __global__ __launch_bounds__(kAcaThreads) void k_aca(int nb, const float2* __res
   __shared__ float wmax[kAcaThreads / 64];
   __shared__ int warg[kAcaThreads / 64];
   __shared__ double wsum[kAcaThreads / 64];
-  __shared__ int s_piv;
-  __shared__ double s_sum;
   float2* U = fac + t * 2 * kPT * (i64)rmax;
   float2* V = U + (i64)rmax * kPT;
-  int k = 0;
-  for (;; ++k) {
-    // largest residual entry (ties: lowest index) and the residual norm
+  // this thread's largest residual entry (ties: lowest index) and its share
+  // of ||R||^2, reduced over the wave; lane 0 publishes the wave's result
+  auto publish = [&]() {
     float best = -1.f;
     int bq = 0;
     double sum = 0;
@@ -142,28 +140,32 @@ __global__ __launch_bounds__(kAcaThreads) void k_aca(int nb, const float2* __res
       warg[w] = gidx;
       wsum[w] = sum;
     }
+  };
+  publish();
+  int k = 0;
+  for (;; ++k) {
+    // two barriers per step: the waves' results are read between the first
+    // and the second, and rewritten only after the second
     __syncthreads();
-    if (tid == 0) {
-      float b = wmax[0];
-      int g = warg[0];
-      double s = wsum[0];
-      for (int q = 1; q < kAcaThreads / 64; ++q) {
-        if (wmax[q] > b || (wmax[q] == b && warg[q] < g)) {
-          b = wmax[q];
-          g = warg[q];
-        }
-        s += wsum[q];
+    float b = wmax[0];
+    int piv_idx = warg[0];
+    double tot = wsum[0];
+#pragma unroll
+    for (int q = 1; q < kAcaThreads / 64; ++q) {
+      const float bq = wmax[q];
+      const int gq = warg[q];
+      if (bq > b || (bq == b && gq < piv_idx)) {
+        b = bq;
+        piv_idx = gq;
       }
-      s_piv = g;
-      s_sum = s;
+      tot += wsum[q];
     }
-    __syncthreads();
-    if (s_sum <= tau2) break;  // rank k reaches the tolerance
+    if (tot <= tau2) break;  // rank k reaches the tolerance
     if (k == rmax) {
       k = -1;  // not compressible within rmax: stays dense
       break;
     }
-    const int pj = s_piv / kPT, pi = s_piv % kPT;
+    const int pj = piv_idx / kPT, pi = piv_idx % kPT;
     if (c == pj) {
 #pragma unroll
       for (int q = 0; q < 32; ++q) colb[r0 + q] = make_float2(Rr[q], Ri[q]);
@@ -192,7 +194,7 @@ __global__ __launch_bounds__(kAcaThreads) void k_aca(int nb, const float2* __res
       Rr[q] -= u.x * vc.x - u.y * vc.y;
       Ri[q] -= u.x * vc.y + u.y * vc.x;
     }
-    __syncthreads();  // colb / rowb / s_piv are rewritten in the next step
+    publish();
   }
   if (tid == 0) rank[t] = k;
 }

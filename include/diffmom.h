@@ -52,6 +52,11 @@ extern "C" {
 /* complex64 packed storage whose off-diagonal tiles may be cross-approximated
  * (dm_zmat_compress); usable for products and solves, not for entry reads */
 #define DM_Z32H 16
+/* dm_assemble_z flag: the complex64 forms are the rounding of the FP64
+ * assembly (one launch writes both) instead of an FP32-quadrature assembly;
+ * needs the complex128 form of the same storage in the mask.  The rounded
+ * matrix carries only FP32 rounding noise, so it compresses much better. */
+#define DM_Z32_ROUNDED 32
 
 typedef struct dm_ctx dm_ctx;
 typedef struct dm_mesh dm_mesh;

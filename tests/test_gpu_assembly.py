@@ -298,8 +298,10 @@ def test_compressed_form_error_bound(dm, mk):
     m = dm.make_rect_plate(0.3, 0.25, 30, 22) if mk == "plate" else dm.make_icosphere(0.1, 3)
     r = dm.build_rwg(m)
     N = r.nedges
-    Zd = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64P | dm.DM_Z32P, device=True)
+    Zd = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64P | dm.DM_Z32P | dm.DM_Z32_ROUNDED, device=True)
+    assert Zd.forms == dm.DM_Z64P | dm.DM_Z32P
     P = Zd.to_numpy(dm.DM_Z32P).astype(np.complex128)
+    assert np.array_equal(P, Zd.to_numpy(dm.DM_Z64P).astype(np.complex64))
     for eps in (1e-3, 1e-6):
         st = Zd.compress(eps=eps, rmax=48)
         nb = -(-N // 128)
@@ -317,6 +319,15 @@ def test_compressed_form_error_bound(dm, mk):
         assert np.linalg.norm(H - H.T) / np.linalg.norm(P) <= 1e-12
     x = np.random.default_rng(2).standard_normal(N) + 0j
     assert np.linalg.norm(Zd.matmul(x, form=dm.DM_Z32H) - P @ x) / np.linalg.norm(P @ x) < 1e-5
+    # the FP32-quadrature matrix carries quadrature noise above FP32 rounding,
+    # which the cross approximation cannot drop: it compresses no better
+    Zq = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z32P, device=True)
+    Zr = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64P | dm.DM_Z32P | dm.DM_Z32_ROUNDED, device=True)
+    sq, sr = Zq.compress(eps=1e-6), Zr.compress(eps=1e-6)
+    print(f"eps 1e-6: FP32 quadrature {sq}, rounded FP64 {sr}")
+    assert sr["bytes_ratio"] <= sq["bytes_ratio"]
+    with pytest.raises(ValueError):
+        dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z32P | dm.DM_Z32_ROUNDED, device=True)
     with pytest.raises(Exception):
         Zd.compress(eps=0.0)
     with pytest.raises(Exception):
