@@ -310,6 +310,19 @@ def main():
                        note="plain path not re-run per wave; per-RHS comparisons use the single-RHS times")
         log(f"batched {a.batch_rhs} plane waves: {dt:.2f} s ({dt / a.batch_rhs * 1e3:.1f} ms per RHS), "
             f"{st_b.niter} it total, worst residual {st_b.residual:.2e}")
+        # far fields of all currents on a 5-degree grid (E = G I on the matrix cores)
+        grid = dm.make_sph_grid(36, 72)
+        nom = grid.rhat.shape[0]
+        dm.farfield(mesh, rwg, grid, k, X[:, :8])
+        ctx.sync()
+        t0 = time.perf_counter()
+        E = dm.farfield(mesh, rwg, grid, k, X)
+        t_ff = time.perf_counter() - t0
+        rcs = dm.bistatic_rcs(E[:, :, 0])
+        batched["far_field"] = dict(directions=nom, currents=int(X.shape[1]), s=round(t_ff, 4),
+                                    gemm_tflops_incl_host_copies=round(8.0 * 3 * nom * N * X.shape[1] / t_ff / 1e12, 2),
+                                    rcs_max_m2=float(rcs.max()))
+        log(f"far field: {nom} directions x {X.shape[1]} currents in {t_ff * 1e3:.1f} ms")
 
     # roofline of the dominant kernel of the chosen form
     nq = 3

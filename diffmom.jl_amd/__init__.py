@@ -121,6 +121,7 @@ def lib():
         "dm_zmat_free": ([vp], i32),
         "dm_apply": ([vp, vp, i32, i32, vp, i64, i64, vp, i64], i32),
         "dm_zmat_compress": ([vp, C.c_double, i32, vp], i32),
+        "dm_farfield": ([vp, vp, vp, i64, C.c_double, C.c_double, i32, vp, i64, i64, vp, i64], i32),
         "dm_mf_create": ([vp, vp, d, d, d, i32, pp], i32), "dm_mf_apply": ([vp, vp, i32, vp, vp], i32),
         "dm_mf_free": ([vp], i32),
         "dm_excitation_planewave": ([vp, vp, vp, vp, vp, i64, i32, vp], i32),
@@ -837,6 +838,24 @@ def compute_farfield(G, I, NΩ):
     if G.shape[0] != 3 * NΩ:
         raise DimensionMismatch(f"G has {G.shape[0]} rows, expected {3 * NΩ}")
     return (G @ np.asarray(I, np.complex128)).reshape((3, NΩ), order="F")
+
+
+def farfield(mesh, rwg, grid, k, I, quad_order=3, eta0=ETA0):
+    """E∞ for one current vector (3, NΩ) or many (3, NΩ, M): compute_farfield
+    of radiation_vectors (src/postprocessing/FarField.jl:425-605) without G
+    leaving the GPU (chunks of directions, E = G I on the FP64 matrix cores)."""
+    _check_pair(mesh, rwg)
+    rhat = np.ascontiguousarray(grid.rhat if isinstance(grid, SphGrid) else grid, dtype=np.float64)
+    no = rhat.shape[0]
+    I = np.asarray(I, dtype=np.complex128)
+    vec = I.ndim == 1
+    If = np.asfortranarray(I.reshape(rwg.nedges, -1))
+    M = If.shape[1]
+    E = np.zeros((3 * no, M), dtype=np.complex128, order="F")
+    _check(lib().dm_farfield(mesh.ctx.h, rwg.h, _ptr(rhat), no, float(k), float(eta0), int(quad_order),
+                             If.ctypes.data_as(C.c_void_p), rwg.nedges, M, E.ctypes.data_as(C.c_void_p), 3 * no))
+    E = E.reshape((3, no, M), order="F")
+    return E[:, :, 0] if vec else E
 
 
 def bistatic_rcs(E_ff, E0=1.0):

@@ -920,6 +920,56 @@ int dm_radiation_vectors(dm_ctx* ctx, const dm_rwg* rwg, const double* rhat, int
   });
 }
 
+int dm_farfield(dm_ctx* ctx, const dm_rwg* rwg, const double* rhat, int64_t nomega, double k, double eta0,
+                int quad_order, const double* I, int64_t ldi, int64_t ncur, double* E, int64_t lde) {
+  return guard([&] {
+    check_ctx(ctx, "context");
+    check_ctx(rwg, "rwg");
+    check_ctx(I, "currents");
+    check_ctx(E, "output");
+    DM_REQUIRE(nomega > 0 && ncur >= 1, DM_ERR_ARGUMENT, "NΩ and the current count must be positive");
+    DM_REQUIRE(ldi >= rwg->n && lde >= 3 * nomega, DM_ERR_DIMENSION, "leading dimension too small");
+    DM_REQUIRE(std::isfinite(k) && std::isfinite(eta0), DM_ERR_ARGUMENT, "wavenumber and eta0 must be finite");
+    for (i64 q = 0; q < nomega; ++q) {
+      double nn = rhat[3 * q] * rhat[3 * q] + rhat[3 * q + 1] * rhat[3 * q + 1] + rhat[3 * q + 2] * rhat[3 * q + 2];
+      DM_REQUIRE(std::isfinite(nn) && nn > 0, DM_ERR_ARGUMENT, "far-field direction must be finite and nonzero");
+    }
+    CtxScope sc(ctx);
+    dm_rwg& r = const_cast<dm_rwg&>(*rwg);
+    const TriTables& T = dmh::tables(r, quad_order);
+    hipStream_t s = ctx->stream;
+    const i64 n = r.n;
+    // radiation vectors of a chunk of directions (<= ~1 GiB), then E = G I on
+    // the matrix cores, column by column of currents in blocks of 1024
+    const i64 chunk = std::max<i64>(1, std::min<i64>(nomega, (1LL << 30) / (48 * std::max<i64>(n, 1))));
+    const int RB = (int)std::min<i64>(ncur, 1024);
+    DBuf<double> dr, dg, di, de, ws;
+    dr.upload(rhat, 3 * nomega, s);
+    dg.alloc((size_t)2 * 3 * chunk * n);
+    di.alloc((size_t)2 * n * RB);
+    de.alloc((size_t)2 * 3 * chunk * RB);
+    ws.alloc((size_t)2 * n * RB);
+    for (i64 j0 = 0; j0 < ncur; j0 += RB) {
+      const int R = (int)std::min<i64>(RB, ncur - j0);
+      HIP_OK(hipMemcpy2DAsync(di.p, 16 * n, I + 2 * ldi * j0, 16 * ldi, 16 * n, R, hipMemcpyHostToDevice, s));
+      for (i64 q0 = 0; q0 < nomega; q0 += chunk) {
+        const i64 nq = std::min(chunk, nomega - q0);
+        {
+          ProfScope p(ctx, "radiation", s);
+          dmf::radiation_vectors(r, T, dr.p + 3 * q0, nq, k, eta0, dg.p, s);
+        }
+        {
+          ProfScope p(ctx, "farfield_gemm", s);
+          dml::gemm_c128(3 * nq, n, dg.p, 3 * nq, di.p, n, R, de.p, 3 * nq, ws.p, s);
+        }
+        HIP_OK(hipMemcpy2DAsync(E + 2 * (lde * j0 + 3 * q0), 16 * lde, de.p, 16 * 3 * nq, 16 * 3 * nq, R,
+                                hipMemcpyDeviceToHost, s));
+      }
+    }
+    HIP_OK(hipStreamSynchronize(s));
+  });
+}
+
 int dm_precond_create(dm_ctx* ctx, const dm_zmat* z, int kind, double nf_cutoff, int block_size, int form,
                       dm_precond** out) {
   return guard([&] {

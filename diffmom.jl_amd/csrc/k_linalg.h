@@ -50,6 +50,9 @@ void symv_hier(i64 n, const float* z32p, const int* rank, const float* fac, int 
                void* ws, int tpw, hipStream_t s);
 std::vector<int> symv_hier_shapes(i64 n);
 size_t zx_workspace_bytes(i64 n, int R);
+// Y (mr x R) = A (mr x n, lda) X (n x R), complex128 on FP64 MFMA; ws: 2 n R doubles.
+void gemm_c128(i64 mr, i64 n, const double* A, i64 lda, const double* X, i64 ldx, int R, double* Y, i64 ldy,
+               void* ws, hipStream_t s);
 void zx_mfma(i64 n, i64 ld, const float* z32, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
              hipStream_t s);
 void zx_mfma64(i64 n, i64 ld, const double* z64, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,

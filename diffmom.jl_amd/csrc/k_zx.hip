@@ -131,7 +131,9 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 // FP64 variant (complex128 Z, v_mfma_f64_16x16x4f64): the refinement residual
 // of many right-hand sides.  Same tiling; one K chunk (FP64 accumulation).
-__global__ __launch_bounds__(256) void k_zx_mfma64(i64 n, i64 ld, const double* __restrict__ Z,
+// Rows mr (complex) and inner dimension n may differ: Y (mr x R) = Z (mr x n,
+// leading dimension ld) X (n x R).
+__global__ __launch_bounds__(256) void k_zx_mfma64(i64 mr, i64 n, i64 ld, const double* __restrict__ Z,
                                                    const double* __restrict__ Xr, const double* __restrict__ Xi, i64 ldx,
                                                    int R, double2* __restrict__ Y, i64 ldy) {
   __shared__ double As[ZX_BK][2 * ZX_BM + 2];
@@ -152,7 +154,7 @@ __global__ __launch_bounds__(256) void k_zx_mfma64(i64 n, i64 ld, const double* 
     for (int e = tid; e < ZX_BK * 2 * ZX_BM; e += 256) {
       int kk = e / (2 * ZX_BM), rr = e % (2 * ZX_BM);
       i64 col = k0 + kk, row2 = 2 * m0 + rr;
-      As[kk][rr] = (col < n && row2 < 2 * n) ? Z[col * 2 * ld + row2] : 0.0;
+      As[kk][rr] = (col < n && row2 < 2 * mr) ? Z[col * 2 * ld + row2] : 0.0;
     }
     for (int e = tid; e < ZX_BK * ZX_BN; e += 256) {
       int kk = e % ZX_BK, jj = e / ZX_BK;
@@ -202,7 +204,7 @@ __global__ __launch_bounds__(256) void k_zx_mfma64(i64 n, i64 ld, const double* 
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const i64 m = m0 + ((32 * wave + 16 * rb + g + 4 * r) >> 1);
-        if (m >= n) continue;
+        if (m >= mr) continue;
         Y[m + (i64)col * ldy] = make_double2(p[r] - qo[r], q[r] + po[r]);
       }
     }
@@ -239,7 +241,21 @@ void zx_mfma64(i64 n, i64 ld, const double* z64, const double* X, i64 ldx, int R
                                                                   Xi);
   dim3 grid((unsigned)((n + ZX_BM - 1) / ZX_BM), (unsigned)((R + ZX_BN - 1) / ZX_BN), 1);
   DM_REQUIRE(grid.y <= 65535, DM_ERR_ARGUMENT, "too many right-hand sides for one product");
-  k_zx_mfma64<<<grid, 256, 0, s>>>(n, ld, z64, Xr, Xi, n, R, reinterpret_cast<double2*>(Y), ldy);
+  k_zx_mfma64<<<grid, 256, 0, s>>>(n, n, ld, z64, Xr, Xi, n, R, reinterpret_cast<double2*>(Y), ldy);
+  HIP_OK(hipGetLastError());
+}
+
+void gemm_c128(i64 mr, i64 n, const double* A, i64 lda, const double* X, i64 ldx, int R, double* Y, i64 ldy,
+               void* ws, hipStream_t s) {
+  DM_REQUIRE(lda >= mr && ldx >= n && ldy >= mr && R >= 1, DM_ERR_ARGUMENT, "complex GEMM: bad dimensions");
+  double* Xr = reinterpret_cast<double*>(ws);
+  double* Xi = Xr + n * R;
+  const i64 tot = n * R;
+  k_split_planes64<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, R, reinterpret_cast<const double2*>(X), ldx, Xr,
+                                                                  Xi);
+  dim3 grid((unsigned)((mr + ZX_BM - 1) / ZX_BM), (unsigned)((R + ZX_BN - 1) / ZX_BN), 1);
+  DM_REQUIRE(grid.x < (1u << 31) && grid.y <= 65535, DM_ERR_ARGUMENT, "complex GEMM too large for one launch");
+  k_zx_mfma64<<<grid, 256, 0, s>>>(mr, n, lda, A, Xr, Xi, n, R, reinterpret_cast<double2*>(Y), ldy);
   HIP_OK(hipGetLastError());
 }
 

@@ -300,6 +300,15 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg, double k_re, double k
 int dm_radiation_vectors(dm_ctx* ctx, const dm_rwg* rwg, const double* rhat, int64_t nomega,
                          double k, double eta0, int quad_order, double* G);
 
+/* Far fields of many currents: E (3 NΩ x ncur, column-major, leading
+ * dimension lde) = G I with G the radiation vectors above and I (N x ncur,
+ * caller basis order, leading dimension ldi) -- compute_farfield
+ * (src/postprocessing/FarField.jl:584-605) for a batch of solutions, without
+ * G leaving the device: G is formed for chunks of directions and multiplied
+ * on the FP64 matrix cores. */
+int dm_farfield(dm_ctx* ctx, const dm_rwg* rwg, const double* rhat, int64_t nomega, double k, double eta0,
+                int quad_order, const double* I, int64_t ldi, int64_t ncur, double* E, int64_t lde);
+
 #ifdef __cplusplus
 }
 #endif

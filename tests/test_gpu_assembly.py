@@ -353,3 +353,33 @@ def test_compressed_product_shapes(dm):
         y = Zd.matmul(x, form=dm.DM_Z32H)
         assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-13, tpw
     ctx.tune_set(got)
+
+
+def test_farfield_of_many_currents(dm, O):
+    """dm_farfield: E = G I for a batch of currents on the FP64 matrix cores
+    (compute_farfield, src/postprocessing/FarField.jl:584-605), against the
+    oracle's radiation vectors; a 12000-direction grid on N = 1920 spans two
+    direction chunks, checked on a spread subset of directions."""
+    m = dm.make_icosphere(0.05, 2)
+    r = dm.build_rwg(m)
+    rng = np.random.default_rng(4)
+    grid = dm.make_sph_grid(9, 8)
+    Go = O.radiation_vectors(oracle_rwg(O, m, r), grid.rhat, K0)
+    I = rng.standard_normal((r.nedges, 70)) + 1j * rng.standard_normal((r.nedges, 70))
+    E = dm.farfield(m, r, grid, K0, I)
+    assert E.shape == (3, 72, 70)
+    ref = (Go @ I).reshape((3, 72, 70), order="F")
+    assert np.linalg.norm(E - ref) / np.linalg.norm(ref) < 1e-13
+    e1 = dm.farfield(m, r, grid, K0, I[:, 5])
+    assert np.linalg.norm(e1 - dm.compute_farfield(Go, I[:, 5], 72)) / np.linalg.norm(e1) < 1e-13
+    m2 = dm.make_icosphere(0.1, 3)
+    r2 = dm.build_rwg(m2)
+    big = dm.make_sph_grid(120, 100)
+    I2 = rng.standard_normal((r2.nedges, 9)) + 1j * rng.standard_normal((r2.nedges, 9))
+    E2 = dm.farfield(m2, r2, big, K0, I2)
+    S = np.unique(np.concatenate([np.arange(0, 12000, 97), [11999, 11650, 11651]]))
+    G2 = dm.radiation_vectors(m2, r2, big.rhat[S], K0)
+    ref2 = (G2 @ I2).reshape((3, len(S), 9), order="F")
+    assert np.linalg.norm(E2[:, S, :] - ref2) / np.linalg.norm(ref2) < 1e-13
+    with pytest.raises(ValueError):
+        dm.farfield(m, r, np.zeros((2, 3)), K0, I)
