@@ -121,20 +121,29 @@ def random_planewaves(dm, k, count, seed=20261202):
 
 
 COPY_BYTES = 1 << 30
-TRAFFIC_FILES = {("symv_z32", 5): "symv_z32_c2_sphere_traffic.md"}
+TRAFFIC_FILES = {("symv_z32", 5): "symv_z32_c2_sphere_traffic.md",
+                 ("symv_z32h", 5): "symv_z32h_c2_sphere_traffic.md"}
 
 
-def measured_traffic(kernel, subdiv):
+def measured_traffic(kernel, subdiv, tuned):
     """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary of this kernel and workload (scaled by the copy calibration), or
-    None when no such summary exists."""
+    None when no such summary exists.  A line tagged [symvh=<shape>] applies
+    only when the tuner chose that shape in this run."""
     f = TRAFFIC_FILES.get((kernel, subdiv))
     path = os.path.join(ROOT, "profiles", f) if f else None
     if not path or not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_product:"):
-            return float(line.split(":")[1])
+        if not line.startswith("traffic_bytes_per_product"):
+            continue
+        tag, val = line.split(":", 1)
+        if "[" in tag:
+            key, shape = tag[tag.index("[") + 1:tag.index("]")].split("=")
+            prefix = {"symvh": "symvh/c64/"}[key]
+            if not any(k.startswith(prefix) and v == int(shape) for k, v in tuned.items()):
+                continue
+        return float(val)
     return None
 
 
@@ -352,7 +361,7 @@ def main():
         dom = "symv_z32" if "symv_z32" in fracs else "gemv_z32"
     roof = fracs[dom]
     roofline = dict(kernel=dom, bound=roof["bound"], achieved=roof["achieved"], peak=roof["peak"], unit=roof["unit"],
-                    frac=roof["frac"], traffic=measured_traffic(dom, a.subdiv))
+                    frac=roof["frac"], traffic=measured_traffic(dom, a.subdiv, ctx.tune_get()))
     cpu = None if a.no_cpu else cpu_baseline(mesh, rwg, k, plain["none"]["iterations"])
     out = {
         "metric": METRIC, "value": round(t_c, 4), "unit": "s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
