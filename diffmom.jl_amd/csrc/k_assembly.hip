@@ -392,23 +392,26 @@ __device__ inline void far_block(const TileSide<T, NQ>& R, int i, const TileSide
   T dc[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) dc[c] = (T)(R.c[3 * i + c] - C.c[3 * j + c]);
-  T ur[NQ][3], uc[NQ][3];
+  T uc[NQ][3];
 #pragma unroll
   for (int q = 0; q < NQ; ++q)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      ur[q][c] = R.u[(i * NQ + q) * 3 + c];
-      uc[q][c] = C.u[(j * NQ + q) * 3 + c];
-    }
+    for (int c = 0; c < 3; ++c) uc[q][c] = C.u[(j * NQ + q) * 3 + c];
   T m0r = 0, m0i = 0, muvr = 0, muvi = 0;
   T mur[3] = {0, 0, 0}, mui[3] = {0, 0, 0}, mvr[3] = {0, 0, 0}, mvi[3] = {0, 0, 0};
-#pragma unroll
+  // FP64: the outer quadrature loop stays rolled (fewer live registers, so
+  // more waves per SIMD); FP32 unrolls it
+  constexpr int UQ = sizeof(T) == 8 ? 1 : NQ;
+#pragma unroll UQ
   for (int q = 0; q < NQ; ++q) {
+    T urq[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) urq[c] = R.u[(i * NQ + q) * 3 + c];
 #pragma unroll
     for (int p = 0; p < NQ; ++p) {
-      T dx = dc[0] + (ur[q][0] - uc[p][0]);
-      T dy = dc[1] + (ur[q][1] - uc[p][1]);
-      T dz = dc[2] + (ur[q][2] - uc[p][2]);
+      T dx = dc[0] + (urq[0] - uc[p][0]);
+      T dy = dc[1] + (urq[1] - uc[p][1]);
+      T dz = dc[2] + (urq[2] - uc[p][2]);
       T R2 = dx * dx + dy * dy + dz * dz;
       T sn, cs, Rd, g;
       if constexpr (sizeof(T) == 8) {
@@ -427,12 +430,12 @@ __device__ inline void far_block(const TileSide<T, NQ>& R, int i, const TileSide
       }
       if (ki != (T)0) g *= exp(ki * Rd);
       T gr = g * cs, gi = -g * sn;
-      T uv = ur[q][0] * uc[p][0] + ur[q][1] * uc[p][1] + ur[q][2] * uc[p][2];
+      T uv = urq[0] * uc[p][0] + urq[1] * uc[p][1] + urq[2] * uc[p][2];
       m0r += gr; m0i += gi;
       muvr += gr * uv; muvi += gi * uv;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        mur[c] += gr * ur[q][c]; mui[c] += gi * ur[q][c];
+        mur[c] += gr * urq[c]; mui[c] += gi * urq[c];
         mvr[c] += gr * uc[p][c]; mvi[c] += gi * uc[p][c];
       }
     }
@@ -750,7 +753,9 @@ size_t tile_smem(int nq, int TB, int cap) {
 
 template <typename T, typename AccT, int MODE>
 void launch_tile(const AsmArgs& a, int cap, AccT* z, const double* x, double* y, hipStream_t s) {
-  constexpr int NTHR = 512;
+  // FP64: 1024 threads (16 waves, <= 128 VGPRs each) fill the CU that the
+  // ~98 KB of LDS per tile limits to one workgroup
+  constexpr int NTHR = sizeof(T) == 8 ? 1024 : 512;
   size_t smem = tile_smem<T, AccT>(a.nq, a.TB, cap);
   DM_REQUIRE(smem <= 160 * 1024, DM_ERR_ARGUMENT, "tile working set exceeds LDS");
   i64 T_ = (i64)a.nblk * (a.nblk + 1) / 2;
