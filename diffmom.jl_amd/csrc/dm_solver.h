@@ -72,19 +72,25 @@ struct OpZ32 : DevOp {
 namespace dmt {
 // Tuned launch shape of the symmetric product (timed on first use per size).
 int symv_shape(dm_ctx* ctx, const dm_zmat* z, int form, void* ws, hipStream_t s);
-// Tuned tiles per wave of the compressed product.
+// Tuned wave count of the compressed product, as a multiple of the resident
+// wave slots (1..64).
 int symv_hier_shape(dm_ctx* ctx, const dm_zmat* z, void* ws, hipStream_t s);
 }  // namespace dmt
 namespace dms {
 struct OpHier : DevOp {
   const dm_zmat* z;
-  int tpw = 0;
-  DBuf<char> ws;
+  DBuf<char> ws, rbuf;
+  dml::HierRanges hr;  // wave ranges, set up on the first product (ranks known by then)
   explicit OpHier(const dm_zmat* z_) : z(z_) { ws.alloc(dml::symv_workspace_bytes(z->n)); }
   void apply(const double* x, double* y, hipStream_t s) override {
-    if (!tpw) tpw = dmt::symv_hier_shape(z->ctx, z, ws.p, s);
+    if (!hr.W) {
+      const int mult = dmt::symv_hier_shape(z->ctx, z, ws.p, s);
+      const int W = mult * dml::hier_wave_slots();
+      rbuf.alloc(dml::hier_ranges_bytes(z->n, W));
+      dml::hier_ranges(z->n, z->hrank.p, W, rbuf.p, hr, s);
+    }
     ProfScope p(z->ctx, "symv_z32h", s);
-    dml::symv_hier(z->n, z->z32p.p, z->hrank.p, z->hfac.p, z->hrmax, x, y, ws.p, tpw, s);
+    dml::symv_hier(z->n, z->z32p.p, z->hrank.p, z->hfac.p, z->hrmax, x, y, ws.p, hr, s);
   }
   void apply_batch(const double* X, double* Y, int R, void*, hipStream_t s) override {
     for (int j = 0; j < R; ++j) apply(X + 2 * z->n * j, Y + 2 * z->n * j, s);

@@ -70,20 +70,28 @@ int symv_hier_shape(dm_ctx* ctx, const dm_zmat* z, void* ws, hipStream_t s) {
   const int fallback = 2;
   const std::string key = "symvh/c64/" + std::to_string(z->n);
   auto it = ctx->tune.find(key);
-  if (it != ctx->tune.end()) return it->second;
+  if (it != ctx->tune.end()) {
+    DM_REQUIRE(it->second >= 1 && it->second <= 64, DM_ERR_ARGUMENT, "compressed product wave multiple out of range");
+    return it->second;
+  }
   if (!tuning_enabled()) return fallback;
   DBuf<double> x, y;
   x.alloc(2 * z->n);
   y.alloc(2 * z->n);
   HIP_OK(hipMemsetAsync(x.p, 0, x.bytes(), s));
+  const int slots = dml::hier_wave_slots();
   int best = fallback;
   double best_ms = 1e300;
-  for (int tpw : dml::symv_hier_shapes(z->n)) {
+  for (int mult : {1, 2, 4, 8}) {
+    DBuf<char> rb;
+    rb.alloc(dml::hier_ranges_bytes(z->n, mult * slots));
+    dml::HierRanges hr;
+    dml::hier_ranges(z->n, z->hrank.p, mult * slots, rb.p, hr, s);
     double ms = time_ms(
-        [&] { dml::symv_hier(z->n, z->z32p.p, z->hrank.p, z->hfac.p, z->hrmax, x.p, y.p, ws, tpw, s); }, s);
+        [&] { dml::symv_hier(z->n, z->z32p.p, z->hrank.p, z->hfac.p, z->hrmax, x.p, y.p, ws, hr, s); }, s);
     if (ms < best_ms) {
       best_ms = ms;
-      best = tpw;
+      best = mult;
     }
   }
   ctx->tune[key] = best;

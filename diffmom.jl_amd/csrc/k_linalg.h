@@ -42,13 +42,21 @@ std::vector<int> symv_shapes(i64 n, bool f32);
 // Compressed packed complex64 form: the cross approximation of the
 // off-diagonal tiles to ||Z - Z_H||_F <= eps ||Z||_F (rank[t] = -1: dense;
 // fac: per tile U then V, rmax columns of 128 complex64 each; work: T + 1
-// doubles) and its symmetric product with tpw stored tiles per wave
-// (workspace: symv_workspace_bytes(n)).
+// doubles) and its symmetric product (workspace: symv_workspace_bytes(n)).
 void aca_compress(i64 n, const float* z32p, double eps, int rmax, double* work, int* rank, float* fac,
                   hipStream_t s);
+// Wave ranges of the compressed product: W waves, wstart[W + 1] tile
+// boundaries carrying about equal bytes, rend[T] = 1 where a column run ends.
+struct HierRanges {
+  int W = 0;
+  int* wstart = nullptr;
+  unsigned char* rend = nullptr;
+};
+size_t hier_ranges_bytes(i64 n, int W);
+int hier_wave_slots();  // resident waves of the product kernel on this device
+void hier_ranges(i64 n, const int* rank, int W, void* buf, HierRanges& hr, hipStream_t s);
 void symv_hier(i64 n, const float* z32p, const int* rank, const float* fac, int rmax, const double* x, double* y,
-               void* ws, int tpw, hipStream_t s);
-std::vector<int> symv_hier_shapes(i64 n);
+               void* ws, const HierRanges& hr, hipStream_t s);
 size_t zx_workspace_bytes(i64 n, int R);
 // Y (mr x R) = A (mr x n, lda) X (n x R), complex128 on FP64 MFMA; ws: 2 n R doubles.
 void gemm_c128(i64 mr, i64 n, const double* A, i64 lda, const double* X, i64 ldx, int R, double* Y, i64 ldy,

@@ -474,7 +474,10 @@ __global__ __launch_bounds__(256) void k_symv_w(i64 n, int nb, int tpw, const vo
 // 4 + 4 dot products reduced by the butterfly and broadcast with readlane.
 // Column sums are kept in this lane's row layout (columns 2 lane, 2 lane + 1);
 // the dense tiles' butterfly-layout sums are moved there with one gather at
-// each flush.
+// each flush.  Waves take contiguous ranges of stored tiles that carry about
+// the same number of bytes (wstart, from the tiles' ranks), so dense tiles
+// and long factor lists do not leave some waves with several times the work
+// of others.
 __device__ inline double2 cfma(double2 acc, double2 a, double2 b) {
   acc.x += a.x * b.x - a.y * b.y;
   acc.y += a.x * b.y + a.y * b.x;
@@ -483,17 +486,18 @@ __device__ inline double2 cfma(double2 acc, double2 a, double2 b) {
 
 __device__ inline double2 bcast_lane(double2 v, int l) { return make_double2(rl(v.x, l), rl(v.y, l)); }
 
-__global__ __launch_bounds__(256, 3) void k_symv_h(i64 n, int nb, int tpw, const void* __restrict__ zp,
-                                                const int* __restrict__ rank, const float2* __restrict__ fac,
-                                                int rmax, const double2* __restrict__ x,
-                                                double2* __restrict__ rowpart, double2* __restrict__ colpart) {
+__global__ __launch_bounds__(256, 3) void k_symv_h(i64 n, int nb, int W, const int* __restrict__ wstart,
+                                                const void* __restrict__ zp, const int* __restrict__ rank,
+                                                const float2* __restrict__ fac, int rmax,
+                                                const double2* __restrict__ x, double2* __restrict__ rowpart,
+                                                double2* __restrict__ colpart) {
   constexpr i64 kTileF4 = (i64)kPT * kPT * sizeof(float2) / 16;
   const int lane = threadIdx.x & 63;
   const i64 wave = (i64)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const i64 T = (i64)nb * (nb + 1) / 2;
-  const i64 t0 = wave * tpw;
-  if (t0 >= T) return;
-  const i64 t1 = min(t0 + (i64)tpw, T);
+  if (wave >= W) return;
+  // this wave's range of stored tiles (about the same bytes for every wave)
+  const i64 t0 = wstart[wave], t1 = wstart[wave + 1];
+  if (t0 >= t1) return;
   int I, J;
   tile_ij(t0, I, J);
   double2 accC0 = make_double2(0.0, 0.0), accC1 = make_double2(0.0, 0.0);
@@ -574,7 +578,8 @@ __global__ __launch_bounds__(256, 3) void k_symv_h(i64 n, int nb, int tpw, const
 //              + sum over the column runs of tile column R of their colpart
 // (a run ends at tile t = R (R + 1) / 2 + I when t + 1 is a multiple of tpw
 // or I = R); fixed order, deterministic.
-__global__ __launch_bounds__(256) void k_symv_w_reduce(i64 n, int nb, int tpw, const double2* __restrict__ rowpart,
+__global__ __launch_bounds__(256) void k_symv_w_reduce(i64 n, int nb, int tpw, const unsigned char* __restrict__ rend,
+                                                       const double2* __restrict__ rowpart,
                                                        const double2* __restrict__ colpart, double2* __restrict__ y) {
   // 32 rows per block, 8 groups sharing the tiles of a row block round robin;
   // the groups' sums are added in a fixed order
@@ -606,7 +611,7 @@ __global__ __launch_bounds__(256) void k_symv_w_reduce(i64 n, int nb, int tpw, c
     const i64 c0 = (i64)R * (R + 1) / 2;
     for (int I = grp; I <= R; I += 8) {
       const i64 t = c0 + I;
-      if (I == R || (t + 1) % tpw == 0) {
+      if (rend ? rend[t] != 0 : (I == R || (t + 1) % tpw == 0)) {
         const double2 v = colpart[t * kPT + r];
         sr += v.x;
         si += v.y;
@@ -627,39 +632,113 @@ __global__ __launch_bounds__(256) void k_symv_w_reduce(i64 n, int nb, int tpw, c
   }
 }
 
+// Bytes-balanced wave ranges of the compressed product.  cost(t): the bytes
+// a product reads for stored tile t plus a fixed 4 KB for its x loads and
+// partials; pre: inclusive prefix of cost (one block, chunked scan).
+__device__ inline i64 hcost(int r) { return r < 0 ? (i64)kPT * kPT * 8 + 4096 : (i64)2 * kPT * r * 8 + 4096; }
+
+__global__ __launch_bounds__(1024) void k_hscan(i64 T, const int* __restrict__ rank, i64* __restrict__ pre) {
+  __shared__ i64 part[1024];
+  const int tid = threadIdx.x;
+  const i64 per = (T + 1023) / 1024;
+  const i64 a = min(T, tid * per), b = min(T, a + per);
+  i64 sum = 0;
+  for (i64 t = a; t < b; ++t) sum += hcost(rank[t]);
+  part[tid] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const i64 v = tid >= o ? part[tid - o] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  i64 base = tid ? part[tid - 1] : 0;
+  for (i64 t = a; t < b; ++t) {
+    base += hcost(rank[t]);
+    pre[t] = base;
+  }
+}
+
+// rend[t] = 1 for diagonal tiles (the end of every tile column)
+__global__ void k_hflags(i64 T, unsigned char* __restrict__ rend) {
+  const i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  int I, J;
+  tile_ij(t, I, J);
+  rend[t] = I == J ? 1 : 0;
+}
+
+// wstart[w] = first tile whose prefix cost exceeds w / W of the total (the
+// tile holding that byte offset starts range w); the tile before each range
+// start ends a column run.
+__global__ void k_hranges(i64 T, int W, const i64* __restrict__ pre, int* __restrict__ wstart,
+                          unsigned char* __restrict__ rend) {
+  const i64 w = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (w > W) return;
+  if (w == 0) {
+    wstart[0] = 0;
+    return;
+  }
+  if (w == W) {
+    wstart[W] = (int)T;
+    return;
+  }
+  const i64 target = (i64)((double)pre[T - 1] * (double)w / (double)W);
+  i64 lo = 0, hi = T;  // first t with pre[t] > target
+  while (lo < hi) {
+    const i64 mid = (lo + hi) / 2;
+    if (pre[mid] > target)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  wstart[w] = (int)lo;
+  if (lo > 0) rend[lo - 1] = 1;
+}
+
 }  // namespace
 
 namespace dml {
 
-void symv_hier(i64 n, const float* z32p, const int* rank, const float* fac, int rmax, const double* x, double* y,
-               void* ws, int tpw, hipStream_t s) {
-  DM_REQUIRE(tpw >= 1 && tpw <= 1024, DM_ERR_ARGUMENT, "tiles per wave out of range");
-  const i64 nb = (n + kPT - 1) / kPT;
-  const i64 T = packed_tiles(n);
-  const i64 blocks = ((T + tpw - 1) / tpw + 3) / 4;
-  DM_REQUIRE(blocks < (1LL << 31), DM_ERR_ARGUMENT, "matrix too large for the packed product");
-  double2* rowpart = reinterpret_cast<double2*>(ws);
-  double2* colpart = rowpart + T * kPT;
-  k_symv_h<<<(unsigned)blocks, 256, 0, s>>>(n, (int)nb, tpw, z32p, rank, reinterpret_cast<const float2*>(fac), rmax,
-                                            reinterpret_cast<const double2*>(x), rowpart, colpart);
-  k_symv_w_reduce<<<(unsigned)((n + 31) / 32), 256, 0, s>>>(n, (int)nb, tpw, rowpart, colpart,
-                                                            reinterpret_cast<double2*>(y));
-  HIP_OK(hipGetLastError());
+size_t hier_ranges_bytes(i64 n, int W) {
+  return (size_t)packed_tiles(n) * (sizeof(i64) + 1) + sizeof(int) * ((size_t)W + 1) + 64;
 }
 
-std::vector<int> symv_hier_shapes(i64 n) {
-  std::vector<int> c = {1, 2, 4};
+int hier_wave_slots() {
   int dev = 0, ncu = 0, per_cu = 0;
   HIP_OK(hipGetDevice(&dev));
   HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_symv_h, 256, 0));
-  const i64 slots = (i64)std::max(per_cu, 1) * ncu * 4;
+  return std::max(per_cu, 1) * ncu * 4;
+}
+
+void hier_ranges(i64 n, const int* rank, int W, void* buf, HierRanges& hr, hipStream_t s) {
   const i64 T = packed_tiles(n);
-  for (int k = 1; k <= 4; k *= 2) {
-    const i64 t = (T + k * slots - 1) / (k * slots);
-    if (t > 4 && t <= 1024) c.push_back((int)t);
-  }
-  return c;
+  DM_REQUIRE(W >= 1 && W < (1 << 30), DM_ERR_ARGUMENT, "wave count out of range");
+  i64* pre = reinterpret_cast<i64*>(buf);
+  hr.rend = reinterpret_cast<unsigned char*>(pre + T);
+  hr.wstart = reinterpret_cast<int*>(reinterpret_cast<uintptr_t>(hr.rend + T + 15) & ~uintptr_t(15));
+  hr.W = W;
+  k_hscan<<<1, 1024, 0, s>>>(T, rank, pre);
+  k_hflags<<<(unsigned)((T + 255) / 256), 256, 0, s>>>(T, hr.rend);
+  k_hranges<<<(unsigned)((W + 1 + 255) / 256), 256, 0, s>>>(T, W, pre, hr.wstart, hr.rend);
+  HIP_OK(hipGetLastError());
+}
+
+void symv_hier(i64 n, const float* z32p, const int* rank, const float* fac, int rmax, const double* x, double* y,
+               void* ws, const HierRanges& hr, hipStream_t s) {
+  const i64 nb = (n + kPT - 1) / kPT;
+  const i64 T = packed_tiles(n);
+  const i64 blocks = ((i64)hr.W + 3) / 4;
+  DM_REQUIRE(hr.W >= 1 && blocks < (1LL << 31), DM_ERR_ARGUMENT, "wave count out of range");
+  double2* rowpart = reinterpret_cast<double2*>(ws);
+  double2* colpart = rowpart + T * kPT;
+  k_symv_h<<<(unsigned)blocks, 256, 0, s>>>(n, (int)nb, hr.W, hr.wstart, z32p, rank,
+                                            reinterpret_cast<const float2*>(fac), rmax,
+                                            reinterpret_cast<const double2*>(x), rowpart, colpart);
+  k_symv_w_reduce<<<(unsigned)((n + 31) / 32), 256, 0, s>>>(n, (int)nb, 1, hr.rend, rowpart, colpart,
+                                                            reinterpret_cast<double2*>(y));
+  HIP_OK(hipGetLastError());
 }
 
 std::vector<int> symv_shapes(i64 n, bool f32) {
@@ -703,7 +782,7 @@ void symv_packed(i64 n, const ZRef& z, const double* x, double* y, void* ws, int
       k_symv_w<1><<<(unsigned)blocks, 256, 0, s>>>(n, (int)nb, tpw, z.p, xv, rowpart, colpart);
     else
       k_symv_w<0><<<(unsigned)blocks, 256, 0, s>>>(n, (int)nb, tpw, z.p, xv, rowpart, colpart);
-    k_symv_w_reduce<<<(unsigned)((n + 31) / 32), 256, 0, s>>>(n, (int)nb, tpw, rowpart, colpart,
+    k_symv_w_reduce<<<(unsigned)((n + 31) / 32), 256, 0, s>>>(n, (int)nb, tpw, nullptr, rowpart, colpart,
                                                               reinterpret_cast<double2*>(y));
     HIP_OK(hipGetLastError());
     return;

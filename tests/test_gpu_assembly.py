@@ -335,23 +335,29 @@ def test_compressed_form_error_bound(dm, mk):
 
 
 def test_compressed_product_shapes(dm):
-    """Every tiles-per-wave shape of the compressed product gives the same
-    result (FP64 sums in another order)."""
+    """Every wave count of the compressed product (a multiple of the resident
+    wave slots; the waves take tile ranges of about equal bytes) gives the
+    same result (FP64 sums in another order); out-of-range multiples are
+    refused."""
     m = dm.make_icosphere(0.1, 3)
     r = dm.build_rwg(m)
-    Zd = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z32P, device=True)
-    Zd.compress(eps=1e-6)
+    Zd = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64P | dm.DM_Z32P | dm.DM_Z32_ROUNDED, device=True)
+    Zd.compress(eps=1e-4)
     rng = np.random.default_rng(9)
     x = rng.standard_normal(r.nedges) + 1j * rng.standard_normal(r.nedges)
     ctx = dm.default_context()
     ref = Zd.matmul(x, form=dm.DM_Z32H)
     key = f"symvh/c64/{r.nedges}"
     got = ctx.tune_get()
-    assert 1 <= got[key] <= 1024
-    for tpw in (1, 2, 3, 7, 40):
-        ctx.tune_set({key: tpw})
+    assert got[key] in (1, 2, 4, 8)
+    for mult in (1, 2, 3, 5, 64):
+        ctx.tune_set({key: mult})
         y = Zd.matmul(x, form=dm.DM_Z32H)
-        assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-13, tpw
+        assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-13, mult
+    for bad in (0, 65):
+        ctx.tune_set({key: bad})
+        with pytest.raises(Exception):
+            Zd.matmul(x, form=dm.DM_Z32H)
     ctx.tune_set(got)
 
 
