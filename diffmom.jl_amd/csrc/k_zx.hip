@@ -24,18 +24,61 @@ constexpr int ZX_BN = 64;   // right-hand sides per workgroup
 constexpr int ZX_BK = 16;   // complex columns of Z per LDS stage
 constexpr int ZX_KC = 4096; // columns per FP32 accumulation chunk
 
-// Xr, Xi: n x R (ldx) real planes of the right-hand sides (FP32)
-// part: [kchunk][R][n] complex128 partial products
+constexpr int ZX_BK2 = 32;  // complex columns of Z per stage (FP32 kernel)
+
+// Xr, Xi: np x R real planes of the right-hand sides (FP32; rows n .. np - 1
+// zero, np a multiple of 4).  part: [kchunk][R][n] complex128 partials.
+// Stages of ZX_BK2 columns: 16-byte global loads into registers for stage
+// s + 1 while stage s is multiplied out of LDS (two LDS buffers, one barrier
+// per stage).
 __global__ __launch_bounds__(256) void k_zx_mfma(i64 n, i64 ld, const float* __restrict__ Z, const float* __restrict__ Xr,
-                                                 const float* __restrict__ Xi, i64 ldx, int R, double2* __restrict__ part) {
-  __shared__ float As[ZX_BK][2 * ZX_BM + 4];
-  __shared__ float Br[ZX_BK][ZX_BN + 4];
-  __shared__ float Bi[ZX_BK][ZX_BN + 4];
+                                                 const float* __restrict__ Xi, i64 np, int R, double2* __restrict__ part) {
+  __shared__ float As[2][ZX_BK2][2 * ZX_BM + 4];
+  __shared__ float Br[2][ZX_BK2][ZX_BN + 4];
+  __shared__ float Bi[2][ZX_BK2][ZX_BN + 4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const i64 m0 = (i64)blockIdx.x * ZX_BM;   // complex row base
   const int c0 = blockIdx.y * ZX_BN;        // rhs base
   const i64 k_begin = (i64)blockIdx.z * ZX_KC;
   const i64 k_end = k_begin + ZX_KC < n ? k_begin + ZX_KC : n;
+  const int nst = (int)((k_end - k_begin + ZX_BK2 - 1) / ZX_BK2);
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  f4 ra[4], rb[2], rc[2];
+  // stage loads: A' 32 columns x 128 real rows (thread: 4 x float4); X planes
+  // 64 rhs x 32 rows (thread: 2 x float4 each)
+  auto gload = [&](i64 k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int idx = tid + 256 * q, kk = idx >> 5, v = idx & 31;
+      const i64 col = k0 + kk;
+      ra[q] = col < k_end ? *reinterpret_cast<const f4*>(Z + col * 2 * ld + 2 * m0 + 4 * v) : f4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int idx = tid + 256 * q, jj = idx >> 3, v = idx & 7;
+      const i64 row = k0 + 4 * v;
+      const int col = c0 + jj;
+      const bool ok = col < R && row < np;
+      rb[q] = ok ? *reinterpret_cast<const f4*>(Xr + row + (i64)col * np) : f4{0.f, 0.f, 0.f, 0.f};
+      rc[q] = ok ? *reinterpret_cast<const f4*>(Xi + row + (i64)col * np) : f4{0.f, 0.f, 0.f, 0.f};
+    }
+  };
+  auto lstore = [&](int b) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int idx = tid + 256 * q, kk = idx >> 5, v = idx & 31;
+      *reinterpret_cast<f4*>(&As[b][kk][4 * v]) = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int idx = tid + 256 * q, jj = idx >> 3, v = idx & 7;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        Br[b][4 * v + e][jj] = rb[q][e];
+        Bi[b][4 * v + e][jj] = rc[q][e];
+      }
+    }
+  };
   v4f accP[2][4], accQ[2][4];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -44,53 +87,43 @@ __global__ __launch_bounds__(256) void k_zx_mfma(i64 n, i64 ld, const float* __r
       accP[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
       accQ[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
     }
-  for (i64 k0 = k_begin; k0 < k_end; k0 += ZX_BK) {
-    // A' tile: ZX_BK columns x 128 real rows (columns of Z are contiguous)
-    for (int e = tid; e < ZX_BK * 2 * ZX_BM; e += 256) {
-      int kk = e / (2 * ZX_BM), rr = e % (2 * ZX_BM);
-      i64 col = k0 + kk, row2 = 2 * m0 + rr;
-      As[kk][rr] = (col < k_end && row2 < 2 * n) ? Z[col * 2 * ld + row2] : 0.f;
-    }
-    // X tiles: ZX_BK rows x 64 rhs
-    for (int e = tid; e < ZX_BK * ZX_BN; e += 256) {
-      int kk = e % ZX_BK, jj = e / ZX_BK;
-      i64 row = k0 + kk;
-      int col = c0 + jj;
-      bool ok = row < k_end && col < R;
-      Br[kk][jj] = ok ? Xr[row + (i64)col * ldx] : 0.f;
-      Bi[kk][jj] = ok ? Xi[row + (i64)col * ldx] : 0.f;
-    }
-    __syncthreads();
+  gload(k_begin);
+  lstore(0);
+  __syncthreads();
+  for (int st = 0; st < nst; ++st) {
+    const int b = st & 1;
+    if (st + 1 < nst) gload(k_begin + (i64)(st + 1) * ZX_BK2);
 #pragma unroll
-    for (int kq = 0; kq < ZX_BK / 4; ++kq) {
+    for (int kq = 0; kq < ZX_BK2 / 4; ++kq) {
       const int kk = 4 * kq + (lane >> 4);
       float a[2];
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb) a[rb] = As[kk][32 * wave + 16 * rb + (lane & 15)];
+      for (int r = 0; r < 2; ++r) a[r] = As[b][kk][32 * wave + 16 * r + (lane & 15)];
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) {
-        float br = Br[kk][16 * cb + (lane & 15)];
-        float bi = Bi[kk][16 * cb + (lane & 15)];
+        const float br = Br[b][kk][16 * cb + (lane & 15)];
+        const float bi = Bi[b][kk][16 * cb + (lane & 15)];
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb) {
-          accP[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb], br, accP[rb][cb], 0, 0, 0);
-          accQ[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb], bi, accQ[rb][cb], 0, 0, 0);
+        for (int r = 0; r < 2; ++r) {
+          accP[r][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], br, accP[r][cb], 0, 0, 0);
+          accQ[r][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], bi, accQ[r][cb], 0, 0, 0);
         }
       }
     }
+    if (st + 1 < nst) lstore(b ^ 1);
     __syncthreads();
   }
   // lane holds real rows 4*(lane/16) + r (r = 0..3) of each 16-row block and
   // column lane%16: rows (0,1) and (2,3) are the (re, im) rows of two complex rows
   double2* out = part + (i64)blockIdx.z * (i64)R * n;
 #pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
+  for (int r = 0; r < 2; ++r)
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) {
       const int col = c0 + 16 * cb + (lane & 15);
       if (col >= R) continue;
-      v4f p = accP[rb][cb], q = accQ[rb][cb];
-      const i64 real_row = 32 * wave + 16 * rb + 4 * (lane >> 4);  // even
+      v4f p = accP[r][cb], q = accQ[r][cb];
+      const i64 real_row = 32 * wave + 16 * r + 4 * (lane >> 4);  // even
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         i64 m = m0 + (real_row >> 1) + h;
@@ -116,12 +149,13 @@ __global__ void k_zx_reduce(i64 n, int R, int nk, const double2* __restrict__ pa
 }
 
 // complex128 n x R (ldx) -> FP32 real / imaginary planes (n x R)
-__global__ void k_split_planes(i64 n, int R, const double2* __restrict__ X, i64 ldx, float* __restrict__ Xr,
+// rows n .. np - 1 of each plane column are zero
+__global__ void k_split_planes(i64 n, i64 np, int R, const double2* __restrict__ X, i64 ldx, float* __restrict__ Xr,
                                float* __restrict__ Xi) {
   i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
-  if (idx >= n * R) return;
-  i64 m = idx % n, j = idx / n;
-  double2 v = X[m + j * ldx];
+  if (idx >= np * R) return;
+  i64 m = idx % np, j = idx / np;
+  double2 v = m < n ? X[m + j * ldx] : make_double2(0.0, 0.0);
   Xr[idx] = (float)v.x;
   Xi[idx] = (float)v.y;
 }
@@ -226,7 +260,8 @@ namespace dml {
 
 size_t zx_workspace_bytes(i64 n, int R) {
   i64 nk = (n + ZX_KC - 1) / ZX_KC;
-  size_t a = (size_t)nk * R * n * sizeof(double2) + 2 * (size_t)n * R * sizeof(float);
+  const i64 np = (n + 31) / 32 * 32;
+  size_t a = (size_t)nk * R * n * sizeof(double2) + 2 * (size_t)np * R * sizeof(float) + 64;
   size_t b = 2 * (size_t)n * R * sizeof(double);
   return a > b ? a : b;
 }
@@ -266,13 +301,16 @@ void zx_mfma(i64 n, i64 ld, const float* z32, const double* X, i64 ldx, int R, d
   DM_REQUIRE((ld & 1) == 0, DM_ERR_ARGUMENT, "complex64 matrix leading dimension must be even");
   i64 nk = (n + ZX_KC - 1) / ZX_KC;
   double2* part = reinterpret_cast<double2*>(ws);
-  float* Xr = reinterpret_cast<float*>(part + nk * R * n);
-  float* Xi = Xr + n * R;
+  DM_REQUIRE(ld % ZX_BM == 0 && ld >= n, DM_ERR_ARGUMENT, "complex64 matrix leading dimension must be a multiple of 64");
+  const i64 np = (n + 31) / 32 * 32;  // plane rows, zero past n (16-byte loads never cross a column)
+  float* Xr = reinterpret_cast<float*>(reinterpret_cast<uintptr_t>(part + nk * R * n + 1) & ~uintptr_t(15));
+  float* Xi = Xr + np * R;
   i64 tot = n * R;
-  k_split_planes<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, R, reinterpret_cast<const double2*>(X), ldx, Xr, Xi);
+  k_split_planes<<<(unsigned)((np * R + 255) / 256), 256, 0, s>>>(n, np, R, reinterpret_cast<const double2*>(X), ldx,
+                                                                   Xr, Xi);
   dim3 grid((unsigned)((n + ZX_BM - 1) / ZX_BM), (unsigned)((R + ZX_BN - 1) / ZX_BN), (unsigned)nk);
   DM_REQUIRE(grid.y <= 65535 && grid.z <= 65535, DM_ERR_ARGUMENT, "too many right-hand sides for one product");
-  k_zx_mfma<<<grid, 256, 0, s>>>(n, ld, z32, Xr, Xi, n, R, part);
+  k_zx_mfma<<<grid, 256, 0, s>>>(n, ld, z32, Xr, Xi, np, R, part);
   k_zx_reduce<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, R, (int)nk, part, reinterpret_cast<double2*>(Y), ldy);
   HIP_OK(hipGetLastError());
 }
