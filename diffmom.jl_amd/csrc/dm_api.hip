@@ -948,7 +948,7 @@ int dm_farfield(dm_ctx* ctx, const dm_rwg* rwg, const double* rhat, int64_t nome
     dg.alloc((size_t)2 * 3 * chunk * n);
     di.alloc((size_t)2 * n * RB);
     de.alloc((size_t)2 * 3 * chunk * RB);
-    ws.alloc((size_t)2 * n * RB);
+    ws.alloc(dml::gemm_c128_workspace_bytes(n, RB) / sizeof(double));
     for (i64 j0 = 0; j0 < ncur; j0 += RB) {
       const int R = (int)std::min<i64>(RB, ncur - j0);
       HIP_OK(hipMemcpy2DAsync(di.p, 16 * n, I + 2 * ldi * j0, 16 * ldi, 16 * n, R, hipMemcpyHostToDevice, s));

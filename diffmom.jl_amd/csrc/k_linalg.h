@@ -58,7 +58,9 @@ void hier_ranges(i64 n, const int* rank, int W, void* buf, HierRanges& hr, hipSt
 void symv_hier(i64 n, const float* z32p, const int* rank, const float* fac, int rmax, const double* x, double* y,
                void* ws, const HierRanges& hr, hipStream_t s);
 size_t zx_workspace_bytes(i64 n, int R);
-// Y (mr x R) = A (mr x n, lda) X (n x R), complex128 on FP64 MFMA; ws: 2 n R doubles.
+// Y (mr x R) = A (mr x n, lda) X (n x R), complex128 on FP64 MFMA; ws:
+// gemm_c128_workspace_bytes(n, R).
+size_t gemm_c128_workspace_bytes(i64 n, int R);
 void gemm_c128(i64 mr, i64 n, const double* A, i64 lda, const double* X, i64 ldx, int R, double* Y, i64 ldy,
                void* ws, hipStream_t s);
 void zx_mfma(i64 n, i64 ld, const float* z32, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,

@@ -167,15 +167,53 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // of many right-hand sides.  Same tiling; one K chunk (FP64 accumulation).
 // Rows mr (complex) and inner dimension n may differ: Y (mr x R) = Z (mr x n,
 // leading dimension ld) X (n x R).
-__global__ __launch_bounds__(256) void k_zx_mfma64(i64 mr, i64 n, i64 ld, const double* __restrict__ Z,
-                                                   const double* __restrict__ Xr, const double* __restrict__ Xi, i64 ldx,
+__global__ __launch_bounds__(256, 2) void k_zx_mfma64(i64 mr, i64 n, i64 ld, const double* __restrict__ Z,
+                                                   const double* __restrict__ Xr, const double* __restrict__ Xi, i64 np,
                                                    int R, double2* __restrict__ Y, i64 ldy) {
-  __shared__ double As[ZX_BK][2 * ZX_BM + 2];
-  __shared__ double Br[ZX_BK][ZX_BN + 2];
-  __shared__ double Bi[ZX_BK][ZX_BN + 2];
+  // stages of ZX_BK columns: 16-byte loads (one complex of Z, two rows of a
+  // plane) into registers for stage s + 1 while stage s is multiplied out of
+  // LDS (two buffers, one barrier per stage).  Xr, Xi: np x R, rows past n zero.
+  __shared__ double As[2][ZX_BK][2 * ZX_BM + 2];
+  __shared__ double Br[2][ZX_BK][ZX_BN + 2];
+  __shared__ double Bi[2][ZX_BK][ZX_BN + 2];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const i64 m0 = (i64)blockIdx.x * ZX_BM;
   const int c0 = blockIdx.y * ZX_BN;
+  const int nst = (int)((n + ZX_BK - 1) / ZX_BK);
+  double2 ra[4], rb[2], rc[2];
+  auto gload = [&](i64 k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int idx = tid + 256 * q, kk = idx >> 6, v = idx & 63;
+      const i64 col = k0 + kk, m = m0 + v;
+      ra[q] = (col < n && m < mr) ? *reinterpret_cast<const double2*>(Z + col * 2 * ld + 2 * m) : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int idx = tid + 256 * q, jj = idx >> 3, v = idx & 7;
+      const i64 row = k0 + 2 * v;
+      const int col = c0 + jj;
+      const bool ok = col < R && row < np;
+      rb[q] = ok ? *reinterpret_cast<const double2*>(Xr + row + (i64)col * np) : make_double2(0.0, 0.0);
+      rc[q] = ok ? *reinterpret_cast<const double2*>(Xi + row + (i64)col * np) : make_double2(0.0, 0.0);
+    }
+  };
+  auto lstore = [&](int b) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int idx = tid + 256 * q, kk = idx >> 6, v = idx & 63;
+      As[b][kk][2 * v] = ra[q].x;
+      As[b][kk][2 * v + 1] = ra[q].y;
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int idx = tid + 256 * q, jj = idx >> 3, v = idx & 7;
+      Br[b][2 * v][jj] = rb[q].x;
+      Br[b][2 * v + 1][jj] = rb[q].y;
+      Bi[b][2 * v][jj] = rc[q].x;
+      Bi[b][2 * v + 1][jj] = rc[q].y;
+    }
+  };
   v4d accP[2][4], accQ[2][4];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -184,38 +222,30 @@ __global__ __launch_bounds__(256) void k_zx_mfma64(i64 mr, i64 n, i64 ld, const 
       accP[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
       accQ[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
     }
-  for (i64 k0 = 0; k0 < n; k0 += ZX_BK) {
-    for (int e = tid; e < ZX_BK * 2 * ZX_BM; e += 256) {
-      int kk = e / (2 * ZX_BM), rr = e % (2 * ZX_BM);
-      i64 col = k0 + kk, row2 = 2 * m0 + rr;
-      As[kk][rr] = (col < n && row2 < 2 * mr) ? Z[col * 2 * ld + row2] : 0.0;
-    }
-    for (int e = tid; e < ZX_BK * ZX_BN; e += 256) {
-      int kk = e % ZX_BK, jj = e / ZX_BK;
-      i64 row = k0 + kk;
-      int col = c0 + jj;
-      bool ok = row < n && col < R;
-      Br[kk][jj] = ok ? Xr[row + (i64)col * ldx] : 0.0;
-      Bi[kk][jj] = ok ? Xi[row + (i64)col * ldx] : 0.0;
-    }
-    __syncthreads();
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int st = 0; st < nst; ++st) {
+    const int b = st & 1;
+    if (st + 1 < nst) gload((i64)(st + 1) * ZX_BK);
 #pragma unroll
     for (int kq = 0; kq < ZX_BK / 4; ++kq) {
       const int kk = 4 * kq + (lane >> 4);
       double a[2];
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb) a[rb] = As[kk][32 * wave + 16 * rb + (lane & 15)];
+      for (int r = 0; r < 2; ++r) a[r] = As[b][kk][32 * wave + 16 * r + (lane & 15)];
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) {
-        double br = Br[kk][16 * cb + (lane & 15)];
-        double bi = Bi[kk][16 * cb + (lane & 15)];
+        const double br = Br[b][kk][16 * cb + (lane & 15)];
+        const double bi = Bi[b][kk][16 * cb + (lane & 15)];
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb) {
-          accP[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], br, accP[rb][cb], 0, 0, 0);
-          accQ[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], bi, accQ[rb][cb], 0, 0, 0);
+        for (int r = 0; r < 2; ++r) {
+          accP[r][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], br, accP[r][cb], 0, 0, 0);
+          accQ[r][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], bi, accQ[r][cb], 0, 0, 0);
         }
       }
     }
+    if (st + 1 < nst) lstore(b ^ 1);
     __syncthreads();
   }
   // FP64 MFMA output layout: lane l holds real rows (l / 16) + 4 r (r = 0..3)
@@ -223,11 +253,11 @@ __global__ __launch_bounds__(256) void k_zx_mfma64(i64 mr, i64 n, i64 ld, const 
   // complex row sit in lanes l and l ^ 16: swap, then the even lane writes.
   const int g = lane >> 4;
 #pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
+  for (int r2 = 0; r2 < 2; ++r2)
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) {
       const int col = c0 + 16 * cb + (lane & 15);
-      v4d p = accP[rb][cb], q = accQ[rb][cb];
+      v4d p = accP[r2][cb], q = accQ[r2][cb];
       double po[4], qo[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -237,19 +267,19 @@ __global__ __launch_bounds__(256) void k_zx_mfma64(i64 mr, i64 n, i64 ld, const 
       if (col >= R || (g & 1)) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const i64 m = m0 + ((32 * wave + 16 * rb + g + 4 * r) >> 1);
+        const i64 m = m0 + ((32 * wave + 16 * r2 + g + 4 * r) >> 1);
         if (m >= mr) continue;
         Y[m + (i64)col * ldy] = make_double2(p[r] - qo[r], q[r] + po[r]);
       }
     }
 }
 
-__global__ void k_split_planes64(i64 n, int R, const double2* __restrict__ X, i64 ldx, double* __restrict__ Xr,
-                                 double* __restrict__ Xi) {
+__global__ void k_split_planes64(i64 n, i64 np, int R, const double2* __restrict__ X, i64 ldx,
+                                 double* __restrict__ Xr, double* __restrict__ Xi) {
   i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
-  if (idx >= n * R) return;
-  i64 m = idx % n, j = idx / n;
-  double2 v = X[m + j * ldx];
+  if (idx >= np * R) return;
+  i64 m = idx % np, j = idx / np;
+  double2 v = m < n ? X[m + j * ldx] : make_double2(0.0, 0.0);
   Xr[idx] = v.x;
   Xi[idx] = v.y;
 }
@@ -262,35 +292,37 @@ size_t zx_workspace_bytes(i64 n, int R) {
   i64 nk = (n + ZX_KC - 1) / ZX_KC;
   const i64 np = (n + 31) / 32 * 32;
   size_t a = (size_t)nk * R * n * sizeof(double2) + 2 * (size_t)np * R * sizeof(float) + 64;
-  size_t b = 2 * (size_t)n * R * sizeof(double);
+  size_t b = 2 * (size_t)((n + 15) / 16 * 16) * R * sizeof(double);
   return a > b ? a : b;
 }
+
+size_t gemm_c128_workspace_bytes(i64 n, int R) { return 2 * (size_t)((n + 15) / 16 * 16) * R * sizeof(double); }
 
 // Y = Z64 X with FP64 MFMA (complex128 everything).
 void zx_mfma64(i64 n, i64 ld, const double* z64, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
                hipStream_t s) {
+  const i64 np = (n + 15) / 16 * 16;
   double* Xr = reinterpret_cast<double*>(ws);
-  double* Xi = Xr + n * R;
-  i64 tot = n * R;
-  k_split_planes64<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, R, reinterpret_cast<const double2*>(X), ldx, Xr,
-                                                                  Xi);
+  double* Xi = Xr + np * R;
+  k_split_planes64<<<(unsigned)((np * R + 255) / 256), 256, 0, s>>>(n, np, R, reinterpret_cast<const double2*>(X), ldx,
+                                                                     Xr, Xi);
   dim3 grid((unsigned)((n + ZX_BM - 1) / ZX_BM), (unsigned)((R + ZX_BN - 1) / ZX_BN), 1);
   DM_REQUIRE(grid.y <= 65535, DM_ERR_ARGUMENT, "too many right-hand sides for one product");
-  k_zx_mfma64<<<grid, 256, 0, s>>>(n, n, ld, z64, Xr, Xi, n, R, reinterpret_cast<double2*>(Y), ldy);
+  k_zx_mfma64<<<grid, 256, 0, s>>>(n, n, ld, z64, Xr, Xi, np, R, reinterpret_cast<double2*>(Y), ldy);
   HIP_OK(hipGetLastError());
 }
 
 void gemm_c128(i64 mr, i64 n, const double* A, i64 lda, const double* X, i64 ldx, int R, double* Y, i64 ldy,
                void* ws, hipStream_t s) {
   DM_REQUIRE(lda >= mr && ldx >= n && ldy >= mr && R >= 1, DM_ERR_ARGUMENT, "complex GEMM: bad dimensions");
+  const i64 np = (n + 15) / 16 * 16;
   double* Xr = reinterpret_cast<double*>(ws);
-  double* Xi = Xr + n * R;
-  const i64 tot = n * R;
-  k_split_planes64<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, R, reinterpret_cast<const double2*>(X), ldx, Xr,
-                                                                  Xi);
+  double* Xi = Xr + np * R;
+  k_split_planes64<<<(unsigned)((np * R + 255) / 256), 256, 0, s>>>(n, np, R, reinterpret_cast<const double2*>(X), ldx,
+                                                                     Xr, Xi);
   dim3 grid((unsigned)((mr + ZX_BM - 1) / ZX_BM), (unsigned)((R + ZX_BN - 1) / ZX_BN), 1);
   DM_REQUIRE(grid.x < (1u << 31) && grid.y <= 65535, DM_ERR_ARGUMENT, "complex GEMM too large for one launch");
-  k_zx_mfma64<<<grid, 256, 0, s>>>(mr, n, lda, A, Xr, Xi, n, R, reinterpret_cast<double2*>(Y), ldy);
+  k_zx_mfma64<<<grid, 256, 0, s>>>(mr, n, lda, A, Xr, Xi, np, R, reinterpret_cast<double2*>(Y), ldy);
   HIP_OK(hipGetLastError());
 }
 
