@@ -486,7 +486,7 @@ __device__ inline double2 cfma(double2 acc, double2 a, double2 b) {
 
 __device__ inline double2 bcast_lane(double2 v, int l) { return make_double2(rl(v.x, l), rl(v.y, l)); }
 
-__global__ __launch_bounds__(256, 3) void k_symv_h(i64 n, int nb, int W, const int* __restrict__ wstart,
+__global__ __launch_bounds__(256, 2) void k_symv_h(i64 n, int nb, int W, const int* __restrict__ wstart,
                                                 const void* __restrict__ zp, const int* __restrict__ rank,
                                                 const float2* __restrict__ fac, int rmax,
                                                 const double2* __restrict__ x, double2* __restrict__ rowpart,
@@ -516,9 +516,9 @@ __global__ __launch_bounds__(256, 3) void k_symv_h(i64 n, int nb, int W, const i
     } else {
       const f4v* U = reinterpret_cast<const f4v*>(fac + t * 2 * kPT * (i64)rmax);
       const f4v* V = U + (i64)rmax * kPT / 2;
-#pragma unroll 1
-      for (int k0 = 0; k0 < rk; k0 += 4) {
-        f4v u[4], v[4];
+      // factor columns 4 at a time, the next group's loads issued before the
+      // current group's butterflies (two register sets, ping-pong)
+      auto fload = [&](int k0, f4v(&u)[4], f4v(&v)[4]) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           if (k0 + c < rk) {
@@ -529,6 +529,8 @@ __global__ __launch_bounds__(256, 3) void k_symv_h(i64 n, int nb, int W, const i
             v[c] = f4v{0.f, 0.f, 0.f, 0.f};
           }
         }
+      };
+      auto fstep = [&](const f4v(&u)[4], const f4v(&v)[4]) {
         double2 ps[4], pr[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -547,6 +549,16 @@ __global__ __launch_bounds__(256, 3) void k_symv_h(i64 n, int nb, int W, const i
           accJ[0] = cfma(accJ[0], make_double2(v[c][0], v[c][1]), sc);
           accJ[1] = cfma(accJ[1], make_double2(v[c][2], v[c][3]), sc);
         }
+      };
+      f4v ua[4], va[4], ub[4], vb[4];
+      if (rk > 0) fload(0, ua, va);
+#pragma unroll 1
+      for (int k0 = 0; k0 < rk; k0 += 8) {
+        if (k0 + 4 < rk) fload(k0 + 4, ub, vb);
+        fstep(ua, va);
+        if (k0 + 4 >= rk) break;
+        if (k0 + 8 < rk) fload(k0 + 8, ua, va);
+        fstep(ub, vb);
       }
     }
 #pragma unroll
