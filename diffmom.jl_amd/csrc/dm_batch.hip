@@ -2,9 +2,9 @@
 // incidence angles of one scatterer).  One Z X product per Krylov step serves
 // every column of the batch (MFMA kernel, Z streamed once per 64 columns);
 // orthogonalisation is FP64 CGS2 per column, with each column's Hessenberg
-// matrix, Givens rotations and stopping test kept on the host.  The
-// per-column arithmetic is that of the single-column solver in dm_solver.hip,
-// so a batch of one reproduces it.
+// matrix, Givens rotations and stopping test on the device (one block per
+// column).  The per-column arithmetic is that of the single-column solver in
+// dm_solver.hip, so a batch of one reproduces it.
 #include <algorithm>
 #include <cmath>
 
@@ -47,21 +47,6 @@ struct BatchKrylov {
   }
 };
 
-// Per-column GMRES state of one cycle.
-struct Column {
-  std::vector<cd> H, g, sn;
-  std::vector<double> cs;
-  int j = 0;  // Arnoldi steps taken in this cycle
-  void reset(int m, double beta) {
-    H.assign((size_t)(m + 1) * m, cd(0));
-    g.assign(m + 1, cd(0));
-    sn.assign(m, cd(0));
-    cs.assign(m, 0.0);
-    g[0] = beta;
-    j = 0;
-  }
-};
-
 std::vector<double> col_norms(BatchKrylov& K, const double* W, hipStream_t s) {
   std::vector<double> out(K.R);
   dmb::norm2(K.n, K.R, W, nullptr, K.nr.p, s);
@@ -101,23 +86,33 @@ void gmres_batch(dm_ctx* ctx, i64 n, int R, DevOp& A, DevPrec* M, const double* 
     else
       solved[r] = 1;
   }
-  std::vector<Column> col(R);
-  std::vector<int> act_h(R), cnt_h(R);
-  std::vector<double> sc_h(R), nr_h(R);
-  std::vector<cd> hh1((size_t)R * K.kcap), hh2((size_t)R * K.kcap), yh((size_t)R * K.kcap);
+  // device Givens state of every column (see dmb::givens)
+  DBuf<double> Hd, csd, snd, gd, epsd, betad;
+  DBuf<int> jcnt, its, sol;
+  Hd.alloc((size_t)2 * R * (mcap + 1) * mcap);
+  csd.alloc((size_t)R * mcap);
+  snd.alloc((size_t)2 * R * mcap);
+  gd.alloc((size_t)2 * R * (mcap + 1));
+  epsd.upload(eps, s);
+  betad.alloc(R);
+  jcnt.alloc(R);
+  its.alloc(R);
+  sol.alloc(R);
+  std::vector<int> act_h(R), its_h(R), sol_h(R);
   int total = 0;
   auto any_active = [&]() { return std::any_of(active.begin(), active.end(), [](char a) { return a != 0; }); };
   while (any_active() && total < maxiter) {
-    for (int r = 0; r < R; ++r) {
-      act_h[r] = active[r];
-      sc_h[r] = active[r] ? 1.0 / beta[r] : 0.0;
-      if (active[r]) col[r].reset(mcap, beta[r]);
-      else col[r].j = 0;
-    }
-    HIP_OK(hipMemcpyAsync(K.sc.p, sc_h.data(), sizeof(double) * R, hipMemcpyHostToDevice, s));
+    for (int r = 0; r < R; ++r) act_h[r] = active[r];
     HIP_OK(hipMemcpyAsync(K.act.p, act_h.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(betad.p, beta.data(), sizeof(double) * R, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemsetAsync(gd.p, 0, gd.bytes(), s));
+    HIP_OK(hipMemsetAsync(sol.p, 0, sizeof(int) * R, s));
+    dmb::givens_init(R, mcap, betad.p, K.act.p, gd.p, K.sc.p, jcnt.p, its.p, s);
     dmb::scale(n, R, K.V.p, K.sc.p, K.V.p, s);
-    for (int j = 0; j < mcap && total < maxiter && any_active(); ++j) {
+    // steps are queued without a host round trip; the host reads the active
+    // flags every 4 steps (columns that converge in between are masked on
+    // the device, so only the Z X of a fully converged batch is wasted)
+    for (int j = 0; j < mcap && total < maxiter; ++j) {
       double* vj = vblk(K.V.p, j);
       if (M) {
         double* zj = vblk(K.Zs.p, j);
@@ -139,76 +134,31 @@ void gmres_batch(dm_ctx* ctx, i64 n, int R, DevOp& A, DevPrec* M, const double* 
         dmb::dots(n, R, k, K.kcap, K.V.p, K.W.p, K.act.p, K.part.p, h2, s);
         dmb::update(n, R, k, K.kcap, K.V.p, h2, K.act.p, K.W.p, s);
         dmb::norm2(n, R, K.W.p, K.act.p, K.nr.p, s);
-      }
-      const size_t pitch = sizeof(cd) * K.kcap;
-      HIP_OK(hipMemcpy2DAsync(hh1.data(), pitch, h1, pitch, sizeof(cd) * k, R, hipMemcpyDeviceToHost, s));
-      HIP_OK(hipMemcpy2DAsync(hh2.data(), pitch, h2, pitch, sizeof(cd) * k, R, hipMemcpyDeviceToHost, s));
-      HIP_OK(hipMemcpyAsync(nr_h.data(), K.nr.p, sizeof(double) * R, hipMemcpyDeviceToHost, s));
-      HIP_OK(hipStreamSynchronize(s));
-      for (int r = 0; r < R; ++r) {
-        sc_h[r] = 0.0;
-        if (!active[r]) continue;
-        Column& c = col[r];
-        cd* Hj = &c.H[(size_t)j * (mcap + 1)];
-        const cd* a1 = &hh1[(size_t)r * K.kcap];
-        const cd* a2 = &hh2[(size_t)r * K.kcap];
-        for (int i = 0; i < k; ++i) Hj[i] = a1[i] + a2[i];
-        const double hn = std::sqrt(nr_h[r]);
-        Hj[k] = hn;
-        for (int i = 0; i < j; ++i) {
-          cd t = c.cs[i] * Hj[i] + c.sn[i] * Hj[i + 1];
-          Hj[i + 1] = -std::conj(c.sn[i]) * Hj[i] + c.cs[i] * Hj[i + 1];
-          Hj[i] = t;
-        }
-        cd a = Hj[j], bb = Hj[j + 1];
-        double rho = std::hypot(std::abs(a), std::abs(bb));
-        if (std::abs(a) == 0) {
-          c.cs[j] = 0;
-          c.sn[j] = std::abs(bb) > 0 ? std::conj(bb) / std::abs(bb) : cd(1);
-        } else {
-          c.cs[j] = std::abs(a) / rho;
-          c.sn[j] = (a / std::abs(a)) * std::conj(bb) / rho;
-        }
-        Hj[j] = c.cs[j] * a + c.sn[j] * bb;
-        Hj[j + 1] = 0;
-        c.g[j + 1] = -std::conj(c.sn[j]) * c.g[j];
-        c.g[j] = c.cs[j] * c.g[j];
-        c.j = k;
-        iters[r] += 1;
-        const double est = std::abs(c.g[j + 1]);
-        if (est <= eps[r] || hn == 0) {
-          if (est <= eps[r]) solved[r] = 1;
-          active[r] = 0;
-        } else {
-          sc_h[r] = 1.0 / hn;
-        }
+        dmb::givens(R, j, mcap, K.kcap, h1, h2, K.nr.p, Hd.p, csd.p, snd.p, gd.p, K.sc.p, K.act.p, epsd.p, jcnt.p,
+                    its.p, sol.p, s);
+        // v_{j+1} = w / ||w|| for the columns still iterating (sc = 0 otherwise)
+        if (k <= mcap) dmb::scale(n, R, K.W.p, K.sc.p, vblk(K.V.p, k), s);
       }
       ++total;
-      // v_{j+1} = w / ||w|| for the columns still iterating
-      for (int r = 0; r < R; ++r) act_h[r] = active[r];
-      HIP_OK(hipMemcpyAsync(K.sc.p, sc_h.data(), sizeof(double) * R, hipMemcpyHostToDevice, s));
-      HIP_OK(hipMemcpyAsync(K.act.p, act_h.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
-      if (k <= mcap) dmb::scale(n, R, K.W.p, K.sc.p, vblk(K.V.p, k), s);
-    }
-    // x_r += (Zs or V) y_r with y_r = H_r^-1 g_r over the column's own steps
-    std::fill(yh.begin(), yh.end(), cd(0));
-    int kmax = 0;
-    for (int r = 0; r < R; ++r) {
-      Column& c = col[r];
-      cnt_h[r] = c.j;
-      kmax = std::max(kmax, c.j);
-      cd* yr = &yh[(size_t)r * K.kcap];
-      for (int i = c.j - 1; i >= 0; --i) {
-        cd acc = c.g[i];
-        for (int l = i + 1; l < c.j; ++l) acc -= c.H[(size_t)l * (mcap + 1) + i] * yr[l];
-        yr[i] = acc / c.H[(size_t)i * (mcap + 1) + i];
+      if (k % 4 == 0 || k == mcap || total >= maxiter) {
+        HIP_OK(hipMemcpyAsync(act_h.data(), K.act.p, sizeof(int) * R, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        bool any = false;
+        for (int r = 0; r < R; ++r) any = any || act_h[r] != 0;
+        if (!any) break;
       }
     }
-    if (kmax > 0) {
-      HIP_OK(hipMemcpyAsync(K.y.p, yh.data(), sizeof(cd) * yh.size(), hipMemcpyHostToDevice, s));
-      HIP_OK(hipMemcpyAsync(K.cnt.p, cnt_h.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
-      dmb::combine(n, R, K.kcap, M ? K.Zs.p : K.V.p, K.y.p, K.cnt.p, X, s);
+    HIP_OK(hipMemcpyAsync(its_h.data(), its.p, sizeof(int) * R, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(sol_h.data(), sol.p, sizeof(int) * R, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    for (int r = 0; r < R; ++r) {
+      iters[r] += its_h[r];
+      if (sol_h[r]) solved[r] = 1;
+      active[r] = 0;
     }
+    // x_r += (Zs or V) y_r with y_r = H_r^-1 g_r over the column's own steps
+    dmb::backsolve(R, mcap, K.kcap, Hd.p, gd.p, jcnt.p, K.y.p, s);
+    dmb::combine(n, R, K.kcap, M ? K.Zs.p : K.V.p, K.y.p, jcnt.p, X, s);
     // a column that filled its basis (or broke down) unconverged restarts
     // from its true residual, as the single-column solver does
     bool pending = false;

@@ -7,6 +7,8 @@
 // coefficient i at r * kcap + i.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "dm_common.h"
 #include "k_batch.h"
 
@@ -166,6 +168,127 @@ unsigned row_blocks(i64 n) { return (unsigned)((n + BT - 1) / BT); }
 
 }  // namespace
 
+namespace {
+
+// Per-column state of the device Givens step: column r's Hessenberg matrix at
+// H + r ldh m (ldh = m + 1 rows, m columns), rotations at cs / sn + r m,
+// rotated right-hand side at g + r ldh.
+__global__ void k_bg_init(int R, int ldh, const double* __restrict__ beta, const int* __restrict__ act,
+                          double2* __restrict__ g, double* __restrict__ sc, int* __restrict__ jcnt,
+                          int* __restrict__ its) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  g[(i64)r * ldh] = make_double2(act[r] ? beta[r] : 0.0, 0.0);
+  sc[r] = act[r] && beta[r] > 0 ? 1.0 / beta[r] : 0.0;
+  jcnt[r] = 0;
+  its[r] = 0;
+}
+
+// Step j of every active column (one block each): the Hessenberg column from
+// the two CGS2 passes (h1, h2: pitch kcap) and ||w||^2 (nr), previous
+// rotations applied from LDS by one thread, the new rotation, g updated; a
+// column whose estimate |g[j+1]| reaches eps[r] (or that breaks down) is
+// deactivated.  sc[r] = 1 / h_{j+1,j} for the next basis vector (0 when
+// inactive).
+__global__ __launch_bounds__(64) void k_bgivens(int j, int m, int kcap, const double2* __restrict__ h1,
+                                                const double2* __restrict__ h2, const double* __restrict__ nr,
+                                                double2* __restrict__ H, double* __restrict__ cs,
+                                                double2* __restrict__ sn, double2* __restrict__ g,
+                                                double* __restrict__ sc, int* __restrict__ act,
+                                                const double* __restrict__ eps, int* __restrict__ jcnt,
+                                                int* __restrict__ its, int* __restrict__ solved) {
+  const int r = blockIdx.x;
+  if (!act[r]) return;
+  const int k = j + 1, ldh = m + 1;
+  extern __shared__ double2 sh[];
+  double2* col = sh;         // k + 1
+  double2* rs = sh + k + 1;  // j
+  double* rc = reinterpret_cast<double*>(rs + j);
+  const double2* a1 = h1 + (i64)r * kcap;
+  const double2* a2 = h2 + (i64)r * kcap;
+  for (int i = threadIdx.x; i < k; i += blockDim.x) col[i] = make_double2(a1[i].x + a2[i].x, a1[i].y + a2[i].y);
+  for (int i = threadIdx.x; i < j; i += blockDim.x) {
+    rs[i] = sn[(i64)r * m + i];
+    rc[i] = cs[(i64)r * m + i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double hn = sqrt(nr[r]);
+    double2 a = col[0];
+    for (int i = 0; i < j; ++i) {
+      const double c = rc[i];
+      const double2 s = rs[i], b = col[i + 1];
+      col[i] = make_double2(c * a.x + s.x * b.x - s.y * b.y, c * a.y + s.x * b.y + s.y * b.x);
+      a = make_double2(-(s.x * a.x + s.y * a.y) + c * b.x, -(s.x * a.y - s.y * a.x) + c * b.y);
+    }
+    const double bb = hn, aa = hypot(a.x, a.y), rho = hypot(aa, bb);
+    double c;
+    double2 s;
+    if (aa == 0) {
+      c = 0;
+      s = make_double2(1.0, 0.0);
+    } else {
+      c = aa / rho;
+      s = make_double2(a.x / aa * bb / rho, a.y / aa * bb / rho);
+    }
+    cs[(i64)r * m + j] = c;
+    sn[(i64)r * m + j] = s;
+    col[j] = make_double2(c * a.x + s.x * bb, c * a.y + s.y * bb);
+    col[k] = make_double2(0.0, 0.0);
+    double2* gr = g + (i64)r * ldh;
+    const double2 gj = gr[j];
+    const double2 gk = make_double2(-(s.x * gj.x + s.y * gj.y), -(s.x * gj.y - s.y * gj.x));
+    gr[k] = gk;
+    gr[j] = make_double2(c * gj.x, c * gj.y);
+    jcnt[r] = k;
+    its[r] += 1;
+    const double est = hypot(gk.x, gk.y);
+    if (est <= eps[r] || hn == 0) {
+      if (est <= eps[r]) solved[r] = 1;
+      act[r] = 0;
+      sc[r] = 0.0;
+    } else {
+      sc[r] = 1.0 / hn;
+    }
+  }
+  __syncthreads();
+  double2* Hj = H + (i64)r * ldh * m + (i64)j * ldh;
+  for (int i = threadIdx.x; i <= k; i += blockDim.x) Hj[i] = col[i];
+}
+
+// y_r = R_r^{-1} g_r over the column's jcnt[r] steps (one block per column).
+__global__ __launch_bounds__(64) void k_bbacksolve(int m, int kcap, const double2* __restrict__ H,
+                                                   const double2* __restrict__ g, const int* __restrict__ jcnt,
+                                                   double2* __restrict__ y) {
+  const int r = blockIdx.x, jn = jcnt[r], ldh = m + 1;
+  extern __shared__ double2 rhs[];
+  __shared__ double2 yi_s;
+  const double2* Hr = H + (i64)r * ldh * m;
+  double2* yr = y + (i64)r * kcap;
+  for (int i = threadIdx.x; i < jn; i += blockDim.x) rhs[i] = g[(i64)r * ldh + i];
+  __syncthreads();
+  for (int i = jn - 1; i >= 0; --i) {
+    if (threadIdx.x == 0) {
+      const double2 d = Hr[(i64)i * ldh + i], acc = rhs[i];
+      const double dd = d.x * d.x + d.y * d.y;
+      const double2 yi = make_double2((acc.x * d.x + acc.y * d.y) / dd, (acc.y * d.x - acc.x * d.y) / dd);
+      yi_s = yi;
+      yr[i] = yi;
+    }
+    __syncthreads();
+    const double2 yi = yi_s;
+    const double2* hc = Hr + (i64)i * ldh;
+    for (int l = threadIdx.x; l < i; l += blockDim.x) {
+      const double2 h = hc[l];
+      rhs[l].x -= h.x * yi.x - h.y * yi.y;
+      rhs[l].y -= h.x * yi.y + h.y * yi.x;
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
 namespace dmb {
 
 int max_basis() { return BKMAX; }
@@ -206,6 +329,30 @@ void combine(i64 n, int R, int kcap, const double* V, const double* y, const int
 
 void norm2(i64 n, int R, const double* W, const int* act, double* out, hipStream_t s) {
   k_bnorm2<<<R, 1024, 0, s>>>(n, reinterpret_cast<const double2*>(W), act, out);
+  HIP_OK(hipGetLastError());
+}
+
+void givens_init(int R, int m, const double* beta, const int* act, double* g, double* sc, int* jcnt, int* its,
+                 hipStream_t s) {
+  k_bg_init<<<(R + 63) / 64, 64, 0, s>>>(R, m + 1, beta, act, reinterpret_cast<double2*>(g), sc, jcnt, its);
+  HIP_OK(hipGetLastError());
+}
+
+void givens(int R, int j, int m, int kcap, const double* h1, const double* h2, const double* nr, double* H,
+            double* cs, double* sn, double* g, double* sc, int* act, const double* eps, int* jcnt, int* its,
+            int* solved, hipStream_t s) {
+  DM_REQUIRE(j < m && m < BKMAX, DM_ERR_ARGUMENT, "Krylov basis too large for the device Givens step");
+  const size_t smem = sizeof(double2) * (2 * (size_t)j + 2) + sizeof(double) * j;
+  k_bgivens<<<R, 64, smem, s>>>(j, m, kcap, reinterpret_cast<const double2*>(h1), reinterpret_cast<const double2*>(h2),
+                                nr, reinterpret_cast<double2*>(H), cs, reinterpret_cast<double2*>(sn),
+                                reinterpret_cast<double2*>(g), sc, act, eps, jcnt, its, solved);
+  HIP_OK(hipGetLastError());
+}
+
+void backsolve(int R, int m, int kcap, const double* H, const double* g, const int* jcnt, double* y, hipStream_t s) {
+  const size_t smem = sizeof(double2) * (size_t)std::max(m, 1);
+  k_bbacksolve<<<R, 64, smem, s>>>(m, kcap, reinterpret_cast<const double2*>(H), reinterpret_cast<const double2*>(g),
+                                   jcnt, reinterpret_cast<double2*>(y));
   HIP_OK(hipGetLastError());
 }
 
