@@ -62,7 +62,8 @@ std::vector<double> col_norms(BatchKrylov& K, const double* W, hipStream_t s) {
 // steps of each column to iters[r]; solved[r] reports convergence.
 void gmres_batch(dm_ctx* ctx, i64 n, int R, DevOp& A, DevPrec* M, const double* B, double* X,
                  const std::vector<double>& rtol, const std::vector<char>& live, int maxiter, int restart,
-                 BatchKrylov& K, hipStream_t s, std::vector<int>& iters, std::vector<char>& solved) {
+                 BatchKrylov& K, hipStream_t s, std::vector<int>& iters, std::vector<char>& solved,
+                 bool x_zero = false) {
   int mcap = restart > 0 ? std::min(restart, maxiter) : maxiter;
   mcap = std::min(mcap, dmb::max_basis() - 1);
   K.reserve(n, R, mcap, M != nullptr, A.batch_ws(R));
@@ -70,10 +71,14 @@ void gmres_batch(dm_ctx* ctx, i64 n, int R, DevOp& A, DevPrec* M, const double* 
   auto vblk = [&](double* base, int j) { return base + blk * j; };
   solved.assign(R, 0);
   // r0 = b - A x0
+  // (a zero initial guess needs no product: r0 = b)
   auto residual = [&]() {
-    A.apply_batch(X, K.T.p, R, K.ws.p, s);
     HIP_OK(hipMemcpyAsync(K.V.p, B, 8 * blk, hipMemcpyDeviceToDevice, s));
-    dml::axpby(n * R, -1, 0, K.T.p, 1, 0, K.V.p, s);
+    if (!x_zero) {
+      A.apply_batch(X, K.T.p, R, K.ws.p, s);
+      dml::axpby(n * R, -1, 0, K.T.p, 1, 0, K.V.p, s);
+    }
+    x_zero = false;
     return col_norms(K, K.V.p, s);
   };
   std::vector<double> beta = residual(), eps(R);
@@ -215,7 +220,7 @@ void solve_batch(dm_ctx* ctx, const dm_solve_opts& o, DevOp* A64, DevOp* A32, De
     DevOp& A = A64 ? *A64 : *A32;
     if (A64 && z64_ready) HIP_OK(hipStreamWaitEvent(s, z64_ready, 0));
     std::vector<double> rtol(R, o.tol);
-    gmres_batch(ctx, n, R, A, M, B, X, rtol, live, o.maxiter, o.restart, K, s, iters, solved);
+    gmres_batch(ctx, n, R, A, M, B, X, rtol, live, o.maxiter, o.restart, K, s, iters, solved, true);
     DBuf<double> Rr;
     DBuf<char> ws;
     Rr.alloc(blk);
@@ -239,7 +244,7 @@ void solve_batch(dm_ctx* ctx, const dm_solve_opts& o, DevOp* A64, DevOp* A32, De
     for (int r = 0; r < R; ++r) live[r] = rel[r] > o.tol;
     if (left <= 0 || !std::any_of(live.begin(), live.end(), [](char c) { return c != 0; })) break;
     HIP_OK(hipMemsetAsync(D.p, 0, 8 * blk, s));
-    gmres_batch(ctx, n, R, *A32, M, RR.p, D.p, tau, live, left, o.restart, K, s, iters, solved);
+    gmres_batch(ctx, n, R, *A32, M, RR.p, D.p, tau, live, left, o.restart, K, s, iters, solved, true);
     outer += 1;
     dml::axpby(n * R, 1, 0, D.p, 1, 0, X, s);
     if (z64_ready && !waited) {

@@ -191,8 +191,9 @@ struct GmresOut {
   double est = 0;
 };
 
+// x_zero: x is known to be zero on entry (the first residual is then b).
 GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double* b, double* x, double rtol,
-               int maxiter, int restart, Krylov& K, hipStream_t s);
+               int maxiter, int restart, Krylov& K, hipStream_t s, bool x_zero = false);
 double true_residual(DevOp& A, i64 n, const double* b, const double* x, double* tmp, double* hbuf, hipStream_t s);
 double solve_one(dm_ctx* ctx, const dm_solve_opts& o, DevOp* A64, DevOp* A32, DevPrec* M, const double* b,
                  double* x, i64 n, Krylov& K, dm_solve_stats& st, bool& ok, hipEvent_t z64_ready);

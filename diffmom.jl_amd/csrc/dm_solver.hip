@@ -88,7 +88,7 @@ std::unique_ptr<DevPrec> build_precond(const dm_zmat* z, int form, const dm_solv
 // preconditioning follows Krylov.jl's semantics (stop on ||M r|| <= rtol ||M b||).
 // x holds the initial guess on entry.
 GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double* b, double* x, double rtol,
-               int maxiter, int restart, Krylov& K, hipStream_t s) {
+               int maxiter, int restart, Krylov& K, hipStream_t s, bool x_zero) {
   GmresOut out;
   int mcap = restart > 0 ? std::min(restart, maxiter) : maxiter;
   bool right = M && side == 1;
@@ -98,10 +98,14 @@ GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double*
   const i64 ldv = 2 * n;  // doubles per column
   auto col = [&](double* base, int j) { return base + (size_t)ldv * j; };
   // eps from the initial (preconditioned) residual with x0
+  // (a zero initial guess needs no product: r0 = b)
   auto residual = [&](double* r) {
-    A.apply(x, K.t.p, s);
     HIP_OK(hipMemcpyAsync(r, b, 16 * n, hipMemcpyDeviceToDevice, s));
-    dml::axpby(n, -1, 0, K.t.p, 1, 0, r, s);
+    if (!x_zero) {
+      A.apply(x, K.t.p, s);
+      dml::axpby(n, -1, 0, K.t.p, 1, 0, r, s);
+    }
+    x_zero = false;
     if (left) {
       M->apply(r, K.t.p, s);
       HIP_OK(hipMemcpyAsync(r, K.t.p, 16 * n, hipMemcpyDeviceToDevice, s));
@@ -250,7 +254,7 @@ double solve_one(dm_ctx* ctx, const dm_solve_opts& o, DevOp* A64, DevOp* A32, De
   if (o.method == DM_METHOD_GMRES) {
     DevOp& A = A64 ? *A64 : *A32;
     if (A64 && z64_ready) HIP_OK(hipStreamWaitEvent(s, z64_ready, 0));
-    GmresOut g = gmres(ctx, n, A, M, o.precond_side, b, x, o.tol, o.maxiter, o.restart, K, s);
+    GmresOut g = gmres(ctx, n, A, M, o.precond_side, b, x, o.tol, o.maxiter, o.restart, K, s, true);
     st.iterations += g.iters;
     ok = g.solved;
     return true_residual(A, n, b, x, tmp.p, hb.p, s);
@@ -265,7 +269,7 @@ double solve_one(dm_ctx* ctx, const dm_solve_opts& o, DevOp* A64, DevOp* A32, De
   bool waited = false;
   for (int outer = 0; outer < std::max(1, o.max_outer) && left > 0; ++outer) {
     HIP_OK(hipMemsetAsync(d.p, 0, 16 * n, s));
-    GmresOut g = gmres(ctx, n, *A32, M, 1, rr.p, d.p, tau, left, o.restart, K, s);
+    GmresOut g = gmres(ctx, n, *A32, M, 1, rr.p, d.p, tau, left, o.restart, K, s, true);
     st.iterations += g.iters;
     left -= g.iters;
     st.outer_iterations += 1;
