@@ -38,10 +38,12 @@ function _check(rc::Cint)
     error(msg)  # 4: convergence (ErrorException), 5/6: device / memory
 end
 
+# field for field the C struct dm_solve_opts (include/diffmom.h)
 struct dm_solve_opts
     method::Cint; precond::Cint; precond_side::Cint; maxiter::Cint
     tol::Cdouble; nf_cutoff::Cdouble; restart::Cint; check_convergence::Cint
     inner_tol::Cdouble; max_outer::Cint; block_size::Cint
+    batch::Cint; learned::Ptr{Cvoid}; adjoint::Cint; compress_tol::Cdouble
 end
 mutable struct dm_solve_stats
     iterations::Cint; outer_iterations::Cint; solved::Cint
@@ -176,21 +178,27 @@ end
 
 """
     solve_scattering(mesh, freq_hz, pw::PlaneWaveExcitation; method=:dense_gmres,
-                     precision=:fp64, gmres_tol=1e-6, gmres_maxiter=300, c0=299792458.0)
+                     precision=:fp64, gmres_tol=1e-6, gmres_maxiter=300, c0=299792458.0,
+                     compress_tol=5e-8)
 
 Dense branches of `DiffMoM.solve_scattering` (src/Workflow.jl:67-317) end to end on
 the GPU.  `precision=:mixed` iterates on the complex64 matrix with FP64 Arnoldi
-and FP64 refinement to the same true residual.
+and FP64 refinement to the same true residual; with `compress_tol > 0` the
+inner iterations use the cross-approximated complex64 matrix
+(||Z - Z_H||_F <= compress_tol ||Z||_F) while the refinement residual stays the
+dense complex128 product.  `compress_tol=0` iterates on the dense complex64 matrix.
 """
 function solve_scattering(mesh::TriMesh, freq_hz::Real, pw::PlaneWaveExcitation; method::Symbol=:dense_gmres,
                           precision::Symbol=:fp64, gmres_tol::Float64=1e-6, gmres_maxiter::Int=300,
-                          quad_order::Int=3, c0::Real=299792458.0, nf_cutoff::Float64=0.0)
+                          quad_order::Int=3, c0::Real=299792458.0, nf_cutoff::Float64=0.0,
+                          compress_tol::Float64=5e-8)
     rwg = build_rwg(mesh)
     rh = _rwg_handle(mesh, rwg)
     k = 2π * Float64(freq_hz) / Float64(c0)
     meth = method === :dense_direct ? DM_METHOD_LU : (precision === :mixed ? DM_METHOD_MIXED : DM_METHOD_GMRES)
     opts = dm_solve_opts(meth, method === :dense_direct ? DM_PRECOND_NONE : DM_PRECOND_NEARFIELD, 1,
-                         gmres_maxiter, gmres_tol, nf_cutoff, 0, 1, 0.0, 10, 128)
+                         gmres_maxiter, gmres_tol, nf_cutoff, 0, 1, 0.0, 10, 128,
+                         0, C_NULL, 0, precision === :mixed ? compress_tol : 0.0)
     X = Vector{ComplexF64}(undef, rwg.nedges)
     st = dm_solve_stats(); tasm = Ref{Cdouble}(0.0)
     kv = Vector(pw.k_vec); pol = Vector(pw.pol); E0 = [pw.E0]

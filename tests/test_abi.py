@@ -30,3 +30,42 @@ def test_error_string_without_device(dm):
     lib = dm.lib()
     assert lib.dm_version() >= 100
     assert isinstance(lib.dm_last_error(), bytes)
+
+
+def c_layout(tmp_path, struct, fields):
+    """(sizeof, [offsetof ...]) of a header struct, from a program built with gcc."""
+    src = tmp_path / "layout.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "diffmom.h"', 'int main(void) {',
+             f'  printf("%zu\\n", sizeof({struct}));']
+    lines += [f'  printf("%zu\\n", offsetof({struct}, {f}));' for f in fields]
+    lines += ['  return 0;', '}']
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    return out[0], out[1:]
+
+
+JL_TYPES = {"Cint": ctypes.c_int, "Cdouble": ctypes.c_double, "Ptr{Cvoid}": ctypes.c_void_p}
+
+
+def julia_fields(struct):
+    """(name, ctypes type) of a struct in the Julia shim, in declaration order."""
+    src = open(os.path.join(ROOT, "diffmom.jl_amd", "julia", "DiffMoMHIP.jl")).read()
+    body = re.search(r"^(?:mutable )?struct " + struct + r"\n(.*?)^end", src, flags=re.S | re.M).group(1)
+    return [(n, JL_TYPES[t]) for n, t in re.findall(r"(\w+)::([\w{}]+)", body)]
+
+
+def test_option_structs_match_the_header(dm, tmp_path):
+    """The ctypes and Julia mirrors of dm_solve_opts / dm_solve_stats have the
+    header's fields, in its order, at its offsets (a shorter mirror makes the
+    library read past the caller's struct)."""
+    for cls in (dm.dm_solve_opts, dm.dm_solve_stats):
+        names = [f for f, _ in cls._fields_]
+        size, offs = c_layout(tmp_path, cls.__name__, names)
+        assert ctypes.sizeof(cls) == size
+        assert [getattr(cls, f).offset for f in names] == offs
+        jl = julia_fields(cls.__name__)
+        assert [n for n, _ in jl] == names
+        assert [t for _, t in jl] == [t for _, t in cls._fields_]
