@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 
 #include "dm_common.h"
 
@@ -293,18 +294,24 @@ struct TileSide {
   int* nb;      // cap*3 edge neighbours
 };
 
+// Rounds p up to 16 bytes as an offset from the 16-byte aligned LDS base:
+// pointer arithmetic keeps the LDS address space visible to the compiler
+// (an integer round trip would turn every access through it into a flat one).
+__device__ inline char* lds_align16(char* base, char* p) {
+  return base + (((p - base) + 15) & ~(ptrdiff_t)15);
+}
+
 template <typename T, int NQ>
-__device__ inline char* carve_side(char* p, int cap, TileSide<T, NQ>& s) {
+__device__ inline char* carve_side(char* base, char* p, int cap, TileSide<T, NQ>& s) {
   s.c = reinterpret_cast<double*>(p); p += sizeof(double) * cap * 3;
   s.sig = reinterpret_cast<double*>(p); p += sizeof(double) * cap * 3;
   s.u = reinterpret_cast<T*>(p); p += sizeof(T) * cap * NQ * 3;
   s.av = reinterpret_cast<T*>(p); p += sizeof(T) * cap * 9;
-  p = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(p) + 15) & ~uintptr_t(15));
+  p = lds_align16(base, p);
   s.slot = reinterpret_cast<int*>(p); p += sizeof(int) * cap * 3;
   s.gid = reinterpret_cast<int*>(p); p += sizeof(int) * cap;
   s.nb = reinterpret_cast<int*>(p); p += sizeof(int) * cap * 3;
-  p = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(p) + 15) & ~uintptr_t(15));
-  return p;
+  return lds_align16(base, p);
 }
 
 template <typename T, int NQ>
@@ -385,7 +392,9 @@ __device__ inline double rsqrt_nr(double x) {
 
 // Far-pair J block in compute type T (moment form in centroid-relative
 // coordinates, so FP32 keeps its precision for nearby pairs).
-template <typename T, int NQ>
+// LOSSY: complex wavenumber (the exp(-Im(k) R) attenuation is applied); the
+// choice is made once per tile so the real-k loop carries no test for it.
+template <typename T, int NQ, bool LOSSY>
 __device__ inline void far_block(const TileSide<T, NQ>& R, int i, const TileSide<T, NQ>& C, int j,
                                  const T* ww, T kr, T ki, T fik_re, T fik_im, T Jr[9], T Ji[9]) {
   const T kr_turns = kr * (T)0.15915494309189535;  // k / (2 pi)
@@ -428,7 +437,7 @@ __device__ inline void far_block(const TileSide<T, NQ>& R, int i, const TileSide
         sn = __builtin_amdgcn_sinf(turns);
         cs = __builtin_amdgcn_cosf(turns);
       }
-      if (ki != (T)0) g *= exp(ki * Rd);
+      if constexpr (LOSSY) g *= exp(ki * Rd);
       T gr = g * cs, gi = -g * sn;
       T uv = urq[0] * uc[p][0] + urq[1] * uc[p][1] + urq[2] * uc[p][2];
       m0r += gr; m0i += gi;
@@ -498,10 +507,10 @@ __device__ inline void tile_body(const AsmArgs& a, int cap, AccT* __restrict__ z
   const int ldt = TB + 1;
   AccT* zt = reinterpret_cast<AccT*>(smem);
   char* p = smem + sizeof(AccT) * 2 * TB * ldt;
-  p = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(p) + 15) & ~uintptr_t(15));
+  p = lds_align16(smem, p);
   TileSide<T, NQ> R, C;
-  p = carve_side<T, NQ>(p, cap, R);
-  p = carve_side<T, NQ>(p, cap, C);
+  p = carve_side<T, NQ>(smem, p, cap, R);
+  p = carve_side<T, NQ>(smem, p, cap, C);
   int* extr = reinterpret_cast<int*>(p);
   int* extc = extr + TB;
   T* ww = reinterpret_cast<T*>(extc + TB + 2);
@@ -540,58 +549,64 @@ __device__ inline void tile_body(const AsmArgs& a, int cap, AccT* __restrict__ z
   const c2* jadj = reinterpret_cast<const c2*>(a.jadj);
 
   const int npairs = nI * nJ;
-  for (int pidx = tid; pidx < npairs; pidx += NTHR) {
-    const int i = pidx / nJ;
-    const int j = pidx - i * nJ;
-    const int gi = R.gid[i], gj = C.gid[j];
-    int jn = -1;
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-      if (R.nb[3 * i + k] == gj) jn = k;
-    if (gi == gj || jn >= 0) {
-      // self / edge-adjacent: FP64 table lookup with the caller-order rule
-      int jn2 = 0;
+  auto pair_loop = [&](auto lossy) {
+    for (int pidx = tid; pidx < npairs; pidx += NTHR) {
+      const int i = pidx / nJ;
+      const int j = pidx - i * nJ;
+      const int gi = R.gid[i], gj = C.gid[j];
+      int jn = -1;
 #pragma unroll
       for (int k = 0; k < 3; ++k)
-        if (C.nb[3 * j + k] == gi) jn2 = k;
+        if (R.nb[3 * i + k] == gj) jn = k;
+      if (gi == gj || jn >= 0) {
+        // self / edge-adjacent: FP64 table lookup with the caller-order rule
+        int jn2 = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          if (C.nb[3 * j + k] == gi) jn2 = k;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          int rs = R.slot[3 * i + e];
+          if (rs < 0) continue;
+#pragma unroll
+          for (int f = 0; f < 3; ++f) {
+            int cs = C.slot[3 * j + f];
+            if (cs < 0) continue;
+            bool fwd = extr[rs] <= extc[cs];
+            c2 v;
+            if (gi == gj)
+              v = fwd ? jself[(i64)gi * 9 + 3 * e + f] : jself[(i64)gi * 9 + 3 * f + e];
+            else
+              v = fwd ? jadj[((i64)gi * 3 + jn) * 9 + 3 * e + f] : jadj[((i64)gj * 3 + jn2) * 9 + 3 * f + e];
+            double s = R.sig[3 * i + e] * C.sig[3 * j + f];
+            atomicAdd(zt + rs * ldt + cs, (AccT)(v.re * s));
+            atomicAdd(zt + TB * ldt + rs * ldt + cs, (AccT)(v.im * s));
+          }
+        }
+        continue;
+      }
+      T Jr[9], Ji[9];
+      far_block<T, NQ, decltype(lossy)::value>(R, i, C, j, ww, kr, ki, (T)f4r, (T)f4i, Jr, Ji);
 #pragma unroll
       for (int e = 0; e < 3; ++e) {
         int rs = R.slot[3 * i + e];
         if (rs < 0) continue;
+        T sr = (T)R.sig[3 * i + e];
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
           int cs = C.slot[3 * j + f];
           if (cs < 0) continue;
-          bool fwd = extr[rs] <= extc[cs];
-          c2 v;
-          if (gi == gj)
-            v = fwd ? jself[(i64)gi * 9 + 3 * e + f] : jself[(i64)gi * 9 + 3 * f + e];
-          else
-            v = fwd ? jadj[((i64)gi * 3 + jn) * 9 + 3 * e + f] : jadj[((i64)gj * 3 + jn2) * 9 + 3 * f + e];
-          double s = R.sig[3 * i + e] * C.sig[3 * j + f];
-          atomicAdd(zt + rs * ldt + cs, (AccT)(v.re * s));
-          atomicAdd(zt + TB * ldt + rs * ldt + cs, (AccT)(v.im * s));
+          T s = sr * (T)C.sig[3 * j + f];
+          atomicAdd(zt + rs * ldt + cs, (AccT)(Jr[3 * e + f] * s));
+          atomicAdd(zt + TB * ldt + rs * ldt + cs, (AccT)(Ji[3 * e + f] * s));
         }
       }
-      continue;
     }
-    T Jr[9], Ji[9];
-    far_block<T, NQ>(R, i, C, j, ww, kr, ki, (T)f4r, (T)f4i, Jr, Ji);
-#pragma unroll
-    for (int e = 0; e < 3; ++e) {
-      int rs = R.slot[3 * i + e];
-      if (rs < 0) continue;
-      T sr = (T)R.sig[3 * i + e];
-#pragma unroll
-      for (int f = 0; f < 3; ++f) {
-        int cs = C.slot[3 * j + f];
-        if (cs < 0) continue;
-        T s = sr * (T)C.sig[3 * j + f];
-        atomicAdd(zt + rs * ldt + cs, (AccT)(Jr[3 * e + f] * s));
-        atomicAdd(zt + TB * ldt + rs * ldt + cs, (AccT)(Ji[3 * e + f] * s));
-      }
-    }
-  }
+  };
+  if (a.k_im != 0.0)
+    pair_loop(std::true_type{});
+  else
+    pair_loop(std::false_type{});
   __syncthreads();
 
   // Z_mn = -i k eta0 len_m len_n * tile ; P = -i k eta0 = (k_im eta0, -k_re eta0)
@@ -731,7 +746,8 @@ template <typename T, int NQ>
 __global__ void k_build_sides(AsmArgs a, int cap, char* img, i64 stride) {
   const int b = blockIdx.x;
   TileSide<T, NQ> S;
-  carve_side<T, NQ>(img + (i64)b * stride, cap, S);
+  char* base = img + (i64)b * stride;  // stride: a multiple of 16 bytes
+  carve_side<T, NQ>(base, base, cap, S);
   load_side<T, NQ>(a, b, a.blk_off[b + 1] - a.blk_off[b], S, threadIdx.x, blockDim.x);
 }
 
