@@ -390,13 +390,34 @@ __device__ inline double rsqrt_nr(double x) {
   return y;
 }
 
+// sin/cos of a phase x >= 0 through a table of the 256 angles 2 pi j / 256
+// (tab[j] = (cos, sin), built in LDS with sincos_phase): x = 2 pi n / 256 + r
+// with |r| <= pi / 256 (two-term reduction through FMA), sin r and cos r by
+// Taylor polynomials (sin to r^5, cos to r^6: truncation < 1e-17), then the
+// angle-addition formulas.  About half the instructions of sincos_phase.
+constexpr int kPhaseTable = 256;
+__device__ inline void sincos_table(double x, const double2* __restrict__ tab, double* s, double* c) {
+  const double STEP_HI = 0.02454369260617026, STEP_LO = 9.567553118338697e-19;  // pi / 128 = hi + lo
+  const double INV_STEP = 40.74366543152521;                                      // 128 / pi
+  const double n = rint(x * INV_STEP);
+  double r = fma(-n, STEP_HI, x);
+  r = fma(-n, STEP_LO, r);
+  const double z = r * r;
+  const double sr = fma(r * z, fma(z, 0.008333333333333333, -0.16666666666666666), r);
+  const double cr = fma(z, fma(z, fma(z, -0.001388888888888889, 0.041666666666666664), -0.5), 1.0);
+  const double2 t = tab[(int)n & (kPhaseTable - 1)];
+  *s = fma(t.y, cr, t.x * sr);
+  *c = fma(t.x, cr, -(t.y * sr));
+}
+
 // Far-pair J block in compute type T (moment form in centroid-relative
 // coordinates, so FP32 keeps its precision for nearby pairs).
 // LOSSY: complex wavenumber (the exp(-Im(k) R) attenuation is applied); the
 // choice is made once per tile so the real-k loop carries no test for it.
 template <typename T, int NQ, bool LOSSY>
 __device__ inline void far_block(const TileSide<T, NQ>& R, int i, const TileSide<T, NQ>& C, int j,
-                                 const T* ww, T kr, T ki, T fik_re, T fik_im, T Jr[9], T Ji[9]) {
+                                 const T* ww, const double2* __restrict__ ptab, T kr, T ki, T fik_re, T fik_im,
+                                 T Jr[9], T Ji[9]) {
   const T kr_turns = kr * (T)0.15915494309189535;  // k / (2 pi)
   T dc[3];
 #pragma unroll
@@ -427,7 +448,7 @@ __device__ inline void far_block(const TileSide<T, NQ>& R, int i, const TileSide
         T iR = rsqrt_nr(R2);
         Rd = R2 * iR;
         g = ww[q * NQ + p] * iR;
-        sincos_phase(kr * Rd, &sn, &cs);
+        sincos_table(kr * Rd, ptab, &sn, &cs);
       } else {
         // hardware rsq / sin / cos; the phase goes in revolutions (turns)
         T iR = __builtin_amdgcn_rsqf(R2);
@@ -514,6 +535,8 @@ __device__ inline void tile_body(const AsmArgs& a, int cap, AccT* __restrict__ z
   int* extr = reinterpret_cast<int*>(p);
   int* extc = extr + TB;
   T* ww = reinterpret_cast<T*>(extc + TB + 2);
+  // FP64 phase table (cos, sin of 2 pi j / 256), used by the FP64 far pairs
+  double2* ptab = reinterpret_cast<double2*>(lds_align16(smem, reinterpret_cast<char*>(ww + 49)));
   const int tid = threadIdx.x;
   const int nI = a.blk_off[bi + 1] - a.blk_off[bi];
   const int nJ = a.blk_off[bj + 1] - a.blk_off[bj];
@@ -538,6 +561,14 @@ __device__ inline void tile_body(const AsmArgs& a, int cap, AccT* __restrict__ z
     extc[k] = gj < a.n ? a.ext[gj] : 0x7fffffff;
   }
   if (tid < NQ * NQ) ww[tid] = (T)(a.w[tid / NQ] * a.w[tid % NQ] * INV4PI);
+  if constexpr (sizeof(T) == 8) {
+    for (int k = tid; k < kPhaseTable; k += NTHR) {
+      double sn, cs;
+      // k pi / 128 with pi / 128 split in two parts (hi + lo)
+      sincos_phase(fma((double)k, 0.02454369260617026, (double)k * 9.567553118338697e-19), &sn, &cs);
+      ptab[k] = make_double2(cs, sn);
+    }
+  }
   __syncthreads();
 
   const T kr = (T)a.k_re, ki = (T)a.k_im;
@@ -586,7 +617,7 @@ __device__ inline void tile_body(const AsmArgs& a, int cap, AccT* __restrict__ z
         continue;
       }
       T Jr[9], Ji[9];
-      far_block<T, NQ, decltype(lossy)::value>(R, i, C, j, ww, kr, ki, (T)f4r, (T)f4i, Jr, Ji);
+      far_block<T, NQ, decltype(lossy)::value>(R, i, C, j, ww, ptab, kr, ki, (T)f4r, (T)f4i, Jr, Ji);
 #pragma unroll
       for (int e = 0; e < 3; ++e) {
         int rs = R.slot[3 * i + e];
@@ -764,6 +795,7 @@ size_t tile_smem(int nq, int TB, int cap) {
   }
   b += 2 * sb;
   b += sizeof(int) * (2 * TB + 2) + sizeof(T) * 49;
+  if (sizeof(T) == 8) b = ((b + 15) & ~size_t(15)) + sizeof(double) * 2 * 256;  // phase table
   return b;
 }
 
