@@ -381,13 +381,13 @@ __device__ inline void sincos_phase(double x, double* s, double* c) {
   *c = ((q + 1) & 2) ? -cc : cc;
 }
 
-// 1/sqrt in FP64: hardware estimate refined by two Newton steps.
+// 1/sqrt in FP64: the hardware estimate (relative error ~2^-23) refined by
+// one third-order step, y (1 + e/2 + 3e^2/8) with e = 1 - x y^2 (error
+// O(e^3), below FP64 rounding).
 __device__ inline double rsqrt_nr(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  double h = 0.5 * x;
-  y = y * fma(-h * y, y, 1.5);
-  y = y * fma(-h * y, y, 1.5);
-  return y;
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
 // sin/cos of a phase x >= 0 through a table of the 256 angles 2 pi j / 256
@@ -427,8 +427,18 @@ __device__ inline void far_block(const TileSide<T, NQ>& R, int i, const TileSide
   for (int q = 0; q < NQ; ++q)
 #pragma unroll
     for (int c = 0; c < 3; ++c) uc[q][c] = C.u[(j * NQ + q) * 3 + c];
+  // Moments of G over the quadrature pairs: M0 = sum G, Mu = sum G u,
+  // Mv = sum G v, Muv = sum G u.v.  For rules of up to 4 points Mu is formed
+  // from the row sums of G (one per outer point) and Mv from its column sums
+  // (one per inner point), so each pair adds only to the row, column and u.v
+  // sums; the 7-point rule accumulates every moment per pair (its column sums
+  // would not fit the register budget).
+  constexpr bool kSums = NQ <= 4;
   T m0r = 0, m0i = 0, muvr = 0, muvi = 0;
   T mur[3] = {0, 0, 0}, mui[3] = {0, 0, 0}, mvr[3] = {0, 0, 0}, mvi[3] = {0, 0, 0};
+  T cpr[kSums ? NQ : 1], cpi[kSums ? NQ : 1];
+#pragma unroll
+  for (int p = 0; p < (kSums ? NQ : 1); ++p) cpr[p] = cpi[p] = 0;
   // FP64: the outer quadrature loop stays rolled (fewer live registers, so
   // more waves per SIMD); FP32 unrolls it
   constexpr int UQ = sizeof(T) == 8 ? 1 : NQ;
@@ -437,6 +447,7 @@ __device__ inline void far_block(const TileSide<T, NQ>& R, int i, const TileSide
     T urq[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) urq[c] = R.u[(i * NQ + q) * 3 + c];
+    T sr = 0, si = 0, wr = 0, wi = 0;
 #pragma unroll
     for (int p = 0; p < NQ; ++p) {
       T dx = dc[0] + (urq[0] - uc[p][0]);
@@ -461,12 +472,34 @@ __device__ inline void far_block(const TileSide<T, NQ>& R, int i, const TileSide
       if constexpr (LOSSY) g *= exp(ki * Rd);
       T gr = g * cs, gi = -g * sn;
       T uv = urq[0] * uc[p][0] + urq[1] * uc[p][1] + urq[2] * uc[p][2];
-      m0r += gr; m0i += gi;
-      muvr += gr * uv; muvi += gi * uv;
+      wr += gr * uv; wi += gi * uv;
+      if constexpr (kSums) {
+        sr += gr; si += gi;
+        cpr[p] += gr; cpi[p] += gi;
+      } else {
+        m0r += gr; m0i += gi;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          mur[c] += gr * urq[c]; mui[c] += gi * urq[c];
+          mvr[c] += gr * uc[p][c]; mvi[c] += gi * uc[p][c];
+        }
+      }
+    }
+    muvr += wr; muvi += wi;
+    if constexpr (kSums) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        mur[c] += gr * urq[c]; mui[c] += gi * urq[c];
-        mvr[c] += gr * uc[p][c]; mvi[c] += gi * uc[p][c];
+        mur[c] += sr * urq[c]; mui[c] += si * urq[c];
+      }
+    }
+  }
+  if constexpr (kSums) {
+#pragma unroll
+    for (int p = 0; p < NQ; ++p) {
+      m0r += cpr[p]; m0i += cpi[p];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        mvr[c] += cpr[p] * uc[p][c]; mvi[c] += cpi[p] * uc[p][c];
       }
     }
   }
@@ -763,7 +796,7 @@ __global__ __launch_bounds__(NTHR) void k_tile_multi(const AsmArgs* __restrict__
     i64 b = (i64)((sqrt(8.0 * (double)l + 1.0) - 1.0) * 0.5);
     while (b * (b + 1) / 2 > l) --b;
     while ((b + 1) * (b + 2) / 2 <= l) ++b;
-    const AsmArgs a = args[m];
+    const AsmArgs& a = args[m];  // read in place: a private copy would live in scratch
     tile_body<T, AccT, NQ, NTHR, MODE>(a, cap, zouts[m], nullptr, nullptr, (int)(l - b * (b + 1) / 2), (int)b);
     __syncthreads();
   }
