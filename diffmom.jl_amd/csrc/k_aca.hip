@@ -70,12 +70,9 @@ __global__ __launch_bounds__(256) void k_tau2(i64 T, int nb, const double* __res
   }
 }
 
-__device__ inline float2 cdivf(float2 a, float2 b) {
-  const float d = b.x * b.x + b.y * b.y;
-  return make_float2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
-}
-
-// Thread t owns column t / 4, rows 32 (t % 4) .. + 31 of the tile.
+// Thread t owns column t % 128, rows 32 (t / 128) .. + 31 of the tile: the
+// rows a wave holds form one quarter of the tile, so extracting the pivot row
+// is work for the two waves of that quarter only.
 __global__ __launch_bounds__(kAcaThreads) void k_aca(int nb, const float2* __restrict__ zp,
                                                      const double* __restrict__ tau2p, int rmax, int* __restrict__ rank,
                                                      float2* __restrict__ fac) {
@@ -90,7 +87,7 @@ __global__ __launch_bounds__(kAcaThreads) void k_aca(int nb, const float2* __res
     if (tid == 0) rank[t] = -1;
     return;
   }
-  const int c = tid >> 2, r0 = (tid & 3) * 32, lane = tid & 63, w = tid >> 6;
+  const int c = tid & (kPT - 1), qd = tid >> 7, r0 = qd * 32, lane = tid & 63, w = tid >> 6;
   float Rr[32], Ri[32];
   {
     const f4v* src = reinterpret_cast<const f4v*>(zp + t * kPT * kPT + (i64)c * kPT + r0);
@@ -174,7 +171,7 @@ __global__ __launch_bounds__(kAcaThreads) void k_aca(int nb, const float2* __res
 #pragma unroll
       for (int q = 0; q < 32; ++q) colb[r0 + q] = make_float2(Rr[q], Ri[q]);
     }
-    if ((pi >> 5) == (tid & 3)) {
+    if ((pi >> 5) == qd) {  // uniform over the wave
       const int qq = pi & 31;
       float vr = 0.f, vi = 0.f;
 #pragma unroll
@@ -186,17 +183,21 @@ __global__ __launch_bounds__(kAcaThreads) void k_aca(int nb, const float2* __res
       rowb[c] = make_float2(vr, vi);
     }
     __syncthreads();
+    // 1 / pivot once per step (the same value in every thread), then products
     const float2 piv = colb[pi];
+    const float pd = 1.0f / (piv.x * piv.x + piv.y * piv.y);
+    const float2 ip = make_float2(piv.x * pd, -piv.y * pd);
+    auto over_piv = [&](float2 a) { return make_float2(a.x * ip.x - a.y * ip.y, a.x * ip.y + a.y * ip.x); };
     if (tid < kPT)
       U[(i64)k * kPT + tid] = colb[tid];
     else if (tid < 2 * kPT)
-      V[(i64)k * kPT + tid - kPT] = cdivf(rowb[tid - kPT], piv);
-    const float2 vc = cdivf(rowb[c], piv);
+      V[(i64)k * kPT + tid - kPT] = over_piv(rowb[tid - kPT]);
+    const float2 vc = over_piv(rowb[c]);
 #pragma unroll
     for (int q = 0; q < 32; ++q) {
       const float2 u = colb[r0 + q];
-      Rr[q] -= u.x * vc.x - u.y * vc.y;
-      Ri[q] -= u.x * vc.y + u.y * vc.x;
+      Rr[q] = fmaf(-u.x, vc.x, fmaf(u.y, vc.y, Rr[q]));
+      Ri[q] = fmaf(-u.x, vc.y, fmaf(-u.y, vc.x, Ri[q]));
     }
     publish();
   }
