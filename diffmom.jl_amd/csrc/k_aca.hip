@@ -6,12 +6,14 @@
 // entries computed on demand (src/fast/ACA.jl:558-688).  Here the dense tile
 // is already in HBM after assembly, so the cross approximation runs with full
 // pivoting on the whole tile held in registers (512 threads x 32 entries):
-// each step picks the largest residual entry (i*, j*), takes U[:, k] =
-// R[:, j*] and V[:, k] = R[i*, :] / R[i*, j*], and subtracts U[:, k] V[:, k]^T
-// from the residual.  The Frobenius norm of the residual is summed exactly at
-// every step (FP64 sum of FP32 entries), so the stopping test ||R||_F <= tau is
-// a bound on the tile's approximation error (up to the FP32 rounding of the
-// updates, ~1e-7 of the tile's norm), not an estimate.  With
+// each step picks the largest residual entry (i*, j*) (to 2^-12 relative in
+// |R|^2), takes U[:, k] = R[:, j*] and V[:, k] = R[i*, :] / R[i*, j*], and
+// subtracts U[:, k] V[:, k]^T from the residual.  The Frobenius norm of the
+// residual is summed over all its entries at every step (FP32 within a wave,
+// FP64 across the waves: relative error ~1e-6), so the stopping test
+// ||R||_F <= tau is a bound on the tile's approximation error (up to that
+// and the FP32 rounding of the updates, ~1e-7 of the tile's norm), not an
+// estimate.  With
 // tau^2 = eps^2 ||Z||_F^2 / (2 T_off) for T_off stored off-diagonal tiles
 // (computed on the device from the tiles' norms, so the whole compression is
 // queued without a host round trip), the
@@ -70,10 +72,37 @@ __global__ __launch_bounds__(256) void k_tau2(i64 T, int nb, const double* __res
   }
 }
 
+// Wave reductions through DPP row operations (no LDS traffic): quad
+// permutes, half-row and row mirrors, then the row 15 / row 31 broadcasts;
+// lane 63 ends with the result over all 64 lanes (the identity 0 enters
+// where a broadcast leaves a row out).
+__device__ inline unsigned wave_max_u32(unsigned x) {
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false));  // row_half_mirror
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false));  // row_mirror
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false));  // row_bcast15
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false));  // row_bcast31
+  return (unsigned)__builtin_amdgcn_readlane((int)x, 63);
+}
+template <int CTRL, int ROWS>
+__device__ inline float dpp_add(float v) {
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+__device__ inline float wave_sum_f32(float x) {
+  x = dpp_add<0xB1, 0xF>(x);
+  x = dpp_add<0x4E, 0xF>(x);
+  x = dpp_add<0x141, 0xF>(x);
+  x = dpp_add<0x140, 0xF>(x);
+  x = dpp_add<0x142, 0xA>(x);
+  x = dpp_add<0x143, 0xC>(x);
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+}
+
 // Thread t owns column t % 128, rows 32 (t / 128) .. + 31 of the tile: the
 // rows a wave holds form one quarter of the tile, so extracting the pivot row
 // is work for the two waves of that quarter only.
-__global__ __launch_bounds__(kAcaThreads) void k_aca(int nb, const float2* __restrict__ zp,
+__global__ __launch_bounds__(kAcaThreads, 4) void k_aca(int nb, const float2* __restrict__ zp,
                                                      const double* __restrict__ tau2p, int rmax, int* __restrict__ rank,
                                                      float2* __restrict__ fac) {
   const i64 t = blockIdx.x;
@@ -101,46 +130,36 @@ __global__ __launch_bounds__(kAcaThreads) void k_aca(int nb, const float2* __res
     }
   }
   __shared__ float2 colb[kPT], rowb[kPT];
-  __shared__ float wmax[kAcaThreads / 64];
-  __shared__ int warg[kAcaThreads / 64];
+  __shared__ unsigned wkey[kAcaThreads / 64];
   __shared__ double wsum[kAcaThreads / 64];
   float2* U = fac + t * 2 * kPT * (i64)rmax;
   float2* V = U + (i64)rmax * kPT;
-  // this thread's largest residual entry (ties: lowest index) and its share
-  // of ||R||^2, reduced over the wave; lane 0 publishes the wave's result
+  // The wave's largest residual entry and its share of ||R||^2.  The maximum
+  // is taken over keys: |R|^2 with the low 11 mantissa bits replaced by the
+  // entry's place in the wave (lane, register), so one integer max finds the
+  // pivot (the largest entry to 2^-12 relative in |R|^2; equal keys resolve
+  // to the higher place).  The FP32 sums (relative error ~1e-6 of the wave's
+  // share) are far more accurate than the stopping test needs.
   auto publish = [&]() {
-    // FP32 sum of the thread's 32 terms (relative error ~2e-6 of a share of
-    // ||R||^2, far below what the stopping test needs).  The maximum is taken
-    // over keys |R|^2 with the entry's index in the low 5 mantissa bits (the
-    // pivot is then the largest entry to 4e-6 relative), one integer max per
-    // entry instead of a compare and two selects.
     float sumf = 0.f;
     unsigned key = 0;
 #pragma unroll
     for (int q = 0; q < 32; ++q) {
       const float a = Rr[q] * Rr[q] + Ri[q] * Ri[q];
       sumf += a;
-      key = max(key, (__float_as_uint(a) & ~31u) | (unsigned)q);
+      key = max(key, (__float_as_uint(a) & ~0x7ffu) | (unsigned)q);
     }
-    double sum = (double)sumf;
-    float best = __uint_as_float(key & ~31u);
-    const int bq = (int)(key & 31u);
-    int gidx = c * kPT + r0 + bq;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const float ob = __shfl_xor(best, o, 64);
-      const int oi = __shfl_xor(gidx, o, 64);
-      if (ob > best || (ob == best && oi < gidx)) {
-        best = ob;
-        gidx = oi;
-      }
-      sum += __shfl_xor(sum, o, 64);
-    }
+    key = wave_max_u32(key | ((unsigned)lane << 5));
+    const float sum = wave_sum_f32(sumf);
     if (lane == 0) {
-      wmax[w] = best;
-      warg[w] = gidx;
-      wsum[w] = sum;
+      wkey[w] = key;
+      wsum[w] = (double)sum;
     }
+  };
+  // flat index (column * 128 + row) of the entry behind wave wv's key
+  auto key_index = [](int wv, unsigned kk) {
+    const int col = (wv & 1) * 64 + (int)((kk >> 5) & 63u), row = (wv >> 1) * 32 + (int)(kk & 31u);
+    return col * kPT + row;
   };
   publish();
   int k = 0;
@@ -148,19 +167,20 @@ __global__ __launch_bounds__(kAcaThreads) void k_aca(int nb, const float2* __res
     // two barriers per step: the waves' results are read between the first
     // and the second, and rewritten only after the second
     __syncthreads();
-    float b = wmax[0];
-    int piv_idx = warg[0];
+    // the waves' results in wave order (ties between waves: the lower wave)
+    unsigned bk = wkey[0] & ~0x7ffu;
+    int bw = 0;
     double tot = wsum[0];
 #pragma unroll
     for (int q = 1; q < kAcaThreads / 64; ++q) {
-      const float bq = wmax[q];
-      const int gq = warg[q];
-      if (bq > b || (bq == b && gq < piv_idx)) {
-        b = bq;
-        piv_idx = gq;
+      const unsigned kq = wkey[q] & ~0x7ffu;
+      if (kq > bk) {
+        bk = kq;
+        bw = q;
       }
       tot += wsum[q];
     }
+    const int piv_idx = key_index(bw, wkey[bw]);
     if (tot <= tau2) break;  // rank k reaches the tolerance
     if (k == rmax) {
       k = -1;  // not compressible within rmax: stays dense
