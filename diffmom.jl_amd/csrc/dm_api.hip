@@ -14,6 +14,45 @@
 
 using namespace dms;
 
+thread_local DevCache* tl_cache = nullptr;
+
+void* DevCache::take(size_t b) {
+  auto it = free_blocks.find(b);
+  if (it == free_blocks.end()) return nullptr;
+  void* p = it->second;
+  free_blocks.erase(it);
+  bytes -= b;
+  return p;
+}
+bool DevCache::give(void* p, size_t b) {
+  if (bytes + b > cap) return false;
+  pending.emplace_back(b, p);
+  bytes += b;
+  return true;
+}
+void DevCache::promote(hipStream_t a, hipStream_t b) {
+  if (pending.empty()) return;
+  if (hipStreamQuery(a) != hipSuccess || hipStreamQuery(b) != hipSuccess) {
+    (void)hipGetLastError();  // hipErrorNotReady: work still queued
+    return;
+  }
+  for (auto& e : pending) free_blocks.emplace(e.first, e.second);
+  pending.clear();
+}
+void DevCache::trim() {
+  for (auto& e : free_blocks) {
+    (void)hipFree(e.second);
+    bytes -= e.first;
+  }
+  free_blocks.clear();
+}
+void DevCache::clear() {
+  trim();
+  for (auto& e : pending) (void)hipFree(e.second);
+  pending.clear();
+  bytes = 0;
+}
+
 namespace {
 thread_local std::string g_err;
 }  // namespace
@@ -39,9 +78,21 @@ int guard(F&& f) {
   }
 }
 
+// A public call on a context: its lock, its device, and its memory cache for
+// the buffers the call allocates and releases.
 struct CtxScope {
   std::lock_guard<std::mutex> lk;
-  explicit CtxScope(dm_ctx* c) : lk(c->mu) { HIP_OK(hipSetDevice(c->device)); }
+  dm_ctx* c;
+  DevCache* prev;
+  explicit CtxScope(dm_ctx* c_) : lk(c_->mu), c(c_), prev(tl_cache) {
+    HIP_OK(hipSetDevice(c->device));
+    c->cache.promote(c->stream, c->stream2);
+    tl_cache = &c->cache;
+  }
+  ~CtxScope() {
+    c->cache.promote(c->stream, c->stream2);
+    tl_cache = prev;
+  }
 };
 
 
@@ -331,6 +382,8 @@ int dm_create(int device, dm_ctx** out) {
     hipDeviceProp_t prop;
     HIP_OK(hipGetDeviceProperties(&prop, device));
     c->cu_count = prop.multiProcessorCount;
+    c->cache.device = device;
+    c->cache.cap = std::min<size_t>(size_t(64) << 30, prop.totalGlobalMem / 4);
     *out = c.release();
   });
 }
@@ -341,6 +394,7 @@ int dm_destroy(dm_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamSynchronize(ctx->stream2);
+    ctx->cache.clear();
     for (auto& r : ctx->recs) {
       (void)hipEventDestroy(r.a);
       (void)hipEventDestroy(r.b);
@@ -1065,7 +1119,10 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
     const TriTables& T = dmh::tables(r, quad_order);
     hipStream_t s = ctx->stream;
     double t0 = now();
-    auto z = std::make_unique<dm_zmat>();
+    // the matrix buffers of this handle's previous solve, when it kept them
+    // (the previous call ended with the stream synchronised)
+    std::unique_ptr<dm_zmat> z = std::move(r.spare);
+    if (!z || z->n != n) z = std::make_unique<dm_zmat>();
     z->ctx = ctx;
     z->rwg = rwg_c;
     z->n = n;
@@ -1077,6 +1134,13 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
       z->forms = batched ? (DM_Z64 | DM_Z32) : (DM_Z64P | DM_Z32P);
     else
       z->forms = DM_Z64;
+    // buffers of a kept matrix that this solve does not use
+    if (!(z->forms & DM_Z64)) z->z64.release();
+    if (!(z->forms & DM_Z32)) z->z32.release();
+    if (!(z->forms & DM_Z64P)) z->z64p.release();
+    if (!(z->forms & DM_Z32P)) z->z32p.release();
+    z->hrmax = 0;
+    z->hbytes = 0;
     z->k_re = k_re;
     z->k_im = k_im;
     z->eta0 = eta0;
@@ -1122,6 +1186,11 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
       }
     } evg{e_asm, e_h};
     const bool compress = o.method == DM_METHOD_MIXED && !batched && o.compress_tol > 0;
+    if (!compress) {
+      z->hrank.release();
+      z->hfac.release();
+      z->hwork.release();
+    }
     if (compress) {
       HIP_OK(hipEventCreateWithFlags(&e_asm, hipEventDisableTiming));
       HIP_OK(hipEventCreateWithFlags(&e_h, hipEventDisableTiming));
@@ -1188,7 +1257,10 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
     st.solve_time_s = now() - t1 - st.precond_time_s;
     if (stats) *stats = st;
     if (assembly_time_s) *assembly_time_s = t_asm;
-    if (zout) *zout = z.release();
+    if (zout)
+      *zout = z.release();
+    else
+      r.spare = std::move(z);
     if (!all_ok && o.check_convergence) unconverged = true;
   });
   if (rc == DM_OK && unconverged) {

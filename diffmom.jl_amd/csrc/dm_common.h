@@ -36,16 +36,42 @@ struct DmError : std::runtime_error {
   } while (0)
 
 // --------------------------------------------------------------- device buffers
+// Device memory a context keeps for reuse.  hipFree waits for the whole
+// device and hipMalloc of a large block costs ~0.3 ms, so blocks released
+// during a public call of a context are kept (up to `cap` bytes) and handed
+// out again for requests of exactly the same size.  A
+// released block first waits in `pending`: it becomes reusable only once both
+// of the context's streams have been seen idle, so no queued kernel can still
+// touch it.  dm_destroy frees everything kept; a failed hipMalloc frees the
+// reusable blocks and retries.
+struct DevCache {
+  int device = -1;
+  size_t cap = 0, bytes = 0;
+  std::multimap<size_t, void*> free_blocks;
+  std::vector<std::pair<size_t, void*>> pending;
+  void* take(size_t b);
+  bool give(void* p, size_t b);
+  void promote(hipStream_t a, hipStream_t b);  // pending -> reusable when both streams are idle
+  void trim();                                 // free the reusable blocks
+  void clear();                                // free everything
+};
+// the cache of the context whose public call runs on this thread (or null)
+extern thread_local DevCache* tl_cache;
+
 template <class T>
 struct DBuf {
   T* p = nullptr;
   size_t n = 0;
+  int dev = -1;
   DBuf() = default;
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      const size_t b = n * sizeof(T);
+      if (!(tl_cache && tl_cache->device == dev && tl_cache->give(p, b))) (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
   }
@@ -53,7 +79,21 @@ struct DBuf {
     if (count == n && p) return;
     release();
     if (count == 0) return;
-    HIP_OK(hipMalloc(&p, count * sizeof(T)));
+    const size_t b = count * sizeof(T);
+    void* q = tl_cache ? tl_cache->take(b) : nullptr;
+    if (q) {
+      p = static_cast<T*>(q);
+      dev = tl_cache->device;
+    } else {
+      hipError_t e = hipMalloc(&p, b);
+      if (e == hipErrorOutOfMemory && tl_cache) {
+        (void)hipGetLastError();
+        tl_cache->trim();
+        e = hipMalloc(&p, b);
+      }
+      HIP_OK(e);
+      HIP_OK(hipGetDevice(&dev));
+    }
     n = count;
   }
   void upload(const T* host, size_t count, hipStream_t s) {
@@ -80,6 +120,7 @@ struct dm_ctx {
   std::vector<ProfRec> recs;
   std::vector<hipEvent_t> ev_free;  // recycled profiling events
   std::map<std::string, int> tune;  // launch parameters chosen by the tuner, by shape key
+  DevCache cache;                   // device memory kept for reuse by this context's calls
   hipEvent_t ev_get() {
     if (ev_free.empty()) {
       hipEvent_t e;
@@ -144,6 +185,7 @@ struct SpaiPattern {
   DBuf<int> d_jptr, d_jidx, d_iptr, d_iidx, d_rowptr, d_colidx, d_map;
 };
 
+struct dm_zmat;
 struct dm_rwg {
   const dm_mesh* mesh = nullptr;
   dm_ctx* ctx = nullptr;
@@ -178,6 +220,11 @@ struct dm_rwg {
   std::shared_ptr<SpaiPattern> spai;  // latest pattern; preconditioners keep their own reference
   // tile-side images per quadrature order and precision (built lazily)
   DBuf<char> side_img[8][2];
+  // the matrix buffers of the last dm_solve_scattering_pw that did not hand
+  // its matrix to the caller, kept for the next solve on this handle (same
+  // sizes: no allocation, no free at the end of the solve); freed with the
+  // handle
+  std::unique_ptr<dm_zmat> spare;
 };
 
 // ------------------------------------------------------------ Z storage
