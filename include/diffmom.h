@@ -65,6 +65,9 @@ typedef struct dm_zmat dm_zmat;
 typedef struct dm_mfop dm_mfop;
 
 /* ------------------------------------------------------------------ context */
+/* A context keeps device blocks released by its calls for reuse by later
+ * calls of the same size (at most min(64 GB, a quarter of the device));
+ * dm_destroy frees them. */
 int dm_create(int device, dm_ctx** out);
 int dm_destroy(dm_ctx* ctx);
 const char* dm_last_error(void);
@@ -284,11 +287,14 @@ int dm_mpnn_apply(dm_ctx* ctx, const dm_mpnn* net, const dm_rwg* rwg, double k, 
 /* solve_scattering(mesh, freq, excitation; method=:dense_direct|:dense_gmres)
  * for plane waves, end to end on the device (src/Workflow.jl:67-317): assembly,
  * excitation, preconditioner and solve of nrhs waves (kvec, pol: 3 x nrhs).
- * opts->method DM_METHOD_LU | DM_METHOD_GMRES | DM_METHOD_MIXED; with MIXED the
- * complex128 matrix (needed only by the refinement residual) is assembled on a
- * second stream while the complex64 solve runs.  X: N x nrhs complex128,
- * caller basis order.  assembly_time_s (optional): wall time until the matrix
- * the solver iterates on is ready.  zout (optional): receives the matrix. */
+ * opts->method DM_METHOD_LU | DM_METHOD_GMRES | DM_METHOD_MIXED; with MIXED one
+ * FP64 assembly writes the complex128 matrix (for the refinement residual) and
+ * its complex64 rounding (for the inner solve), which opts->compress_tol > 0
+ * compresses on a second stream beside the preconditioner build.  X: N x nrhs
+ * complex128, caller basis order.  assembly_time_s (optional): wall time until
+ * the matrix the solver iterates on is ready.  zout (optional): receives the
+ * matrix; without it the RWG handle keeps the matrix buffers for its next
+ * solve (freed by dm_rwg_free). */
 int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg, double k_re, double k_im, double eta0, int quad_order,
                            const double* kvec, const double* pol, const double* E0, int64_t nrhs,
                            const dm_solve_opts* opts, double* X, dm_solve_stats* stats, double* assembly_time_s,
