@@ -380,6 +380,50 @@ def test_mixed_solve_on_compressed_matrix(dm):
     assert np.linalg.norm(Z64 @ x - V[:, 0]) / np.linalg.norm(V[:, 0]) <= 1e-7 * 1.001
 
 
+def test_repeated_solves_reuse_matrix_buffers(dm):
+    """An RWG handle keeps its last solve's matrix buffers and a context keeps
+    released device blocks: repeated solves on one handle, interleaved with
+    solves of other forms, with a matrix handed to the caller and with another
+    handle of a different size, give the same currents (to the solve's own
+    accuracy: the tile kernel's LDS atomics add a pair's four contributions in
+    no fixed order, so repeated assemblies may differ in the last bit) and the
+    true FP64 residual stays below the tolerance."""
+    m = dm.make_icosphere(0.1, 2)
+    r = dm.build_rwg(m)
+    m2 = dm.make_rect_plate(0.2, 0.15, 12, 9)
+    r2 = dm.build_rwg(m2)
+    pws = _random_planewaves(dm, np.random.default_rng(31), K0, 1)
+    P = dm.build_nearfield_preconditioner(None, m, r, 2.0 * float(np.mean(r.len)))
+    P2 = dm.build_nearfield_preconditioner(None, m2, r2, 2.0 * float(np.mean(r2.len)))
+
+    def chosen():
+        X, st, _ = dm.solve_planewaves(m, r, K0, pws, "mixed", P, tol=1e-6, maxiter=300, batch=-1,
+                                       compress_tol=5e-8)
+        assert st.solved and st.residual <= 1e-6
+        return X
+
+    def same(X, Y):
+        return np.linalg.norm(X - Y) / np.linalg.norm(Y) < 1e-5
+
+    X0 = chosen()
+    assert same(chosen(), X0)  # the kept buffers
+    Xf, sf, _ = dm.solve_planewaves(m, r, K0, pws, "fp64", P, tol=1e-6, maxiter=300)
+    assert sf.solved
+    assert same(chosen(), X0)  # after a solve that kept other forms
+    Xp, sp, _ = dm.solve_planewaves(m2, r2, K0, pws, "mixed", P2, tol=1e-6, maxiter=300, batch=-1,
+                                    compress_tol=5e-8)
+    assert sp.solved
+    Xk, sk, _, Z = dm.solve_planewaves(m, r, K0, pws, "mixed", P, tol=1e-6, maxiter=300, batch=-1,
+                                       compress_tol=5e-8, keep_matrix=True)
+    assert same(Xk, X0)  # the matrix goes to the caller ...
+    assert same(chosen(), X0)  # ... and the next solve allocates anew
+    Xb, sb, _ = dm.solve_planewaves(m, r, K0, pws * 8, "mixed", P, tol=1e-6, maxiter=300)  # lockstep batch
+    assert sb.solved
+    assert same(chosen(), X0)
+    assert same(Xf, X0)
+    del Z
+
+
 def test_c1_plate_chosen_form_matches_plain_fp64(dm):
     """configs[1] plate (0.5 m, 60 x 60, N = 10680, theta = 30 deg, phi = 0,
     theta-polarised): the chosen form (one FP64 assembly with its complex64
