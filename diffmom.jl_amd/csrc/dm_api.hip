@@ -32,8 +32,13 @@ bool DevCache::give(void* p, size_t b) {
 }
 void DevCache::promote(hipStream_t a, hipStream_t b) {
   if (pending.empty()) return;
-  if (hipStreamQuery(a) != hipSuccess || hipStreamQuery(b) != hipSuccess) {
-    (void)hipGetLastError();  // hipErrorNotReady: work still queued
+  const hipError_t ea = hipStreamQuery(a);
+  const hipError_t eb = ea == hipSuccess ? hipStreamQuery(b) : hipSuccess;
+  if (ea != hipSuccess || eb != hipSuccess) {
+    // work still queued: drop only that status, so that a later check of the
+    // last error does not see it (any other error is left for its owner)
+    if ((ea == hipSuccess || ea == hipErrorNotReady) && (eb == hipSuccess || eb == hipErrorNotReady))
+      (void)hipGetLastError();
     return;
   }
   for (auto& e : pending) free_blocks.emplace(e.first, e.second);

@@ -39,11 +39,11 @@ struct DmError : std::runtime_error {
 // Device memory a context keeps for reuse.  hipFree waits for the whole
 // device and hipMalloc of a large block costs ~0.3 ms, so blocks released
 // during a public call of a context are kept (up to `cap` bytes) and handed
-// out again for requests of exactly the same size.  A
-// released block first waits in `pending`: it becomes reusable only once both
-// of the context's streams have been seen idle, so no queued kernel can still
-// touch it.  dm_destroy frees everything kept; a failed hipMalloc frees the
-// reusable blocks and retries.
+// out again for requests of exactly the same size.  A released block first
+// waits in `pending`: it becomes reusable only once both of the context's
+// streams have been seen idle, so no queued kernel can still touch it.
+// dm_destroy frees everything kept; a failed hipMalloc frees the reusable
+// blocks and retries.
 struct DevCache {
   int device = -1;
   size_t cap = 0, bytes = 0;
