@@ -50,6 +50,7 @@ struct OpZ64 : DevOp {
 };
 struct OpZ32 : DevOp {
   const dm_zmat* z;
+  DBuf<char> planes;  // bf16 x 3 planes of Z32, built on the first batched product
   explicit OpZ32(const dm_zmat* z_) : z(z_) {}
   void apply(const double* x, double* y, hipStream_t s) override {
     ProfScope p(z->ctx, "gemv_z32", s);
@@ -60,10 +61,17 @@ struct OpZ32 : DevOp {
       for (int j = 0; j < R; ++j) apply(X + 2 * z->n * j, Y + 2 * z->n * j, s);
       return;
     }
+    if (!planes.p) {
+      ProfScope p(z->ctx, "zx_split", s);
+      planes.alloc(dml::zx_bf16_planes_elems(z->ld, z->n) * 2);
+      dml::zx_bf16_split(z->n, z->ld, z->z32.p, planes.p, s);
+    }
     ProfScope p(z->ctx, "zx_z32", s);
-    dml::zx_mfma(z->n, z->ld, z->z32.p, X, z->n, R, Y, z->n, ws, s);
+    dml::zx_bf16(z->n, z->ld, planes.p, X, z->n, R, Y, z->n, ws, s);
   }
-  size_t batch_ws(int R) const override { return dml::zx_workspace_bytes(z->n, R); }
+  size_t batch_ws(int R) const override {
+    return std::max(dml::zx_workspace_bytes(z->n, R), dml::zx_bf16_workspace_bytes(z->n, R));
+  }
 };
 
 // Packed symmetric storage (DM_Z64P / DM_Z32P): one symmetric product per

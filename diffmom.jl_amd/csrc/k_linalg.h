@@ -58,6 +58,15 @@ void hier_ranges(i64 n, const int* rank, int W, void* buf, HierRanges& hr, hipSt
 void symv_hier(i64 n, const float* z32p, const int* rank, const float* fac, int rmax, const double* x, double* y,
                void* ws, const HierRanges& hr, hipStream_t s);
 size_t zx_workspace_bytes(i64 n, int R);
+// The complex64 Z X on the bf16 matrix cores at FP32 accuracy (each FP32
+// value split into three bf16 parts, six part products): planes of Z32 built
+// once by zx_bf16_split (zx_bf16_planes_elems(ld, n) bf16), workspace
+// zx_bf16_workspace_bytes(n, R) per product.
+size_t zx_bf16_planes_elems(i64 ld, i64 n);
+void zx_bf16_split(i64 n, i64 ld, const float* z32, void* planes, hipStream_t s);
+size_t zx_bf16_workspace_bytes(i64 n, int R);
+void zx_bf16(i64 n, i64 ld, const void* planes, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
+             hipStream_t s);
 // Y (mr x R) = A (mr x n, lda) X (n x R), complex128 on FP64 MFMA; ws:
 // gemm_c128_workspace_bytes(n, R).
 size_t gemm_c128_workspace_bytes(i64 n, int R);

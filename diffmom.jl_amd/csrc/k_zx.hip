@@ -284,6 +284,184 @@ __global__ void k_split_planes64(i64 n, i64 np, int R, const double2* __restrict
   Xi[idx] = v.y;
 }
 
+
+// ------------------------------------------------------------------ bf16 x 3
+// The same product on the bf16 matrix cores at FP32 accuracy: every FP32
+// value x is split exactly into three bf16 parts, x = h + m + l (h = bf16(x),
+// m = bf16(x - h), l = x - h - m; each difference is exact in FP32 and l
+// fits bf16), and A X is formed from the six part products h h', h m', m h',
+// h l', l h', m m' (the dropped ones are below 2^-24 relative), accumulated
+// in FP32 by v_mfma_f32_16x16x32_bf16 (lane l supplies row l % 16 and eight
+// consecutive k from 8 (l / 16); the output layout is that of the FP32
+// 16x16 product).  Planes of A' are stored in groups of 8 columns, (k, r) at
+// ((k / 8) R2 + r) 8 + k % 8, so an operand is one 16-byte read; the X planes
+// are column-major with k contiguous.  A workgroup covers 64 complex rows x
+// 64 right-hand sides for one K chunk; the four column blocks of a row block
+// are dispatched back to back on one XCD, which share its A' stages in L2.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+constexpr int ZB_LDA = 40;  // bf16 per LDS row (32 + 8 padding)
+
+__device__ inline void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
+  h = (__bf16)x;
+  const float r1 = x - (float)h;
+  m = (__bf16)r1;
+  l = (__bf16)(r1 - (float)m);
+}
+
+// A' (FP32, 2 ld x n column-major) -> three bf16 planes in 8-column groups,
+// ngroups groups (columns past n zero).  One thread per (row, group).
+__global__ void k_split_a_bf16(i64 R2, i64 n, i64 ngroups, const float* __restrict__ z, __bf16* __restrict__ out,
+                               i64 plane) {
+  const i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (idx >= R2 * ngroups) return;
+  const i64 r = idx % R2, g = idx / R2;
+  v8bf h, m, l;
+#pragma unroll
+  for (int kk = 0; kk < 8; ++kk) {
+    const i64 k = 8 * g + kk;
+    const float x = k < n ? z[k * R2 + r] : 0.f;
+    __bf16 a, b, c;
+    split3(x, a, b, c);
+    h[kk] = a;
+    m[kk] = b;
+    l[kk] = c;
+  }
+  const i64 at = (g * R2 + r) * 8;
+  *reinterpret_cast<v8bf*>(out + at) = h;
+  *reinterpret_cast<v8bf*>(out + plane + at) = m;
+  *reinterpret_cast<v8bf*>(out + 2 * plane + at) = l;
+}
+
+// complex128 X (n x R, ldx) -> six bf16 planes np x Rp (Re h, m, l, Im h, m,
+// l), rows past n and columns past R zero.
+__global__ void k_split_x_bf16(i64 n, i64 np, int R, int Rp, const double2* __restrict__ X, i64 ldx,
+                               __bf16* __restrict__ out) {
+  const i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  const i64 plane = np * Rp;
+  if (idx >= plane) return;
+  const i64 k = idx % np, c = idx / np;
+  const double2 v = (k < n && c < R) ? X[k + c * ldx] : make_double2(0.0, 0.0);
+  __bf16 a, b, d;
+  split3((float)v.x, a, b, d);
+  out[idx] = a;
+  out[plane + idx] = b;
+  out[2 * plane + idx] = d;
+  split3((float)v.y, a, b, d);
+  out[3 * plane + idx] = a;
+  out[4 * plane + idx] = b;
+  out[5 * plane + idx] = d;
+}
+
+__global__ __launch_bounds__(256, 1) void k_zx_bf16(i64 n, i64 R2, int nrb, int ncb, const __bf16* __restrict__ Ap,
+                                                    i64 aplane,
+                                                    const __bf16* __restrict__ Xp, i64 np, i64 xplane, int R,
+                                                    double2* __restrict__ part) {
+  __shared__ __align__(16) __bf16 As[3][2 * ZX_BM][ZB_LDA];
+  __shared__ __align__(16) __bf16 Bs[6][ZX_BN][ZB_LDA];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // XCD-aware tile order: workgroup w runs on XCD w % 8; its index j = w / 8
+  // on that XCD gives row block 8 (j / ncb) + w % 8 and column block j % ncb
+  const int w = blockIdx.x, j = w / 8;
+  const int rb_idx = 8 * (j / ncb) + w % 8, cb_idx = j % ncb;
+  if (rb_idx >= nrb) return;
+  const i64 m0 = (i64)rb_idx * ZX_BM;  // complex row base
+  const int c0 = cb_idx * ZX_BN;       // rhs base
+  const i64 k_begin = (i64)blockIdx.z * ZX_KC;
+  const i64 k_end = k_begin + ZX_KC < n ? k_begin + ZX_KC : n;
+  const int nst = (int)((k_end - k_begin + 31) / 32);
+  typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+  u4v ra[6], rb[6];
+  // stage of 32 columns: A' 3 planes x 4 groups x 128 rows, X 6 planes x 64
+  // columns x 4 groups of 8 rows: 1536 + 1536 16-byte pieces, 6 + 6 per thread
+  auto gload = [&](i64 k0) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const int c = tid + 256 * i, p = c >> 9, rem = c & 511, kg = rem >> 7, r = rem & 127;
+      ra[i] = *reinterpret_cast<const u4v*>(Ap + p * aplane + ((k0 / 8 + kg) * R2 + 2 * m0 + r) * 8);
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const int c = tid + 256 * i, q = c >> 8, rem = c & 255, col = rem >> 2, kq = rem & 3;
+      rb[i] = *reinterpret_cast<const u4v*>(Xp + q * xplane + (i64)(c0 + col) * np + k0 + 8 * kq);
+    }
+  };
+  auto lstore = [&]() {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const int c = tid + 256 * i, p = c >> 9, rem = c & 511, kg = rem >> 7, r = rem & 127;
+      *reinterpret_cast<u4v*>(&As[p][r][8 * kg]) = ra[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const int c = tid + 256 * i, q = c >> 8, rem = c & 255, col = rem >> 2, kq = rem & 3;
+      *reinterpret_cast<u4v*>(&Bs[q][col][8 * kq]) = rb[i];
+    }
+  };
+  v4f accP[2][4], accQ[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      accP[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+      accQ[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+    }
+  gload(k_begin);
+  lstore();
+  __syncthreads();
+  const int g = lane >> 4;
+  for (int st = 0; st < nst; ++st) {
+    if (st + 1 < nst) gload(k_begin + (i64)(st + 1) * 32);
+    // all operands of the stage, then the six part products, each over the
+    // wave's 16 independent accumulator tiles (no back-to-back dependence)
+    v8bf A[3][2], Br[3][4], Bi[3][4];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+#pragma unroll
+      for (int r = 0; r < 2; ++r) A[q][r] = *reinterpret_cast<const v8bf*>(&As[q][32 * wave + 16 * r + (lane & 15)][8 * g]);
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        Br[q][cb] = *reinterpret_cast<const v8bf*>(&Bs[q][16 * cb + (lane & 15)][8 * g]);
+        Bi[q][cb] = *reinterpret_cast<const v8bf*>(&Bs[3 + q][16 * cb + (lane & 15)][8 * g]);
+      }
+    }
+    constexpr int kPa[6] = {1, 0, 2, 0, 1, 0}, kPb[6] = {1, 2, 0, 1, 0, 0};  // smallest terms first
+#pragma unroll
+    for (int t = 0; t < 6; ++t)
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+          accP[r][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[kPa[t]][r], Br[kPb[t]][cb], accP[r][cb], 0, 0, 0);
+          accQ[r][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[kPa[t]][r], Bi[kPb[t]][cb], accQ[r][cb], 0, 0, 0);
+        }
+    __syncthreads();
+    if (st + 1 < nst) {
+      lstore();
+      __syncthreads();
+    }
+  }
+  // output layout as in k_zx_mfma: lane holds real rows 4 (lane / 16) + 0..3
+  // of each 16-row block, column lane % 16
+  double2* out = part + (i64)blockIdx.z * (i64)R * n;
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+      const int col = c0 + 16 * cb + (lane & 15);
+      if (col >= R) continue;
+      v4f p = accP[r][cb], q = accQ[r][cb];
+      const i64 real_row = 32 * wave + 16 * r + 4 * (lane >> 4);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        i64 m = m0 + (real_row >> 1) + h;
+        if (m >= n) continue;
+        float pr = h ? p[2] : p[0], pi = h ? p[3] : p[1];
+        float qr = h ? q[2] : q[0], qi = h ? q[3] : q[1];
+        out[m + (i64)col * n] = make_double2((double)pr - (double)qi, (double)qr + (double)pi);
+      }
+    }
+}
+
 }  // namespace
 
 namespace dml {
@@ -343,6 +521,44 @@ void zx_mfma(i64 n, i64 ld, const float* z32, const double* X, i64 ldx, int R, d
   dim3 grid((unsigned)((n + ZX_BM - 1) / ZX_BM), (unsigned)((R + ZX_BN - 1) / ZX_BN), (unsigned)nk);
   DM_REQUIRE(grid.y <= 65535 && grid.z <= 65535, DM_ERR_ARGUMENT, "too many right-hand sides for one product");
   k_zx_mfma<<<grid, 256, 0, s>>>(n, ld, z32, Xr, Xi, np, R, part);
+  k_zx_reduce<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, R, (int)nk, part, reinterpret_cast<double2*>(Y), ldy);
+  HIP_OK(hipGetLastError());
+}
+
+
+// bf16 x 3 form of Z32 (see k_zx_bf16): planes of A' built once per matrix.
+i64 zx_bf16_groups(i64 n) { return (n + 31) / 32 * 4; }
+size_t zx_bf16_planes_elems(i64 ld, i64 n) { return (size_t)3 * zx_bf16_groups(n) * 2 * ld * 8; }
+void zx_bf16_split(i64 n, i64 ld, const float* z32, void* planes, hipStream_t s) {
+  const i64 R2 = 2 * ld, ng = zx_bf16_groups(n);
+  const i64 tot = R2 * ng;
+  k_split_a_bf16<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(R2, n, ng, z32, reinterpret_cast<__bf16*>(planes),
+                                                               ng * R2 * 8);
+  HIP_OK(hipGetLastError());
+}
+size_t zx_bf16_workspace_bytes(i64 n, int R) {
+  const i64 nk = (n + ZX_KC - 1) / ZX_KC, np = (n + 31) / 32 * 32, Rp = ((i64)R + ZX_BN - 1) / ZX_BN * ZX_BN;
+  return (size_t)nk * R * n * sizeof(double2) + 64 + (size_t)6 * np * Rp * 2;
+}
+void zx_bf16(i64 n, i64 ld, const void* planes, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
+             hipStream_t s) {
+  DM_REQUIRE(ld % ZX_BM == 0 && ld >= n, DM_ERR_ARGUMENT, "complex64 matrix leading dimension must be a multiple of 64");
+  const i64 nk = (n + ZX_KC - 1) / ZX_KC, np = (n + 31) / 32 * 32;
+  const int Rp = (R + ZX_BN - 1) / ZX_BN * ZX_BN;
+  double2* part = reinterpret_cast<double2*>(ws);
+  __bf16* Xp = reinterpret_cast<__bf16*>(reinterpret_cast<char*>(ws) +
+                                         (((size_t)nk * R * n * sizeof(double2) + 63) & ~size_t(63)));
+  const i64 xplane = np * Rp;
+  k_split_x_bf16<<<(unsigned)((xplane + 255) / 256), 256, 0, s>>>(n, np, R, Rp, reinterpret_cast<const double2*>(X),
+                                                                   ldx, Xp);
+  const int nrb = (int)(ld / ZX_BM), ncb = Rp / ZX_BN;
+  const i64 nwg = (i64)((nrb + 7) / 8) * 8 * ncb;
+  DM_REQUIRE(nwg < (1LL << 31) && nk <= 65535 && ncb <= 65535, DM_ERR_ARGUMENT, "product too large for one launch");
+  const i64 R2 = 2 * ld, ng = zx_bf16_groups(n);
+  dim3 grid((unsigned)nwg, 1, (unsigned)nk);
+  k_zx_bf16<<<grid, 256, 0, s>>>(n, R2, nrb, ncb, reinterpret_cast<const __bf16*>(planes), ng * R2 * 8, Xp, np,
+                                 xplane, R, part);
+  const i64 tot = n * (i64)R;
   k_zx_reduce<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, R, (int)nk, part, reinterpret_cast<double2*>(Y), ldy);
   HIP_OK(hipGetLastError());
 }

@@ -131,9 +131,13 @@ def test_products_and_matrix_free(dm, O):
 
 @pytest.mark.parametrize("subdiv,nrhs", [(2, 8), (2, 70), (4, 72)])
 def test_multi_rhs_products_on_matrix_cores(dm, subdiv, nrhs):
-    """dm_apply with >= 8 right-hand sides runs Z X on MFMA (complex64: FP32
-    products, FP64 sum of 4096-column chunks; complex128: FP64 MFMA).  subdiv 4
-    (N = 7680) spans two K chunks; 70 and 72 leave partial 64-column blocks."""
+    """dm_apply with >= 8 right-hand sides runs Z X on the matrix cores
+    (complex64: three-part bf16 split of Z and X, six part products per tile
+    accumulated in FP32, FP64 sum of 4096-column chunks; complex128: FP64
+    MFMA).  subdiv 4 (N = 7680) spans two K chunks; 70 and 72 leave partial
+    64-column blocks.  The complex64 product is checked against the exact
+    product of the complex64 entries to FP32 accuracy (a dropped part product
+    would leave ~1e-5)."""
     m = dm.make_icosphere(0.1, subdiv)
     r = dm.build_rwg(m)
     Zd = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64 | dm.DM_Z32, device=True)
@@ -145,6 +149,9 @@ def test_multi_rhs_products_on_matrix_cores(dm, subdiv, nrhs):
     assert np.linalg.norm(Y64 - ref) / np.linalg.norm(ref) < 1e-12
     Y32 = Zd.matmul(X, form=dm.DM_Z32)
     assert np.linalg.norm(Y32 - ref) / np.linalg.norm(ref) < 1e-5
+    Z32 = Zd.to_numpy(dm.DM_Z32).astype(np.complex128)
+    ref32 = Z32 @ X.astype(np.complex64).astype(np.complex128)
+    assert np.linalg.norm(Y32 - ref32) / np.linalg.norm(ref32) < 2e-6
     # every column, not only the aggregate, and against the one-vector GEMV path
     col_err = np.linalg.norm(Y32 - ref, axis=0) / np.linalg.norm(ref, axis=0)
     assert col_err.max() < 1e-5
