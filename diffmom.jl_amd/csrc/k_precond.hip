@@ -173,6 +173,60 @@ __global__ __launch_bounds__(256) void k_spmv(i64 n, const int* __restrict__ row
   if (r < n && j == 0) y[r] = make_double2(ar, ai);
 }
 
+// Y[:, c] = M X[:, c] for kSpmmCols columns per workgroup row (blockIdx.y
+// selects the column group): the row's values and column indices are read once
+// per group instead of once per column.  Lanes take nonzeros as in k_spmv; the
+// 8-lane reduction is a reduce-scatter (offsets 4, 2, 1, each lane keeping half
+// of its remaining columns), which adds the same partial sums in the same
+// order as k_spmv's butterfly and leaves lane j with columns 2j and 2j + 1.
+constexpr int kSpmmCols = 2 * kSpmvLanes;
+
+__global__ __launch_bounds__(256) void k_spmm(i64 n, int R, const int* __restrict__ rowptr,
+                                              const int* __restrict__ colidx, const double2* __restrict__ val,
+                                              const double2* __restrict__ X, double2* __restrict__ Y) {
+  const int c0 = blockIdx.y * kSpmmCols;
+  const int nc = min(kSpmmCols, R - c0);
+  const i64 r = (blockIdx.x * (i64)blockDim.x + threadIdx.x) / kSpmvLanes;
+  const int j = threadIdx.x % kSpmvLanes;
+  const double2* xb = X + (i64)c0 * n;
+  double ar[kSpmmCols], ai[kSpmmCols];
+#pragma unroll
+  for (int c = 0; c < kSpmmCols; ++c) ar[c] = ai[c] = 0.0;
+  if (r < n) {
+    const int p1 = rowptr[r + 1];
+    for (int p = rowptr[r] + j; p < p1; p += kSpmvLanes) {
+      const double2 a = val[p];
+      const double2* xc = xb + colidx[p];
+#pragma unroll
+      for (int c = 0; c < kSpmmCols; ++c) {
+        if (c < nc) {
+          const double2 v = xc[(i64)c * n];
+          ar[c] += a.x * v.x - a.y * v.y;
+          ai[c] += a.x * v.y + a.y * v.x;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int o = kSpmvLanes / 2, m = kSpmmCols; o > 0; o >>= 1, m >>= 1) {
+    const bool hi = (j & o) != 0;
+#pragma unroll
+    for (int i = 0; i < m / 2; ++i) {
+      const double sr = hi ? ar[i] : ar[i + m / 2], si = hi ? ai[i] : ai[i + m / 2];
+      const double kr = hi ? ar[i + m / 2] : ar[i], ki = hi ? ai[i + m / 2] : ai[i];
+      ar[i] = kr + __shfl_xor(sr, o, kSpmvLanes);
+      ai[i] = ki + __shfl_xor(si, o, kSpmvLanes);
+    }
+  }
+  if (r < n) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int c = 2 * j + i;
+      if (c < nc) Y[(i64)(c0 + c) * n + r] = make_double2(ar[i], ai[i]);
+    }
+  }
+}
+
 __global__ void k_gather_vals(i64 nnz, const int* __restrict__ map, const double2* __restrict__ src,
                               double2* __restrict__ dst) {
   i64 p = blockIdx.x * (i64)blockDim.x + threadIdx.x;
@@ -213,8 +267,8 @@ void spmm(i64 n, int R, const int* rowptr, const int* colidx, const double* val,
           hipStream_t s) {
   if (R <= 0) return;
   DM_REQUIRE(R <= 65535, DM_ERR_ARGUMENT, "too many right-hand sides for one sparse product");
-  dim3 g((unsigned)((n * kSpmvLanes + 255) / 256), (unsigned)R);
-  k_spmv<<<g, 256, 0, s>>>(n, rowptr, colidx, reinterpret_cast<const double2*>(val),
+  dim3 g((unsigned)((n * kSpmvLanes + 255) / 256), (unsigned)((R + kSpmmCols - 1) / kSpmmCols));
+  k_spmm<<<g, 256, 0, s>>>(n, R, rowptr, colidx, reinterpret_cast<const double2*>(val),
                            reinterpret_cast<const double2*>(X), reinterpret_cast<double2*>(Y));
   HIP_OK(hipGetLastError());
 }
