@@ -309,14 +309,34 @@ def main():
     batched = None
     if not a.no_batch and a.batch_rhs > 0:
         waves = random_planewaves(dm, k, a.batch_rhs)
+        ctx.profile_reset()
+        ctx.profile(True)
         dt, X, st_b, tb = solve("mixed", nf, waves, maxiter=300)
+        ctx.profile(False)
+        bkern = {}
+        for name in ("assemble_z64_z32", "zx_split", "zx_z32", "zx_z64", "arnoldi", "precond"):
+            c, ms = ctx.profile_read(name)
+            if c:
+                bkern[name] = dict(launches=c, total_ms=round(ms, 3), avg_ms=round(ms / c, 4))
+        ctx.profile_reset()
         batched = dict(rhs=a.batch_rhs, s=round(dt, 3), s_per_rhs=round(dt / a.batch_rhs, 5),
                        iterations_total=st_b.niter, iterations_mean=round(st_b.niter / a.batch_rhs, 1),
                        refinements=st_b.outer, worst_true_residual=st_b.residual, matrix_ready_s=round(tb, 4),
                        single_rhs_chosen_s=round(t_c, 4),
                        speedup_per_rhs_vs_single_chosen=round(t_c * a.batch_rhs / dt, 2),
                        speedup_per_rhs_vs_single_plain=round(t_p * a.batch_rhs / dt, 2),
-                       note="plain path not re-run per wave; per-RHS comparisons use the single-RHS times")
+                       note="plain path not re-run per wave; per-RHS comparisons use the single-RHS times",
+                       kernels=bkern)
+        if "zx_z32" in bkern:
+            # lockstep complex64 Z X over all columns: 8 N^2 R useful FLOP per product,
+            # six bf16 part products of that size on the matrix cores (plus the FP64
+            # sum of the K chunks inside the same scope)
+            zx_ms = bkern["zx_z32"]["avg_ms"]
+            useful = 8.0 * N * N * a.batch_rhs
+            batched["zx_roofline"] = dict(bound="mfma", unit="TFLOP/s", avg_ms=zx_ms,
+                                          useful_tflops=round(useful / zx_ms / 1e9, 1),
+                                          bf16_mfma_tflops=round(6 * useful / zx_ms / 1e9, 1),
+                                          peak_bf16=2500.0, frac=round(6 * useful / zx_ms / 1e9 / 2500.0, 4))
         log(f"batched {a.batch_rhs} plane waves: {dt:.2f} s ({dt / a.batch_rhs * 1e3:.1f} ms per RHS), "
             f"{st_b.niter} it total, worst residual {st_b.residual:.2e}")
         # far fields of all currents on a 5-degree grid (E = G I on the matrix cores)

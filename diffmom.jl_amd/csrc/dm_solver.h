@@ -70,7 +70,7 @@ struct OpZ32 : DevOp {
     dml::zx_bf16(z->n, z->ld, planes.p, X, z->n, R, Y, z->n, ws, s);
   }
   size_t batch_ws(int R) const override {
-    return std::max(dml::zx_workspace_bytes(z->n, R), dml::zx_bf16_workspace_bytes(z->n, R));
+    return dml::zx_bf16_workspace_bytes(z->n, R);
   }
 };
 

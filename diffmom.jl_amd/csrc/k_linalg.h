@@ -57,6 +57,7 @@ int hier_wave_slots();  // resident waves of the product kernel on this device
 void hier_ranges(i64 n, const int* rank, int W, void* buf, HierRanges& hr, hipStream_t s);
 void symv_hier(i64 n, const float* z32p, const int* rank, const float* fac, int rmax, const double* x, double* y,
                void* ws, const HierRanges& hr, hipStream_t s);
+// Workspace of either batched product (complex64 or complex128) for R columns.
 size_t zx_workspace_bytes(i64 n, int R);
 // The complex64 Z X on the bf16 matrix cores at FP32 accuracy (each FP32
 // value split into three bf16 parts, six part products): planes of Z32 built
@@ -72,8 +73,6 @@ void zx_bf16(i64 n, i64 ld, const void* planes, const double* X, i64 ldx, int R,
 size_t gemm_c128_workspace_bytes(i64 n, int R);
 void gemm_c128(i64 mr, i64 n, const double* A, i64 lda, const double* X, i64 ldx, int R, double* Y, i64 ldy,
                void* ws, hipStream_t s);
-void zx_mfma(i64 n, i64 ld, const float* z32, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
-             hipStream_t s);
 void zx_mfma64(i64 n, i64 ld, const double* z64, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
                hipStream_t s);
 }  // namespace dml
