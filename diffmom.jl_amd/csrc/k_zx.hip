@@ -162,7 +162,7 @@ __global__ void k_split_planes64(i64 n, i64 np, int R, const double2* __restrict
 
 
 // ------------------------------------------------------------------ bf16 x 3
-// The same product on the bf16 matrix cores at FP32 accuracy: every FP32
+// The complex64 product on the bf16 matrix cores at FP32 accuracy: every FP32
 // value x is split exactly into three bf16 parts, x = h + m + l (h = bf16(x),
 // m = bf16(x - h), l = x - h - m; each difference is exact in FP32 and l
 // fits bf16), and A X is formed from the six part products h h', h m', m h',
@@ -172,8 +172,9 @@ __global__ void k_split_planes64(i64 n, i64 np, int R, const double2* __restrict
 // 16x16 product).  Planes of A' are stored in groups of 8 columns, (k, r) at
 // ((k / 8) R2 + r) 8 + k % 8, so an operand is one 16-byte read; the X planes
 // are column-major with k contiguous.  A workgroup covers 64 complex rows x
-// 64 right-hand sides for one K chunk; the four column blocks of a row block
-// are dispatched back to back on one XCD, which share its A' stages in L2.
+// 64 right-hand sides for one K chunk (four waves of 32 real rows each); the
+// four column blocks of a row block are dispatched back to back on one XCD,
+// which share its A' stages in L2.
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 constexpr int ZB_LDA = 40;  // bf16 per LDS row (32 + 8 padding)
 
@@ -232,8 +233,7 @@ __global__ __launch_bounds__(256, 1) void k_zx_bf16(i64 n, i64 R2, int nrb, int 
                                                     i64 aplane,
                                                     const __bf16* __restrict__ Xp, i64 np, i64 xplane, int R,
                                                     double2* __restrict__ part) {
-  __shared__ __align__(16) __bf16 As[3][2 * ZX_BM][ZB_LDA];
-  __shared__ __align__(16) __bf16 Bs[6][ZX_BN][ZB_LDA];
+  __shared__ __align__(16) __bf16 Bs[2][6][ZX_BN][ZB_LDA];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // XCD-aware tile order: workgroup w runs on XCD w % 8; its index j = w / 8
   // on that XCD gives row block 8 (j / ncb) + w % 8 and column block j % ncb
@@ -246,31 +246,33 @@ __global__ __launch_bounds__(256, 1) void k_zx_bf16(i64 n, i64 R2, int nrb, int 
   const i64 k_end = k_begin + ZX_KC < n ? k_begin + ZX_KC : n;
   const int nst = (int)((k_end - k_begin + 31) / 32);
   typedef unsigned int u4v __attribute__((ext_vector_type(4)));
-  u4v ra[6], rb[6];
-  // stage of 32 columns: A' 3 planes x 4 groups x 128 rows, X 6 planes x 64
-  // columns x 4 groups of 8 rows: 1536 + 1536 16-byte pieces, 6 + 6 per thread
+  // A' rows belong to one wave each (32 real rows per wave): they go from
+  // global memory straight into the MFMA operand layout (lane l: row l % 16,
+  // columns 8 (l / 16) .. + 7 of the stage, one 16-byte load from the
+  // 8-column-group planes).  X is shared by the four waves and staged in LDS
+  // (two buffers, one barrier per stage): 6 planes x 64 columns x 4 groups of
+  // 8 rows = 1536 16-byte pieces, 6 per thread.
+  const int g = lane >> 4;
+  const i64 arow = 2 * m0 + 32 * wave + (lane & 15);
+  v8bf ra[3][2];
+  u4v rb[6];
   auto gload = [&](i64 k0) {
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      const int c = tid + 256 * i, p = c >> 9, rem = c & 511, kg = rem >> 7, r = rem & 127;
-      ra[i] = *reinterpret_cast<const u4v*>(Ap + p * aplane + ((k0 / 8 + kg) * R2 + 2 * m0 + r) * 8);
-    }
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+        ra[q][r] = *reinterpret_cast<const v8bf*>(Ap + q * aplane + ((k0 / 8 + g) * R2 + arow + 16 * r) * 8);
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
       const int c = tid + 256 * i, q = c >> 8, rem = c & 255, col = rem >> 2, kq = rem & 3;
       rb[i] = *reinterpret_cast<const u4v*>(Xp + q * xplane + (i64)(c0 + col) * np + k0 + 8 * kq);
     }
   };
-  auto lstore = [&]() {
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      const int c = tid + 256 * i, p = c >> 9, rem = c & 511, kg = rem >> 7, r = rem & 127;
-      *reinterpret_cast<u4v*>(&As[p][r][8 * kg]) = ra[i];
-    }
+  auto lstore = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
       const int c = tid + 256 * i, q = c >> 8, rem = c & 255, col = rem >> 2, kq = rem & 3;
-      *reinterpret_cast<u4v*>(&Bs[q][col][8 * kq]) = rb[i];
+      *reinterpret_cast<u4v*>(&Bs[buf][q][col][8 * kq]) = rb[i];
     }
   };
   v4f accP[2][4], accQ[2][4];
@@ -282,24 +284,24 @@ __global__ __launch_bounds__(256, 1) void k_zx_bf16(i64 n, i64 R2, int nrb, int 
       accQ[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
     }
   gload(k_begin);
-  lstore();
+  lstore(0);
   __syncthreads();
-  const int g = lane >> 4;
   for (int st = 0; st < nst; ++st) {
-    if (st + 1 < nst) gload(k_begin + (i64)(st + 1) * 32);
+    const int buf = st & 1;
     // all operands of the stage, then the six part products, each over the
     // wave's 16 independent accumulator tiles (no back-to-back dependence)
     v8bf A[3][2], Br[3][4], Bi[3][4];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
 #pragma unroll
-      for (int r = 0; r < 2; ++r) A[q][r] = *reinterpret_cast<const v8bf*>(&As[q][32 * wave + 16 * r + (lane & 15)][8 * g]);
+      for (int r = 0; r < 2; ++r) A[q][r] = ra[q][r];
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) {
-        Br[q][cb] = *reinterpret_cast<const v8bf*>(&Bs[q][16 * cb + (lane & 15)][8 * g]);
-        Bi[q][cb] = *reinterpret_cast<const v8bf*>(&Bs[3 + q][16 * cb + (lane & 15)][8 * g]);
+        Br[q][cb] = *reinterpret_cast<const v8bf*>(&Bs[buf][q][16 * cb + (lane & 15)][8 * g]);
+        Bi[q][cb] = *reinterpret_cast<const v8bf*>(&Bs[buf][3 + q][16 * cb + (lane & 15)][8 * g]);
       }
     }
+    if (st + 1 < nst) gload(k_begin + (i64)(st + 1) * 32);
     constexpr int kPa[6] = {1, 0, 2, 0, 1, 0}, kPb[6] = {1, 2, 0, 1, 0, 0};  // smallest terms first
 #pragma unroll
     for (int t = 0; t < 6; ++t)
@@ -310,9 +312,10 @@ __global__ __launch_bounds__(256, 1) void k_zx_bf16(i64 n, i64 R2, int nrb, int 
           accP[r][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[kPa[t]][r], Br[kPb[t]][cb], accP[r][cb], 0, 0, 0);
           accQ[r][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[kPa[t]][r], Bi[kPb[t]][cb], accQ[r][cb], 0, 0, 0);
         }
-    __syncthreads();
+    // the other buffer was last read in stage st - 1, before the barrier that
+    // ended it
     if (st + 1 < nst) {
-      lstore();
+      lstore(buf ^ 1);
       __syncthreads();
     }
   }
