@@ -169,7 +169,7 @@ std::unique_ptr<DevPrec> make_learned_precond(const dm_zmat* z, int form, const 
 struct Krylov {
   i64 n = 0;
   int m = 0;  // basis capacity
-  DBuf<double> V, Zs, w, t, hbuf, ybuf, part;
+  DBuf<double> V, Zs, w, t, hbuf, ybuf, part, nrm;  // nrm: partials of ||w||^2
   // device Givens state: H ((m + 1) x m complex), rotations, rotated rhs,
   // per-step (estimate, subdiagonal), 1 / subdiagonal
   DBuf<double> Hd, csd, snd, gd, stt, scd;
@@ -184,6 +184,7 @@ struct Krylov {
     stt.alloc(2 * (size_t)m);
     scd.alloc(1);
     part.alloc(2 * dml::dots_part_count(n, m + 1));
+    nrm.alloc(dml::norm_part_count(n));
     V.alloc((size_t)2 * n * (m + 1));
     if (flexible) Zs.alloc((size_t)2 * n * m);
     w.alloc(2 * n);

@@ -166,12 +166,12 @@ GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double*
         ProfScope p(ctx, "arnoldi", s);
         double* h1 = K.hbuf.p;
         double* h2 = K.hbuf.p + 2 * k;
-        dml::dots(n, ldv / 2, k, V, K.w.p, h1, K.part.p, s);
-        dml::update(n, ldv / 2, k, V, h1, K.w.p, s);
-        dml::dots(n, ldv / 2, k, V, K.w.p, h2, K.part.p, s);
-        dml::update(n, ldv / 2, k, V, h2, K.w.p, s);
-        dml::norm2(n, K.w.p, K.hbuf.p + 4 * k, s);
-        dml::givens(j, mcap + 1, K.hbuf.p, K.Hd.p, K.csd.p, K.snd.p, K.gd.p, K.stt.p, K.scd.p, s);
+        dml::dots(n, ldv / 2, k, V, K.w.p, K.part.p, s);
+        dml::update(n, ldv / 2, k, V, K.part.p, h1, K.w.p, nullptr, s);
+        dml::dots(n, ldv / 2, k, V, K.w.p, K.part.p, s);
+        dml::update(n, ldv / 2, k, V, K.part.p, h2, K.w.p, K.nrm.p, s);
+        dml::givens(j, mcap + 1, K.hbuf.p, K.nrm.p, dml::norm_part_count(n), K.Hd.p, K.csd.p, K.snd.p, K.gd.p,
+                    K.stt.p, K.scd.p, s);
         dml::scale_dev(n, K.scd.p, K.w.p, col(V, k), s);
       }
       ++j;

@@ -5,10 +5,15 @@
 #include "dm_common.h"
 
 namespace dml {
-// h[i] = V_i^H w for i < k; part holds dots_part_count(n, k) complex partials
+// One CGS pass: dots writes the row-chunk partials of V_i^H w (i < k) to part
+// (dots_part_count(n, k) complex); update adds them into h[i] and sets
+// w -= V h.  With nrm (norm_part_count(n) doubles) update also leaves the
+// per-workgroup partials of ||w||^2, which givens adds up.
 size_t dots_part_count(i64 n, int k);
-void dots(i64 n, i64 ldv, int k, const double* V, const double* w, double* h, double* part, hipStream_t s);
-void update(i64 n, i64 ldv, int k, const double* V, const double* h, double* w, hipStream_t s);
+int norm_part_count(i64 n);
+void dots(i64 n, i64 ldv, int k, const double* V, const double* w, double* part, hipStream_t s);
+void update(i64 n, i64 ldv, int k, const double* V, const double* part, double* h, double* w, double* nrm,
+            hipStream_t s);
 void combine(i64 n, i64 ldv, int k, const double* V, const double* y, double* x, hipStream_t s);
 void norm2(i64 n, const double* w, double* out, hipStream_t s);
 void axpby(i64 n, double ar, double ai, const double* x, double br, double bi, double* y, hipStream_t s);
@@ -23,12 +28,13 @@ void lu_solve(i64 n, i64 ld, const double* A, const int* ipiv, double* B, i64 ld
 void unpermute(i64 n, const ZRef& z, i64 c0, i64 nc, const int* iofe, double* out, hipStream_t s);
 void entries(const ZRef& z, i64 count, const int* iofe, const i64* m, const i64* n, double* out, hipStream_t s);
 void conj(i64 n, double* v, hipStream_t s);
-// GMRES on the device: Givens step of Arnoldi column j (H: ldh x m complex,
+// GMRES on the device: Givens step of Arnoldi column j (hbuf: the two CGS
+// passes' h; nrm: nb partials of ||w||^2 from update; H: ldh x m complex,
 // column-major; cs, sn, g: rotations and rotated right-hand side; st[2j],
 // st[2j+1] = residual estimate, subdiagonal; sc[0] = 1 / subdiagonal),
 // triangular solve y = R^{-1} g of the leading j columns, y = a[0] x.
-void givens(int j, int ldh, const double* hbuf, double* H, double* cs, double* sn, double* g, double* st,
-            double* sc, hipStream_t s);
+void givens(int j, int ldh, const double* hbuf, const double* nrm, int nb, double* H, double* cs, double* sn,
+            double* g, double* st, double* sc, hipStream_t s);
 void backsolve(int j, int ldh, const double* H, const double* g, double* y, hipStream_t s);
 void scale_dev(i64 n, const double* a, const double* x, double* y, hipStream_t s);
 // y = Z x from packed symmetric storage; ws >= symv_workspace_bytes(n)

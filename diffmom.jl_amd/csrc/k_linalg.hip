@@ -127,7 +127,7 @@ __global__ void k_permute(i64 n, const int* __restrict__ idx, const double2* __r
 // ---------------------------------------------------------------- Krylov ops
 // Row-chunked dot products for the Arnoldi step: workgroup (c, g) forms the
 // partial sums of vectors g*kDotVG .. over 1024 rows (w read once per group,
-// every V entry once), reduced over chunks by k_dots_reduce.
+// every V entry once); k_update adds them over the chunks.
 constexpr int kDotVG = 4;
 constexpr int kDotRows = 4;  // rows per thread
 
@@ -164,40 +164,54 @@ __global__ __launch_bounds__(256) void k_dots_part(i64 n, i64 ldv, int k, const 
   __syncthreads();
   if (threadIdx.x < cnt) {
     const int q = threadIdx.x;
-    part[(i64)(i0 + q) * nch + c] = make_double2(red[0][q].x + red[1][q].x + red[2][q].x + red[3][q].x,
-                                                 red[0][q].y + red[1][q].y + red[2][q].y + red[3][q].y);
+    part[(i64)c * k + i0 + q] = make_double2(red[0][q].x + red[1][q].x + red[2][q].x + red[3][q].x,
+                                             red[0][q].y + red[1][q].y + red[2][q].y + red[3][q].y);
   }
 }
 
-__global__ void k_dots_reduce(int nch, const double2* __restrict__ part, double2* __restrict__ h) {
-  const int i = blockIdx.x;
-  double sr = 0, si = 0;
-  for (int c = threadIdx.x; c < nch; c += 64) {
-    double2 v = part[(i64)i * nch + c];
-    sr += v.x;
-    si += v.y;
-  }
-  sr = wave_sum(sr);
-  si = wave_sum(si);
-  if (threadIdx.x == 0) h[i] = make_double2(sr, si);
-}
-
-// w[n] -= sum_{i<k} V[n, i] h[i]  (acc_h: h_acc[i] += h[i] for CGS2 bookkeeping)
+// w[m] -= sum_{i<k} V[m, i] h[i], where h[i] = sum_c part[c * k + i] are
+// the chunk partials of k_dots_part: every workgroup adds them itself (thread
+// i over the chunks in order, all loads independent, so all workgroups hold
+// the same h) and workgroup 0 stores h.  nrm, when given, receives each
+// workgroup's sum of |w[m]|^2 over its rows after the update.
 __global__ __launch_bounds__(256) void k_update(i64 n, i64 ldv, int k, const double2* __restrict__ V,
-                                                const double2* __restrict__ h, double2* __restrict__ w) {
+                                                const double2* __restrict__ part, int nch, double2* __restrict__ h,
+                                                double2* __restrict__ w, double* __restrict__ nrm) {
   __shared__ double2 hs[1024];
-  for (int i = threadIdx.x; i < k; i += 256) hs[i] = h[i];
-  __syncthreads();
-  i64 m = blockIdx.x * (i64)256 + threadIdx.x;
-  if (m >= n) return;
-  double ar = 0, ai = 0;
-  for (int i = 0; i < k; ++i) {
-    double2 a = V[(i64)i * ldv + m], b = hs[i];
-    ar += a.x * b.x - a.y * b.y;
-    ai += a.x * b.y + a.y * b.x;
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < k; i += 256) {
+    double sr = 0, si = 0;
+#pragma unroll 8
+    for (int c = 0; c < nch; ++c) {
+      const double2 v = part[(i64)c * k + i];
+      sr += v.x;
+      si += v.y;
+    }
+    hs[i] = make_double2(sr, si);
+    if (blockIdx.x == 0) h[i] = make_double2(sr, si);
   }
-  double2 o = w[m];
-  w[m] = make_double2(o.x - ar, o.y - ai);
+  __syncthreads();
+  const i64 m = blockIdx.x * (i64)256 + threadIdx.x;
+  double s2 = 0;
+  if (m < n) {
+    double ar = 0, ai = 0;
+    for (int i = 0; i < k; ++i) {
+      double2 a = V[(i64)i * ldv + m], b = hs[i];
+      ar += a.x * b.x - a.y * b.y;
+      ai += a.x * b.y + a.y * b.x;
+    }
+    const double2 o = w[m];
+    const double2 u = make_double2(o.x - ar, o.y - ai);
+    w[m] = u;
+    s2 = u.x * u.x + u.y * u.y;
+  }
+  if (nrm) {
+    s2 = wave_sum(s2);
+    if (lane == 0) red[wave] = s2;
+    __syncthreads();
+    if (threadIdx.x == 0) nrm[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+  }
 }
 
 // out[0] = ||w||^2 (single workgroup of 1024 threads)
@@ -478,14 +492,16 @@ __global__ __launch_bounds__(1024) void k_lu_solve(i64 n, i64 ld, const double2*
 
 // ------------------------------------------------ GMRES Givens on the device
 // Column j of the Hessenberg matrix from the two CGS2 passes and the norm of
-// the orthogonalised vector (hbuf = [h1 (k complex), h2 (k complex), ||w||^2],
-// k = j + 1), the previous rotations applied, a new rotation chosen so that
+// the orthogonalised vector (hbuf = [h1 (k complex), h2 (k complex)], k =
+// j + 1; ||w||^2 = the sum of the nb per-workgroup partials nrm of the last
+// update, added by one wave in a fixed order), the previous rotations
+// applied, a new rotation chosen so that
 // H[j+1][j] = 0, and g (the rotated right-hand side beta e_1) updated.
 // st[2j], st[2j+1] = |g[j+1]| (the residual estimate) and the subdiagonal
 // (0 = breakdown); sc[0] = 1 / subdiagonal (0 at breakdown) scales v_{j+1}.
-__global__ void k_givens(int j, int ldh, const double* __restrict__ hbuf, double2* __restrict__ H,
-                         double* __restrict__ cs, double2* __restrict__ sn, double2* __restrict__ g,
-                         double* __restrict__ st, double* __restrict__ sc) {
+__global__ void k_givens(int j, int ldh, const double* __restrict__ hbuf, const double* __restrict__ nrm, int nb,
+                         double2* __restrict__ H, double* __restrict__ cs, double2* __restrict__ sn,
+                         double2* __restrict__ g, double* __restrict__ st, double* __restrict__ sc) {
   const int k = j + 1;
   double2* Hj = H + (i64)j * ldh;
   // the column and the previous rotations are staged in LDS by all threads;
@@ -500,9 +516,16 @@ __global__ void k_givens(int j, int ldh, const double* __restrict__ hbuf, double
     rs[i] = sn[i];
     rc[i] = cs[i];
   }
+  __shared__ double wn2;
+  if (threadIdx.x < 64) {
+    double t = 0;
+    for (int i = threadIdx.x; i < nb; i += 64) t += nrm[i];
+    t = wave_sum(t);
+    if (threadIdx.x == 0) wn2 = t;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const double hn = sqrt(hbuf[4 * k]);
+    const double hn = sqrt(wn2);
     // previous rotations: [c s; -conj(s) c]
     double2 a = col[0];
     for (int i = 0; i < j; ++i) {
@@ -637,20 +660,23 @@ namespace dml {
 
 size_t dots_part_count(i64 n, int k) { return (size_t)k * (size_t)nblocks(n, 256 * kDotRows); }
 
-void dots(i64 n, i64 ldv, int k, const double* V, const double* w, double* h, double* part, hipStream_t s) {
+int norm_part_count(i64 n) { return (int)nblocks(n, 256); }
+void dots(i64 n, i64 ldv, int k, const double* V, const double* w, double* part, hipStream_t s) {
   if (k <= 0) return;
   const int nch = (int)nblocks(n, 256 * kDotRows);
   dim3 g((unsigned)nch, (unsigned)((k + kDotVG - 1) / kDotVG));
   k_dots_part<<<g, 256, 0, s>>>(n, ldv, k, reinterpret_cast<const double2*>(V), reinterpret_cast<const double2*>(w),
                                 reinterpret_cast<double2*>(part), nch);
-  k_dots_reduce<<<k, 64, 0, s>>>(nch, reinterpret_cast<const double2*>(part), reinterpret_cast<double2*>(h));
   HIP_OK(hipGetLastError());
 }
-void update(i64 n, i64 ldv, int k, const double* V, const double* h, double* w, hipStream_t s) {
+void update(i64 n, i64 ldv, int k, const double* V, const double* part, double* h, double* w, double* nrm,
+            hipStream_t s) {
   if (k <= 0) return;
   DM_REQUIRE(k <= 1024, DM_ERR_ARGUMENT, "Krylov basis larger than 1024 vectors");
+  const int nch = (int)nblocks(n, 256 * kDotRows);
   k_update<<<nblocks(n, 256), 256, 0, s>>>(n, ldv, k, reinterpret_cast<const double2*>(V),
-                                            reinterpret_cast<const double2*>(h), reinterpret_cast<double2*>(w));
+                                            reinterpret_cast<const double2*>(part), nch, reinterpret_cast<double2*>(h),
+                                            reinterpret_cast<double2*>(w), nrm);
   HIP_OK(hipGetLastError());
 }
 void combine(i64 n, i64 ldv, int k, const double* V, const double* y, double* x, hipStream_t s) {
@@ -660,12 +686,12 @@ void combine(i64 n, i64 ldv, int k, const double* V, const double* y, double* x,
                                              reinterpret_cast<const double2*>(y), reinterpret_cast<double2*>(x));
   HIP_OK(hipGetLastError());
 }
-void givens(int j, int ldh, const double* hbuf, double* H, double* cs, double* sn, double* g, double* st,
-            double* sc, hipStream_t s) {
+void givens(int j, int ldh, const double* hbuf, const double* nrm, int nb, double* H, double* cs, double* sn,
+            double* g, double* st, double* sc, hipStream_t s) {
   const size_t smem = sizeof(double2) * (2 * (size_t)j + 2) + sizeof(double) * j;
-  DM_REQUIRE(smem <= 64 * 1024, DM_ERR_ARGUMENT, "Krylov basis too large for the device Givens step");
-  k_givens<<<1, 256, smem, s>>>(j, ldh, hbuf, reinterpret_cast<double2*>(H), cs, reinterpret_cast<double2*>(sn),
-                                reinterpret_cast<double2*>(g), st, sc);
+  DM_REQUIRE(smem <= 60 * 1024, DM_ERR_ARGUMENT, "Krylov basis too large for the device Givens step");
+  k_givens<<<1, 256, smem, s>>>(j, ldh, hbuf, nrm, nb, reinterpret_cast<double2*>(H), cs,
+                                reinterpret_cast<double2*>(sn), reinterpret_cast<double2*>(g), st, sc);
   HIP_OK(hipGetLastError());
 }
 
