@@ -170,9 +170,9 @@ GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double*
         dml::update(n, ldv / 2, k, V, K.part.p, h1, K.w.p, nullptr, s);
         dml::dots(n, ldv / 2, k, V, K.w.p, K.part.p, s);
         dml::update(n, ldv / 2, k, V, K.part.p, h2, K.w.p, K.nrm.p, s);
-        dml::givens(j, mcap + 1, K.hbuf.p, K.nrm.p, dml::norm_part_count(n), K.Hd.p, K.csd.p, K.snd.p, K.gd.p,
-                    K.stt.p, K.scd.p, s);
-        dml::scale_dev(n, K.scd.p, K.w.p, col(V, k), s);
+        // rotations, and v_{j+1} = w / ||w|| in the same launch
+        dml::givens(j, mcap + 1, K.hbuf.p, K.nrm.p, dml::norm_part_count(n), n, K.w.p, col(V, k), K.Hd.p, K.csd.p,
+                    K.snd.p, K.gd.p, K.stt.p, K.scd.p, s);
       }
       ++j;
       ++total;

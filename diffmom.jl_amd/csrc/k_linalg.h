@@ -29,14 +29,13 @@ void unpermute(i64 n, const ZRef& z, i64 c0, i64 nc, const int* iofe, double* ou
 void entries(const ZRef& z, i64 count, const int* iofe, const i64* m, const i64* n, double* out, hipStream_t s);
 void conj(i64 n, double* v, hipStream_t s);
 // GMRES on the device: Givens step of Arnoldi column j (hbuf: the two CGS
-// passes' h; nrm: nb partials of ||w||^2 from update; H: ldh x m complex,
-// column-major; cs, sn, g: rotations and rotated right-hand side; st[2j],
-// st[2j+1] = residual estimate, subdiagonal; sc[0] = 1 / subdiagonal),
-// triangular solve y = R^{-1} g of the leading j columns, y = a[0] x.
-void givens(int j, int ldh, const double* hbuf, const double* nrm, int nb, double* H, double* cs, double* sn,
-            double* g, double* st, double* sc, hipStream_t s);
+// passes' h; nrm: nb partials of ||w||^2 from update; vnext = w / ||w||, n
+// rows; H: ldh x m complex, column-major; cs, sn, g: rotations and rotated
+// right-hand side; st[2j], st[2j+1] = residual estimate, subdiagonal; sc[0] =
+// 1 / subdiagonal), triangular solve y = R^{-1} g of the leading j columns.
+void givens(int j, int ldh, const double* hbuf, const double* nrm, int nb, i64 n, const double* w, double* vnext,
+            double* H, double* cs, double* sn, double* g, double* st, double* sc, hipStream_t s);
 void backsolve(int j, int ldh, const double* H, const double* g, double* y, hipStream_t s);
-void scale_dev(i64 n, const double* a, const double* x, double* y, hipStream_t s);
 // y = Z x from packed symmetric storage; ws >= symv_workspace_bytes(n)
 size_t symv_workspace_bytes(i64 n);
 // shape: 2..64 = LDS-staged form with that many tiles per workgroup segment;

@@ -498,10 +498,15 @@ __global__ __launch_bounds__(1024) void k_lu_solve(i64 n, i64 ld, const double2*
 // applied, a new rotation chosen so that
 // H[j+1][j] = 0, and g (the rotated right-hand side beta e_1) updated.
 // st[2j], st[2j+1] = |g[j+1]| (the residual estimate) and the subdiagonal
-// (0 = breakdown); sc[0] = 1 / subdiagonal (0 at breakdown) scales v_{j+1}.
-__global__ void k_givens(int j, int ldh, const double* __restrict__ hbuf, const double* __restrict__ nrm, int nb,
-                         double2* __restrict__ H, double* __restrict__ cs, double2* __restrict__ sn,
-                         double2* __restrict__ g, double* __restrict__ st, double* __restrict__ sc) {
+// (0 = breakdown); sc[0] = 1 / subdiagonal (0 at breakdown).  The launch
+// covers the n rows: every workgroup forms the same norm and writes its rows
+// of v_{j+1} = sc w; workgroup 0 also does the rotations.
+__global__ __launch_bounds__(256) void k_givens(int j, int ldh, const double* __restrict__ hbuf,
+                                                const double* __restrict__ nrm, int nb, i64 n,
+                                                const double2* __restrict__ w, double2* __restrict__ vnext,
+                                                double2* __restrict__ H, double* __restrict__ cs,
+                                                double2* __restrict__ sn, double2* __restrict__ g,
+                                                double* __restrict__ st, double* __restrict__ sc) {
   const int k = j + 1;
   double2* Hj = H + (i64)j * ldh;
   // the column and the previous rotations are staged in LDS by all threads;
@@ -510,11 +515,13 @@ __global__ void k_givens(int j, int ldh, const double* __restrict__ hbuf, const 
   double2* col = sh;          // k + 1
   double2* rs = sh + k + 1;   // j rotations' s
   double* rc = reinterpret_cast<double*>(rs + j);  // j rotations' c
-  for (int i = threadIdx.x; i < k; i += blockDim.x)
-    col[i] = make_double2(hbuf[2 * i] + hbuf[2 * k + 2 * i], hbuf[2 * i + 1] + hbuf[2 * k + 2 * i + 1]);
-  for (int i = threadIdx.x; i < j; i += blockDim.x) {
-    rs[i] = sn[i];
-    rc[i] = cs[i];
+  if (blockIdx.x == 0) {
+    for (int i = threadIdx.x; i < k; i += blockDim.x)
+      col[i] = make_double2(hbuf[2 * i] + hbuf[2 * k + 2 * i], hbuf[2 * i + 1] + hbuf[2 * k + 2 * i + 1]);
+    for (int i = threadIdx.x; i < j; i += blockDim.x) {
+      rs[i] = sn[i];
+      rc[i] = cs[i];
+    }
   }
   __shared__ double wn2;
   if (threadIdx.x < 64) {
@@ -524,6 +531,15 @@ __global__ void k_givens(int j, int ldh, const double* __restrict__ hbuf, const 
     if (threadIdx.x == 0) wn2 = t;
   }
   __syncthreads();
+  {
+    const double hn = sqrt(wn2), scl = hn > 0 ? 1.0 / hn : 0.0;
+    const i64 m = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+    if (m < n) {
+      const double2 v = w[m];
+      vnext[m] = make_double2(scl * v.x, scl * v.y);
+    }
+  }
+  if (blockIdx.x != 0) return;
   if (threadIdx.x == 0) {
     const double hn = sqrt(wn2);
     // previous rotations: [c s; -conj(s) c]
@@ -593,15 +609,6 @@ __global__ __launch_bounds__(256) void k_backsolve(int j, int ldh, const double2
     }
     __syncthreads();
   }
-}
-
-__global__ void k_scale_dev(i64 n, const double* __restrict__ a, const double2* __restrict__ x,
-                            double2* __restrict__ y) {
-  const i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double s = a[0];
-  const double2 v = x[i];
-  y[i] = make_double2(s * v.x, s * v.y);
 }
 
 inline unsigned nblocks(i64 n, int t) { return (unsigned)((n + t - 1) / t); }
@@ -686,12 +693,13 @@ void combine(i64 n, i64 ldv, int k, const double* V, const double* y, double* x,
                                              reinterpret_cast<const double2*>(y), reinterpret_cast<double2*>(x));
   HIP_OK(hipGetLastError());
 }
-void givens(int j, int ldh, const double* hbuf, const double* nrm, int nb, double* H, double* cs, double* sn,
-            double* g, double* st, double* sc, hipStream_t s) {
+void givens(int j, int ldh, const double* hbuf, const double* nrm, int nb, i64 n, const double* w, double* vnext,
+            double* H, double* cs, double* sn, double* g, double* st, double* sc, hipStream_t s) {
   const size_t smem = sizeof(double2) * (2 * (size_t)j + 2) + sizeof(double) * j;
   DM_REQUIRE(smem <= 60 * 1024, DM_ERR_ARGUMENT, "Krylov basis too large for the device Givens step");
-  k_givens<<<1, 256, smem, s>>>(j, ldh, hbuf, nrm, nb, reinterpret_cast<double2*>(H), cs,
-                                reinterpret_cast<double2*>(sn), reinterpret_cast<double2*>(g), st, sc);
+  k_givens<<<nblocks(n, 256), 256, smem, s>>>(j, ldh, hbuf, nrm, nb, n, reinterpret_cast<const double2*>(w),
+                                              reinterpret_cast<double2*>(vnext), reinterpret_cast<double2*>(H), cs,
+                                              reinterpret_cast<double2*>(sn), reinterpret_cast<double2*>(g), st, sc);
   HIP_OK(hipGetLastError());
 }
 
@@ -703,11 +711,6 @@ void backsolve(int j, int ldh, const double* H, const double* g, double* y, hipS
   HIP_OK(hipGetLastError());
 }
 
-void scale_dev(i64 n, const double* a, const double* x, double* y, hipStream_t s) {
-  k_scale_dev<<<nblocks(n, 256), 256, 0, s>>>(n, a, reinterpret_cast<const double2*>(x),
-                                              reinterpret_cast<double2*>(y));
-  HIP_OK(hipGetLastError());
-}
 
 void norm2(i64 n, const double* w, double* out, hipStream_t s) {
   k_norm2<<<1, 1024, 0, s>>>(n, reinterpret_cast<const double2*>(w), out);
