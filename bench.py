@@ -211,8 +211,8 @@ def main():
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
-              "(the sharding design is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU; runs on more than one GPU are not supported.",
+              file=sys.stderr)
         sys.exit(2)
     dm = load_pkg()
     if a.traffic_probe:

@@ -6,7 +6,7 @@
  * The reference exposes no FFI: its "operator API" is Julia's AbstractMatrix
  * protocol plus the functions cited below.  Each entry point names the
  * reference function it replaces (path:line in DiffMoM.jl).  A Julia host binds
- * these with `ccall` (one `ccall` per function, see INTEGRATION.md).
+ * these with `ccall` (one `ccall` per function).
  *
  * Conventions (identical to Julia's in-memory layout, so Julia arrays pass
  * through unchanged):

@@ -9,7 +9,7 @@
 * int1overR_mp.json — S(P) = int_T 1/|P - r'| dS' for seeded triangles / points:
                    in-plane points by the exact polar reduction (apex-P sub-triangles,
                    int rho_max(theta) dtheta, mpmath.quad), off-plane by mpmath 2-D quadrature.
-* c0_plate.npz   — configs[0] problem (make_rect_plate(0.1,0.1,6,6), k=2pi/0.1,
+* c0_plate.npz   — the 0.1 m plate problem (make_rect_plate(0.1,0.1,6,6), k=2pi/0.1,
                    quad 3, normal incidence x-pol): RWG arrays, Z, v, I from the oracle.
 """
 import json

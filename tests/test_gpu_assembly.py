@@ -91,7 +91,7 @@ def test_plate_30x30_full_matrix(dm, O):
 
 
 def test_c1_plate_sampled_entries(dm, O):
-    """configs[1] plate (0.5 m, 60x60, N=10680): 2000 sampled entries including
+    """The 0.5 m plate (60x60, N=10680): 2000 sampled entries including
     the diagonal and near entries, FP64 and complex64."""
     m = dm.make_rect_plate(0.5, 0.5, 60, 60)
     r = dm.build_rwg(m)

@@ -82,7 +82,7 @@ def test_multiple_rhs(dm, c0):
 
 
 def test_plate_10k_gmres_true_residual(dm):
-    """configs[1]-shaped solve at reduced size (plate 0.5 m, 30x30): GMRES to
+    """The 0.5 m plate solve at reduced size (30x30 instead of 60x60): GMRES to
     1e-8 with the near-field preconditioner and a theta=30 deg incidence."""
     m = dm.make_rect_plate(0.5, 0.5, 30, 30)
     r = dm.build_rwg(m)
@@ -428,7 +428,7 @@ def test_repeated_solves_reuse_matrix_buffers(dm):
 
 
 def test_c1_plate_chosen_form_matches_plain_fp64(dm):
-    """configs[1] plate (0.5 m, 60 x 60, N = 10680, theta = 30 deg, phi = 0,
+    """The 0.5 m plate (60 x 60, N = 10680, theta = 30 deg, phi = 0,
     theta-polarised): the chosen form (one FP64 assembly with its complex64
     rounding, compressed complex64 matrix, SPAI, FGMRES, FP64 refinement)
     reaches a true FP64 residual <= 1e-6 against the dense complex128 matrix
