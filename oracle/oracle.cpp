@@ -10,8 +10,8 @@
 // 3 x Nt), complex values are interleaved (re, im) doubles.
 //
 // The reference's BigFloat fallbacks (extreme exponents, cancellation) are not
-// restated: they never trigger on the meshes and wavenumbers this build
-// targets (coordinates ~1e-3..1 m, k ~ 60 rad/m).  The fast-path formulas are
+// restated: they never trigger on the meshes and wavenumbers of the tests and
+// benchmark (coordinates ~1e-3..1 m, k ~ 60 rad/m).  The fast-path formulas are
 // restated exactly, including evaluation order where it is cheap to keep.
 //
 // Pinning status (what checks each part; see tests/test_oracle_*.py):
