@@ -15,7 +15,7 @@
 namespace {
 
 constexpr int BT = 256;          // threads per workgroup
-constexpr int BROWS = 4;         // rows per thread in the dot kernel
+constexpr int BROWS = 16;        // rows per thread in the dot kernel (one wave reduction per 16 KB read)
 constexpr int BCHUNK = BT * BROWS;
 constexpr int BKMAX = 512;       // basis vectors per column
 
