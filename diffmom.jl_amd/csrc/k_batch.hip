@@ -85,7 +85,9 @@ __global__ void k_bdots_reduce(int R, int k, int kcap, int nch, const double2* _
   }
 }
 
-// W[r] -= sum_{i<k} V_i[r] h[r, i]
+// W[r] -= sum_{i<k} V_i[r] h[r, i]; each thread takes BUROWS rows (stride
+// BT), so a workgroup's staging of h serves BT * BUROWS rows
+constexpr int BUROWS = 4;
 __global__ __launch_bounds__(BT) void k_bupdate(i64 n, int R, int k, int kcap, const double2* __restrict__ V,
                                                 const double2* __restrict__ h, const int* __restrict__ act,
                                                 double2* __restrict__ W) {
@@ -94,17 +96,29 @@ __global__ __launch_bounds__(BT) void k_bupdate(i64 n, int R, int k, int kcap, c
   __shared__ double2 hs[BKMAX];
   for (int i = threadIdx.x; i < k; i += BT) hs[i] = h[(i64)r * kcap + i];
   __syncthreads();
-  const i64 m = blockIdx.x * (i64)BT + threadIdx.x;
-  if (m >= n) return;
-  const i64 NR = n * R, off = (i64)r * n + m;
-  double ar = 0, ai = 0;
+  const i64 m0 = blockIdx.x * (i64)(BT * BUROWS) + threadIdx.x;
+  const i64 NR = n * R, off = (i64)r * n + m0;
+  double ar[BUROWS], ai[BUROWS];
+#pragma unroll
+  for (int u = 0; u < BUROWS; ++u) ar[u] = ai[u] = 0.0;
   for (int i = 0; i < k; ++i) {
-    double2 a = V[(i64)i * NR + off], b = hs[i];
-    ar += a.x * b.x - a.y * b.y;
-    ai += a.x * b.y + a.y * b.x;
+    const double2 b = hs[i];
+#pragma unroll
+    for (int u = 0; u < BUROWS; ++u) {
+      if (m0 + BT * u < n) {
+        const double2 a = V[(i64)i * NR + off + BT * u];
+        ar[u] += a.x * b.x - a.y * b.y;
+        ai[u] += a.x * b.y + a.y * b.x;
+      }
+    }
   }
-  double2 o = W[off];
-  W[off] = make_double2(o.x - ar, o.y - ai);
+#pragma unroll
+  for (int u = 0; u < BUROWS; ++u) {
+    if (m0 + BT * u < n) {
+      const double2 o = W[off + BT * u];
+      W[off + BT * u] = make_double2(o.x - ar[u], o.y - ai[u]);
+    }
+  }
 }
 
 // X[r] += sum_{i<cnt[r]} V_i[r] y[r, i]
@@ -313,9 +327,9 @@ void update(i64 n, int R, int k, int kcap, const double* V, const double* h, con
             hipStream_t s) {
   if (k <= 0) return;
   DM_REQUIRE(k <= BKMAX && k <= kcap, DM_ERR_ARGUMENT, "batched Krylov basis larger than 512 vectors");
-  k_bupdate<<<dim3(row_blocks(n), (unsigned)R), BT, 0, s>>>(n, R, k, kcap, reinterpret_cast<const double2*>(V),
-                                                            reinterpret_cast<const double2*>(h), act,
-                                                            reinterpret_cast<double2*>(W));
+  const dim3 g((unsigned)((n + BT * BUROWS - 1) / (BT * BUROWS)), (unsigned)R);
+  k_bupdate<<<g, BT, 0, s>>>(n, R, k, kcap, reinterpret_cast<const double2*>(V), reinterpret_cast<const double2*>(h),
+                             act, reinterpret_cast<double2*>(W));
   HIP_OK(hipGetLastError());
 }
 
