@@ -621,7 +621,30 @@ __global__ __launch_bounds__(256) void k_symv_w_reduce(i64 n, int nb, int tpw, c
       si += v.y;
     }
     const i64 c0 = (i64)R * (R + 1) / 2;
-    for (int I = grp; I <= R; I += 8) {
+    int I = grp;
+    // four tiles per step: their end flags, then the flagged partials, are
+    // loaded together (an unflagged tile adds an exact zero, so the sum is
+    // the same as tile by tile)
+    for (; I + 24 <= R; I += 32) {
+      bool f[4];
+      double2 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const i64 t = c0 + I + 8 * u;
+        f[u] = rend ? rend[t] != 0 : (I + 8 * u == R || (t + 1) % tpw == 0);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const i64 t = c0 + I + 8 * u;
+        v[u] = f[u] ? colpart[t * kPT + r] : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        sr += v[u].x;
+        si += v[u].y;
+      }
+    }
+    for (; I <= R; I += 8) {
       const i64 t = c0 + I;
       if (rend ? rend[t] != 0 : (I == R || (t + 1) % tpw == 0)) {
         const double2 v = colpart[t * kPT + r];
