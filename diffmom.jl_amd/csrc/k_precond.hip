@@ -159,6 +159,7 @@ __global__ __launch_bounds__(256) void k_spmv(i64 n, const int* __restrict__ row
   double ar = 0, ai = 0;
   if (r < n) {
     const int p1 = rowptr[r + 1];
+#pragma unroll 4
     for (int p = rowptr[r] + j; p < p1; p += kSpmvLanes) {
       double2 a = val[p], v = x[colidx[p]];
       ar += a.x * v.x - a.y * v.y;
