@@ -248,6 +248,28 @@ def test_batched_solve_matches_per_column(dm, sphere2, method, precond):
     assert np.linalg.norm(Xb - Id) / np.linalg.norm(Id) < 1e-4
 
 
+def test_batched_nearfield_many_column_groups(dm, sphere2):
+    """40 right-hand sides in one lockstep batch with the near-field
+    preconditioner: the batched sparse product takes 16 columns per pass over
+    each row (groups of 16, 16 and 8 here), and every column must match its
+    one-column solve (the product adds each row's partial sums in the
+    one-column kernel's order)."""
+    m, r, Z, _ = sphere2
+    rng = np.random.default_rng(20261203)
+    V = dm.assemble_multiple_excitations(m, r, _random_planewaves(dm, rng, K0, 40))
+    P = dm.build_nearfield_preconditioner(Z, m, r, 2.0 * float(np.mean(r.len)))
+    kw = dict(preconditioner=P, tol=1e-8, maxiter=300, precond_side="right")
+    Xb, sb = dm.solve_gmres(Z, V, batch=1, **kw)
+    Xs, ss = dm.solve_gmres(Z, V, batch=-1, **kw)
+    assert sb.solved and ss.solved
+    Zh = Z.to_numpy()
+    res = np.linalg.norm(Zh @ Xb - V, axis=0) / np.linalg.norm(V, axis=0)
+    assert res.max() <= max(1.05e-8, sb.residual * 1.01)
+    rel = np.linalg.norm(Xb - Xs, axis=0) / np.linalg.norm(Xs, axis=0)
+    assert rel.max() < 1e-5
+    assert abs(sb.niter - ss.niter) <= max(2, 0.03 * ss.niter)
+
+
 def test_batched_restart_single_column_and_nonconvergence(dm, sphere2):
     m, r, Z, V = sphere2
     Zh = Z.to_numpy()
