@@ -32,6 +32,30 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// The same sum through DPP moves (no LDS round trip per step): pairs, quads,
+// half rows, rows, then row broadcasts 15 and 31; lane 63 holds the total,
+// which readlane hands to every lane.  Each step moves the two 32-bit halves
+// of the double; lanes outside a step's row mask add zero.
+template <int CTRL, int ROWS>
+__device__ inline double dpp_add_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffLL), CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xF, false);
+  return v + __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ inline double wave_sum_dpp(double x) {
+  x = dpp_add_f64<0xB1, 0xF>(x);   // quad_perm 1,0,3,2
+  x = dpp_add_f64<0x4E, 0xF>(x);   // quad_perm 2,3,0,1
+  x = dpp_add_f64<0x141, 0xF>(x);  // row_half_mirror
+  x = dpp_add_f64<0x140, 0xF>(x);  // row_mirror
+  x = dpp_add_f64<0x142, 0xA>(x);  // row_bcast15
+  x = dpp_add_f64<0x143, 0xC>(x);  // row_bcast31
+  const long long b = __double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffLL), 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (long long)lo);
+}
+
 // y = Z x, complex128 Z.  Each wave owns NC consecutive columns; lane l reads
 // rows l, l+64, ...; x is shared by the NC columns from registers.
 template <int NC>
@@ -157,8 +181,8 @@ __global__ __launch_bounds__(256) void k_dots_part(i64 n, i64 ldv, int k, const 
         ai += a.x * wm[u].y - a.y * wm[u].x;
       }
     }
-    ar = wave_sum(ar);
-    ai = wave_sum(ai);
+    ar = wave_sum_dpp(ar);
+    ai = wave_sum_dpp(ai);
     if (lane == 0) red[wave][q] = make_double2(ar, ai);
   }
   __syncthreads();
