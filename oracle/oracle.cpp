@@ -14,16 +14,27 @@
 // benchmark (coordinates ~1e-3..1 m, k ~ 60 rad/m).  The fast-path formulas are
 // restated exactly, including evaluation order where it is cheap to keep.
 //
-// Pinning status (what checks each part; see tests/test_oracle_*.py):
-//   quadrature            literal tables + polynomial exactness
-//   greens / greens_smooth mpmath closed form at 50 digits, R=0 special values
-//   int_1overR            reference golden values (Test 44) + mpmath 2-D quadrature
-//   self/adjacent cells   mpmath brute-force of the same split on a 2-triangle mesh
+// Pinning status (the checks in tests/test_oracle.py; fixtures and the scripts
+// that wrote them under tests/golden/):
+//   quadrature            literal tables + polynomial exactness (and inexactness
+//                         one degree higher)
+//   greens / greens_smooth mpmath closed forms at 50 digits (green_mp.json), R=0
+//                         special values, exprel branch continuity
+//   int_1overR            reference golden values (Test 44, test44.json) +
+//                         mpmath polar reduction / 2-D quadrature (int1overR_mp.json)
+//   self / adjacent / far independent 30-digit mpmath restatement of the pair
+//   pair terms            terms on a bent, jittered 2 x 2-cell plate, real and
+//                         lossy k, every adjacent pair in both (outer, inner)
+//                         orders (cells_mp.json, make_cells_mp.py; S there by
+//                         polar reduction, not the closed form)
+//   efie entry            the same fixture's entries Z[m,n] and Z[n,m] (outer =
+//                         basis min(m,n)), separated from the swapped rule;
+//                         symmetry, matrix-free == dense, quad-order 3->4 < 15 %
 //   adjacency             40-pair known answer of the 4x4 plate (Test 45)
-//   efie entry / Z        symmetry, matrix-free == dense, quad-order 3->4 < 15 %,
-//                         LU residual, sphere-vs-Mie RCS gate (Test 13)
-//   plane-wave RHS        closed form on a single triangle pair (mpmath)
-//   radiation vectors     Mie gate + closed form of a single RWG dipole moment
+//   solves / physics      LU residual, GMRES == direct, sphere-vs-Mie RCS gate
+//                         (Test 13)
+//   plane-wave RHS        closed form at broadside on a single RWG (numpy)
+//   radiation vectors     closed form at broadside on a single RWG + Mie gate
 
 #include <algorithm>
 #include <array>
@@ -405,42 +416,46 @@ cd adjacent_cell(const Cache& c, i64 tm, i64 tn, const V3* fm, const V3* fn, con
   return smooth + sing;
 }
 
-// One EFIE entry (0-based m, n): src/assembly/EFIE.jl:429-512
+// The term of one triangle pair in _efie_entry (src/assembly/EFIE.jl:453-508):
+// side itm of basis m (0-based) is the outer triangle, side itn of basis n the
+// inner one; self, edge-adjacent or far branch by the triangles' relation.
+cd pair_term(const Cache& c, i64 m, int itm, i64 n, int itn) {
+  const Rwg& r = c.rwg;
+  i64 tm = itm == 0 ? r.tplus[m] : r.tminus[m];
+  double Am = c.areas[tm - 1];
+  double dvm = c.divv[2 * m + itm];
+  const V3* fm = &c.fv[(m * 2 + itm) * c.nq];
+  const V3* fmh = &c.fv_hi[(m * 2 + itm) * 7];
+  i64 tn = itn == 0 ? r.tplus[n] : r.tminus[n];
+  double An = c.areas[tn - 1];
+  double dvn = c.divv[2 * n + itn];
+  const V3* fn = &c.fv[(n * 2 + itn) * c.nq];
+  const V3* fnh = &c.fv_hi[(n * 2 + itn) * 7];
+  if (tm == tn) return self_cell(c, tm, fmh, fnh, dvm, dvn, Am);
+  if (is_adjacent(c, tm, tn)) return adjacent_cell(c, tm, tn, fm, fn, fmh, fnh, dvm, dvn, Am, An);
+  cd val(0, 0);
+  cd dv = dvm * dvn * c.inv_k2;
+  const V3* qm = &c.qp[(tm - 1) * c.nq];
+  const V3* qn = &c.qp[(tn - 1) * c.nq];
+  for (int a = 0; a < c.nq; ++a)
+    for (int b = 0; b < c.nq; ++b) {
+      cd G = greens(qm[a], qn[b], c.k);
+      cd vec = vdot(fm[a], fn[b]) * G;
+      cd scl = dv * G;
+      double wt = c.wq[a] * c.wq[b] * (2 * Am) * (2 * An);
+      val += (vec - scl) * wt;
+    }
+  return val;
+}
+
+// One EFIE entry (0-based m, n): src/assembly/EFIE.jl:429-512.  Non-Bloch
+// entries are evaluated as (min, max) (:443-445), so basis min(m, n) owns the
+// outer triangles.
 cd efie_entry(const Cache& c, i64 m, i64 n) {
   if (m > n) std::swap(m, n);
   cd val(0, 0);
-  const Rwg& r = c.rwg;
-  for (int itm = 0; itm < 2; ++itm) {
-    i64 tm = itm == 0 ? r.tplus[m] : r.tminus[m];
-    double Am = c.areas[tm - 1];
-    double dvm = c.divv[2 * m + itm];
-    const V3* fm = &c.fv[(m * 2 + itm) * c.nq];
-    const V3* fmh = &c.fv_hi[(m * 2 + itm) * 7];
-    for (int itn = 0; itn < 2; ++itn) {
-      i64 tn = itn == 0 ? r.tplus[n] : r.tminus[n];
-      double An = c.areas[tn - 1];
-      double dvn = c.divv[2 * n + itn];
-      const V3* fn = &c.fv[(n * 2 + itn) * c.nq];
-      const V3* fnh = &c.fv_hi[(n * 2 + itn) * 7];
-      if (tm == tn) {
-        val += self_cell(c, tm, fmh, fnh, dvm, dvn, Am);
-      } else if (is_adjacent(c, tm, tn)) {
-        val += adjacent_cell(c, tm, tn, fm, fn, fmh, fnh, dvm, dvn, Am, An);
-      } else {
-        cd dv = dvm * dvn * c.inv_k2;
-        const V3* qm = &c.qp[(tm - 1) * c.nq];
-        const V3* qn = &c.qp[(tn - 1) * c.nq];
-        for (int a = 0; a < c.nq; ++a)
-          for (int b = 0; b < c.nq; ++b) {
-            cd G = greens(qm[a], qn[b], c.k);
-            cd vec = vdot(fm[a], fn[b]) * G;
-            cd scl = dv * G;
-            double wt = c.wq[a] * c.wq[b] * (2 * Am) * (2 * An);
-            val += (vec - scl) * wt;
-          }
-      }
-    }
-  }
+  for (int itm = 0; itm < 2; ++itm)
+    for (int itn = 0; itn < 2; ++itn) val += pair_term(c, m, itm, n, itn);
   return cd(0, -1) * c.omega_mu0 * val;  // _finalize_efie_entry, src/assembly/EFIE.jl:429-437
 }
 
@@ -607,6 +622,18 @@ void or_efie_entries(void* h, const i64* ms, const i64* ns, i64 count, double* o
 #pragma omp parallel for schedule(dynamic, 64)
   for (i64 i = 0; i < count; ++i) {
     cd z = efie_entry(c, ms[i], ns[i]);
+    out[2 * i] = z.real();
+    out[2 * i + 1] = z.imag();
+  }
+}
+
+// Triangle-pair terms (before -i k eta0): outer = side sm[i] of basis m[i],
+// inner = side sn[i] of basis n[i] (0-based bases, side 0 = T+, 1 = T-).
+void or_pair_terms(void* h, const i64* ms, const int* sm, const i64* ns, const int* sn, i64 count,
+                   double* out) {
+  const Cache& c = ((OrCtx*)h)->c;
+  for (i64 i = 0; i < count; ++i) {
+    cd z = pair_term(c, ms[i], sm[i], ns[i], sn[i]);
     out[2 * i] = z.real();
     out[2 * i + 1] = z.imag();
   }

@@ -70,6 +70,8 @@ def lib():
     L.or_efie_create.restype = vp
     L.or_efie_free.argtypes = [vp]
     L.or_efie_entries.argtypes = [vp, i64p, i64p, C.c_int64, f64p]
+    i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    L.or_pair_terms.argtypes = [vp, i64p, i32p, i64p, i32p, C.c_int64, f64p]
     L.or_assemble_z_rows.argtypes = [vp, C.c_int64, C.c_int64, f64p, C.c_int64]
     L.or_mf_apply.argtypes = [vp, f64p, f64p]
     L.or_assemble_rows_sum.argtypes = [vp, C.c_int64, C.c_int64, f64p]
@@ -213,6 +215,16 @@ class EFIE:
         m = np.ascontiguousarray(m, dtype=np.int64); n = np.ascontiguousarray(n, dtype=np.int64)
         out = np.zeros(2 * len(m))
         lib().or_efie_entries(self.h, m, n, len(m), out)
+        return out.view(np.complex128)
+
+    def pair_terms(self, m, sm, n, sn):
+        """Triangle-pair terms of _efie_entry before -i k eta0
+        (src/assembly/EFIE.jl:453-508): side sm of basis m (0 = T+, 1 = T-) is
+        the outer triangle, side sn of basis n the inner one (0-based bases)."""
+        m = np.ascontiguousarray(m, dtype=np.int64); n = np.ascontiguousarray(n, dtype=np.int64)
+        sm = np.ascontiguousarray(sm, dtype=np.int32); sn = np.ascontiguousarray(sn, dtype=np.int32)
+        out = np.zeros(2 * len(m))
+        lib().or_pair_terms(self.h, m, sm, n, sn, len(m), out)
         return out.view(np.complex128)
 
     def assemble(self, row0=0, row1=None):

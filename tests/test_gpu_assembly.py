@@ -53,6 +53,33 @@ def test_caller_basis_order(dm, O, seed):
     assert rel_fro(Z, _oracle_Z(O, m, r, K0)) <= 1e-10
 
 
+@pytest.mark.parametrize("case", [0, 1], ids=["real_k", "lossy_k"])
+def test_entries_vs_mp30_fixture(dm, case):
+    """The HIP FP64 and FP32 matrices of the bent, jittered 2 x 2-cell plate in
+    the fixture's permuted caller order against its 30-digit entries
+    (tests/golden/make_cells_mp.py, written from src/assembly/EFIE.jl:429-684
+    without the C++ oracle), Z[m, n] and Z[n, m]."""
+    import json
+    fx = json.load(open(os.path.join(GOLD, "cells_mp.json")))
+    m = dm.TriMesh(np.array(fx["xyz"], float), np.array(fx["tri"], np.int64))
+    a = fx["rwg"]
+    r = dm.rwg_from_arrays(m, np.array(a["tplus"]), np.array(a["tminus"]), np.array(a["vplus_opp"]),
+                           np.array(a["vminus_opp"]), np.array(a["len"]), np.array(a["area_plus"]),
+                           np.array(a["area_minus"]))
+    c = fx["cases"][case]
+    k = complex(*c["k"])
+    Zd = dm.assemble_Z_efie(m, r, k if k.imag else k.real, forms=dm.DM_Z64 | dm.DM_Z32, device=True)
+    Z64, Z32 = Zd.to_numpy(dm.DM_Z64), Zd.to_numpy(dm.DM_Z32)
+    E = c["entries"]
+    mm = np.array([e["m"] for e in E]); nn = np.array([e["n"] for e in E])
+    want = np.array([complex(float(e["value"][0]), float(e["value"][1])) for e in E])
+    for got64, got32 in ((Z64[mm, nn], Z32[mm, nn]), (Z64[nn, mm], Z32[nn, mm])):
+        e64 = np.abs(got64 - want) / np.abs(want)
+        e32 = np.linalg.norm(got32 - want) / np.linalg.norm(want)
+        print(f"max entry rel fp64 {e64.max():.2e}, fp32 rel {e32:.2e}")
+        assert e64.max() <= 1e-12 and e32 <= 1e-5
+
+
 @pytest.mark.parametrize("quad_order", [1, 4, 7])
 def test_quad_orders(dm, O, quad_order):
     m = dm.make_rect_plate(0.1, 0.1, 5, 5)

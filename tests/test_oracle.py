@@ -188,6 +188,85 @@ def test_singular_entry_vs_polar_quadrature(O, size, lam, tol):
     assert abs(z - exact) / abs(exact) < tol, (z, exact)
 
 
+def _cells_fixture(O):
+    fx = json.load(open(os.path.join(GOLD, "cells_mp.json")))
+    mesh = O.Mesh(np.ascontiguousarray(fx["xyz"]), np.ascontiguousarray(fx["tri"], dtype=np.int64))
+    r = fx["rwg"]
+    rwg = O.RWG(mesh, r["tplus"], r["tminus"], r["evert"], r["vplus_opp"], r["vminus_opp"], r["len"],
+                r["area_plus"], r["area_minus"])
+    return fx, rwg
+
+
+def _cval(pair):
+    return complex(float(pair[0]), float(pair[1]))
+
+
+@pytest.mark.parametrize("case", [0, 1], ids=["real_k", "lossy_k"])
+def test_pair_terms_vs_mp30(O, case):
+    """Self, edge-adjacent and far triangle-pair terms of _efie_entry
+    (src/assembly/EFIE.jl:453-508, 534-684) against an independent 30-digit
+    restatement (tests/golden/make_cells_mp.py: closed-form G / G_smooth,
+    S by polar reduction, not the Wilton closed form) on a bent, jittered
+    2 x 2-cell plate; every adjacent pair in both (outer, inner) orders."""
+    fx, rwg = _cells_fixture(O)
+    c = fx["cases"][case]
+    ef = O.EFIE(rwg, complex(*c["k"]))
+    T = c["terms"]
+    want = np.array([_cval(t["value"]) for t in T])
+    got = ef.pair_terms([t["m"] for t in T], [t["sm"] for t in T], [t["n"] for t in T], [t["sn"] for t in T])
+    kinds = [t["kind"] for t in T]
+    for kind in ("self", "adjacent", "far"):
+        idx = [i for i, k in enumerate(kinds) if k == kind]
+        assert len(idx) >= 10, kind
+        rel = np.abs(got[idx] - want[idx]) / np.abs(want[idx])
+        assert rel.max() <= 1e-12, (kind, rel.max())
+
+
+@pytest.mark.parametrize("case", [0, 1], ids=["real_k", "lossy_k"])
+def test_adjacent_cell_orders_differ(O, case):
+    """_adjacent_cell_cached is not symmetric in (outer, inner): the 7-point
+    outer rule runs on the outer triangle and S over the inner one
+    (src/assembly/EFIE.jl:611-684).  The two orders of every adjacent pair
+    differ well above the parity bar, and the oracle matches each order's own
+    value, not the other's (a role swap fails here)."""
+    fx, rwg = _cells_fixture(O)
+    c = fx["cases"][case]
+    ef = O.EFIE(rwg, complex(*c["k"]))
+    val = {(t["m"], t["sm"], t["n"], t["sn"]): _cval(t["value"]) for t in c["terms"]}
+    adj = [t for t in c["terms"] if t["kind"] == "adjacent"]
+    assert len(adj) >= 40
+    got = ef.pair_terms([t["m"] for t in adj], [t["sm"] for t in adj], [t["n"] for t in adj],
+                        [t["sn"] for t in adj])
+    for t, g in zip(adj, got):
+        mine = val[(t["m"], t["sm"], t["n"], t["sn"])]
+        other = val[(t["n"], t["sn"], t["m"], t["sm"])]
+        assert abs(mine - other) > 1e-6 * abs(mine), t
+        assert abs(g - mine) <= 1e-12 * abs(mine)
+        assert abs(g - other) > 1e-7 * abs(mine)
+
+
+@pytest.mark.parametrize("case", [0, 1], ids=["real_k", "lossy_k"])
+def test_entries_vs_mp30_min_rule(O, case):
+    """Whole entries -i k eta0 sum of 4 pair terms (src/assembly/EFIE.jl:429-512)
+    against the 30-digit fixture, both Z[m, n] and Z[n, m]: the basis
+    min(m, n) owns the outer triangles (:443-445).  The fixture also holds the
+    entries under the swapped rule; wherever they differ, the oracle must match
+    the reference rule and not the swapped one."""
+    fx, rwg = _cells_fixture(O)
+    c = fx["cases"][case]
+    ef = O.EFIE(rwg, complex(*c["k"]))
+    E = c["entries"]
+    m = np.array([e["m"] for e in E]); n = np.array([e["n"] for e in E])
+    want = np.array([_cval(e["value"]) for e in E])
+    swapped = np.array([_cval(e["swapped"]) for e in E])
+    for got in (ef.entries(m, n), ef.entries(n, m)):
+        rel = np.abs(got - want) / np.abs(want)
+        assert rel.max() <= 1e-12, rel.max()
+        sep = np.abs(swapped - want) > 1e-9 * np.abs(want)
+        assert sep.sum() >= 10
+        assert np.all(np.abs(got[sep] - swapped[sep]) > 1e-10 * np.abs(want[sep]))
+
+
 def test_plane_wave_closed_form(O):
     """Normal incidence on z = 0: the integrand is linear, so the order-3 rule is
     exact and v = -E0 pol.(len/2)[(c+ - v+) - (c- - v-)]."""
