@@ -9,18 +9,21 @@ subdivisions (Nt = 20480, N = 30720 RWG), k = 2 pi / 0.1, quad order 3, plane
 wave k_vec = (0, 0, -k), x-polarised, E0 = 1.  Synthetic mesh, no data files.
 
 One step = one complete solve of the chosen form starting from the
-device-resident mesh + RWG handle: assembly of both stored forms in packed
-symmetric storage (complex64 from FP32 far-pair quadrature + FP64 singular
-terms; complex128 from FP64 quadrature, on a second stream, for the
-refinement residual), preconditioner set-up (near-field SPAI, or SPAI plus the
-trained message-passing network), FGMRES on the complex64 matrix with FP64
-Arnoldi, FP64 iterative refinement until ||b - Z64 x|| / ||b|| <= 1e-6.  The
-plain FP64 path (FP64 assembly, full complex128 storage and GEMV, FP64 GMRES
-to the same true residual, with no preconditioner or the SPAI, whichever is
-faster) is timed in the same run.
+device-resident mesh + RWG handle: one FP64 assembly writing the packed
+symmetric complex128 matrix and its complex64 rounding, cross approximation
+of the complex64 tiles (5e-8) on a second stream beside the near-field SPAI
+build, FGMRES on the compressed complex64 matrix with FP64 Arnoldi, FP64
+iterative refinement until ||b - Z64 x|| / ||b|| <= 1e-6 against the dense
+complex128 matrix.  The plain FP64 path (FP64 assembly, packed complex128
+storage and product, FP64 GMRES to the same true residual, with the SPAI or
+no preconditioner, whichever is faster) is timed in the same run; full
+N x N complex128 storage is timed as a variant.
 
 Also measured (not the headline value): the many-angle case, 256 seeded plane
-waves on the same sphere solved in lockstep batches (Z X on MFMA).
+waves on the same sphere solved in lockstep batches (Z X on the matrix
+cores), chosen and plain FP64; the error of the chosen currents and RCS
+against a 1e-10 FP64 solve; the CPU baseline by phase, with the oracle's
+check of the GPU matrix and currents on sampled rows.
 
 Prints one JSON line (time-like metric: value = seconds, lower is better).
 """
@@ -64,45 +67,83 @@ def far_pairs(mesh, rwg):
     return nt * (nt - 1) // 2 - rwg.adjacent_pair_count()
 
 
-def cpu_baseline(mesh, rwg, k, plain_iters, budget_s=12.0):
-    """The oracle (FP64 restatement of the reference's entry-wise assembly, OpenMP
-    over rows) timed on a bounded row sample of the same matrix, plus a numpy
-    complex128 GEMV on a row block; both extrapolated to the full time-to-solve
-    (assembly + plain-path iteration count x dense product)."""
+def _best_of(fn, reps=3):
+    """Minimum wall time of reps calls (and the last call's result)."""
+    best, out = float("inf"), None
+    for _ in range(reps):
+        t = time.perf_counter()
+        out = fn()
+        best = min(best, time.perf_counter() - t)
+    return best, out
+
+
+def cpu_baseline(mesh, rwg, k, cutoff, plain_iters, check, budget_s=15.0):
+    """The reference's algorithm on the host cores, phase by phase, each the
+    minimum of 3 runs: the oracle's entry-wise FP64 assembly (OpenMP over rows)
+    on a bounded row sample, extrapolated to N(N+1)/2 entries; the plane-wave
+    excitation in full; the near-field LU preconditioner in full (entries on
+    demand, sparse LU); per GMRES iteration one N x N complex128 product
+    (numpy on a row block, extrapolated), one Arnoldi step at the mean basis
+    size and one near-field LU solve, times the plain GPU path's iteration
+    count.  Then, untimed, the oracle's check of the GPU results: entries of
+    the GPU complex128 matrix on sampled full rows and the chosen currents'
+    residual on those rows."""
     sys.path.insert(0, ROOT)
     from oracle import oracle as O
     N = rwg.nedges
     orr = O.RWG(O.Mesh(mesh.xyz, mesh.tri), rwg.tplus, rwg.tminus, rwg.evert, rwg.vplus_opp, rwg.vminus_opp,
                 rwg.len, rwg.area_plus, rwg.area_minus)
     ef = O.EFIE(orr, k)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    # assembly: a row sample sized to ~budget/6 s per run
     t = time.perf_counter()
-    cnt, _ = ef.rows_sum(0, 8)
+    ef.rows_sum(0, 8)
     t_cal = time.perf_counter() - t
-    rows = max(8, min(N, int(8 * 0.7 * budget_s / max(t_cal, 1e-3))))
-    t = time.perf_counter()
-    cnt, _ = ef.rows_sum(0, rows)
-    t_rows = time.perf_counter() - t
-    per_entry = t_rows / cnt
-    t_asm = per_entry * N * (N + 1) / 2
+    rows = max(8, min(N, int(8 * budget_s / 6 / max(t_cal, 1e-3))))
+    t_rows, (cnt, _) = _best_of(lambda: ef.rows_sum(0, rows))
+    t_asm = t_rows / cnt * N * (N + 1) / 2
+    # excitation
+    kv, pol = [0.0, 0.0, -k], [1.0, 0.0, 0.0]
+    t_exc, vo = _best_of(lambda: O.assemble_v_plane_wave(orr, kv, 1.0, pol))
+    # near-field LU over the same cutoff as the GPU preconditioner
+    t_nf, P = _best_of(lambda: O.NearFieldLU(ef, orr, cutoff), reps=1)  # ~10-20 s: once
+    # one GMRES iteration: dense product + Arnoldi step + preconditioner solve
     B = min(N, 2048)
     rng = np.random.default_rng(0)
     Zb = (rng.standard_normal((B, N)) + 1j * rng.standard_normal((B, N))).astype(np.complex128)
     x = rng.standard_normal(N) + 1j * rng.standard_normal(N)
-    Zb @ x
-    t = time.perf_counter()
-    reps = 3
-    for _ in range(reps):
-        Zb @ x
-    t_blk = (time.perf_counter() - t) / reps
+    t_blk, _ = _best_of(lambda: Zb @ x)
     t_mv = t_blk * N / B
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    value = t_asm + plain_iters * t_mv
-    return dict(value=round(value, 3), unit="s", cores=cores, kind="port",
-                sample=(f"oracle entry-wise FP64 assembly of rows [0,{rows}) x (n>=m) = {cnt} entries in "
-                        f"{t_rows:.2f} s on {cores} threads -> {t_asm:.1f} s for N(N+1)/2; numpy complex128 "
-                        f"GEMV {B}x{N} in {t_blk * 1e3:.1f} ms -> {t_mv * 1e3:.1f} ms per N x N product x "
-                        f"{plain_iters} plain-path iterations"),
-                assembly_s=round(t_asm, 3), matvec_ms=round(t_mv * 1e3, 2))
+    del Zb
+    j = max(1, plain_iters // 2)
+    Vb = rng.standard_normal((N, j)) + 1j * rng.standard_normal((N, j))
+    t_arn, _ = _best_of(lambda: x - Vb @ (Vb.conj().T @ x))
+    del Vb
+    t_lu, _ = _best_of(lambda: P(x))
+    t_solve = plain_iters * (t_mv + t_arn + t_lu)
+    value = t_asm + t_exc + t_nf + t_solve
+    out = dict(value=round(value, 3), unit="s", cores=cores, kind="port",
+               sample=(f"oracle entry-wise FP64 assembly of rows [0,{rows}) x (n>=m) = {cnt} entries in "
+                       f"{t_rows:.2f} s on {cores} threads -> {t_asm:.1f} s for N(N+1)/2; excitation in full; "
+                       f"near-field LU (cutoff {cutoff:.4g} m, nnz ratio {P.nnz_ratio:.2e}) in full; per iteration "
+                       f"numpy complex128 GEMV {B}x{N} extrapolated to N x N ({t_mv * 1e3:.1f} ms) + Arnoldi step "
+                       f"at j={j} ({t_arn * 1e3:.1f} ms) + LU solve ({t_lu * 1e3:.1f} ms), x {plain_iters} "
+                       "iterations of the plain GPU path; each phase the minimum of 3 runs except the "
+                       "near-field build (one run)"),
+               phases_s=dict(assembly=round(t_asm, 3), excitation=round(t_exc, 4), nearfield_build=round(t_nf, 3),
+                             solve=round(t_solve, 3)),
+               matvec_ms=round(t_mv * 1e3, 2))
+    if check is None:
+        return out
+    # the oracle's check of the GPU results (not timed)
+    rows_c = check["rows"]
+    Zr = ef.entries(np.repeat(rows_c, N), np.tile(np.arange(N), len(rows_c))).reshape(len(rows_c), N)
+    e_z = float(np.linalg.norm(check["z64_rows"] - Zr) / np.linalg.norm(Zr))
+    e_v = float(np.linalg.norm(check["v"] - vo) / np.linalg.norm(vo))
+    res = float(np.linalg.norm(Zr @ check["x"] - vo[rows_c]) / np.linalg.norm(vo[rows_c]))
+    out["oracle_check"] = dict(rows=len(rows_c), z64_rows_rel_err=e_z, excitation_rel_err=e_v,
+                               chosen_residual_on_rows=res)
+    return out
 
 
 def random_planewaves(dm, k, count, seed=20261202):
@@ -236,11 +277,11 @@ def main():
     chosen_P = learned if a.precond == "learned" else nf
     pw = [dm.make_plane_wave(k * khat, 1.0, pol)]
 
-    def solve(method, P, waves=pw, maxiter=1000, ctol=0.0):
+    def solve(method, P, waves=pw, maxiter=1000, ctol=0.0, storage="packed", tol=1e-6):
         ctx.sync()
         t0 = time.perf_counter()
-        X, st, t_asm = dm.solve_planewaves(mesh, rwg, k, waves, method, P, tol=1e-6, maxiter=maxiter,
-                                           compress_tol=ctol, batch=-1 if ctol > 0 else 0)
+        X, st, t_asm = dm.solve_planewaves(mesh, rwg, k, waves, method, P, tol=tol, maxiter=maxiter,
+                                           compress_tol=ctol, batch=-1 if ctol > 0 else 0, storage=storage)
         ctx.sync()
         return time.perf_counter() - t0, X, st, t_asm
 
@@ -266,12 +307,16 @@ def main():
             kern[name] = dict(launches=c, total_ms=round(ms, 3), avg_ms=round(ms / c, 4))
     ctx.profile_reset()
     variants = {}
-    comp = None
+    comp = check = None
     if ctol > 0:
         # compression statistics of the chosen form (same deterministic ranks)
         _, _, _, Zk = dm.solve_planewaves(mesh, rwg, k, pw, "mixed", chosen_P, tol=1e-6, maxiter=1000,
                                           compress_tol=ctol, batch=-1, keep_matrix=True)
         comp = Zk.compress(eps=ctol)
+        check_rows = np.sort(np.random.default_rng(64).choice(N, 64, replace=False))
+        check = dict(rows=check_rows, x=x_c, v=dm.assemble_v_plane_wave(mesh, rwg, k * khat, 1.0, pol),
+                     z64_rows=Zk.entries(np.repeat(check_rows, N), np.tile(np.arange(N), len(check_rows)),
+                                         dm.DM_Z64P).reshape(len(check_rows), N))
         del Zk
         solve("mixed", chosen_P)
         dt, X, st_d, _ = solve("mixed", chosen_P)
@@ -287,24 +332,43 @@ def main():
         log(f"mixed ({other[0]}): {dt:.3f} s, {st_o.niter} it, residual {st_o.residual:.2e}")
     plain = {}
     ctx.profile(True)
-    for label, P in (("none", None), ("nearfield", nf)):
-        solve("fp64", P)  # warm
+    for label, P, storage in (("nearfield", nf, "packed"), ("none", None, "packed")):
+        solve("fp64", P, storage=storage)  # warm
         pt = []
         for i in range(max(1, min(a.steps, 2))):
-            dt, X, st_p, _ = solve("fp64", P)
+            dt, X, st_p, _ = solve("fp64", P, storage=storage)
             pt.append(dt)
-            log(f"plain fp64 ({label}) {i}: {dt:.3f} s, {st_p.niter} it, residual {st_p.residual:.2e}")
-        plain[label] = dict(s=float(np.median(pt)), iterations=st_p.niter, true_residual=st_p.residual, x=X[:, 0])
+            log(f"plain fp64 ({label}, {storage}) {i}: {dt:.3f} s, {st_p.niter} it, residual {st_p.residual:.2e}")
+        plain[label] = dict(s=float(np.median(pt)), iterations=st_p.niter, true_residual=st_p.residual, x=X[:, 0],
+                            storage=storage)
     ctx.profile(False)
-    for name in ("gemv_z64", "assemble_z64"):
+    for name in ("symv_z64", "gemv_z64", "assemble_z64", "arnoldi", "precond"):
         c, ms = ctx.profile_read(name)
         if c:
             kern["plain_" + name] = dict(launches=c, total_ms=round(ms, 3), avg_ms=round(ms / c, 4))
     ctx.profile_reset()
+    # the reference's storage: full N x N complex128 (variant, not the baseline)
+    solve("fp64", nf, storage="full")
+    dt, X, st_f, _ = solve("fp64", nf, storage="full")
+    variants["plain_fp64_full_storage_nearfield"] = dict(s=round(dt, 4), iterations=st_f.niter,
+                                                         true_residual=st_f.residual)
+    log(f"plain fp64 (nearfield, full storage): {dt:.3f} s, {st_f.niter} it")
     best = min(plain, key=lambda l: plain[l]["s"])
     t_c, t_p = float(np.median(times)), plain[best]["s"]
     x_p = plain[best]["x"]
     dx = float(np.linalg.norm(x_c - x_p) / np.linalg.norm(x_p))
+    # error of the chosen currents and RCS against a tight FP64 solve
+    _, Xt, st_t, _ = solve("fp64", nf, tol=1e-10)
+    x_t = Xt[:, 0]
+    grid = dm.make_sph_grid(36, 72)
+    Et = dm.farfield(mesh, rwg, grid, k, np.stack([x_t, x_c], axis=1))
+    rcs_t, rcs_c = dm.bistatic_rcs(Et[:, :, 0]), dm.bistatic_rcs(Et[:, :, 1])
+    vs_fp64 = dict(reference=f"FP64 GMRES to 1e-10 on the packed complex128 matrix ({st_t.niter} it, residual "
+                             f"{st_t.residual:.2e})",
+                   currents_rel_err=float(np.linalg.norm(x_c - x_t) / np.linalg.norm(x_t)),
+                   rcs_rel_err=float(np.linalg.norm(rcs_c - rcs_t) / np.linalg.norm(rcs_t)),
+                   rcs_grid=f"{len(grid.w)} directions")
+    log(f"chosen vs FP64 1e-10: currents {vs_fp64['currents_rel_err']:.2e}, RCS {vs_fp64['rcs_rel_err']:.2e}")
 
     batched = None
     if not a.no_batch and a.batch_rhs > 0:
@@ -325,7 +389,6 @@ def main():
                        single_rhs_chosen_s=round(t_c, 4),
                        speedup_per_rhs_vs_single_chosen=round(t_c * a.batch_rhs / dt, 2),
                        speedup_per_rhs_vs_single_plain=round(t_p * a.batch_rhs / dt, 2),
-                       note="plain path not re-run per wave; per-RHS comparisons use the single-RHS times",
                        kernels=bkern)
         if "zx_z32" in bkern:
             # lockstep complex64 Z X over all columns: 8 N^2 R useful FLOP per product,
@@ -339,6 +402,12 @@ def main():
                                           peak_bf16=2500.0, frac=round(6 * useful / zx_ms / 1e9 / 2500.0, 4))
         log(f"batched {a.batch_rhs} plane waves: {dt:.2f} s ({dt / a.batch_rhs * 1e3:.1f} ms per RHS), "
             f"{st_b.niter} it total, worst residual {st_b.residual:.2e}")
+        # the plain FP64 path on the same batch: lockstep GMRES, complex128 Z X on the FP64 matrix cores
+        dtp, _, st_bp, _ = solve("fp64", nf, waves, maxiter=300)
+        batched["plain_fp64"] = dict(s=round(dtp, 3), s_per_rhs=round(dtp / a.batch_rhs, 5),
+                                     iterations_total=st_bp.niter, worst_true_residual=st_bp.residual)
+        batched["speedup_per_rhs_vs_batched_plain"] = round(dtp / dt, 2)
+        log(f"batched plain fp64: {dtp:.2f} s, {st_bp.niter} it total, worst residual {st_bp.residual:.2e}")
         # far fields of all currents on a 5-degree grid (E = G I on the matrix cores)
         grid = dm.make_sph_grid(36, 72)
         nom = grid.rhat.shape[0]
@@ -364,6 +433,7 @@ def main():
            "gemv_z32": ("hbm", 8.0 * N * N + 32.0 * N, HBM_PEAK_GBS, "GB/s"),
            "gemv_z64": ("hbm", 16.0 * N * N + 32.0 * N, HBM_PEAK_GBS, "GB/s"),
            "plain_gemv_z64": ("hbm", 16.0 * N * N + 32.0 * N, HBM_PEAK_GBS, "GB/s"),
+           "plain_symv_z64": ("hbm", 2 * packed + 32.0 * N, HBM_PEAK_GBS, "GB/s"),
            "assemble_z32": ("valu", 52.0 * nq * nq * pf, FP32_PEAK_TFLOPS, "TFLOP/s"),
            "assemble_z64_z32": ("valu", 52.0 * nq * nq * pf, FP64_PEAK_TFLOPS, "TFLOP/s"),
            "assemble_z64": ("valu", 52.0 * nq * nq * pf, FP64_PEAK_TFLOPS, "TFLOP/s"),
@@ -382,7 +452,7 @@ def main():
     roof = fracs[dom]
     roofline = dict(kernel=dom, bound=roof["bound"], achieved=roof["achieved"], peak=roof["peak"], unit=roof["unit"],
                     frac=roof["frac"], traffic=measured_traffic(dom, a.subdiv, ctx.tune_get()))
-    cpu = None if a.no_cpu else cpu_baseline(mesh, rwg, k, plain["none"]["iterations"])
+    cpu = None if a.no_cpu else cpu_baseline(mesh, rwg, k, a.nf_cutoff * h, plain["nearfield"]["iterations"], check)
     out = {
         "metric": METRIC, "value": round(t_c, 4), "unit": "s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(t_c * 1e3, 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
@@ -400,10 +470,11 @@ def main():
                    "iterations": st_c.niter, "refinements": st_c.outer, "true_residual": st_c.residual,
                    "matrix_ready_s": tasm_c, "precond_s": st_c.precond_time_s},
         "variants": variants,
-        "plain": {l: {"s": round(v["s"], 4), "iterations": v["iterations"], "true_residual": v["true_residual"]}
-                  for l, v in plain.items()},
+        "plain": {l: {"s": round(v["s"], 4), "iterations": v["iterations"], "true_residual": v["true_residual"],
+                      "storage": v["storage"]} for l, v in plain.items()},
         "plain_best": best,
         "currents_rel_diff_chosen_vs_plain": dx,
+        "chosen_vs_fp64": vs_fp64,
         "batched": batched,
         "kernels": kern, "kernel_roofline": fracs, "tuned": ctx.tune_get(),
         "roofline": roofline, "cpu_baseline": cpu,

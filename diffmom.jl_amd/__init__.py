@@ -53,6 +53,37 @@ DM_PRECOND_NONE, DM_PRECOND_DIAG, DM_PRECOND_NEARFIELD, DM_PRECOND_LEARNED, DM_P
 ETA0 = 376.730313668
 C0_DEFAULT = 299792458.0
 _DEFAULT_MAX_DENSE_PAYLOAD_BYTES = 2_000_000_000
+_DEFAULT_MAX_EFIE_CACHE_BYTES = 2_000_000_000  # src/assembly/EFIE.jl:14
+_DEFAULT_MAX_EFIE_ADJACENCY_PAIRS = 20_000_000  # src/assembly/EFIE.jl:15
+_DEFAULT_MAX_EXCITATION_WORK_BYTES = 512 * 1024 * 1024  # src/assembly/Excitation.jl:19
+_DEFAULT_MAX_EXCITATION_TERMS = 200_000_000  # src/assembly/Excitation.jl:20
+_DEFAULT_MAX_GMRES_WORKSPACE_BYTES = 512 * 1024 * 1024  # src/solver/IterativeSolve.jl:10
+
+
+def _resource_limit(name, value, allow_zero=False):
+    """_validated_resource_limit / _validated_nonnegative_resource_limit
+    (src/Types.jl:41-65)."""
+    v = int(value)
+    if v != value or not (v >= 0 if allow_zero else v > 0):
+        raise ValueError(f"{name} must be {'nonnegative' if allow_zero else 'positive'}, got {value}")
+    return v
+
+
+def _efie_limits(mesh, rwg, max_cache_bytes, max_adjacency_pairs):
+    """The EFIE cache gates of _build_triangle_adjacency
+    (src/assembly/EFIE.jl:140-231): the edge-adjacent triangle pairs against
+    max_adjacency_pairs, and the device tables this library builds (per
+    triangle: quadrature points of both rules, area, neighbours; per basis:
+    triangles, signs, lengths; the near-field tables) against max_cache_bytes."""
+    cache = _resource_limit("max_cache_bytes", max_cache_bytes)
+    pairs_limit = _resource_limit("max_adjacency_pairs", max_adjacency_pairs, allow_zero=True)
+    pairs = rwg.adjacent_pair_count()
+    if pairs > pairs_limit:
+        raise ValueError(f"triangle adjacency exceeds max_adjacency_pairs={pairs_limit}")
+    nt, n = mesh.nt, rwg.nedges
+    est = nt * (8 * 3 * (1 + 3 + 3 + 7) + 8 + 8 * 3 + 16 * 9 * 4) + n * (8 * 2 + 8 * 2 + 8 * 3)
+    if est > cache:
+        raise ValueError(f"EFIE cache requires at most {est} estimated bytes, exceeding max_cache_bytes={cache}")
 
 
 class ConvergenceError(RuntimeError):
@@ -84,7 +115,7 @@ class dm_solve_opts(C.Structure):
                 ("tol", C.c_double), ("nf_cutoff", C.c_double), ("restart", C.c_int),
                 ("check_convergence", C.c_int), ("inner_tol", C.c_double), ("max_outer", C.c_int),
                 ("block_size", C.c_int), ("batch", C.c_int), ("learned", C.c_void_p), ("adjoint", C.c_int),
-                ("compress_tol", C.c_double)]
+                ("compress_tol", C.c_double), ("storage", C.c_int)]
 
 
 class dm_solve_stats(C.Structure):
@@ -93,6 +124,7 @@ class dm_solve_stats(C.Structure):
 
 
 _L = None
+SIGNATURES = None  # name -> (argtypes, restype) of every entry point the package binds
 
 
 def lib():
@@ -132,6 +164,7 @@ def lib():
                                     C.POINTER(dm_solve_stats), C.POINTER(d), pp], i32),
         "dm_precond_create": ([vp, vp, i32, d, i32, i32, pp], i32), "dm_precond_apply": ([vp, vp, vp, vp], i32),
         "dm_precond_free": ([vp], i32),
+        "dm_nearfield_pattern_info": ([vp, d, C.POINTER(i64), C.POINTER(d), C.POINTER(i32), C.POINTER(i64)], i32),
         "dm_profile_enable": ([vp, i32], i32), "dm_profile_reset": ([vp], i32),
         "dm_profile_read": ([vp, C.c_char_p, C.POINTER(i64), C.POINTER(d)], i32),
         "dm_tune_get": ([vp, C.c_char_p, i64, C.POINTER(i64)], i32), "dm_tune_set": ([vp, C.c_char_p], i32),
@@ -144,6 +177,8 @@ def lib():
         "dm_mpnn_train": ([vp, vp, vp, i32, i64, d, C.c_uint64, vp], i32),
         "dm_mpnn_apply": ([vp, vp, vp, d, vp, vp, vp], i32),
     }
+    global SIGNATURES
+    SIGNATURES = sig
     for name, (args, res) in sig.items():
         f = getattr(L, name)
         f.argtypes = args
@@ -480,7 +515,8 @@ def _k_parts(k):
 
 def assemble_Z_efie(mesh, rwg, k, quad_order=3, eta0=ETA0, mesh_precheck=True, allow_boundary=True,
                     require_closed=False, area_tol_rel=1e-12, max_output_bytes=_DEFAULT_MAX_DENSE_PAYLOAD_BYTES,
-                    forms=DM_Z64, device=False):
+                    max_cache_bytes=_DEFAULT_MAX_EFIE_CACHE_BYTES,
+                    max_adjacency_pairs=_DEFAULT_MAX_EFIE_ADJACENCY_PAIRS, forms=DM_Z64, device=False):
     """Dense EFIE matrix (src/assembly/EFIE.jl:699-753).  Returns a complex128
     numpy matrix like the reference, or the device-resident ZMatrix when
     device=True (forms selects complex128 and/or complex64 storage; with
@@ -492,6 +528,7 @@ def assemble_Z_efie(mesh, rwg, k, quad_order=3, eta0=ETA0, mesh_precheck=True, a
         raise ValueError(f"dense EFIE matrix requires {N * N * 16} bytes, exceeding max_output_bytes={max_output_bytes}")
     if mesh_precheck:
         assert_mesh_quality(mesh, allow_boundary, require_closed, area_tol_rel)
+    _efie_limits(mesh, rwg, max_cache_bytes, max_adjacency_pairs)
     kr, ki = _k_parts(k)
     h = C.c_void_p()
     _check(lib().dm_assemble_z(mesh.ctx.h, rwg.h, kr, ki, float(eta0), int(quad_order), int(forms),
@@ -552,10 +589,13 @@ def assemble_Z_efie_multi(meshes, rwgs, k, quad_order=3, eta0=ETA0, forms=DM_Z64
 
 
 def matrixfree_efie_operator(mesh, rwg, k, quad_order=3, eta0=ETA0, mesh_precheck=True, allow_boundary=True,
-                             require_closed=False, area_tol_rel=1e-12):
+                             require_closed=False, area_tol_rel=1e-12, max_cache_bytes=_DEFAULT_MAX_EFIE_CACHE_BYTES,
+                             max_adjacency_pairs=_DEFAULT_MAX_EFIE_ADJACENCY_PAIRS):
+    """src/assembly/EFIE.jl:975-1000."""
     _check_pair(mesh, rwg)
     if mesh_precheck:
         assert_mesh_quality(mesh, allow_boundary, require_closed, area_tol_rel)
+    _efie_limits(mesh, rwg, max_cache_bytes, max_adjacency_pairs)
     kr, ki = _k_parts(k)
     h = C.c_void_p()
     _check(lib().dm_mf_create(mesh.ctx.h, rwg.h, kr, ki, float(eta0), int(quad_order), C.byref(h)))
@@ -575,9 +615,21 @@ def make_plane_wave(k_vec, E0, pol):
     return PlaneWaveExcitation(np.asarray(k_vec, float), float(E0), np.asarray(pol, float))
 
 
-def assemble_multiple_excitations(mesh, rwg, excitations, quad_order=3):
+def assemble_multiple_excitations(mesh, rwg, excitations, quad_order=3,
+                                  max_output_bytes=_DEFAULT_MAX_DENSE_PAYLOAD_BYTES,
+                                  max_work_bytes=_DEFAULT_MAX_EXCITATION_WORK_BYTES,
+                                  max_terms=_DEFAULT_MAX_EXCITATION_TERMS):
+    """src/assembly/Excitation.jl:2738-2797 for plane waves: the output byte
+    gate as the reference's; the work and term limits are range-checked (the
+    device path keeps no per-term workspace)."""
     _check_pair(mesh, rwg)
     M = len(excitations)
+    out = _resource_limit("max_output_bytes", max_output_bytes)
+    if 16 * rwg.nedges * M > out:
+        raise ValueError(f"multiple-excitation RHS matrix requires {16 * rwg.nedges * M} raw bytes, exceeding "
+                         f"max_output_bytes={out}")
+    _resource_limit("max_work_bytes", max_work_bytes)
+    _resource_limit("max_terms", max_terms)
     kv = np.ascontiguousarray(np.array([e.k_vec for e in excitations], float).reshape(M, 3))
     pol = np.ascontiguousarray(np.array([e.pol for e in excitations], float).reshape(M, 3))
     E0 = np.ascontiguousarray([float(e.E0) for e in excitations])
@@ -602,23 +654,41 @@ def assemble_excitation(mesh, rwg, excitation, quad_order=3):
 class NearFieldPreconditionerData:
     """Near-field preconditioner spec (src/solver/NearFieldPreconditioner.jl:84-110).
 
-    factorization="lu": the reference factors the near-field matrix (entries with
-    RWG-centre distance <= cutoff) with a sparse LU.  On the GPU the same
-    near-field coupling is inverted approximately: a sparse approximate inverse
-    M minimising ||Z M - I||_F over the cutoff pattern (at most 80 nearest bases
-    per column), applied as one SpMV per iteration.
+    factorization="spai": a sparse approximate inverse M of the near-field
+    matrix (entries with RWG-centre distance <= cutoff), minimising
+    ||Z M - I||_F column by column over the cutoff pattern (the nearest 80
+    bases within the cutoff per column), applied as one SpMV per iteration.
+    The reference factors the same near-field matrix with a sparse LU
+    (factorization=:lu, :700-712) or ILU (:732-759); neither factorisation is
+    offered on the GPU path, and asking for one raises ValueError.
     factorization="diag": Jacobi (src/solver/NearFieldPreconditioner.jl:623-668).
     factorization="block": block Jacobi over block_size spatially consecutive bases."""
     cutoff: float
-    factorization: str = "lu"
+    factorization: str = "spai"
     block_size: int = 128
 
+    def pattern_info(self, rwg):
+        """(nnz, mean radius covered per column in metres, column cap, columns
+        the cap cut short of the cutoff) of the SPAI pattern this cutoff gives
+        on rwg."""
+        nnz, rad, cap, capped = C.c_int64(), C.c_double(), C.c_int(), C.c_int64()
+        _check(lib().dm_nearfield_pattern_info(rwg.h, float(self.cutoff), C.byref(nnz), C.byref(rad), C.byref(cap),
+                                               C.byref(capped)))
+        return nnz.value, rad.value, cap.value, capped.value
 
-def build_nearfield_preconditioner(Z, mesh, rwg, cutoff, factorization="lu", block_size=128):
-    if factorization not in ("lu", "diag", "block"):
-        raise ValueError(f"unsupported factorization {factorization!r} on the GPU path (use 'lu', 'diag' or 'block')")
-    if not (cutoff >= 0):
-        raise ValueError("cutoff must be nonnegative")
+
+_NO_FACTOR = ("factorization={!r}: the GPU path does not factor the near-field matrix; it builds a sparse "
+              "approximate inverse of it (factorization='spai'), or Jacobi ('diag') / block Jacobi ('block')")
+
+
+def build_nearfield_preconditioner(Z, mesh, rwg, cutoff, factorization="spai", block_size=128):
+    if factorization in ("lu", "ilu"):
+        raise ValueError(_NO_FACTOR.format(factorization))
+    if factorization not in ("spai", "diag", "block"):
+        raise ValueError(f"unsupported factorization {factorization!r} on the GPU path (use 'spai', 'diag' or "
+                         "'block')")
+    if not (cutoff >= 0 and math.isfinite(cutoff)):
+        raise ValueError("cutoff must be finite and nonnegative")
     return NearFieldPreconditionerData(float(cutoff), factorization, int(block_size))
 
 
@@ -680,14 +750,19 @@ def _solve(Z, B, method, precond=None, side="right", tol=1e-6, maxiter=300, rest
 
 
 def _opts(method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, check=True, inner_tol=0.0,
-          max_outer=10, batch=0, compress_tol=0.0):
+          max_outer=10, batch=0, compress_tol=0.0, storage="packed"):
     """dm_solve_opts.  batch: 0 = lockstep batches of right-hand sides when there
     are >= 8 of them, 1 = always, -1 = one right-hand side at a time.
-    compress_tol > 0: the mixed solve compresses its complex64 matrix."""
+    compress_tol > 0: the mixed solve compresses its complex64 matrix.
+    storage: "packed" (symmetric tiles, half the bytes) or "full" for the
+    one-column FP64 GMRES path of solve_planewaves."""
+    if storage not in ("packed", "full"):
+        raise ValueError(f"storage must be 'packed' or 'full', got {storage!r}")
     o = dm_solve_opts()
     lib().dm_solve_opts_default(C.byref(o))
     o.method, o.tol, o.maxiter, o.restart = method, float(tol), int(maxiter), int(restart)
     o.batch = int(batch)
+    o.storage = 0 if storage == "packed" else 1
     o.compress_tol = float(compress_tol)
     o.check_convergence, o.inner_tol, o.max_outer = int(check), float(inner_tol), int(max_outer)
     o.precond_side = 1 if side == "right" else 0
@@ -712,7 +787,7 @@ def _opts(method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, 
 
 def solve_planewaves(mesh, rwg, k, excitations, method="mixed", preconditioner=None, tol=1e-6, maxiter=300,
                      quad_order=3, eta0=ETA0, check_convergence=True, keep_matrix=False, batch=0, restart=0,
-                     compress_tol=0.0):
+                     compress_tol=0.0, storage="packed"):
     """End-to-end device solve for plane waves (the dense branches of
     solve_scattering, src/Workflow.jl:67-317): assembly, excitation,
     preconditioner and Krylov solve in one call.  method: "direct" (dense LU),
@@ -721,6 +796,9 @@ def solve_planewaves(mesh, rwg, k, excitations, method="mixed", preconditioner=N
     complex128 matrix; one FP64 assembly writes both).  compress_tol > 0 (mixed,
     one wave at a time): the complex64 matrix is compressed first
     (ZMatrix.compress) and the inner products use the compressed form.
+    storage ("fp64", one wave at a time): "packed" keeps the upper triangle of
+    128 x 128 tiles (Z = Z^T exactly; each product reads half the bytes),
+    "full" the N x N matrix as the reference stores it.
     Returns (X, stats, assembly_time_s[, ZMatrix])."""
     _check_pair(mesh, rwg)
     M = len(excitations)
@@ -729,7 +807,7 @@ def solve_planewaves(mesh, rwg, k, excitations, method="mixed", preconditioner=N
     E0 = np.ascontiguousarray([float(e.E0) for e in excitations])
     meth = {"direct": DM_METHOD_LU, "fp64": DM_METHOD_GMRES, "mixed": DM_METHOD_MIXED}[method]
     o = _opts(meth, preconditioner, "right", tol, maxiter, restart=restart, check=check_convergence, batch=batch,
-              compress_tol=compress_tol)
+              compress_tol=compress_tol, storage=storage)
     X = np.zeros((rwg.nedges, M), dtype=np.complex128, order="F")
     st = dm_solve_stats()
     t_asm = C.c_double()
@@ -762,7 +840,8 @@ def solve_forward(Z, v, solver="direct", **kw):
 
 
 def solve_gmres(Z, rhs, preconditioner=None, precond_side="left", tol=1e-8, maxiter=200, memory=20,
-                verbose=False, check_gmres_convergence=True, restart=0, batch=0):
+                max_workspace_bytes=_DEFAULT_MAX_GMRES_WORKSPACE_BYTES, verbose=False,
+                check_gmres_convergence=True, restart=0, batch=0):
     """src/solver/IterativeSolve.jl:341-392 (full GMRES unless restart > 0).
     Several right-hand sides with right or no preconditioning run as lockstep
     batches (one Z X product per step) unless batch=-1."""
@@ -774,12 +853,25 @@ def solve_gmres(Z, rhs, preconditioner=None, precond_side="left", tol=1e-8, maxi
         raise ValueError(f"GMRES memory must be at least 1, got {memory}")
     if precond_side not in ("left", "right"):
         raise ValueError(f"Invalid precond_side: {precond_side}")
+    _gmres_workspace_gate(Z.N, memory, max_workspace_bytes)
     return _solve(Z, rhs, DM_METHOD_GMRES, preconditioner, precond_side, tol, maxiter, restart,
                   check_gmres_convergence, batch=batch)
 
 
+def _gmres_workspace_gate(N, memory, max_workspace_bytes):
+    """_preflight_gmres_workspace / _gmres_workspace_bytes
+    (src/solver/IterativeSolve.jl:12-36): Krylov.jl's GMRES storage for
+    min(N, memory) basis vectors against max_workspace_bytes."""
+    lim = _resource_limit("max_workspace_bytes", max_workspace_bytes)
+    m = min(N, memory)
+    need = 16 * ((m + 5) * N + 2 * m + m * (m + 1) // 2) + 8 * m
+    if need > lim:
+        raise ValueError(f"GMRES Krylov workspace requires {need} raw bytes, exceeding max_workspace_bytes={lim}")
+
+
 def solve_gmres_adjoint(Z, rhs, preconditioner=None, precond_side="left", tol=1e-8, maxiter=200, memory=20,
-                        verbose=False, check_gmres_convergence=True, restart=0, batch=0):
+                        max_workspace_bytes=_DEFAULT_MAX_GMRES_WORKSPACE_BYTES, verbose=False,
+                        check_gmres_convergence=True, restart=0, batch=0):
     """Z^H x = rhs (src/solver/IterativeSolve.jl:409-460).  Z is exactly
     symmetric, so the library solves Z conj(x) = conj(rhs) with the forward
     operator and preconditioner and conjugates the result."""
@@ -789,6 +881,7 @@ def solve_gmres_adjoint(Z, rhs, preconditioner=None, precond_side="left", tol=1e
         raise ValueError(f"GMRES maxiter must be at least 1, got {maxiter}")
     if precond_side not in ("left", "right"):
         raise ValueError(f"Invalid precond_side: {precond_side}")
+    _gmres_workspace_gate(Z.N, memory, max_workspace_bytes)
     return _solve(Z, rhs, DM_METHOD_GMRES, preconditioner, precond_side, tol, maxiter, restart,
                   check_gmres_convergence, batch=batch, adjoint=True)
 
@@ -895,7 +988,8 @@ def mesh_resolution_report(mesh, freq_hz, points_per_wavelength=10.0, c0=C0_DEFA
 def solve_scattering(mesh, freq_hz, excitation, method="auto", dense_direct_limit=2000, dense_gmres_limit=10000,
                      mlfma_threshold=50000, check_resolution=True, points_per_wavelength=10.0,
                      error_on_underresolved=False, gmres_tol=1e-6, gmres_maxiter=300, check_gmres_convergence=True,
-                     nf_cutoff_lambda=1.0, preconditioner="auto", verbose=False, quad_order=3, c0=C0_DEFAULT,
+                     nf_cutoff_lambda=1.0, preconditioner="auto", aca_tol=1e-6, aca_leaf_size=64, aca_eta=1.5,
+                     aca_max_rank=50, verbose=False, quad_order=3, c0=C0_DEFAULT,
                      max_dense_matrix_bytes=_DEFAULT_MAX_DENSE_PAYLOAD_BYTES, precision="fp64", nf_block_size=128):
     """src/Workflow.jl:67-317 for the dense methods (:dense_direct, :dense_gmres).
     ``precision="mixed"`` selects the complex64 + FP64-refinement solve.  The
@@ -905,8 +999,13 @@ def solve_scattering(mesh, freq_hz, excitation, method="auto", dense_direct_limi
         raise ValueError(f"solve_scattering: freq_hz must be finite and positive, got {freq_hz}")
     if method not in ("auto", "dense_direct", "dense_gmres", "aca_gmres", "mlfma"):
         raise ValueError("solve_scattering: method must be :auto, :dense_direct, :dense_gmres, :aca_gmres, or :mlfma")
-    if preconditioner not in ("auto", "lu", "ilu", "diag", "none"):
+    if preconditioner not in ("auto", "lu", "ilu", "diag", "none", "spai"):
         raise ValueError("solve_scattering: preconditioner must be :auto, :lu, :ilu, :diag, or :none")
+    if preconditioner in ("lu", "ilu"):
+        raise ValueError("solve_scattering: " + _NO_FACTOR.format(preconditioner) +
+                         "; preconditioner=:auto builds the sparse approximate inverse")
+    if not (math.isfinite(nf_cutoff_lambda) and nf_cutoff_lambda > 0):
+        raise ValueError(f"solve_scattering: nf_cutoff_lambda must be finite and positive, got {nf_cutoff_lambda}")
     warnings = []
     lam = c0 / f
     k = 2 * math.pi * f / c0
@@ -930,8 +1029,6 @@ def solve_scattering(mesh, freq_hz, excitation, method="auto", dense_direct_limi
     if sel in ("aca_gmres", "mlfma"):
         raise NotImplementedError(f"solve_scattering: method {sel} (N={N}) is outside the dense GPU path; "
                                   "force method='dense_gmres'")
-    if preconditioner == "ilu":
-        raise ValueError("solve_scattering: preconditioner=:ilu is supported only by the MLFMA path")
     if N * N * 16 > max_dense_matrix_bytes:
         raise ValueError("solve_scattering dense EFIE matrix exceeds max_dense_matrix_bytes")
     if isinstance(excitation, PlaneWaveExcitation):
@@ -944,8 +1041,13 @@ def solve_scattering(mesh, freq_hz, excitation, method="auto", dense_direct_limi
         raise ValueError("solve_scattering: excitation vector must contain only finite values")
     P = None
     if sel == "dense_gmres":
-        if preconditioner in ("auto", "lu"):
-            P = build_nearfield_preconditioner(None, mesh, rwg, 0.0 if nf_cutoff_lambda >= 1.0 else nf_cutoff_lambda * lam)
+        if preconditioner in ("auto", "spai"):
+            P = build_nearfield_preconditioner(None, mesh, rwg, nf_cutoff_lambda * lam, "spai")
+            nnz, rad, cap, capped = P.pattern_info(rwg)
+            if capped:
+                warnings.append(f"near-field preconditioner: sparse approximate inverse over nf_cutoff_lambda="
+                                f"{nf_cutoff_lambda:g} ({P.cutoff:.4g} m) keeps the {cap} nearest bases per column "
+                                f"({capped} of {N} columns cut short; mean radius {rad:.4g} m, {nnz} nonzeros)")
         elif preconditioner == "diag":
             P = build_nearfield_preconditioner(None, mesh, rwg, nf_cutoff_lambda * lam, "diag")
     method_gpu = "direct" if sel == "dense_direct" else ("mixed" if precision == "mixed" else "fp64")

@@ -362,6 +362,7 @@ void validate_opts(const dm_solve_opts& o) {
   DM_REQUIRE(o.method >= DM_METHOD_LU && o.method <= DM_METHOD_MIXED, DM_ERR_ARGUMENT, "unknown solve method");
   DM_REQUIRE(std::isfinite(o.compress_tol) && o.compress_tol >= 0 && o.compress_tol < 1, DM_ERR_ARGUMENT,
              "compress_tol must be in [0, 1)");
+  DM_REQUIRE(o.storage == 0 || o.storage == 1, DM_ERR_ARGUMENT, "storage must be 0 (packed) or 1 (full)");
 }
 
 void check_ctx(const void* a, const char* what) { DM_REQUIRE(a != nullptr, DM_ERR_ARGUMENT, std::string(what) + " is NULL"); }
@@ -1052,6 +1053,22 @@ int dm_precond_create(dm_ctx* ctx, const dm_zmat* z, int kind, double nf_cutoff,
   });
 }
 
+int dm_nearfield_pattern_info(const dm_rwg* rwg, double nf_cutoff, int64_t* nnz, double* mean_radius,
+                              int* max_cols, int64_t* capped_cols) {
+  return guard([&] {
+    check_ctx(rwg, "rwg");
+    DM_REQUIRE(std::isfinite(nf_cutoff), DM_ERR_ARGUMENT, "near-field cutoff must be finite");
+    CtxScope sc(rwg->ctx);
+    dm_rwg& r = const_cast<dm_rwg&>(*rwg);
+    const double cutJ = nf_cutoff > 0 ? nf_cutoff : 2.5 * r.mean_len;
+    auto pat = dmh::spai_pattern(r, cutJ, dmp::spai_max_cols());
+    if (nnz) *nnz = pat->nnz;
+    if (mean_radius) *mean_radius = pat->mean_radius;
+    if (max_cols) *max_cols = dmp::spai_max_cols();
+    if (capped_cols) *capped_cols = pat->capped;
+  });
+}
+
 int dm_precond_apply(dm_ctx* ctx, const dm_precond* P, const double* x, double* y) {
   return guard([&] {
     check_ctx(ctx, "context");
@@ -1099,6 +1116,7 @@ void dm_solve_opts_default(dm_solve_opts* o) {
   o->batch = 0;
   o->learned = nullptr;
   o->adjoint = 0;
+  o->storage = 0;
 }
 
 int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double k_im, double eta0, int quad_order,
@@ -1132,13 +1150,15 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
     z->rwg = rwg_c;
     z->n = n;
     z->ld = round_ld(n);
-    // mixed precision: packed symmetric forms for one-column solves, full forms
-    // for lockstep batches (MFMA); the plain FP64 path keeps full storage
+    // packed symmetric forms for one-column solves (half the bytes per
+    // product), full forms for lockstep batches (the MFMA product reads full
+    // storage) and dense LU; the FP64 GMRES path packs unless asked for full
+    // storage
     const bool batched = batched_solve(o, nrhs);
     if (o.method == DM_METHOD_MIXED)
       z->forms = batched ? (DM_Z64 | DM_Z32) : (DM_Z64P | DM_Z32P);
     else
-      z->forms = DM_Z64;
+      z->forms = (o.method == DM_METHOD_LU || batched || o.storage == 1) ? DM_Z64 : DM_Z64P;
     // buffers of a kept matrix that this solve does not use
     if (!(z->forms & DM_Z64)) z->z64.release();
     if (!(z->forms & DM_Z32)) z->z32.release();
@@ -1174,6 +1194,11 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
         launch_assemble_z64p(a, z->z64p.p, r.max_blk_tris, s);
       }
       a.z32_dual = nullptr;
+    } else if (z->forms & DM_Z64P) {
+      z->z64p.alloc(packed_scalars(n));
+      zero_packed_padding(n, z->z64p.p, sizeof(double), s);
+      ProfScope p(ctx, "assemble_z64", s);
+      launch_assemble_z64p(a, z->z64p.p, r.max_blk_tris, s);
     } else {
       z->z64.alloc((size_t)z->ld * n * 2);
       ProfScope p(ctx, "assemble_z64", s);

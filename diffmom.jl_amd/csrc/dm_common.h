@@ -179,6 +179,8 @@ struct TriTables {
 // Near-field sparse-approximate-inverse pattern (internal basis order).
 struct SpaiPattern {
   double cutJ = 0, cutI = 0;
+  double mean_radius = 0;  // mean over columns of the radius J_j covers
+  i64 capped = 0;          // columns whose cap left bases within the cutoff out
   i64 nnz = 0;
   std::vector<int> jptr, jidx, iptr, iidx;   // per column: support J_j, rows I_j
   std::vector<int> rowptr, colidx, map;      // CSR of M and CSR position -> CSC position

@@ -387,6 +387,11 @@ void finalize_rwg(dm_rwg& r) {
 // within 1.6 x the radius actually covered by J_j (at most 512 nearest).
 // Centres are the midpoints of the two triangle centroids
 // (rwg_centers, src/solver/NearFieldPreconditioner.jl:247-257).
+// The hash grid's cells are sized to the radius the maxJ nearest bases cover
+// on a surface of this mesh's mean edge (~2.7 mean edges for 80), not to the
+// cutoff, so a cutoff of a wavelength does not make every column scan the
+// whole mesh; a column whose J or I radius reaches past the scanned cells is
+// scanned again over a wider block of cells.
 std::shared_ptr<const SpaiPattern> spai_pattern(dm_rwg& r, double cutJ, int maxJ) {
   std::lock_guard<std::mutex> lk(r.tab_mu);
   if (r.spai && r.spai->cutJ == cutJ) return r.spai;
@@ -397,8 +402,8 @@ std::shared_ptr<const SpaiPattern> spai_pattern(dm_rwg& r, double cutJ, int maxJ
   double lo[3] = {INFINITY, INFINITY, INFINITY};
   for (i64 i = 0; i < n; ++i)
     for (int k = 0; k < 3; ++k) lo[k] = std::min(lo[k], c[3 * i + k]);
-  // hash grid with cells of the row cutoff (upper bound 1.6 cutJ)
-  double cell = 1.6 * cutJ;
+  const double reach = std::sqrt((double)maxJ / 3.0) * r.mean_len;  // ~ radius of maxJ nearest
+  const double cell = 1.6 * std::max(std::min(cutJ, reach), 1e-300);
   std::map<std::array<long, 3>, std::vector<int>> grid;
   auto key = [&](i64 i) {
     return std::array<long, 3>{(long)std::floor((c[3 * i] - lo[0]) / cell), (long)std::floor((c[3 * i + 1] - lo[1]) / cell),
@@ -408,40 +413,55 @@ std::shared_ptr<const SpaiPattern> spai_pattern(dm_rwg& r, double cutJ, int maxJ
   P->jptr.assign(n + 1, 0);
   P->iptr.assign(n + 1, 0);
   std::vector<std::pair<double, int>> cand;
+  double rsum = 0;
+  i64 capped = 0;
   for (i64 j = 0; j < n; ++j) {
-    cand.clear();
     auto kj = key(j);
-    for (int dx = -1; dx <= 1; ++dx)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dz = -1; dz <= 1; ++dz) {
-          auto it = grid.find({kj[0] + dx, kj[1] + dy, kj[2] + dz});
-          if (it == grid.end()) continue;
-          for (int m : it->second) {
-            double ddx = c[3 * m] - c[3 * j], ddy = c[3 * m + 1] - c[3 * j + 1], ddz = c[3 * m + 2] - c[3 * j + 2];
-            double d = std::sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-            if (d <= cell) cand.emplace_back(d, m);
-          }
-        }
-    std::sort(cand.begin(), cand.end());
-    int nj = 0;
+    std::vector<int> J, I;
     double rJ = 0;
-    std::vector<int> J;
-    for (auto& pr : cand) {
-      if (pr.first > cutJ || nj >= maxJ) break;
-      J.push_back(pr.second);
-      rJ = pr.first;
-      ++nj;
+    bool capped_col = false;
+    for (long w = 1;; ++w) {
+      // every centre within w * cell of centre j lies in this block of cells
+      cand.clear();
+      for (long dx = -w; dx <= w; ++dx)
+        for (long dy = -w; dy <= w; ++dy)
+          for (long dz = -w; dz <= w; ++dz) {
+            auto it = grid.find({kj[0] + dx, kj[1] + dy, kj[2] + dz});
+            if (it == grid.end()) continue;
+            for (int m : it->second) {
+              double ddx = c[3 * m] - c[3 * j], ddy = c[3 * m + 1] - c[3 * j + 1], ddz = c[3 * m + 2] - c[3 * j + 2];
+              double d = std::sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+              if (d <= w * cell) cand.emplace_back(d, m);
+            }
+          }
+      std::sort(cand.begin(), cand.end());
+      J.clear();
+      rJ = 0;
+      for (auto& pr : cand) {
+        if (pr.first > cutJ || (int)J.size() >= maxJ) break;
+        J.push_back(pr.second);
+        rJ = pr.first;
+      }
+      const bool j_done = (int)J.size() >= maxJ || cutJ <= w * cell || (i64)cand.size() == n;
+      // the cap cut the cutoff short: a further candidate lies within cutJ
+      capped_col = (int)J.size() >= maxJ && J.size() < cand.size() && cand[J.size()].first <= cutJ;
+      const double rI = std::max(1.6 * rJ, 1e-300);
+      const bool i_done = rI <= w * cell || (i64)cand.size() == n;
+      if (j_done && i_done) {
+        I.clear();
+        for (auto& pr : cand) {
+          if (pr.first > rI || (int)I.size() >= 512) break;
+          I.push_back(pr.second);
+        }
+        break;
+      }
     }
+    rsum += rJ;
+    capped += capped_col ? 1 : 0;
     if (J.empty() || J[0] != (int)j) {  // the basis itself is always in its own support
       J.erase(std::remove(J.begin(), J.end(), (int)j), J.end());
       J.insert(J.begin(), (int)j);
       if ((int)J.size() > maxJ) J.resize(maxJ);
-    }
-    double rI = std::max(1.6 * rJ, 1e-300);
-    std::vector<int> I;
-    for (auto& pr : cand) {
-      if (pr.first > rI || (int)I.size() >= 512) break;
-      I.push_back(pr.second);
     }
     std::sort(J.begin(), J.end());
     std::sort(I.begin(), I.end());
@@ -450,6 +470,8 @@ std::shared_ptr<const SpaiPattern> spai_pattern(dm_rwg& r, double cutJ, int maxJ
     P->jptr[j + 1] = (int)P->jidx.size();
     P->iptr[j + 1] = (int)P->iidx.size();
   }
+  P->mean_radius = rsum / (double)std::max<i64>(n, 1);
+  P->capped = capped;
   P->nnz = (i64)P->jidx.size();
   // CSR of M by transposing the CSC support lists
   P->rowptr.assign(n + 1, 0);

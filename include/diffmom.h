@@ -198,6 +198,11 @@ typedef struct dm_solve_opts {
                           > 0 compresses the complex64 matrix (dm_zmat_compress with
                           this eps) for the inner solve; the refinement residual
                           stays the dense complex128 product.  0 = off */
+  int storage;         /* dm_solve_scattering_pw, GMRES one right-hand side at a time:
+                          0 = packed symmetric complex128 storage (half the bytes per
+                          product), 1 = full N x N storage (the reference's dense
+                          Matrix{ComplexF64}).  Batches and dense LU always use full
+                          storage; MIXED always packs its one-column forms. */
 } dm_solve_opts;
 
 typedef struct dm_solve_stats {
@@ -221,6 +226,16 @@ typedef struct dm_precond dm_precond;
 int dm_precond_create(dm_ctx* ctx, const dm_zmat* z, int kind, double nf_cutoff, int block_size, int form,
                       dm_precond** out);
 int dm_precond_apply(dm_ctx* ctx, const dm_precond* p, const double* x, double* y);
+/* The near-field pattern the sparse approximate inverse uses for a cutoff
+ * (nf_cutoff metres, <= 0: 2.5 x mean edge; rwg_centers and the cutoff
+ * search, src/solver/NearFieldPreconditioner.jl:247-338): its nonzeros, the
+ * mean radius its columns actually cover, the column cap (each column keeps
+ * the nearest max_cols bases within the cutoff) and how many columns the cap
+ * cut short of the cutoff.  A cutoff wider than the cap reaches is reported
+ * here, not silently changed.  Any output pointer may be NULL. */
+int dm_nearfield_pattern_info(const dm_rwg* rwg, double nf_cutoff, int64_t* nnz, double* mean_radius,
+                              int* max_cols, int64_t* capped_cols);
+
 int dm_precond_free(dm_precond* p);
 /* Solve Z X = B for nrhs right-hand sides (host buffers, caller order). */
 int dm_solve(dm_ctx* ctx, const dm_zmat* z, const double* B, int64_t ldb, int64_t nrhs,

@@ -386,16 +386,34 @@ def nearfield_pairs(centers, cutoff):
     return np.asarray(I), np.asarray(J)
 
 
+def nearfield_pairs_tree(centers, cutoff):
+    """The same (m, n) set as nearfield_pairs, from a k-d tree (scipy, compiled):
+    for timing the near-field build at full size."""
+    from scipy.spatial import cKDTree
+    N = len(centers)
+    if cutoff <= 0:
+        i = np.arange(N)
+        return i, i
+    P = cKDTree(centers).query_pairs(cutoff, output_type="ndarray")
+    i = np.arange(N)
+    I = np.concatenate([i, P[:, 0], P[:, 1]])
+    J = np.concatenate([i, P[:, 1], P[:, 0]])
+    return I, J
+
+
 class NearFieldLU:
-    """Sparse near-field LU preconditioner: src/solver/NearFieldPreconditioner.jl:670-717."""
+    """Sparse near-field LU preconditioner: src/solver/NearFieldPreconditioner.jl:670-717.
+    Z: the dense matrix, or an EFIE whose entries are computed on demand (the
+    reference's matrix-free build, :847-901)."""
 
     def __init__(self, Z, rwg, cutoff):
         import scipy.sparse as sp
         import scipy.sparse.linalg as spl
         c = rwg_centers(rwg)
-        I, J = nearfield_pairs(c, cutoff)
+        I, J = nearfield_pairs_tree(c, cutoff) if isinstance(Z, EFIE) else nearfield_pairs(c, cutoff)
         N = rwg.nedges
-        A = sp.csc_matrix((Z[I, J], (I, J)), shape=(N, N))
+        vals = Z.entries(I, J) if isinstance(Z, EFIE) else Z[I, J]
+        A = sp.csc_matrix((vals, (I, J)), shape=(N, N))
         self.nnz_ratio = A.nnz / N / N
         self.lu = spl.splu(A)
 

@@ -26,6 +26,14 @@ def test_library_exports_every_declared_symbol(dm):
     assert not missing, missing
 
 
+def test_every_declared_entry_point_has_a_ctypes_signature(dm):
+    """The package binds every header function with explicit argtypes (an
+    unbound one would take Python floats / arrays without conversion)."""
+    dm.lib()
+    missing = [n for n in declared() if n not in dm.SIGNATURES]
+    assert not missing, missing
+
+
 def test_error_string_without_device(dm):
     lib = dm.lib()
     assert lib.dm_version() >= 100
@@ -69,3 +77,31 @@ def test_option_structs_match_the_header(dm, tmp_path):
         jl = julia_fields(cls.__name__)
         assert [n for n, _ in jl] == names
         assert [t for _, t in jl] == [t for _, t in cls._fields_]
+
+
+def _jl_signatures():
+    import json
+    return json.load(open(os.path.join(ROOT, "tests", "golden", "jl_signatures.json")))
+
+
+def test_julia_shim_takes_the_reference_keywords():
+    """Every keyword of the reference functions the shim replaces
+    (tests/golden/jl_signatures.json, read from the reference's source by
+    tests/golden/make_jl_signatures.py) is a keyword of the shim's method."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mk", os.path.join(ROOT, "tests", "golden", "make_jl_signatures.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    shim = open(os.path.join(ROOT, "diffmom.jl_amd", "julia", "DiffMoMHIP.jl")).read()
+    for name, ref_kw in _jl_signatures().items():
+        mine = mk.keywords(shim, name)
+        missing = [k for k in ref_kw if k not in mine]
+        assert not missing, (name, missing)
+
+
+def test_python_mirror_takes_the_reference_keywords(dm):
+    import inspect
+    for name, ref_kw in _jl_signatures().items():
+        params = inspect.signature(getattr(dm, name)).parameters
+        missing = [k for k in ref_kw if k not in params]
+        assert not missing, (name, missing)

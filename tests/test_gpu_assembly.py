@@ -263,7 +263,7 @@ def test_packed_symmetric_storage_and_product(dm, mk):
     Ya = Zd.matmul(X, form=dm.DM_Z64P, adjoint=True)
     assert np.linalg.norm(Ya - P64.conj().T @ X) / np.linalg.norm(ref) < 1e-13
     # diagonal, block and near-field preconditioners read packed storage too
-    for fact, cut in (("diag", 0.0), ("block", 0.0), ("lu", 2.0 * float(np.mean(r.len)))):
+    for fact, cut in (("diag", 0.0), ("block", 0.0), ("spai", 2.0 * float(np.mean(r.len)))):
         P = dm.build_nearfield_preconditioner(Zd, m, r, cut, factorization=fact, block_size=64)
         for f_full, f_packed, tol in ((dm.DM_Z64, dm.DM_Z64P, 1e-11), (dm.DM_Z32, dm.DM_Z32P, 1e-4)):
             x = X[:, 0]

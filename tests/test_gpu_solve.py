@@ -134,6 +134,31 @@ def test_solve_scattering_sphere_mie_gate(dm, O):
         assert np.linalg.norm(rg.I_coeffs - Io) / np.linalg.norm(Io) < 1e-6
 
 
+def test_solve_scattering_nearfield_options_are_honoured_or_refused(dm):
+    """preconditioner=:lu / :ilu are refused (the GPU path builds a sparse
+    approximate inverse, not a factorisation); nf_cutoff_lambda is used as
+    given, and a cutoff wider than the pattern's column cap reaches is
+    reported in the result's warnings with the radius actually covered."""
+    m = dm.make_icosphere(0.05, 2)
+    f = 2.0e9
+    k = 2 * math.pi * f / dm.C0_DEFAULT
+    pw = dm.make_plane_wave([0, 0, -k], 1.0, [1.0, 0, 0])
+    for pc in ("lu", "ilu"):
+        with pytest.raises(ValueError):
+            dm.solve_scattering(m, f, pw, method="dense_gmres", preconditioner=pc)
+    res = dm.solve_scattering(m, f, pw, method="dense_gmres", gmres_tol=1e-8)  # nf_cutoff_lambda = 1
+    assert any("keeps the 80 nearest" in w for w in res.warnings)
+    r = dm.build_rwg(m)
+    lam = dm.C0_DEFAULT / f
+    small = dm.build_nearfield_preconditioner(None, m, r, 0.2 * lam)
+    nnz, rad, cap, capped = small.pattern_info(r)
+    assert cap == 80 and capped == 0 and 0 < rad <= 0.2 * lam and nnz < 80 * r.nedges
+    assert dm.build_nearfield_preconditioner(None, m, r, lam).pattern_info(r)[3] == r.nedges
+    res2 = dm.solve_scattering(m, f, pw, method="dense_gmres", gmres_tol=1e-8, nf_cutoff_lambda=0.2)
+    assert not any("nearest" in w for w in res2.warnings)
+    assert np.linalg.norm(res2.I_coeffs - res.I_coeffs) / np.linalg.norm(res.I_coeffs) < 1e-6
+
+
 def test_nearfield_preconditioner_columns_and_gain(dm, O):
     """Each column of the near-field sparse approximate inverse solves the same
     least-squares problem numpy solves on the oracle's matrix, and the

@@ -334,6 +334,40 @@ def test_solves_match(O):
         assert ok and it2 < it and np.linalg.norm(x - I) / np.linalg.norm(I) < 1e-6
 
 
+def test_nearfield_pairs_tree_and_entry_build(O):
+    """The k-d tree pair set equals the spatial-hash one
+    (src/solver/NearFieldPreconditioner.jl:259-338), and the near-field LU
+    built from entries computed on demand equals the one from the dense Z."""
+    m = O.make_icosphere(0.1, 2)
+    r = O.build_rwg(m)
+    c = O.rwg_centers(r)
+    for cut in (0.0, 0.01, 0.03):
+        a = set(zip(*O.nearfield_pairs(c, cut)))
+        b = set(zip(*O.nearfield_pairs_tree(c, cut)))
+        assert a == b and len(a) >= r.nedges
+    k = 2 * np.pi / 0.1
+    ef = O.EFIE(r, k)
+    Z = ef.assemble()
+    rhs = np.random.default_rng(1).standard_normal(r.nedges) + 0j
+    y1 = O.NearFieldLU(Z, r, 0.02)(rhs)
+    y2 = O.NearFieldLU(ef, r, 0.02)(rhs)
+    assert np.linalg.norm(y1 - y2) <= 1e-12 * np.linalg.norm(y1)
+
+
+def test_build_nearfield_rejects_factorizations_it_does_not_build(dm):
+    """The GPU path builds a sparse approximate inverse of the near-field
+    matrix; asking for the reference's LU / ILU factorisation is an error,
+    not a silent substitution."""
+    import pytest as _pt
+    for f in ("lu", "ilu", "cholesky"):
+        with _pt.raises(ValueError):
+            dm.build_nearfield_preconditioner(None, None, None, 0.01, factorization=f)
+    P = dm.build_nearfield_preconditioner(None, None, None, 0.01)
+    assert P.factorization == "spai" and P.cutoff == 0.01
+    with _pt.raises(ValueError):
+        dm.build_nearfield_preconditioner(None, None, None, float("nan"))
+
+
 def test_c0_fixture_reproduces(O):
     npz = np.load(os.path.join(GOLD, "c0_plate.npz"))
     m = O.make_rect_plate(0.1, 0.1, 6, 6)
