@@ -152,6 +152,7 @@ def lib():
         "dm_zmat_get": ([vp, i32, vp], i32), "dm_zmat_entries": ([vp, i32, vp, vp, i64, vp], i32),
         "dm_zmat_free": ([vp], i32),
         "dm_apply": ([vp, vp, i32, i32, vp, i64, i64, vp, i64], i32),
+        "dm_apply_noalloc": ([vp, vp, i32, i32, vp, vp], i32),
         "dm_zmat_compress": ([vp, C.c_double, i32, vp], i32),
         "dm_farfield": ([vp, vp, vp, i64, C.c_double, C.c_double, i32, vp, i64, i64, vp, i64], i32),
         "dm_mf_create": ([vp, vp, d, d, d, i32, pp], i32), "dm_mf_apply": ([vp, vp, i32, vp, vp], i32),

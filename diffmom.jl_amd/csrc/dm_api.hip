@@ -16,6 +16,35 @@ using namespace dms;
 
 thread_local DevCache* tl_cache = nullptr;
 
+// Scratch of the allocation-free product (dm_apply_noalloc): a file-scope
+// device array, one copy per GPU for the life of the process.  Host code never
+// uses the array's name as an address (that is a host-side stand-in): dm_create
+// takes each device's address of it once with hipGetSymbolAddress, with that
+// device current, and kernels get that pointer.
+static constexpr size_t kScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+// with `device` current
+void scratch_init(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  DM_REQUIRE(device >= 0 && device < kMaxDevices, DM_ERR_ARGUMENT, "device index out of range for the scratch table");
+  if (g_scratch[device]) return;
+  void* ptr = nullptr;
+  HIP_OK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  DM_REQUIRE(ptr != nullptr, DM_ERR_DEVICE, "no device address for the scratch array");
+  g_scratch[device] = ptr;
+}
+
+void* scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return device >= 0 && device < kMaxDevices ? g_scratch[device] : nullptr;
+}
+}  // namespace
+
 void* DevCache::take(size_t b) {
   auto it = free_blocks.find(b);
   if (it == free_blocks.end()) return nullptr;
@@ -390,6 +419,7 @@ int dm_create(int device, dm_ctx** out) {
     c->cu_count = prop.multiProcessorCount;
     c->cache.device = device;
     c->cache.cap = std::min<size_t>(size_t(64) << 30, prop.totalGlobalMem / 4);
+    scratch_init(device);
     *out = c.release();
   });
 }
@@ -865,6 +895,49 @@ int dm_apply(dm_ctx* ctx, const dm_zmat* z, int form, int adjoint, const double*
       HIP_OK(hipMemcpy2DAsync(Y + 2 * ldy * j0, 16 * ldy, xe.p, 16 * n, 16 * n, R, hipMemcpyDeviceToHost, s));
       HIP_OK(hipStreamSynchronize(s));
     }
+  });
+}
+
+int dm_apply_noalloc(dm_ctx* ctx, const dm_zmat* z, int form, int adjoint, const double* x, double* y) {
+  return guard([&] {
+    check_ctx(ctx, "context");
+    check_ctx(z, "zmat");
+    DM_REQUIRE(x != nullptr && y != nullptr, DM_ERR_ARGUMENT, "x and y must be device pointers");
+    DM_REQUIRE(z->ctx == ctx, DM_ERR_ARGUMENT, "matrix belongs to another context");
+    DM_REQUIRE(form == DM_Z64 || form == DM_Z32 || form == DM_Z64P || form == DM_Z32P, DM_ERR_ARGUMENT,
+               "dm_apply_noalloc takes one stored form: DM_Z64, DM_Z32, DM_Z64P or DM_Z32P");
+    need_form(z, form);
+    const i64 n = z->n;
+    const bool packed = form == DM_Z64P || form == DM_Z32P;
+    // scratch layout: x and y in internal order, then the packed product's partials
+    const size_t vec = (size_t)16 * n;
+    const size_t vec_al = (vec + 255) & ~(size_t)255;
+    const size_t need = 2 * vec_al + (packed ? dml::symv_workspace_bytes(n) : 0);
+    if (need > kScratchBytes)
+      throw DmError(DM_ERR_ARGUMENT, "dm_apply_noalloc: N = " + std::to_string(n) + " needs " + std::to_string(need) +
+                                         " bytes of scratch for this form, more than its " +
+                                         std::to_string(kScratchBytes) + " bytes");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_OK(hipSetDevice(ctx->device));
+    unsigned char* base = static_cast<unsigned char*>(scratch_of(ctx->device));
+    DM_REQUIRE(base != nullptr, DM_ERR_DEVICE, "scratch of this device was not set up (dm_create)");
+    double* xi = reinterpret_cast<double*>(base);
+    double* yi = reinterpret_cast<double*>(base + vec_al);
+    void* ws = base + 2 * vec_al;
+    hipStream_t s = ctx->stream;
+    const int* ext = z->rwg->d_ext.p;
+    launch_permute(n, ext, x, xi, 1, s);
+    if (adjoint) dml::conj(n, xi, s);
+    const ZRef zr = z->ref(form);
+    if (form == DM_Z64)
+      launch_gemv_z64(n, z->ld, z->z64.p, xi, yi, s);
+    else if (form == DM_Z32)
+      launch_gemv_z32(n, z->ld, z->z32.p, xi, yi, s);
+    else
+      dml::symv_packed(n, zr, xi, yi, ws, dmt::symv_shape_known(ctx, z, form), s);
+    launch_permute(n, ext, yi, y, 0, s);
+    if (adjoint) dml::conj(n, y, s);
+    HIP_OK(hipGetLastError());
   });
 }
 

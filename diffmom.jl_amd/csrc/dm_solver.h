@@ -80,6 +80,8 @@ struct OpZ32 : DevOp {
 namespace dmt {
 // Tuned launch shape of the symmetric product (timed on first use per size).
 int symv_shape(dm_ctx* ctx, const dm_zmat* z, int form, void* ws, hipStream_t s);
+// The shape already chosen for this size, or the default; never times anything.
+int symv_shape_known(const dm_ctx* ctx, const dm_zmat* z, int form);
 // Tuned wave count of the compressed product, as a multiple of the resident
 // wave slots (1..64).
 int symv_hier_shape(dm_ctx* ctx, const dm_zmat* z, void* ws, hipStream_t s);

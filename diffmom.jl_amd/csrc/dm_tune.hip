@@ -42,9 +42,21 @@ double time_ms(Launch&& run, hipStream_t s) {
 
 }  // namespace
 
+namespace {
+std::string symv_key(const dm_zmat* z, int form) {
+  return std::string("symv/") + (form == DM_Z32P ? "c64/" : "c128/") + std::to_string(z->n);
+}
+constexpr int kSymvFallback = 8;
+}  // namespace
+
+int symv_shape_known(const dm_ctx* ctx, const dm_zmat* z, int form) {
+  auto it = ctx->tune.find(symv_key(z, form));
+  return it != ctx->tune.end() ? it->second : kSymvFallback;
+}
+
 int symv_shape(dm_ctx* ctx, const dm_zmat* z, int form, void* ws, hipStream_t s) {
-  const int fallback = 8;
-  const std::string key = std::string("symv/") + (form == DM_Z32P ? "c64/" : "c128/") + std::to_string(z->n);
+  const int fallback = kSymvFallback;
+  const std::string key = symv_key(z, form);
   auto it = ctx->tune.find(key);
   if (it != ctx->tune.end()) return it->second;
   if (!tuning_enabled()) return fallback;

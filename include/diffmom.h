@@ -147,6 +147,23 @@ int dm_zmat_free(dm_zmat* z);
 int dm_apply(dm_ctx* ctx, const dm_zmat* z, int form, int adjoint, const double* X, int64_t ldx,
              int64_t nrhs, double* Y, int64_t ldy);
 
+/* No reference counterpart: an allocation-free form of dm_apply for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  y = op(Z) x for one
+ * right-hand side; x and y are DEVICE pointers to N complex128 values in the
+ * caller's basis order (x == y allowed).  form: DM_Z64, DM_Z32 (full storage)
+ * or DM_Z64P, DM_Z32P (packed, with the launch shape the tuner chose for this
+ * size or the default; never timed here).  The work is queued on the context's
+ * stream and the call returns without waiting (dm_sync waits).  It calls no
+ * hipMalloc / hipFree: its intermediates live in one 16 MiB device array per
+ * GPU, set up by dm_create, so N is limited to what fits (full storage:
+ * 32 N bytes, N <= 524 288; packed: 32 N bytes plus the symmetric product's
+ * partials, N <= 11 392); larger N is refused with DM_ERR_ARGUMENT.  Every call
+ * on one GPU shares that array: calls that overlap in time on one GPU (two
+ * contexts' streams, or two host threads with different contexts) race and
+ * must be ordered by the caller. */
+int dm_apply_noalloc(dm_ctx* ctx, const dm_zmat* z, int form, int adjoint, const double* x, double* y);
+
 /* ------------------------------------------------------- matrix-free operator */
 /* matrixfree_efie_operator / mul! / adjoint: src/assembly/EFIE.jl:759-1007.
  * Each product recomputes every entry on the device (FP64), never storing Z. */
