@@ -146,19 +146,40 @@ def cpu_baseline(mesh, rwg, k, cutoff, plain_iters, check, budget_s=15.0):
     return out
 
 
-def random_planewaves(dm, k, count, seed=20261202):
-    """Directions: normalised N(0, I3) draws; polarisations: random unit vectors
-    projected perpendicular to k (the many-angle workload's recipe)."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(count):
-        d = rng.standard_normal(3)
-        d /= np.linalg.norm(d)
-        p = rng.standard_normal(3)
-        p -= d * (p @ d)
-        p /= np.linalg.norm(p)
-        out.append(dm.make_plane_wave(k * d, 1.0, p))
-    return out
+def many_plates(dm, ctx, k, specs):
+    """The 64 seeded plates: their complex128 matrices in one persistent
+    assembly launch (one tile queue over all plates) against one launch per
+    plate; roofline of the persistent launch by the far-pair FLOP definition."""
+    meshes = [dm.make_rect_plate(*sp) for sp in specs]
+    rwgs = [dm.build_rwg(m) for m in meshes]
+    Zs = dm.assemble_Z_efie_multi(meshes, rwgs, k, forms=dm.DM_Z64)  # warm-up (tables, code)
+    for Z in Zs:
+        Z.free()
+    ctx.sync()
+
+    def best(fn, reps=3):
+        t_best = float("inf")
+        for _ in range(reps):
+            ctx.sync()
+            t0 = time.perf_counter()
+            out = fn()
+            ctx.sync()
+            t_best = min(t_best, time.perf_counter() - t0)
+            for Z in out:
+                Z.free()
+        return t_best
+
+    t_multi = best(lambda: dm.assemble_Z_efie_multi(meshes, rwgs, k, forms=dm.DM_Z64))
+    t_single = best(lambda: [dm.assemble_Z_efie(m, r, k, forms=dm.DM_Z64, device=True, mesh_precheck=False,
+                                                max_output_bytes=None) for m, r in zip(meshes, rwgs)])
+    flop = sum(52.0 * 9 * far_pairs(m, r) for m, r in zip(meshes, rwgs))
+    ns = [r.nedges for r in rwgs]
+    log(f"64 plates (N {min(ns)}..{max(ns)}): one launch {t_multi * 1e3:.2f} ms, one per plate {t_single * 1e3:.2f} ms")
+    return dict(count=len(specs), N_min=min(ns), N_max=max(ns), N_sum=sum(ns), one_launch_s=round(t_multi, 5),
+                one_launch_per_plate_s=round(t_single, 5), speedup=round(t_single / t_multi, 2),
+                roofline=dict(bound="valu", unit="TFLOP/s", achieved=round(flop / t_multi / 1e12, 2),
+                              peak=FP64_PEAK_TFLOPS, frac=round(flop / t_multi / 1e12 / FP64_PEAK_TFLOPS, 4),
+                              note="52 x 9 FLOP per far triangle pair, wall time of the launch incl. host set-up"))
 
 
 COPY_BYTES = 1 << 30
@@ -372,7 +393,7 @@ def main():
 
     batched = None
     if not a.no_batch and a.batch_rhs > 0:
-        waves = random_planewaves(dm, k, a.batch_rhs)
+        waves, plate_specs = dm.many_angle_workload(k, a.batch_rhs, 64)
         ctx.profile_reset()
         ctx.profile(True)
         dt, X, st_b, tb = solve("mixed", nf, waves, maxiter=300)
@@ -421,6 +442,7 @@ def main():
                                     gemm_tflops_incl_host_copies=round(8.0 * 3 * nom * N * X.shape[1] / t_ff / 1e12, 2),
                                     rcs_max_m2=float(rcs.max()))
         log(f"far field: {nom} directions x {X.shape[1]} currents in {t_ff * 1e3:.1f} ms")
+        batched["plates"] = many_plates(dm, ctx, k, plate_specs)
 
     # roofline of the dominant kernel of the chosen form
     nq = 3

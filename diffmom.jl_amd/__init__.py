@@ -616,6 +616,44 @@ def make_plane_wave(k_vec, E0, pol):
     return PlaneWaveExcitation(np.asarray(k_vec, float), float(E0), np.asarray(pol, float))
 
 
+MANY_ANGLE_SEED = 20261202
+
+
+def seeded_planewaves(k, count, rng):
+    """count plane waves from a numpy Generator: directions are normalised
+    N(0, I3) draws, polarisations random unit vectors projected perpendicular
+    to the direction, E0 = 1 (the many-angle workload)."""
+    out = []
+    for _ in range(count):
+        d = rng.standard_normal(3)
+        d /= np.linalg.norm(d)
+        p = rng.standard_normal(3)
+        p -= d * (p @ d)
+        p /= np.linalg.norm(p)
+        out.append(make_plane_wave(k * d, 1.0, p))
+    return out
+
+
+def seeded_plate_specs(count, rng):
+    """(Lx, Ly, Nx, Ny) of count plates with sides ~ U[0.05, 0.3] m and
+    Nx, Ny ~ U{4..30} (make_rect_plate, src/geometry/Mesh.jl:309-396), drawn
+    from a numpy Generator."""
+    out = []
+    for _ in range(count):
+        lx, ly = rng.uniform(0.05, 0.3, 2)
+        nx, ny = rng.integers(4, 31, 2)
+        out.append((float(lx), float(ly), int(nx), int(ny)))
+    return out
+
+
+def many_angle_workload(k, nwaves=256, nplates=64, seed=MANY_ANGLE_SEED):
+    """The many-angle / many-plate workload: nwaves seeded plane waves, then
+    nplates seeded plate specs from the same generator stream."""
+    rng = np.random.default_rng(seed)
+    waves = seeded_planewaves(k, nwaves, rng)
+    return waves, seeded_plate_specs(nplates, rng)
+
+
 def assemble_multiple_excitations(mesh, rwg, excitations, quad_order=3,
                                   max_output_bytes=_DEFAULT_MAX_DENSE_PAYLOAD_BYTES,
                                   max_work_bytes=_DEFAULT_MAX_EXCITATION_WORK_BYTES,

@@ -176,23 +176,37 @@ struct Krylov {
   // per-step (estimate, subdiagonal), 1 / subdiagonal
   DBuf<double> Hd, csd, snd, gd, stt, scd;
   double g0[2] = {0.0, 0.0};
+  // Buffers only grow: m is the logical capacity (the kernels index with it),
+  // so the refinement loop, whose remaining iteration budget shrinks at every
+  // step, reuses the first step's buffers instead of reallocating them.
   void reserve(i64 n_, int m_, bool flexible) {
+    if (n_ != n) {
+      V.release();
+      Zs.release();
+      w.release();
+      t.release();
+      part.release();
+      nrm.release();
+    }
     n = n_;
     m = m_;
-    Hd.alloc((size_t)2 * (m + 1) * m);
-    csd.alloc(m);
-    snd.alloc(2 * (size_t)m);
-    gd.alloc(2 * (size_t)(m + 1));
-    stt.alloc(2 * (size_t)m);
-    scd.alloc(1);
-    part.alloc(2 * dml::dots_part_count(n, m + 1));
-    nrm.alloc(dml::norm_part_count(n));
-    V.alloc((size_t)2 * n * (m + 1));
-    if (flexible) Zs.alloc((size_t)2 * n * m);
-    w.alloc(2 * n);
-    t.alloc(2 * n);
-    hbuf.alloc(2 * (2 * (m + 1) + 1));
-    ybuf.alloc(2 * (m + 1));
+    grow(Hd, (size_t)2 * (m + 1) * m);
+    grow(csd, m);
+    grow(snd, 2 * (size_t)m);
+    grow(gd, 2 * (size_t)(m + 1));
+    grow(stt, 2 * (size_t)m);
+    grow(scd, 1);
+    grow(part, 2 * dml::dots_part_count(n, m + 1));
+    grow(nrm, dml::norm_part_count(n));
+    grow(V, (size_t)2 * n * (m + 1));
+    if (flexible) grow(Zs, (size_t)2 * n * m);
+    grow(w, 2 * n);
+    grow(t, 2 * n);
+    grow(hbuf, 2 * (2 * (m + 1) + 1));
+    grow(ybuf, 2 * (m + 1));
+  }
+  static void grow(DBuf<double>& b, size_t count) {
+    if (b.n < count) b.alloc(count);
   }
 };
 

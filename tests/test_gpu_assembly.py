@@ -290,6 +290,37 @@ def test_multi_mesh_persistent_assembly(dm, O):
             assert rel_fro(A, _oracle_Z(O, m, r, K0)) <= 1e-10
 
 
+def test_64_seeded_plates_in_one_launch(dm, O):
+    """The many-plate workload: 64 seeded plates (0.05-0.3 m sides, 4-30
+    cells, src/geometry/Mesh.jl:309-396) from the many-angle generator stream,
+    assembled in one persistent launch.  Every matrix equals its single-mesh
+    assembly (to the rounding of the accumulation order) and 40 sampled
+    entries of each (diagonal, near and far) match the oracle to 1e-10."""
+    _, specs = dm.many_angle_workload(K0, 256, 64)
+    meshes = [dm.make_rect_plate(*sp) for sp in specs]
+    rwgs = [dm.build_rwg(m) for m in meshes]
+    Ns = [r.nedges for r in rwgs]
+    print(f"64 plates, N {min(Ns)}..{max(Ns)}, total {sum(Ns)}", flush=True)
+    Zs = dm.assemble_Z_efie_multi(meshes, rwgs, K0, forms=dm.DM_Z64)
+    rng = np.random.default_rng(64)
+    worst = 0.0
+    for i, (m, r, Z) in enumerate(zip(meshes, rwgs, Zs)):
+        single = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64, device=True)
+        n = r.nedges
+        M = rng.integers(0, n, 40)
+        Nn = np.concatenate([M[:10], np.clip(M[10:25] + rng.integers(-2, 3, 15), 0, n - 1), rng.integers(0, n, 15)])
+        zm = Z.entries(M, Nn, dm.DM_Z64)
+        assert np.array_equal(zm, single.entries(M, Nn, dm.DM_Z64)) or \
+            np.abs(zm - single.entries(M, Nn, dm.DM_Z64)).max() <= 1e-14 * np.abs(zm).max(), i
+        zo = O.EFIE(oracle_rwg(O, m, r), K0).entries(M, Nn)
+        err = np.linalg.norm(zm - zo) / np.linalg.norm(zo)
+        worst = max(worst, err)
+        assert err <= 1e-10, (i, specs[i], err)
+        single.free()
+        Z.free()
+    print(f"worst sampled-entry error vs oracle {worst:.2e}", flush=True)
+
+
 @pytest.mark.parametrize("mk", ["plate", "sphere"])
 def test_tuned_symmetric_product_shapes(dm, mk):
     """The tuner times the symmetric product's launch shapes on first use and

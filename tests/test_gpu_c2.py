@@ -16,7 +16,9 @@ to 5e-8, near-field SPAI, FGMRES with FP64 Arnoldi, FP64 refinement), then:
     rows;
   * currents and bistatic RCS against a tight FP64 GMRES solve (1e-10) on the
     dense complex128 matrix, at the workload's tolerance (1e-6) and after
-    refinement to 1e-8 (the 1e-6 currents / RCS bar).
+    refinement to 1e-8 (the 1e-6 currents / RCS bar);
+  * the many-angle case: 256 seeded plane waves in lockstep batches, every
+    column's true FP64 residual against the dense complex128 matrix.
 """
 import math
 
@@ -151,3 +153,30 @@ def test_c2_currents_and_rcs_vs_tight_fp64(dm, c2):
           f"tol 1e-8 currents {e8:.2e} RCS {r8:.2e} ({st8.niter} it, {st8.outer} refinements)", flush=True)
     assert e6 <= 1e-4 and r6 <= 1e-5
     assert e8 <= 1e-6 and r8 <= 1e-6
+
+
+def test_c2_lockstep_256_waves_true_residuals(dm, O, c2):
+    """256 seeded plane waves on the sphere in lockstep batches (complex64 Z X
+    on the matrix cores, per-column FGMRES, FP64 refinement): every column's
+    true residual ||v - Z64 x|| / ||v||, recomputed here from the library's
+    complex128 product of all columns, is <= 1e-6; the right-hand sides of
+    four waves match the oracle's assemble_plane_wave; one column matches a
+    one-wave solve."""
+    m, r, P = c2["m"], c2["r"], c2["P"]
+    waves, _ = dm.many_angle_workload(K0, 256, 64)
+    print("[c2] 256 plane waves, lockstep", flush=True)
+    X, st, _, Z = dm.solve_planewaves(m, r, K0, waves, "mixed", P, tol=1e-6, maxiter=300, keep_matrix=True)
+    assert st.solved and X.shape == (r.nedges, 256)
+    V = dm.assemble_multiple_excitations(m, r, waves)
+    R = Z.matmul(X, form=dm.DM_Z64) - V
+    res = np.linalg.norm(R, axis=0) / np.linalg.norm(V, axis=0)
+    print(f"[c2] lockstep: {st.niter} iterations, worst column residual {res.max():.2e}", flush=True)
+    assert res.max() <= 1e-6
+    orr = oracle_rwg(O, m, r)
+    for j in (0, 85, 170, 255):
+        vo = O.assemble_v_plane_wave(orr, waves[j].k_vec, 1.0, waves[j].pol)
+        assert np.linalg.norm(V[:, j] - vo) <= 1e-13 * np.linalg.norm(vo)
+    Z.free()
+    x1, st1, _ = dm.solve_planewaves(m, r, K0, [waves[85]], "mixed", P, tol=1e-6, maxiter=1000, batch=-1,
+                                     compress_tol=5e-8)
+    assert np.linalg.norm(X[:, 85] - x1[:, 0]) <= 1e-4 * np.linalg.norm(x1[:, 0])
