@@ -152,7 +152,11 @@ struct NearTables {
   DBuf<int> flag;
 };
 
-AsmArgs make_args(dm_rwg& r, const TriTables& T, double kr, double ki, double eta0, i64 ld, NearTables& nt) {
+// Assembly arguments with the near-field tables written to caller-provided
+// device buffers (jself: Nt x 18, jadj: Nt x 54 doubles; flag: one int that the
+// caller cleared, shared by several meshes if it likes).
+AsmArgs make_args_into(dm_rwg& r, const TriTables& T, double kr, double ki, double eta0, i64 ld, double* jself,
+                       double* jadj, int* flag, bool launch_near = true) {
   AsmArgs a;
   a.n = r.n;
   a.ld = ld;
@@ -178,14 +182,10 @@ AsmArgs make_args(dm_rwg& r, const TriTables& T, double kr, double ki, double et
   a.k_im = ki;
   a.eta0 = eta0;
   hipStream_t s = r.ctx->stream;
-  nt.jself.alloc(a.nt * 18);
-  nt.jadj.alloc(a.nt * 54);
-  nt.flag.alloc(1);
-  HIP_OK(hipMemsetAsync(nt.flag.p, 0, sizeof(int), s));
-  a.jself = nt.jself.p;
-  a.jadj = nt.jadj.p;
-  a.flag = nt.flag.p;
-  launch_near_tables(a, T, nt.jself.p, nt.jadj.p, s);
+  a.jself = jself;
+  a.jadj = jadj;
+  a.flag = flag;
+  if (launch_near) launch_near_tables(a, T, jself, jadj, s);
   // tile-side images (depend on the RWG and quadrature order only; kept on the RWG)
   a.side_cap = r.max_blk_tris;
   {
@@ -207,6 +207,15 @@ AsmArgs make_args(dm_rwg& r, const TriTables& T, double kr, double ki, double et
     }
   }
   return a;
+}
+
+AsmArgs make_args(dm_rwg& r, const TriTables& T, double kr, double ki, double eta0, i64 ld, NearTables& nt) {
+  const i64 ntri = r.mesh->nt;
+  nt.jself.alloc(ntri * 18);
+  nt.jadj.alloc(ntri * 54);
+  nt.flag.alloc(1);
+  HIP_OK(hipMemsetAsync(nt.flag.p, 0, sizeof(int), r.ctx->stream));
+  return make_args_into(r, T, kr, ki, eta0, ld, nt.jself.p, nt.jadj.p, nt.flag.p);
 }
 
 void check_flag(NearTables& nt, hipStream_t s) {
@@ -706,9 +715,18 @@ int dm_assemble_z_multi(dm_ctx* ctx, int64_t nmesh, const dm_rwg* const* rwgs, d
     CtxScope sc(ctx);
     hipStream_t s = ctx->stream;
     std::vector<std::unique_ptr<dm_zmat>> zs(nmesh);
-    std::vector<NearTables> nts(nmesh);
     std::vector<AsmArgs> args(nmesh);
     int cap = 0;
+    // near-field tables of all meshes in one pair of buffers, one overflow flag
+    std::vector<i64> toff(nmesh + 1, 0);
+    for (i64 m = 0; m < nmesh; ++m) toff[m + 1] = toff[m] + rwgs[m]->mesh->nt;
+    NearTables nt;
+    nt.jself.alloc((size_t)toff[nmesh] * 18);
+    nt.jadj.alloc((size_t)toff[nmesh] * 54);
+    nt.flag.alloc(1);
+    HIP_OK(hipMemsetAsync(nt.flag.p, 0, sizeof(int), s));
+    std::vector<const TriTables*> tabs(nmesh);
+    DBuf<char> near_staging;
     {
       ProfScope p(ctx, "near_tables", s);
       for (i64 m = 0; m < nmesh; ++m) {
@@ -724,10 +742,14 @@ int dm_assemble_z_multi(dm_ctx* ctx, int64_t nmesh, const dm_rwg* const* rwgs, d
         z->k_im = k_im;
         z->eta0 = eta0;
         z->quad_order = quad_order;
-        args[m] = make_args(r, T, k_re, k_im, eta0, z->ld, nts[m]);
+        args[m] = make_args_into(r, T, k_re, k_im, eta0, z->ld, nt.jself.p + 18 * toff[m], nt.jadj.p + 54 * toff[m],
+                                 nt.flag.p, false);
+        tabs[m] = &T;
         cap = std::max(cap, r.max_blk_tris);
         zs[m] = std::move(z);
       }
+      // every mesh's near-field tables in one launch per kind
+      launch_near_tables_multi(args, tabs, nt.jself.p, nt.jadj.p, near_staging, s);
     }
     // work list: largest meshes first
     std::vector<i64> order(nmesh);
@@ -736,16 +758,20 @@ int dm_assemble_z_multi(dm_ctx* ctx, int64_t nmesh, const dm_rwg* const* rwgs, d
     std::vector<AsmArgs> oargs(nmesh);
     for (i64 q = 0; q < nmesh; ++q) oargs[q] = args[order[q]];
     // the persistent kernel lays out LDS for the largest block; meshes with
-    // smaller blocks get side images built for that layout
-    std::vector<DBuf<char>> imgs(2 * nmesh);
+    // smaller blocks get side images built for that layout (kept on the RWG
+    // set per quadrature order and layout, like its own-layout images)
     for (i64 q = 0; q < nmesh; ++q) {
       AsmArgs& a = oargs[q];
       if (a.side_cap == cap) continue;
+      dm_rwg& r = const_cast<dm_rwg&>(*rwgs[order[q]]);
+      std::lock_guard<std::mutex> lk(r.tab_mu);
       for (int f = 0; f < 2; ++f) {
         const size_t stride = side_image_bytes(a.nq, f == 1, cap);
-        DBuf<char>& img = imgs[2 * q + f];
-        img.alloc(stride * a.nblk);
-        launch_build_sides(a, cap, f == 1, img.p, s);
+        DBuf<char>& img = r.side_img_cap[{quad_order * 2 + f, cap}];
+        if (!img.p) {
+          img.alloc(stride * a.nblk);
+          launch_build_sides(a, cap, f == 1, img.p, s);
+        }
         if (f == 1) {
           a.side_img64 = img.p;
           a.side_stride64 = (i64)stride;
@@ -781,7 +807,7 @@ int dm_assemble_z_multi(dm_ctx* ctx, int64_t nmesh, const dm_rwg* const* rwgs, d
       ProfScope p(ctx, "assemble_multi", s);
       launch_assemble_multi(oargs, ptrs, form, cap, s);
     }
-    for (i64 m = 0; m < nmesh; ++m) check_flag(nts[m], s);
+    check_flag(nt, s);
     for (i64 m = 0; m < nmesh; ++m) outs[m] = zs[m].release();
   });
 }

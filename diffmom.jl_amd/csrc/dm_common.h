@@ -222,6 +222,9 @@ struct dm_rwg {
   std::shared_ptr<SpaiPattern> spai;  // latest pattern; preconditioners keep their own reference
   // tile-side images per quadrature order and precision (built lazily)
   DBuf<char> side_img[8][2];
+  // side images for another mesh's block layout (many-mesh launches), keyed by
+  // (2 x quadrature order + FP64 flag, triangles per block of the layout)
+  std::map<std::pair<int, int>, DBuf<char>> side_img_cap;
   // the matrix buffers of the last dm_solve_scattering_pw that did not hand
   // its matrix to the caller, kept for the next solve on this handle (same
   // sizes: no allocation, no free at the end of the solve); freed with the
@@ -362,6 +365,11 @@ struct AsmArgs {
 
 void launch_near_tables(const AsmArgs& a, const TriTables& t, double* jself, double* jadj,
                         hipStream_t s);
+// near-field tables of several meshes in one launch per kind: mesh m's tables
+// start at jself + 18 x (triangles of meshes 0..m-1), jadj + 54 x the same;
+// staging keeps the per-mesh arguments on the device until the work is done
+void launch_near_tables_multi(const std::vector<AsmArgs>& a, const std::vector<const TriTables*>& t, double* jself,
+                              double* jadj, DBuf<char>& staging, hipStream_t s);
 // Bytes of one block's tile-side image and the kernel that fills nblk of them.
 size_t side_image_bytes(int nq, bool f64, int cap);
 void launch_build_sides(const AsmArgs& a, int cap, bool f64, char* img, hipStream_t s);

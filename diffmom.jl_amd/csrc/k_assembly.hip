@@ -147,9 +147,7 @@ struct NearArgs {
 //   J = sum_ab (f_e(a).f_e'(b) - 4/k^2) Gs_ab w_a w_b
 //     + sum_a w_a [ (f_e(a).f_e'(a) - 4/k^2) S_a/(4 pi 2A) + sum_b f_e(a).(r_b - r_a) w_b/(4 pi R_ab) ]
 // with f_e(r) = r - v_e.
-__global__ __launch_bounds__(64) void k_near_self(NearArgs a, double* jself) {
-  i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x;
-  if (t >= a.nt) return;
+__device__ void near_self_one(const NearArgs& a, i64 t, double* jself) {
   const double* P = a.rq7 + t * 21;
   const double* V = a.vtx + t * 9;
   double A = a.area[t];
@@ -214,9 +212,7 @@ __global__ __launch_bounds__(64) void k_near_self(NearArgs a, double* jself) {
 // triangle u = nb[t][j], normalised by (2A_t)(2A_u):
 //   J = sum_ab^{nq} (f_e(a).f'_e'(b) - 4/k^2) Gs_ab w_a w_b
 //     + sum_a^7 w_a [ sum_b^7 f_e(a).f'_e'(b) w_b/(4 pi R_ab) - 4/k^2 S_u(a)/(4 pi 2A_u) ]
-__global__ __launch_bounds__(64) void k_near_adj(NearArgs a, double* jadj) {
-  i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
-  if (idx >= a.nt * 3) return;
+__device__ void near_adj_one(const NearArgs& a, i64 idx, double* jadj) {
   i64 t = idx / 3;
   int u = a.nb[idx];
   c2* out = reinterpret_cast<c2*>(jadj) + idx * 9;
@@ -276,6 +272,44 @@ __global__ __launch_bounds__(64) void k_near_adj(NearArgs a, double* jadj) {
   }
 #pragma unroll
   for (int i = 0; i < 9; ++i) out[i] = J[i];
+}
+
+__global__ __launch_bounds__(64) void k_near_self(NearArgs a, double* jself) {
+  const i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (t < a.nt) near_self_one(a, t, jself);
+}
+__global__ __launch_bounds__(64) void k_near_adj(NearArgs a, double* jadj) {
+  const i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (idx < a.nt * 3) near_adj_one(a, idx, jadj);
+}
+
+// Many meshes in one launch: item g of the concatenated lists (per mesh m,
+// items per triangle x its triangles starting at toff[m]) belongs to the mesh
+// found by binary search; each mesh's tables start at per x 9 complex x toff[m].
+__device__ inline int mesh_of(const i64* __restrict__ toff, int nmesh, i64 g) {
+  int lo = 0, hi = nmesh - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (toff[mid] <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+__global__ __launch_bounds__(64) void k_near_self_multi(const NearArgs* __restrict__ args,
+                                                        const i64* __restrict__ toff, int nmesh, double* jself) {
+  const i64 g = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (g >= toff[nmesh]) return;
+  const int m = mesh_of(toff, nmesh, g);
+  near_self_one(args[m], g - toff[m], jself + 18 * toff[m]);
+}
+__global__ __launch_bounds__(64) void k_near_adj_multi(const NearArgs* __restrict__ args,
+                                                       const i64* __restrict__ toff, int nmesh, double* jadj) {
+  const i64 g = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (g >= 3 * toff[nmesh]) return;
+  const int m = mesh_of(toff, nmesh, g / 3);
+  near_adj_one(args[m], g - 3 * toff[m], jadj + 54 * toff[m]);
 }
 
 // ------------------------------------------------------------------ far tiles
@@ -934,6 +968,29 @@ void launch_near_tables(const AsmArgs& a, const TriTables& t, double* jself, dou
   unsigned g1 = (unsigned)((a.nt + 63) / 64), g2 = (unsigned)((3 * a.nt + 63) / 64);
   k_near_self<<<g1, 64, 0, s>>>(n, jself);
   k_near_adj<<<g2, 64, 0, s>>>(n, jadj);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_near_tables_multi(const std::vector<AsmArgs>& a, const std::vector<const TriTables*>& t, double* jself,
+                              double* jadj, DBuf<char>& staging, hipStream_t s) {
+  const int nmesh = (int)a.size();
+  DM_REQUIRE(nmesh >= 1 && t.size() == a.size(), DM_ERR_ARGUMENT, "one table set per mesh");
+  std::vector<NearArgs> h(nmesh);
+  std::vector<i64> toff(nmesh + 1, 0);
+  for (int m = 0; m < nmesh; ++m) {
+    h[m] = near_args(a[m], *t[m]);
+    toff[m + 1] = toff[m] + a[m].nt;
+  }
+  const size_t ab = sizeof(NearArgs) * nmesh, ob = sizeof(i64) * (nmesh + 1);
+  const size_t obase = (ab + 15) & ~size_t(15);
+  staging.alloc(obase + ob);
+  HIP_OK(hipMemcpyAsync(staging.p, h.data(), ab, hipMemcpyHostToDevice, s));
+  HIP_OK(hipMemcpyAsync(staging.p + obase, toff.data(), ob, hipMemcpyHostToDevice, s));
+  const NearArgs* dargs = reinterpret_cast<const NearArgs*>(staging.p);
+  const i64* doff = reinterpret_cast<const i64*>(staging.p + obase);
+  const i64 total = toff[nmesh];
+  k_near_self_multi<<<(unsigned)((total + 63) / 64), 64, 0, s>>>(dargs, doff, nmesh, jself);
+  k_near_adj_multi<<<(unsigned)((3 * total + 63) / 64), 64, 0, s>>>(dargs, doff, nmesh, jadj);
   HIP_OK(hipGetLastError());
 }
 
