@@ -260,6 +260,32 @@ def traffic_probe(dm, subdiv):
     print(f"copies done ({COPY_BYTES} bytes each)", flush=True)
 
 
+def kernel_probe(dm, subdiv, what):
+    """Dispatches for counter runs (no timing): 'assembly' = three FP64
+    assemblies of the packed complex128 matrix with its complex64 rounding;
+    'zx' = five complex64 Z X products of 256 seeded plane-wave columns on the
+    matrix cores (after the one-time split of Z into bf16 planes)."""
+    k = 2 * math.pi / 0.1
+    mesh = dm.make_icosphere(0.1, subdiv)
+    rwg = dm.build_rwg(mesh)
+    if what == "assembly":
+        for i in range(3):
+            Z = dm.assemble_Z_efie(mesh, rwg, k, forms=dm.DM_Z64P | dm.DM_Z32P | dm.DM_Z32_ROUNDED, device=True,
+                                   max_output_bytes=None, mesh_precheck=False)
+            mesh.ctx.sync()
+            Z.free()
+            print(f"assembly {i} done", flush=True)
+        return
+    Z = dm.assemble_Z_efie(mesh, rwg, k, forms=dm.DM_Z64 | dm.DM_Z32 | dm.DM_Z32_ROUNDED, device=True,
+                           max_output_bytes=None, mesh_precheck=False)
+    waves, _ = dm.many_angle_workload(k, 256, 0)
+    V = dm.assemble_multiple_excitations(mesh, rwg, waves)
+    for i in range(5):
+        Z.matmul(V, form=dm.DM_Z32)
+        mesh.ctx.sync()
+        print(f"Z X product {i} done", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -275,6 +301,8 @@ def main():
     ap.add_argument("--compress-tol", type=float, default=5e-8,
                     help="chosen form: cross-approximate the complex64 matrix to this relative Frobenius error "
                          "(0 = dense complex64)")
+    ap.add_argument("--kernel-probe", choices=["assembly", "zx"], default=None,
+                    help="no timing: dispatches of one kernel kind for rocprofv3 --pmc runs")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="no timing: 5 symmetric products of the packed complex64 matrix, its compression "
                          "(5e-8) and 5 compressed products, then 3 plain 1 GiB device-to-device copies (counter "
@@ -288,6 +316,8 @@ def main():
     dm = load_pkg()
     if a.traffic_probe:
         return traffic_probe(dm, a.subdiv)
+    if a.kernel_probe:
+        return kernel_probe(dm, a.subdiv, a.kernel_probe)
     k = 2 * math.pi / 0.1
     mesh = dm.make_icosphere(0.1, a.subdiv)
     rwg = dm.build_rwg(mesh)
