@@ -172,8 +172,10 @@ def lib():
         "dm_mpnn_param_count": ([], i64), "dm_mpnn_create": ([vp, vp, pp], i32),
         "dm_mpnn_get_params": ([vp, vp], i32), "dm_mpnn_free": ([vp], i32),
         "dm_rwg_graph": ([vp, d, vp, vp, vp], i32),
-        "dm_mpnn_data_create": ([vp, i64, vp, vp, vp, vp, vp, vp, i64, vp, pp], i32),
+        "dm_mpnn_data_create": ([vp, i64, vp, vp, vp, vp, vp, i64, vp, pp], i32),
         "dm_mpnn_data_free": ([vp], i32),
+        "dm_mpnn_data_set_matrix": ([vp, i64, i64, vp, i64], i32),
+        "dm_mpnn_residual_operator": ([vp, vp, vp, vp], i32),
         "dm_mpnn_loss_grad": ([vp, vp, vp, i64, i64, C.POINTER(d), vp], i32),
         "dm_mpnn_train": ([vp, vp, vp, i32, i64, d, C.c_uint64, vp], i32),
         "dm_mpnn_apply": ([vp, vp, vp, d, vp, vp, vp], i32),
@@ -1176,7 +1178,7 @@ class MPNN:
         return out
 
     def loss_grad(self, data, s0=0, s1=None):
-        """Mean loss over samples [s0, s1) of `data` and its gradient."""
+        """Mean objective over samples [s0, s1) of `data` and its gradient."""
         s1 = data.nsample if s1 is None else s1
         loss = C.c_double()
         g = np.zeros(mpnn_param_count(), np.float32)
@@ -1191,7 +1193,8 @@ class MPNN:
         return hist
 
     def apply(self, rwg, k, z, r):
-        """z + network(z, r) for one mesh (caller order)."""
+        """The network's correction d(z, r) for one mesh (caller order, in the
+        units of r); the preconditioner returns z + S d."""
         z = np.ascontiguousarray(z, np.complex128)
         r = np.ascontiguousarray(r, np.complex128)
         out = np.zeros_like(z)
@@ -1213,22 +1216,40 @@ class MPNN:
 
 
 class MPNNData:
-    """Training samples on the device.  Each sample is a dict with z, r, e
-    (complex, N) and nbr, sgn, geom of its mesh (rwg_graph)."""
+    """Training samples on the device.  Each sample is a dict with z = S r and
+    r (complex, N), nbr, sgn, geom of its mesh (rwg_graph) and the mesh's
+    operator A = Z S under "A" (N x N complex64, caller order; the samples of
+    one mesh share one array and are consecutive), of which the device keeps
+    a copy.  The objective of a sample is ||Z M r - r||^2 / ||r||^2, the
+    relative residual one application of the preconditioner leaves."""
 
     def __init__(self, samples, ctx=None):
         self.ctx = ctx or default_context()
+        for s in samples:
+            if s.get("A") is None:
+                raise ValueError("every training sample needs its mesh's operator A = Z S under 'A'")
         sizes = [len(s["z"]) for s in samples]
         ptr = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
         nbr = np.concatenate([np.where(s["nbr"] >= 0, s["nbr"] + off, -1) for s, off in zip(samples, ptr[:-1])])
         cat = lambda key, dt: np.ascontiguousarray(np.concatenate([s[key] for s in samples]), dt)  # noqa: E731
         nbr = np.ascontiguousarray(nbr, np.int32)
         sgn, geom = cat("sgn", np.float32), cat("geom", np.float32)
-        z, r, e = cat("z", np.complex128), cat("r", np.complex128), cat("e", np.complex128)
+        z, r = cat("z", np.complex128), cat("r", np.complex128)
         h = C.c_void_p()
         _check(lib().dm_mpnn_data_create(self.ctx.h, int(ptr[-1]), _ptr(nbr), _ptr(sgn), _ptr(geom), _ptr(z), _ptr(r),
-                                         _ptr(e), len(samples), _ptr(ptr), C.byref(h)))
+                                         len(samples), _ptr(ptr), C.byref(h)))
         self.h, self.nsample, self.nnode = h, len(samples), int(ptr[-1])
+        a = 0
+        while a < len(samples):
+            A = samples[a]["A"]
+            b = a + 1
+            while b < len(samples) and samples[b]["A"] is A:
+                b += 1
+            A = np.asfortranarray(A, np.complex64)
+            if A.shape != (sizes[a], sizes[a]):
+                raise DimensionMismatch(f"operator shape {A.shape} != ({sizes[a]}, {sizes[a]})")
+            _check(lib().dm_mpnn_data_set_matrix(self.h, a, b, A.ctypes.data_as(C.c_void_p), sizes[a]))
+            a = b
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -1236,10 +1257,20 @@ class MPNNData:
             self.h = None
 
 
+def residual_operator(S):
+    """A = Z S of a near-field operator S (NearFieldOperator built on a matrix
+    that holds DM_Z64), complex64, caller order: the operator whose residual
+    the learned preconditioner is trained on."""
+    out = np.zeros((S.N, S.N), np.complex64, order="F")
+    _check(lib().dm_mpnn_residual_operator(S.Z.ctx.h, S.Z.h, S.h, out.ctypes.data_as(C.c_void_p)))
+    return out
+
+
 @dataclass
 class LearnedPreconditioner:
-    """FGMRES preconditioner M r = S r + network(S r, r); S is the near-field
-    SPAI over `cutoff` metres (<= 0: the library default)."""
+    """FGMRES preconditioner M r = S (r + d(S r, r)), d the network's
+    correction; S is the near-field SPAI over `cutoff` metres (<= 0: the
+    library default)."""
     net: MPNN
     cutoff: float = 0.0
 
@@ -1275,33 +1306,59 @@ def make_training_meshes(count, seed=0, wavelength=0.1, jitter=0.1, max_sphere_s
     return out
 
 
-def generate_training_samples(meshes, k, iters=(1, 2, 4, 8, 16, 32), cutoff_edges=2.0, seed=0, tol=1e-10):
-    """(residual, error) pairs from the library's GMRES on each mesh: a random
-    plane wave b, the converged x* (right SPAI-preconditioned GMRES to tol),
-    and the iterates x_k after k = iters steps: r = b - Z x_k, e = x* - x_k,
-    z = S r."""
+def generate_training_samples(meshes, k, iters=(1, 2, 4, 8, 16, 32), cutoff_edges=2.0, seed=0, tol=1e-10,
+                              inputs="residual"):
+    """Training samples from the library's GMRES on each mesh with a random
+    plane wave b: inputs="residual": the converged x* (right
+    SPAI-preconditioned GMRES to tol) and the iterates x_k after k = iters
+    steps give the residuals r = b - Z x_k (with errors e = x* - x_k);
+    inputs="arnoldi": the Arnoldi basis vectors v_k (k = iters) of the
+    SPAI-preconditioned Krylov space of b, the vectors FGMRES hands its
+    preconditioner.  Each sample holds r (the input), z = S r and the mesh's
+    operator A = Z S (shared by its samples)."""
+    if inputs not in ("residual", "arnoldi"):
+        raise ValueError("inputs must be 'residual' or 'arnoldi'")
     rng = np.random.default_rng(seed)
     samples = []
     for m in meshes:
         rwg = build_rwg(m)
-        Z = assemble_Z_efie(m, rwg, k, forms=DM_Z64P, device=True, mesh_precheck=False, max_output_bytes=None)
+        Z = assemble_Z_efie(m, rwg, k, forms=DM_Z64, device=True, mesh_precheck=False, max_output_bytes=None)
         P = build_nearfield_preconditioner(None, m, rwg, cutoff_edges * float(np.mean(rwg.len)))
         S = NearFieldOperator(P, Z)
+        A = residual_operator(S)
         d = rng.standard_normal(3)
         d /= np.linalg.norm(d)
         pol = rng.standard_normal(3)
         pol -= d * (pol @ d)
         pol /= np.linalg.norm(pol)
         b = assemble_v_plane_wave(m, rwg, k * d, 1.0, pol)
-        xs, _ = solve_gmres(Z, b, preconditioner=P, precond_side="right", tol=tol, maxiter=1000)
         nbr, sgn, geom = rwg_graph(rwg, k)
+        if inputs == "arnoldi":
+            V = [b / np.linalg.norm(b)]
+            for j in range(1, max(iters)):
+                w = Z.matmul(S @ V[-1])
+                for _ in range(2):
+                    for v in V:
+                        w = w - (np.vdot(v, w)) * v
+                nw = np.linalg.norm(w)
+                if not nw > 1e-10 * np.linalg.norm(b):
+                    break
+                V.append(w / nw)
+            for it in iters:
+                if it <= len(V):
+                    v = V[it - 1]
+                    samples.append(dict(z=S @ v, r=v, nbr=nbr, sgn=sgn, geom=geom, A=A))
+            del S
+            Z.free()
+            continue
+        xs, _ = solve_gmres(Z, b, preconditioner=P, precond_side="right", tol=tol, maxiter=1000)
         for it in iters:
             xk, _ = solve_gmres(Z, b, preconditioner=P, precond_side="right", tol=1e-15, maxiter=int(it),
                                 check_gmres_convergence=False)
             r = b - Z.matmul(xk)
             if not np.linalg.norm(r) > 1e-12 * np.linalg.norm(b):
                 continue
-            samples.append(dict(z=S @ r, r=r, e=xs - xk, nbr=nbr, sgn=sgn, geom=geom))
+            samples.append(dict(z=S @ r, r=r, e=xs - xk, nbr=nbr, sgn=sgn, geom=geom, A=A))
         del S
         Z.free()
     return samples

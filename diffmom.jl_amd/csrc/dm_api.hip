@@ -249,12 +249,6 @@ void assemble_packed(dm_zmat* z, const AsmArgs& a, const dm_rwg& r, int forms, h
 
 }  // namespace
 
-struct dm_precond {
-  dm_ctx* ctx = nullptr;
-  const dm_zmat* z = nullptr;
-  std::unique_ptr<DevPrec> p;
-};
-
 namespace {
 
 // Widest lockstep batch that fits in half of the free device memory: per

@@ -237,3 +237,10 @@ void solve_batch(dm_ctx* ctx, const dm_solve_opts& o, DevOp* A64, DevOp* A32, De
                  hipEvent_t z64_ready);
 
 }  // namespace dms
+
+// A preconditioner handed to the caller (dm_precond_create).
+struct dm_precond {
+  dm_ctx* ctx = nullptr;
+  const dm_zmat* z = nullptr;
+  std::unique_ptr<dms::DevPrec> p;
+};

@@ -3,16 +3,25 @@
 //
 // Nodes are RWG bases, edges join bases that share a triangle (at most 4 per
 // basis), signed by the product of the two bases' orientations on that
-// triangle.  For a residual r with near-field preconditioned z = S r:
-//   rho = sqrt(mean |z|^2), q = sum z |z|, c = conj(q) / (|q| rho)
-//   u_i = [Re c z_i, Im c z_i, Re c r_i, Im c r_i, g1_i, g2_i]
+// triangle.  For a vector r (a residual or a Krylov basis vector) with
+// near-field preconditioned z = S r:
+//   rho = sqrt(mean |r|^2), q = sum r |r|, c = conj(q) / (|q| rho)
+//   tau = sqrt(mean |c z|^2)
+//   u_i = [Re c z_i / tau, Im c z_i / tau, Re c r_i, Im c r_i, g1_i, g2_i]
 //   h^0 = W_e u;  l = 0..3:  m_i = sum_j s_ij h_j / max(deg_i, 1),
 //                            h <- h + tanh(W_s h + W_n m + b)
-//   y = W_d h^4;  M r = z + (y_0 + i y_1) / c
+//   y = W_d h^4;  d = (y_0 + i y_1) / c   (a correction in residual units)
+//   M r = z + S d = S (r + d)
 // g1 = k len / (2 pi / 10), g2 = (A+ + A-) / len^2.  Parameters are kept in
 // FP32 (master copy for Adam) and used rounded to bfloat16; activations and
-// accumulation are FP32.  The network is scale- and phase-equivariant in r
-// through c.
+// accumulation are FP32.  M is scale- and phase-equivariant: M(a r) = a M(r)
+// for complex a, through c.
+//
+// Training objective of a sample: the relative residual that one application
+// of M leaves, ||Z M r - r||^2 / ||r||^2.  With A = Z S (the sample's mesh,
+// node order, a complex64 copy on the device) and p = r - A r it is
+// ||A y - c p||^2 / (|c|^2 ||r||^2), whose gradient in y is
+// 2 A^H (A y - c p) / (|c|^2 ||r||^2).
 #pragma once
 #include <cstdint>
 
@@ -51,20 +60,19 @@ struct Cache {
   float* y;   // n x 2
 };
 
-// Per-sample normaliser c (complex) and loss denominator |c|^2 ||e||^2 for the
-// samples [s0, s0 + ns) whose nodes are ptr[s] .. ptr[s + 1] (global node ids,
-// e may be null).
-void stats(int ns, const int64_t* ptr, int64_t node0, const double* z, const double* e, double* cden, hipStream_t s);
-// u and (when e != null) target c (e - z) for nodes [0, n) of a batch whose
-// first sample is s0; sample of local node i is smp[i] (global sample id).
+// Per-sample normaliser c (complex), den = |c|^2 ||r||^2 and tau for the
+// samples [s0, s0 + ns) whose nodes are ptr[s] .. ptr[s + 1] (global node
+// ids): cden[4 s + {0, 1, 2, 3}] = Re c, Im c, den, tau.
+void stats(int ns, const int64_t* ptr, int64_t node0, const double* z, const double* r, double* cden, hipStream_t s);
+// Input features u of nodes [0, n) of a batch whose first sample is s0;
+// sample of local node i is smp[i] (global sample id).
 void features(i64 n, const int* smp, int s0, const double* cden, const double* z, const double* r, const float* geom,
-              const double* e, float* u, float* tgt, hipStream_t s);
+              float* u, hipStream_t s);
 // Forward through the network; bf16: NPARAM parameters (bfloat16 bits).
 // gnode0: global id of local node 0 (neighbour ids are global).
 void forward(const Graph& g, i64 gnode0, const uint16_t* bf16, const Cache& c, float* hbuf, hipStream_t s);
-// Loss of the batch (mean over ns samples) and dL/dy; loss_parts: n doubles.
-void loss(i64 n, const int* smp, int s0, const double* cden, int ns, const float* y, const float* tgt, float* gy,
-          double* loss_parts, double* loss_out, hipStream_t s);
+// out[0] = sum of parts[0 .. n) (fixed order).
+void sum(i64 n, const double* parts, double* out, hipStream_t s);
 // Gradient of the loss w.r.t. the (rounded) parameters into grad (NPARAM floats).
 // work: backward scratch of backward_floats(n) floats; part: outer-product
 // partials of outer_floats() floats.
@@ -76,7 +84,21 @@ void backward(const Graph& g, i64 gnode0, const uint16_t* bf16, const Cache& c, 
 void adam(float* p, const float* grad, float* m, float* v, int step, float lr, uint16_t* bf16, hipStream_t s);
 // bfloat16 copy of the parameters.
 void to_bf16(const float* p, uint16_t* bf16, hipStream_t s);
-// Preconditioner output out = z + (y0 + i y1) / c (c from cden[0]).
-void output(i64 n, const double* z, const float* y, const double* cden, double* out, hipStream_t s);
+// The correction d = (y0 + i y1) / c of one sample (c from cden[0..1]).
+void correction(i64 n, const float* y, const double* cden, double* d, hipStream_t s);
+// The n x n complex128 identity (column-major).
+void identity(i64 n, double* out, hipStream_t s);
+// out = a + b (complex128, n).
+void add(i64 n, const double* a, const double* b, double* out, hipStream_t s);
+// Node-order complex64 copy of an internal-order complex128 matrix (n x n,
+// column-major): out[a + b n] = A[iofe[a] + iofe[b] n].
+void gather_c64(i64 n, const double* a_int, const int* iofe, float* out, hipStream_t s);
+// p = r - A r for k samples of one matrix (columns of n nodes, consecutive).
+void res_rhs(i64 n, int k, const float* A, const double* r, double* p, hipStream_t s);
+// For k samples of one matrix (columns of y, p consecutive; cden of those
+// samples): q = A y - c p and loss parts |q_i|^2 / (den ns); when gy is not
+// null also gy = 2 A^H q / (den ns).  work: 2 n k doubles.
+void res_loss(i64 n, int k, const float* A, const float* y, const double* p, const double* cden, int ns, float* q,
+              double* parts, float* gy, double* work, hipStream_t s);
 
 }  // namespace dmn

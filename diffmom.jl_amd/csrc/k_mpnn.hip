@@ -1,6 +1,9 @@
 // k_mpnn.hip — forward, backward and Adam kernels of the learned
 // preconditioner's message-passing network (layout and equations in k_mpnn.h).
 //
+// Training products with the samples' matrices A = Z S are small (n up to a
+// few thousand): a row per thread for A y, a wave per row for A^H q.
+//
 // Layers run 64 nodes per workgroup: the layer's two 64 x 64 weight matrices
 // (bfloat16 in memory, widened to FP32 in LDS, rows padded to 65 floats), the
 // nodes' hidden states and their aggregated neighbour messages are staged in
@@ -8,6 +11,8 @@
 // gradients are sums of per-node outer products, reduced per workgroup into
 // FP32 partials and across workgroups in FP64 (fixed order).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "k_mpnn.h"
 
@@ -39,34 +44,32 @@ __device__ inline double bsum(double v, double* red) {
   return t;
 }
 
-// one workgroup per sample
+// one workgroup per sample: c from r, then tau = rms |c z| and den
 __global__ __launch_bounds__(256) void k_stats(const int64_t* __restrict__ ptr, int64_t node0,
-                                               const double2* __restrict__ z, const double2* __restrict__ e,
+                                               const double2* __restrict__ z, const double2* __restrict__ r,
                                                double* __restrict__ cden) {
   __shared__ double red[4];
   const int s = blockIdx.x;
   const int64_t a = ptr[s] - node0, b = ptr[s + 1] - node0;
-  double s2 = 0, qr = 0, qi = 0, e2 = 0;
+  double r2 = 0, qr = 0, qi = 0, z2 = 0;
   for (int64_t i = a + threadIdx.x; i < b; i += 256) {
-    double2 v = z[i];
-    double m2 = v.x * v.x + v.y * v.y, m = sqrt(m2);
-    s2 += m2;
+    const double2 v = r[i];
+    const double m2 = v.x * v.x + v.y * v.y, m = sqrt(m2);
+    r2 += m2;
     qr += v.x * m;
     qi += v.y * m;
-    if (e) {
-      double2 w = e[i];
-      e2 += w.x * w.x + w.y * w.y;
-    }
+    const double2 w = z[i];
+    z2 += w.x * w.x + w.y * w.y;
   }
-  s2 = bsum(s2, red);
+  r2 = bsum(r2, red);
   qr = bsum(qr, red);
   qi = bsum(qi, red);
-  e2 = bsum(e2, red);
+  z2 = bsum(z2, red);
   if (threadIdx.x == 0) {
     double cr = 0, ci = 0;
-    double rho = b > a ? sqrt(s2 / (double)(b - a)) : 0.0;
+    const double rho = b > a ? sqrt(r2 / (double)(b - a)) : 0.0;
     if (rho > 0) {
-      double qa = sqrt(qr * qr + qi * qi);
+      const double qa = sqrt(qr * qr + qi * qi);
       if (qa > 0) {
         cr = qr / (qa * rho);
         ci = -qi / (qa * rho);
@@ -74,34 +77,31 @@ __global__ __launch_bounds__(256) void k_stats(const int64_t* __restrict__ ptr, 
         cr = 1.0 / rho;
       }
     }
+    const double c2 = cr * cr + ci * ci;
     cden[4 * s] = cr;
     cden[4 * s + 1] = ci;
-    cden[4 * s + 2] = (cr * cr + ci * ci) * e2;
-    cden[4 * s + 3] = 0;
+    cden[4 * s + 2] = c2 * r2;
+    cden[4 * s + 3] = b > a ? sqrt(c2 * z2 / (double)(b - a)) : 0.0;
   }
 }
 
 __global__ void k_features(i64 n, const int* __restrict__ smp, int s0, const double* __restrict__ cden,
                            const double2* __restrict__ z, const double2* __restrict__ r, const float2* __restrict__ geom,
-                           const double2* __restrict__ e, float* __restrict__ u, float2* __restrict__ tgt) {
+                           float* __restrict__ u) {
   i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int s = smp[i] - s0;
-  const double cr = cden[4 * s], ci = cden[4 * s + 1];
-  double2 zv = z[i], rv = r[i];
+  const double cr = cden[4 * s], ci = cden[4 * s + 1], tau = cden[4 * s + 3];
+  const double it = tau > 0 ? 1.0 / tau : 0.0;
+  const double2 zv = z[i], rv = r[i];
   float* ui = u + i * FIN;
-  ui[0] = (float)(cr * zv.x - ci * zv.y);
-  ui[1] = (float)(cr * zv.y + ci * zv.x);
+  ui[0] = (float)((cr * zv.x - ci * zv.y) * it);
+  ui[1] = (float)((cr * zv.y + ci * zv.x) * it);
   ui[2] = (float)(cr * rv.x - ci * rv.y);
   ui[3] = (float)(cr * rv.y + ci * rv.x);
-  float2 g = geom[i];
+  const float2 g = geom[i];
   ui[4] = g.x;
   ui[5] = g.y;
-  if (e) {
-    double2 ev = e[i];
-    double dr = ev.x - zv.x, di = ev.y - zv.y;
-    tgt[i] = make_float2((float)(cr * dr - ci * di), (float)(cr * di + ci * dr));
-  }
 }
 
 __global__ void k_encode(i64 n, const float* __restrict__ u, const uint16_t* __restrict__ we, float* __restrict__ h) {
@@ -184,23 +184,6 @@ __global__ void k_decode(i64 n, const float* __restrict__ h, const uint16_t* __r
   }
   y[2 * i] = a;
   y[2 * i + 1] = b;
-}
-
-__global__ void k_loss(i64 n, const int* __restrict__ smp, int s0, const double* __restrict__ cden, int ns,
-                       const float2* __restrict__ y, const float2* __restrict__ tgt, float2* __restrict__ gy,
-                       double* __restrict__ parts) {
-  i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double den = cden[4 * (smp[i] - s0) + 2];
-  float2 yv = y[i], tv = tgt[i];
-  double dr = (double)yv.x - tv.x, di = (double)yv.y - tv.y;
-  if (den > 0) {
-    parts[i] = (dr * dr + di * di) / den / ns;
-    gy[i] = make_float2((float)(2 * dr / den / ns), (float)(2 * di / den / ns));
-  } else {
-    parts[i] = 0;
-    gy[i] = make_float2(0.f, 0.f);
-  }
 }
 
 __global__ __launch_bounds__(1024) void k_sum(i64 n, const double* __restrict__ v, double* __restrict__ out) {
@@ -383,25 +366,158 @@ __global__ void k_to_bf16(int np, const float* __restrict__ p, uint16_t* __restr
   if (i < np) bf16[i] = to_bf(p[i]);
 }
 
-__global__ void k_output(i64 n, const double2* __restrict__ z, const float2* __restrict__ y,
-                         const double* __restrict__ cden, double2* __restrict__ out) {
+// d = (y0 + i y1) / c = (y0 + i y1) conj(c) / |c|^2
+__global__ void k_correction(i64 n, const float2* __restrict__ y, const double* __restrict__ cden,
+                             double2* __restrict__ d) {
   i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double cr = cden[0], ci = cden[1];
   const double c2 = cr * cr + ci * ci;
-  double2 zv = z[i];
   if (!(c2 > 0)) {
-    out[i] = zv;
+    d[i] = make_double2(0.0, 0.0);
     return;
   }
-  // (y0 + i y1) / c = (y0 + i y1) conj(c) / |c|^2
-  float2 yv = y[i];
-  double dr = ((double)yv.x * cr + (double)yv.y * ci) / c2;
-  double di = ((double)yv.y * cr - (double)yv.x * ci) / c2;
-  out[i] = make_double2(zv.x + dr, zv.y + di);
+  const float2 yv = y[i];
+  d[i] = make_double2(((double)yv.x * cr + (double)yv.y * ci) / c2, ((double)yv.y * cr - (double)yv.x * ci) / c2);
+}
+
+__global__ void k_identity(i64 n, double2* __restrict__ out) {
+  const i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (idx < n * n) out[idx] = make_double2(idx % n == idx / n ? 1.0 : 0.0, 0.0);
+}
+
+__global__ void k_add(i64 n, const double2* __restrict__ a, const double2* __restrict__ b, double2* __restrict__ out) {
+  i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_double2(a[i].x + b[i].x, a[i].y + b[i].y);
+}
+
+// out[:, c] = A x[:, c] for c < k: one row per thread, the row's k sums in
+// FP64; A is read down its columns, which a wave reads as consecutive rows.
+template <int KMAX, class TX>
+__global__ __launch_bounds__(256) void k_acols(i64 n, int k, const float2* __restrict__ A, const TX* __restrict__ x,
+                                               double2* __restrict__ out) {
+  const i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double ar[KMAX], ai[KMAX];
+#pragma unroll
+  for (int c = 0; c < KMAX; ++c) ar[c] = ai[c] = 0.0;
+  for (i64 j = 0; j < n; ++j) {
+    const float2 af = A[i + j * n];
+    const double zr = af.x, zi = af.y;
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c) {
+      if (c < k) {
+        const TX xv = x[j + c * n];
+        const double xr = xv.x, xi = xv.y;
+        ar[c] += zr * xr - zi * xi;
+        ai[c] += zr * xi + zi * xr;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < KMAX; ++c)
+    if (c < k) out[i + c * n] = make_double2(ar[c], ai[c]);
+}
+
+// out[:, c] = A^H x[:, c] for c < k: one wave per output row i (column i of
+// A, read contiguously by the lanes), FP64 lane sums reduced across the wave.
+template <int KMAX>
+__global__ __launch_bounds__(256) void k_acols_adj(i64 n, int k, const float2* __restrict__ A,
+                                                   const float2* __restrict__ x, double2* __restrict__ out) {
+  const i64 i = blockIdx.x * (i64)(blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  double ar[KMAX], ai[KMAX];
+#pragma unroll
+  for (int c = 0; c < KMAX; ++c) ar[c] = ai[c] = 0.0;
+  const float2* col = A + i * n;
+  for (i64 j = lane; j < n; j += 64) {
+    const float2 af = col[j];
+    const double zr = af.x, zi = -(double)af.y;
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c) {
+      if (c < k) {
+        const float2 xv = x[j + c * n];
+        ar[c] += zr * xv.x - zi * xv.y;
+        ai[c] += zr * xv.y + zi * xv.x;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < KMAX; ++c) {
+    for (int o = 32; o > 0; o >>= 1) {
+      ar[c] += __shfl_xor(ar[c], o, 64);
+      ai[c] += __shfl_xor(ai[c], o, 64);
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c)
+      if (c < k) out[i + c * n] = make_double2(ar[c], ai[c]);
+  }
+}
+
+__global__ void k_gather_c64(i64 n, const double2* __restrict__ a, const int* __restrict__ iofe,
+                             float2* __restrict__ out) {
+  const i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const i64 r = idx % n, c = idx / n;
+  const double2 v = a[(i64)iofe[r] + (i64)iofe[c] * n];
+  out[idx] = make_float2((float)v.x, (float)v.y);
+}
+
+// p = r - Ar
+__global__ void k_res_rhs(i64 nk, const double2* __restrict__ r, const double2* __restrict__ ar,
+                          double2* __restrict__ p) {
+  const i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (i >= nk) return;
+  p[i] = make_double2(r[i].x - ar[i].x, r[i].y - ar[i].y);
+}
+
+// q = Ay - c p, parts = |q|^2 / (den ns)
+__global__ void k_res_q(i64 n, int k, const double2* __restrict__ zy, const double2* __restrict__ p,
+                        const double* __restrict__ cden, int ns, float2* __restrict__ q, double* __restrict__ parts) {
+  const i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (idx >= n * k) return;
+  const int c = (int)(idx / n);
+  const double cr = cden[4 * c], ci = cden[4 * c + 1], den = cden[4 * c + 2];
+  const double2 pv = p[idx], v = zy[idx];
+  const double qr = v.x - (cr * pv.x - ci * pv.y), qi = v.y - (cr * pv.y + ci * pv.x);
+  q[idx] = make_float2((float)qr, (float)qi);
+  parts[idx] = den > 0 ? (qr * qr + qi * qi) / den / ns : 0.0;
+}
+
+// gy = 2 A^H q / (den ns)
+__global__ void k_res_gy(i64 n, int k, const double2* __restrict__ g, const double* __restrict__ cden, int ns,
+                         float2* __restrict__ gy) {
+  const i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (idx >= n * k) return;
+  const double den = cden[4 * (idx / n) + 2];
+  const double f = den > 0 ? 2.0 / den / ns : 0.0;
+  gy[idx] = make_float2((float)(f * g[idx].x), (float)(f * g[idx].y));
 }
 
 unsigned nb256(i64 n) { return (unsigned)((n + 255) / 256); }
+
+constexpr int ZC_K = 8;  // columns per pass of the small products
+
+template <class TX>
+void acols(i64 n, int k, const float* A, const TX* x, double* out, hipStream_t s) {
+  for (int c0 = 0; c0 < k; c0 += ZC_K) {
+    const int kk = std::min(ZC_K, k - c0);
+    k_acols<ZC_K, TX><<<nb256(n), 256, 0, s>>>(n, kk, reinterpret_cast<const float2*>(A), x + (i64)c0 * n,
+                                               reinterpret_cast<double2*>(out) + (i64)c0 * n);
+  }
+}
+
+void acols_adj(i64 n, int k, const float* A, const float2* x, double* out, hipStream_t s) {
+  for (int c0 = 0; c0 < k; c0 += ZC_K) {
+    const int kk = std::min(ZC_K, k - c0);
+    k_acols_adj<ZC_K><<<(unsigned)((n + 3) / 4), 256, 0, s>>>(n, kk, reinterpret_cast<const float2*>(A),
+                                                              x + (i64)c0 * n,
+                                                              reinterpret_cast<double2*>(out) + (i64)c0 * n);
+  }
+}
 
 template <int LA, int KB>
 void outer(i64 n, const float* A, const float* T, const float* B, int bias, float* part, float* grad, int woff,
@@ -416,18 +532,17 @@ void outer(i64 n, const float* A, const float* T, const float* B, int bias, floa
 
 namespace dmn {
 
-void stats(int ns, const int64_t* ptr, int64_t node0, const double* z, const double* e, double* cden, hipStream_t s) {
+void stats(int ns, const int64_t* ptr, int64_t node0, const double* z, const double* r, double* cden, hipStream_t s) {
   if (ns <= 0) return;
-  k_stats<<<ns, 256, 0, s>>>(ptr, node0, reinterpret_cast<const double2*>(z), reinterpret_cast<const double2*>(e),
+  k_stats<<<ns, 256, 0, s>>>(ptr, node0, reinterpret_cast<const double2*>(z), reinterpret_cast<const double2*>(r),
                              cden);
   HIP_OK(hipGetLastError());
 }
 
 void features(i64 n, const int* smp, int s0, const double* cden, const double* z, const double* r, const float* geom,
-              const double* e, float* u, float* tgt, hipStream_t s) {
+              float* u, hipStream_t s) {
   k_features<<<nb256(n), 256, 0, s>>>(n, smp, s0, cden, reinterpret_cast<const double2*>(z),
-                                      reinterpret_cast<const double2*>(r), reinterpret_cast<const float2*>(geom),
-                                      reinterpret_cast<const double2*>(e), u, reinterpret_cast<float2*>(tgt));
+                                      reinterpret_cast<const double2*>(r), reinterpret_cast<const float2*>(geom), u);
   HIP_OK(hipGetLastError());
 }
 
@@ -449,11 +564,8 @@ void forward(const Graph& g, i64 gnode0, const uint16_t* prm, const Cache& c, fl
   HIP_OK(hipGetLastError());
 }
 
-void loss(i64 n, const int* smp, int s0, const double* cden, int ns, const float* y, const float* tgt, float* gy,
-          double* loss_parts, double* loss_out, hipStream_t s) {
-  k_loss<<<nb256(n), 256, 0, s>>>(n, smp, s0, cden, ns, reinterpret_cast<const float2*>(y),
-                                  reinterpret_cast<const float2*>(tgt), reinterpret_cast<float2*>(gy), loss_parts);
-  k_sum<<<1, 1024, 0, s>>>(n, loss_parts, loss_out);
+void sum(i64 n, const double* parts, double* out, hipStream_t s) {
+  k_sum<<<1, 1024, 0, s>>>(n, parts, out);
   HIP_OK(hipGetLastError());
 }
 
@@ -494,9 +606,47 @@ void to_bf16(const float* p, uint16_t* bf16, hipStream_t s) {
   HIP_OK(hipGetLastError());
 }
 
-void output(i64 n, const double* z, const float* y, const double* cden, double* out, hipStream_t s) {
-  k_output<<<nb256(n), 256, 0, s>>>(n, reinterpret_cast<const double2*>(z), reinterpret_cast<const float2*>(y), cden,
-                                    reinterpret_cast<double2*>(out));
+void gather_c64(i64 n, const double* a_int, const int* iofe, float* out, hipStream_t s) {
+  k_gather_c64<<<nb256(n * n), 256, 0, s>>>(n, reinterpret_cast<const double2*>(a_int), iofe,
+                                            reinterpret_cast<float2*>(out));
+  HIP_OK(hipGetLastError());
+}
+
+void res_rhs(i64 n, int k, const float* A, const double* r, double* p, hipStream_t s) {
+  // A r into p, then p = r - p
+  acols(n, k, A, reinterpret_cast<const double2*>(r), p, s);
+  k_res_rhs<<<nb256(n * k), 256, 0, s>>>(n * k, reinterpret_cast<const double2*>(r),
+                                         reinterpret_cast<const double2*>(p), reinterpret_cast<double2*>(p));
+  HIP_OK(hipGetLastError());
+}
+
+void res_loss(i64 n, int k, const float* A, const float* y, const double* p, const double* cden, int ns, float* q,
+              double* parts, float* gy, double* work, hipStream_t s) {
+  acols(n, k, A, reinterpret_cast<const float2*>(y), work, s);
+  k_res_q<<<nb256(n * k), 256, 0, s>>>(n, k, reinterpret_cast<const double2*>(work),
+                                       reinterpret_cast<const double2*>(p), cden, ns, reinterpret_cast<float2*>(q),
+                                       parts);
+  if (gy) {
+    acols_adj(n, k, A, reinterpret_cast<const float2*>(q), work, s);
+    k_res_gy<<<nb256(n * k), 256, 0, s>>>(n, k, reinterpret_cast<const double2*>(work), cden, ns,
+                                          reinterpret_cast<float2*>(gy));
+  }
+  HIP_OK(hipGetLastError());
+}
+
+void correction(i64 n, const float* y, const double* cden, double* d, hipStream_t s) {
+  k_correction<<<nb256(n), 256, 0, s>>>(n, reinterpret_cast<const float2*>(y), cden, reinterpret_cast<double2*>(d));
+  HIP_OK(hipGetLastError());
+}
+
+void identity(i64 n, double* out, hipStream_t s) {
+  k_identity<<<nb256(n * n), 256, 0, s>>>(n, reinterpret_cast<double2*>(out));
+  HIP_OK(hipGetLastError());
+}
+
+void add(i64 n, const double* a, const double* b, double* out, hipStream_t s) {
+  k_add<<<nb256(n), 256, 0, s>>>(n, reinterpret_cast<const double2*>(a), reinterpret_cast<const double2*>(b),
+                                 reinterpret_cast<double2*>(out));
   HIP_OK(hipGetLastError());
 }
 

@@ -283,10 +283,12 @@ int dm_tune_set(dm_ctx* ctx, const char* json);
 /* ------------------------------------------------- learned preconditioner */
 /* No reference counterpart: a 4-layer message-passing network over the RWG
  * edge-adjacency graph (bases sharing a triangle), hidden width 64, bfloat16
- * weights with FP32 accumulation, that corrects the near-field SPAI inside
- * FGMRES: M r = S r + network(S r, r) (opts.precond = DM_PRECOND_LEARNED,
- * opts.learned = the network).  Trained on the GPU by the library's own
- * forward / backward kernels and Adam on (residual, error) pairs. */
+ * weights with FP32 accumulation, that corrects the near-field SPAI S inside
+ * FGMRES: M r = S (r + d(S r, r)) with d the network's correction
+ * (opts.precond = DM_PRECOND_LEARNED, opts.learned = the network).  Trained
+ * on the GPU by the library's own forward / backward kernels and Adam on the
+ * relative residual one application of M leaves, ||Z M r - r||^2 / ||r||^2,
+ * over residuals of the library's GMRES iterates on seeded meshes. */
 int64_t dm_mpnn_param_count(void);
 int dm_mpnn_create(dm_ctx* ctx, const float* params, dm_mpnn** out);
 int dm_mpnn_get_params(const dm_mpnn* net, float* out);
@@ -296,23 +298,32 @@ int dm_mpnn_free(dm_mpnn* net);
 int dm_rwg_graph(const dm_rwg* rwg, double k, int32_t* nbr, float* sgn, float* geom);
 /* Training samples resident on the device: nsample meshes' nodes concatenated
  * (sample s owns nodes sample_ptr[s] .. sample_ptr[s+1]-1; nbr holds global
- * node ids), z = S r, residual r and error e = Z^-1 r (complex128, N). */
+ * node ids), residual r and z = S r (complex128, N). */
 typedef struct dm_mpnn_data dm_mpnn_data;
 int dm_mpnn_data_create(dm_ctx* ctx, int64_t nnode, const int32_t* nbr, const float* sgn, const float* geom,
-                        const double* z, const double* r, const double* e, int64_t nsample,
-                        const int64_t* sample_ptr, dm_mpnn_data** out);
+                        const double* z, const double* r, int64_t nsample, const int64_t* sample_ptr,
+                        dm_mpnn_data** out);
 int dm_mpnn_data_free(dm_mpnn_data* data);
-/* Mean loss over samples [s0, s1) and its gradient w.r.t. the bfloat16-rounded
- * parameters (grad may be NULL). */
+/* The operator A = Z S of one mesh in caller order (N x N complex64,
+ * column-major, host memory): S the preconditioner p built on z (which must
+ * hold DM_Z64), N <= 16384. */
+int dm_mpnn_residual_operator(dm_ctx* ctx, const dm_zmat* z, const dm_precond* p, float* out);
+/* The operator A (N x N complex64, caller order, host memory) of samples
+ * [s0, s1), all of one mesh with N nodes: the device keeps a copy and
+ * p = r - A r for each of those samples.  Every sample needs one before
+ * dm_mpnn_loss_grad or dm_mpnn_train. */
+int dm_mpnn_data_set_matrix(dm_mpnn_data* data, int64_t s0, int64_t s1, const float* a, int64_t n);
+/* Mean objective over samples [s0, s1) and its gradient w.r.t. the
+ * bfloat16-rounded parameters (grad may be NULL). */
 int dm_mpnn_loss_grad(dm_ctx* ctx, dm_mpnn* net, const dm_mpnn_data* data, int64_t s0, int64_t s1,
                       double* loss, float* grad);
 /* Adam training: `epochs` passes over mini-batches of batch_samples samples
- * in a seeded shuffled order; epoch_loss (may be NULL) receives the mean loss
- * of each epoch. */
+ * in a seeded shuffled order; epoch_loss (may be NULL) receives the mean
+ * objective of each epoch. */
 int dm_mpnn_train(dm_ctx* ctx, dm_mpnn* net, const dm_mpnn_data* data, int epochs, int64_t batch_samples,
                   double lr, uint64_t seed, double* epoch_loss);
-/* The preconditioner's correction for one mesh: out = z + network(z, r)
- * (caller order, complex128). */
+/* The network's correction d for one mesh (caller order, complex128, in the
+ * units of r): the preconditioner returns z + S d. */
 int dm_mpnn_apply(dm_ctx* ctx, const dm_mpnn* net, const dm_rwg* rwg, double k, const double* z,
                   const double* r, double* out);
 

@@ -11,19 +11,27 @@ pass is checked against central finite differences of the forward pass
 and the equivariances (global complex scale of the residual, basis renumbering)
 against their closed forms.
 
-Network (per mesh, nodes = RWG bases, edges = bases sharing a triangle):
-  z = S r                        (near-field sparse approximate inverse)
-  rho = sqrt(mean |z|^2), q = sum z |z|, c = conj(q) / (|q| rho)
-  u_i = [Re c z_i, Im c z_i, Re c r_i, Im c r_i, g1_i, g2_i]
+Network (per mesh, nodes = RWG bases, edges = bases sharing a triangle), for a
+vector r (a residual or a Krylov basis vector) and z = S r (near-field sparse
+approximate inverse):
+  rho = sqrt(mean |r|^2), q = sum r |r|, c = conj(q) / (|q| rho)
+  tau = sqrt(mean |c z|^2)
+  u_i = [Re c z_i / tau, Im c z_i / tau, Re c r_i, Im c r_i, g1_i, g2_i]
   h^0 = W_e u
   l = 0..3:  m_i = sum_j s_ij h_j / max(deg_i, 1);  a = W_s h + W_n m + b;  h <- h + tanh(a)
   y = W_d h^4  (2 channels: Re, Im)
-  M r = z + (y_0 + i y_1) / c
+  d = (y_0 + i y_1) / c          (the correction, in the units of r)
+  M r = z + S d = S (r + d)
 Geometry: g1 = k len_i / (2 pi / 10), g2 = (A+_i + A-_i) / len_i^2.  Edge sign
 s_ij = sigma_i(T) sigma_j(T) on the shared triangle T (sigma = +1 on T+, -1 on T-).
 Weights are used rounded to bfloat16 (round to nearest even); activations and
 accumulation are FP32 on the device (FP64 here).
-Loss of one sample (r, e = Z^-1 r):  sum_i |y_i - (c e_i - c z_i)|^2 / sum_i |c e_i|^2.
+Objective of one sample: the relative residual one application of M leaves,
+||Z M r - r||^2 / ||r||^2 = ||A y - c p||^2 / (|c|^2 ||r||^2) with A = Z S and
+p = r - A r.  Pinned by finite differences like the rest, by its closed form
+at W_d = 0 (the SPAI's own relative residual ||r - A r||^2 / ||r||^2), by the
+exact correction d = A^-1 r - r (objective 0), and by recomputing
+||Z M r - r|| from M r directly.
 """
 import numpy as np
 
@@ -115,21 +123,23 @@ def geometry(length, area_plus, area_minus, k):
     return np.stack([g1, g2], axis=1)
 
 
-def normaliser(z):
-    """c of the docstring (0 when z == 0)."""
-    z = np.asarray(z, np.complex128)
-    rho = np.sqrt(np.mean(np.abs(z) ** 2))
+def normaliser(r):
+    """c of the docstring (0 when r == 0)."""
+    r = np.asarray(r, np.complex128)
+    rho = np.sqrt(np.mean(np.abs(r) ** 2))
     if not rho > 0:
         return 0.0 + 0.0j
-    q = np.sum(z * np.abs(z))
+    q = np.sum(r * np.abs(r))
     if abs(q) > 0:
         return np.conj(q) / (abs(q) * rho)
     return 1.0 / rho + 0.0j
 
 
 def features(z, r, geom):
-    c = normaliser(z)
+    c = normaliser(r)
     cz, cr = c * z, c * r
+    tau = np.sqrt(np.mean(np.abs(cz) ** 2))
+    cz = cz / tau if tau > 0 else 0 * cz
     u = np.column_stack([cz.real, cz.imag, cr.real, cr.imag, geom[:, 0], geom[:, 1]])
     return u, c
 
@@ -173,15 +183,16 @@ def forward(theta, u, nbr, sgn, deg, bf16=True):
     return y, cache
 
 
-def sample_loss_and_target(y, z, e, c):
-    """Loss of one sample and d(loss)/dy."""
-    tgt = c * (e - z)
-    ce = c * e
-    den = np.sum(np.abs(ce) ** 2)
-    d = y[:, 0] + 1j * y[:, 1] - tgt
-    loss = np.sum(np.abs(d) ** 2) / den
-    gy = np.column_stack([2 * d.real, 2 * d.imag]) / den
-    return loss, gy
+def sample_residual_loss(y, r, c, A):
+    """Objective of one sample and d(objective)/dy:
+    ||A y - c p||^2 / (|c|^2 ||r||^2), p = r - A r."""
+    A = np.asarray(A, np.complex128)
+    p = r - A @ r
+    q = A @ (y[:, 0] + 1j * y[:, 1]) - c * p
+    den = abs(c) ** 2 * np.sum(np.abs(r) ** 2)
+    loss = np.sum(np.abs(q) ** 2) / den
+    g = 2 * (np.conj(A).T @ q) / den
+    return loss, np.column_stack([g.real, g.imag])
 
 
 def backward(cache, gy, nbr, sgn, deg):
@@ -204,25 +215,32 @@ def backward(cache, gy, nbr, sgn, deg):
 
 
 def loss_and_grad(theta, samples, bf16=True):
-    """Mean loss over samples and its gradient.  Each sample is a dict with
-    z, r, e (complex N), geom (N x 2), nbr, sgn, deg."""
+    """Mean objective over samples and its gradient.  Each sample is a dict
+    with z = S r, r (complex N), geom (N x 2), nbr, sgn, deg and its mesh's
+    operator A = Z S (N x N)."""
     tot, grad = 0.0, np.zeros(n_params())
     for s in samples:
         u, c = features(s["z"], s["r"], s["geom"])
         y, cache = forward(theta, u, s["nbr"], s["sgn"], s["deg"], bf16)
-        loss, gy = sample_loss_and_target(y, s["z"], s["e"], c)
+        loss, gy = sample_residual_loss(y, s["r"], c, s["A"])
         tot += loss
         grad += backward(cache, gy, s["nbr"], s["sgn"], s["deg"])
     return tot / len(samples), grad / len(samples)
 
 
 def apply(theta, z, r, geom, nbr, sgn, deg):
-    """Preconditioner output M r = z + (y0 + i y1) / c."""
+    """The network's correction d = (y0 + i y1) / c (0 when r == 0)."""
     u, c = features(z, r, geom)
     if c == 0:
-        return np.zeros_like(np.asarray(z, np.complex128))
+        return np.zeros_like(np.asarray(r, np.complex128))
     y, _ = forward(theta, u, nbr, sgn, deg)
-    return z + (y[:, 0] + 1j * y[:, 1]) / c
+    return (y[:, 0] + 1j * y[:, 1]) / c
+
+
+def precondition(theta, S, r, geom, nbr, sgn, deg):
+    """M r = S r + S d(S r, r) with S a dense matrix."""
+    z = S @ r
+    return z + S @ apply(theta, z, r, geom, nbr, sgn, deg)
 
 
 def adam_step(theta, grad, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):

@@ -1,7 +1,9 @@
 """Pins the CPU oracle of the learned preconditioner's network: the graph
 against brute-force triangle sharing, the backward pass against central
-finite differences of the forward pass, the equivariances against their
-closed forms, and Adam against its textbook recurrence."""
+finite differences of the forward pass and objective, the objective against
+its closed forms and against ||Z M r - r|| recomputed from M r, the
+equivariances against their closed forms, and Adam against its textbook
+recurrence."""
 import os
 import sys
 
@@ -55,7 +57,8 @@ def _sample(seed, nx=4, ny=3):
     N = len(r.tplus)
     cplx = lambda: rng.standard_normal(N) + 1j * rng.standard_normal(N)  # noqa: E731
     geom = NN.geometry(r.len, r.area_plus, r.area_minus, K0)
-    return dict(z=cplx(), r=cplx(), e=cplx(), geom=geom, nbr=nbr, sgn=sgn, deg=deg)
+    A = np.eye(N) + 0.3 * (rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) / np.sqrt(N)
+    return dict(z=cplx(), r=cplx(), geom=geom, nbr=nbr, sgn=sgn, deg=deg, A=A)
 
 
 def _theta(seed):
@@ -114,11 +117,11 @@ def test_apply_equivariances():
     alpha = 3.7 * np.exp(0.9j)
     scaled = NN.apply(th, alpha * s["z"], alpha * s["r"], s["geom"], s["nbr"], s["sgn"], s["deg"])
     assert np.linalg.norm(scaled - alpha * base) <= 1e-12 * np.linalg.norm(alpha * base)
-    # W_d = 0: the network adds nothing and M r = z
+    # W_d = 0: the network adds nothing
     p = NN.unflatten(th)
     p["W_d"][:] = 0
     out = NN.apply(NN.flatten(p), s["z"], s["r"], s["geom"], s["nbr"], s["sgn"], s["deg"])
-    assert np.array_equal(out, s["z"])
+    assert not np.any(out)
     # renumbering the bases permutes the output
     N = len(s["z"])
     perm = np.random.default_rng(9).permutation(N)
@@ -143,3 +146,66 @@ def test_bf16_rounding_and_adam():
     t1, m, v = NN.adam_step(th, g, np.zeros(2), np.zeros(2), 1, 0.1)
     # first Adam step moves each coordinate by lr * sign(g) (bias-corrected m/sqrt(v) = g/|g|)
     assert np.allclose(t1, th - 0.1 * np.sign(g), atol=1e-7)
+
+
+def _efie_sample(seed, nx=4, ny=3):
+    """A plate's EFIE matrix Z (the oracle's), a dense stand-in S for the
+    near-field inverse (Z^-1 perturbed), A = Z S and a random residual."""
+    m = O.make_rect_plate(0.1, 0.08, nx, ny)
+    r = O.build_rwg(m)
+    nbr, sgn, deg = NN.build_graph(r.tplus - 1, r.tminus - 1, m.nt)
+    Z = O.assemble_Z_efie(r, K0, quad_order=3)
+    rng = np.random.default_rng(seed)
+    N = len(r.tplus)
+    Zi = np.linalg.inv(Z)
+    S = Zi + 0.2 * np.linalg.norm(Zi, 2) * (rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) / N
+    res = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    geom = NN.geometry(r.len, r.area_plus, r.area_minus, K0)
+    return dict(z=S @ res, r=res, geom=geom, nbr=nbr, sgn=sgn, deg=deg, A=Z @ S, Z=Z, S=S)
+
+
+def test_objective_matches_finite_differences_on_efie_operator():
+    samples = [_efie_sample(11), _efie_sample(12, 3, 3)]
+    th = _theta(13)
+    f = lambda t: NN.loss_and_grad(t, samples, bf16=False)[0]  # noqa: E731
+    _, g = NN.loss_and_grad(th, samples, bf16=False)
+    rng = np.random.default_rng(14)
+    h = 1e-6
+    for _ in range(3):
+        d = rng.standard_normal(th.size)
+        d /= np.linalg.norm(d)
+        fd = (f(th + h * d) - f(th - h * d)) / (2 * h)
+        assert abs(fd - g @ d) <= 1e-6 * max(1.0, abs(fd)), (fd, g @ d)
+    # the decoder block coordinate by coordinate (where the objective enters)
+    off = NN.n_params() - 2 * NN.HIDDEN
+    for i in rng.integers(off, NN.n_params(), 6):
+        e = np.zeros_like(th)
+        e[i] = h
+        fd = (f(th + e) - f(th - e)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, fd, g[i])
+
+
+def test_objective_is_the_residual_one_application_leaves():
+    """The objective equals ||Z M r - r||^2 / ||r||^2 with M r recomputed from
+    its definition S r + S d; at W_d = 0 it is the SPAI's own relative
+    residual; the exact correction d = A^-1 r - r gives 0.  A planted mistake
+    (p = r - Z r instead of r - A r) changes the value."""
+    s = _efie_sample(15)
+    th = _theta(16)
+    loss, _ = NN.loss_and_grad(th, [s])  # both with the bfloat16-rounded weights
+    Mr = NN.precondition(th, s["S"], s["r"], s["geom"], s["nbr"], s["sgn"], s["deg"])
+    ref = np.linalg.norm(s["Z"] @ Mr - s["r"]) ** 2 / np.linalg.norm(s["r"]) ** 2
+    assert abs(loss - ref) <= 1e-9 * ref, (loss, ref)
+    p = NN.unflatten(th)
+    p["W_d"][:] = 0
+    loss0, _ = NN.loss_and_grad(NN.flatten(p), [s], bf16=False)
+    ref0 = np.linalg.norm(s["r"] - s["Z"] @ s["z"]) ** 2 / np.linalg.norm(s["r"]) ** 2
+    assert abs(loss0 - ref0) <= 1e-12 * ref0
+    _, c = NN.features(s["z"], s["r"], s["geom"])
+    d = np.linalg.solve(s["A"], s["r"]) - s["r"]
+    y = c * d
+    l_exact, g_exact = NN.sample_residual_loss(np.column_stack([y.real, y.imag]), s["r"], c, s["A"])
+    assert l_exact <= 1e-20 and np.linalg.norm(g_exact) <= 1e-8
+    bad = dict(s, A=s["Z"])
+    l_bad, _ = NN.loss_and_grad(NN.flatten(p), [bad], bf16=False)
+    assert abs(l_bad - ref0) > 1e-3 * ref0
