@@ -286,6 +286,45 @@ def kernel_probe(dm, subdiv, what):
         print(f"Z X product {i} done", flush=True)
 
 
+def graph_probe(dm):
+    """Krylov steps launched kernel by kernel vs replayed from a captured HIP
+    graph: FP64 GMRES (right SPAI, tol 1e-8) on plates of 8..60 cells per side
+    (0.5 m at 60 cells), then the chosen mixed form on the 30720-basis sphere;
+    best of 5 / 3 solves, one line each."""
+    k = 2 * math.pi / 0.1
+
+    def best(f, reps):
+        ts, out = [], None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out = f()
+            ts.append(time.perf_counter() - t0)
+        return min(ts), out
+
+    for nx in (8, 16, 32, 60):
+        m = dm.make_rect_plate(0.5 * nx / 60, 0.5 * nx / 60, nx, nx)
+        r = dm.build_rwg(m)
+        Z = dm.assemble_Z_efie(m, r, k, forms=dm.DM_Z64P, device=True, max_output_bytes=None)
+        v = dm.assemble_v_plane_wave(m, r, [0, 0, -k], 1.0, [1, 0, 0])
+        P = dm.build_nearfield_preconditioner(None, m, r, 2.0 * float(np.mean(r.len)))
+        row = []
+        for g in (False, True):
+            t, (_, st) = best(lambda: dm.solve_gmres(Z, v, preconditioner=P, precond_side="right", tol=1e-8,
+                                                     maxiter=1000, graph=g), 5)
+            row.append(f"{'graph' if g else 'launched'} {1e3 * t:.2f} ms ({st.niter} it, {1e6 * t / st.niter:.1f} us/it)")
+        print(f"plate N={r.nedges}: " + "  ".join(row), flush=True)
+        Z.free()
+    m = dm.make_icosphere(0.1, 5)
+    r = dm.build_rwg(m)
+    P = dm.build_nearfield_preconditioner(None, m, r, 2.0 * float(np.mean(r.len)))
+    pw = [dm.make_plane_wave([0, 0, -k], 1.0, [1, 0, 0])]
+    for g in (False, True):
+        t, (_, st, _) = best(lambda: dm.solve_planewaves(m, r, k, pw, "mixed", P, tol=1e-6, maxiter=1000, batch=-1,
+                                                         compress_tol=5e-8, graph=g), 3)
+        print(f"sphere N=30720 chosen form, {'graph' if g else 'launched'}: {1e3 * t:.2f} ms, {st.niter} it",
+              flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -301,6 +340,8 @@ def main():
     ap.add_argument("--compress-tol", type=float, default=5e-8,
                     help="chosen form: cross-approximate the complex64 matrix to this relative Frobenius error "
                          "(0 = dense complex64)")
+    ap.add_argument("--graph-probe", action="store_true",
+                    help="time Krylov steps launched vs replayed from a captured HIP graph, then exit")
     ap.add_argument("--kernel-probe", choices=["assembly", "zx"], default=None,
                     help="no timing: dispatches of one kernel kind for rocprofv3 --pmc runs")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -318,6 +359,8 @@ def main():
         return traffic_probe(dm, a.subdiv)
     if a.kernel_probe:
         return kernel_probe(dm, a.subdiv, a.kernel_probe)
+    if a.graph_probe:
+        return graph_probe(dm)
     k = 2 * math.pi / 0.1
     mesh = dm.make_icosphere(0.1, a.subdiv)
     rwg = dm.build_rwg(mesh)

@@ -115,7 +115,7 @@ class dm_solve_opts(C.Structure):
                 ("tol", C.c_double), ("nf_cutoff", C.c_double), ("restart", C.c_int),
                 ("check_convergence", C.c_int), ("inner_tol", C.c_double), ("max_outer", C.c_int),
                 ("block_size", C.c_int), ("batch", C.c_int), ("learned", C.c_void_p), ("adjoint", C.c_int),
-                ("compress_tol", C.c_double), ("storage", C.c_int)]
+                ("compress_tol", C.c_double), ("storage", C.c_int), ("graph", C.c_int)]
 
 
 class dm_solve_stats(C.Structure):
@@ -771,7 +771,7 @@ class SolveStats:
 
 
 def _solve(Z, B, method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, check=True,
-           inner_tol=0.0, max_outer=10, batch=0, adjoint=False):
+           inner_tol=0.0, max_outer=10, batch=0, adjoint=False, graph=False):
     if not isinstance(Z, ZMatrix):
         raise TypeError("GPU solves take the device matrix (assemble_Z_efie(..., device=True))")
     B = np.asarray(B, dtype=np.complex128)
@@ -780,7 +780,7 @@ def _solve(Z, B, method, precond=None, side="right", tol=1e-6, maxiter=300, rest
     if Bf.shape[0] != Z.N:
         raise DimensionMismatch(f"rhs length {Bf.shape[0]} != {Z.N}")
     X = np.zeros_like(Bf, order="F")
-    o = _opts(method, precond, side, tol, maxiter, restart, check, inner_tol, max_outer, batch)
+    o = _opts(method, precond, side, tol, maxiter, restart, check, inner_tol, max_outer, batch, graph=graph)
     o.adjoint = int(adjoint)
     st = dm_solve_stats()
     _check(lib().dm_solve(Z.ctx.h, Z.h, Bf.ctypes.data_as(C.c_void_p), Z.N, Bf.shape[1], C.byref(o),
@@ -791,12 +791,14 @@ def _solve(Z, B, method, precond=None, side="right", tol=1e-6, maxiter=300, rest
 
 
 def _opts(method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, check=True, inner_tol=0.0,
-          max_outer=10, batch=0, compress_tol=0.0, storage="packed"):
+          max_outer=10, batch=0, compress_tol=0.0, storage="packed", graph=False):
     """dm_solve_opts.  batch: 0 = lockstep batches of right-hand sides when there
     are >= 8 of them, 1 = always, -1 = one right-hand side at a time.
     compress_tol > 0: the mixed solve compresses its complex64 matrix.
     storage: "packed" (symmetric tiles, half the bytes) or "full" for the
-    one-column FP64 GMRES path of solve_planewaves."""
+    one-column FP64 GMRES path of solve_planewaves.  graph: one right-hand
+    side at a time, each Krylov step after the first of a cycle replays a
+    captured HIP graph."""
     if storage not in ("packed", "full"):
         raise ValueError(f"storage must be 'packed' or 'full', got {storage!r}")
     o = dm_solve_opts()
@@ -804,6 +806,7 @@ def _opts(method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, 
     o.method, o.tol, o.maxiter, o.restart = method, float(tol), int(maxiter), int(restart)
     o.batch = int(batch)
     o.storage = 0 if storage == "packed" else 1
+    o.graph = 1 if graph else 0
     o.compress_tol = float(compress_tol)
     o.check_convergence, o.inner_tol, o.max_outer = int(check), float(inner_tol), int(max_outer)
     o.precond_side = 1 if side == "right" else 0
@@ -828,7 +831,7 @@ def _opts(method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, 
 
 def solve_planewaves(mesh, rwg, k, excitations, method="mixed", preconditioner=None, tol=1e-6, maxiter=300,
                      quad_order=3, eta0=ETA0, check_convergence=True, keep_matrix=False, batch=0, restart=0,
-                     compress_tol=0.0, storage="packed"):
+                     compress_tol=0.0, storage="packed", graph=False):
     """End-to-end device solve for plane waves (the dense branches of
     solve_scattering, src/Workflow.jl:67-317): assembly, excitation,
     preconditioner and Krylov solve in one call.  method: "direct" (dense LU),
@@ -839,7 +842,8 @@ def solve_planewaves(mesh, rwg, k, excitations, method="mixed", preconditioner=N
     (ZMatrix.compress) and the inner products use the compressed form.
     storage ("fp64", one wave at a time): "packed" keeps the upper triangle of
     128 x 128 tiles (Z = Z^T exactly; each product reads half the bytes),
-    "full" the N x N matrix as the reference stores it.
+    "full" the N x N matrix as the reference stores it.  graph (one wave at
+    a time): Krylov steps replay a captured HIP graph.
     Returns (X, stats, assembly_time_s[, ZMatrix])."""
     _check_pair(mesh, rwg)
     M = len(excitations)
@@ -848,7 +852,7 @@ def solve_planewaves(mesh, rwg, k, excitations, method="mixed", preconditioner=N
     E0 = np.ascontiguousarray([float(e.E0) for e in excitations])
     meth = {"direct": DM_METHOD_LU, "fp64": DM_METHOD_GMRES, "mixed": DM_METHOD_MIXED}[method]
     o = _opts(meth, preconditioner, "right", tol, maxiter, restart=restart, check=check_convergence, batch=batch,
-              compress_tol=compress_tol, storage=storage)
+              compress_tol=compress_tol, storage=storage, graph=graph)
     X = np.zeros((rwg.nedges, M), dtype=np.complex128, order="F")
     st = dm_solve_stats()
     t_asm = C.c_double()
@@ -882,10 +886,11 @@ def solve_forward(Z, v, solver="direct", **kw):
 
 def solve_gmres(Z, rhs, preconditioner=None, precond_side="left", tol=1e-8, maxiter=200, memory=20,
                 max_workspace_bytes=_DEFAULT_MAX_GMRES_WORKSPACE_BYTES, verbose=False,
-                check_gmres_convergence=True, restart=0, batch=0):
+                check_gmres_convergence=True, restart=0, batch=0, graph=False):
     """src/solver/IterativeSolve.jl:341-392 (full GMRES unless restart > 0).
     Several right-hand sides with right or no preconditioning run as lockstep
-    batches (one Z X product per step) unless batch=-1."""
+    batches (one Z X product per step) unless batch=-1.  graph: one
+    right-hand side at a time, Krylov steps replay a captured HIP graph."""
     if not (math.isfinite(tol) and tol > 0):
         raise ValueError(f"GMRES tol must be finite and positive, got {tol}")
     if maxiter < 1:
@@ -896,7 +901,7 @@ def solve_gmres(Z, rhs, preconditioner=None, precond_side="left", tol=1e-8, maxi
         raise ValueError(f"Invalid precond_side: {precond_side}")
     _gmres_workspace_gate(Z.N, memory, max_workspace_bytes)
     return _solve(Z, rhs, DM_METHOD_GMRES, preconditioner, precond_side, tol, maxiter, restart,
-                  check_gmres_convergence, batch=batch)
+                  check_gmres_convergence, batch=batch, graph=graph)
 
 
 def _gmres_workspace_gate(N, memory, max_workspace_bytes):
@@ -928,12 +933,12 @@ def solve_gmres_adjoint(Z, rhs, preconditioner=None, precond_side="left", tol=1e
 
 
 def solve_mixed(Z, rhs, preconditioner=None, tol=1e-6, maxiter=300, inner_tol=0.0, max_outer=10,
-                check_convergence=True, restart=0, batch=0):
+                check_convergence=True, restart=0, batch=0, graph=False):
     """Mixed precision: FGMRES on the complex64 matrix with FP64 Arnoldi, FP64
     iterative refinement on the complex128 matrix (no reference counterpart).
     Several right-hand sides run as lockstep batches (Z X on MFMA) unless batch=-1."""
     return _solve(Z, rhs, DM_METHOD_MIXED, preconditioner, "right", tol, maxiter, restart, check_convergence,
-                  inner_tol, max_outer, batch)
+                  inner_tol, max_outer, batch, graph=graph)
 
 
 # --------------------------------------------------------------- far field

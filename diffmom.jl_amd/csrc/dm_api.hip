@@ -229,6 +229,21 @@ i64 round_ld(i64 n) { return (n + 63) / 64 * 64; }
 
 constexpr int kAllForms = DM_Z64 | DM_Z32 | DM_Z64P | DM_Z32P;
 
+// FP64 tile launches with the tuned threads per workgroup: key
+// asm/<form>/q<quadrature points>/<n>, form z64 / z64p, "+z32" when the same
+// launch also writes the complex64 rounding.
+void asm_tuned(dm_ctx* ctx, const AsmArgs& a, double* z, int cap, i64 n, bool packed, hipStream_t s) {
+  const std::string key = std::string("asm/") + (packed ? "z64p" : "z64") + (a.z32_dual ? "+z32" : "") + "/q" +
+                          std::to_string(a.nq) + "/" + std::to_string(n);
+  auto run = [&](int t) {
+    if (packed)
+      launch_assemble_z64p(a, z, cap, s, t);
+    else
+      launch_assemble_z64(a, z, cap, s, t);
+  };
+  run(dmt::tune_choice(ctx, key, {1024, 512}, 1024, run, s));
+}
+
 // Packed symmetric forms requested in `forms` (storage zeroed first: entries
 // past n stay zero).
 void assemble_packed(dm_zmat* z, const AsmArgs& a, const dm_rwg& r, int forms, hipStream_t s) {
@@ -236,7 +251,7 @@ void assemble_packed(dm_zmat* z, const AsmArgs& a, const dm_rwg& r, int forms, h
     z->z64p.alloc(packed_scalars(z->n));
     zero_packed_padding(z->n, z->z64p.p, sizeof(double), s);
     ProfScope p(z->ctx, "assemble_z64", s);
-    launch_assemble_z64p(a, z->z64p.p, r.max_blk_tris, s);
+    asm_tuned(z->ctx, a, z->z64p.p, r.max_blk_tris, z->n, true, s);
   }
   if (forms & DM_Z32P) {
     z->z32p.alloc(packed_scalars(z->n));
@@ -395,6 +410,7 @@ void validate_opts(const dm_solve_opts& o) {
   DM_REQUIRE(std::isfinite(o.compress_tol) && o.compress_tol >= 0 && o.compress_tol < 1, DM_ERR_ARGUMENT,
              "compress_tol must be in [0, 1)");
   DM_REQUIRE(o.storage == 0 || o.storage == 1, DM_ERR_ARGUMENT, "storage must be 0 (packed) or 1 (full)");
+  DM_REQUIRE(o.graph == 0 || o.graph == 1, DM_ERR_ARGUMENT, "graph must be 0 or 1");
 }
 
 void check_ctx(const void* a, const char* what) { DM_REQUIRE(a != nullptr, DM_ERR_ARGUMENT, std::string(what) + " is NULL"); }
@@ -668,7 +684,7 @@ int dm_assemble_z(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double k_im, do
       z->z64.alloc((size_t)z->ld * n * 2);
       ProfScope p(ctx, "assemble_z64", s);
       a.z32_dual = rounded && (forms & DM_Z32) ? z->z32.p : nullptr;
-      launch_assemble_z64(a, z->z64.p, r.max_blk_tris, s);
+      asm_tuned(ctx, a, z->z64.p, r.max_blk_tris, n, false, s);
       a.z32_dual = nullptr;
     }
     if ((forms & DM_Z32) && !rounded) {
@@ -682,7 +698,7 @@ int dm_assemble_z(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double k_im, do
       zero_packed_padding(n, z->z32p.p, sizeof(float), s);
       ProfScope p(ctx, "assemble_z64_z32", s);
       a.z32_dual = z->z32p.p;
-      launch_assemble_z64p(a, z->z64p.p, r.max_blk_tris, s);
+      asm_tuned(ctx, a, z->z64p.p, r.max_blk_tris, n, true, s);
       a.z32_dual = nullptr;
     } else {
       assemble_packed(z.get(), a, r, forms, s);
@@ -1210,6 +1226,7 @@ void dm_solve_opts_default(dm_solve_opts* o) {
   o->learned = nullptr;
   o->adjoint = 0;
   o->storage = 0;
+  o->graph = 0;
 }
 
 int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double k_im, double eta0, int quad_order,
@@ -1277,25 +1294,25 @@ int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double
         z->z64.alloc((size_t)z->ld * n * 2);
         z->z32.alloc((size_t)z->ld * n * 2);
         a.z32_dual = z->z32.p;
-        launch_assemble_z64(a, z->z64.p, r.max_blk_tris, s);
+        asm_tuned(ctx, a, z->z64.p, r.max_blk_tris, n, false, s);
       } else {
         z->z64p.alloc(packed_scalars(n));
         z->z32p.alloc(packed_scalars(n));
         zero_packed_padding(n, z->z64p.p, sizeof(double), s);
         zero_packed_padding(n, z->z32p.p, sizeof(float), s);
         a.z32_dual = z->z32p.p;
-        launch_assemble_z64p(a, z->z64p.p, r.max_blk_tris, s);
+        asm_tuned(ctx, a, z->z64p.p, r.max_blk_tris, n, true, s);
       }
       a.z32_dual = nullptr;
     } else if (z->forms & DM_Z64P) {
       z->z64p.alloc(packed_scalars(n));
       zero_packed_padding(n, z->z64p.p, sizeof(double), s);
       ProfScope p(ctx, "assemble_z64", s);
-      launch_assemble_z64p(a, z->z64p.p, r.max_blk_tris, s);
+      asm_tuned(ctx, a, z->z64p.p, r.max_blk_tris, n, true, s);
     } else {
       z->z64.alloc((size_t)z->ld * n * 2);
       ProfScope p(ctx, "assemble_z64", s);
-      launch_assemble_z64(a, z->z64.p, r.max_blk_tris, s);
+      asm_tuned(ctx, a, z->z64.p, r.max_blk_tris, n, false, s);
     }
     // compression of the complex64 matrix on the second stream, overlapped
     // with the preconditioner build (both only read the packed matrix)

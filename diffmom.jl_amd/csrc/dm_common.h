@@ -373,11 +373,12 @@ void launch_near_tables_multi(const std::vector<AsmArgs>& a, const std::vector<c
 // Bytes of one block's tile-side image and the kernel that fills nblk of them.
 size_t side_image_bytes(int nq, bool f64, int cap);
 void launch_build_sides(const AsmArgs& a, int cap, bool f64, char* img, hipStream_t s);
-void launch_assemble_z64(const AsmArgs& a, double* z, int max_tris, hipStream_t s);
+// nthr: threads per workgroup of the FP64 tile kernel (0 = 1024, or 512)
+void launch_assemble_z64(const AsmArgs& a, double* z, int max_tris, hipStream_t s, int nthr = 0);
 void launch_assemble_z32(const AsmArgs& a, float* z, int max_tris, hipStream_t s);
 // packed symmetric storage (buffer of packed_scalars(n); entries past n must be
 // zeroed beforehand: zero_packed_padding)
-void launch_assemble_z64p(const AsmArgs& a, double* z, int max_tris, hipStream_t s);
+void launch_assemble_z64p(const AsmArgs& a, double* z, int max_tris, hipStream_t s, int nthr = 0);
 void launch_assemble_z32p(const AsmArgs& a, float* z, int max_tris, hipStream_t s);
 // Zero the entries of packed storage that lie past n (only the last tile
 // column holds any); every other entry is written by the assembly.

@@ -3,11 +3,22 @@
 // products) or a lockstep batch of right-hand sides (Z X products on MFMA).
 #pragma once
 #include <complex>
+#include <functional>
+#include <string>
 #include <memory>
 #include <vector>
 
 #include "dm_common.h"
 #include "k_linalg.h"
+
+namespace dmt {
+// The fastest of cands for key: timed once per context with run(c) on stream
+// s (one warm-up and three timed runs each), then kept in the context and
+// exported by dm_tune_get; DM_TUNE=0 returns fallback without timing.  run
+// must leave the same results whichever candidate it is given.
+int tune_choice(dm_ctx* ctx, const std::string& key, const std::vector<int>& cands, int fallback,
+                const std::function<void(int)>& run, hipStream_t s);
+}  // namespace dmt
 
 namespace dms {
 
@@ -66,8 +77,12 @@ struct OpZ32 : DevOp {
       planes.alloc(dml::zx_bf16_planes_elems(z->ld, z->n) * 2);
       dml::zx_bf16_split(z->n, z->ld, z->z32.p, planes.p, s);
     }
+    // FP32 accumulation chunk: key zx/c64/<n>/<R>
+    const int kc = dmt::tune_choice(
+        z->ctx, "zx/c64/" + std::to_string(z->n) + "/" + std::to_string(R), dml::zx_bf16_chunks(), 4096,
+        [&](int c) { dml::zx_bf16(z->n, z->ld, planes.p, X, z->n, R, Y, z->n, ws, s, c); }, s);
     ProfScope p(z->ctx, "zx_z32", s);
-    dml::zx_bf16(z->n, z->ld, planes.p, X, z->n, R, Y, z->n, ws, s);
+    dml::zx_bf16(z->n, z->ld, planes.p, X, z->n, R, Y, z->n, ws, s, kc);
   }
   size_t batch_ws(int R) const override {
     return dml::zx_bf16_workspace_bytes(z->n, R);
@@ -168,10 +183,31 @@ std::unique_ptr<DevPrec> build_precond(const dm_zmat* z, int form, const dm_solv
 std::unique_ptr<DevPrec> make_learned_precond(const dm_zmat* z, int form, const dm_solve_opts& o, hipStream_t s);
 
 // ------------------------------------------------------------------- GMRES
+// One captured Krylov step: the graph executable for steps with k <= kb,
+// valid for the operator, preconditioner and buffers it was captured with.
+struct StepGraph {
+  int kb = 0;
+  hipGraphExec_t exec = nullptr;
+};
+
 struct Krylov {
   i64 n = 0;
   int m = 0;  // basis capacity
   DBuf<double> V, Zs, w, t, hbuf, ybuf, part, nrm;  // nrm: partials of ||w||^2
+  // captured steps: the step index on the device, the step's input vector
+  // (v_j) and preconditioned vector, the graphs and what they were captured for
+  DBuf<int> jd;
+  DBuf<double> vcur, zcur;
+  int jhost = 0;
+  std::vector<StepGraph> graphs;
+  std::vector<const void*> graph_key;
+  ~Krylov() { drop_graphs(); }
+  void drop_graphs() {
+    for (auto& g : graphs)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+    graph_key.clear();
+  }
   // device Givens state: H ((m + 1) x m complex), rotations, rotated rhs,
   // per-step (estimate, subdiagonal), 1 / subdiagonal
   DBuf<double> Hd, csd, snd, gd, stt, scd;
@@ -180,6 +216,7 @@ struct Krylov {
   // so the refinement loop, whose remaining iteration budget shrinks at every
   // step, reuses the first step's buffers instead of reallocating them.
   void reserve(i64 n_, int m_, bool flexible) {
+    if (n_ != n) drop_graphs();
     if (n_ != n) {
       V.release();
       Zs.release();
@@ -204,6 +241,9 @@ struct Krylov {
     grow(t, 2 * n);
     grow(hbuf, 2 * (2 * (m + 1) + 1));
     grow(ybuf, 2 * (m + 1));
+    if (!jd.p) jd.alloc(1);
+    grow(vcur, 2 * n);
+    grow(zcur, 2 * n);
   }
   static void grow(DBuf<double>& b, size_t count) {
     if (b.n < count) b.alloc(count);
@@ -217,8 +257,9 @@ struct GmresOut {
 };
 
 // x_zero: x is known to be zero on entry (the first residual is then b).
+// graph: steps after the first of each cycle replay a captured HIP graph.
 GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double* b, double* x, double rtol,
-               int maxiter, int restart, Krylov& K, hipStream_t s, bool x_zero = false);
+               int maxiter, int restart, Krylov& K, hipStream_t s, bool x_zero = false, bool graph = false);
 double true_residual(DevOp& A, i64 n, const double* b, const double* x, double* tmp, double* hbuf, hipStream_t s);
 double solve_one(dm_ctx* ctx, const dm_solve_opts& o, DevOp* A64, DevOp* A32, DevPrec* M, const double* b,
                  double* x, i64 n, Krylov& K, dm_solve_stats& st, bool& ok, hipEvent_t z64_ready);

@@ -82,13 +82,89 @@ std::unique_ptr<DevPrec> build_precond(const dm_zmat* z, int form, const dm_solv
   dm_throw(DM_ERR_ARGUMENT, "preconditioner kind not available for this solve");
 }
 
+namespace {
+
+// Ends a stream capture that an exception interrupts.
+struct CaptureGuard {
+  hipStream_t s;
+  bool active = true;
+  ~CaptureGuard() {
+    if (!active) return;
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+  }
+};
+
+// The graph of one Krylov step for k = j + 1 <= kb, captured on first use:
+// preconditioner and product on K.vcur (v_j), the preconditioned vector also
+// copied into column j of K.Zs, two CGS passes and the Givens step (which
+// writes v_{j+1} to V and K.vcur), then j += 1, all with j read from K.jd.
+hipGraphExec_t step_graph(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, int mcap, int kb, Krylov& K,
+                          hipStream_t s) {
+  const std::vector<const void*> key = {&A, M, (const void*)(intptr_t)side, (const void*)(intptr_t)mcap,
+                                        K.V.p, K.Zs.p, K.w.p, K.t.p, K.part.p, K.hbuf.p, K.Hd.p, K.gd.p};
+  if (key != K.graph_key) {
+    K.drop_graphs();
+    K.graph_key = key;
+  }
+  for (auto& g : K.graphs)
+    if (g.kb == kb) return g.exec;
+  const bool right = M && side == 1, left = M && side == 0;
+  const i64 ldv = n;  // complex columns
+  const bool prof = ctx->prof;
+  ctx->prof = false;  // no profiling events inside the graph
+  hipGraph_t graph = nullptr;
+  {
+    HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    CaptureGuard guard{s};
+    try {
+      if (right) {
+        M->apply(K.vcur.p, K.zcur.p, s);
+        dml::copy_col_dev(n, K.zcur.p, K.Zs.p, ldv, K.jd.p, s);
+        A.apply(K.zcur.p, K.w.p, s);
+      } else if (left) {
+        A.apply(K.vcur.p, K.t.p, s);
+        M->apply(K.t.p, K.w.p, s);
+      } else {
+        A.apply(K.vcur.p, K.w.p, s);
+      }
+      dml::dots_dev(n, ldv, kb, K.jd.p, K.V.p, K.w.p, K.part.p, s);
+      dml::update_dev(n, ldv, K.jd.p, 0, K.V.p, K.part.p, K.hbuf.p, K.w.p, nullptr, s);
+      dml::dots_dev(n, ldv, kb, K.jd.p, K.V.p, K.w.p, K.part.p, s);
+      dml::update_dev(n, ldv, K.jd.p, 1, K.V.p, K.part.p, K.hbuf.p, K.w.p, K.nrm.p, s);
+      dml::givens_dev(kb - 1, K.jd.p, mcap + 1, K.hbuf.p, K.nrm.p, dml::norm_part_count(n), n, K.w.p, K.V.p, ldv,
+                      K.vcur.p, K.Hd.p, K.csd.p, K.snd.p, K.gd.p, K.stt.p, K.scd.p, s);
+      dml::step_inc(K.jd.p, s);
+    } catch (...) {
+      ctx->prof = prof;
+      throw;
+    }
+    guard.active = false;
+    HIP_OK(hipStreamEndCapture(s, &graph));
+  }
+  ctx->prof = prof;
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIP_OK(e);
+  K.graphs.push_back({kb, exec});
+  return exec;
+}
+
+}  // namespace
+
 // Restarted (restart = 0: full) GMRES with classical Gram-Schmidt applied
 // twice (CGS2) for the Arnoldi step, FP64 throughout; right preconditioning is
 // flexible (FGMRES: the preconditioned directions are stored), left
 // preconditioning follows Krylov.jl's semantics (stop on ||M r|| <= rtol ||M b||).
-// x holds the initial guess on entry.
+// x holds the initial guess on entry.  graph: the first step of each cycle
+// runs its kernels directly (which also finishes any set-up the operator and
+// preconditioner do on first use); the later steps replay a captured graph
+// per step, the graph chosen by the bucket kb (a power of two >= 8, at most
+// m + 1) that bounds k = j + 1, so the CGS grids grow with the basis.
 GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double* b, double* x, double rtol,
-               int maxiter, int restart, Krylov& K, hipStream_t s, bool x_zero) {
+               int maxiter, int restart, Krylov& K, hipStream_t s, bool x_zero, bool graph) {
   GmresOut out;
   int mcap = restart > 0 ? std::min(restart, maxiter) : maxiter;
   bool right = M && side == 1;
@@ -147,36 +223,53 @@ GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double*
     est = beta;
     while (j < mcap && total < maxiter) {
       double* vj = col(V, j);
-      if (right) {
-        double* zj = col(K.Zs.p, j);
-        {
-          ProfScope p(ctx, "precond", s);
-          M->apply(vj, zj, s);
+      if (graph && j >= 1) {
+        if (j == 1) {  // the replayed steps of this cycle start at j = 1, v_1
+          K.jhost = 1;
+          HIP_OK(hipMemcpyAsync(K.jd.p, &K.jhost, sizeof(int), hipMemcpyHostToDevice, s));
+          HIP_OK(hipMemcpyAsync(K.vcur.p, vj, 16 * n, hipMemcpyDeviceToDevice, s));
         }
-        A.apply(zj, K.w.p, s);
-      } else if (left) {
-        A.apply(vj, K.t.p, s);
-        ProfScope p(ctx, "precond", s);
-        M->apply(K.t.p, K.w.p, s);
+        int kb = 8;
+        while (kb < j + 1) kb *= 2;
+        kb = std::min(kb, mcap + 1);
+        hipGraphExec_t ex = step_graph(ctx, n, A, M, side, mcap, kb, K, s);
+        ProfScope p(ctx, "step_graph", s);
+        HIP_OK(hipGraphLaunch(ex, s));
+        ++j;
+        ++total;
+        if (j - seen < step && j < mcap && total < maxiter) continue;
       } else {
-        A.apply(vj, K.w.p, s);
+        if (right) {
+          double* zj = col(K.Zs.p, j);
+          {
+            ProfScope p(ctx, "precond", s);
+            M->apply(vj, zj, s);
+          }
+          A.apply(zj, K.w.p, s);
+        } else if (left) {
+          A.apply(vj, K.t.p, s);
+          ProfScope p(ctx, "precond", s);
+          M->apply(K.t.p, K.w.p, s);
+        } else {
+          A.apply(vj, K.w.p, s);
+        }
+        const int k = j + 1;
+        {
+          ProfScope p(ctx, "arnoldi", s);
+          double* h1 = K.hbuf.p;
+          double* h2 = K.hbuf.p + 2 * k;
+          dml::dots(n, ldv / 2, k, V, K.w.p, K.part.p, s);
+          dml::update(n, ldv / 2, k, V, K.part.p, h1, K.w.p, nullptr, s);
+          dml::dots(n, ldv / 2, k, V, K.w.p, K.part.p, s);
+          dml::update(n, ldv / 2, k, V, K.part.p, h2, K.w.p, K.nrm.p, s);
+          // rotations, and v_{j+1} = w / ||w|| in the same launch
+          dml::givens(j, mcap + 1, K.hbuf.p, K.nrm.p, dml::norm_part_count(n), n, K.w.p, col(V, k), K.Hd.p, K.csd.p,
+                      K.snd.p, K.gd.p, K.stt.p, K.scd.p, s);
+        }
+        ++j;
+        ++total;
+        if (j - seen < step && j < mcap && total < maxiter) continue;
       }
-      const int k = j + 1;
-      {
-        ProfScope p(ctx, "arnoldi", s);
-        double* h1 = K.hbuf.p;
-        double* h2 = K.hbuf.p + 2 * k;
-        dml::dots(n, ldv / 2, k, V, K.w.p, K.part.p, s);
-        dml::update(n, ldv / 2, k, V, K.part.p, h1, K.w.p, nullptr, s);
-        dml::dots(n, ldv / 2, k, V, K.w.p, K.part.p, s);
-        dml::update(n, ldv / 2, k, V, K.part.p, h2, K.w.p, K.nrm.p, s);
-        // rotations, and v_{j+1} = w / ||w|| in the same launch
-        dml::givens(j, mcap + 1, K.hbuf.p, K.nrm.p, dml::norm_part_count(n), n, K.w.p, col(V, k), K.Hd.p, K.csd.p,
-                    K.snd.p, K.gd.p, K.stt.p, K.scd.p, s);
-      }
-      ++j;
-      ++total;
-      if (j - seen < step && j < mcap && total < maxiter) continue;
       HIP_OK(hipMemcpyAsync(sth.data() + 2 * seen, K.stt.p + 2 * seen, sizeof(double) * 2 * (j - seen),
                             hipMemcpyDeviceToHost, s));
       HIP_OK(hipStreamSynchronize(s));
@@ -254,7 +347,7 @@ double solve_one(dm_ctx* ctx, const dm_solve_opts& o, DevOp* A64, DevOp* A32, De
   if (o.method == DM_METHOD_GMRES) {
     DevOp& A = A64 ? *A64 : *A32;
     if (A64 && z64_ready) HIP_OK(hipStreamWaitEvent(s, z64_ready, 0));
-    GmresOut g = gmres(ctx, n, A, M, o.precond_side, b, x, o.tol, o.maxiter, o.restart, K, s, true);
+    GmresOut g = gmres(ctx, n, A, M, o.precond_side, b, x, o.tol, o.maxiter, o.restart, K, s, true, o.graph == 1);
     st.iterations += g.iters;
     ok = g.solved;
     return true_residual(A, n, b, x, tmp.p, hb.p, s);
@@ -269,7 +362,7 @@ double solve_one(dm_ctx* ctx, const dm_solve_opts& o, DevOp* A64, DevOp* A32, De
   bool waited = false;
   for (int outer = 0; outer < std::max(1, o.max_outer) && left > 0; ++outer) {
     HIP_OK(hipMemsetAsync(d.p, 0, 16 * n, s));
-    GmresOut g = gmres(ctx, n, *A32, M, 1, rr.p, d.p, tau, left, o.restart, K, s, true);
+    GmresOut g = gmres(ctx, n, *A32, M, 1, rr.p, d.p, tau, left, o.restart, K, s, true, o.graph == 1);
     st.iterations += g.iters;
     left -= g.iters;
     st.outer_iterations += 1;

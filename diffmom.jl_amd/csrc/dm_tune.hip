@@ -7,7 +7,10 @@
 // it.  Results can be exported to / imported from a flat JSON object so a
 // caller can persist them across processes (dm_tune_get / dm_tune_set).
 // DM_TUNE=0 in the environment disables timing and uses the defaults.
+#include <algorithm>
 #include <cstdlib>
+#include <functional>
+#include <vector>
 #include <cstring>
 #include <sstream>
 #include <string>
@@ -41,6 +44,28 @@ double time_ms(Launch&& run, hipStream_t s) {
 }
 
 }  // namespace
+
+int tune_choice(dm_ctx* ctx, const std::string& key, const std::vector<int>& cands, int fallback,
+                const std::function<void(int)>& run, hipStream_t s) {
+  auto it = ctx->tune.find(key);
+  if (it != ctx->tune.end()) {
+    DM_REQUIRE(std::find(cands.begin(), cands.end(), it->second) != cands.end(), DM_ERR_ARGUMENT,
+               "tuned value out of range for " + key);
+    return it->second;
+  }
+  if (!tuning_enabled()) return fallback;
+  int best = fallback;
+  double best_ms = 1e300;
+  for (int c : cands) {
+    const double ms = time_ms([&] { run(c); }, s);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  ctx->tune[key] = best;
+  return best;
+}
 
 namespace {
 std::string symv_key(const dm_zmat* z, int form) {

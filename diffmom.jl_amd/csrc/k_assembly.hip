@@ -866,11 +866,8 @@ size_t tile_smem(int nq, int TB, int cap) {
   return b;
 }
 
-template <typename T, typename AccT, int MODE>
-void launch_tile(const AsmArgs& a, int cap, AccT* z, const double* x, double* y, hipStream_t s) {
-  // FP64: 1024 threads (16 waves, <= 128 VGPRs each) fill the CU that the
-  // ~98 KB of LDS per tile limits to one workgroup
-  constexpr int NTHR = sizeof(T) == 8 ? 1024 : 512;
+template <typename T, typename AccT, int MODE, int NTHR>
+void launch_tile_n(const AsmArgs& a, int cap, AccT* z, const double* x, double* y, hipStream_t s) {
   size_t smem = tile_smem<T, AccT>(a.nq, a.TB, cap);
   DM_REQUIRE(smem <= 160 * 1024, DM_ERR_ARGUMENT, "tile working set exceeds LDS");
   i64 T_ = (i64)a.nblk * (a.nblk + 1) / 2;
@@ -892,6 +889,20 @@ void launch_tile(const AsmArgs& a, int cap, AccT* z, const double* x, double* y,
       dm_throw(DM_ERR_ARGUMENT, "unsupported quadrature size");
   }
   HIP_OK(hipGetLastError());
+}
+
+// FP64: 1024 threads (16 waves, <= 128 VGPRs each) fill the CU that the ~98 KB
+// of LDS per tile limits to one workgroup; 512 is the tuner's other candidate.
+// FP32: 512.
+template <typename T, typename AccT, int MODE>
+void launch_tile(const AsmArgs& a, int cap, AccT* z, const double* x, double* y, hipStream_t s, int nthr = 0) {
+  if (sizeof(T) == 8 && nthr != 512) {
+    DM_REQUIRE(nthr == 0 || nthr == 1024, DM_ERR_ARGUMENT, "FP64 tile threads must be 512 or 1024");
+    launch_tile_n<T, AccT, MODE, 1024>(a, cap, z, x, y, s);
+  } else {
+    DM_REQUIRE(nthr == 0 || nthr == 512, DM_ERR_ARGUMENT, "FP32 tile threads must be 512");
+    launch_tile_n<T, AccT, MODE, 512>(a, cap, z, x, y, s);
+  }
 }
 
 template <typename T, typename AccT, int MODE>
@@ -1002,15 +1013,15 @@ void zero_packed_padding(i64 n, void* z, size_t scalar_bytes, hipStream_t s) {
   HIP_OK(hipMemsetAsync(static_cast<char*>(z) + first, 0, (size_t)nb * tile_bytes, s));
 }
 
-void launch_assemble_z64(const AsmArgs& a, double* z, int cap, hipStream_t s) {
-  launch_tile<double, double, 0>(a, cap, z, nullptr, nullptr, s);
+void launch_assemble_z64(const AsmArgs& a, double* z, int cap, hipStream_t s, int nthr) {
+  launch_tile<double, double, 0>(a, cap, z, nullptr, nullptr, s, nthr);
 }
 void launch_assemble_z32(const AsmArgs& a, float* z, int cap, hipStream_t s) {
   launch_tile<float, float, 0>(a, cap, z, nullptr, nullptr, s);
 }
-void launch_assemble_z64p(const AsmArgs& a, double* z, int cap, hipStream_t s) {
+void launch_assemble_z64p(const AsmArgs& a, double* z, int cap, hipStream_t s, int nthr) {
   DM_REQUIRE(2 * a.TB == kPT, DM_ERR_ARGUMENT, "packed storage needs assembly tiles of half the storage tile");
-  launch_tile<double, double, 2>(a, cap, z, nullptr, nullptr, s);
+  launch_tile<double, double, 2>(a, cap, z, nullptr, nullptr, s, nthr);
 }
 void launch_assemble_z32p(const AsmArgs& a, float* z, int cap, hipStream_t s) {
   DM_REQUIRE(2 * a.TB == kPT, DM_ERR_ARGUMENT, "packed storage needs assembly tiles of half the storage tile");

@@ -36,6 +36,19 @@ void conj(i64 n, double* v, hipStream_t s);
 void givens(int j, int ldh, const double* hbuf, const double* nrm, int nb, i64 n, const double* w, double* vnext,
             double* H, double* cs, double* sn, double* g, double* st, double* sc, hipStream_t s);
 void backsolve(int j, int ldh, const double* H, const double* g, double* y, hipStream_t s);
+// The same steps for a captured (replayable) GMRES step: the step index j is
+// read from device memory (*jd), the grids are sized for k = j + 1 <= kmax
+// (givens: j <= jmax), givens writes v_{j+1} into column j + 1 of V and into
+// vcur, copy_col_dev writes src into column *jd of base, step_inc adds 1 to *jd.
+void dots_dev(i64 n, i64 ldv, int kmax, const int* jd, const double* V, const double* w, double* part,
+              hipStream_t s);
+void update_dev(i64 n, i64 ldv, const int* jd, int hsel, const double* V, const double* part, double* h, double* w,
+                double* nrm, hipStream_t s);
+void givens_dev(int jmax, const int* jd, int ldh, const double* hbuf, const double* nrm, int nb, i64 n,
+                const double* w, double* V, i64 ldv, double* vcur, double* H, double* cs, double* sn, double* g,
+                double* st, double* sc, hipStream_t s);
+void copy_col_dev(i64 n, const double* src, double* base, i64 ldv, const int* jd, hipStream_t s);
+void step_inc(int* jd, hipStream_t s);
 // y = Z x from packed symmetric storage; ws >= symv_workspace_bytes(n)
 size_t symv_workspace_bytes(i64 n);
 // shape: 2..64 = LDS-staged form with that many tiles per workgroup segment;
@@ -71,8 +84,10 @@ size_t zx_workspace_bytes(i64 n, int R);
 size_t zx_bf16_planes_elems(i64 ld, i64 n);
 void zx_bf16_split(i64 n, i64 ld, const float* z32, void* planes, hipStream_t s);
 size_t zx_bf16_workspace_bytes(i64 n, int R);
+// kc: columns per FP32 accumulation chunk (0: 4096; one of zx_bf16_chunks()).
+std::vector<int> zx_bf16_chunks();
 void zx_bf16(i64 n, i64 ld, const void* planes, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
-             hipStream_t s);
+             hipStream_t s, int kc = 0);
 // Y (mr x R) = A (mr x n, lda) X (n x R), complex128 on FP64 MFMA; ws:
 // gemm_c128_workspace_bytes(n, R).
 size_t gemm_c128_workspace_bytes(i64 n, int R);

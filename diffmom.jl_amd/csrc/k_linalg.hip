@@ -155,11 +155,15 @@ __global__ void k_permute(i64 n, const int* __restrict__ idx, const double2* __r
 constexpr int kDotVG = 4;
 constexpr int kDotRows = 4;  // rows per thread
 
-__global__ __launch_bounds__(256) void k_dots_part(i64 n, i64 ldv, int k, const double2* __restrict__ V,
-                                                   const double2* __restrict__ w, double2* __restrict__ part,
-                                                   int nch) {
+// jd (captured steps): k = *jd + 1 from device memory, the grid sized for
+// the largest k of the graph (workgroups past k exit).
+__global__ __launch_bounds__(256) void k_dots_part(i64 n, i64 ldv, int k, const int* __restrict__ jd,
+                                                   const double2* __restrict__ V, const double2* __restrict__ w,
+                                                   double2* __restrict__ part, int nch) {
   __shared__ double2 red[4][kDotVG];
+  if (jd) k = *jd + 1;
   const int c = blockIdx.x, i0 = blockIdx.y * kDotVG;
+  if (i0 >= k) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const i64 m0 = (i64)c * 256 * kDotRows + threadIdx.x;
   double2 wm[kDotRows];
@@ -196,13 +200,17 @@ __global__ __launch_bounds__(256) void k_dots_part(i64 n, i64 ldv, int k, const 
 // w[m] -= sum_{i<k} V[m, i] h[i], where h[i] = sum_c part[c * k + i] are
 // the chunk partials of k_dots_part: every workgroup adds them itself (thread
 // i over the chunks in order, all loads independent, so all workgroups hold
-// the same h) and workgroup 0 stores h.  nrm, when given, receives each
-// workgroup's sum of |w[m]|^2 over its rows after the update.
-__global__ __launch_bounds__(256) void k_update(i64 n, i64 ldv, int k, const double2* __restrict__ V,
-                                                const double2* __restrict__ part, int nch, double2* __restrict__ h,
-                                                double2* __restrict__ w, double* __restrict__ nrm) {
+// the same h) and workgroup 0 stores h (at h + hsel k).  nrm, when given,
+// receives each workgroup's sum of |w[m]|^2 over its rows after the update.
+// jd (captured steps): k = *jd + 1 from device memory.
+__global__ __launch_bounds__(256) void k_update(i64 n, i64 ldv, int k, const int* __restrict__ jd, int hsel,
+                                                const double2* __restrict__ V, const double2* __restrict__ part,
+                                                int nch, double2* __restrict__ h, double2* __restrict__ w,
+                                                double* __restrict__ nrm) {
   __shared__ double2 hs[1024];
   __shared__ double red[4];
+  if (jd) k = *jd + 1;
+  h += (i64)hsel * k;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < k; i += 256) {
     double sr = 0, si = 0;
@@ -236,6 +244,17 @@ __global__ __launch_bounds__(256) void k_update(i64 n, i64 ldv, int k, const dou
     __syncthreads();
     if (threadIdx.x == 0) nrm[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
   }
+}
+
+// dst column *jd of base (leading dimension ldv) = src
+__global__ void k_copy_col(i64 n, const double2* __restrict__ src, double2* __restrict__ base, i64 ldv,
+                           const int* __restrict__ jd) {
+  const i64 m = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (m < n) base[(i64)(*jd) * ldv + m] = src[m];
+}
+
+__global__ void k_step_inc(int* jd) {
+  if (threadIdx.x == 0) *jd += 1;
 }
 
 // out[0] = ||w||^2 (single workgroup of 1024 threads)
@@ -525,12 +544,20 @@ __global__ __launch_bounds__(1024) void k_lu_solve(i64 n, i64 ld, const double2*
 // (0 = breakdown); sc[0] = 1 / subdiagonal (0 at breakdown).  The launch
 // covers the n rows: every workgroup forms the same norm and writes its rows
 // of v_{j+1} = sc w; workgroup 0 also does the rotations.
-__global__ __launch_bounds__(256) void k_givens(int j, int ldh, const double* __restrict__ hbuf,
-                                                const double* __restrict__ nrm, int nb, i64 n,
-                                                const double2* __restrict__ w, double2* __restrict__ vnext,
-                                                double2* __restrict__ H, double* __restrict__ cs,
-                                                double2* __restrict__ sn, double2* __restrict__ g,
-                                                double* __restrict__ st, double* __restrict__ sc) {
+// jd (captured steps): j = *jd from device memory, v_{j+1} written to column
+// j + 1 of Vb (leading dimension ldv) and to vcur (the next step's input).
+__global__ __launch_bounds__(256) void k_givens(int j, const int* __restrict__ jd, int ldh,
+                                                const double* __restrict__ hbuf, const double* __restrict__ nrm,
+                                                int nb, i64 n, const double2* __restrict__ w,
+                                                double2* __restrict__ vnext, double2* __restrict__ Vb, i64 ldv,
+                                                double2* __restrict__ vcur, double2* __restrict__ H,
+                                                double* __restrict__ cs, double2* __restrict__ sn,
+                                                double2* __restrict__ g, double* __restrict__ st,
+                                                double* __restrict__ sc) {
+  if (jd) {
+    j = *jd;
+    vnext = Vb + (i64)(j + 1) * ldv;
+  }
   const int k = j + 1;
   double2* Hj = H + (i64)j * ldh;
   // the column and the previous rotations are staged in LDS by all threads;
@@ -560,7 +587,9 @@ __global__ __launch_bounds__(256) void k_givens(int j, int ldh, const double* __
     const i64 m = blockIdx.x * (i64)blockDim.x + threadIdx.x;
     if (m < n) {
       const double2 v = w[m];
-      vnext[m] = make_double2(scl * v.x, scl * v.y);
+      const double2 u = make_double2(scl * v.x, scl * v.y);
+      vnext[m] = u;
+      if (vcur) vcur[m] = u;
     }
   }
   if (blockIdx.x != 0) return;
@@ -696,8 +725,8 @@ void dots(i64 n, i64 ldv, int k, const double* V, const double* w, double* part,
   if (k <= 0) return;
   const int nch = (int)nblocks(n, 256 * kDotRows);
   dim3 g((unsigned)nch, (unsigned)((k + kDotVG - 1) / kDotVG));
-  k_dots_part<<<g, 256, 0, s>>>(n, ldv, k, reinterpret_cast<const double2*>(V), reinterpret_cast<const double2*>(w),
-                                reinterpret_cast<double2*>(part), nch);
+  k_dots_part<<<g, 256, 0, s>>>(n, ldv, k, nullptr, reinterpret_cast<const double2*>(V),
+                                reinterpret_cast<const double2*>(w), reinterpret_cast<double2*>(part), nch);
   HIP_OK(hipGetLastError());
 }
 void update(i64 n, i64 ldv, int k, const double* V, const double* part, double* h, double* w, double* nrm,
@@ -705,7 +734,7 @@ void update(i64 n, i64 ldv, int k, const double* V, const double* part, double* 
   if (k <= 0) return;
   DM_REQUIRE(k <= 1024, DM_ERR_ARGUMENT, "Krylov basis larger than 1024 vectors");
   const int nch = (int)nblocks(n, 256 * kDotRows);
-  k_update<<<nblocks(n, 256), 256, 0, s>>>(n, ldv, k, reinterpret_cast<const double2*>(V),
+  k_update<<<nblocks(n, 256), 256, 0, s>>>(n, ldv, k, nullptr, 0, reinterpret_cast<const double2*>(V),
                                             reinterpret_cast<const double2*>(part), nch, reinterpret_cast<double2*>(h),
                                             reinterpret_cast<double2*>(w), nrm);
   HIP_OK(hipGetLastError());
@@ -721,9 +750,47 @@ void givens(int j, int ldh, const double* hbuf, const double* nrm, int nb, i64 n
             double* H, double* cs, double* sn, double* g, double* st, double* sc, hipStream_t s) {
   const size_t smem = sizeof(double2) * (2 * (size_t)j + 2) + sizeof(double) * j;
   DM_REQUIRE(smem <= 60 * 1024, DM_ERR_ARGUMENT, "Krylov basis too large for the device Givens step");
-  k_givens<<<nblocks(n, 256), 256, smem, s>>>(j, ldh, hbuf, nrm, nb, n, reinterpret_cast<const double2*>(w),
-                                              reinterpret_cast<double2*>(vnext), reinterpret_cast<double2*>(H), cs,
+  k_givens<<<nblocks(n, 256), 256, smem, s>>>(j, nullptr, ldh, hbuf, nrm, nb, n,
+                                              reinterpret_cast<const double2*>(w), reinterpret_cast<double2*>(vnext),
+                                              nullptr, 0, nullptr, reinterpret_cast<double2*>(H), cs,
                                               reinterpret_cast<double2*>(sn), reinterpret_cast<double2*>(g), st, sc);
+  HIP_OK(hipGetLastError());
+}
+
+void dots_dev(i64 n, i64 ldv, int kmax, const int* jd, const double* V, const double* w, double* part,
+              hipStream_t s) {
+  const int nch = (int)nblocks(n, 256 * kDotRows);
+  dim3 g((unsigned)nch, (unsigned)((kmax + kDotVG - 1) / kDotVG));
+  k_dots_part<<<g, 256, 0, s>>>(n, ldv, kmax, jd, reinterpret_cast<const double2*>(V),
+                                reinterpret_cast<const double2*>(w), reinterpret_cast<double2*>(part), nch);
+  HIP_OK(hipGetLastError());
+}
+void update_dev(i64 n, i64 ldv, const int* jd, int hsel, const double* V, const double* part, double* h, double* w,
+                double* nrm, hipStream_t s) {
+  const int nch = (int)nblocks(n, 256 * kDotRows);
+  k_update<<<nblocks(n, 256), 256, 0, s>>>(n, ldv, 0, jd, hsel, reinterpret_cast<const double2*>(V),
+                                            reinterpret_cast<const double2*>(part), nch, reinterpret_cast<double2*>(h),
+                                            reinterpret_cast<double2*>(w), nrm);
+  HIP_OK(hipGetLastError());
+}
+void givens_dev(int jmax, const int* jd, int ldh, const double* hbuf, const double* nrm, int nb, i64 n,
+                const double* w, double* V, i64 ldv, double* vcur, double* H, double* cs, double* sn, double* g,
+                double* st, double* sc, hipStream_t s) {
+  const size_t smem = sizeof(double2) * (2 * (size_t)jmax + 2) + sizeof(double) * jmax;
+  DM_REQUIRE(smem <= 60 * 1024, DM_ERR_ARGUMENT, "Krylov basis too large for the device Givens step");
+  k_givens<<<nblocks(n, 256), 256, smem, s>>>(0, jd, ldh, hbuf, nrm, nb, n, reinterpret_cast<const double2*>(w),
+                                              nullptr, reinterpret_cast<double2*>(V), ldv,
+                                              reinterpret_cast<double2*>(vcur), reinterpret_cast<double2*>(H), cs,
+                                              reinterpret_cast<double2*>(sn), reinterpret_cast<double2*>(g), st, sc);
+  HIP_OK(hipGetLastError());
+}
+void copy_col_dev(i64 n, const double* src, double* base, i64 ldv, const int* jd, hipStream_t s) {
+  k_copy_col<<<nblocks(n, 256), 256, 0, s>>>(n, reinterpret_cast<const double2*>(src),
+                                              reinterpret_cast<double2*>(base), ldv, jd);
+  HIP_OK(hipGetLastError());
+}
+void step_inc(int* jd, hipStream_t s) {
+  k_step_inc<<<1, 64, 0, s>>>(jd);
   HIP_OK(hipGetLastError());
 }
 

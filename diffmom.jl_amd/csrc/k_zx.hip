@@ -232,7 +232,7 @@ __global__ void k_split_x_bf16(i64 n, i64 np, int R, int Rp, const double2* __re
 __global__ __launch_bounds__(256, 1) void k_zx_bf16(i64 n, i64 R2, int nrb, int ncb, const __bf16* __restrict__ Ap,
                                                     i64 aplane,
                                                     const __bf16* __restrict__ Xp, i64 np, i64 xplane, int R,
-                                                    double2* __restrict__ part) {
+                                                    i64 kc, double2* __restrict__ part) {
   __shared__ __align__(16) __bf16 Bs[2][6][ZX_BN][ZB_LDA];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // XCD-aware tile order: workgroup w runs on XCD w % 8; its index j = w / 8
@@ -242,8 +242,8 @@ __global__ __launch_bounds__(256, 1) void k_zx_bf16(i64 n, i64 R2, int nrb, int 
   if (rb_idx >= nrb) return;
   const i64 m0 = (i64)rb_idx * ZX_BM;  // complex row base
   const int c0 = cb_idx * ZX_BN;       // rhs base
-  const i64 k_begin = (i64)blockIdx.z * ZX_KC;
-  const i64 k_end = k_begin + ZX_KC < n ? k_begin + ZX_KC : n;
+  const i64 k_begin = (i64)blockIdx.z * kc;
+  const i64 k_end = k_begin + kc < n ? k_begin + kc : n;
   const int nst = (int)((k_end - k_begin + 31) / 32);
   typedef unsigned int u4v __attribute__((ext_vector_type(4)));
   // A' rows belong to one wave each (32 real rows per wave): they go from
@@ -395,10 +395,14 @@ size_t zx_bf16_workspace_bytes(i64 n, int R) {
   const i64 nk = (n + ZX_KC - 1) / ZX_KC, np = (n + 31) / 32 * 32, Rp = ((i64)R + ZX_BN - 1) / ZX_BN * ZX_BN;
   return (size_t)nk * R * n * sizeof(double2) + 64 + (size_t)6 * np * Rp * 2;
 }
+std::vector<int> zx_bf16_chunks() { return {ZX_KC, 2 * ZX_KC}; }
 void zx_bf16(i64 n, i64 ld, const void* planes, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
-             hipStream_t s) {
+             hipStream_t s, int kc) {
   DM_REQUIRE(ld % ZX_BM == 0 && ld >= n, DM_ERR_ARGUMENT, "complex64 matrix leading dimension must be a multiple of 64");
-  const i64 nk = (n + ZX_KC - 1) / ZX_KC, np = (n + 31) / 32 * 32;
+  if (kc <= 0) kc = ZX_KC;
+  // the workspace holds the partials of ZX_KC-column chunks; longer chunks need fewer
+  DM_REQUIRE(kc % 32 == 0 && kc >= ZX_KC, DM_ERR_ARGUMENT, "Z X column chunk must be a multiple of 32, >= 4096");
+  const i64 nk = (n + kc - 1) / kc, np = (n + 31) / 32 * 32;
   const int Rp = (R + ZX_BN - 1) / ZX_BN * ZX_BN;
   double2* part = reinterpret_cast<double2*>(ws);
   __bf16* Xp = reinterpret_cast<__bf16*>(reinterpret_cast<char*>(ws) +
@@ -412,7 +416,7 @@ void zx_bf16(i64 n, i64 ld, const void* planes, const double* X, i64 ldx, int R,
   const i64 R2 = 2 * ld, ng = zx_bf16_groups(n);
   dim3 grid((unsigned)nwg, 1, (unsigned)nk);
   k_zx_bf16<<<grid, 256, 0, s>>>(n, R2, nrb, ncb, reinterpret_cast<const __bf16*>(planes), ng * R2 * 8, Xp, np,
-                                 xplane, R, part);
+                                 xplane, R, (i64)kc, part);
   const i64 tot = n * (i64)R;
   k_zx_reduce<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, R, (int)nk, part, reinterpret_cast<double2*>(Y), ldy);
   HIP_OK(hipGetLastError());

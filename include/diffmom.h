@@ -220,6 +220,10 @@ typedef struct dm_solve_opts {
                           product), 1 = full N x N storage (the reference's dense
                           Matrix{ComplexF64}).  Batches and dense LU always use full
                           storage; MIXED always packs its one-column forms. */
+  int graph;           /* one right-hand side at a time: 1 captures one Krylov step
+                          (preconditioner, product, CGS2, Givens) in a HIP graph and
+                          replays it, the step index read from device memory; 0 =
+                          launches each kernel (default). */
 } dm_solve_opts;
 
 typedef struct dm_solve_stats {

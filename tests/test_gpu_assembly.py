@@ -353,6 +353,58 @@ def test_tuned_symmetric_product_shapes(dm, mk):
         ctx.tune_set(got)
 
 
+def test_tuned_assembly_threads_and_zx_chunks(dm):
+    """Tuner keys of the FP64 tile kernel's threads per workgroup
+    (asm/<form>/q<points>/<N>: 1024 or 512) and of the bf16 Z X's FP32
+    accumulation chunk (zx/c64/<N>/<R>: 4096 or 8192 columns): recorded on
+    first use and exported by tune_get; every allowed value gives the same
+    matrix (FP64 sums in another order) or the same product to FP32 accuracy;
+    values outside the candidates are refused."""
+    ctx = dm.default_context()
+    m = dm.make_icosphere(0.1, 3)
+    r = dm.build_rwg(m)
+    forms = dm.DM_Z64P | dm.DM_Z32P | dm.DM_Z32_ROUNDED
+    Z = dm.assemble_Z_efie(m, r, K0, forms=forms, device=True)
+    key = f"asm/z64p+z32/q3/{r.nedges}"
+    got = ctx.tune_get()
+    assert got[key] in (512, 1024)
+    P64 = Z.to_numpy(dm.DM_Z64P)
+    for t in (512, 1024):
+        ctx.tune_set({key: t})
+        Zt = dm.assemble_Z_efie(m, r, K0, forms=forms, device=True)
+        Pt = Zt.to_numpy(dm.DM_Z64P)
+        assert np.linalg.norm(Pt - P64) <= 1e-14 * np.linalg.norm(P64), t
+        assert np.array_equal(Zt.to_numpy(dm.DM_Z32P), Pt.astype(np.complex64).astype(np.complex128))
+        Zt.free()
+    ctx.tune_set({key: 256})
+    with pytest.raises(Exception):
+        dm.assemble_Z_efie(m, r, K0, forms=forms, device=True)
+    ctx.tune_set(got)
+    # Z X on a matrix wider than one 4096-column chunk
+    ms = dm.make_icosphere(0.1, 4)
+    rs = dm.build_rwg(ms)
+    Zs = dm.assemble_Z_efie(ms, rs, K0, forms=dm.DM_Z64 | dm.DM_Z32 | dm.DM_Z32_ROUNDED, device=True)
+    rng = np.random.default_rng(13)
+    X = rng.standard_normal((rs.nedges, 16)) + 1j * rng.standard_normal((rs.nedges, 16))
+    ref = Zs.matmul(X, form=dm.DM_Z64)
+    Y = Zs.matmul(X, form=dm.DM_Z32)
+    zkey = f"zx/c64/{rs.nedges}/16"
+    got = ctx.tune_get()
+    assert got[zkey] in (4096, 8192)
+    for kc in (4096, 8192):
+        ctx.tune_set({zkey: kc})
+        Yk = Zs.matmul(X, form=dm.DM_Z32)
+        err = np.linalg.norm(Yk - ref) / np.linalg.norm(ref)
+        print(f"Z X chunk {kc}: error vs complex128 product {err:.2e}")
+        assert err <= 1e-6, kc
+    assert np.linalg.norm(Y - ref) <= 1e-6 * np.linalg.norm(ref)
+    ctx.tune_set({zkey: 2048})
+    with pytest.raises(Exception):
+        Zs.matmul(X, form=dm.DM_Z32)
+    ctx.tune_set(got)
+    Zs.free()
+
+
 @pytest.mark.parametrize("mk", ["plate", "sphere"])
 def test_compressed_form_error_bound(dm, mk):
     """dm_zmat_compress: off-diagonal stored tiles become cross approximations

@@ -497,3 +497,44 @@ def test_c1_plate_chosen_form_matches_plain_fp64(dm):
     print(f"C1 plate: chosen {sc.niter} it / {sc.outer} refinements, residual {res:.2e}; plain {sp.niter} it")
     assert res <= 1e-6 * 1.001 and abs(res - sc.residual) <= 1e-9
     assert np.linalg.norm(Xc - Xp) / np.linalg.norm(Xp) < 1e-4
+
+
+def test_graph_replayed_steps_match_launched_steps(dm):
+    """Krylov steps replayed from a captured HIP graph (the step index read
+    from device memory, one graph per basis-size bucket) against the same
+    solves with every kernel launched: equal iteration counts and solutions
+    to 1e-13, for right (SPAI), left (SPAI) and no preconditioning, full and
+    restarted GMRES, and the mixed solve (two graph captures, one per
+    refinement step); each graph solve runs three times in a row with other
+    device work between, every run checked."""
+    m = dm.make_rect_plate(0.1, 0.1, 12, 12)
+    r = dm.build_rwg(m)
+    Z = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64P | dm.DM_Z32P, device=True)
+    v = dm.assemble_v_plane_wave(m, r, [0, 0, -K0], 1.0, [1, 0, 0])
+    P = dm.build_nearfield_preconditioner(None, m, r, 2.0 * float(np.mean(r.len)))
+    for Pc, side, restart in ((P, "right", 0), (P, "left", 0), (None, "right", 0), (P, "right", 12)):
+        x0, s0 = dm.solve_gmres(Z, v, preconditioner=Pc, precond_side=side, tol=1e-10, maxiter=600, restart=restart)
+        assert s0.solved
+        for rep in range(3):
+            x1, s1 = dm.solve_gmres(Z, v, preconditioner=Pc, precond_side=side, tol=1e-10, maxiter=600,
+                                    restart=restart, graph=True)
+            print(f"graph {side} restart={restart} rep {rep}: {s1.niter} it (launched {s0.niter})", flush=True)
+            assert s1.solved and s1.niter == s0.niter
+            assert np.linalg.norm(x1 - x0) <= 1e-13 * np.linalg.norm(x0)
+            Z.matmul(x1)  # other device work between the replays
+    xm0, sm0 = dm.solve_mixed(Z, v, preconditioner=P, tol=1e-9)
+    for rep in range(2):
+        xm1, sm1 = dm.solve_mixed(Z, v, preconditioner=P, tol=1e-9, graph=True)
+        assert sm1.solved and (sm1.niter, sm1.outer) == (sm0.niter, sm0.outer)
+        assert np.linalg.norm(xm1 - xm0) <= 1e-13 * np.linalg.norm(xm0)
+    # end to end on a sphere with the compressed complex64 form
+    ms = dm.make_icosphere(0.1, 3)
+    rs = dm.build_rwg(ms)
+    Ps = dm.build_nearfield_preconditioner(None, ms, rs, 2.0 * float(np.mean(rs.len)))
+    pw = [dm.make_plane_wave([0, 0, -K0], 1.0, [1, 0, 0])]
+    X0, st0, _ = dm.solve_planewaves(ms, rs, K0, pw, "mixed", Ps, tol=1e-6, maxiter=500, batch=-1,
+                                     compress_tol=5e-8)
+    X1, st1, _ = dm.solve_planewaves(ms, rs, K0, pw, "mixed", Ps, tol=1e-6, maxiter=500, batch=-1,
+                                     compress_tol=5e-8, graph=True)
+    assert st1.solved and st1.niter == st0.niter
+    assert np.linalg.norm(X1 - X0) <= 1e-13 * np.linalg.norm(X0)
