@@ -268,8 +268,9 @@ int dm_solve(dm_ctx* ctx, const dm_zmat* z, const double* B, int64_t ldb, int64_
  * demand.  Here the assembled packed complex64 form (DM_Z32P, required) is
  * compressed tile by tile: every off-diagonal 128 x 128 tile whose cross
  * approximation reaches the per-tile bound within rmax terms (1..64) is stored
- * as U V^T, so that ||Z32P - Z32H||_F <= eps ||Z32P||_F (plus the FP32
- * rounding of the residual updates, a few 1e-8 ||Z||_F).  Adds DM_Z32H to the
+ * as U V^T, so that ||Z32P - Z32H||_F <= eps sqrt(D) <= eps ||Z32P||_F with
+ * D the diagonal tiles' share of ||Z32P||_F^2 (plus the FP32 rounding of the
+ * residual updates, a few 1e-8 ||Z||_F).  Adds DM_Z32H to the
  * forms.  stats (may be NULL): [0] stored tiles, [1] low-rank tiles, [2] mean
  * rank of the low-rank tiles, [3] bytes a product reads relative to DM_Z32P. */
 int dm_zmat_compress(dm_zmat* z, double eps, int rmax, double* stats);
