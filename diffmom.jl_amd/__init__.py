@@ -115,7 +115,7 @@ class dm_solve_opts(C.Structure):
                 ("tol", C.c_double), ("nf_cutoff", C.c_double), ("restart", C.c_int),
                 ("check_convergence", C.c_int), ("inner_tol", C.c_double), ("max_outer", C.c_int),
                 ("block_size", C.c_int), ("batch", C.c_int), ("learned", C.c_void_p), ("adjoint", C.c_int),
-                ("compress_tol", C.c_double), ("storage", C.c_int), ("graph", C.c_int)]
+                ("compress_tol", C.c_double), ("storage", C.c_int), ("graph", C.c_int), ("relax", C.c_double)]
 
 
 class dm_solve_stats(C.Structure):
@@ -169,6 +169,7 @@ def lib():
         "dm_profile_enable": ([vp, i32], i32), "dm_profile_reset": ([vp], i32),
         "dm_profile_read": ([vp, C.c_char_p, C.POINTER(i64), C.POINTER(d)], i32),
         "dm_tune_get": ([vp, C.c_char_p, i64, C.POINTER(i64)], i32), "dm_tune_set": ([vp, C.c_char_p], i32),
+        "dm_zmat_hlevel": ([vp, i32, vp], i32),
         "dm_mpnn_param_count": ([], i64), "dm_mpnn_create": ([vp, vp, pp], i32),
         "dm_mpnn_get_params": ([vp, vp], i32), "dm_mpnn_free": ([vp], i32),
         "dm_rwg_graph": ([vp, d, vp, vp, vp], i32),
@@ -474,6 +475,16 @@ class ZMatrix:
         _check(lib().dm_zmat_compress(self.h, float(eps), int(rmax), _ptr(st)))
         self.forms |= DM_Z32H
         return dict(tiles=int(st[0]), lowrank_tiles=int(st[1]), mean_rank=float(st[2]), bytes_ratio=float(st[3]))
+
+    def compress_level(self, level):
+        """Select the compressed form's level for later DM_Z32H products
+        (dm_zmat_hlevel): level l uses the prefix of every tile's
+        cross-approximation factors at 10^l times the tolerance (error bound
+        eps 10^l).  Returns that level's statistics and its bound."""
+        st = np.zeros(5)
+        _check(lib().dm_zmat_hlevel(self.h, int(level), _ptr(st)))
+        return dict(tiles=int(st[0]), lowrank_tiles=int(st[1]), mean_rank=float(st[2]), bytes_ratio=float(st[3]),
+                    eps=float(st[4]))
 
     def to_numpy(self, form=None):
         out = np.zeros((self.N, self.N), dtype=np.complex128, order="F")
@@ -791,14 +802,15 @@ def _solve(Z, B, method, precond=None, side="right", tol=1e-6, maxiter=300, rest
 
 
 def _opts(method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, check=True, inner_tol=0.0,
-          max_outer=10, batch=0, compress_tol=0.0, storage="packed", graph=False):
+          max_outer=10, batch=0, compress_tol=0.0, storage="packed", graph=False, relax=0.0):
     """dm_solve_opts.  batch: 0 = lockstep batches of right-hand sides when there
     are >= 8 of them, 1 = always, -1 = one right-hand side at a time.
     compress_tol > 0: the mixed solve compresses its complex64 matrix.
     storage: "packed" (symmetric tiles, half the bytes) or "full" for the
     one-column FP64 GMRES path of solve_planewaves.  graph: one right-hand
     side at a time, each Krylov step after the first of a cycle replays a
-    captured HIP graph."""
+    captured HIP graph.  relax > 0 (mixed, compressed, one right-hand side
+    at a time): inner products relaxed as the residual falls."""
     if storage not in ("packed", "full"):
         raise ValueError(f"storage must be 'packed' or 'full', got {storage!r}")
     o = dm_solve_opts()
@@ -807,6 +819,7 @@ def _opts(method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, 
     o.batch = int(batch)
     o.storage = 0 if storage == "packed" else 1
     o.graph = 1 if graph else 0
+    o.relax = float(relax)
     o.compress_tol = float(compress_tol)
     o.check_convergence, o.inner_tol, o.max_outer = int(check), float(inner_tol), int(max_outer)
     o.precond_side = 1 if side == "right" else 0
@@ -831,7 +844,7 @@ def _opts(method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, 
 
 def solve_planewaves(mesh, rwg, k, excitations, method="mixed", preconditioner=None, tol=1e-6, maxiter=300,
                      quad_order=3, eta0=ETA0, check_convergence=True, keep_matrix=False, batch=0, restart=0,
-                     compress_tol=0.0, storage="packed", graph=False):
+                     compress_tol=0.0, storage="packed", graph=False, relax=0.0, inner_tol=0.0):
     """End-to-end device solve for plane waves (the dense branches of
     solve_scattering, src/Workflow.jl:67-317): assembly, excitation,
     preconditioner and Krylov solve in one call.  method: "direct" (dense LU),
@@ -843,7 +856,11 @@ def solve_planewaves(mesh, rwg, k, excitations, method="mixed", preconditioner=N
     storage ("fp64", one wave at a time): "packed" keeps the upper triangle of
     128 x 128 tiles (Z = Z^T exactly; each product reads half the bytes),
     "full" the N x N matrix as the reference stores it.  graph (one wave at
-    a time): Krylov steps replay a captured HIP graph.
+    a time): Krylov steps replay a captured HIP graph.  relax > 0 (mixed,
+    compress_tol > 0, one wave at a time): the inner FGMRES relaxes its
+    products as its residual falls (inexact Krylov), reading a prefix of the
+    cross-approximation factors; inner_tol: the first inner solve's relative
+    tolerance (0: tol / 2).
     Returns (X, stats, assembly_time_s[, ZMatrix])."""
     _check_pair(mesh, rwg)
     M = len(excitations)
@@ -852,7 +869,7 @@ def solve_planewaves(mesh, rwg, k, excitations, method="mixed", preconditioner=N
     E0 = np.ascontiguousarray([float(e.E0) for e in excitations])
     meth = {"direct": DM_METHOD_LU, "fp64": DM_METHOD_GMRES, "mixed": DM_METHOD_MIXED}[method]
     o = _opts(meth, preconditioner, "right", tol, maxiter, restart=restart, check=check_convergence, batch=batch,
-              compress_tol=compress_tol, storage=storage, graph=graph)
+              compress_tol=compress_tol, storage=storage, graph=graph, relax=relax, inner_tol=inner_tol)
     X = np.zeros((rwg.nedges, M), dtype=np.complex128, order="F")
     st = dm_solve_stats()
     t_asm = C.c_double()

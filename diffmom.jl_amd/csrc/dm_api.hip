@@ -323,18 +323,22 @@ void compress_z32p(dm_zmat* z, double eps, int rmax, hipStream_t s) {
   const i64 n = z->n, T = packed_tiles(n);
   z->hwork.alloc(T + 1);
   z->hrank.alloc(T);
+  z->hrank_lv.alloc((size_t)kHLevels * T);
   z->hfac.alloc((size_t)T * 2 * kPT * rmax * 2);
   z->hrmax = rmax;
-  dml::aca_compress(n, z->z32p.p, eps, rmax, z->hwork.p, z->hrank.p, z->hfac.p, s);
+  z->heps = eps;
+  z->hlevel = 0;
+  dml::aca_compress(n, z->z32p.p, eps, rmax, z->hwork.p, z->hrank.p, z->hrank_lv.p, z->hfac.p, s);
   z->forms |= DM_Z32H;
 }
 
 // [0] stored tiles, [1] low-rank tiles, [2] their mean rank, [3] bytes a
 // product reads relative to the dense packed form.
-void compression_stats(dm_zmat* z, double* stats, hipStream_t s) {
+void compression_stats(dm_zmat* z, double* stats, hipStream_t s, int level = 0) {
   const i64 T = packed_tiles(z->n);
   std::vector<int> rk(T);
-  HIP_OK(hipMemcpyAsync(rk.data(), z->hrank.p, sizeof(int) * T, hipMemcpyDeviceToHost, s));
+  const int* src = level == 0 ? z->hrank.p : z->hrank_lv.p + (size_t)(level - 1) * T;
+  HIP_OK(hipMemcpyAsync(rk.data(), src, sizeof(int) * T, hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   i64 low = 0;
   double rsum = 0, bytes = 0;
@@ -347,7 +351,7 @@ void compression_stats(dm_zmat* z, double* stats, hipStream_t s) {
       bytes += (double)kPT * kPT * 8;
     }
   }
-  z->hbytes = bytes;
+  if (level == 0) z->hbytes = bytes;
   stats[0] = (double)T;
   stats[1] = (double)low;
   stats[2] = low ? rsum / low : 0.0;
@@ -411,6 +415,7 @@ void validate_opts(const dm_solve_opts& o) {
              "compress_tol must be in [0, 1)");
   DM_REQUIRE(o.storage == 0 || o.storage == 1, DM_ERR_ARGUMENT, "storage must be 0 (packed) or 1 (full)");
   DM_REQUIRE(o.graph == 0 || o.graph == 1, DM_ERR_ARGUMENT, "graph must be 0 or 1");
+  DM_REQUIRE(std::isfinite(o.relax) && o.relax >= 0, DM_ERR_ARGUMENT, "relax must be finite and >= 0");
 }
 
 void check_ctx(const void* a, const char* what) { DM_REQUIRE(a != nullptr, DM_ERR_ARGUMENT, std::string(what) + " is NULL"); }
@@ -847,6 +852,20 @@ int dm_zmat_compress(dm_zmat* z, double eps, int rmax, double* stats) {
   });
 }
 
+int dm_zmat_hlevel(dm_zmat* z, int level, double* stats) {
+  return guard([&] {
+    check_ctx(z, "zmat");
+    DM_REQUIRE(z->forms & DM_Z32H, DM_ERR_ARGUMENT, "the matrix has no compressed form");
+    DM_REQUIRE(level >= 0 && level <= kHLevels, DM_ERR_ARGUMENT, "compression level out of range");
+    CtxScope sc(z->ctx);
+    z->hlevel = level;
+    if (stats) {
+      compression_stats(z, stats, z->ctx->stream, level);
+      stats[4] = z->heps * std::pow(10.0, level);
+    }
+  });
+}
+
 int dm_zmat_get(const dm_zmat* z, int form, double* out) {
   return guard([&] {
     check_ctx(z, "zmat");
@@ -1227,6 +1246,7 @@ void dm_solve_opts_default(dm_solve_opts* o) {
   o->adjoint = 0;
   o->storage = 0;
   o->graph = 0;
+  o->relax = 0.0;
 }
 
 int dm_solve_scattering_pw(dm_ctx* ctx, const dm_rwg* rwg_c, double k_re, double k_im, double eta0, int quad_order,

@@ -237,6 +237,10 @@ struct dm_rwg {
 // storage: tiles of kPT x kPT bases (I <= J) at tile index J (J + 1) / 2 + I,
 // each tile column-major; entries past n are zero.
 constexpr int kPT = 128;
+// Looser levels of the compressed form: level l (1 .. kHLevels) bounds the
+// error by eps 10^l with the prefix of the factors the cross approximation
+// had at that tolerance.
+constexpr int kHLevels = 5;
 inline i64 packed_tiles(i64 n) {
   i64 nb = (n + kPT - 1) / kPT;
   return nb * (nb + 1) / 2;
@@ -286,6 +290,9 @@ struct dm_zmat {
   DBuf<int> hrank;
   DBuf<float> hfac;
   DBuf<double> hwork;  // tile norms and the tolerance
+  DBuf<int> hrank_lv;  // kHLevels x T: per level the rank of each tile (-1: dense)
+  double heps = 0;     // eps of the compression
+  int hlevel = 0;      // level of the products dm_apply runs on the compressed form
   int hrmax = 0;
   double hbytes = 0;  // bytes one product reads
   ZRef ref(int form) const {

@@ -2,6 +2,7 @@
 // dm_solve / dm_solve_scattering_pw: one right-hand side at a time (GEMV
 // products) or a lockstep batch of right-hand sides (Z X products on MFMA).
 #pragma once
+#include <cmath>
 #include <complex>
 #include <functional>
 #include <string>
@@ -38,6 +39,10 @@ constexpr i64 kMfmaMaxRhs = 256;
 // batch_ws(R) bytes.
 struct DevOp {
   virtual ~DevOp() = default;
+  // Relaxed products (inexact Krylov): later products may use a form whose
+  // relative error bound is at most eta; returns the bound in use (operators
+  // without looser forms ignore it and return 0).
+  virtual double relax(double /*eta*/) { return 0.0; }
   virtual void apply(const double* x, double* y, hipStream_t s) = 0;
   virtual void apply_batch(const double* X, double* Y, int R, void* ws, hipStream_t s) = 0;
   virtual size_t batch_ws(int R) const = 0;
@@ -104,18 +109,29 @@ int symv_hier_shape(dm_ctx* ctx, const dm_zmat* z, void* ws, hipStream_t s);
 namespace dms {
 struct OpHier : DevOp {
   const dm_zmat* z;
-  DBuf<char> ws, rbuf;
-  dml::HierRanges hr;  // wave ranges, set up on the first product (ranks known by then)
-  explicit OpHier(const dm_zmat* z_) : z(z_) { ws.alloc(dml::symv_workspace_bytes(z->n)); }
+  DBuf<char> ws;
+  // per level (0: the compression's eps, l: eps 10^l) the tiles' ranks and
+  // the wave ranges, set up on the first product at that level
+  DBuf<char> rbuf[1 + kHLevels];
+  dml::HierRanges hr[1 + kHLevels];
+  int level;
+  explicit OpHier(const dm_zmat* z_) : z(z_), level(z_->hlevel) { ws.alloc(dml::symv_workspace_bytes(z->n)); }
+  const int* ranks(int l) const { return l == 0 ? z->hrank.p : z->hrank_lv.p + (size_t)(l - 1) * packed_tiles(z->n); }
+  double relax(double eta) override {
+    int l = 0;
+    while (l < kHLevels && z->heps * std::pow(10.0, l + 1) <= eta) ++l;
+    level = l;
+    return z->heps * std::pow(10.0, l);
+  }
   void apply(const double* x, double* y, hipStream_t s) override {
-    if (!hr.W) {
+    if (!hr[level].W) {
       const int mult = dmt::symv_hier_shape(z->ctx, z, ws.p, s);
       const int W = mult * dml::hier_wave_slots();
-      rbuf.alloc(dml::hier_ranges_bytes(z->n, W));
-      dml::hier_ranges(z->n, z->hrank.p, W, rbuf.p, hr, s);
+      rbuf[level].alloc(dml::hier_ranges_bytes(z->n, W));
+      dml::hier_ranges(z->n, ranks(level), W, rbuf[level].p, hr[level], s);
     }
-    ProfScope p(z->ctx, "symv_z32h", s);
-    dml::symv_hier(z->n, z->z32p.p, z->hrank.p, z->hfac.p, z->hrmax, x, y, ws.p, hr, s);
+    ProfScope p(z->ctx, level ? "symv_z32h_relaxed" : "symv_z32h", s);
+    dml::symv_hier(z->n, z->z32p.p, ranks(level), z->hfac.p, z->hrmax, x, y, ws.p, hr[level], s);
   }
   void apply_batch(const double* X, double* Y, int R, void*, hipStream_t s) override {
     for (int j = 0; j < R; ++j) apply(X + 2 * z->n * j, Y + 2 * z->n * j, s);
@@ -258,8 +274,11 @@ struct GmresOut {
 
 // x_zero: x is known to be zero on entry (the first residual is then b).
 // graph: steps after the first of each cycle replay a captured HIP graph.
+// relax > 0: relaxed products (A.relax(relax rtol beta / est) at every check
+// point, est the residual estimate and beta the cycle's initial residual).
 GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double* b, double* x, double rtol,
-               int maxiter, int restart, Krylov& K, hipStream_t s, bool x_zero = false, bool graph = false);
+               int maxiter, int restart, Krylov& K, hipStream_t s, bool x_zero = false, bool graph = false,
+               double relax = 0.0);
 double true_residual(DevOp& A, i64 n, const double* b, const double* x, double* tmp, double* hbuf, hipStream_t s);
 double solve_one(dm_ctx* ctx, const dm_solve_opts& o, DevOp* A64, DevOp* A32, DevPrec* M, const double* b,
                  double* x, i64 n, Krylov& K, dm_solve_stats& st, bool& ok, hipEvent_t z64_ready);

@@ -164,7 +164,7 @@ hipGraphExec_t step_graph(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, in
 // per step, the graph chosen by the bucket kb (a power of two >= 8, at most
 // m + 1) that bounds k = j + 1, so the CGS grids grow with the basis.
 GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double* b, double* x, double rtol,
-               int maxiter, int restart, Krylov& K, hipStream_t s, bool x_zero, bool graph) {
+               int maxiter, int restart, Krylov& K, hipStream_t s, bool x_zero, bool graph, double relax) {
   GmresOut out;
   int mcap = restart > 0 ? std::min(restart, maxiter) : maxiter;
   bool right = M && side == 1;
@@ -221,6 +221,9 @@ GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double*
     int jprev = 0;
     double eprev = beta;
     est = beta;
+    // relaxed products: the product error a step may carry grows as the
+    // residual falls (inexact Krylov); a cycle starts at the tight form
+    if (relax > 0) A.relax(relax * rtol);
     while (j < mcap && total < maxiter) {
       double* vj = col(V, j);
       if (graph && j >= 1) {
@@ -282,6 +285,7 @@ GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double*
       }
       seen = j;
       if (jstop >= 0) break;
+      if (relax > 0 && est > 0) A.relax(relax * rtol * beta / est);
       // next check: ~70 % of the steps the recent decay rate says are left
       const double rate = j > jprev && eprev > 0 ? std::pow(est / eprev, 1.0 / (j - jprev)) : 1.0;
       if (rate > 0 && rate < 0.999 && est > eps) {
@@ -295,6 +299,7 @@ GmresOut gmres(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, const double*
     }
     if (jstop < 0) jstop = j;
     total -= j - jstop;  // queued steps past the stopping point do not count
+    if (relax > 0) A.relax(0.0);
     // y = H^-1 g, x += V y (or Zs y)
     if (jstop > 0) {
       dml::backsolve(jstop, mcap + 1, K.Hd.p, K.gd.p, K.ybuf.p, s);
@@ -362,7 +367,7 @@ double solve_one(dm_ctx* ctx, const dm_solve_opts& o, DevOp* A64, DevOp* A32, De
   bool waited = false;
   for (int outer = 0; outer < std::max(1, o.max_outer) && left > 0; ++outer) {
     HIP_OK(hipMemsetAsync(d.p, 0, 16 * n, s));
-    GmresOut g = gmres(ctx, n, *A32, M, 1, rr.p, d.p, tau, left, o.restart, K, s, true, o.graph == 1);
+    GmresOut g = gmres(ctx, n, *A32, M, 1, rr.p, d.p, tau, left, o.restart, K, s, true, o.graph == 1, o.relax);
     st.iterations += g.iters;
     left -= g.iters;
     st.outer_iterations += 1;

@@ -20,7 +20,14 @@
 // diagonal tiles carry most of the EFIE matrix's norm, ~90 % on a sphere, and
 // reading them is 1/(nb/2 + 1/2) of reading the whole matrix), the
 // whole matrix (where each such tile appears twice) satisfies
-// ||Z - Z_H||_F <= eps sqrt(D) <= eps ||Z||_F.  Tiles that need more than rmax steps stay
+// ||Z - Z_H||_F <= eps sqrt(D) <= eps ||Z||_F.
+//
+// The steps of one run are nested: the first k columns of the factors are the
+// cross approximation a looser tolerance would have stopped at.  The kernel
+// records, per tile, the first step at which ||R||_F fell below
+// 10^l tau (l = 1 .. kHLevels, "level l"), so a product can read only that
+// prefix of the factors (error bound eps 10^l) at no extra compression cost;
+// tiles that stay dense at eps may be low-rank at a looser level.  Tiles that need more than rmax steps stay
 // dense (rank -1), as do the diagonal tiles.
 #include <hip/hip_runtime.h>
 
@@ -100,7 +107,7 @@ __device__ inline float wave_sum_f32(float x) {
 // is work for the two waves of that quarter only.
 __global__ __launch_bounds__(kAcaThreads, 4) void k_aca(int nb, const float2* __restrict__ zp,
                                                      const double* __restrict__ tau2p, int rmax, int* __restrict__ rank,
-                                                     float2* __restrict__ fac) {
+                                                     int* __restrict__ lvrank, i64 T, float2* __restrict__ fac) {
   const i64 t = blockIdx.x;
   const double tau2 = tau2p[0];
   int J = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
@@ -109,9 +116,17 @@ __global__ __launch_bounds__(kAcaThreads, 4) void k_aca(int nb, const float2* __
   const int I = (int)(t - (i64)J * (J + 1) / 2);
   const int tid = threadIdx.x;
   if (I == J) {
-    if (tid == 0) rank[t] = -1;
+    if (tid == 0) {
+      rank[t] = -1;
+      if (lvrank)
+        for (int l = 0; l < kHLevels; ++l) lvrank[l * T + t] = -1;
+    }
     return;
   }
+  // level l + 1 stops at ||R||^2 <= 100^(l + 1) tau^2 (uniform over the block)
+  int lvk[kHLevels];
+#pragma unroll
+  for (int l = 0; l < kHLevels; ++l) lvk[l] = -1;
   const int c = tid & (kPT - 1), qd = tid >> 7, r0 = qd * 32, lane = tid & 63, w = tid >> 6;
   float Rr[32], Ri[32];
   {
@@ -177,6 +192,14 @@ __global__ __launch_bounds__(kAcaThreads, 4) void k_aca(int nb, const float2* __
       tot += wsum[q];
     }
     const int piv_idx = key_index(bw, wkey[bw]);
+    {
+      double lt = tau2;
+#pragma unroll
+      for (int l = 0; l < kHLevels; ++l) {
+        lt *= 100.0;
+        if (lvk[l] < 0 && tot <= lt) lvk[l] = k;
+      }
+    }
     if (tot <= tau2) break;  // rank k reaches the tolerance
     if (k == rmax) {
       k = -1;  // not compressible within rmax: stays dense
@@ -217,14 +240,18 @@ __global__ __launch_bounds__(kAcaThreads, 4) void k_aca(int nb, const float2* __
     }
     publish();
   }
-  if (tid == 0) rank[t] = k;
+  if (tid == 0) {
+    rank[t] = k;
+    if (lvrank)
+      for (int l = 0; l < kHLevels; ++l) lvrank[l * T + t] = lvk[l];
+  }
 }
 
 }  // namespace
 
 namespace dml {
 
-void aca_compress(i64 n, const float* z32p, double eps, int rmax, double* work, int* rank, float* fac,
+void aca_compress(i64 n, const float* z32p, double eps, int rmax, double* work, int* rank, int* lvrank, float* fac,
                   hipStream_t s) {
   const i64 nb = (n + kPT - 1) / kPT;
   const i64 T = packed_tiles(n);
@@ -233,7 +260,7 @@ void aca_compress(i64 n, const float* z32p, double eps, int rmax, double* work, 
   k_tile_norm2<<<(unsigned)nb, 256, 0, s>>>(reinterpret_cast<const float2*>(z32p), work);
   k_tau2<<<1, 256, 0, s>>>(T, (int)nb, work, eps * eps, work + T);
   k_aca<<<(unsigned)T, kAcaThreads, 0, s>>>((int)nb, reinterpret_cast<const float2*>(z32p), work + T, rmax, rank,
-                                            reinterpret_cast<float2*>(fac));
+                                            lvrank, T, reinterpret_cast<float2*>(fac));
   HIP_OK(hipGetLastError());
 }
 

@@ -59,9 +59,10 @@ void symv_packed(i64 n, const ZRef& z, const double* x, double* y, void* ws, int
 std::vector<int> symv_shapes(i64 n, bool f32);
 // Compressed packed complex64 form: the cross approximation of the
 // off-diagonal tiles to ||Z - Z_H||_F <= eps ||Z||_F (rank[t] = -1: dense;
-// fac: per tile U then V, rmax columns of 128 complex64 each; work: T + 1
-// doubles) and its symmetric product (workspace: symv_workspace_bytes(n)).
-void aca_compress(i64 n, const float* z32p, double eps, int rmax, double* work, int* rank, float* fac,
+// lvrank (kHLevels x T, may be null): the ranks at the looser levels; fac:
+// per tile U then V, rmax columns of 128 complex64 each; work: T + 1 doubles)
+// and its symmetric product (workspace: symv_workspace_bytes(n)).
+void aca_compress(i64 n, const float* z32p, double eps, int rmax, double* work, int* rank, int* lvrank, float* fac,
                   hipStream_t s);
 // Wave ranges of the compressed product: W waves, wstart[W + 1] tile
 // boundaries carrying about equal bytes, rend[T] = 1 where a column run ends.

@@ -52,7 +52,7 @@ struct dm_solve_opts
     tol::Cdouble; nf_cutoff::Cdouble; restart::Cint; check_convergence::Cint
     inner_tol::Cdouble; max_outer::Cint; block_size::Cint
     batch::Cint; learned::Ptr{Cvoid}; adjoint::Cint; compress_tol::Cdouble
-    storage::Cint; graph::Cint
+    storage::Cint; graph::Cint; relax::Cdouble
 end
 mutable struct dm_solve_stats
     iterations::Cint; outer_iterations::Cint; solved::Cint
@@ -62,10 +62,11 @@ end
 
 function _opts(; method=DM_METHOD_GMRES, precond=DM_PRECOND_NEARFIELD, precond_side=1, maxiter=300, tol=1e-6,
                nf_cutoff=0.0, restart=0, check_convergence=true, compress_tol=0.0, storage=0, block_size=128,
-               graph=0)
+               graph=0, relax=0.0)
     return dm_solve_opts(Cint(method), Cint(precond), Cint(precond_side), Cint(maxiter), Float64(tol),
                          Float64(nf_cutoff), Cint(restart), Cint(check_convergence), 0.0, Cint(10), Cint(block_size),
-                         Cint(0), C_NULL, Cint(0), Float64(compress_tol), Cint(storage), Cint(graph))
+                         Cint(0), C_NULL, Cint(0), Float64(compress_tol), Cint(storage), Cint(graph),
+                         Float64(relax))
 end
 
 # ------------------------------------------------------------------ handles

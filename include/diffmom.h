@@ -224,6 +224,13 @@ typedef struct dm_solve_opts {
                           (preconditioner, product, CGS2, Givens) in a HIP graph and
                           replays it, the step index read from device memory; 0 =
                           launches each kernel (default). */
+  double relax;        /* MIXED with a compressed form, one right-hand side at a time:
+                          > 0 relaxes the inner products as the residual falls
+                          (inexact Krylov): at each check point the product may use
+                          the compression's looser level whose error bound is at
+                          most relax * inner_tol * ||r0|| / (residual estimate);
+                          the refinement residual stays the dense complex128
+                          product.  0 = off (default). */
 } dm_solve_opts;
 
 typedef struct dm_solve_stats {
@@ -274,6 +281,15 @@ int dm_solve(dm_ctx* ctx, const dm_zmat* z, const double* B, int64_t ldb, int64_
  * forms.  stats (may be NULL): [0] stored tiles, [1] low-rank tiles, [2] mean
  * rank of the low-rank tiles, [3] bytes a product reads relative to DM_Z32P. */
 int dm_zmat_compress(dm_zmat* z, double eps, int rmax, double* stats);
+/* No reference counterpart.  The compression's looser levels: the steps of
+ * one cross approximation are nested, so level l (1..5) reuses the prefix of
+ * each tile's factors at which its residual fell below 10^l times the tile
+ * tolerance (error bound eps 10^l; some tiles dense at eps are low-rank
+ * there).  Sets the level that dm_apply products on DM_Z32H use (0 = eps) and,
+ * with stats (5 doubles, may be NULL), reports it: the four dm_zmat_compress
+ * statistics at that level, then its error bound eps 10^l.  Solves choose
+ * their own levels (dm_solve_opts.relax). */
+int dm_zmat_hlevel(dm_zmat* z, int level, double* stats);
 
 /* ------------------------------------------------------------------ tuning */
 /* No reference counterpart.  Launch shapes (the symmetric product's tiles per

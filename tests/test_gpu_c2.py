@@ -4,7 +4,8 @@ normal-incidence x-polarised plane wave.
 
 One end-to-end chosen-form solve (FP64 assembly writing the packed complex128
 matrix and its complex64 rounding, cross approximation of the complex64 tiles
-to 5e-8, near-field SPAI, FGMRES with FP64 Arnoldi, FP64 refinement), then:
+to 5e-8, near-field SPAI, FGMRES with FP64 Arnoldi on the compressed form with
+its products relaxed as the residual falls, FP64 refinement), then:
   * 2000 sampled entries of both stored forms against the oracle's
     _efie_entry (src/assembly/EFIE.jl:439-512), by class: diagonal, bases
     sharing a triangle, bases on edge-adjacent triangles, bases on triangles
@@ -41,7 +42,7 @@ def c2(dm, O):
     P = dm.build_nearfield_preconditioner(None, m, r, 2.0 * h, factorization="spai")
     pw = [dm.make_plane_wave([0, 0, -K0], 1.0, [1, 0, 0])]
     X, st, _, Z = dm.solve_planewaves(m, r, K0, pw, "mixed", P, tol=1e-6, maxiter=1000, batch=-1,
-                                      compress_tol=5e-8, keep_matrix=True)
+                                      compress_tol=5e-8, relax=0.01, inner_tol=3e-7, keep_matrix=True)
     assert st.solved and st.residual <= 1e-6
     return dict(m=m, r=r, P=P, pw=pw, x=X[:, 0], st=st, Z=Z, ef=O.EFIE(oracle_rwg(O, m, r), K0))
 

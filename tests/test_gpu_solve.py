@@ -538,3 +538,29 @@ def test_graph_replayed_steps_match_launched_steps(dm):
                                      compress_tol=5e-8, graph=True)
     assert st1.solved and st1.niter == st0.niter
     assert np.linalg.norm(X1 - X0) <= 1e-13 * np.linalg.norm(X0)
+
+
+def test_relaxed_products_reach_the_true_residual(dm):
+    """Inexact Krylov on the compressed form (dm_solve_opts.relax): the inner
+    FGMRES reads looser prefixes of the cross-approximation factors as its
+    residual falls; the FP64 refinement against the dense complex128 matrix
+    still brings the true residual to the tolerance, with about the same
+    number of iterations as the unrelaxed solve."""
+    m = dm.make_icosphere(0.1, 4)
+    r = dm.build_rwg(m)
+    P = dm.build_nearfield_preconditioner(None, m, r, 2.0 * float(np.mean(r.len)))
+    pw = [dm.make_plane_wave([0, 0, -K0], 1.0, [1, 0, 0])]
+    X0, st0, _ = dm.solve_planewaves(m, r, K0, pw, "mixed", P, tol=1e-6, maxiter=500, batch=-1, compress_tol=5e-8)
+    for c in (0.003, 0.01):
+        X1, st1, _, Z = dm.solve_planewaves(m, r, K0, pw, "mixed", P, tol=1e-6, maxiter=500, batch=-1,
+                                            compress_tol=5e-8, relax=c, keep_matrix=True)
+        v = dm.assemble_v_plane_wave(m, r, [0, 0, -K0], 1.0, [1, 0, 0])
+        res = np.linalg.norm(Z.matmul(X1[:, 0], form=dm.DM_Z64P) - v) / np.linalg.norm(v)
+        print(f"relax {c}: {st1.niter} it, {st1.outer} refinements, true residual {res:.2e} "
+              f"(unrelaxed {st0.niter} it, {st0.outer})", flush=True)
+        assert st1.solved and res <= 1e-6
+        assert st1.niter <= 1.05 * st0.niter + 2 and st1.outer <= st0.outer + 1
+        assert np.linalg.norm(X1 - X0) <= 1e-4 * np.linalg.norm(X0)
+        Z.free()
+    with pytest.raises(ValueError):
+        dm.solve_planewaves(m, r, K0, pw, "mixed", P, tol=1e-6, batch=-1, compress_tol=5e-8, relax=-1.0)
