@@ -477,10 +477,10 @@ class ZMatrix:
         return dict(tiles=int(st[0]), lowrank_tiles=int(st[1]), mean_rank=float(st[2]), bytes_ratio=float(st[3]))
 
     def compress_level(self, level):
-        """Select the compressed form's level for later DM_Z32H products
-        (dm_zmat_hlevel): level l uses the prefix of every tile's
-        cross-approximation factors at 10^l times the tolerance (error bound
-        eps 10^l).  Returns that level's statistics and its bound."""
+        """Select the compressed form's level (0..10) for later DM_Z32H
+        products (dm_zmat_hlevel): level l uses the prefix of every tile's
+        cross-approximation factors at 10^(l/2) times the tolerance (error
+        bound eps 10^(l/2)).  Returns that level's statistics and its bound."""
         st = np.zeros(5)
         _check(lib().dm_zmat_hlevel(self.h, int(level), _ptr(st)))
         return dict(tiles=int(st[0]), lowrank_tiles=int(st[1]), mean_rank=float(st[2]), bytes_ratio=float(st[3]),

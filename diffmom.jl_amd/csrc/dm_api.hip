@@ -861,7 +861,7 @@ int dm_zmat_hlevel(dm_zmat* z, int level, double* stats) {
     z->hlevel = level;
     if (stats) {
       compression_stats(z, stats, z->ctx->stream, level);
-      stats[4] = z->heps * std::pow(10.0, level);
+      stats[4] = z->heps * hlevel_factor(level);
     }
   });
 }

@@ -1,5 +1,6 @@
 // dm_common.h — shared internals of libdiffmom (host + device).
 #pragma once
+#include <cmath>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -238,9 +239,10 @@ struct dm_rwg {
 // each tile column-major; entries past n are zero.
 constexpr int kPT = 128;
 // Looser levels of the compressed form: level l (1 .. kHLevels) bounds the
-// error by eps 10^l with the prefix of the factors the cross approximation
-// had at that tolerance.
-constexpr int kHLevels = 5;
+// error by eps 10^(l/2) with the prefix of the factors the cross
+// approximation had at that tolerance (half-decade steps up to 10^5 eps).
+constexpr int kHLevels = 10;
+inline double hlevel_factor(int l) { return std::pow(10.0, 0.5 * l); }
 inline i64 packed_tiles(i64 n) {
   i64 nb = (n + kPT - 1) / kPT;
   return nb * (nb + 1) / 2;

@@ -119,16 +119,23 @@ struct OpHier : DevOp {
   const int* ranks(int l) const { return l == 0 ? z->hrank.p : z->hrank_lv.p + (size_t)(l - 1) * packed_tiles(z->n); }
   double relax(double eta) override {
     int l = 0;
-    while (l < kHLevels && z->heps * std::pow(10.0, l + 1) <= eta) ++l;
+    while (l < kHLevels && z->heps * hlevel_factor(l + 1) <= eta) ++l;
     level = l;
-    return z->heps * std::pow(10.0, l);
+    return z->heps * hlevel_factor(l);
   }
   void apply(const double* x, double* y, hipStream_t s) override {
     if (!hr[level].W) {
+      // the wave ranges of every level in one batch of launches
       const int mult = dmt::symv_hier_shape(z->ctx, z, ws.p, s);
       const int W = mult * dml::hier_wave_slots();
-      rbuf[level].alloc(dml::hier_ranges_bytes(z->n, W));
-      dml::hier_ranges(z->n, ranks(level), W, rbuf[level].p, hr[level], s);
+      const int* rk[1 + kHLevels];
+      void* bufs[1 + kHLevels];
+      for (int l = 0; l <= kHLevels; ++l) {
+        rbuf[l].alloc(dml::hier_ranges_bytes(z->n, W));
+        rk[l] = ranks(l);
+        bufs[l] = rbuf[l].p;
+      }
+      dml::hier_ranges_levels(z->n, 1 + kHLevels, rk, W, bufs, hr, s);
     }
     ProfScope p(z->ctx, level ? "symv_z32h_relaxed" : "symv_z32h", s);
     dml::symv_hier(z->n, z->z32p.p, ranks(level), z->hfac.p, z->hrmax, x, y, ws.p, hr[level], s);

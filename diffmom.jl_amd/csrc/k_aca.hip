@@ -25,9 +25,10 @@
 // The steps of one run are nested: the first k columns of the factors are the
 // cross approximation a looser tolerance would have stopped at.  The kernel
 // records, per tile, the first step at which ||R||_F fell below
-// 10^l tau (l = 1 .. kHLevels, "level l"), so a product can read only that
-// prefix of the factors (error bound eps 10^l) at no extra compression cost;
-// tiles that stay dense at eps may be low-rank at a looser level.  Tiles that need more than rmax steps stay
+// 10^(l/2) tau (l = 1 .. kHLevels, "level l"), so a product can read only
+// that prefix of the factors (error bound eps 10^(l/2)) at no extra
+// compression cost; tiles that stay dense at eps may be low-rank at a looser
+// level.  Tiles that need more than rmax steps stay
 // dense (rank -1), as do the diagonal tiles.
 #include <hip/hip_runtime.h>
 
@@ -123,7 +124,7 @@ __global__ __launch_bounds__(kAcaThreads, 4) void k_aca(int nb, const float2* __
     }
     return;
   }
-  // level l + 1 stops at ||R||^2 <= 100^(l + 1) tau^2 (uniform over the block)
+  // level l + 1 stops at ||R||^2 <= 10^(l + 1) tau^2 (uniform over the block)
   int lvk[kHLevels];
 #pragma unroll
   for (int l = 0; l < kHLevels; ++l) lvk[l] = -1;
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(kAcaThreads, 4) void k_aca(int nb, const float2* __
       double lt = tau2;
 #pragma unroll
       for (int l = 0; l < kHLevels; ++l) {
-        lt *= 100.0;
+        lt *= 10.0;
         if (lvk[l] < 0 && tot <= lt) lvk[l] = k;
       }
     }

@@ -74,6 +74,9 @@ struct HierRanges {
 size_t hier_ranges_bytes(i64 n, int W);
 int hier_wave_slots();  // resident waves of the product kernel on this device
 void hier_ranges(i64 n, const int* rank, int W, void* buf, HierRanges& hr, hipStream_t s);
+// The same for nl rank arrays at once (buffers of hier_ranges_bytes(n, W) each).
+void hier_ranges_levels(i64 n, int nl, const int* const* ranks, int W, void* const* bufs, HierRanges* hr,
+                        hipStream_t s);
 void symv_hier(i64 n, const float* z32p, const int* rank, const float* fac, int rmax, const double* x, double* y,
                void* ws, const HierRanges& hr, hipStream_t s);
 // Workspace of either batched product (complex64 or complex128) for R columns.

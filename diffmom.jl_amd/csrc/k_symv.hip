@@ -731,6 +731,76 @@ __global__ void k_hranges(i64 T, int W, const i64* __restrict__ pre, int* __rest
   if (lo > 0) rend[lo - 1] = 1;
 }
 
+// The three range kernels over several rank arrays at once (one per level):
+// the level is blockIdx.x (scan) or blockIdx.y.
+constexpr int kMaxLevels = 16;
+struct LevelPtrs {
+  const int* rank[kMaxLevels];
+  i64* pre[kMaxLevels];
+  unsigned char* rend[kMaxLevels];
+  int* wstart[kMaxLevels];
+};
+
+__global__ __launch_bounds__(1024) void k_hscan_lv(i64 T, LevelPtrs lp) {
+  const int l = blockIdx.x;
+  const int* __restrict__ rank = lp.rank[l];
+  i64* __restrict__ pre = lp.pre[l];
+  __shared__ i64 part[1024];
+  const int tid = threadIdx.x;
+  const i64 per = (T + 1023) / 1024;
+  const i64 a = min(T, tid * per), b = min(T, a + per);
+  i64 sum = 0;
+  for (i64 t = a; t < b; ++t) sum += hcost(rank[t]);
+  part[tid] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const i64 v = tid >= o ? part[tid - o] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  i64 base = tid ? part[tid - 1] : 0;
+  for (i64 t = a; t < b; ++t) {
+    base += hcost(rank[t]);
+    pre[t] = base;
+  }
+}
+
+__global__ void k_hflags_lv(i64 T, LevelPtrs lp) {
+  const i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  int I, J;
+  tile_ij(t, I, J);
+  lp.rend[blockIdx.y][t] = I == J ? 1 : 0;
+}
+
+__global__ void k_hranges_lv(i64 T, int W, LevelPtrs lp) {
+  const i64 w = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  const int l = blockIdx.y;
+  if (w > W) return;
+  int* wstart = lp.wstart[l];
+  if (w == 0) {
+    wstart[0] = 0;
+    return;
+  }
+  if (w == W) {
+    wstart[W] = (int)T;
+    return;
+  }
+  const i64* pre = lp.pre[l];
+  const i64 target = (i64)((double)pre[T - 1] * (double)w / (double)W);
+  i64 lo = 0, hi = T;  // first t with pre[t] > target
+  while (lo < hi) {
+    const i64 mid = (lo + hi) / 2;
+    if (pre[mid] > target)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  wstart[w] = (int)lo;
+  if (lo > 0) lp.rend[l][lo - 1] = 1;
+}
+
 }  // namespace
 
 namespace dml {
@@ -757,6 +827,27 @@ void hier_ranges(i64 n, const int* rank, int W, void* buf, HierRanges& hr, hipSt
   k_hscan<<<1, 1024, 0, s>>>(T, rank, pre);
   k_hflags<<<(unsigned)((T + 255) / 256), 256, 0, s>>>(T, hr.rend);
   k_hranges<<<(unsigned)((W + 1 + 255) / 256), 256, 0, s>>>(T, W, pre, hr.wstart, hr.rend);
+  HIP_OK(hipGetLastError());
+}
+
+void hier_ranges_levels(i64 n, int nl, const int* const* ranks, int W, void* const* bufs, HierRanges* hr,
+                        hipStream_t s) {
+  const i64 T = packed_tiles(n);
+  DM_REQUIRE(W >= 1 && W < (1 << 30) && nl >= 1 && nl <= kMaxLevels, DM_ERR_ARGUMENT, "wave ranges out of range");
+  LevelPtrs lp;
+  for (int l = 0; l < nl; ++l) {
+    i64* pre = reinterpret_cast<i64*>(bufs[l]);
+    hr[l].rend = reinterpret_cast<unsigned char*>(pre + T);
+    hr[l].wstart = reinterpret_cast<int*>(reinterpret_cast<uintptr_t>(hr[l].rend + T + 15) & ~uintptr_t(15));
+    hr[l].W = W;
+    lp.rank[l] = ranks[l];
+    lp.pre[l] = pre;
+    lp.rend[l] = hr[l].rend;
+    lp.wstart[l] = hr[l].wstart;
+  }
+  k_hscan_lv<<<nl, 1024, 0, s>>>(T, lp);
+  k_hflags_lv<<<dim3((unsigned)((T + 255) / 256), nl), 256, 0, s>>>(T, lp);
+  k_hranges_lv<<<dim3((unsigned)((W + 1 + 255) / 256), nl), 256, 0, s>>>(T, W, lp);
   HIP_OK(hipGetLastError());
 }
 

@@ -282,12 +282,12 @@ int dm_solve(dm_ctx* ctx, const dm_zmat* z, const double* B, int64_t ldb, int64_
  * rank of the low-rank tiles, [3] bytes a product reads relative to DM_Z32P. */
 int dm_zmat_compress(dm_zmat* z, double eps, int rmax, double* stats);
 /* No reference counterpart.  The compression's looser levels: the steps of
- * one cross approximation are nested, so level l (1..5) reuses the prefix of
- * each tile's factors at which its residual fell below 10^l times the tile
- * tolerance (error bound eps 10^l; some tiles dense at eps are low-rank
- * there).  Sets the level that dm_apply products on DM_Z32H use (0 = eps) and,
+ * one cross approximation are nested, so level l (1..10) reuses the prefix of
+ * each tile's factors at which its residual fell below 10^(l/2) times the
+ * tile tolerance (error bound eps 10^(l/2); some tiles dense at eps are
+ * low-rank there).  Sets the level that dm_apply products on DM_Z32H use (0 = eps) and,
  * with stats (5 doubles, may be NULL), reports it: the four dm_zmat_compress
- * statistics at that level, then its error bound eps 10^l.  Solves choose
+ * statistics at that level, then its error bound eps 10^(l/2).  Solves choose
  * their own levels (dm_solve_opts.relax). */
 int dm_zmat_hlevel(dm_zmat* z, int level, double* stats);
 
