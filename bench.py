@@ -325,6 +325,40 @@ def graph_probe(dm):
               flush=True)
 
 
+def relax_probe(dm):
+    """The chosen form's relaxation constant swept on the 30720-basis sphere:
+    each (relax, first inner tolerance) pair solved 4 times, best and median
+    wall time, iterations, refinements and true residual of the last solve;
+    the unrelaxed solve first and last."""
+    k = 2 * math.pi / 0.1
+    m = dm.make_icosphere(0.1, 5)
+    r = dm.build_rwg(m)
+    P = dm.build_nearfield_preconditioner(None, m, r, 2.0 * float(np.mean(r.len)))
+    pw = [dm.make_plane_wave([0, 0, -k], 1.0, [1, 0, 0])]
+    ctx = dm.default_context()
+
+    def run(c, itol):
+        ts = []
+        try:
+            for _ in range(4):
+                ctx.sync()
+                t0 = time.perf_counter()
+                _, st, _ = dm.solve_planewaves(m, r, k, pw, "mixed", P, tol=1e-6, maxiter=600, batch=-1,
+                                               compress_tol=5e-8, relax=c, inner_tol=itol)
+                ctx.sync()
+                ts.append(time.perf_counter() - t0)
+            ts.sort()
+            print(f"relax {c:6.3f} inner_tol {itol:.1e}: best {1e3 * ts[0]:7.2f} median {1e3 * ts[len(ts) // 2]:7.2f} "
+                  f"ms  {st.niter} it  {st.outer} ref  res {st.residual:.2e}", flush=True)
+        except Exception as e:  # a constant too loose may not converge within maxiter
+            print(f"relax {c:6.3f} inner_tol {itol:.1e}: failed: {e}", flush=True)
+
+    run(0.0, 0.0)
+    for c, itol in ((0.003, 3e-7), (0.01, 3e-7), (0.02, 3e-7), (0.01, 0.0), (0.02, 2e-7), (0.05, 2e-7)):
+        run(c, itol)
+    run(0.0, 0.0)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -344,6 +378,8 @@ def main():
                     help="chosen form: relaxation constant of the inner products (0: off)")
     ap.add_argument("--inner-tol", type=float, default=3e-7,
                     help="chosen form: relative tolerance of the first inner solve")
+    ap.add_argument("--relax-probe", action="store_true",
+                    help="sweep the chosen form's relaxation constant on the sphere, then exit")
     ap.add_argument("--graph-probe", action="store_true",
                     help="time Krylov steps launched vs replayed from a captured HIP graph, then exit")
     ap.add_argument("--kernel-probe", choices=["assembly", "zx"], default=None,
@@ -365,6 +401,8 @@ def main():
         return kernel_probe(dm, a.subdiv, a.kernel_probe)
     if a.graph_probe:
         return graph_probe(dm)
+    if a.relax_probe:
+        return relax_probe(dm)
     k = 2 * math.pi / 0.1
     mesh = dm.make_icosphere(0.1, a.subdiv)
     rwg = dm.build_rwg(mesh)
