@@ -606,10 +606,12 @@ def main():
             ach = work / avg_s / (1e9 if unit == "GB/s" else 1e12)
             fracs[name] = dict(bound=bound, achieved=round(ach, 2), peak=peak, unit=unit, frac=round(ach / peak, 4),
                                work_per_launch=work, avg_ms=kern[name]["avg_ms"])
-    chosen_k = {n: v for n, v in kern.items() if not n.startswith("plain_")}
+    chosen_k = {n: dict(v) for n, v in kern.items() if not n.startswith("plain_")}
+    if "symv_z32h_relaxed" in chosen_k and "symv_z32h" in chosen_k:
+        # one kernel at two kinds of levels: its total decides dominance; the roofline
+        # is taken over its launches at the compression's own level (exact bytes)
+        chosen_k["symv_z32h"]["total_ms"] += chosen_k.pop("symv_z32h_relaxed")["total_ms"]
     dom = max(chosen_k, key=lambda n: chosen_k[n]["total_ms"])
-    if dom == "symv_z32h_relaxed" and "symv_z32h" in fracs:
-        dom = "symv_z32h"  # the same kernel; its launches at the compression's own level have exact bytes
     if dom not in fracs:
         dom = "symv_z32" if "symv_z32" in fracs else "gemv_z32"
     roof = fracs[dom]
