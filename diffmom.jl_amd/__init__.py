@@ -214,8 +214,11 @@ class Context:
         _check(lib().dm_sync(self.h))
 
     def profile(self, on=True):
-        """Bracket tracked kernel launches with HIP events on the library's stream."""
-        _check(lib().dm_profile_enable(self.h, int(on)))
+        """Bracket tracked kernel launches with HIP events on the library's
+        stream: True (every tracked launch), "products" (only the operator
+        products, "symv*" / "gemv*"), False (off)."""
+        mode = 2 if on == "products" else (1 if on else 0)
+        _check(lib().dm_profile_enable(self.h, mode))
 
     def profile_reset(self):
         _check(lib().dm_profile_reset(self.h))

@@ -479,7 +479,8 @@ int dm_profile_enable(dm_ctx* ctx, int on) {
   return guard([&] {
     check_ctx(ctx, "context");
     CtxScope sc(ctx);
-    ctx->prof = on != 0;
+    DM_REQUIRE(on >= 0 && on <= 2, DM_ERR_ARGUMENT, "profile mode must be 0 (off), 1 (all) or 2 (products)");
+    ctx->prof = on;
   });
 }
 

@@ -3,6 +3,8 @@
 #include <cmath>
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -117,7 +119,7 @@ struct dm_ctx {
   hipStream_t stream2 = nullptr;  // second stream for overlapped work
   std::mutex mu;
   int cu_count = 256;
-  bool prof = false;              // record HIP events around tracked launches
+  int prof = 0;  // HIP events around tracked launches: 0 none, 1 all, 2 operator products only
   std::vector<ProfRec> recs;
   std::vector<hipEvent_t> ev_free;  // recycled profiling events
   std::map<std::string, int> tune;  // launch parameters chosen by the tuner, by shape key
@@ -140,7 +142,10 @@ struct ProfScope {
   hipStream_t s;
   ProfRec rec;
   bool on;
-  ProfScope(dm_ctx* c_, const char* name, hipStream_t s_) : c(c_), s(s_), on(c_ && c_->prof) {
+  ProfScope(dm_ctx* c_, const char* name, hipStream_t s_)
+      : c(c_), s(s_),
+        on(c_ && (c_->prof == 1 ||
+                  (c_->prof == 2 && (std::strncmp(name, "symv", 4) == 0 || std::strncmp(name, "gemv", 4) == 0)))) {
     if (!on) return;
     rec.name = name;
     rec.a = c->ev_get();

@@ -112,8 +112,8 @@ hipGraphExec_t step_graph(dm_ctx* ctx, i64 n, DevOp& A, DevPrec* M, int side, in
     if (g.kb == kb) return g.exec;
   const bool right = M && side == 1, left = M && side == 0;
   const i64 ldv = n;  // complex columns
-  const bool prof = ctx->prof;
-  ctx->prof = false;  // no profiling events inside the graph
+  const int prof = ctx->prof;
+  ctx->prof = 0;  // no profiling events inside the graph
   hipGraph_t graph = nullptr;
   {
     HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));

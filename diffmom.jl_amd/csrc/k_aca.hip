@@ -109,12 +109,19 @@ __device__ inline float wave_sum_f32(float x) {
 __global__ __launch_bounds__(kAcaThreads, 4) void k_aca(int nb, const float2* __restrict__ zp,
                                                      const double* __restrict__ tau2p, int rmax, int* __restrict__ rank,
                                                      int* __restrict__ lvrank, i64 T, float2* __restrict__ fac) {
-  const i64 t = blockIdx.x;
+  // Workgroups take the tiles by distance from the diagonal, d = J - I
+  // ascending (S(d) = d nb - d (d - 1) / 2 tiles are nearer): the tiles
+  // near the diagonal, which need the most steps, start first and the cheap
+  // far tiles fill the end of the launch.
   const double tau2 = tau2p[0];
-  int J = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-  while ((i64)J * (J + 1) / 2 > t) --J;
-  while ((i64)(J + 1) * (J + 2) / 2 <= t) ++J;
-  const int I = (int)(t - (i64)J * (J + 1) / 2);
+  const i64 b = blockIdx.x;
+  auto S = [&](i64 dd) { return dd * nb - dd * (dd - 1) / 2; };
+  const double A = (double)nb + 0.5;
+  int d = (int)(A - sqrt(A * A - 2.0 * (double)b));
+  while (d > 0 && S(d) > b) --d;
+  while (S(d + 1) <= b) ++d;
+  const int I = (int)(b - S(d)), J = I + d;
+  const i64 t = (i64)J * (J + 1) / 2 + I;
   const int tid = threadIdx.x;
   if (I == J) {
     if (tid == 0) {

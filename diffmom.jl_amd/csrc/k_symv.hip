@@ -469,22 +469,105 @@ __global__ __launch_bounds__(256) void k_symv_w(i64 n, int nb, int tpw, const vo
 // fac holds U then V, each rmax columns of 128 complex64).  Dense tiles
 // (rank[t] < 0, always the diagonal ones) are read from the packed matrix.
 // For a low-rank tile, with s = U^T x_I and r = V^T x_J:
-//   y_I += U r,  y_J += V s,
-// the factor columns taken 4 at a time (one coalesced 1 KB read each), the
-// 4 + 4 dot products reduced by the butterfly and broadcast with readlane.
-// Column sums are kept in this lane's row layout (columns 2 lane, 2 lane + 1);
-// the dense tiles' butterfly-layout sums are moved there with one gather at
-// each flush.  Waves take contiguous ranges of stored tiles that carry about
-// the same number of bytes (wstart, from the tiles' ranks), so dense tiles
-// and long factor lists do not leave some waves with several times the work
-// of others.
+//   y_I += U r,  y_J += V s.
+// Waves take contiguous ranges of stored tiles that carry about the same
+// number of bytes (wstart, from the tiles' ranks).  A wave's tiles form one
+// stream of groups: four U and four V columns of a low-rank tile, or eight
+// columns of a dense tile (1 KB each, one coalesced 16-byte load per lane).
+// The next group's eight loads (possibly of the next tile) and the next
+// tile's x rows are issued unconditionally before the current group is
+// reduced, into the other of two register sets: no load sits behind a
+// branch, so the compiler's wait counts let them stay in flight.  Columns
+// past a tile's rank re-read its last column and drop out through the
+// broadcast sums.  The eight dot products of a group (or the eight column
+// sums of a dense group) are reduced by one transposing butterfly on the
+// cross-half permlane swaps and DPP row moves, then broadcast with readlane.
+// Row sums go to rowpart per tile; column sums are kept in this lane's row
+// layout (columns 2 lane, 2 lane + 1) and flushed to colpart at the end of a
+// column run (the diagonal tile) or of the wave's range.  Both partials are
+// stored lane-major (row 2 l + e at position 64 e + l), so each store
+// instruction writes 1 KB contiguously.
 __device__ inline double2 cfma(double2 acc, double2 a, double2 b) {
   acc.x += a.x * b.x - a.y * b.y;
   acc.y += a.x * b.y + a.y * b.x;
   return acc;
 }
 
+// acc += a b (complex), four FMAs
+__device__ inline void cmac(double2& acc, double2 a, double2 b) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
+}
+
 __device__ inline double2 bcast_lane(double2 v, int l) { return make_double2(rl(v.x, l), rl(v.y, l)); }
+
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int CTRL>
+__device__ inline void dpp_acc(double2& a) {
+  a.x += dpp_f64<CTRL>(a.x);
+  a.y += dpp_f64<CTRL>(a.y);
+}
+
+// v_permlane32_swap / v_permlane16_swap on one double: a keeps its lower
+// half-wave (row pair) and takes b's, b takes a's upper one.
+__device__ inline void swap32(double& a, double& b) {
+  const long long A = __double_as_longlong(a), B = __double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)(A & 0xffffffffLL), (unsigned)(B & 0xffffffffLL), false,
+                                                   false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(A >> 32), (unsigned)(B >> 32), false, false);
+  a = __longlong_as_double(((long long)hi[0] << 32) | (unsigned long long)lo[0]);
+  b = __longlong_as_double(((long long)hi[1] << 32) | (unsigned long long)lo[1]);
+}
+__device__ inline void swap16(double& a, double& b) {
+  const long long A = __double_as_longlong(a), B = __double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)(A & 0xffffffffLL), (unsigned)(B & 0xffffffffLL), false,
+                                                   false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(A >> 32), (unsigned)(B >> 32), false, false);
+  a = __longlong_as_double(((long long)hi[0] << 32) | (unsigned long long)lo[0]);
+  b = __longlong_as_double(((long long)hi[1] << 32) | (unsigned long long)lo[1]);
+}
+// lanes 0..31: a(l) + a(l + 32); lanes 32..63: b(l - 32) + b(l)
+__device__ inline double2 half32(double2 a, double2 b) {
+  swap32(a.x, b.x);
+  swap32(a.y, b.y);
+  return make_double2(a.x + b.x, a.y + b.y);
+}
+// lanes with bit 4 clear: a(l) + a(l ^ 16); the others: b(l) + b(l ^ 16)
+__device__ inline double2 half16(double2 a, double2 b) {
+  swap16(a.x, b.x);
+  swap16(a.y, b.y);
+  return make_double2(a.x + b.x, a.y + b.y);
+}
+
+// The same result as butterfly8 (p[0] of lane l: the sum over all lanes of
+// value l >> 3) without LDS: the xor-32 and xor-16 stages on permlane swaps,
+// xor 8 as a row rotation by 8, the last three on quad permutes and the
+// half-row mirror (every lane of an 8-lane group ends with the group's sum).
+__device__ inline void butterfly8x(double2 (&p)[8], int lane) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) p[i] = half32(p[i], p[i + 4]);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) p[i] = half16(p[i], p[i + 2]);
+  {
+    const bool h3 = lane & 8;
+    double2 keep = h3 ? p[1] : p[0];
+    const double2 send = h3 ? p[0] : p[1];
+    keep.x += dpp_f64<0x128>(send.x);  // row_ror 8
+    keep.y += dpp_f64<0x128>(send.y);
+    p[0] = keep;
+  }
+  dpp_acc<0xB1>(p[0]);   // quad_perm 1,0,3,2
+  dpp_acc<0x4E>(p[0]);   // quad_perm 2,3,0,1
+  dpp_acc<0x141>(p[0]);  // row_half_mirror
+}
 
 __global__ __launch_bounds__(256, 2) void k_symv_h(i64 n, int nb, int W, const int* __restrict__ wstart,
                                                 const void* __restrict__ zp, const int* __restrict__ rank,
@@ -493,96 +576,178 @@ __global__ __launch_bounds__(256, 2) void k_symv_h(i64 n, int nb, int W, const i
                                                 double2* __restrict__ colpart) {
   constexpr i64 kTileF4 = (i64)kPT * kPT * sizeof(float2) / 16;
   const int lane = threadIdx.x & 63;
-  const i64 wave = (i64)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int wave = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   if (wave >= W) return;
-  // this wave's range of stored tiles (about the same bytes for every wave)
-  const i64 t0 = wstart[wave], t1 = wstart[wave + 1];
+  const int t0 = wstart[wave], t1 = wstart[wave + 1];
   if (t0 >= t1) return;
+  const f4v* Z4 = reinterpret_cast<const f4v*>(zp);
+  const f4v* F4 = reinterpret_cast<const f4v*>(fac);
+  const i64 slot = (i64)rmax * kPT;  // 16-byte words per factor slot
   int I, J;
   tile_ij(t0, I, J);
+  I = __builtin_amdgcn_readfirstlane(I);
+  J = __builtin_amdgcn_readfirstlane(J);
   double2 accC0 = make_double2(0.0, 0.0), accC1 = make_double2(0.0, 0.0);
   double2 accJ[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
-  for (i64 t = t0; t < t1; ++t) {
-    double2 xI[2], xJ[2], accR[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+  double2 accR[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+  double2 xI[2], xJ[2], xIn[2];
+  // x rows 2 lane, 2 lane + 1 of row block i (zero past n; the load itself
+  // is clamped so that it is issued unconditionally)
+  auto xload = [&](int i, double2(&o)[2]) {
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
-      const i64 gi = (i64)I * kPT + 2 * lane + e, gj = (i64)J * kPT + 2 * lane + e;
-      xI[e] = gi < n ? x[gi] : make_double2(0.0, 0.0);
-      xJ[e] = gj < n ? x[gj] : make_double2(0.0, 0.0);
+      const i64 g = (i64)i * kPT + 2 * lane + e;
+      const double2 v = x[min(g, n - 1)];
+      o[e] = g < n ? v : make_double2(0.0, 0.0);
     }
-    const int rk = rank[t];
-    if (rk < 0) {
-      dense_tile<1>(reinterpret_cast<const f4v*>(zp) + t * kTileF4, lane, I != J, xI, xJ, accR, accC0, accC1);
-    } else {
-      const f4v* U = reinterpret_cast<const f4v*>(fac + t * 2 * kPT * (i64)rmax);
-      const f4v* V = U + (i64)rmax * kPT / 2;
-      // factor columns 4 at a time, the next group's loads issued before the
-      // current group's butterflies (two register sets, ping-pong)
-      auto fload = [&](int k0, f4v(&u)[4], f4v(&v)[4]) {
+  };
+  // the eight loads of group k of tile t (rank rk; -1: dense, columns
+  // k .. k + 7; rank 0: four finite columns of the dense tile, all dropped)
+  auto gload = [&](int t, int k, int rk, f4v(&a)[4], f4v(&b)[4]) {
+    const f4v* pa = Z4 + (i64)t * kTileF4;
+    const f4v* pb = pa;
+    int ca = 0, cb = 0, cl = 3;
+    if (rk > 0) {
+      pa = F4 + (i64)t * slot;
+      pb = pa + slot / 2;
+      ca = cb = k;
+      cl = rk - 1;
+    } else if (rk < 0) {
+      ca = k;
+      cb = k + 4;
+      cl = kPT - 1;
+    }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          if (k0 + c < rk) {
-            u[c] = __builtin_nontemporal_load(U + (k0 + c) * (kPT / 2) + lane);
-            v[c] = __builtin_nontemporal_load(V + (k0 + c) * (kPT / 2) + lane);
-          } else {
-            u[c] = f4v{0.f, 0.f, 0.f, 0.f};
-            v[c] = f4v{0.f, 0.f, 0.f, 0.f};
-          }
-        }
-      };
-      auto fstep = [&](const f4v(&u)[4], const f4v(&v)[4]) {
-        double2 ps[4], pr[4];
+    for (int c = 0; c < 4; ++c) {
+      a[c] = __builtin_nontemporal_load(pa + (i64)min(ca + c, cl) * (kPT / 2) + lane);
+      b[c] = __builtin_nontemporal_load(pb + (i64)min(cb + c, cl) * (kPT / 2) + lane);
+    }
+  };
+  // factor columns k .. k + 3 (cnt of them below the rank)
+  auto lowrank = [&](const f4v(&u)[4], const f4v(&v)[4], int cnt) {
+    double2 p[8];  // p[c]: U_c . x_I over this lane's rows; p[4 + c]: V_c . x_J
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const double2 u0 = make_double2(u[c][0], u[c][1]), u1 = make_double2(u[c][2], u[c][3]);
-          const double2 v0 = make_double2(v[c][0], v[c][1]), v1 = make_double2(v[c][2], v[c][3]);
-          ps[c] = cfma(cfma(make_double2(0.0, 0.0), u0, xI[0]), u1, xI[1]);
-          pr[c] = cfma(cfma(make_double2(0.0, 0.0), v0, xJ[0]), v1, xJ[1]);
-        }
-        butterfly4(ps, lane);
-        butterfly4(pr, lane);
+    for (int c = 0; c < 4; ++c) {
+      p[c] = p[4 + c] = make_double2(0.0, 0.0);
+      cmac(p[c], make_double2(u[c][0], u[c][1]), xI[0]);
+      cmac(p[c], make_double2(u[c][2], u[c][3]), xI[1]);
+      cmac(p[4 + c], make_double2(v[c][0], v[c][1]), xJ[0]);
+      cmac(p[4 + c], make_double2(v[c][2], v[c][3]), xJ[1]);
+    }
+    butterfly8x(p, lane);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const double2 sc = bcast_lane(ps[0], 16 * c), rc = bcast_lane(pr[0], 16 * c);
-          accR[0] = cfma(accR[0], make_double2(u[c][0], u[c][1]), rc);
-          accR[1] = cfma(accR[1], make_double2(u[c][2], u[c][3]), rc);
-          accJ[0] = cfma(accJ[0], make_double2(v[c][0], v[c][1]), sc);
-          accJ[1] = cfma(accJ[1], make_double2(v[c][2], v[c][3]), sc);
-        }
-      };
-      f4v ua[4], va[4], ub[4], vb[4];
-      if (rk > 0) fload(0, ua, va);
-#pragma unroll 1
-      for (int k0 = 0; k0 < rk; k0 += 8) {
-        if (k0 + 4 < rk) fload(k0 + 4, ub, vb);
-        fstep(ua, va);
-        if (k0 + 4 >= rk) break;
-        if (k0 + 8 < rk) fload(k0 + 8, ua, va);
-        fstep(ub, vb);
+    for (int c = 0; c < 4; ++c) {
+      double2 sc = bcast_lane(p[0], 8 * c), rc = bcast_lane(p[0], 32 + 8 * c);
+      if (c >= cnt) sc = rc = make_double2(0.0, 0.0);
+      cmac(accR[0], make_double2(u[c][0], u[c][1]), rc);
+      cmac(accR[1], make_double2(u[c][2], u[c][3]), rc);
+      cmac(accJ[0], make_double2(v[c][0], v[c][1]), sc);
+      cmac(accJ[1], make_double2(v[c][2], v[c][3]), sc);
+    }
+  };
+  // dense columns 8 b .. 8 b + 7; the column sums end in lanes 8 c + (b & 7)
+  // (column 8 b + c), accC0 for b < 8, accC1 after
+  auto dense8 = [&](int b, bool offdiag, const f4v(&a)[4], const f4v(&c4)[4]) {
+    double2 p[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const f4v& q = c < 4 ? a[c] : c4[c - 4];
+      const double2 z0 = make_double2(q[0], q[1]), z1 = make_double2(q[2], q[3]);
+      const double2 xc = bcast<1>(xJ, 8 * b + c);
+      cmac(accR[0], z0, xc);
+      cmac(accR[1], z1, xc);
+      p[c] = make_double2(0.0, 0.0);
+      cmac(p[c], z0, xI[0]);
+      cmac(p[c], z1, xI[1]);
+    }
+    if (offdiag) {
+      butterfly8x(p, lane);
+      if ((lane & 7) == (b & 7)) {
+        if (b < 8)
+          cadd(accC0, p[0]);
+        else
+          cadd(accC1, p[0]);
       }
     }
+  };
+  auto finish = [&](int t, bool flush) {
 #pragma unroll
-    for (int e = 0; e < 2; ++e) rowpart[t * kPT + 2 * lane + e] = accR[e];
-    if (I == J || t + 1 == t1) {
+    for (int e = 0; e < 2; ++e) rowpart[(i64)t * kPT + 64 * e + lane] = accR[e];
+    if (flush) {
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
-        const int c = 2 * lane + e, cm = c & 63;
-        const int src = ((cm & 7) << 3) | (cm >> 3);
+        const int c = 2 * lane + e;
+        const int src = 8 * (c & 7) + ((c >> 3) & 7);
         const double2 g0 = make_double2(__shfl(accC0.x, src, 64), __shfl(accC0.y, src, 64));
         const double2 g1 = make_double2(__shfl(accC1.x, src, 64), __shfl(accC1.y, src, 64));
         double2 v = c < 64 ? g0 : g1;
         cadd(v, accJ[e]);
-        colpart[t * kPT + c] = v;
+        colpart[(i64)t * kPT + 64 * e + lane] = v;
       }
       accC0 = accC1 = make_double2(0.0, 0.0);
       accJ[0] = accJ[1] = make_double2(0.0, 0.0);
     }
-    if (I == J) {
-      ++J;
-      I = 0;
-    } else {
-      ++I;
+  };
+
+  int tc = t0, kc = 0;
+  int rkc = __builtin_amdgcn_readfirstlane(rank[tc]);
+  f4v Au[4], Av[4], Bu[4], Bv[4];
+  xload(I, xI);
+  xload(J, xJ);
+  gload(tc, kc, rkc, Au, Av);
+  bool done = false;
+  // one group from (cu, cv); the next group's loads go to (nu, nv) first
+  auto step = [&](const f4v(&cu)[4], const f4v(&cv)[4], f4v(&nu)[4], f4v(&nv)[4]) {
+    const bool dense = rkc < 0;
+    const int lim = dense ? kPT : max(rkc, 1);
+    const int adv = dense ? 8 : 4;
+    const bool newtile = kc + adv >= lim;
+    const bool endr = newtile && tc + 1 >= t1;
+    int tn = tc, kn = kc + adv, rkn = rkc;
+    if (newtile) {
+      kn = kc;  // at the end of the range: the same group again (loads unused)
+      if (!endr) {
+        tn = tc + 1;
+        kn = 0;
+        rkn = __builtin_amdgcn_readfirstlane(rank[tn]);
+      }
     }
+    int In = I, Jn = J;
+    if (newtile && !endr) {
+      if (I == J) {
+        Jn = J + 1;
+        In = 0;
+      } else {
+        In = I + 1;
+      }
+    }
+    xload(In, xIn);
+    gload(tn, kn, rkn, nu, nv);
+    if (dense)
+      dense8(kc >> 3, I != J, cu, cv);
+    else
+      lowrank(cu, cv, rkc - kc);
+    if (newtile) {
+      finish(tc, I == J || endr);
+      accR[0] = accR[1] = make_double2(0.0, 0.0);
+      const bool newcol = Jn != J;
+      I = In;
+      J = Jn;
+      if (newcol) xload(J, xJ);
+    }
+    xI[0] = xIn[0];
+    xI[1] = xIn[1];
+    tc = tn;
+    kc = kn;
+    rkc = rkn;
+    done = endr;
+  };
+#pragma unroll 1
+  for (;;) {
+    step(Au, Av, Bu, Bv);
+    if (done) break;
+    step(Bu, Bv, Au, Av);
+    if (done) break;
   }
 }
 
@@ -591,14 +756,15 @@ __global__ __launch_bounds__(256, 2) void k_symv_h(i64 n, int nb, int W, const i
 // (a run ends at tile t = R (R + 1) / 2 + I when t + 1 is a multiple of tpw
 // or I = R); fixed order, deterministic.
 __global__ __launch_bounds__(256) void k_symv_w_reduce(i64 n, int nb, int tpw, const unsigned char* __restrict__ rend,
-                                                       const double2* __restrict__ rowpart,
+                                                       int lanemajor, const double2* __restrict__ rowpart,
                                                        const double2* __restrict__ colpart, double2* __restrict__ y) {
   // 32 rows per block, 8 groups sharing the tiles of a row block round robin;
   // the groups' sums are added in a fixed order
   const int rl = threadIdx.x & 31, grp = threadIdx.x >> 5;
   const i64 g = blockIdx.x * (i64)32 + rl;
   const int R = (int)min((i64)(nb - 1), (blockIdx.x * (i64)32) / kPT);
-  const int r = (int)(g % kPT);
+  // partial of row r: at r, or at 64 (r & 1) + (r >> 1) in the lane-major layout
+  const int r = lanemajor ? (int)(((g & 1) << 6) + ((g % kPT) >> 1)) : (int)(g % kPT);
   double sr = 0, si = 0;
   if (g < n) {
     int J = R + grp;
@@ -862,7 +1028,7 @@ void symv_hier(i64 n, const float* z32p, const int* rank, const float* fac, int 
   k_symv_h<<<(unsigned)blocks, 256, 0, s>>>(n, (int)nb, hr.W, hr.wstart, z32p, rank,
                                             reinterpret_cast<const float2*>(fac), rmax,
                                             reinterpret_cast<const double2*>(x), rowpart, colpart);
-  k_symv_w_reduce<<<(unsigned)((n + 31) / 32), 256, 0, s>>>(n, (int)nb, 1, hr.rend, rowpart, colpart,
+  k_symv_w_reduce<<<(unsigned)((n + 31) / 32), 256, 0, s>>>(n, (int)nb, 1, hr.rend, 1, rowpart, colpart,
                                                             reinterpret_cast<double2*>(y));
   HIP_OK(hipGetLastError());
 }
@@ -908,7 +1074,7 @@ void symv_packed(i64 n, const ZRef& z, const double* x, double* y, void* ws, int
       k_symv_w<1><<<(unsigned)blocks, 256, 0, s>>>(n, (int)nb, tpw, z.p, xv, rowpart, colpart);
     else
       k_symv_w<0><<<(unsigned)blocks, 256, 0, s>>>(n, (int)nb, tpw, z.p, xv, rowpart, colpart);
-    k_symv_w_reduce<<<(unsigned)((n + 31) / 32), 256, 0, s>>>(n, (int)nb, tpw, nullptr, rowpart, colpart,
+    k_symv_w_reduce<<<(unsigned)((n + 31) / 32), 256, 0, s>>>(n, (int)nb, tpw, nullptr, 0, rowpart, colpart,
                                                               reinterpret_cast<double2*>(y));
     HIP_OK(hipGetLastError());
     return;

@@ -76,9 +76,11 @@ int dm_version(void);
 int dm_sync(dm_ctx* ctx);
 
 /* Kernel timing on the library's own streams (no reference counterpart; for
- * benchmarks): when enabled, tracked launches ("assemble_z64", "assemble_z32",
- * "near_tables", "gemv_z64", "gemv_z32", "arnoldi", "precond") are bracketed
- * with HIP events; dm_profile_read sums the elapsed times of one name. */
+ * benchmarks): tracked launches ("assemble_z64", "assemble_z32",
+ * "near_tables", "gemv_z64", "symv_z32h", "arnoldi", "precond", ...) are
+ * bracketed with HIP events; dm_profile_read sums the elapsed times of one
+ * name.  on: 0 off, 1 every tracked launch, 2 only the operator products
+ * (names "symv*" and "gemv*"; fewer events in a timed solve). */
 int dm_profile_enable(dm_ctx* ctx, int on);
 int dm_profile_reset(dm_ctx* ctx);
 int dm_profile_read(dm_ctx* ctx, const char* name, int64_t* count, double* total_ms);
