@@ -92,11 +92,11 @@ size_t zx_bf16_workspace_bytes(i64 n, int R);
 std::vector<int> zx_bf16_chunks();
 void zx_bf16(i64 n, i64 ld, const void* planes, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
              hipStream_t s, int kc = 0);
-// Y (mr x R) = A (mr x n, lda) X (n x R), complex128 on FP64 MFMA; ws:
-// gemm_c128_workspace_bytes(n, R).
+// Y (mr x R) = A (mr x n, lda) X (n x R) (subtract: Y -= A X), complex128 on
+// FP64 MFMA; ws: gemm_c128_workspace_bytes(n, R).
 size_t gemm_c128_workspace_bytes(i64 n, int R);
 void gemm_c128(i64 mr, i64 n, const double* A, i64 lda, const double* X, i64 ldx, int R, double* Y, i64 ldy,
-               void* ws, hipStream_t s);
+               void* ws, hipStream_t s, bool subtract = false);
 void zx_mfma64(i64 n, i64 ld, const double* z64, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
                hipStream_t s);
 }  // namespace dml

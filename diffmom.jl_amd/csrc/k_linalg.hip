@@ -441,62 +441,127 @@ __global__ void k_diag_apply(i64 n, const double2* __restrict__ d, const double2
 }
 
 // ------------------------------------------------------------ dense LU
-// Right-looking LU with partial pivoting, one column step per pair of
-// launches (pivot+swap+scale, then rank-1 update).  Used for the dense direct
-// method (solve_forward :direct), sized for N up to a few thousand.
-__global__ __launch_bounds__(1024) void k_lu_pivot(i64 n, i64 ld, int k, double2* __restrict__ A, int* __restrict__ ipiv,
-                                                   int* status) {
+// Blocked right-looking LU with partial pivoting (the dense direct method,
+// solve_forward :direct), panels of kLuW columns: one workgroup factors the
+// panel (pivot search, row swap inside the panel, scaling and the panel's own
+// rank-1 updates, column by column), k_lu_laswp applies the panel's swaps to
+// the other columns, k_lu_trsm forms U12 = L11^-1 A12, and the trailing
+// matrix takes A22 -= L21 U12 as one complex128 GEMM on the FP64 matrix cores
+// (gemm_c128, subtract mode).  Pivots: the largest |a|^2 in the column, the
+// smallest row index among equal ones.
+constexpr int kLuW = 32;
+
+__global__ __launch_bounds__(1024) void k_lu_panel(i64 n, i64 ld, int k0, int w, double2* __restrict__ A,
+                                                   int* __restrict__ ipiv, int* __restrict__ status) {
   __shared__ double best[1024];
   __shared__ int bidx[1024];
-  double m = -1;
-  int p = k;
-  for (i64 i = k + threadIdx.x; i < n; i += 1024) {
-    double2 v = A[i + (i64)k * ld];
-    double a = v.x * v.x + v.y * v.y;
-    if (a > m) { m = a; p = (int)i; }
-  }
-  best[threadIdx.x] = m;
-  bidx[threadIdx.x] = p;
-  __syncthreads();
-  for (int s = 512; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      if (best[threadIdx.x + s] > best[threadIdx.x] ||
-          (best[threadIdx.x + s] == best[threadIdx.x] && bidx[threadIdx.x + s] < bidx[threadIdx.x])) {
-        best[threadIdx.x] = best[threadIdx.x + s];
-        bidx[threadIdx.x] = bidx[threadIdx.x + s];
+  __shared__ double2 prow[kLuW];
+  const int tid = threadIdx.x;
+  for (int j = 0; j < w; ++j) {
+    const i64 c = (i64)k0 + j;
+    double m = -1;
+    int p = (int)c;
+    for (i64 i = c + tid; i < n; i += 1024) {
+      const double2 v = A[i + c * ld];
+      const double a = v.x * v.x + v.y * v.y;
+      if (a > m) {
+        m = a;
+        p = (int)i;
+      }
+    }
+    best[tid] = m;
+    bidx[tid] = p;
+    __syncthreads();
+    for (int st = 512; st > 0; st >>= 1) {
+      if (tid < st) {
+        if (best[tid + st] > best[tid] || (best[tid + st] == best[tid] && bidx[tid + st] < bidx[tid])) {
+          best[tid] = best[tid + st];
+          bidx[tid] = bidx[tid + st];
+        }
+      }
+      __syncthreads();
+    }
+    const bool sing = !(best[0] > 0.0);
+    const int pv = sing ? (int)c : bidx[0];
+    __syncthreads();  // every thread has read best[0] / bidx[0]
+    if (tid == 0) {
+      ipiv[c] = pv;
+      if (sing && status[0] == 0) status[0] = (int)c + 1;
+    }
+    if (pv != c && tid < w) {
+      const i64 col = (i64)k0 + tid;
+      const double2 t = A[c + col * ld];
+      A[c + col * ld] = A[pv + col * ld];
+      A[pv + col * ld] = t;
+    }
+    __syncthreads();
+    if (tid < w) prow[tid] = A[c + ((i64)k0 + tid) * ld];
+    __syncthreads();
+    if (!sing) {
+      const double2 d = prow[j];
+      const double dn = d.x * d.x + d.y * d.y;
+      const double2 inv = make_double2(d.x / dn, -d.y / dn);
+      for (i64 i = c + 1 + tid; i < n; i += 1024) {
+        const double2 v = A[i + c * ld];
+        const double2 l = make_double2(v.x * inv.x - v.y * inv.y, v.x * inv.y + v.y * inv.x);
+        A[i + c * ld] = l;
+        for (int q = j + 1; q < w; ++q) {
+          const double2 u = prow[q];
+          double2* e = A + i + ((i64)k0 + q) * ld;
+          const double2 o = *e;
+          *e = make_double2(o.x - (l.x * u.x - l.y * u.y), o.y - (l.x * u.y + l.y * u.x));
+        }
       }
     }
     __syncthreads();
   }
-  int piv = bidx[0];
-  if (best[0] <= 0.0) {
-    if (threadIdx.x == 0) status[0] = k + 1;
-    return;
-  }
-  if (threadIdx.x == 0) ipiv[k] = piv;
-  if (piv != k) {
-    for (i64 c = threadIdx.x; c < n; c += 1024) {
-      double2 t = A[k + c * ld];
-      A[k + c * ld] = A[piv + c * ld];
-      A[piv + c * ld] = t;
+}
+
+// the panel's row swaps (in order) on every column outside the panel
+__global__ __launch_bounds__(256) void k_lu_laswp(i64 n, i64 ld, int k0, int w, double2* __restrict__ A,
+                                                  const int* __restrict__ ipiv) {
+  const i64 col = blockIdx.x * (i64)256 + threadIdx.x;
+  if (col >= n || (col >= k0 && col < (i64)k0 + w)) return;
+  for (int j = 0; j < w; ++j) {
+    const i64 r = (i64)k0 + j;
+    const i64 p = ipiv[r];
+    if (p != r) {
+      const double2 t = A[r + col * ld];
+      A[r + col * ld] = A[p + col * ld];
+      A[p + col * ld] = t;
     }
-  }
-  __syncthreads();
-  double2 d = A[k + (i64)k * ld];
-  double dn = d.x * d.x + d.y * d.y;
-  double2 inv = make_double2(d.x / dn, -d.y / dn);
-  for (i64 i = k + 1 + threadIdx.x; i < n; i += 1024) {
-    double2 v = A[i + (i64)k * ld];
-    A[i + (i64)k * ld] = make_double2(v.x * inv.x - v.y * inv.y, v.x * inv.y + v.y * inv.x);
   }
 }
 
-__global__ __launch_bounds__(256) void k_lu_update(i64 n, i64 ld, int k, double2* __restrict__ A) {
-  i64 r = k + 1 + blockIdx.x * (i64)256 + threadIdx.x;
-  i64 c = k + 1 + blockIdx.y;
-  if (r >= n || c >= n) return;
-  double2 l = A[r + (i64)k * ld], u = A[k + c * ld], a = A[r + c * ld];
-  A[r + c * ld] = make_double2(a.x - (l.x * u.x - l.y * u.y), a.y - (l.x * u.y + l.y * u.x));
+// U12 = L11^-1 A12 (unit lower L11 of the panel), one column past the panel
+// per thread
+__global__ __launch_bounds__(256) void k_lu_trsm(i64 n, i64 ld, int k0, int w, double2* __restrict__ A) {
+  __shared__ double2 L[kLuW][kLuW + 1];
+  for (int e = threadIdx.x; e < w * w; e += 256) {
+    const int i = e % w, j = e / w;
+    L[i][j] = A[((i64)k0 + i) + ((i64)k0 + j) * ld];
+  }
+  __syncthreads();
+  const i64 col = (i64)k0 + w + blockIdx.x * (i64)256 + threadIdx.x;
+  if (col >= n) return;
+  double2* x = A + k0 + col * ld;
+  double2 v[kLuW];
+#pragma unroll
+  for (int i = 0; i < kLuW; ++i) v[i] = i < w ? x[i] : make_double2(0.0, 0.0);
+#pragma unroll
+  for (int i = 1; i < kLuW; ++i) {
+    double2 a = v[i];
+#pragma unroll
+    for (int j = 0; j < i; ++j) {
+      const double2 l = L[i][j < w ? j : 0];
+      a.x -= l.x * v[j].x - l.y * v[j].y;
+      a.y -= l.x * v[j].y + l.y * v[j].x;
+    }
+    v[i] = a;
+  }
+#pragma unroll
+  for (int i = 0; i < kLuW; ++i)
+    if (i < w) x[i] = v[i];
 }
 
 // Solve with the LU factors for one right-hand side per workgroup (in place).
@@ -838,11 +903,20 @@ void diag_apply(i64 n, const double* d, const double* x, double* y, hipStream_t 
   HIP_OK(hipGetLastError());
 }
 void lu_factor(i64 n, i64 ld, double* A, int* ipiv, int* status, hipStream_t s) {
-  for (i64 k = 0; k < n; ++k) {
-    k_lu_pivot<<<1, 1024, 0, s>>>(n, ld, (int)k, reinterpret_cast<double2*>(A), ipiv, status);
-    if (k + 1 < n) {
-      dim3 g(nblocks(n - k - 1, 256), (unsigned)(n - k - 1));
-      k_lu_update<<<g, 256, 0, s>>>(n, ld, (int)k, reinterpret_cast<double2*>(A));
+  DM_REQUIRE(n >= 1 && n < (1LL << 31) && ld >= n, DM_ERR_ARGUMENT, "dense LU: bad dimensions");
+  double2* a = reinterpret_cast<double2*>(A);
+  DBuf<double> ws;
+  ws.alloc(gemm_c128_workspace_bytes(kLuW, (int)std::max<i64>(n - kLuW, 1)) / sizeof(double));
+  for (i64 k0 = 0; k0 < n; k0 += kLuW) {
+    const int w = (int)std::min<i64>(kLuW, n - k0);
+    k_lu_panel<<<1, 1024, 0, s>>>(n, ld, (int)k0, w, a, ipiv, status);
+    k_lu_laswp<<<nblocks(n, 256), 256, 0, s>>>(n, ld, (int)k0, w, a, ipiv);
+    const i64 rest = n - k0 - w;
+    if (rest > 0) {
+      k_lu_trsm<<<nblocks(rest, 256), 256, 0, s>>>(n, ld, (int)k0, w, a);
+      // A22 -= L21 U12
+      gemm_c128(rest, w, A + 2 * ((k0 + w) + k0 * ld), ld, A + 2 * (k0 + (k0 + w) * ld), ld, (int)rest,
+                A + 2 * ((k0 + w) + (k0 + w) * ld), ld, ws.p, s, true);
     }
   }
   HIP_OK(hipGetLastError());

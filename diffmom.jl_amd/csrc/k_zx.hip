@@ -45,7 +45,7 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // leading dimension ld) X (n x R).
 __global__ __launch_bounds__(256, 2) void k_zx_mfma64(i64 mr, i64 n, i64 ld, const double* __restrict__ Z,
                                                    const double* __restrict__ Xr, const double* __restrict__ Xi, i64 np,
-                                                   int R, double2* __restrict__ Y, i64 ldy) {
+                                                   int R, double2* __restrict__ Y, i64 ldy, int sub) {
   // stages of ZX_BK columns: 16-byte loads (one complex of Z, two rows of a
   // plane) into registers for stage s + 1 while stage s is multiplied out of
   // LDS (two buffers, one barrier per stage).  Xr, Xi: np x R, rows past n zero.
@@ -145,7 +145,13 @@ __global__ __launch_bounds__(256, 2) void k_zx_mfma64(i64 mr, i64 n, i64 ld, con
       for (int r = 0; r < 4; ++r) {
         const i64 m = m0 + ((32 * wave + 16 * r2 + g + 4 * r) >> 1);
         if (m >= mr) continue;
-        Y[m + (i64)col * ldy] = make_double2(p[r] - qo[r], q[r] + po[r]);
+        const double2 v = make_double2(p[r] - qo[r], q[r] + po[r]);
+        if (sub) {  // Y -= Z X
+          const double2 o = Y[m + (i64)col * ldy];
+          Y[m + (i64)col * ldy] = make_double2(o.x - v.x, o.y - v.y);
+        } else {
+          Y[m + (i64)col * ldy] = v;
+        }
       }
     }
 }
@@ -363,12 +369,12 @@ void zx_mfma64(i64 n, i64 ld, const double* z64, const double* X, i64 ldx, int R
                                                                      Xr, Xi);
   dim3 grid((unsigned)((n + ZX_BM - 1) / ZX_BM), (unsigned)((R + ZX_BN - 1) / ZX_BN), 1);
   DM_REQUIRE(grid.y <= 65535, DM_ERR_ARGUMENT, "too many right-hand sides for one product");
-  k_zx_mfma64<<<grid, 256, 0, s>>>(n, n, ld, z64, Xr, Xi, np, R, reinterpret_cast<double2*>(Y), ldy);
+  k_zx_mfma64<<<grid, 256, 0, s>>>(n, n, ld, z64, Xr, Xi, np, R, reinterpret_cast<double2*>(Y), ldy, 0);
   HIP_OK(hipGetLastError());
 }
 
 void gemm_c128(i64 mr, i64 n, const double* A, i64 lda, const double* X, i64 ldx, int R, double* Y, i64 ldy,
-               void* ws, hipStream_t s) {
+               void* ws, hipStream_t s, bool subtract) {
   DM_REQUIRE(lda >= mr && ldx >= n && ldy >= mr && R >= 1, DM_ERR_ARGUMENT, "complex GEMM: bad dimensions");
   const i64 np = (n + 15) / 16 * 16;
   double* Xr = reinterpret_cast<double*>(ws);
@@ -377,7 +383,8 @@ void gemm_c128(i64 mr, i64 n, const double* A, i64 lda, const double* X, i64 ldx
                                                                      Xr, Xi);
   dim3 grid((unsigned)((mr + ZX_BM - 1) / ZX_BM), (unsigned)((R + ZX_BN - 1) / ZX_BN), 1);
   DM_REQUIRE(grid.x < (1u << 31) && grid.y <= 65535, DM_ERR_ARGUMENT, "complex GEMM too large for one launch");
-  k_zx_mfma64<<<grid, 256, 0, s>>>(mr, n, lda, A, Xr, Xi, np, R, reinterpret_cast<double2*>(Y), ldy);
+  k_zx_mfma64<<<grid, 256, 0, s>>>(mr, n, lda, A, Xr, Xi, np, R, reinterpret_cast<double2*>(Y), ldy,
+                                   subtract ? 1 : 0);
   HIP_OK(hipGetLastError());
 }
 

@@ -33,6 +33,22 @@ def test_dense_direct(dm, c0):
     assert np.linalg.norm(I - npz["I"]) / np.linalg.norm(npz["I"]) < 1e-9
 
 
+def test_dense_direct_blocked_panels(dm):
+    """Blocked LU over many 32-column panels with a partial last one (N =
+    1976 on a 26 x 26 plate): true residual < 1e-10 and the currents of
+    numpy's LU (LAPACK zgesv on the same matrix) to 1e-9."""
+    m = dm.make_rect_plate(0.3, 0.3, 26, 26)
+    r = dm.build_rwg(m)
+    assert r.nedges % 32 != 0 and r.nedges > 40 * 32
+    Z = dm.assemble_Z_efie(m, r, K0, forms=dm.DM_Z64, device=True)
+    v = dm.assemble_v_plane_wave(m, r, [0, 0, -K0], 1.0, [1, 0, 0])
+    I = dm.solve_forward(Z, v, solver="direct")
+    Zh = Z.to_numpy()
+    assert np.linalg.norm(Zh @ I - v) / np.linalg.norm(v) < 1e-10
+    In = np.linalg.solve(Zh, v)
+    assert np.linalg.norm(I - In) / np.linalg.norm(In) < 1e-9
+
+
 @pytest.mark.parametrize("precond", [None, "diag", "nf", "block"])
 @pytest.mark.parametrize("side", ["left", "right"])
 def test_gmres_matches_direct(dm, c0, precond, side):
