@@ -444,8 +444,9 @@ __global__ void k_diag_apply(i64 n, const double2* __restrict__ d, const double2
 // Blocked right-looking LU with partial pivoting (the dense direct method,
 // solve_forward :direct), panels of kLuW columns: one workgroup factors the
 // panel (pivot search, row swap inside the panel, scaling and the panel's own
-// rank-1 updates, column by column), k_lu_laswp applies the panel's swaps to
-// the other columns, k_lu_trsm forms U12 = L11^-1 A12, and the trailing
+// rank-1 updates, column by column, then the inverse of its unit lower
+// triangle L11), k_lu_laswp applies the panel's swaps to the other columns,
+// U12 = L11^-1 A12 is a small complex128 GEMM with that inverse, and the trailing
 // matrix takes A22 -= L21 U12 as one complex128 GEMM on the FP64 matrix cores
 // (gemm_c128, subtract mode).  Pivots: the largest |a|^2 in the column, the
 // smallest row index among equal ones.
@@ -453,10 +454,10 @@ constexpr int kLuW = 32;
 
 __global__ __launch_bounds__(1024) void k_lu_panel(i64 n, i64 ld, int k0, int w, double2* __restrict__ A,
                                                    int* __restrict__ ipiv, int* __restrict__ status) {
-  __shared__ double best[1024];
-  __shared__ int bidx[1024];
+  __shared__ double wbest[16];
+  __shared__ int widx[16];
   __shared__ double2 prow[kLuW];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int j = 0; j < w; ++j) {
     const i64 c = (i64)k0 + j;
     double m = -1;
@@ -469,52 +470,106 @@ __global__ __launch_bounds__(1024) void k_lu_panel(i64 n, i64 ld, int k0, int w,
         p = (int)i;
       }
     }
-    best[tid] = m;
-    bidx[tid] = p;
-    __syncthreads();
-    for (int st = 512; st > 0; st >>= 1) {
-      if (tid < st) {
-        if (best[tid + st] > best[tid] || (best[tid + st] == best[tid] && bidx[tid + st] < bidx[tid])) {
-          best[tid] = best[tid + st];
-          bidx[tid] = bidx[tid + st];
-        }
+    // largest |a|^2, smallest row among equal ones: within the wave, then
+    // over the 16 waves (every thread reduces the same 16 values)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double om = __shfl_xor(m, o, 64);
+      const int op = __shfl_xor(p, o, 64);
+      if (om > m || (om == m && op < p)) {
+        m = om;
+        p = op;
       }
-      __syncthreads();
     }
-    const bool sing = !(best[0] > 0.0);
-    const int pv = sing ? (int)c : bidx[0];
-    __syncthreads();  // every thread has read best[0] / bidx[0]
+    if (lane == 0) {
+      wbest[wave] = m;
+      widx[wave] = p;
+    }
+    __syncthreads();
+    double bm = wbest[0];
+    int bp = widx[0];
+    for (int q = 1; q < 16; ++q) {
+      if (wbest[q] > bm || (wbest[q] == bm && widx[q] < bp)) {
+        bm = wbest[q];
+        bp = widx[q];
+      }
+    }
+    const bool sing = !(bm > 0.0);
+    const int pv = sing ? (int)c : bp;
     if (tid == 0) {
       ipiv[c] = pv;
       if (sing && status[0] == 0) status[0] = (int)c + 1;
     }
-    if (pv != c && tid < w) {
+    // swap rows c and pv inside the panel; the pivot row to LDS
+    if (tid < w) {
       const i64 col = (i64)k0 + tid;
-      const double2 t = A[c + col * ld];
-      A[c + col * ld] = A[pv + col * ld];
-      A[pv + col * ld] = t;
+      const double2 rc = A[c + col * ld];
+      double2 pr = rc;
+      if (pv != c) {
+        pr = A[pv + col * ld];
+        A[c + col * ld] = pr;
+        A[pv + col * ld] = rc;
+      }
+      prow[tid] = pr;
     }
-    __syncthreads();
-    if (tid < w) prow[tid] = A[c + ((i64)k0 + tid) * ld];
     __syncthreads();
     if (!sing) {
       const double2 d = prow[j];
       const double dn = d.x * d.x + d.y * d.y;
       const double2 inv = make_double2(d.x / dn, -d.y / dn);
       for (i64 i = c + 1 + tid; i < n; i += 1024) {
-        const double2 v = A[i + c * ld];
-        const double2 l = make_double2(v.x * inv.x - v.y * inv.y, v.x * inv.y + v.y * inv.x);
+        const double2 ej = A[i + c * ld];
+        const double2 l = make_double2(ej.x * inv.x - ej.y * inv.y, ej.x * inv.y + ej.y * inv.x);
         A[i + c * ld] = l;
-        for (int q = j + 1; q < w; ++q) {
-          const double2 u = prow[q];
-          double2* e = A + i + ((i64)k0 + q) * ld;
-          const double2 o = *e;
-          *e = make_double2(o.x - (l.x * u.x - l.y * u.y), o.y - (l.x * u.y + l.y * u.x));
+        double2* row = A + i + (i64)k0 * ld;
+        // the row's entries past column j, eight loads issued together per round
+#pragma unroll 1
+        for (int q0 = 0; q0 < kLuW; q0 += 8) {
+          if (q0 + 7 <= j || q0 >= w) continue;
+          double2 e[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (q0 + u > j && q0 + u < w) e[u] = row[(i64)(q0 + u) * ld];
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (q0 + u > j && q0 + u < w) {
+              const double2 pu = prow[q0 + u];
+              row[(i64)(q0 + u) * ld] =
+                  make_double2(e[u].x - (l.x * pu.x - l.y * pu.y), e[u].y - (l.x * pu.y + l.y * pu.x));
+            }
         }
       }
     }
     __syncthreads();
   }
+
+}
+
+// inverse of the panel's unit lower triangle L11 (w x w, column-major with
+// leading dimension kLuW): thread t forward-substitutes e_t
+__global__ __launch_bounds__(64) void k_lu_linv(i64 ld, int k0, int w, const double2* __restrict__ A,
+                                                double2* __restrict__ linv) {
+  const int tid = threadIdx.x;
+  if (tid >= w) return;
+  double2 x[kLuW];
+#pragma unroll
+  for (int i = 0; i < kLuW; ++i) x[i] = make_double2(i == tid ? 1.0 : 0.0, 0.0);
+#pragma unroll
+  for (int i = 1; i < kLuW; ++i) {
+    if (i < w) {
+      double2 a = x[i];
+#pragma unroll
+      for (int q = 0; q < i; ++q) {
+        const double2 l = A[((i64)k0 + i) + ((i64)k0 + q) * ld];
+        a.x -= l.x * x[q].x - l.y * x[q].y;
+        a.y -= l.x * x[q].y + l.y * x[q].x;
+      }
+      x[i] = a;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kLuW; ++i)
+    if (i < w) linv[i + (i64)tid * kLuW] = x[i];
 }
 
 // the panel's row swaps (in order) on every column outside the panel
@@ -533,69 +588,105 @@ __global__ __launch_bounds__(256) void k_lu_laswp(i64 n, i64 ld, int k0, int w, 
   }
 }
 
-// U12 = L11^-1 A12 (unit lower L11 of the panel), one column past the panel
-// per thread
-__global__ __launch_bounds__(256) void k_lu_trsm(i64 n, i64 ld, int k0, int w, double2* __restrict__ A) {
-  __shared__ double2 L[kLuW][kLuW + 1];
-  for (int e = threadIdx.x; e < w * w; e += 256) {
-    const int i = e % w, j = e / w;
-    L[i][j] = A[((i64)k0 + i) + ((i64)k0 + j) * ld];
-  }
-  __syncthreads();
-  const i64 col = (i64)k0 + w + blockIdx.x * (i64)256 + threadIdx.x;
-  if (col >= n) return;
-  double2* x = A + k0 + col * ld;
-  double2 v[kLuW];
-#pragma unroll
-  for (int i = 0; i < kLuW; ++i) v[i] = i < w ? x[i] : make_double2(0.0, 0.0);
-#pragma unroll
-  for (int i = 1; i < kLuW; ++i) {
-    double2 a = v[i];
-#pragma unroll
-    for (int j = 0; j < i; ++j) {
-      const double2 l = L[i][j < w ? j : 0];
-      a.x -= l.x * v[j].x - l.y * v[j].y;
-      a.y -= l.x * v[j].y + l.y * v[j].x;
-    }
-    v[i] = a;
-  }
-#pragma unroll
-  for (int i = 0; i < kLuW; ++i)
-    if (i < w) x[i] = v[i];
-}
+// Solve with the LU factors for one right-hand side per workgroup (in place):
+// the row swaps in order (one thread), then L y = P b and U x = y in blocks
+// of 32 rows: one wave solves the block's triangle with its rows in
+// registers, then every thread updates its rows below (above) the block with
+// the 32 solved values.  b sits in LDS (dynamic) when n <= kLuSolveLds.
+constexpr i64 kLuSolveLds = 8192;
 
-// Solve with the LU factors for one right-hand side per workgroup (in place).
-__global__ __launch_bounds__(1024) void k_lu_solve(i64 n, i64 ld, const double2* __restrict__ A, const int* __restrict__ ipiv,
-                                                   double2* __restrict__ B, i64 ldb) {
-  double2* b = B + (i64)blockIdx.x * ldb;
-  if (threadIdx.x == 0)
+__device__ inline double2 cmul2(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+
+template <bool LDSB>
+__global__ __launch_bounds__(1024) void k_lu_solve(i64 n, i64 ld, const double2* __restrict__ A,
+                                                   const int* __restrict__ ipiv, double2* __restrict__ B, i64 ldb) {
+  extern __shared__ double2 sb[];
+  __shared__ double2 xs[32];
+  double2* bg = B + (i64)blockIdx.x * ldb;
+  double2* b = LDSB ? sb : bg;
+  const int tid = threadIdx.x, lane = tid & 63;
+  if (LDSB) {
+    for (i64 i = tid; i < n; i += 1024) sb[i] = bg[i];
+    __syncthreads();
+  }
+  if (tid == 0)
     for (i64 k = 0; k < n; ++k) {
-      int p = ipiv[k];
-      if (p != k) { double2 t = b[k]; b[k] = b[p]; b[p] = t; }
+      const int p = ipiv[k];
+      if (p != k) {
+        const double2 t = b[k];
+        b[k] = b[p];
+        b[p] = t;
+      }
     }
   __syncthreads();
-  for (i64 k = 0; k < n; ++k) {  // L y = b (unit diagonal)
-    double2 bk = b[k];
-    for (i64 i = k + 1 + threadIdx.x; i < n; i += 1024) {
-      double2 l = A[i + k * ld], v = b[i];
-      b[i] = make_double2(v.x - (l.x * bk.x - l.y * bk.y), v.y - (l.x * bk.y + l.y * bk.x));
+  // L y = P b (unit lower triangle)
+  for (i64 k0 = 0; k0 < n; k0 += 32) {
+    const int w = (int)min((i64)32, n - k0);
+    if (tid < 64) {
+      double2 v = lane < w ? b[k0 + lane] : make_double2(0.0, 0.0);
+      for (int j = 0; j < w; ++j) {
+        const double2 xj = make_double2(__shfl(v.x, j, 64), __shfl(v.y, j, 64));
+        if (lane > j && lane < w) {
+          const double2 t = cmul2(A[(k0 + lane) + (k0 + j) * ld], xj);
+          v.x -= t.x;
+          v.y -= t.y;
+        }
+      }
+      if (lane < w) {
+        b[k0 + lane] = v;
+        xs[lane] = v;
+      }
+    }
+    __syncthreads();
+    for (i64 i = k0 + w + tid; i < n; i += 1024) {
+      double2 v = b[i];
+      for (int j = 0; j < w; ++j) {
+        const double2 t = cmul2(A[i + (k0 + j) * ld], xs[j]);
+        v.x -= t.x;
+        v.y -= t.y;
+      }
+      b[i] = v;
     }
     __syncthreads();
   }
-  for (i64 k = n - 1; k >= 0; --k) {  // U x = y
-    if (threadIdx.x == 0) {
-      double2 d = A[k + k * ld], v = b[k];
-      double dn = d.x * d.x + d.y * d.y;
-      b[k] = make_double2((v.x * d.x + v.y * d.y) / dn, (v.y * d.x - v.x * d.y) / dn);
+  // U x = y
+  for (i64 k1 = n; k1 > 0; k1 -= 32) {
+    const i64 k0 = k1 > 32 ? k1 - 32 : 0;
+    const int w = (int)(k1 - k0);
+    if (tid < 64) {
+      double2 v = lane < w ? b[k0 + lane] : make_double2(0.0, 0.0);
+      for (int j = w - 1; j >= 0; --j) {
+        if (lane == j) {
+          const double2 d = A[(k0 + j) + (k0 + j) * ld];
+          const double dn = d.x * d.x + d.y * d.y;
+          v = make_double2((v.x * d.x + v.y * d.y) / dn, (v.y * d.x - v.x * d.y) / dn);
+        }
+        const double2 xj = make_double2(__shfl(v.x, j, 64), __shfl(v.y, j, 64));
+        if (lane < j) {
+          const double2 t = cmul2(A[(k0 + lane) + (k0 + j) * ld], xj);
+          v.x -= t.x;
+          v.y -= t.y;
+        }
+      }
+      if (lane < w) {
+        b[k0 + lane] = v;
+        xs[lane] = v;
+      }
     }
     __syncthreads();
-    double2 xk = b[k];
-    for (i64 i = threadIdx.x; i < k; i += 1024) {
-      double2 u = A[i + k * ld], v = b[i];
-      b[i] = make_double2(v.x - (u.x * xk.x - u.y * xk.y), v.y - (u.x * xk.y + u.y * xk.x));
+    for (i64 i = tid; i < k0; i += 1024) {
+      double2 v = b[i];
+      for (int j = 0; j < w; ++j) {
+        const double2 t = cmul2(A[i + (k0 + j) * ld], xs[j]);
+        v.x -= t.x;
+        v.y -= t.y;
+      }
+      b[i] = v;
     }
     __syncthreads();
   }
+  if (LDSB)
+    for (i64 i = tid; i < n; i += 1024) bg[i] = sb[i];
 }
 
 // ------------------------------------------------ GMRES Givens on the device
@@ -905,24 +996,36 @@ void diag_apply(i64 n, const double* d, const double* x, double* y, hipStream_t 
 void lu_factor(i64 n, i64 ld, double* A, int* ipiv, int* status, hipStream_t s) {
   DM_REQUIRE(n >= 1 && n < (1LL << 31) && ld >= n, DM_ERR_ARGUMENT, "dense LU: bad dimensions");
   double2* a = reinterpret_cast<double2*>(A);
-  DBuf<double> ws;
+  DBuf<double> ws, linv;
   ws.alloc(gemm_c128_workspace_bytes(kLuW, (int)std::max<i64>(n - kLuW, 1)) / sizeof(double));
+  linv.alloc((size_t)2 * kLuW * kLuW);
   for (i64 k0 = 0; k0 < n; k0 += kLuW) {
     const int w = (int)std::min<i64>(kLuW, n - k0);
+    const i64 rest = n - k0 - w;
     k_lu_panel<<<1, 1024, 0, s>>>(n, ld, (int)k0, w, a, ipiv, status);
     k_lu_laswp<<<nblocks(n, 256), 256, 0, s>>>(n, ld, (int)k0, w, a, ipiv);
-    const i64 rest = n - k0 - w;
     if (rest > 0) {
-      k_lu_trsm<<<nblocks(rest, 256), 256, 0, s>>>(n, ld, (int)k0, w, a);
+      k_lu_linv<<<1, 64, 0, s>>>(ld, (int)k0, w, a, reinterpret_cast<double2*>(linv.p));
+      double* a12 = A + 2 * (k0 + (k0 + w) * ld);
+      // U12 = L11^-1 A12 (in place: A12 is copied into the product's planes first)
+      gemm_c128(w, w, linv.p, kLuW, a12, ld, (int)rest, a12, ld, ws.p, s);
       // A22 -= L21 U12
-      gemm_c128(rest, w, A + 2 * ((k0 + w) + k0 * ld), ld, A + 2 * (k0 + (k0 + w) * ld), ld, (int)rest,
-                A + 2 * ((k0 + w) + (k0 + w) * ld), ld, ws.p, s, true);
+      gemm_c128(rest, w, A + 2 * ((k0 + w) + k0 * ld), ld, a12, ld, (int)rest, A + 2 * ((k0 + w) + (k0 + w) * ld),
+                ld, ws.p, s, true);
     }
   }
   HIP_OK(hipGetLastError());
 }
 void lu_solve(i64 n, i64 ld, const double* A, const int* ipiv, double* B, i64 ldb, int nrhs, hipStream_t s) {
-  k_lu_solve<<<nrhs, 1024, 0, s>>>(n, ld, reinterpret_cast<const double2*>(A), ipiv, reinterpret_cast<double2*>(B), ldb);
+  if (n <= kLuSolveLds) {
+    const size_t smem = (size_t)n * sizeof(double2);
+    HIP_OK(hipFuncSetAttribute((const void*)k_lu_solve<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    k_lu_solve<true><<<nrhs, 1024, smem, s>>>(n, ld, reinterpret_cast<const double2*>(A), ipiv,
+                                               reinterpret_cast<double2*>(B), ldb);
+  } else {
+    k_lu_solve<false><<<nrhs, 1024, 0, s>>>(n, ld, reinterpret_cast<const double2*>(A), ipiv,
+                                             reinterpret_cast<double2*>(B), ldb);
+  }
   HIP_OK(hipGetLastError());
 }
 
