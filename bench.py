@@ -21,7 +21,8 @@ N x N complex128 storage is timed as a variant.
 
 Also measured (not the headline value): the many-angle case, 256 seeded plane
 waves on the same sphere solved in lockstep batches (Z X on the matrix
-cores), chosen and plain FP64; the error of the chosen currents and RCS
+cores; relaxed to three bf16 part products once the residual allows it),
+chosen, unrelaxed and plain FP64; the error of the chosen currents and RCS
 against a 1e-10 FP64 solve; the CPU baseline by phase, with the oracle's
 check of the GPU matrix and currents on sampled rows.
 
@@ -375,7 +376,7 @@ def main():
                     help="chosen form: cross-approximate the complex64 matrix to this relative Frobenius error "
                          "(0 = dense complex64)")
     ap.add_argument("--relax", type=float, default=0.01,
-                    help="chosen form: relaxation constant of the inner products (0: off)")
+                    help="chosen form and lockstep batches: relaxation constant of the inner products (0: off)")
     ap.add_argument("--inner-tol", type=float, default=3e-7,
                     help="chosen form: relative tolerance of the first inner solve")
     ap.add_argument("--relax-probe", action="store_true",
@@ -570,10 +571,10 @@ def main():
         ctx.sync()
         ctx.profile_reset()
         ctx.profile(True)
-        dt, X, st_b, tb = solve("mixed", nf, waves, maxiter=300)
+        dt, X, st_b, tb = solve("mixed", nf, waves, maxiter=300, relax=a.relax)
         ctx.profile(False)
         bkern = {}
-        for name in ("assemble_z64_z32", "zx_split", "zx_z32", "zx_z64", "arnoldi", "precond"):
+        for name in ("assemble_z64_z32", "zx_split", "zx_z32", "zx_z32_relaxed", "zx_z64", "arnoldi", "precond"):
             c, ms = ctx.profile_read(name)
             if c:
                 bkern[name] = dict(launches=c, total_ms=round(ms, 3), avg_ms=round(ms / c, 4))
@@ -584,7 +585,7 @@ def main():
                        single_rhs_chosen_s=round(t_c, 4),
                        speedup_per_rhs_vs_single_chosen=round(t_c * a.batch_rhs / dt, 2),
                        speedup_per_rhs_vs_single_plain=round(t_p * a.batch_rhs / dt, 2),
-                       kernels=bkern)
+                       relax=a.relax, kernels=bkern)
         if "zx_z32" in bkern:
             # lockstep complex64 Z X over all columns: 8 N^2 R useful FLOP per product,
             # six bf16 part products of that size on the matrix cores (plus the FP64
@@ -595,8 +596,22 @@ def main():
                                           useful_tflops=round(useful / zx_ms / 1e9, 1),
                                           bf16_mfma_tflops=round(6 * useful / zx_ms / 1e9, 1),
                                           peak_bf16=2500.0, frac=round(6 * useful / zx_ms / 1e9 / 2500.0, 4))
+        if "zx_z32_relaxed" in bkern:
+            # the relaxed products: three bf16 part products of the same size
+            zx_ms = bkern["zx_z32_relaxed"]["avg_ms"]
+            useful = 8.0 * N * N * a.batch_rhs
+            batched["zx_relaxed_roofline"] = dict(bound="mfma", unit="TFLOP/s", avg_ms=zx_ms,
+                                                  bf16_mfma_tflops=round(3 * useful / zx_ms / 1e9, 1),
+                                                  peak_bf16=2500.0,
+                                                  frac=round(3 * useful / zx_ms / 1e9 / 2500.0, 4))
         log(f"batched {a.batch_rhs} plane waves: {dt:.2f} s ({dt / a.batch_rhs * 1e3:.1f} ms per RHS), "
             f"{st_b.niter} it total, worst residual {st_b.residual:.2e}")
+        if a.relax > 0:
+            # ablation: the same batch with every product at six part products
+            dtu, _, st_bu, _ = solve("mixed", nf, waves, maxiter=300)
+            batched["unrelaxed"] = dict(s=round(dtu, 3), iterations_total=st_bu.niter,
+                                        worst_true_residual=st_bu.residual)
+            log(f"batched unrelaxed: {dtu:.2f} s, {st_bu.niter} it total, worst residual {st_bu.residual:.2e}")
         # the plain FP64 path on the same batch: lockstep GMRES, complex128 Z X on the FP64 matrix cores
         dtp, _, st_bp, _ = solve("fp64", nf, waves, maxiter=300)
         batched["plain_fp64"] = dict(s=round(dtp, 3), s_per_rhs=round(dtp / a.batch_rhs, 5),

@@ -812,8 +812,10 @@ def _opts(method, precond=None, side="right", tol=1e-6, maxiter=300, restart=0, 
     storage: "packed" (symmetric tiles, half the bytes) or "full" for the
     one-column FP64 GMRES path of solve_planewaves.  graph: one right-hand
     side at a time, each Krylov step after the first of a cycle replays a
-    captured HIP graph.  relax > 0 (mixed, compressed, one right-hand side
-    at a time): inner products relaxed as the residual falls."""
+    captured HIP graph.  relax > 0 (mixed): inner products relaxed as the
+    residual falls -- prefixes of the cross-approximation factors (compressed,
+    one right-hand side at a time) or three instead of six bf16 part products
+    (lockstep batches)."""
     if storage not in ("packed", "full"):
         raise ValueError(f"storage must be 'packed' or 'full', got {storage!r}")
     o = dm_solve_opts()
@@ -859,10 +861,12 @@ def solve_planewaves(mesh, rwg, k, excitations, method="mixed", preconditioner=N
     storage ("fp64", one wave at a time): "packed" keeps the upper triangle of
     128 x 128 tiles (Z = Z^T exactly; each product reads half the bytes),
     "full" the N x N matrix as the reference stores it.  graph (one wave at
-    a time): Krylov steps replay a captured HIP graph.  relax > 0 (mixed,
-    compress_tol > 0, one wave at a time): the inner FGMRES relaxes its
-    products as its residual falls (inexact Krylov), reading a prefix of the
-    cross-approximation factors; inner_tol: the first inner solve's relative
+    a time): Krylov steps replay a captured HIP graph.  relax > 0 (mixed):
+    the inner FGMRES relaxes its products as its residual falls (inexact
+    Krylov) -- one wave at a time with compress_tol > 0 it reads a prefix of
+    the cross-approximation factors, a lockstep batch drops its complex64
+    Z X from six to three bf16 part products once the worst active column
+    allows a 5e-5 product error; inner_tol: the first inner solve's relative
     tolerance (0: tol / 2).
     Returns (X, stats, assembly_time_s[, ZMatrix])."""
     _check_pair(mesh, rwg)

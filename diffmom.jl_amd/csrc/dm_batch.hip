@@ -63,7 +63,7 @@ std::vector<double> col_norms(BatchKrylov& K, const double* W, hipStream_t s) {
 void gmres_batch(dm_ctx* ctx, i64 n, int R, DevOp& A, DevPrec* M, const double* B, double* X,
                  const std::vector<double>& rtol, const std::vector<char>& live, int maxiter, int restart,
                  BatchKrylov& K, hipStream_t s, std::vector<int>& iters, std::vector<char>& solved,
-                 bool x_zero = false) {
+                 bool x_zero = false, double relax = 0.0) {
   int mcap = restart > 0 ? std::min(restart, maxiter) : maxiter;
   mcap = std::min(mcap, dmb::max_basis() - 1);
   K.reserve(n, R, mcap, M != nullptr, A.batch_ws(R));
@@ -104,6 +104,22 @@ void gmres_batch(dm_ctx* ctx, i64 n, int R, DevOp& A, DevPrec* M, const double* 
   its.alloc(R);
   sol.alloc(R);
   std::vector<int> act_h(R), its_h(R), sol_h(R);
+  // relaxed products (inexact Krylov): at every check the product may carry
+  // a relative error of relax * eps_r / est_r for the most demanding active
+  // column (eps_r = rtol_r beta_r, est_r its residual estimate)
+  DBuf<double> estd;
+  std::vector<double> est_h(R);
+  if (relax > 0) estd.alloc(R);
+  auto relax_to = [&](bool start) {
+    if (relax <= 0) return;
+    double eta = INFINITY;
+    for (int r = 0; r < R; ++r) {
+      if (!active[r] || (!start && !act_h[r])) continue;
+      const double e = start ? rtol[r] : (est_h[r] > 0 ? eps[r] / est_h[r] : INFINITY);
+      eta = std::min(eta, relax * e);
+    }
+    A.relax(std::isfinite(eta) ? eta : 0.0);
+  };
   int total = 0;
   auto any_active = [&]() { return std::any_of(active.begin(), active.end(), [](char a) { return a != 0; }); };
   while (any_active() && total < maxiter) {
@@ -114,6 +130,7 @@ void gmres_batch(dm_ctx* ctx, i64 n, int R, DevOp& A, DevPrec* M, const double* 
     HIP_OK(hipMemsetAsync(sol.p, 0, sizeof(int) * R, s));
     dmb::givens_init(R, mcap, betad.p, K.act.p, gd.p, K.sc.p, jcnt.p, its.p, s);
     dmb::scale(n, R, K.V.p, K.sc.p, K.V.p, s);
+    relax_to(true);
     // steps are queued without a host round trip; the host reads the active
     // flags every 4 steps (columns that converge in between are masked on
     // the device, so only the Z X of a fully converged batch is wasted)
@@ -140,17 +157,19 @@ void gmres_batch(dm_ctx* ctx, i64 n, int R, DevOp& A, DevPrec* M, const double* 
         dmb::update(n, R, k, K.kcap, K.V.p, h2, K.act.p, K.W.p, s);
         dmb::norm2(n, R, K.W.p, K.act.p, K.nr.p, s);
         dmb::givens(R, j, mcap, K.kcap, h1, h2, K.nr.p, Hd.p, csd.p, snd.p, gd.p, K.sc.p, K.act.p, epsd.p, jcnt.p,
-                    its.p, sol.p, s);
+                    its.p, sol.p, s, relax > 0 ? estd.p : nullptr);
         // v_{j+1} = w / ||w|| for the columns still iterating (sc = 0 otherwise)
         if (k <= mcap) dmb::scale(n, R, K.W.p, K.sc.p, vblk(K.V.p, k), s);
       }
       ++total;
       if (k % 4 == 0 || k == mcap || total >= maxiter) {
         HIP_OK(hipMemcpyAsync(act_h.data(), K.act.p, sizeof(int) * R, hipMemcpyDeviceToHost, s));
+        if (relax > 0) HIP_OK(hipMemcpyAsync(est_h.data(), estd.p, sizeof(double) * R, hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
         bool any = false;
         for (int r = 0; r < R; ++r) any = any || act_h[r] != 0;
         if (!any) break;
+        relax_to(false);
       }
     }
     HIP_OK(hipMemcpyAsync(its_h.data(), its.p, sizeof(int) * R, hipMemcpyDeviceToHost, s));
@@ -179,6 +198,7 @@ void gmres_batch(dm_ctx* ctx, i64 n, int R, DevOp& A, DevPrec* M, const double* 
         active[r] = 1;
     }
   }
+  if (relax > 0) A.relax(0.0);
   HIP_OK(hipStreamSynchronize(s));
 }
 
@@ -244,7 +264,7 @@ void solve_batch(dm_ctx* ctx, const dm_solve_opts& o, DevOp* A64, DevOp* A32, De
     for (int r = 0; r < R; ++r) live[r] = rel[r] > o.tol;
     if (left <= 0 || !std::any_of(live.begin(), live.end(), [](char c) { return c != 0; })) break;
     HIP_OK(hipMemsetAsync(D.p, 0, 8 * blk, s));
-    gmres_batch(ctx, n, R, *A32, M, RR.p, D.p, tau, live, left, o.restart, K, s, iters, solved, true);
+    gmres_batch(ctx, n, R, *A32, M, RR.p, D.p, tau, live, left, o.restart, K, s, iters, solved, true, o.relax);
     outer += 1;
     dml::axpby(n * R, 1, 0, D.p, 1, 0, X, s);
     if (z64_ready && !waited) {

@@ -64,10 +64,18 @@ struct OpZ64 : DevOp {
   }
   size_t batch_ws(int R) const override { return dml::zx_workspace_bytes(z->n, R); }
 };
+// Error bound used for the three-part bf16 Z X (dropped part products below
+// 2^-18 relative each, three of them; five times that for margin).
+constexpr double kZx3TermBound = 5e-5;
 struct OpZ32 : DevOp {
   const dm_zmat* z;
   DBuf<char> planes;  // bf16 x 3 planes of Z32, built on the first batched product
+  int nterms = 6;     // bf16 part products of a batched product (3 once relaxed)
   explicit OpZ32(const dm_zmat* z_) : z(z_) {}
+  double relax(double eta) override {
+    nterms = eta >= kZx3TermBound ? 3 : 6;
+    return nterms == 3 ? kZx3TermBound : 0.0;
+  }
   void apply(const double* x, double* y, hipStream_t s) override {
     ProfScope p(z->ctx, "gemv_z32", s);
     launch_gemv_z32(z->n, z->ld, z->z32.p, x, y, s);
@@ -86,8 +94,8 @@ struct OpZ32 : DevOp {
     const int kc = dmt::tune_choice(
         z->ctx, "zx/c64/" + std::to_string(z->n) + "/" + std::to_string(R), dml::zx_bf16_chunks(), 4096,
         [&](int c) { dml::zx_bf16(z->n, z->ld, planes.p, X, z->n, R, Y, z->n, ws, s, c); }, s);
-    ProfScope p(z->ctx, "zx_z32", s);
-    dml::zx_bf16(z->n, z->ld, planes.p, X, z->n, R, Y, z->n, ws, s, kc);
+    ProfScope p(z->ctx, nterms == 3 ? "zx_z32_relaxed" : "zx_z32", s);
+    dml::zx_bf16(z->n, z->ld, planes.p, X, z->n, R, Y, z->n, ws, s, kc, nterms);
   }
   size_t batch_ws(int R) const override {
     return dml::zx_bf16_workspace_bytes(z->n, R);

@@ -21,11 +21,12 @@ void norm2(i64 n, int R, const double* W, const int* act, double* out, hipStream
 void scale(i64 n, int R, const double* src, const double* sc, double* dst, hipStream_t s);
 // Device Givens steps of R columns with bases of up to m vectors: init (g_r =
 // beta_r e_1 for active columns, sc = 1 / beta, counters cleared), step j
-// (deactivates converged columns), back substitution into y (pitch kcap).
+// (deactivates converged columns; est_out, if given: each active column's
+// residual estimate |g[j+1]|), back substitution into y (pitch kcap).
 void givens_init(int R, int m, const double* beta, const int* act, double* g, double* sc, int* jcnt, int* its,
                  hipStream_t s);
 void givens(int R, int j, int m, int kcap, const double* h1, const double* h2, const double* nr, double* H,
             double* cs, double* sn, double* g, double* sc, int* act, const double* eps, int* jcnt, int* its,
-            int* solved, hipStream_t s);
+            int* solved, hipStream_t s, double* est_out = nullptr);
 void backsolve(int R, int m, int kcap, const double* H, const double* g, const int* jcnt, double* y, hipStream_t s);
 }  // namespace dmb

@@ -210,7 +210,8 @@ __global__ __launch_bounds__(64) void k_bgivens(int j, int m, int kcap, const do
                                                 double2* __restrict__ sn, double2* __restrict__ g,
                                                 double* __restrict__ sc, int* __restrict__ act,
                                                 const double* __restrict__ eps, int* __restrict__ jcnt,
-                                                int* __restrict__ its, int* __restrict__ solved) {
+                                                int* __restrict__ its, int* __restrict__ solved,
+                                                double* __restrict__ est_out) {
   const int r = blockIdx.x;
   if (!act[r]) return;
   const int k = j + 1, ldh = m + 1;
@@ -257,6 +258,7 @@ __global__ __launch_bounds__(64) void k_bgivens(int j, int m, int kcap, const do
     jcnt[r] = k;
     its[r] += 1;
     const double est = hypot(gk.x, gk.y);
+    if (est_out) est_out[r] = est;
     if (est <= eps[r] || hn == 0) {
       if (est <= eps[r]) solved[r] = 1;
       act[r] = 0;
@@ -354,12 +356,12 @@ void givens_init(int R, int m, const double* beta, const int* act, double* g, do
 
 void givens(int R, int j, int m, int kcap, const double* h1, const double* h2, const double* nr, double* H,
             double* cs, double* sn, double* g, double* sc, int* act, const double* eps, int* jcnt, int* its,
-            int* solved, hipStream_t s) {
+            int* solved, hipStream_t s, double* est_out) {
   DM_REQUIRE(j < m && m < BKMAX, DM_ERR_ARGUMENT, "Krylov basis too large for the device Givens step");
   const size_t smem = sizeof(double2) * (2 * (size_t)j + 2) + sizeof(double) * j;
   k_bgivens<<<R, 64, smem, s>>>(j, m, kcap, reinterpret_cast<const double2*>(h1), reinterpret_cast<const double2*>(h2),
                                 nr, reinterpret_cast<double2*>(H), cs, reinterpret_cast<double2*>(sn),
-                                reinterpret_cast<double2*>(g), sc, act, eps, jcnt, its, solved);
+                                reinterpret_cast<double2*>(g), sc, act, eps, jcnt, its, solved, est_out);
   HIP_OK(hipGetLastError());
 }
 

@@ -88,10 +88,11 @@ size_t zx_workspace_bytes(i64 n, int R);
 size_t zx_bf16_planes_elems(i64 ld, i64 n);
 void zx_bf16_split(i64 n, i64 ld, const float* z32, void* planes, hipStream_t s);
 size_t zx_bf16_workspace_bytes(i64 n, int R);
-// kc: columns per FP32 accumulation chunk (0: 4096; one of zx_bf16_chunks()).
+// kc: columns per FP32 accumulation chunk (0: 4096; one of zx_bf16_chunks());
+// nterms: 6 (FP32 accuracy) or 3 (relative error ~1e-5, relaxed products).
 std::vector<int> zx_bf16_chunks();
 void zx_bf16(i64 n, i64 ld, const void* planes, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
-             hipStream_t s, int kc = 0);
+             hipStream_t s, int kc = 0, int nterms = 6);
 // Y (mr x R) = A (mr x n, lda) X (n x R) (subtract: Y -= A X), complex128 on
 // FP64 MFMA; ws: gemm_c128_workspace_bytes(n, R).
 size_t gemm_c128_workspace_bytes(i64 n, int R);

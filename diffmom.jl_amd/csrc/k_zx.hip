@@ -235,6 +235,10 @@ __global__ void k_split_x_bf16(i64 n, i64 np, int R, int Rp, const double2* __re
   out[5 * plane + idx] = d;
 }
 
+// NT = 6: the six part products (FP32 accuracy); NT = 3: h h', h m', m h'
+// only (the dropped terms are each below 2^-18 relative, ~1e-5 in all), for
+// relaxed products late in a Krylov solve.
+template <int NT>
 __global__ __launch_bounds__(256, 1) void k_zx_bf16(i64 n, i64 R2, int nrb, int ncb, const __bf16* __restrict__ Ap,
                                                     i64 aplane,
                                                     const __bf16* __restrict__ Xp, i64 np, i64 xplane, int R,
@@ -310,7 +314,7 @@ __global__ __launch_bounds__(256, 1) void k_zx_bf16(i64 n, i64 R2, int nrb, int 
     if (st + 1 < nst) gload(k_begin + (i64)(st + 1) * 32);
     constexpr int kPa[6] = {1, 0, 2, 0, 1, 0}, kPb[6] = {1, 2, 0, 1, 0, 0};  // smallest terms first
 #pragma unroll
-    for (int t = 0; t < 6; ++t)
+    for (int t = 6 - NT; t < 6; ++t)
 #pragma unroll
       for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -404,7 +408,8 @@ size_t zx_bf16_workspace_bytes(i64 n, int R) {
 }
 std::vector<int> zx_bf16_chunks() { return {ZX_KC, 2 * ZX_KC}; }
 void zx_bf16(i64 n, i64 ld, const void* planes, const double* X, i64 ldx, int R, double* Y, i64 ldy, void* ws,
-             hipStream_t s, int kc) {
+             hipStream_t s, int kc, int nterms) {
+  DM_REQUIRE(nterms == 6 || nterms == 3, DM_ERR_ARGUMENT, "bf16 part products must be 6 or 3");
   DM_REQUIRE(ld % ZX_BM == 0 && ld >= n, DM_ERR_ARGUMENT, "complex64 matrix leading dimension must be a multiple of 64");
   if (kc <= 0) kc = ZX_KC;
   // the workspace holds the partials of ZX_KC-column chunks; longer chunks need fewer
@@ -422,8 +427,12 @@ void zx_bf16(i64 n, i64 ld, const void* planes, const double* X, i64 ldx, int R,
   DM_REQUIRE(nwg < (1LL << 31) && nk <= 65535 && ncb <= 65535, DM_ERR_ARGUMENT, "product too large for one launch");
   const i64 R2 = 2 * ld, ng = zx_bf16_groups(n);
   dim3 grid((unsigned)nwg, 1, (unsigned)nk);
-  k_zx_bf16<<<grid, 256, 0, s>>>(n, R2, nrb, ncb, reinterpret_cast<const __bf16*>(planes), ng * R2 * 8, Xp, np,
-                                 xplane, R, (i64)kc, part);
+  if (nterms == 6)
+    k_zx_bf16<6><<<grid, 256, 0, s>>>(n, R2, nrb, ncb, reinterpret_cast<const __bf16*>(planes), ng * R2 * 8, Xp, np,
+                                      xplane, R, (i64)kc, part);
+  else
+    k_zx_bf16<3><<<grid, 256, 0, s>>>(n, R2, nrb, ncb, reinterpret_cast<const __bf16*>(planes), ng * R2 * 8, Xp, np,
+                                      xplane, R, (i64)kc, part);
   const i64 tot = n * (i64)R;
   k_zx_reduce<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, R, (int)nk, part, reinterpret_cast<double2*>(Y), ldy);
   HIP_OK(hipGetLastError());

@@ -226,12 +226,15 @@ typedef struct dm_solve_opts {
                           (preconditioner, product, CGS2, Givens) in a HIP graph and
                           replays it, the step index read from device memory; 0 =
                           launches each kernel (default). */
-  double relax;        /* MIXED with a compressed form, one right-hand side at a time:
-                          > 0 relaxes the inner products as the residual falls
-                          (inexact Krylov): at each check point the product may use
-                          the compression's looser level whose error bound is at
-                          most relax * inner_tol * ||r0|| / (residual estimate);
-                          the refinement residual stays the dense complex128
+  double relax;        /* MIXED: > 0 relaxes the inner products as the residual falls
+                          (inexact Krylov): at each check point the product may
+                          carry an error bound of at most relax * inner_tol *
+                          ||r0|| / (residual estimate) -- with a compressed form,
+                          one right-hand side at a time, the compression's looser
+                          levels; in lockstep batches (the most demanding active
+                          column decides), three instead of six bf16 part
+                          products of the complex64 Z X (bound 5e-5).  The
+                          refinement residual stays the dense complex128
                           product.  0 = off (default). */
 } dm_solve_opts;
 

@@ -349,6 +349,40 @@ def test_batched_planewaves_end_to_end(dm):
     assert np.linalg.norm(Xs - X[:, :3]) / np.linalg.norm(X[:, :3]) < 1e-4
 
 
+def test_batched_relaxed_products_reach_the_true_residual(dm):
+    """Lockstep batches with relaxed products (dm_solve_opts.relax): once the
+    worst active column's residual estimate allows it, the complex64 Z X drops
+    to three bf16 part products (h h', h m', m h'); every column's true FP64
+    residual still meets the tolerance, with about the iterations and currents
+    of the unrelaxed batch."""
+    m = dm.make_icosphere(0.1, 3)
+    r = dm.build_rwg(m)
+    rng = np.random.default_rng(11)
+    pws = _random_planewaves(dm, rng, K0, 16)
+    P = dm.build_nearfield_preconditioner(None, m, r, 2.0 * float(np.mean(r.len)))
+    ctx = m.ctx
+    X0, s0, _ = dm.solve_planewaves(m, r, K0, pws, "mixed", P, tol=1e-6, maxiter=300, batch=1)
+    ctx.profile_reset()
+    ctx.profile(True)
+    X1, s1, _, Z = dm.solve_planewaves(m, r, K0, pws, "mixed", P, tol=1e-6, maxiter=300, batch=1, relax=0.01,
+                                       keep_matrix=True)
+    ctx.profile(False)
+    n3, _ = ctx.profile_read("zx_z32_relaxed")
+    n6, _ = ctx.profile_read("zx_z32")
+    ctx.profile_reset()
+    print(f"relaxed batch: {s1.niter} it, {s1.outer} refinements ({n3} three-part, {n6} six-part products); "
+          f"unrelaxed {s0.niter} it, {s0.outer}", flush=True)
+    assert n3 > 0 and n6 > 0
+    assert s1.solved and s1.residual <= 1e-6
+    V = dm.assemble_multiple_excitations(m, r, pws)
+    Zh = Z.to_numpy()
+    res = np.linalg.norm(Zh @ X1 - V, axis=0) / np.linalg.norm(V, axis=0)
+    assert res.max() <= 1e-6 * 1.001
+    assert s1.niter <= 1.05 * s0.niter + 16 and s1.outer <= s0.outer + 1
+    assert np.linalg.norm(X1 - X0) <= 1e-4 * np.linalg.norm(X0)
+    Z.free()
+
+
 @pytest.mark.parametrize("nw", [1, 12])
 def test_mixed_pipeline_complex64_is_rounded_complex128(dm, O, nw):
     """The mixed solve assembles once in FP64 and stores the complex64 matrix as
