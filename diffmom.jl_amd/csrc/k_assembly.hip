@@ -20,7 +20,8 @@
 // triangle as the outer one, and the tile kernel follows the caller's basis
 // numbering to pick the same table.
 //
-// Tiling.  Bases are in a spatial (Morton) order; block b = TB consecutive
+// Tiling.  Bases are in a spatial order (recursive coordinate bisection of
+// their centres into compact patches); block b = TB consecutive
 // bases, touching the triangle list blk_tri[blk_off[b]..blk_off[b+1]).  One
 // workgroup owns tile (bi <= bj): it evaluates every (row triangle, column
 // triangle) pair of the two lists, accumulates sigma*sigma*J into an LDS tile
