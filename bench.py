@@ -376,7 +376,9 @@ def main():
                     help="chosen form: cross-approximate the complex64 matrix to this relative Frobenius error "
                          "(0 = dense complex64)")
     ap.add_argument("--relax", type=float, default=0.01,
-                    help="chosen form and lockstep batches: relaxation constant of the inner products (0: off)")
+                    help="chosen form: relaxation constant of the inner products (0: off)")
+    ap.add_argument("--batch-relax", type=float, default=0.1,
+                    help="lockstep batches: relaxation constant of the complex64 Z X (0: off)")
     ap.add_argument("--inner-tol", type=float, default=3e-7,
                     help="chosen form: relative tolerance of the first inner solve")
     ap.add_argument("--relax-probe", action="store_true",
@@ -571,7 +573,7 @@ def main():
         ctx.sync()
         ctx.profile_reset()
         ctx.profile(True)
-        dt, X, st_b, tb = solve("mixed", nf, waves, maxiter=300, relax=a.relax)
+        dt, X, st_b, tb = solve("mixed", nf, waves, maxiter=300, relax=a.batch_relax)
         ctx.profile(False)
         bkern = {}
         for name in ("assemble_z64_z32", "zx_split", "zx_z32", "zx_z32_relaxed", "zx_z64", "arnoldi", "precond"):
@@ -585,7 +587,7 @@ def main():
                        single_rhs_chosen_s=round(t_c, 4),
                        speedup_per_rhs_vs_single_chosen=round(t_c * a.batch_rhs / dt, 2),
                        speedup_per_rhs_vs_single_plain=round(t_p * a.batch_rhs / dt, 2),
-                       relax=a.relax, kernels=bkern)
+                       relax=a.batch_relax, kernels=bkern)
         if "zx_z32" in bkern:
             # lockstep complex64 Z X over all columns: 8 N^2 R useful FLOP per product,
             # six bf16 part products of that size on the matrix cores (plus the FP64
@@ -606,7 +608,7 @@ def main():
                                                   frac=round(3 * useful / zx_ms / 1e9 / 2500.0, 4))
         log(f"batched {a.batch_rhs} plane waves: {dt:.2f} s ({dt / a.batch_rhs * 1e3:.1f} ms per RHS), "
             f"{st_b.niter} it total, worst residual {st_b.residual:.2e}")
-        if a.relax > 0:
+        if a.batch_relax > 0:
             # ablation: the same batch with every product at six part products
             dtu, _, st_bu, _ = solve("mixed", nf, waves, maxiter=300)
             batched["unrelaxed"] = dict(s=round(dtu, 3), iterations_total=st_bu.niter,
