@@ -229,11 +229,11 @@ typedef struct dm_solve_opts {
   double relax;        /* MIXED: > 0 relaxes the inner products as the residual falls
                           (inexact Krylov): at each check point the product may
                           carry an error bound of at most relax * inner_tol *
-                          ||r0|| / (residual estimate) -- with a compressed form,
-                          one right-hand side at a time, the compression's looser
-                          levels; in lockstep batches (the most demanding active
-                          column decides), three instead of six bf16 part
-                          products of the complex64 Z X (bound 5e-5).  The
+                          ||r0|| / (residual estimate) -- with a compressed form
+                          the compression's looser levels; on the dense complex64
+                          form in lockstep batches, three instead of six bf16
+                          part products of the Z X (bound 5e-5).  In a lockstep
+                          batch the most demanding active column decides.  The
                           refinement residual stays the dense complex128
                           product.  0 = off (default). */
 } dm_solve_opts;
