@@ -146,7 +146,7 @@ struct OpHier : DevOp {
       dml::hier_ranges_levels(z->n, 1 + kHLevels, rk, W, bufs, hr, s);
     }
     ProfScope p(z->ctx, level ? "symv_z32h_relaxed" : "symv_z32h", s);
-    dml::symv_hier(z->n, z->z32p.p, ranks(level), z->hfac.p, z->hrmax, x, y, ws.p, hr[level], s);
+    dml::symv_hier(z->n, z->z32p.p, ranks(level), z->hfac.p, z->hrmax, x, y, ws.p, hr[level], s, level >= 2);
   }
   void apply_batch(const double* X, double* Y, int R, void*, hipStream_t s) override {
     for (int j = 0; j < R; ++j) apply(X + 2 * z->n * j, Y + 2 * z->n * j, s);

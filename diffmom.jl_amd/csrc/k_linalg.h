@@ -77,8 +77,11 @@ void hier_ranges(i64 n, const int* rank, int W, void* buf, HierRanges& hr, hipSt
 // The same for nl rank arrays at once (buffers of hier_ranges_bytes(n, W) each).
 void hier_ranges_levels(i64 n, int nl, const int* const* ranks, int W, void* const* bufs, HierRanges* hr,
                         hipStream_t s);
+// f32part: the row and column partial vectors in FP32 (half the partial
+// bytes; rounding ~3e-8 relative per partial, for the compression's looser
+// levels, whose own error bound is 5e-7 and more).
 void symv_hier(i64 n, const float* z32p, const int* rank, const float* fac, int rmax, const double* x, double* y,
-               void* ws, const HierRanges& hr, hipStream_t s);
+               void* ws, const HierRanges& hr, hipStream_t s, bool f32part = false);
 // Workspace of either batched product (complex64 or complex128) for R columns.
 size_t zx_workspace_bytes(i64 n, int R);
 // The complex64 Z X on the bf16 matrix cores at FP32 accuracy (each FP32

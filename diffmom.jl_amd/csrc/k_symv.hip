@@ -569,11 +569,22 @@ __device__ inline void butterfly8x(double2 (&p)[8], int lane) {
   dpp_acc<0x141>(p[0]);  // row_half_mirror
 }
 
+// Partial vectors of the products: FP64 (double2), or FP32 (float2) where the
+// product may carry a larger error anyway (the compression's looser levels).
+__device__ inline void put_part(double2* p, double2 v) { *p = v; }
+__device__ inline void put_part(float2* p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
+__device__ inline double2 get_part(const double2* p) { return *p; }
+__device__ inline double2 get_part(const float2* p) {
+  const float2 v = *p;
+  return make_double2((double)v.x, (double)v.y);
+}
+
+template <typename P>
 __global__ __launch_bounds__(256, 2) void k_symv_h(i64 n, int nb, int W, const int* __restrict__ wstart,
                                                 const void* __restrict__ zp, const int* __restrict__ rank,
                                                 const float2* __restrict__ fac, int rmax,
-                                                const double2* __restrict__ x, double2* __restrict__ rowpart,
-                                                double2* __restrict__ colpart) {
+                                                const double2* __restrict__ x, P* __restrict__ rowpart,
+                                                P* __restrict__ colpart) {
   constexpr i64 kTileF4 = (i64)kPT * kPT * sizeof(float2) / 16;
   const int lane = threadIdx.x & 63;
   const int wave = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -672,7 +683,7 @@ __global__ __launch_bounds__(256, 2) void k_symv_h(i64 n, int nb, int W, const i
   };
   auto finish = [&](int t, bool flush) {
 #pragma unroll
-    for (int e = 0; e < 2; ++e) rowpart[(i64)t * kPT + 64 * e + lane] = accR[e];
+    for (int e = 0; e < 2; ++e) put_part(rowpart + (i64)t * kPT + 64 * e + lane, accR[e]);
     if (flush) {
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
@@ -682,7 +693,7 @@ __global__ __launch_bounds__(256, 2) void k_symv_h(i64 n, int nb, int W, const i
         const double2 g1 = make_double2(__shfl(accC1.x, src, 64), __shfl(accC1.y, src, 64));
         double2 v = c < 64 ? g0 : g1;
         cadd(v, accJ[e]);
-        colpart[(i64)t * kPT + 64 * e + lane] = v;
+        put_part(colpart + (i64)t * kPT + 64 * e + lane, v);
       }
       accC0 = accC1 = make_double2(0.0, 0.0);
       accJ[0] = accJ[1] = make_double2(0.0, 0.0);
@@ -755,9 +766,10 @@ __global__ __launch_bounds__(256, 2) void k_symv_h(i64 n, int nb, int W, const i
 //              + sum over the column runs of tile column R of their colpart
 // (a run ends at tile t = R (R + 1) / 2 + I when t + 1 is a multiple of tpw
 // or I = R); fixed order, deterministic.
+template <typename P>
 __global__ __launch_bounds__(256) void k_symv_w_reduce(i64 n, int nb, int tpw, const unsigned char* __restrict__ rend,
-                                                       int lanemajor, const double2* __restrict__ rowpart,
-                                                       const double2* __restrict__ colpart, double2* __restrict__ y) {
+                                                       int lanemajor, const P* __restrict__ rowpart,
+                                                       const P* __restrict__ colpart, double2* __restrict__ y) {
   // 32 rows per block, 8 groups sharing the tiles of a row block round robin;
   // the groups' sums are added in a fixed order
   const int rl = threadIdx.x & 31, grp = threadIdx.x >> 5;
@@ -773,7 +785,7 @@ __global__ __launch_bounds__(256) void k_symv_w_reduce(i64 n, int nb, int tpw, c
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const i64 Ju = J + 8 * u;
-        v[u] = rowpart[(Ju * (Ju + 1) / 2 + R) * kPT + r];
+        v[u] = get_part(rowpart + (Ju * (Ju + 1) / 2 + R) * kPT + r);
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -782,7 +794,7 @@ __global__ __launch_bounds__(256) void k_symv_w_reduce(i64 n, int nb, int tpw, c
       }
     }
     for (; J < nb; J += 8) {
-      const double2 v = rowpart[((i64)J * (J + 1) / 2 + R) * kPT + r];
+      const double2 v = get_part(rowpart + ((i64)J * (J + 1) / 2 + R) * kPT + r);
       sr += v.x;
       si += v.y;
     }
@@ -802,7 +814,7 @@ __global__ __launch_bounds__(256) void k_symv_w_reduce(i64 n, int nb, int tpw, c
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const i64 t = c0 + I + 8 * u;
-        v[u] = f[u] ? colpart[t * kPT + r] : make_double2(0.0, 0.0);
+        v[u] = f[u] ? get_part(colpart + t * kPT + r) : make_double2(0.0, 0.0);
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -813,7 +825,7 @@ __global__ __launch_bounds__(256) void k_symv_w_reduce(i64 n, int nb, int tpw, c
     for (; I <= R; I += 8) {
       const i64 t = c0 + I;
       if (rend ? rend[t] != 0 : (I == R || (t + 1) % tpw == 0)) {
-        const double2 v = colpart[t * kPT + r];
+        const double2 v = get_part(colpart + t * kPT + r);
         sr += v.x;
         si += v.y;
       }
@@ -979,7 +991,7 @@ int hier_wave_slots() {
   int dev = 0, ncu = 0, per_cu = 0;
   HIP_OK(hipGetDevice(&dev));
   HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_symv_h, 256, 0));
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_symv_h<double2>, 256, 0));
   return std::max(per_cu, 1) * ncu * 4;
 }
 
@@ -1017,20 +1029,29 @@ void hier_ranges_levels(i64 n, int nl, const int* const* ranks, int W, void* con
   HIP_OK(hipGetLastError());
 }
 
-void symv_hier(i64 n, const float* z32p, const int* rank, const float* fac, int rmax, const double* x, double* y,
-               void* ws, const HierRanges& hr, hipStream_t s) {
+template <typename P>
+static void symv_hier_launch(i64 n, const float* z32p, const int* rank, const float* fac, int rmax, const double* x,
+                             double* y, void* ws, const HierRanges& hr, hipStream_t s) {
   const i64 nb = (n + kPT - 1) / kPT;
   const i64 T = packed_tiles(n);
   const i64 blocks = ((i64)hr.W + 3) / 4;
   DM_REQUIRE(hr.W >= 1 && blocks < (1LL << 31), DM_ERR_ARGUMENT, "wave count out of range");
-  double2* rowpart = reinterpret_cast<double2*>(ws);
-  double2* colpart = rowpart + T * kPT;
-  k_symv_h<<<(unsigned)blocks, 256, 0, s>>>(n, (int)nb, hr.W, hr.wstart, z32p, rank,
-                                            reinterpret_cast<const float2*>(fac), rmax,
-                                            reinterpret_cast<const double2*>(x), rowpart, colpart);
-  k_symv_w_reduce<<<(unsigned)((n + 31) / 32), 256, 0, s>>>(n, (int)nb, 1, hr.rend, 1, rowpart, colpart,
-                                                            reinterpret_cast<double2*>(y));
+  P* rowpart = reinterpret_cast<P*>(ws);
+  P* colpart = rowpart + T * kPT;
+  k_symv_h<P><<<(unsigned)blocks, 256, 0, s>>>(n, (int)nb, hr.W, hr.wstart, z32p, rank,
+                                               reinterpret_cast<const float2*>(fac), rmax,
+                                               reinterpret_cast<const double2*>(x), rowpart, colpart);
+  k_symv_w_reduce<P><<<(unsigned)((n + 31) / 32), 256, 0, s>>>(n, (int)nb, 1, hr.rend, 1, rowpart, colpart,
+                                                               reinterpret_cast<double2*>(y));
   HIP_OK(hipGetLastError());
+}
+
+void symv_hier(i64 n, const float* z32p, const int* rank, const float* fac, int rmax, const double* x, double* y,
+               void* ws, const HierRanges& hr, hipStream_t s, bool f32part) {
+  if (f32part)
+    symv_hier_launch<float2>(n, z32p, rank, fac, rmax, x, y, ws, hr, s);
+  else
+    symv_hier_launch<double2>(n, z32p, rank, fac, rmax, x, y, ws, hr, s);
 }
 
 std::vector<int> symv_shapes(i64 n, bool f32) {
@@ -1074,7 +1095,7 @@ void symv_packed(i64 n, const ZRef& z, const double* x, double* y, void* ws, int
       k_symv_w<1><<<(unsigned)blocks, 256, 0, s>>>(n, (int)nb, tpw, z.p, xv, rowpart, colpart);
     else
       k_symv_w<0><<<(unsigned)blocks, 256, 0, s>>>(n, (int)nb, tpw, z.p, xv, rowpart, colpart);
-    k_symv_w_reduce<<<(unsigned)((n + 31) / 32), 256, 0, s>>>(n, (int)nb, tpw, nullptr, 0, rowpart, colpart,
+    k_symv_w_reduce<double2><<<(unsigned)((n + 31) / 32), 256, 0, s>>>(n, (int)nb, tpw, nullptr, 0, rowpart, colpart,
                                                               reinterpret_cast<double2*>(y));
     HIP_OK(hipGetLastError());
     return;
