@@ -173,21 +173,30 @@ __global__ __launch_bounds__(256) void k_dots_part(i64 n, i64 ldv, int k, const 
     wm[u] = m < n ? w[m] : make_double2(0.0, 0.0);
   }
   const int cnt = min(kDotVG, k - i0);
-  for (int q = 0; q < cnt; ++q) {
-    const double2* v = V + (i64)(i0 + q) * ldv;
-    double ar = 0, ai = 0;
+  // all kDotVG x kDotRows loads are issued before any sum: vectors past cnt
+  // re-read the last one (clamped, unconditional) and are not stored
+  double2 a[kDotVG][kDotRows];
+#pragma unroll
+  for (int q = 0; q < kDotVG; ++q) {
+    const double2* v = V + (i64)(i0 + min(q, cnt - 1)) * ldv;
 #pragma unroll
     for (int u = 0; u < kDotRows; ++u) {
       const i64 m = m0 + 256 * u;
-      if (m < n) {
-        double2 a = v[m];
-        ar += a.x * wm[u].x + a.y * wm[u].y;
-        ai += a.x * wm[u].y - a.y * wm[u].x;
-      }
+      a[q][u] = v[min(m, n - 1)];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kDotVG; ++q) {
+    double ar = 0, ai = 0;
+#pragma unroll
+    for (int u = 0; u < kDotRows; ++u) {
+      // rows past n carry w = 0
+      ar += a[q][u].x * wm[u].x + a[q][u].y * wm[u].y;
+      ai += a[q][u].x * wm[u].y - a[q][u].y * wm[u].x;
     }
     ar = wave_sum_dpp(ar);
     ai = wave_sum_dpp(ai);
-    if (lane == 0) red[wave][q] = make_double2(ar, ai);
+    if (lane == 0 && q < cnt) red[wave][q] = make_double2(ar, ai);
   }
   __syncthreads();
   if (threadIdx.x < cnt) {
@@ -228,6 +237,8 @@ __global__ __launch_bounds__(256) void k_update(i64 n, i64 ldv, int k, const int
   double s2 = 0;
   if (m < n) {
     double ar = 0, ai = 0;
+    // 16 loads in flight per wave (the loop is latency-bound at k ~ 50)
+#pragma unroll 16
     for (int i = 0; i < k; ++i) {
       double2 a = V[(i64)i * ldv + m], b = hs[i];
       ar += a.x * b.x - a.y * b.y;
