@@ -237,8 +237,6 @@ __global__ __launch_bounds__(256) void k_update(i64 n, i64 ldv, int k, const int
   double s2 = 0;
   if (m < n) {
     double ar = 0, ai = 0;
-    // 16 loads in flight per wave (the loop is latency-bound at k ~ 50)
-#pragma unroll 16
     for (int i = 0; i < k; ++i) {
       double2 a = V[(i64)i * ldv + m], b = hs[i];
       ar += a.x * b.x - a.y * b.y;
