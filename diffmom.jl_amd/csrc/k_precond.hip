@@ -21,13 +21,13 @@
 namespace {
 
 constexpr int SPAI_MAXJ = 80;   // columns per least-squares problem (LDS bound)
-constexpr int SPAI_CHUNK = 16;  // rows of A staged per pass
+constexpr int SPAI_CHUNK = 8;   // rows of A staged per pass
 
 // One column per workgroup.  LDS holds, for the widest support mJ of the
 // pattern (not the cap), the Gram matrix's lower triangle packed by rows
 // (entry (a, b), b <= a, at a (a + 1) / 2 + b), SPAI_CHUNK staged rows of A,
-// the right-hand side and the support's column ids: about 34 KB at mJ = 49,
-// so several columns share a CU.  The four waves form the Gram matrix (each
+// the right-hand side and the support's column ids: about 27 KB at mJ = 49,
+// so five columns share a CU (with at most 102 VGPRs: five waves per SIMD).  The four waves form the Gram matrix (each
 // thread owns 2 x 2 blocks of it: one LDS read per complex multiply-add, two
 // barriers per chunk of rows); the Cholesky factorization (left-looking)
 // and the two triangular solves (right-hand side in registers, each solved
@@ -65,7 +65,7 @@ __device__ inline double bcast(double v, int src) {
 }
 __device__ inline double2 bcast2(double2 v, int src) { return make_double2(bcast(v.x, src), bcast(v.y, src)); }
 
-__global__ __launch_bounds__(kSpaiThreads, 4) void k_spai_build(int n, int per, ZRef z, int mJ,
+__global__ __launch_bounds__(kSpaiThreads, 5) void k_spai_build(int n, int per, ZRef z, int mJ,
                                                              const int* __restrict__ jptr,
                                                              const int* __restrict__ jidx,
                                                              const int* __restrict__ iptr,
