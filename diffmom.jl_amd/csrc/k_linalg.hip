@@ -209,15 +209,18 @@ __global__ __launch_bounds__(256) void k_dots_part(i64 n, i64 ldv, int k, const 
 // w[m] -= sum_{i<k} V[m, i] h[i], where h[i] = sum_c part[c * k + i] are
 // the chunk partials of k_dots_part: every workgroup adds them itself (thread
 // i over the chunks in order, all loads independent, so all workgroups hold
-// the same h) and workgroup 0 stores h (at h + hsel k).  nrm, when given,
-// receives each workgroup's sum of |w[m]|^2 over its rows after the update.
-// jd (captured steps): k = *jd + 1 from device memory.
+// the same h) and workgroup 0 stores h (at h + hsel k).  A workgroup takes
+// kUpdRows rows; wave q sums the vectors i = q mod 4 (a quarter of the
+// dependent loads per wave), and wave 0 adds the four sums in order.  nrm,
+// when given, receives each workgroup's sum of |w[m]|^2 over its rows after
+// the update.  jd (captured steps): k = *jd + 1 from device memory.
+constexpr int kUpdRows = 64;
 __global__ __launch_bounds__(256) void k_update(i64 n, i64 ldv, int k, const int* __restrict__ jd, int hsel,
                                                 const double2* __restrict__ V, const double2* __restrict__ part,
                                                 int nch, double2* __restrict__ h, double2* __restrict__ w,
                                                 double* __restrict__ nrm) {
   __shared__ double2 hs[1024];
-  __shared__ double red[4];
+  __shared__ double2 acc[3][kUpdRows];
   if (jd) k = *jd + 1;
   h += (i64)hsel * k;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -233,15 +236,26 @@ __global__ __launch_bounds__(256) void k_update(i64 n, i64 ldv, int k, const int
     if (blockIdx.x == 0) h[i] = make_double2(sr, si);
   }
   __syncthreads();
-  const i64 m = blockIdx.x * (i64)256 + threadIdx.x;
-  double s2 = 0;
+  const i64 m = blockIdx.x * (i64)kUpdRows + lane;
+  double ar = 0, ai = 0;
   if (m < n) {
-    double ar = 0, ai = 0;
-    for (int i = 0; i < k; ++i) {
-      double2 a = V[(i64)i * ldv + m], b = hs[i];
+    for (int i = wave; i < k; i += 4) {
+      const double2 a = V[(i64)i * ldv + m], b = hs[i];
       ar += a.x * b.x - a.y * b.y;
       ai += a.x * b.y + a.y * b.x;
     }
+  }
+  if (wave > 0) acc[wave - 1][lane] = make_double2(ar, ai);
+  __syncthreads();
+  if (wave != 0) return;
+  double s2 = 0;
+  if (m < n) {
+    ar += acc[0][lane].x;
+    ai += acc[0][lane].y;
+    ar += acc[1][lane].x;
+    ai += acc[1][lane].y;
+    ar += acc[2][lane].x;
+    ai += acc[2][lane].y;
     const double2 o = w[m];
     const double2 u = make_double2(o.x - ar, o.y - ai);
     w[m] = u;
@@ -249,9 +263,7 @@ __global__ __launch_bounds__(256) void k_update(i64 n, i64 ldv, int k, const int
   }
   if (nrm) {
     s2 = wave_sum(s2);
-    if (lane == 0) red[wave] = s2;
-    __syncthreads();
-    if (threadIdx.x == 0) nrm[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+    if (lane == 0) nrm[blockIdx.x] = s2;
   }
 }
 
@@ -885,7 +897,7 @@ namespace dml {
 
 size_t dots_part_count(i64 n, int k) { return (size_t)k * (size_t)nblocks(n, 256 * kDotRows); }
 
-int norm_part_count(i64 n) { return (int)nblocks(n, 256); }
+int norm_part_count(i64 n) { return (int)nblocks(n, kUpdRows); }
 void dots(i64 n, i64 ldv, int k, const double* V, const double* w, double* part, hipStream_t s) {
   if (k <= 0) return;
   const int nch = (int)nblocks(n, 256 * kDotRows);
@@ -899,7 +911,7 @@ void update(i64 n, i64 ldv, int k, const double* V, const double* part, double* 
   if (k <= 0) return;
   DM_REQUIRE(k <= 1024, DM_ERR_ARGUMENT, "Krylov basis larger than 1024 vectors");
   const int nch = (int)nblocks(n, 256 * kDotRows);
-  k_update<<<nblocks(n, 256), 256, 0, s>>>(n, ldv, k, nullptr, 0, reinterpret_cast<const double2*>(V),
+  k_update<<<nblocks(n, kUpdRows), 256, 0, s>>>(n, ldv, k, nullptr, 0, reinterpret_cast<const double2*>(V),
                                             reinterpret_cast<const double2*>(part), nch, reinterpret_cast<double2*>(h),
                                             reinterpret_cast<double2*>(w), nrm);
   HIP_OK(hipGetLastError());
@@ -933,7 +945,7 @@ void dots_dev(i64 n, i64 ldv, int kmax, const int* jd, const double* V, const do
 void update_dev(i64 n, i64 ldv, const int* jd, int hsel, const double* V, const double* part, double* h, double* w,
                 double* nrm, hipStream_t s) {
   const int nch = (int)nblocks(n, 256 * kDotRows);
-  k_update<<<nblocks(n, 256), 256, 0, s>>>(n, ldv, 0, jd, hsel, reinterpret_cast<const double2*>(V),
+  k_update<<<nblocks(n, kUpdRows), 256, 0, s>>>(n, ldv, 0, jd, hsel, reinterpret_cast<const double2*>(V),
                                             reinterpret_cast<const double2*>(part), nch, reinterpret_cast<double2*>(h),
                                             reinterpret_cast<double2*>(w), nrm);
   HIP_OK(hipGetLastError());
