@@ -751,13 +751,17 @@ __global__ __launch_bounds__(256) void k_givens(int j, const int* __restrict__ j
       rc[i] = cs[i];
     }
   }
-  __shared__ double wn2;
-  if (threadIdx.x < 64) {
+  // ||w||^2 from the nb partials: every thread of the (256-thread) block
+  // adds a few, then the four wave sums in order
+  __shared__ double wn2, wpart[4];
+  {
     double t = 0;
-    for (int i = threadIdx.x; i < nb; i += 64) t += nrm[i];
+    for (int i = threadIdx.x; i < nb; i += 256) t += nrm[i];
     t = wave_sum(t);
-    if (threadIdx.x == 0) wn2 = t;
+    if ((threadIdx.x & 63) == 0) wpart[threadIdx.x >> 6] = t;
   }
+  __syncthreads();
+  if (threadIdx.x == 0) wn2 = (wpart[0] + wpart[1]) + (wpart[2] + wpart[3]);
   __syncthreads();
   {
     const double hn = sqrt(wn2), scl = hn > 0 ? 1.0 / hn : 0.0;
