@@ -207,9 +207,10 @@ __global__ __launch_bounds__(256) void k_dots_part(i64 n, i64 ldv, int k, const 
 }
 
 // w[m] -= sum_{i<k} V[m, i] h[i], where h[i] = sum_c part[c * k + i] are
-// the chunk partials of k_dots_part: every workgroup adds them itself (thread
-// i over the chunks in order, all loads independent, so all workgroups hold
-// the same h) and workgroup 0 stores h (at h + hsel k).  A workgroup takes
+// the chunk partials of k_dots_part: every workgroup adds them itself in the
+// same order (for k <= 256 four waves over interleaved chunks, then their
+// four sums; else thread i over the chunks), so all workgroups hold the same
+// h, and workgroup 0 stores h (at h + hsel k).  A workgroup takes
 // kUpdRows rows; wave q sums the vectors i = q mod 4 (a quarter of the
 // dependent loads per wave), and wave 0 adds the four sums in order.  nrm,
 // when given, receives each workgroup's sum of |w[m]|^2 over its rows after
@@ -221,19 +222,42 @@ __global__ __launch_bounds__(256) void k_update(i64 n, i64 ldv, int k, const int
                                                 double* __restrict__ nrm) {
   __shared__ double2 hs[1024];
   __shared__ double2 acc[3][kUpdRows];
+  __shared__ double2 ps[4][256];
   if (jd) k = *jd + 1;
   h += (i64)hsel * k;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < k; i += 256) {
-    double sr = 0, si = 0;
-#pragma unroll 8
-    for (int c = 0; c < nch; ++c) {
-      const double2 v = part[(i64)c * k + i];
-      sr += v.x;
-      si += v.y;
+  if (k <= 256) {
+    // wave q adds the chunks c = q mod 4 of every vector (a quarter of the
+    // dependent loads per thread), then the four sums in order
+    for (int i = lane; i < k; i += 64) {
+      double sr = 0, si = 0;
+#pragma unroll 4
+      for (int c = wave; c < nch; c += 4) {
+        const double2 v = part[(i64)c * k + i];
+        sr += v.x;
+        si += v.y;
+      }
+      ps[wave][i] = make_double2(sr, si);
     }
-    hs[i] = make_double2(sr, si);
-    if (blockIdx.x == 0) h[i] = make_double2(sr, si);
+    __syncthreads();
+    for (int i = threadIdx.x; i < k; i += 256) {
+      const double sr = ((ps[0][i].x + ps[1][i].x) + ps[2][i].x) + ps[3][i].x;
+      const double si = ((ps[0][i].y + ps[1][i].y) + ps[2][i].y) + ps[3][i].y;
+      hs[i] = make_double2(sr, si);
+      if (blockIdx.x == 0) h[i] = make_double2(sr, si);
+    }
+  } else {
+    for (int i = threadIdx.x; i < k; i += 256) {
+      double sr = 0, si = 0;
+#pragma unroll 8
+      for (int c = 0; c < nch; ++c) {
+        const double2 v = part[(i64)c * k + i];
+        sr += v.x;
+        si += v.y;
+      }
+      hs[i] = make_double2(sr, si);
+      if (blockIdx.x == 0) h[i] = make_double2(sr, si);
+    }
   }
   __syncthreads();
   const i64 m = blockIdx.x * (i64)kUpdRows + lane;
